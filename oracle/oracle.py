@@ -1,0 +1,204 @@
+"""ctypes front end of the CPU oracle (oracle/_build/libfem_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, never by the product package.  The C sources
+(fem_oracle.c) cite the reference routines they restate.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "libfem_oracle.so")
+_lib = None
+
+GPSTATE = 38
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+_lp = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+
+
+class OrcMesh(C.Structure):
+    _fields_ = [("nnode", C.c_int), ("coords", C.c_void_p), ("nelem", C.c_int), ("etype", C.c_int),
+                ("conn", C.c_void_p), ("emat", C.c_void_p), ("nmat", C.c_int), ("mat", C.c_void_p),
+                ("eqn", C.c_void_p), ("neq", C.c_int), ("npeq", C.c_int)]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        L.orc_dndx.restype = C.c_double
+        L.orc_detj.restype = C.c_double
+        L.orc_csr_count.restype = C.c_longlong
+        _lib = L
+    return _lib
+
+
+class Mesh:
+    """Keeps the numpy arrays alive behind an orc_mesh struct."""
+
+    def __init__(self, deck):
+        self.deck = deck
+        self._keep = [deck.coords, deck.conn, deck.emat, deck.mat, deck.eqn]
+        self.s = OrcMesh(deck.nnode, deck.coords.ctypes.data, deck.nelem, deck.etype,
+                         deck.conn.ctypes.data, deck.emat.ctypes.data, deck.mat.shape[0],
+                         deck.mat.ctypes.data, deck.eqn.ctypes.data, deck.neq, deck.npeq)
+
+    @property
+    def ref(self):
+        return C.byref(self.s)
+
+
+def _d(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def gauss(etype):
+    xi = np.zeros(81)
+    w = np.zeros(27)
+    n = lib().orc_gauss(etype, xi.ctypes.data_as(C.c_void_p), w.ctypes.data_as(C.c_void_p))
+    return xi[: 3 * n].reshape(n, 3), w[:n]
+
+
+def dndx(etype, X, xi):
+    nen = {1: 8, 2: 4, 3: 20}[etype]
+    out = np.zeros(3 * nen)
+    err = C.c_int(0)
+    det = lib().orc_dndx(etype, _d(X).ctypes.data_as(C.c_void_p), _d(xi).ctypes.data_as(C.c_void_p),
+                         out.ctypes.data_as(C.c_void_p), C.byref(err))
+    return det, out.reshape(nen, 3), err.value
+
+
+def isole_D(E, nu):
+    D = np.zeros(36)
+    lib().orc_isole_D(C.c_double(E), C.c_double(nu), D.ctypes.data_as(C.c_void_p))
+    return D.reshape(6, 6)
+
+
+def element_stiffness(etype, X, D, symmetric=True):
+    nen = {1: 8, 2: 4, 3: 20}[etype]
+    D = _d(D)
+    stride = 0 if D.size == 36 else 36
+    Ke = np.zeros((3 * nen) ** 2)
+    rc = lib().orc_element_stiffness(etype, _d(X).ctypes.data_as(C.c_void_p), D.ctypes.data_as(C.c_void_p),
+                                     stride, int(symmetric), Ke.ctypes.data_as(C.c_void_p))
+    if rc:
+        raise RuntimeError(f"orc_element_stiffness rc={rc}")
+    return Ke.reshape(3 * nen, 3 * nen)
+
+
+def mises_return(mp, eps, epsp_n, kappa_n):
+    sig = np.zeros(6)
+    epsp = np.zeros(6)
+    sdev = np.zeros(6)
+    kap = C.c_double(0)
+    lib().orc_mises_return(_d(mp).ctypes.data_as(C.c_void_p), _d(eps).ctypes.data_as(C.c_void_p),
+                           _d(epsp_n).ctypes.data_as(C.c_void_p), C.c_double(kappa_n),
+                           sig.ctypes.data_as(C.c_void_p), epsp.ctypes.data_as(C.c_void_p), C.byref(kap),
+                           sdev.ctypes.data_as(C.c_void_p))
+    return sig, epsp, kap.value, sdev
+
+
+def mises_tangent(mp, kappa_n, kappa_t, sdev):
+    D = np.zeros(36)
+    lib().orc_mises_tangent(_d(mp).ctypes.data_as(C.c_void_p), C.c_double(kappa_n), C.c_double(kappa_t),
+                            _d(sdev).ctypes.data_as(C.c_void_p), D.ctypes.data_as(C.c_void_p))
+    return D.reshape(6, 6)
+
+
+def csr(deck, state=None, tangent=False):
+    """Assembled free-free stiffness (rowptr int64, colind int32, val) in the
+    reference's element order."""
+    m = Mesh(deck)
+    nnz = lib().orc_csr_count(m.ref)
+    rowptr = np.zeros(deck.neq + 1, dtype=np.int64)
+    colind = np.zeros(nnz, dtype=np.int32)
+    val = np.zeros(nnz)
+    lib().orc_csr_fill(m.ref, rowptr.ctypes.data_as(C.c_void_p), colind.ctypes.data_as(C.c_void_p))
+    st = state.ctypes.data_as(C.c_void_p) if state is not None else None
+    rc = lib().orc_assemble_stiffness(m.ref, rowptr.ctypes.data_as(C.c_void_p), colind.ctypes.data_as(C.c_void_p),
+                                      val.ctypes.data_as(C.c_void_p), st, int(tangent))
+    if rc:
+        raise RuntimeError(f"orc_assemble_stiffness rc={rc}")
+    return rowptr, colind, val
+
+
+def spmv(rowptr, colind, val, x):
+    y = np.zeros(rowptr.shape[0] - 1)
+    lib().orc_spmv(len(y), rowptr.ctypes.data_as(C.c_void_p), colind.ctypes.data_as(C.c_void_p),
+                   val.ctypes.data_as(C.c_void_p), _d(x).ctypes.data_as(C.c_void_p), y.ctypes.data_as(C.c_void_p))
+    return y
+
+
+def pcg(rowptr, colind, val, b, x0=None, tol=1e-10, maxit=100000, precond=1):
+    n = rowptr.shape[0] - 1
+    x = np.zeros(n) if x0 is None else _d(x0).copy()
+    it = C.c_int(0)
+    res = C.c_double(0)
+    rc = lib().orc_pcg(n, rowptr.ctypes.data_as(C.c_void_p), colind.ctypes.data_as(C.c_void_p),
+                       val.ctypes.data_as(C.c_void_p), _d(b).ctypes.data_as(C.c_void_p), x.ctypes.data_as(C.c_void_p),
+                       C.c_double(tol), maxit, precond, C.byref(it), C.byref(res))
+    return x, it.value, res.value, rc
+
+
+def state_new(deck):
+    return np.zeros(GPSTATE * deck.nelem * {1: 8, 2: 1, 3: 27}[deck.etype])
+
+
+def state_commit(deck, state):
+    m = Mesh(deck)
+    lib().orc_state_commit(m.ref, state.ctypes.data_as(C.c_void_p))
+
+
+def internal_forces(deck, u_free, u_presc=None, state=None, update_state=False):
+    m = Mesh(deck)
+    f = np.zeros(deck.neq)
+    fp = np.zeros(deck.npeq)
+    ebe = np.zeros(3)
+    up = _d(u_presc) if u_presc is not None else np.zeros(deck.npeq)
+    st = state.ctypes.data_as(C.c_void_p) if state is not None else None
+    rc = lib().orc_internal_forces(m.ref, _d(u_free).ctypes.data_as(C.c_void_p), up.ctypes.data_as(C.c_void_p),
+                                   st, int(update_state), f.ctypes.data_as(C.c_void_p),
+                                   fp.ctypes.data_as(C.c_void_p), ebe.ctypes.data_as(C.c_void_p))
+    if rc:
+        raise RuntimeError(f"orc_internal_forces rc={rc}")
+    return f, fp, ebe
+
+
+def surface_load(deck, factor=1.0):
+    m = Mesh(deck)
+    f = np.zeros(deck.neq)
+    fp = np.zeros(deck.npeq)
+    el = np.ascontiguousarray(deck.faces[:, 0], dtype=np.int32)
+    sf = np.ascontiguousarray(deck.faces[:, 1], dtype=np.int32)
+    tr = _d(deck.traction * factor)
+    rc = lib().orc_surface_load(m.ref, len(el), el.ctypes.data_as(C.c_void_p), sf.ctypes.data_as(C.c_void_p),
+                                tr.ctypes.data_as(C.c_void_p), f.ctypes.data_as(C.c_void_p), fp.ctypes.data_as(C.c_void_p))
+    if rc:
+        raise RuntimeError(f"orc_surface_load rc={rc}")
+    return f, fp
+
+
+def linear_static(deck, tol=1e-10, maxit=100000):
+    m = Mesh(deck)
+    f, fp = surface_load(deck)
+    up = deck.prescribed_values()
+    u = np.zeros(deck.neq)
+    R = np.zeros(deck.npeq)
+    it = C.c_int(0)
+    res = C.c_double(0)
+    rc = lib().orc_linear_static(m.ref, f.ctypes.data_as(C.c_void_p), fp.ctypes.data_as(C.c_void_p),
+                                 up.ctypes.data_as(C.c_void_p), C.c_double(tol), maxit,
+                                 u.ctypes.data_as(C.c_void_p), R.ctypes.data_as(C.c_void_p), C.byref(it), C.byref(res))
+    return dict(u=u, reactions=R, iters=it.value, resid=res.value, rc=rc)

@@ -1,0 +1,214 @@
+#!/usr/bin/env python3
+"""Benchmark of the assembly-and-solve step on one MI355X.
+
+Workload (default): the 64x64x256 LSpace cantilever (1,048,576 elements,
+3,244,800 free equations), jittered +-0.2 h, log-normal E (median 30e3,
+sigma 0.5), nu = 0.2, clamped at z = 0, 1 MPa traction in -y at z = 4.
+One step = LinearStatic::solveYourselfAt on resident data: assemble K
+(batched evaluator), rhs = f_ext - f_int(prescribed), PCG (Jacobi) from x = 0
+to 1e-8 relative residual, reactions.  value = free DOFs x steps / wall time.
+
+The roofline figure is the PCG SpMV kernel (the step's dominant kernel):
+algorithmic bytes per launch = 12 nnz (FP64 value + int32 column) + 36 N
+(row pointer, gathers of z and p, writes of p and q), divided by the mean
+launch duration measured with HIP events on the library's stream over the
+timed steps.
+
+cpu_baseline: the oracle (C restatement of the reference algorithm, one
+thread) on a 64x64x16 slab of the same mesh family: element assembly,
+f_int and CG iterations are timed there and scaled to the full workload by
+element count, nnz and the GPU run's iteration count.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Newton-step DOFs/s at 1e-8 residual, plain FP64 vs mixed+surrogate; SpMV GB/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def make_deck(args):
+    from oofem_amd import meshgen
+    if args.workload == "cfg0":
+        return meshgen.config0()
+    return meshgen.config1(nx=args.nx, ny=args.ny, nz=args.nz)
+
+
+def run_gpu(args, deck, precision):
+    import numpy as np
+
+    from oofem_amd.hipfem import (CR_CONVERGED, BatchedEvaluator, Context, DeviceVector, HipCSRSparseMtrx,
+                                  HipPCGSolver)
+    ctx = Context(0)
+    K = HipCSRSparseMtrx(ctx)
+    ev = BatchedEvaluator(ctx, deck)
+    t = time.perf_counter()
+    ev.build_structure(K)
+    t_struct = time.perf_counter() - t
+    solver = HipPCGSolver(ctx, lstol=args.tol, lsiter=args.maxit, lsprecond=1, precision=precision,
+                          inner_tol=args.inner_tol)
+    solver.set_profiling(True)
+    f_ext, f_ext_p = deck.external_load()
+    up = deck.prescribed_values()
+    d_fext = DeviceVector(ctx, deck.neq, f_ext)
+    d_fextp = DeviceVector(ctx, max(deck.npeq, 1), f_ext_p if deck.npeq else np.zeros(1))
+    d_up = DeviceVector(ctx, max(deck.npeq, 1), up if deck.npeq else np.zeros(1))
+    d_u0 = DeviceVector(ctx, deck.neq)
+    d_rhs = DeviceVector(ctx, deck.neq)
+    d_u = DeviceVector(ctx, deck.neq)
+    d_f = DeviceVector(ctx, deck.neq)
+    d_R = DeviceVector(ctx, max(deck.npeq, 1))
+    ctx.synchronize()
+
+    def step():
+        ev.assemble_stiffness(K)
+        ev.internal_forces_dev(d_u0.addr, d_up.addr, d_rhs.addr, d_R.addr)
+        d_rhs.axpby(1.0, d_fext, -1.0)                     # rhs = f_ext - f_int
+        d_u.fill(0.0)
+        reason, it, res = solver.solve_dev(K, d_rhs.addr, d_u.addr)
+        ev.internal_forces_dev(d_u.addr, d_up.addr, d_f.addr, d_R.addr, update_state=True)
+        d_R.axpby(-1.0, d_fextp, 1.0)                      # reactions
+        if reason != CR_CONVERGED:
+            raise RuntimeError(f"PCG did not converge: {it} iterations, residual {res:.3e}")
+        return it, res
+
+    for i in range(args.warmup):
+        it, res = step()
+        log(f"[bench] {precision=} warmup {i}: iters={it} res={res:.3e}")
+    ctx.synchronize()
+    n_sp, ms_sp, n32, ms32, iters = 0, 0.0, 0, 0.0, []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        it, res = step()
+        p = solver.profile()
+        n_sp += p["n64"]; ms_sp += p["ms64"]; n32 += p["n32"]; ms32 += p["ms32"]
+        iters.append(it)
+    ctx.synchronize()
+    dt = time.perf_counter() - t0
+    u = d_u.download()
+    R = d_R.download()[: deck.npeq]
+    out = dict(dt=dt, iters=iters, res=res, t_struct=t_struct, nnz=K.nnz(), n_spmv=n_sp, ms_spmv=ms_sp,
+               n_spmv32=n32, ms_spmv32=ms32, outer=solver.last_outer(), u=u, reactions=R)
+    for v in (d_fext, d_fextp, d_up, d_u0, d_rhs, d_u, d_f, d_R):
+        v.close()
+    solver.close(); K.close(); ev.close(); ctx.close()
+    return out
+
+
+def cpu_baseline(args, deck_full, gpu_iters, nnz_full):
+    """Oracle on a bounded slab of the same mesh family, scaled to the full step."""
+    import numpy as np
+
+    from oofem_amd import meshgen
+    from oracle import oracle
+    nzs = args.cpu_sample_nz
+    s = meshgen.config1(nx=args.nx, ny=args.ny, nz=nzs) if args.workload == "cfg1" else meshgen.config0()
+    t = time.perf_counter()
+    rp, ci, val = oracle.csr(s)               # structure + assembly of the sample
+    t_asm = time.perf_counter() - t
+    t = time.perf_counter()
+    oracle.internal_forces(s, np.zeros(s.neq), s.prescribed_values())
+    t_fint = time.perf_counter() - t
+    f, _ = s.external_load()
+    nit = args.cpu_sample_iters
+    t = time.perf_counter()
+    oracle.pcg(rp, ci, val, f, tol=1e-30, maxit=nit)
+    t_cg = (time.perf_counter() - t) / nit
+    scale_e = deck_full.nelem / s.nelem
+    scale_nnz = nnz_full / len(val)
+    t_step = (t_asm + 2 * t_fint) * scale_e + gpu_iters * t_cg * scale_nnz
+    return dict(value=deck_full.neq / t_step, unit="DOFs/s", cores=1, kind="port",
+                sample=(f"oracle (C restatement, 1 thread) on a {args.nx}x{args.ny}x{nzs} slab of the same mesh "
+                        f"family ({s.nelem} elements, {len(val)} nnz): K assembly {t_asm:.2f} s, f_int "
+                        f"{t_fint:.2f} s, {nit} CG iterations at {t_cg * 1e3:.1f} ms each; scaled by elements "
+                        f"x{scale_e:.1f} and nnz x{scale_nnz:.1f} to the full step with the GPU's "
+                        f"{gpu_iters} iterations (estimated step {t_step:.1f} s)"))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", choices=["cfg1", "cfg0"], default="cfg1")
+    ap.add_argument("--nx", type=int, default=64)
+    ap.add_argument("--ny", type=int, default=64)
+    ap.add_argument("--nz", type=int, default=256)
+    ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--maxit", type=int, default=200000)
+    ap.add_argument("--precision", choices=["fp64", "mixed", "both"], default="both")
+    ap.add_argument("--inner-tol", type=float, default=1e-4)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-sample-nz", type=int, default=16)
+    ap.add_argument("--cpu-sample-iters", type=int, default=40)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
+              "(the multi-GPU slab decomposition is a design, not code).", file=sys.stderr)
+        return 2
+
+    t = time.perf_counter()
+    deck = make_deck(args)
+    log(f"[bench] deck {deck.name}: nnode={deck.nnode} nelem={deck.nelem} neq={deck.neq} "
+        f"({time.perf_counter() - t:.1f} s)")
+    modes = {"fp64": [0], "mixed": [1], "both": [0, 1]}[args.precision]
+    res = {}
+    for prec in modes:
+        name = "fp64" if prec == 0 else "mixed"
+        r = run_gpu(args, deck, prec)
+        res[name] = r
+        log(f"[bench] {name}: {args.steps} steps in {r['dt']:.3f} s, iters {r['iters']}, outer {r['outer']}, "
+            f"structure build {r['t_struct']:.1f} s")
+    main_name = "fp64" if 0 in modes else "mixed"
+    r = res[main_name]
+    n, nnz = deck.neq, r["nnz"]
+    ms_per_step = r["dt"] / args.steps * 1e3
+    value = n * args.steps / r["dt"]
+    roof = None
+    if r["n_spmv"]:
+        avg_ms = r["ms_spmv"] / r["n_spmv"]
+        bytes_per = 12.0 * nnz + 36.0 * n
+        ach = bytes_per / (avg_ms * 1e-3) / 1e9
+        roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                    frac=round(ach / HBM_PEAK_GBS, 4), traffic=None, kernel="k_pcg_spmv<16,double,double>",
+                    avg_launch_ms=round(avg_ms, 4), launches=r["n_spmv"], algorithmic_bytes=int(bytes_per))
+    variants = {}
+    for name, rv in res.items():
+        variants[name] = dict(dofs_per_s=n * args.steps / rv["dt"], ms_per_step=rv["dt"] / args.steps * 1e3,
+                              pcg_iters=rv["iters"], outer=rv["outer"], final_resid=rv["res"])
+    if "fp64" in res and "mixed" in res:
+        import numpy as np
+        a, b = res["fp64"], res["mixed"]
+        variants["mixed"]["u_rel_err_vs_fp64"] = float(np.linalg.norm(b["u"] - a["u"]) / np.linalg.norm(a["u"]))
+        variants["mixed"]["speedup_vs_fp64"] = variants["mixed"]["dofs_per_s"] / variants["fp64"]["dofs_per_s"]
+        if b["n_spmv32"]:
+            variants["mixed"]["spmv32_avg_ms"] = b["ms_spmv32"] / b["n_spmv32"]
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(args, deck, int(r["iters"][-1]), nnz)
+        log(f"[bench] cpu baseline: {cpu['value']:.3e} DOFs/s")
+    out = {
+        "metric": METRIC, "value": value, "unit": "DOFs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64" if main_name == "fp64" else "f32-inner/f64-refinement",
+        "data": "synthetic",
+        "config": {"workload": deck.name, "elements": deck.nelem, "free_dofs": n, "nnz": nnz,
+                   "step": "assemble K + rhs + Jacobi-PCG to 1e-8 + reactions", "tol": args.tol,
+                   "precision": main_name, "variants": variants},
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

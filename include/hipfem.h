@@ -1,0 +1,199 @@
+/*
+ * hipfem.h -- C ABI of the MI355X (gfx950) assembly-and-solve path.
+ *
+ * Three drop-in points of the reference are served:
+ *   hipcsr_*  a SparseMtrx (src/core/sparsemtrx.h:60-281), registered in the
+ *             reference as SMT_HipCSR through REGISTER_SparseMtrx
+ *             (src/core/classfactory.h:149);
+ *   hippcg_*  a SparseLinearSystemNM (src/core/sparselinsystemnm.h:56-97),
+ *             registered as ST_HipPCG through REGISTER_SparseLinSolver
+ *             (src/core/classfactory.h:150);
+ *   hfem_model_*  the batched element/material evaluator that replaces the
+ *             element loops of EngngModel::assemble (src/core/engngm.C:889-1037)
+ *             and EngngModel::assembleVectorFromElements (engngm.C:1351-1535)
+ *             for LSpace / LTRSpace / QSpace with IsoLE / MisesMat.
+ *
+ * Conventions
+ *  - Every entry point returns 0 on success and a negative HIPFEM_E* code on
+ *    failure; hipfem_last_error() gives the message (thread-local).  The
+ *    reference raises OOFEM_ERROR (RuntimeException, src/core/error.h:79) on
+ *    the same conditions; its C++ wrapper maps a non-zero status to that.
+ *  - Equation numbers at the boundary are 1-based with 0 = "no equation",
+ *    exactly like the reference's location arrays (element.C:429-455).
+ *  - Dense matrices are row-major.  Host vectors are plain double arrays;
+ *    the *_dev variants take device pointers that live on the context's GPU.
+ *  - One context = one GPU + one HIP stream.  Handles are not re-entrant: a
+ *    handle is used from one host thread at a time, as the reference does.
+ */
+#ifndef HIPFEM_H
+#define HIPFEM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HIPFEM_OK            0
+#define HIPFEM_EINVAL      (-1)   /* bad argument / state */
+#define HIPFEM_EHIP        (-2)   /* HIP runtime error */
+#define HIPFEM_ENOMEM      (-3)   /* allocation failed */
+#define HIPFEM_ERANGE      (-4)   /* index outside the matrix / pattern */
+#define HIPFEM_ENUMERIC    (-5)   /* zero pivot / non-positive volume / NaN */
+#define HIPFEM_ETOOBIG     (-6)   /* size beyond the int32 index range */
+
+/* ConvergedReason values of the reference (src/core/convergedreason.h:41-50) */
+#define HIPFEM_CR_CONVERGED      1
+#define HIPFEM_CR_DIVERGED_ITS (-1)
+
+/* element types (LSpace, LTRSpace, QSpace) and material models */
+#define HIPFEM_LSPACE   1
+#define HIPFEM_LTRSPACE 2
+#define HIPFEM_QSPACE   3
+#define HIPFEM_MAT_ISOLE 0
+#define HIPFEM_MAT_MISES 1
+#define HIPFEM_MATP      8  /* material row: [model, E, nu, sig0, H, 0, 0, 0] */
+
+typedef struct hipfem_ctx    hipfem_ctx;
+typedef struct hipcsr_mtx    hipcsr_mtx;
+typedef struct hippcg_solver hippcg_solver;
+typedef struct hfem_model    hfem_model;
+
+/* ------------------------------------------------------------------ context */
+int         hipfem_ctx_create(int device, hipfem_ctx **out);
+void        hipfem_ctx_destroy(hipfem_ctx *ctx);
+int         hipfem_ctx_synchronize(hipfem_ctx *ctx);
+/* the HIP stream (hipStream_t) every launch of this context goes to */
+void       *hipfem_ctx_stream(hipfem_ctx *ctx);
+const char *hipfem_last_error(void);
+const char *hipfem_version(void);
+
+/* Device vectors of FP64 on the context's GPU (plumbing for callers that keep
+ * their vectors resident; no counterpart in the reference). */
+int hipfem_dev_alloc(hipfem_ctx *ctx, int64_t n, double **out);
+int hipfem_dev_free(hipfem_ctx *ctx, double *p);
+int hipfem_dev_upload(hipfem_ctx *ctx, double *dst, const double *src, int64_t n);
+int hipfem_dev_download(hipfem_ctx *ctx, double *dst, const double *src, int64_t n);
+/* y = a*x + b*y on the context stream (x may be NULL: y = b*y) */
+int hipfem_dev_axpby(hipfem_ctx *ctx, int64_t n, double a, const double *x, double b, double *y);
+/* dot product x.y and 2-norm, FP64, fixed summation order */
+int hipfem_dev_dot(hipfem_ctx *ctx, int64_t n, const double *x, const double *y, double *out);
+
+/* ------------------------------------------------------------------- hipcsr */
+/* SparseMtrx constructor / clone()  (sparsemtrx.h:95-99) */
+int  hipcsr_create(hipfem_ctx *ctx, hipcsr_mtx **out);
+void hipcsr_destroy(hipcsr_mtx *m);
+int  hipcsr_clone(const hipcsr_mtx *m, hipcsr_mtx **out);
+
+/* buildInternalStructure(EngngModel*, di, s) (sparsemtrx.h:198; compcol.C:174-272):
+ * the pattern is the union of loc x loc over the element location arrays the
+ * caller passes (nelem arrays of ndofe 1-based equations, 0 = prescribed). */
+int hipcsr_build_structure(hipcsr_mtx *m, int neq, int nelem, int ndofe, const int *loc);
+/* buildInternalStructure(EngngModel*, n, m, I, J) (sparsemtrx.h:192): pattern from
+ * a full CSR description (0-based sorted columns). */
+int hipcsr_build_structure_csr(hipcsr_mtx *m, int n, const int64_t *rowptr, const int32_t *colind);
+
+/* assemble(loc, mat) / assemble(rloc, cloc, mat) (sparsemtrx.h:220,231):
+ * add mat(i,j) at (loc[i], loc[j]) for loc > 0.  Contributions are staged and
+ * reach the device at assembleEnd(). */
+int hipcsr_assemble(hipcsr_mtx *m, int n, const int *loc, const double *mat);
+int hipcsr_assemble_rc(hipcsr_mtx *m, int nr, const int *rloc, int nc, const int *cloc, const double *mat);
+int hipcsr_assemble_begin(hipcsr_mtx *m);   /* sparsemtrx.h:234 */
+int hipcsr_assemble_end(hipcsr_mtx *m);     /* sparsemtrx.h:236 */
+
+int hipcsr_zero(hipcsr_mtx *m);                          /* zero()        :254 */
+int hipcsr_scale(hipcsr_mtx *m, double a);               /* times(double)  :155 */
+int hipcsr_add(hipcsr_mtx *m, double a, const hipcsr_mtx *other); /* add(x, m) :157 (same pattern) */
+int hipcsr_add_diagonal(hipcsr_mtx *m, double a, const double *d);  /* addDiagonal :160 */
+
+/* times(x, answer) / timesT (sparsemtrx.h:137,143): y = A x, y = A^T x */
+int hipcsr_times(hipcsr_mtx *m, const double *x, double *y);
+int hipcsr_times_t(hipcsr_mtx *m, const double *x, double *y);
+int hipcsr_times_dev(hipcsr_mtx *m, const double *dx, double *dy);
+/* times(FloatMatrix B, answer) (sparsemtrx.h:149): B is n x ncol row-major */
+int hipcsr_times_mat(hipcsr_mtx *m, int ncol, const double *B, double *Y);
+
+/* at(i,j) (sparsemtrx.h:265-267), 1-based; a missing entry reads as 0 and
+ * cannot be written (HIPFEM_ERANGE), like CompCol::at. */
+int hipcsr_get(hipcsr_mtx *m, int i, int j, double *v);
+int hipcsr_set(hipcsr_mtx *m, int i, int j, double v);
+int hipcsr_is_allocated(hipcsr_mtx *m, int i, int j);    /* isAllocatedAt :270 */
+
+int     hipcsr_nrows(const hipcsr_mtx *m);
+int64_t hipcsr_nnz(const hipcsr_mtx *m);
+long    hipcsr_version(const hipcsr_mtx *m);             /* giveVersion()  :82 */
+/* copies of the pattern / values for inspection (host arrays sized n+1, nnz) */
+int hipcsr_download(hipcsr_mtx *m, int64_t *rowptr, int32_t *colind, double *val);
+int hipcsr_upload_values(hipcsr_mtx *m, const double *val);
+/* device pointers of the CSR arrays (row pointers and columns are int32) */
+int hipcsr_device_arrays(hipcsr_mtx *m, const int32_t **rowptr, const int32_t **colind, double **val);
+
+/* ------------------------------------------------------------------- hippcg */
+/* IMLSolver-compatible options (imlsolver.C:67-101): lstol (default 1e-5),
+ * lsiter (default 200), lsprecond (0 void, 1 diagonal). */
+int  hippcg_create(hipfem_ctx *ctx, hippcg_solver **out);
+void hippcg_destroy(hippcg_solver *s);
+int  hippcg_set_options(hippcg_solver *s, double lstol, int lsiter, int lsprecond);
+/* precision: 0 = FP64 PCG; 1 = FP32 inner PCG inside FP64 iterative
+ * refinement (true residual, update and dot products in FP64).
+ * inner_tol: relative reduction asked of each inner solve. */
+int  hippcg_set_precision(hippcg_solver *s, int precision, double inner_tol, int max_outer);
+/* solve(A, b, x) (sparselinsystemnm.h:81; imlsolver.C:102-146): x holds the
+ * initial guess on entry.  *reason receives HIPFEM_CR_CONVERGED or
+ * HIPFEM_CR_DIVERGED_ITS; *iters the (inner) iterations; *achieved the final
+ * relative residual. */
+int hippcg_solve(hippcg_solver *s, hipcsr_mtx *A, const double *b, double *x,
+                 int *reason, int *iters, double *achieved);
+int hippcg_solve_dev(hippcg_solver *s, hipcsr_mtx *A, const double *db, double *dx,
+                     int *reason, int *iters, double *achieved);
+/* outer iterations of the last mixed-precision solve */
+int hippcg_last_outer(hippcg_solver *s);
+/* Instrumentation (no counterpart in the reference): when on, every PCG
+ * SpMV launch of a solve is bracketed by HIP events on the context stream;
+ * hippcg_profile returns the number of timed launches of the last solve
+ * (iterations up to convergence) and their summed duration in ms, and
+ * the FP32 inner launches separately. */
+int hippcg_set_profiling(hippcg_solver *s, int on);
+int hippcg_profile(hippcg_solver *s, int *n_spmv64, double *ms_spmv64, int *n_spmv32, double *ms_spmv32);
+
+/* --------------------------------------------------- batched evaluator */
+int  hfem_model_create(hipfem_ctx *ctx, hfem_model **out);
+void hfem_model_destroy(hfem_model *md);
+/* mesh in the reference's local node order; conn is nelem x nen (0-based);
+ * emat indexes the material table mat (nmat x HIPFEM_MATP); eqn is
+ * nnode x 3 with >0 free equation, <0 minus the prescribed equation, 0 none
+ * (forceEquationNumbering engngm.C:469-531). */
+int hfem_model_set_mesh(hfem_model *md, int nnode, const double *coords, int etype, int nelem,
+                        const int *conn, const int *emat, int nmat, const double *mat,
+                        const int *eqn, int neq, int npeq);
+/* pattern of the free-free stiffness from the model's location arrays
+ * (same result as hipcsr_build_structure over every element), plus the
+ * element->pattern offset map and the element coloring used by assembly. */
+int hfem_model_build_structure(hfem_model *md, hipcsr_mtx *K);
+/* K = sum_e K_e (TangentAssembler assemblercallback.C:246 -> NLStructuralElement::
+ * computeStiffnessMatrix nlstructuralelement.C:332-448).  mode 0: elastic
+ * stiffness; mode 1: consistent tangent from the temp GP state (MisesMat
+ * give3dMaterialStiffnessMatrix misesmat.C:496-544). */
+int hfem_model_assemble_stiffness(hfem_model *md, hipcsr_mtx *K, int mode);
+/* f = sum_e B^T sigma dV (NLStructuralElement::giveInternalForcesVector
+ * nlstructuralelement.C:151-241), MisesMat return mapping per GP
+ * (misesmat.C:181-263) writing the temp state when update_state != 0.
+ * f_free (neq), f_presc (npeq, may be NULL), ebe3 (3 doubles, may be NULL):
+ * sum of squared element force components per dof id (u, v, w). */
+int hfem_model_internal_forces(hfem_model *md, const double *u_free, const double *u_presc,
+                               double *f_free, double *f_presc, double *ebe3, int update_state);
+int hfem_model_internal_forces_dev(hfem_model *md, const double *du_free, const double *du_presc,
+                                   double *df_free, double *df_presc, double *ebe3, int update_state);
+/* MisesMatStatus::updateYourself (misesmat.C:720-728) for every Gauss point */
+int hfem_model_commit(hfem_model *md);
+/* reset every Gauss point to the virgin state */
+int hfem_model_reset_state(hfem_model *md);
+/* Gauss point state, ngp_total x 38 doubles in the oracle's layout */
+int hfem_model_get_state(hfem_model *md, double *state);
+int hfem_model_ngp_total(const hfem_model *md);
+int hfem_model_ncolors(const hfem_model *md);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

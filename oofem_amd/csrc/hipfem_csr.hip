@@ -1,0 +1,790 @@
+// hipcsr: the SparseMtrx of the MI355X path (reference interface
+// src/core/sparsemtrx.h:60-281; storage replaces CompCol/SymCompCol,
+// compcol.C:128-316, symcompcol.C:98-222).
+//
+// Storage: full (unsymmetric) CSR, int32 row pointers and columns, FP64
+// values, sorted columns.  The matrix lives in HBM; a host copy of the
+// pattern serves assemble()/at() lookups.  SpMV: a group of LANES lanes per
+// row (LANES in {4, 8, 16, 32, 64}, picked from the mean row length) reads
+// the row's values and columns with unit stride, gathers x, and reduces with
+// cross-lane shuffles; one wave holds 64/LANES rows.
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+
+#include "hipfem_internal.h"
+
+namespace hipfem {
+
+static thread_local std::string g_err;
+
+void set_error(const char *fmt, ...)
+{
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    g_err = buf;
+}
+
+int ensure_partials(hipfem_ctx *ctx, size_t n)
+{
+    if (ctx->partials_cap >= n) return 0;
+    if (ctx->d_partials) HIPFEM_CHECK(hipFree(ctx->d_partials));
+    ctx->d_partials = nullptr;
+    size_t cap = std::max<size_t>(n, 8192);
+    HIPFEM_CHECK(hipMalloc(&ctx->d_partials, cap * sizeof(double)));
+    ctx->partials_cap = cap;
+    return 0;
+}
+
+// ------------------------------------------------------------------ kernels
+template <int LANES, typename VT>
+__global__ __launch_bounds__(256) void k_spmv(int n, const int32_t *__restrict__ rowptr,
+                                              const int32_t *__restrict__ col,
+                                              const VT *__restrict__ val,
+                                              const double *__restrict__ x, double *__restrict__ y)
+{
+    const int rows_per_block = 256 / LANES;
+    const int lane = threadIdx.x % LANES;
+    for (int row = blockIdx.x * rows_per_block + threadIdx.x / LANES; row < n;
+         row += gridDim.x * rows_per_block) {
+        const int s = rowptr[row], e = rowptr[row + 1];
+        double sum = 0.0;
+        for (int k = s + lane; k < e; k += LANES) sum += (double)val[k] * x[col[k]];
+#pragma unroll
+        for (int off = LANES / 2; off > 0; off >>= 1) sum += __shfl_xor(sum, off, LANES);
+        if (lane == 0) y[row] = sum;
+    }
+}
+
+__global__ void k_diag_pos(int n, const int32_t *__restrict__ rowptr, const int32_t *__restrict__ col,
+                           int32_t *__restrict__ diag)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    int lo = rowptr[r], hi = rowptr[r + 1] - 1, pos = -1;
+    while (lo <= hi) {
+        const int mid = (lo + hi) >> 1;
+        const int c = col[mid];
+        if (c == r) { pos = mid; break; }
+        if (c < r) lo = mid + 1; else hi = mid - 1;
+    }
+    diag[r] = pos;
+}
+
+__global__ void k_axpy_vals(int64_t nnz, double a, const double *__restrict__ src, double *__restrict__ dst)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nnz; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] += a * src[i];
+}
+
+__global__ void k_scale_vals(int64_t nnz, double a, double *__restrict__ v)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nnz; i += (int64_t)gridDim.x * blockDim.x)
+        v[i] *= a;
+}
+
+__global__ void k_to_fp32(int64_t nnz, const double *__restrict__ v, float *__restrict__ o)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nnz; i += (int64_t)gridDim.x * blockDim.x)
+        o[i] = (float)v[i];
+}
+
+__global__ void k_add_diag(int n, double a, const double *__restrict__ d, const int32_t *__restrict__ diag,
+                           double *__restrict__ v)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < n && diag[r] >= 0) v[diag[r]] += a * d[r];
+}
+
+// y = A^T x through the transpose pattern: tmap[k] is the position in val of
+// the k-th transpose entry, so each output row is a plain gather-sum.
+__global__ void k_spmv_t(int n, const int32_t *__restrict__ tptr, const int32_t *__restrict__ tcol,
+                         const int32_t *__restrict__ tmap, const double *__restrict__ val,
+                         const double *__restrict__ x, double *__restrict__ y)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    double s = 0.0;
+    for (int k = tptr[r]; k < tptr[r + 1]; k++) s += val[tmap[k]] * x[tcol[k]];
+    y[r] = s;
+}
+
+static int grid_for(int64_t work, int block = 256, int64_t cap = 8192)
+{
+    int64_t g = (work + block - 1) / block;
+    return (int)std::max<int64_t>(1, std::min(g, cap));
+}
+
+int spmv_lanes(const hipcsr_mtx *m)
+{
+    const double a = m->avg_row;
+    if (a <= 6) return 4;
+    if (a <= 12) return 8;
+    if (a <= 40) return 16;
+    if (a <= 120) return 16;
+    return 32;
+}
+
+template <typename VT>
+static void launch_spmv(int lanes, hipStream_t st, int n, const int32_t *rp, const int32_t *ci, const VT *v,
+                        const double *x, double *y)
+{
+    const int rows_per_block = 256 / lanes;
+    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((n + rows_per_block - 1) / rows_per_block, 1 << 20));
+    switch (lanes) {
+    case 4: k_spmv<4, VT><<<grid, 256, 0, st>>>(n, rp, ci, v, x, y); break;
+    case 8: k_spmv<8, VT><<<grid, 256, 0, st>>>(n, rp, ci, v, x, y); break;
+    case 16: k_spmv<16, VT><<<grid, 256, 0, st>>>(n, rp, ci, v, x, y); break;
+    case 32: k_spmv<32, VT><<<grid, 256, 0, st>>>(n, rp, ci, v, x, y); break;
+    default: k_spmv<64, VT><<<grid, 256, 0, st>>>(n, rp, ci, v, x, y); break;
+    }
+}
+
+int csr_spmv_dev(hipcsr_mtx *m, const double *dx, double *dy)
+{
+    launch_spmv<double>(spmv_lanes(m), m->ctx->stream, m->n, m->d_rowptr, m->d_colind, m->d_val, dx, dy);
+    HIPFEM_CHECK(hipGetLastError());
+    return 0;
+}
+
+int csr_free(hipcsr_mtx *m)
+{
+    for (void *p : {(void *)m->d_rowptr, (void *)m->d_colind, (void *)m->d_diag, (void *)m->d_val,
+                    (void *)m->d_val32, (void *)m->d_tptr, (void *)m->d_tcol, (void *)m->d_tmap})
+        if (p) HIPFEM_CHECK(hipFree(p));
+    m->d_rowptr = m->d_colind = m->d_diag = m->d_tptr = m->d_tcol = m->d_tmap = nullptr;
+    m->d_val = nullptr;
+    m->d_val32 = nullptr;
+    m->has_structure = false;
+    return 0;
+}
+
+int csr_alloc(hipcsr_mtx *m, int n, const std::vector<int32_t> &rowptr, const std::vector<int32_t> &colind)
+{
+    int rc = csr_free(m);
+    if (rc) return rc;
+    m->n = n;
+    m->nnz = (int64_t)colind.size();
+    m->h_rowptr = rowptr;
+    m->h_colind = colind;
+    m->avg_row = n ? (double)m->nnz / n : 0.0;
+    HIPFEM_CHECK(hipSetDevice(m->ctx->device));
+    HIPFEM_CHECK(hipMalloc(&m->d_rowptr, sizeof(int32_t) * (n + 1)));
+    HIPFEM_CHECK(hipMalloc(&m->d_colind, sizeof(int32_t) * std::max<int64_t>(1, m->nnz)));
+    HIPFEM_CHECK(hipMalloc(&m->d_diag, sizeof(int32_t) * std::max(1, n)));
+    HIPFEM_CHECK(hipMalloc(&m->d_val, sizeof(double) * std::max<int64_t>(1, m->nnz)));
+    hipStream_t st = m->ctx->stream;
+    HIPFEM_CHECK(hipMemcpyAsync(m->d_rowptr, rowptr.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, st));
+    if (m->nnz)
+        HIPFEM_CHECK(hipMemcpyAsync(m->d_colind, colind.data(), sizeof(int32_t) * m->nnz, hipMemcpyHostToDevice, st));
+    HIPFEM_CHECK(hipMemsetAsync(m->d_val, 0, sizeof(double) * std::max<int64_t>(1, m->nnz), st));
+    if (n) {
+        k_diag_pos<<<(n + 255) / 256, 256, 0, st>>>(n, m->d_rowptr, m->d_colind, m->d_diag);
+        HIPFEM_CHECK(hipGetLastError());
+    }
+    HIPFEM_CHECK(hipStreamSynchronize(st));
+    m->has_structure = true;
+    m->pattern_id = next_pattern_id();
+    m->h_stage.clear();
+    m->staged = false;
+    m->version++;
+    return 0;
+}
+
+int csr_refresh_fp32(hipcsr_mtx *m)
+{
+    if (m->d_val32 && m->val32_version == m->version) return 0;
+    if (!m->d_val32) HIPFEM_CHECK(hipMalloc(&m->d_val32, sizeof(float) * std::max<int64_t>(1, m->nnz)));
+    k_to_fp32<<<grid_for(m->nnz), 256, 0, m->ctx->stream>>>(m->nnz, m->d_val, m->d_val32);
+    HIPFEM_CHECK(hipGetLastError());
+    m->val32_version = m->version;
+    return 0;
+}
+
+int spmv_fp32_dev(hipcsr_mtx *m, const double *dx, double *dy)
+{
+    launch_spmv<float>(spmv_lanes(m), m->ctx->stream, m->n, m->d_rowptr, m->d_colind, m->d_val32, dx, dy);
+    HIPFEM_CHECK(hipGetLastError());
+    return 0;
+}
+
+static int64_t find_pos(const hipcsr_mtx *m, int r, int c)
+{
+    const int32_t *b = m->h_colind.data() + m->h_rowptr[r];
+    const int32_t *e = m->h_colind.data() + m->h_rowptr[r + 1];
+    const int32_t *p = std::lower_bound(b, e, c);
+    if (p == e || *p != c) return -1;
+    return p - m->h_colind.data();
+}
+
+long next_pattern_id()
+{
+    static long id = 0;
+    return ++id;
+}
+
+static int flush_stage(hipcsr_mtx *m)
+{
+    if (!m->staged) return 0;
+    double *tmp = nullptr;
+    HIPFEM_CHECK(hipMalloc(&tmp, sizeof(double) * m->nnz));
+    HIPFEM_CHECK(hipMemcpyAsync(tmp, m->h_stage.data(), sizeof(double) * m->nnz, hipMemcpyHostToDevice, m->ctx->stream));
+    k_axpy_vals<<<grid_for(m->nnz), 256, 0, m->ctx->stream>>>(m->nnz, 1.0, tmp, m->d_val);
+    HIPFEM_CHECK(hipGetLastError());
+    HIPFEM_CHECK(hipStreamSynchronize(m->ctx->stream));
+    HIPFEM_CHECK(hipFree(tmp));
+    m->h_stage.clear();
+    m->h_stage.shrink_to_fit();
+    m->staged = false;
+    m->version++;
+    return 0;
+}
+
+}  // namespace hipfem
+
+using namespace hipfem;
+
+// ================================================================== context
+extern "C" int hipfem_ctx_create(int device, hipfem_ctx **out)
+{
+    HIPFEM_REQUIRE(out, HIPFEM_EINVAL, "hipfem_ctx_create: out is NULL");
+    int ndev = 0;
+    HIPFEM_CHECK(hipGetDeviceCount(&ndev));
+    HIPFEM_REQUIRE(device >= 0 && device < ndev, HIPFEM_EINVAL, "hipfem_ctx_create: device %d of %d", device, ndev);
+    DeviceGuard g(device);
+    hipfem_ctx *c = new hipfem_ctx();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        set_error("hipStreamCreateWithFlags: %s", hipGetErrorString(e));
+        return HIPFEM_EHIP;
+    }
+    e = hipHostMalloc((void **)&c->h_pinned, 64 * sizeof(double), hipHostMallocDefault);
+    if (e != hipSuccess) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        set_error("hipHostMalloc: %s", hipGetErrorString(e));
+        return HIPFEM_EHIP;
+    }
+    *out = c;
+    return 0;
+}
+
+static void ctx_free(hipfem_ctx *c)
+{
+    DeviceGuard g(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    if (c->d_partials) (void)hipFree(c->d_partials);
+    if (c->h_pinned) (void)hipHostFree(c->h_pinned);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+namespace hipfem {
+void ctx_retain(hipfem_ctx *c) { c->refs++; }
+void ctx_release(hipfem_ctx *c)
+{
+    if (--c->refs == 0 && c->destroy_pending) ctx_free(c);
+}
+}  // namespace hipfem
+
+extern "C" void hipfem_ctx_destroy(hipfem_ctx *c)
+{
+    if (!c) return;
+    if (c->refs > 0) {
+        c->destroy_pending = true;   // freed by the last object's destroy
+        return;
+    }
+    ctx_free(c);
+}
+
+extern "C" int hipfem_ctx_synchronize(hipfem_ctx *c)
+{
+    HIPFEM_REQUIRE(c, HIPFEM_EINVAL, "null context");
+    HIPFEM_CHECK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" void *hipfem_ctx_stream(hipfem_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+// ------------------------------------------------------------ device vectors
+namespace hipfem {
+__global__ void k_axpby(int64_t n, double a, const double *__restrict__ x, double b, double *__restrict__ y)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        y[i] = (x ? a * x[i] : 0.0) + (b != 0.0 ? b * y[i] : 0.0);  // b = 0 never reads y
+}
+
+__global__ __launch_bounds__(256) void k_dot_part(int64_t n, const double *__restrict__ x, const double *__restrict__ y,
+                                                  double *__restrict__ part)
+{
+    __shared__ double sh[4];
+    double s = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) s += x[i] * y[i];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (threadIdx.x % 64 == 0) sh[threadIdx.x / 64] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+
+__global__ void k_dot_final(int nb, const double *__restrict__ part, double *__restrict__ out)
+{
+    __shared__ double sh[4];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nb; i += 256) s += part[i];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if (threadIdx.x % 64 == 0) sh[threadIdx.x / 64] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) out[0] = sh[0] + sh[1] + sh[2] + sh[3];
+}
+}  // namespace hipfem
+
+extern "C" int hipfem_dev_alloc(hipfem_ctx *c, int64_t n, double **out)
+{
+    HIPFEM_REQUIRE(c && out && n >= 0, HIPFEM_EINVAL, "dev_alloc: bad arguments");
+    DeviceGuard g(c->device);
+    HIPFEM_CHECK(hipMalloc((void **)out, sizeof(double) * std::max<int64_t>(1, n)));
+    return 0;
+}
+
+extern "C" int hipfem_dev_free(hipfem_ctx *c, double *p)
+{
+    HIPFEM_REQUIRE(c, HIPFEM_EINVAL, "dev_free: null context");
+    DeviceGuard g(c->device);
+    HIPFEM_CHECK(hipStreamSynchronize(c->stream));
+    if (p) HIPFEM_CHECK(hipFree(p));
+    return 0;
+}
+
+extern "C" int hipfem_dev_upload(hipfem_ctx *c, double *dst, const double *src, int64_t n)
+{
+    HIPFEM_REQUIRE(c && (n == 0 || (dst && src)), HIPFEM_EINVAL, "dev_upload: bad arguments");
+    DeviceGuard g(c->device);
+    if (n) HIPFEM_CHECK(hipMemcpyAsync(dst, src, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    HIPFEM_CHECK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int hipfem_dev_download(hipfem_ctx *c, double *dst, const double *src, int64_t n)
+{
+    HIPFEM_REQUIRE(c && (n == 0 || (dst && src)), HIPFEM_EINVAL, "dev_download: bad arguments");
+    DeviceGuard g(c->device);
+    if (n) HIPFEM_CHECK(hipMemcpyAsync(dst, src, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream));
+    HIPFEM_CHECK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int hipfem_dev_axpby(hipfem_ctx *c, int64_t n, double a, const double *x, double b, double *y)
+{
+    HIPFEM_REQUIRE(c && (n == 0 || y), HIPFEM_EINVAL, "dev_axpby: bad arguments");
+    DeviceGuard g(c->device);
+    if (n) {
+        k_axpby<<<grid_for(n), 256, 0, c->stream>>>(n, a, x, b, y);
+        HIPFEM_CHECK(hipGetLastError());
+    }
+    return 0;
+}
+
+extern "C" int hipfem_dev_dot(hipfem_ctx *c, int64_t n, const double *x, const double *y, double *out)
+{
+    HIPFEM_REQUIRE(c && out && (n == 0 || (x && y)), HIPFEM_EINVAL, "dev_dot: bad arguments");
+    DeviceGuard g(c->device);
+    const int nb = grid_for(n, 256, 1024);
+    int rc = ensure_partials(c, (size_t)nb + 1);
+    if (rc) return rc;
+    if (n) {
+        k_dot_part<<<nb, 256, 0, c->stream>>>(n, x, y, c->d_partials);
+        k_dot_final<<<1, 256, 0, c->stream>>>(nb, c->d_partials, c->d_partials + nb);
+        HIPFEM_CHECK(hipGetLastError());
+        HIPFEM_CHECK(hipMemcpyAsync(c->h_pinned, c->d_partials + nb, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPFEM_CHECK(hipStreamSynchronize(c->stream));
+        *out = c->h_pinned[0];
+    } else {
+        *out = 0.0;
+    }
+    return 0;
+}
+extern "C" const char *hipfem_last_error(void) { return g_err.c_str(); }
+extern "C" const char *hipfem_version(void) { return "hipfem 0.1 gfx950"; }
+
+// =================================================================== hipcsr
+extern "C" int hipcsr_create(hipfem_ctx *ctx, hipcsr_mtx **out)
+{
+    HIPFEM_REQUIRE(ctx && out, HIPFEM_EINVAL, "hipcsr_create: null argument");
+    hipcsr_mtx *m = new hipcsr_mtx();
+    m->ctx = ctx;
+    ctx_retain(ctx);
+    *out = m;
+    return 0;
+}
+
+extern "C" void hipcsr_destroy(hipcsr_mtx *m)
+{
+    if (!m) return;
+    hipfem_ctx *c = m->ctx;
+    {
+        DeviceGuard g(c->device);
+        (void)hipStreamSynchronize(c->stream);
+        csr_free(m);
+        delete m;
+    }
+    ctx_release(c);
+}
+
+extern "C" int hipcsr_clone(const hipcsr_mtx *src, hipcsr_mtx **out)
+{
+    HIPFEM_REQUIRE(src && out, HIPFEM_EINVAL, "hipcsr_clone: null argument");
+    DeviceGuard g(src->ctx->device);
+    hipcsr_mtx *m = new hipcsr_mtx();
+    m->ctx = src->ctx;
+    if (src->has_structure) {
+        int rc = csr_alloc(m, src->n, src->h_rowptr, src->h_colind);
+        if (rc) { delete m; return rc; }
+        HIPFEM_CHECK(hipMemcpyAsync(m->d_val, src->d_val, sizeof(double) * src->nnz, hipMemcpyDeviceToDevice, m->ctx->stream));
+        HIPFEM_CHECK(hipStreamSynchronize(m->ctx->stream));
+        m->h_stage = src->h_stage;
+        m->staged = src->staged;
+    }
+    m->version = src->version;
+    ctx_retain(m->ctx);
+    *out = m;
+    return 0;
+}
+
+extern "C" int hipcsr_build_structure(hipcsr_mtx *m, int neq, int nelem, int ndofe, const int *loc)
+{
+    HIPFEM_REQUIRE(m && neq >= 0 && nelem >= 0 && ndofe > 0 && (loc || nelem == 0), HIPFEM_EINVAL,
+                   "hipcsr_build_structure: bad arguments");
+    DeviceGuard g(m->ctx->device);
+    // rows as sorted unique column lists (CompCol::buildInternalStructure semantics)
+    std::vector<std::vector<int32_t>> rows((size_t)neq);
+    for (int e = 0; e < nelem; e++) {
+        const int *l = loc + (size_t)e * ndofe;
+        for (int i = 0; i < ndofe; i++) {
+            const int ii = l[i];
+            if (ii <= 0) continue;
+            HIPFEM_REQUIRE(ii <= neq, HIPFEM_ERANGE, "location %d > neq %d (element %d)", ii, neq, e + 1);
+            for (int j = 0; j < ndofe; j++)
+                if (l[j] > 0) rows[ii - 1].push_back(l[j] - 1);
+        }
+    }
+    std::vector<int32_t> rowptr((size_t)neq + 1, 0), colind;
+    int64_t total = 0;
+    for (int r = 0; r < neq; r++) {
+        auto &v = rows[r];
+        std::sort(v.begin(), v.end());
+        v.erase(std::unique(v.begin(), v.end()), v.end());
+        total += (int64_t)v.size();
+        HIPFEM_REQUIRE(total < INT32_MAX, HIPFEM_ETOOBIG, "pattern exceeds int32 indices");
+        rowptr[r + 1] = (int32_t)total;
+    }
+    colind.reserve((size_t)total);
+    for (auto &v : rows) colind.insert(colind.end(), v.begin(), v.end());
+    return csr_alloc(m, neq, rowptr, colind);
+}
+
+extern "C" int hipcsr_build_structure_csr(hipcsr_mtx *m, int n, const int64_t *rowptr, const int32_t *colind)
+{
+    HIPFEM_REQUIRE(m && n >= 0 && rowptr && (colind || rowptr[n] == 0), HIPFEM_EINVAL,
+                   "hipcsr_build_structure_csr: bad arguments");
+    HIPFEM_REQUIRE(rowptr[n] < INT32_MAX, HIPFEM_ETOOBIG, "pattern exceeds int32 indices");
+    DeviceGuard g(m->ctx->device);
+    std::vector<int32_t> rp((size_t)n + 1), ci((size_t)rowptr[n]);
+    for (int i = 0; i <= n; i++) rp[i] = (int32_t)rowptr[i];
+    for (int64_t k = 0; k < rowptr[n]; k++) {
+        HIPFEM_REQUIRE(colind[k] >= 0 && colind[k] < n, HIPFEM_ERANGE, "column %d out of range", colind[k]);
+        ci[k] = colind[k];
+    }
+    for (int r = 0; r < n; r++)
+        for (int k = rp[r] + 1; k < rp[r + 1]; k++)
+            HIPFEM_REQUIRE(ci[k - 1] < ci[k], HIPFEM_EINVAL, "columns of row %d not sorted/unique", r);
+    return csr_alloc(m, n, rp, ci);
+}
+
+static int stage_add(hipcsr_mtx *m, int r, int c, double v)
+{
+    if (!m->staged) {
+        m->h_stage.assign((size_t)m->nnz, 0.0);
+        m->staged = true;
+    }
+    const int64_t p = find_pos(m, r, c);
+    HIPFEM_REQUIRE(p >= 0, HIPFEM_ERANGE, "assemble: entry (%d,%d) is not in the pattern", r + 1, c + 1);
+    m->h_stage[p] += v;
+    return 0;
+}
+
+extern "C" int hipcsr_assemble(hipcsr_mtx *m, int n, const int *loc, const double *mat)
+{
+    return hipcsr_assemble_rc(m, n, loc, n, loc, mat);
+}
+
+extern "C" int hipcsr_assemble_rc(hipcsr_mtx *m, int nr, const int *rloc, int nc, const int *cloc, const double *mat)
+{
+    HIPFEM_REQUIRE(m && m->has_structure, HIPFEM_EINVAL, "assemble: matrix has no structure");
+    HIPFEM_REQUIRE(nr >= 0 && nc >= 0 && (nr == 0 || (rloc && mat)) && (nc == 0 || cloc), HIPFEM_EINVAL,
+                   "assemble: bad arguments");
+    for (int i = 0; i < nr; i++) {
+        const int ii = rloc[i];
+        if (ii <= 0) continue;
+        HIPFEM_REQUIRE(ii <= m->n, HIPFEM_ERANGE, "assemble: row %d > %d", ii, m->n);
+        for (int j = 0; j < nc; j++) {
+            const int jj = cloc[j];
+            if (jj <= 0) continue;
+            HIPFEM_REQUIRE(jj <= m->n, HIPFEM_ERANGE, "assemble: column %d > %d", jj, m->n);
+            int rc = stage_add(m, ii - 1, jj - 1, mat[(size_t)i * nc + j]);
+            if (rc) return rc;
+        }
+    }
+    return 0;
+}
+
+extern "C" int hipcsr_assemble_begin(hipcsr_mtx *m)
+{
+    HIPFEM_REQUIRE(m, HIPFEM_EINVAL, "null matrix");
+    return 0;
+}
+
+extern "C" int hipcsr_assemble_end(hipcsr_mtx *m)
+{
+    HIPFEM_REQUIRE(m, HIPFEM_EINVAL, "null matrix");
+    DeviceGuard g(m->ctx->device);
+    int rc = flush_stage(m);
+    if (rc) return rc;
+    HIPFEM_CHECK(hipStreamSynchronize(m->ctx->stream));
+    return 0;
+}
+
+extern "C" int hipcsr_zero(hipcsr_mtx *m)
+{
+    HIPFEM_REQUIRE(m && m->has_structure, HIPFEM_EINVAL, "zero: matrix has no structure");
+    DeviceGuard g(m->ctx->device);
+    if (m->nnz) {
+        // fill kernel rather than a memset node, so callers may capture it
+        k_scale_vals<<<grid_for(m->nnz), 256, 0, m->ctx->stream>>>(m->nnz, 0.0, m->d_val);
+        HIPFEM_CHECK(hipGetLastError());
+    }
+    m->h_stage.clear();
+    m->staged = false;
+    m->version++;
+    return 0;
+}
+
+extern "C" int hipcsr_scale(hipcsr_mtx *m, double a)
+{
+    HIPFEM_REQUIRE(m && m->has_structure, HIPFEM_EINVAL, "times(double): no structure");
+    DeviceGuard g(m->ctx->device);
+    int rc = flush_stage(m);
+    if (rc) return rc;
+    k_scale_vals<<<grid_for(m->nnz), 256, 0, m->ctx->stream>>>(m->nnz, a, m->d_val);
+    HIPFEM_CHECK(hipGetLastError());
+    m->version++;
+    return 0;
+}
+
+extern "C" int hipcsr_add(hipcsr_mtx *m, double a, const hipcsr_mtx *o)
+{
+    HIPFEM_REQUIRE(m && o && m->has_structure && o->has_structure, HIPFEM_EINVAL, "add: no structure");
+    HIPFEM_REQUIRE(m->n == o->n && m->nnz == o->nnz && m->h_rowptr == o->h_rowptr && m->h_colind == o->h_colind,
+                   HIPFEM_EINVAL, "add: patterns differ");
+    DeviceGuard g(m->ctx->device);
+    int rc = flush_stage(m);
+    if (rc) return rc;
+    HIPFEM_REQUIRE(!o->staged, HIPFEM_EINVAL, "add: operand has unflushed assemble() contributions");
+    HIPFEM_CHECK(hipStreamSynchronize(o->ctx->stream));
+    k_axpy_vals<<<grid_for(m->nnz), 256, 0, m->ctx->stream>>>(m->nnz, a, o->d_val, m->d_val);
+    HIPFEM_CHECK(hipGetLastError());
+    m->version++;
+    return 0;
+}
+
+extern "C" int hipcsr_add_diagonal(hipcsr_mtx *m, double a, const double *d)
+{
+    HIPFEM_REQUIRE(m && m->has_structure && d, HIPFEM_EINVAL, "addDiagonal: bad arguments");
+    DeviceGuard g(m->ctx->device);
+    for (int i = 0; i < m->n; i++)
+        HIPFEM_REQUIRE(find_pos(m, i, i) >= 0 || d[i] == 0.0, HIPFEM_ERANGE, "addDiagonal: (%d,%d) not allocated", i + 1, i + 1);
+    int rc = flush_stage(m);
+    if (rc) return rc;
+    double *dd = nullptr;
+    HIPFEM_CHECK(hipMalloc(&dd, sizeof(double) * std::max(1, m->n)));
+    HIPFEM_CHECK(hipMemcpyAsync(dd, d, sizeof(double) * m->n, hipMemcpyHostToDevice, m->ctx->stream));
+    k_add_diag<<<(m->n + 255) / 256, 256, 0, m->ctx->stream>>>(m->n, a, dd, m->d_diag, m->d_val);
+    HIPFEM_CHECK(hipGetLastError());
+    HIPFEM_CHECK(hipStreamSynchronize(m->ctx->stream));
+    HIPFEM_CHECK(hipFree(dd));
+    m->version++;
+    return 0;
+}
+
+extern "C" int hipcsr_times_dev(hipcsr_mtx *m, const double *dx, double *dy)
+{
+    HIPFEM_REQUIRE(m && m->has_structure && dx && dy, HIPFEM_EINVAL, "times: bad arguments");
+    HIPFEM_REQUIRE(!m->staged, HIPFEM_EINVAL, "times: assemble() contributions not flushed (call assembleEnd)");
+    DeviceGuard g(m->ctx->device);
+    if (m->n == 0) return 0;
+    return csr_spmv_dev(m, dx, dy);
+}
+
+extern "C" int hipcsr_times(hipcsr_mtx *m, const double *x, double *y)
+{
+    HIPFEM_REQUIRE(m && m->has_structure && x && y, HIPFEM_EINVAL, "times: bad arguments");
+    HIPFEM_REQUIRE(!m->staged, HIPFEM_EINVAL, "times: assemble() contributions not flushed (call assembleEnd)");
+    DeviceGuard g(m->ctx->device);
+    if (m->n == 0) return 0;
+    double *d = nullptr;
+    HIPFEM_CHECK(hipMalloc(&d, sizeof(double) * 2 * m->n));
+    hipStream_t st = m->ctx->stream;
+    HIPFEM_CHECK(hipMemcpyAsync(d, x, sizeof(double) * m->n, hipMemcpyHostToDevice, st));
+    int rc = csr_spmv_dev(m, d, d + m->n);
+    if (rc) { (void)hipFree(d); return rc; }
+    HIPFEM_CHECK(hipMemcpyAsync(y, d + m->n, sizeof(double) * m->n, hipMemcpyDeviceToHost, st));
+    HIPFEM_CHECK(hipStreamSynchronize(st));
+    HIPFEM_CHECK(hipFree(d));
+    return 0;
+}
+
+static int build_transpose(hipcsr_mtx *m)
+{
+    if (m->d_tptr) return 0;
+    const int n = m->n;
+    std::vector<int32_t> cnt((size_t)n + 1, 0);
+    for (int64_t k = 0; k < m->nnz; k++) cnt[m->h_colind[k] + 1]++;
+    for (int i = 0; i < n; i++) cnt[i + 1] += cnt[i];
+    std::vector<int32_t> fill(cnt.begin(), cnt.end() - 1), tcol((size_t)m->nnz), tmap((size_t)m->nnz);
+    for (int r = 0; r < n; r++)
+        for (int k = m->h_rowptr[r]; k < m->h_rowptr[r + 1]; k++) {
+            const int c = m->h_colind[k];
+            tcol[fill[c]] = r;
+            tmap[fill[c]++] = k;
+        }
+    HIPFEM_CHECK(hipMalloc(&m->d_tptr, sizeof(int32_t) * (n + 1)));
+    HIPFEM_CHECK(hipMalloc(&m->d_tcol, sizeof(int32_t) * std::max<int64_t>(1, m->nnz)));
+    HIPFEM_CHECK(hipMalloc(&m->d_tmap, sizeof(int32_t) * std::max<int64_t>(1, m->nnz)));
+    hipStream_t st = m->ctx->stream;
+    HIPFEM_CHECK(hipMemcpyAsync(m->d_tptr, cnt.data(), sizeof(int32_t) * (n + 1), hipMemcpyHostToDevice, st));
+    HIPFEM_CHECK(hipMemcpyAsync(m->d_tcol, tcol.data(), sizeof(int32_t) * m->nnz, hipMemcpyHostToDevice, st));
+    HIPFEM_CHECK(hipMemcpyAsync(m->d_tmap, tmap.data(), sizeof(int32_t) * m->nnz, hipMemcpyHostToDevice, st));
+    HIPFEM_CHECK(hipStreamSynchronize(st));
+    return 0;
+}
+
+extern "C" int hipcsr_times_t(hipcsr_mtx *m, const double *x, double *y)
+{
+    HIPFEM_REQUIRE(m && m->has_structure && x && y, HIPFEM_EINVAL, "timesT: bad arguments");
+    HIPFEM_REQUIRE(!m->staged, HIPFEM_EINVAL, "timesT: assemble() contributions not flushed (call assembleEnd)");
+    DeviceGuard g(m->ctx->device);
+    if (m->n == 0) return 0;
+    int rc = build_transpose(m);
+    if (rc) return rc;
+    double *d = nullptr;
+    HIPFEM_CHECK(hipMalloc(&d, sizeof(double) * 2 * m->n));
+    hipStream_t st = m->ctx->stream;
+    HIPFEM_CHECK(hipMemcpyAsync(d, x, sizeof(double) * m->n, hipMemcpyHostToDevice, st));
+    k_spmv_t<<<(m->n + 255) / 256, 256, 0, st>>>(m->n, m->d_tptr, m->d_tcol, m->d_tmap, m->d_val, d, d + m->n);
+    HIPFEM_CHECK(hipGetLastError());
+    HIPFEM_CHECK(hipMemcpyAsync(y, d + m->n, sizeof(double) * m->n, hipMemcpyDeviceToHost, st));
+    HIPFEM_CHECK(hipStreamSynchronize(st));
+    HIPFEM_CHECK(hipFree(d));
+    return 0;
+}
+
+extern "C" int hipcsr_times_mat(hipcsr_mtx *m, int ncol, const double *B, double *Y)
+{
+    HIPFEM_REQUIRE(m && m->has_structure && ncol >= 0 && (ncol == 0 || (B && Y)), HIPFEM_EINVAL,
+                   "times(FloatMatrix): bad arguments");
+    std::vector<double> x((size_t)m->n), y((size_t)m->n);
+    for (int c = 0; c < ncol; c++) {
+        for (int i = 0; i < m->n; i++) x[i] = B[(size_t)i * ncol + c];
+        int rc = hipcsr_times(m, x.data(), y.data());
+        if (rc) return rc;
+        for (int i = 0; i < m->n; i++) Y[(size_t)i * ncol + c] = y[i];
+    }
+    return 0;
+}
+
+extern "C" int hipcsr_get(hipcsr_mtx *m, int i, int j, double *v)
+{
+    HIPFEM_REQUIRE(m && m->has_structure && v, HIPFEM_EINVAL, "at: bad arguments");
+    HIPFEM_REQUIRE(i >= 1 && j >= 1 && i <= m->n && j <= m->n, HIPFEM_ERANGE, "at(%d,%d) outside %dx%d", i, j, m->n, m->n);
+    DeviceGuard g(m->ctx->device);
+    const int64_t p = find_pos(m, i - 1, j - 1);
+    if (p < 0) { *v = 0.0; return 0; }
+    HIPFEM_CHECK(hipStreamSynchronize(m->ctx->stream));
+    double d = 0.0;
+    HIPFEM_CHECK(hipMemcpy(&d, m->d_val + p, sizeof(double), hipMemcpyDeviceToHost));
+    if (m->staged) d += m->h_stage[p];
+    *v = d;
+    return 0;
+}
+
+extern "C" int hipcsr_set(hipcsr_mtx *m, int i, int j, double v)
+{
+    HIPFEM_REQUIRE(m && m->has_structure, HIPFEM_EINVAL, "at: no structure");
+    HIPFEM_REQUIRE(i >= 1 && j >= 1 && i <= m->n && j <= m->n, HIPFEM_ERANGE, "at(%d,%d) outside %dx%d", i, j, m->n, m->n);
+    DeviceGuard g(m->ctx->device);
+    const int64_t p = find_pos(m, i - 1, j - 1);
+    HIPFEM_REQUIRE(p >= 0, HIPFEM_ERANGE, "at(%d,%d): entry not in the pattern", i, j);
+    if (m->staged) m->h_stage[p] = 0.0;
+    HIPFEM_CHECK(hipStreamSynchronize(m->ctx->stream));
+    HIPFEM_CHECK(hipMemcpy(m->d_val + p, &v, sizeof(double), hipMemcpyHostToDevice));
+    m->version++;
+    return 0;
+}
+
+extern "C" int hipcsr_is_allocated(hipcsr_mtx *m, int i, int j)
+{
+    if (!m || !m->has_structure || i < 1 || j < 1 || i > m->n || j > m->n) return 0;
+    return find_pos(m, i - 1, j - 1) >= 0;
+}
+
+extern "C" int hipcsr_nrows(const hipcsr_mtx *m) { return m ? m->n : 0; }
+extern "C" int64_t hipcsr_nnz(const hipcsr_mtx *m) { return m ? m->nnz : 0; }
+extern "C" long hipcsr_version(const hipcsr_mtx *m) { return m ? m->version : 0; }
+
+extern "C" int hipcsr_download(hipcsr_mtx *m, int64_t *rowptr, int32_t *colind, double *val)
+{
+    HIPFEM_REQUIRE(m && m->has_structure, HIPFEM_EINVAL, "download: no structure");
+    DeviceGuard g(m->ctx->device);
+    if (rowptr)
+        for (int i = 0; i <= m->n; i++) rowptr[i] = m->h_rowptr[i];
+    if (colind && m->nnz) memcpy(colind, m->h_colind.data(), sizeof(int32_t) * m->nnz);
+    if (val && m->nnz) {
+        HIPFEM_CHECK(hipStreamSynchronize(m->ctx->stream));
+        HIPFEM_CHECK(hipMemcpy(val, m->d_val, sizeof(double) * m->nnz, hipMemcpyDeviceToHost));
+        if (m->staged)
+            for (int64_t k = 0; k < m->nnz; k++) val[k] += m->h_stage[k];
+    }
+    return 0;
+}
+
+extern "C" int hipcsr_upload_values(hipcsr_mtx *m, const double *val)
+{
+    HIPFEM_REQUIRE(m && m->has_structure && (val || !m->nnz), HIPFEM_EINVAL, "upload: bad arguments");
+    DeviceGuard g(m->ctx->device);
+    if (m->nnz) {
+        HIPFEM_CHECK(hipMemcpyAsync(m->d_val, val, sizeof(double) * m->nnz, hipMemcpyHostToDevice, m->ctx->stream));
+        HIPFEM_CHECK(hipStreamSynchronize(m->ctx->stream));
+    }
+    m->h_stage.clear();
+    m->staged = false;
+    m->version++;
+    return 0;
+}
+
+extern "C" int hipcsr_device_arrays(hipcsr_mtx *m, const int32_t **rowptr, const int32_t **colind, double **val)
+{
+    HIPFEM_REQUIRE(m && m->has_structure, HIPFEM_EINVAL, "device_arrays: no structure");
+    if (rowptr) *rowptr = m->d_rowptr;
+    if (colind) *colind = m->d_colind;
+    if (val) *val = m->d_val;
+    return 0;
+}

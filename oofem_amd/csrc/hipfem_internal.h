@@ -1,0 +1,101 @@
+// Internal declarations shared by the HIP translation units of libhipfem.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/hipfem.h"
+
+namespace hipfem {
+
+void set_error(const char *fmt, ...);
+
+// Every device launch of a context goes to ctx->stream (default priority).
+// Small reduction scratch is owned by the context; the PCG keeps its own
+// vectors.
+// Makes the context's device current for the scope of an API call.
+struct DeviceGuard {
+    int prev = -1;
+    explicit DeviceGuard(int dev)
+    {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev) (void)hipSetDevice(dev);
+    }
+    ~DeviceGuard()
+    {
+        int cur = -1;
+        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+}  // namespace hipfem
+
+#define HIPFEM_CHECK(expr)                                                                       \
+    do {                                                                                         \
+        hipError_t e_ = (expr);                                                                  \
+        if (e_ != hipSuccess) {                                                                  \
+            hipfem::set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__,  \
+                              __LINE__);                                                         \
+            return HIPFEM_EHIP;                                                                  \
+        }                                                                                        \
+    } while (0)
+
+#define HIPFEM_REQUIRE(cond, code, ...)       \
+    do {                                      \
+        if (!(cond)) {                        \
+            hipfem::set_error(__VA_ARGS__);   \
+            return (code);                    \
+        }                                     \
+    } while (0)
+
+struct hipfem_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    double *d_partials = nullptr;   // per-block partial sums (reduction scratch)
+    size_t partials_cap = 0;        // in doubles
+    double *h_pinned = nullptr;     // small pinned host buffer for scalar readback
+    // Objects created on the context hold a reference; hipfem_ctx_destroy on a
+    // context with live objects only marks it, and the last object's destroy
+    // frees it.
+    int refs = 0;
+    bool destroy_pending = false;
+};
+
+namespace hipfem {
+void ctx_retain(hipfem_ctx *c);
+void ctx_release(hipfem_ctx *c);
+}  // namespace hipfem
+
+struct hipcsr_mtx {
+    hipfem_ctx *ctx = nullptr;
+    int n = 0;
+    int64_t nnz = 0;
+    bool has_structure = false;
+    std::vector<int32_t> h_rowptr, h_colind;   // host copy of the pattern
+    int32_t *d_rowptr = nullptr, *d_colind = nullptr, *d_diag = nullptr;
+    double *d_val = nullptr;
+    float *d_val32 = nullptr;                  // FP32 copy for mixed precision
+    long val32_version = -1;
+    std::vector<double> h_stage;               // host-side assemble() staging
+    bool staged = false;
+    long version = 0;
+    long pattern_id = 0;                       // changes whenever the pattern is rebuilt
+    // transpose pattern for timesT, built on first use
+    int32_t *d_tptr = nullptr, *d_tcol = nullptr, *d_tmap = nullptr;
+    double avg_row = 0.0;
+};
+
+namespace hipfem {
+int ensure_partials(hipfem_ctx *ctx, size_t n);
+int csr_alloc(hipcsr_mtx *m, int n, const std::vector<int32_t> &rowptr, const std::vector<int32_t> &colind);
+int csr_free(hipcsr_mtx *m);
+int csr_spmv_dev(hipcsr_mtx *m, const double *dx, double *dy);
+int csr_refresh_fp32(hipcsr_mtx *m);
+// launch helper: rows-per-lane group width chosen from the average row length
+int spmv_lanes(const hipcsr_mtx *m);
+int spmv_fp32_dev(hipcsr_mtx *m, const double *dx, double *dy);
+long next_pattern_id();
+}  // namespace hipfem

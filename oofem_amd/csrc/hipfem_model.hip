@@ -1,0 +1,964 @@
+// hfem_model: batched element/material evaluator for LSpace / LTRSpace /
+// QSpace with IsoLE / MisesMat, standing in for the element loops of
+// EngngModel::assemble (engngm.C:889-1037) and
+// EngngModel::assembleVectorFromElements (engngm.C:1351-1535).
+//
+// Data layout in HBM
+//   coords   nnode x 3 FP64 (node-major, gathered per element)
+//   conn     nelem x nen int32 (element-major, the reference's local order)
+//   eqn      nnode x 3 int32 (>0 free equation, <0 -prescribed equation)
+//   eoff     nelem x nen x nen uint16: column offset of node b's first free
+//            column inside every row of node a (all free rows of a node
+//            share one column list), so entry (a,i | b,j) of K_e lands at
+//            rowptr[eq(a,i)-1] + eoff + rank_b(j)
+//   colors   element lists per color; no two elements of one color share a
+//            node, so a color's launch scatters into K and f without atomics
+//            and the result is deterministic (sum order = color order)
+//   state    Gauss-point state, structure of arrays: field k of Gauss point g
+//            at state[k * ngp_total + g], g = e * ngp + local point; fields
+//            follow MisesMatStatus (misesmat.h:157-239): 0-5 committed plastic
+//            strain, 6 committed kappa, 7-12 temp plastic strain, 13 temp
+//            kappa, 14-19 trial deviatoric stress, 20-25 temp stress, 26-31
+//            temp strain, 32-37 committed stress.
+//
+// Kernels
+//   k_assemble<ET>   one 64-lane wave per element.  Lanes first compute the
+//                    Gauss-point geometry (dN/dx, dV) into LDS, then each lane
+//                    owns node pairs (a,b) and accumulates the 3x3 block
+//                    B_a^T D B_b dV over the Gauss points in registers before
+//                    scattering it.  IsoLE uses the closed form
+//                    lambda n_i m_j + G (n_j m_i + delta_ij n.m).
+//   k_internal<ET>   one thread per Gauss point: geometry, strain B u_e, the
+//                    material update (radial return for MisesMat), B^T sigma dV
+//                    into LDS; then per element the Gauss-point sums in
+//                    ascending order are scattered to f.
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+
+#include "hipfem_internal.h"
+
+namespace hipfem {
+
+constexpr int NST = 38;  // state fields per Gauss point
+
+// ------------------------------------------------------------ element rules
+// Gauss rules: gaussintegrationrule.C:190-216, constants :1436-1463, tetra :492-502.
+__device__ __forceinline__ void gauss_point(int et, int g, double xi[3], double &w)
+{
+    if (et == HIPFEM_LSPACE) {
+        const double a = 0.577350269189626;
+        xi[0] = (g & 4) ? a : -a;
+        xi[1] = (g & 2) ? a : -a;
+        xi[2] = (g & 1) ? a : -a;
+        w = 1.0;
+    } else if (et == HIPFEM_QSPACE) {
+        const double x3[3] = {-0.774596669241483, 0.0, 0.774596669241483};
+        const double w3[3] = {0.555555555555555, 0.888888888888888, 0.555555555555555};
+        const int i = g / 9, j = (g / 3) % 3, k = g % 3;
+        xi[0] = x3[i];
+        xi[1] = x3[j];
+        xi[2] = x3[k];
+        w = w3[i] * w3[j] * w3[k];
+    } else {
+        xi[0] = xi[1] = xi[2] = 0.25;
+        w = 1.0 / 6.0;
+    }
+}
+
+// dN/dxi for node a: FEI3dHexaLin::evaldNdxi fei3dhexalin.C:127-175,
+// FEI3dHexaQuad::evaldNdxi fei3dhexaquad.C:191-282.
+__device__ __forceinline__ void dndxi_hexlin(int a, const double xi[3], double d[3])
+{
+    const double sx = (a == 2 || a == 3 || a == 6 || a == 7) ? 1.0 : -1.0;
+    const double sy = (a == 1 || a == 2 || a == 5 || a == 6) ? 1.0 : -1.0;
+    const double sz = (a < 4) ? 1.0 : -1.0;
+    d[0] = sx * 0.125 * (1. + sy * xi[1]) * (1. + sz * xi[2]);
+    d[1] = sy * 0.125 * (1. + sx * xi[0]) * (1. + sz * xi[2]);
+    d[2] = sz * 0.125 * (1. + sx * xi[0]) * (1. + sy * xi[1]);
+}
+
+__device__ __forceinline__ void dndxi_hexquad(int a, const double xi[3], double d[3])
+{
+    const double u = xi[0], v = xi[1], w = xi[2];
+    if (a < 8) {
+        const double sx = (a == 2 || a == 3 || a == 6 || a == 7) ? 1.0 : -1.0;
+        const double sy = (a == 1 || a == 2 || a == 5 || a == 6) ? 1.0 : -1.0;
+        const double sz = (a < 4) ? 1.0 : -1.0;
+        const double pu = 1. + sx * u, pv = 1. + sy * v, pw = 1. + sz * w;
+        d[0] = 0.125 * sx * pv * pw * (2.0 * sx * u + sy * v + sz * w - 1.0);
+        d[1] = 0.125 * sy * pu * pw * (2.0 * sy * v + sx * u + sz * w - 1.0);
+        d[2] = 0.125 * sz * pu * pv * (2.0 * sz * w + sx * u + sy * v - 1.0);
+    } else if (a < 16) {
+        const double sw = a < 12 ? 1.0 : -1.0;
+        const double pw = 1.0 + sw * w;
+        switch (a & 3) {
+        case 0:
+            d[0] = -0.25 * (1.0 - v * v) * pw;
+            d[1] = -0.5 * v * (1.0 - u) * pw;
+            d[2] = sw * 0.25 * (1.0 - v * v) * (1.0 - u);
+            break;
+        case 1:
+            d[0] = -0.5 * u * (1.0 + v) * pw;
+            d[1] = 0.25 * (1.0 - u * u) * pw;
+            d[2] = sw * 0.25 * (1.0 - u * u) * (1.0 + v);
+            break;
+        case 2:
+            d[0] = 0.25 * (1.0 - v * v) * pw;
+            d[1] = -0.5 * v * (1.0 + u) * pw;
+            d[2] = sw * 0.25 * (1.0 - v * v) * (1.0 + u);
+            break;
+        default:
+            d[0] = -0.5 * u * (1.0 - v) * pw;
+            d[1] = -0.25 * (1.0 - u * u) * pw;
+            d[2] = sw * 0.25 * (1.0 - u * u) * (1.0 - v);
+            break;
+        }
+    } else {
+        const int k = a - 16;
+        const double cu = (k >= 2) ? 1.0 : -1.0, cv = (k == 1 || k == 2) ? 1.0 : -1.0;
+        const double pu = 1.0 + cu * u, pv = 1.0 + cv * v;
+        d[0] = cu * 0.25 * pv * (1.0 - w * w);
+        d[1] = cv * 0.25 * pu * (1.0 - w * w);
+        d[2] = -0.5 * pu * pv * w;
+    }
+}
+
+template <int ET>
+struct ElemT;
+template <>
+struct ElemT<HIPFEM_LSPACE> {
+    static constexpr int NEN = 8, NGP = 8;
+};
+template <>
+struct ElemT<HIPFEM_LTRSPACE> {
+    static constexpr int NEN = 4, NGP = 1;
+};
+template <>
+struct ElemT<HIPFEM_QSPACE> {
+    static constexpr int NEN = 20, NGP = 27;
+};
+
+// Geometry at one Gauss point: dNdx (NEN x 3) and detJ.  Hexahedra:
+// J = X dN/dxi, closed-form inverse (floatmatrix.C:768-815), dNdx = dN/dxi J^-1,
+// detJ by FloatMatrix::giveDeterminant (floatmatrix.C:1085-1101).  Tetrahedra:
+// cofactor form of FEI3dTetLin::evaldNdx (fei3dtetlin.C:116-168).
+// X is NEN x 3 (row-major).  Returns false for a degenerate element.
+template <int ET>
+__device__ bool gp_geometry(const double *X, const double xi[3], double *dNdx, double &detJ)
+{
+    constexpr int NEN = ElemT<ET>::NEN;
+    if constexpr (ET == HIPFEM_LTRSPACE) {
+        const double x1 = X[0], y1 = X[1], z1 = X[2], x2 = X[3], y2 = X[4], z2 = X[5];
+        const double x3 = X[6], y3 = X[7], z3 = X[8], x4 = X[9], y4 = X[10], z4 = X[11];
+        detJ = ((x4 - x1) * (y2 - y1) * (z3 - z1) - (x4 - x1) * (y3 - y1) * (z2 - z1) +
+                (x3 - x1) * (y4 - y1) * (z2 - z1) - (x2 - x1) * (y4 - y1) * (z3 - z1) +
+                (x2 - x1) * (y3 - y1) * (z4 - z1) - (x3 - x1) * (y2 - y1) * (z4 - z1));
+        if (!(detJ > 0.0)) return false;
+        const double f = 1. / detJ;
+        dNdx[0] = -((y3 - y2) * (z4 - z2) - (y4 - y2) * (z3 - z2)) * f;
+        dNdx[3] = ((y4 - y3) * (z1 - z3) - (y1 - y3) * (z4 - z3)) * f;
+        dNdx[6] = -((y1 - y4) * (z2 - z4) - (y2 - y4) * (z1 - z4)) * f;
+        dNdx[9] = ((y2 - y1) * (z3 - z1) - (y3 - y1) * (z2 - z1)) * f;
+        dNdx[1] = -((x4 - x2) * (z3 - z2) - (x3 - x2) * (z4 - z2)) * f;
+        dNdx[4] = ((x1 - x3) * (z4 - z3) - (x4 - x3) * (z1 - z3)) * f;
+        dNdx[7] = -((x2 - x4) * (z1 - z4) - (x1 - x4) * (z2 - z4)) * f;
+        dNdx[10] = ((x3 - x1) * (z2 - z1) - (x2 - x1) * (z3 - z1)) * f;
+        dNdx[2] = -((x3 - x2) * (y4 - y2) - (x4 - x2) * (y3 - y2)) * f;
+        dNdx[5] = ((x4 - x3) * (y1 - y3) - (x1 - x3) * (y4 - y3)) * f;
+        dNdx[8] = -((x1 - x4) * (y2 - y4) - (x2 - x4) * (y1 - y4)) * f;
+        dNdx[11] = ((x2 - x1) * (y3 - y1) - (x3 - x1) * (y2 - y1)) * f;
+        return true;
+    } else {
+        double J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        double dN[NEN * 3];
+#pragma unroll
+        for (int a = 0; a < NEN; a++) {
+            if constexpr (ET == HIPFEM_LSPACE)
+                dndxi_hexlin(a, xi, dN + 3 * a);
+            else
+                dndxi_hexquad(a, xi, dN + 3 * a);
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) J[3 * i + j] += X[3 * a + i] * dN[3 * a + j];
+        }
+#define S(r, c) J[3 * ((r)-1) + ((c)-1)]
+        const double det = S(1, 1) * S(2, 2) * S(3, 3) + S(1, 2) * S(2, 3) * S(3, 1) + S(1, 3) * S(2, 1) * S(3, 2) -
+                           S(1, 3) * S(2, 2) * S(3, 1) - S(2, 3) * S(3, 2) * S(1, 1) - S(3, 3) * S(1, 2) * S(2, 1);
+        if (!(fabs(det) > 1.e-30)) return false;
+        double I[9];
+        I[0] = (S(2, 2) * S(3, 3) - S(2, 3) * S(3, 2)) / det;
+        I[3] = (S(2, 3) * S(3, 1) - S(2, 1) * S(3, 3)) / det;
+        I[6] = (S(2, 1) * S(3, 2) - S(2, 2) * S(3, 1)) / det;
+        I[1] = (S(1, 3) * S(3, 2) - S(1, 2) * S(3, 3)) / det;
+        I[4] = (S(1, 1) * S(3, 3) - S(1, 3) * S(3, 1)) / det;
+        I[7] = (S(1, 2) * S(3, 1) - S(1, 1) * S(3, 2)) / det;
+        I[2] = (S(1, 2) * S(2, 3) - S(1, 3) * S(2, 2)) / det;
+        I[5] = (S(1, 3) * S(2, 1) - S(1, 1) * S(2, 3)) / det;
+        I[8] = (S(1, 1) * S(2, 2) - S(1, 2) * S(2, 1)) / det;
+#undef S
+        detJ = J[0] * J[4] * J[8] + J[3] * J[7] * J[2] + J[6] * J[1] * J[5] - J[2] * J[4] * J[6] -
+               J[5] * J[7] * J[0] - J[8] * J[1] * J[3];
+#pragma unroll
+        for (int a = 0; a < NEN; a++)
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+                dNdx[3 * a + j] = dN[3 * a] * I[j] + dN[3 * a + 1] * I[3 + j] + dN[3 * a + 2] * I[6 + j];
+        return true;
+    }
+}
+
+// ----------------------------------------------------------------- MisesMat
+// performPlasticityReturn misesmat.C:181-263 (3D branch :221-256), helpers
+// structuralmaterial.C:1489-1600; no damage (omega_crit = 0).
+struct MisesOut {
+    double sig[6], epsp[6], sdev[6], kappa;
+};
+
+__device__ __forceinline__ void mises_return(double E, double nu, double sig0, double H, const double eps[6],
+                                             const double epsp_n[6], double kappa_n, MisesOut &o)
+{
+    const double G = E / (2.0 * (1. + nu));
+    const double K = E / (3.0 * (1. - 2. * nu));
+    double el[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        o.epsp[i] = epsp_n[i];
+        el[i] = eps[i] - epsp_n[i];
+    }
+    const double mean = (el[0] + el[1] + el[2]) / 3.0;
+    el[0] -= mean;
+    el[1] -= mean;
+    el[2] -= mean;
+    double s[6] = {2. * G * el[0], 2. * G * el[1], 2. * G * el[2], G * el[3], G * el[4], G * el[5]};
+    const double pvol = 3 * K * mean;
+#pragma unroll
+    for (int i = 0; i < 6; i++) o.sdev[i] = s[i];
+    const double trialS = sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2] + 2. * s[3] * s[3] + 2. * s[4] * s[4] +
+                               2. * s[5] * s[5]);
+    double kappa = kappa_n;
+    const double yieldValue = sqrt(3. / 2.) * trialS - (sig0 + H * kappa);
+    if (yieldValue > 0.) {
+        const double dKappa = yieldValue / (H + 3. * G);
+        kappa += dKappa;
+        const double f = sqrt(3. / 2.) * dKappa / trialS;
+        o.epsp[0] += f * s[0];
+        o.epsp[1] += f * s[1];
+        o.epsp[2] += f * s[2];
+        o.epsp[3] += f * (2. * s[3]);
+        o.epsp[4] += f * (2. * s[4]);
+        o.epsp[5] += f * (2. * s[5]);
+        const double sc = 1. - sqrt(6.) * G * dKappa / trialS;
+#pragma unroll
+        for (int i = 0; i < 6; i++) s[i] *= sc;
+    }
+    s[0] += pvol;
+    s[1] += pvol;
+    s[2] += pvol;
+#pragma unroll
+    for (int i = 0; i < 6; i++) o.sig[i] = s[i];
+    o.kappa = kappa;
+}
+
+// give3dMaterialStiffnessMatrix misesmat.C:496-544 (TangentStiffness, no damage)
+__device__ __forceinline__ void mises_tangent(double E, double nu, double sig0, double H, double kappa_n,
+                                              double kappa_t, const double s[6], double D[36])
+{
+    const double G = E / (2.0 * (1. + nu));
+    const double K = E / (3.0 * (1. - 2. * nu));
+    const double idev[36] = {2. / 3.,  -1. / 3., -1. / 3., 0., 0.,  0.,  -1. / 3., 2. / 3., -1. / 3.,
+                             0.,       0.,       0.,       -1. / 3., -1. / 3., 2. / 3., 0., 0.,  0.,
+                             0.,       0.,       0.,       0.5,      0.,       0.,       0., 0.,  0.,
+                             0.,       0.5,      0.,       0.,       0.,       0.,       0., 0.,  0.5};
+#pragma unroll
+    for (int i = 0; i < 36; i++) {
+        const int r = i / 6, c = i % 6;
+        D[i] = 2 * G * idev[i] + ((r < 3 && c < 3) ? K : 0.0);
+    }
+    const double dKappa = kappa_t - kappa_n;
+    if (dKappa <= 0.0) return;
+    const double sigmaY = sig0 + H * kappa_n;
+    const double trialS = sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2] + 2. * s[3] * s[3] + 2. * s[4] * s[4] +
+                               2. * s[5] * s[5]);
+    const double factor = -2. * sqrt(6.) * G * G / trialS;
+    const double factor1 = factor * sigmaY / ((H + 3. * G) * trialS * trialS);
+    const double factor2 = factor * dKappa;
+#pragma unroll
+    for (int i = 0; i < 36; i++) D[i] += factor1 * (s[i / 6] * s[i % 6]);
+#pragma unroll
+    for (int i = 0; i < 36; i++) D[i] += factor2 * idev[i];
+}
+
+// ------------------------------------------------------------ assembly kernel
+template <int ET>
+__global__ __launch_bounds__(64) void k_assemble(const int *__restrict__ elems, int nelem_color,
+                                                 const double *__restrict__ coords, const int *__restrict__ conn,
+                                                 const int *__restrict__ eqn, const int *__restrict__ emat,
+                                                 const double *__restrict__ mat, const uint16_t *__restrict__ eoff,
+                                                 const double *__restrict__ state, int ngp_total, int tangent,
+                                                 const int32_t *__restrict__ rowptr, double *__restrict__ val,
+                                                 int *__restrict__ bad)
+{
+    constexpr int NEN = ElemT<ET>::NEN, NGP = ElemT<ET>::NGP;
+    __shared__ double sX[NEN * 3];
+    __shared__ double sN[NGP][NEN * 3];
+    __shared__ double sdV[NGP];
+    __shared__ double sD[(ET == HIPFEM_LTRSPACE) ? 36 : NGP * 36];
+    __shared__ int sEq[NEN * 3];
+    __shared__ int sGeneral;
+
+    const int ei = blockIdx.x;
+    if (ei >= nelem_color) return;
+    const int e = elems[ei];
+    const int t = threadIdx.x;
+    for (int k = t; k < NEN * 3; k += 64) {
+        const int node = conn[(size_t)e * NEN + k / 3];
+        sX[k] = coords[3 * (size_t)node + k % 3];
+        sEq[k] = eqn[3 * (size_t)node + k % 3];
+    }
+    const double *mp = mat + (size_t)HIPFEM_MATP * emat[e];
+    const int model = (int)mp[0];
+    const double E = mp[1], nu = mp[2];
+    if (t == 0) sGeneral = (model == HIPFEM_MAT_MISES && tangent) ? 1 : 0;
+    __syncthreads();
+    for (int g = t; g < NGP; g += 64) {
+        double xi[3], w, detJ;
+        gauss_point(ET, g, xi, w);
+        if (!gp_geometry<ET>(sX, xi, sN[g], detJ)) atomicMin(bad, e);
+        sdV[g] = fabs(detJ) * w;
+        if (sGeneral) {
+            const size_t gp = (size_t)e * NGP + g;
+            double s[6];
+#pragma unroll
+            for (int i = 0; i < 6; i++) s[i] = state[(size_t)(14 + i) * ngp_total + gp];
+            mises_tangent(E, nu, mp[3], mp[4], state[(size_t)6 * ngp_total + gp], state[(size_t)13 * ngp_total + gp], s,
+                          sD + 36 * g);
+        }
+    }
+    __syncthreads();
+    const double G = E / (2.0 * (1. + nu));
+    const double Kb = E / (3.0 * (1. - 2. * nu));
+    const double lam = Kb - 2.0 * G / 3.0;
+    for (int blk = t; blk < NEN * NEN; blk += 64) {
+        const int a = blk / NEN, b = blk % NEN;
+        double Kab[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        if (!sGeneral) {
+            for (int g = 0; g < NGP; g++) {
+                const double n0 = sN[g][3 * a], n1 = sN[g][3 * a + 1], n2 = sN[g][3 * a + 2];
+                const double m0 = sN[g][3 * b], m1 = sN[g][3 * b + 1], m2 = sN[g][3 * b + 2];
+                const double dv = sdV[g];
+                const double nm = n0 * m0 + n1 * m1 + n2 * m2;
+                const double n[3] = {n0, n1, n2}, m[3] = {m0, m1, m2};
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+#pragma unroll
+                    for (int j = 0; j < 3; j++)
+                        Kab[3 * i + j] += dv * (lam * n[i] * m[j] + G * (n[j] * m[i] + (i == j ? nm : 0.0)));
+            }
+        } else {
+            for (int g = 0; g < NGP; g++) {
+                const double *D = sD + 36 * g;
+                const double n0 = sN[g][3 * a], n1 = sN[g][3 * a + 1], n2 = sN[g][3 * a + 2];
+                const double m0 = sN[g][3 * b], m1 = sN[g][3 * b + 1], m2 = sN[g][3 * b + 2];
+                const double dv = sdV[g];
+                // columns of B_b: x (m0,0,0,0,m2,m1), y (0,m1,0,m2,0,m0), z (0,0,m2,m1,m0,0)
+                double DB[6][3];
+#pragma unroll
+                for (int r = 0; r < 6; r++) {
+                    DB[r][0] = D[6 * r + 0] * m0 + D[6 * r + 4] * m2 + D[6 * r + 5] * m1;
+                    DB[r][1] = D[6 * r + 1] * m1 + D[6 * r + 3] * m2 + D[6 * r + 5] * m0;
+                    DB[r][2] = D[6 * r + 2] * m2 + D[6 * r + 3] * m1 + D[6 * r + 4] * m0;
+                }
+#pragma unroll
+                for (int j = 0; j < 3; j++) {
+                    Kab[0 + j] += dv * (n0 * DB[0][j] + n2 * DB[4][j] + n1 * DB[5][j]);
+                    Kab[3 + j] += dv * (n1 * DB[1][j] + n2 * DB[3][j] + n0 * DB[5][j]);
+                    Kab[6 + j] += dv * (n2 * DB[2][j] + n1 * DB[3][j] + n0 * DB[4][j]);
+                }
+            }
+        }
+        const int off = eoff[((size_t)e * NEN + a) * NEN + b];
+        if (off == 0xFFFF) continue;
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const int ea = sEq[3 * a + i];
+            if (ea <= 0) continue;
+            const int base = rowptr[ea - 1] + off;
+            int rnk = 0;
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                if (sEq[3 * b + j] <= 0) continue;
+                val[base + rnk] += Kab[3 * i + j];
+                rnk++;
+            }
+        }
+    }
+}
+
+// -------------------------------------------------------- internal forces
+template <int ET>
+struct IntCfg {
+    static constexpr int NGP = ElemT<ET>::NGP;
+    static constexpr int EPB = (NGP >= 64) ? 1 : 64 / NGP;  // elements per 64-thread block
+};
+
+template <int ET>
+__global__ __launch_bounds__(64) void k_internal(const int *__restrict__ elems, int nelem_color,
+                                                 const double *__restrict__ coords, const int *__restrict__ conn,
+                                                 const int *__restrict__ eqn, const int *__restrict__ emat,
+                                                 const double *__restrict__ mat, const double *__restrict__ u_free,
+                                                 const double *__restrict__ u_presc, double *__restrict__ state,
+                                                 int ngp_total, int update_state, double *__restrict__ f_free,
+                                                 double *__restrict__ f_presc, double *__restrict__ partials,
+                                                 int npart, int *__restrict__ bad)
+{
+    constexpr int NEN = ElemT<ET>::NEN, NGP = ElemT<ET>::NGP, EPB = IntCfg<ET>::EPB, NC = 3 * NEN;
+    __shared__ double sF[EPB][NGP][NC];
+    __shared__ double sh[3];
+    const int t = threadIdx.x;
+    const int el = t / NGP, g = t % NGP;
+    const int ei = blockIdx.x * EPB + el;
+    const bool active = (el < EPB) && (ei < nelem_color);
+    if (active) {
+        const int e = elems[ei];
+        double X[NC], ue[NC];
+        for (int a = 0; a < NEN; a++) {
+            const int node = conn[(size_t)e * NEN + a];
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                X[3 * a + d] = coords[3 * (size_t)node + d];
+                const int q = eqn[3 * (size_t)node + d];
+                ue[3 * a + d] = q > 0 ? u_free[q - 1] : ((q < 0 && u_presc) ? u_presc[-q - 1] : 0.0);
+            }
+        }
+        double xi[3], w, detJ, dNdx[NC];
+        gauss_point(ET, g, xi, w);
+        if (!gp_geometry<ET>(X, xi, dNdx, detJ)) atomicMin(bad, e);
+        const double dV = fabs(detJ) * w;
+        // strain = B u_e (Voigt xx yy zz yz xz xy, engineering shear)
+        double eps[6] = {0, 0, 0, 0, 0, 0};
+        for (int a = 0; a < NEN; a++) {
+            const double nx = dNdx[3 * a], ny = dNdx[3 * a + 1], nz = dNdx[3 * a + 2];
+            const double ux = ue[3 * a], uy = ue[3 * a + 1], uz = ue[3 * a + 2];
+            eps[0] += nx * ux;
+            eps[1] += ny * uy;
+            eps[2] += nz * uz;
+            eps[3] += nz * uy + ny * uz;
+            eps[4] += nz * ux + nx * uz;
+            eps[5] += ny * ux + nx * uy;
+        }
+        const double *mp = mat + (size_t)HIPFEM_MATP * emat[e];
+        const double E = mp[1], nu = mp[2];
+        const size_t gp = (size_t)e * NGP + g;
+        double sig[6];
+        if ((int)mp[0] == HIPFEM_MAT_MISES) {
+            double epsp_n[6];
+#pragma unroll
+            for (int i = 0; i < 6; i++) epsp_n[i] = state[(size_t)i * ngp_total + gp];
+            MisesOut o;
+            mises_return(E, nu, mp[3], mp[4], eps, epsp_n, state[(size_t)6 * ngp_total + gp], o);
+#pragma unroll
+            for (int i = 0; i < 6; i++) sig[i] = o.sig[i];
+            if (update_state) {
+#pragma unroll
+                for (int i = 0; i < 6; i++) {
+                    state[(size_t)(7 + i) * ngp_total + gp] = o.epsp[i];
+                    state[(size_t)(14 + i) * ngp_total + gp] = o.sdev[i];
+                }
+                state[(size_t)13 * ngp_total + gp] = o.kappa;
+            }
+        } else {
+            // IsoLE: sigma = (2G I_dev6 + K I6_I6) eps
+            const double G = E / (2.0 * (1. + nu));
+            const double K = E / (3.0 * (1. - 2. * nu));
+            const double lam = K - 2.0 * G / 3.0;
+            const double tr = eps[0] + eps[1] + eps[2];
+            sig[0] = lam * tr + 2.0 * G * eps[0];
+            sig[1] = lam * tr + 2.0 * G * eps[1];
+            sig[2] = lam * tr + 2.0 * G * eps[2];
+            sig[3] = G * eps[3];
+            sig[4] = G * eps[4];
+            sig[5] = G * eps[5];
+        }
+        if (update_state) {
+#pragma unroll
+            for (int i = 0; i < 6; i++) {
+                state[(size_t)(20 + i) * ngp_total + gp] = sig[i];
+                state[(size_t)(26 + i) * ngp_total + gp] = eps[i];
+            }
+        }
+        // B^T sigma dV
+        for (int a = 0; a < NEN; a++) {
+            const double nx = dNdx[3 * a], ny = dNdx[3 * a + 1], nz = dNdx[3 * a + 2];
+            sF[el][g][3 * a + 0] = (nx * sig[0] + nz * sig[4] + ny * sig[5]) * dV;
+            sF[el][g][3 * a + 1] = (ny * sig[1] + nz * sig[3] + nx * sig[5]) * dV;
+            sF[el][g][3 * a + 2] = (nz * sig[2] + ny * sig[3] + nx * sig[4]) * dV;
+        }
+    }
+    __syncthreads();
+    double n3[3] = {0, 0, 0};
+    for (int idx = t; idx < EPB * NC; idx += 64) {
+        const int l = idx / NC, c = idx % NC;
+        const int ej = blockIdx.x * EPB + l;
+        if (ej >= nelem_color) continue;
+        const int e = elems[ej];
+        double f = 0.0;
+        for (int gg = 0; gg < NGP; gg++) f += sF[l][gg][c];
+        const int node = conn[(size_t)e * NEN + c / 3];
+        const int q = eqn[3 * (size_t)node + c % 3];
+        if (q > 0 && f_free) f_free[q - 1] += f;
+        if (q < 0 && f_presc) f_presc[-q - 1] += f;
+        n3[c % 3] += f * f;
+    }
+    // block partial of the EBE norms (fixed order)
+    if (partials) {
+        for (int k = 0; k < 3; k++) {
+            double v = n3[k];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+            if (t == 0) sh[k] = v;
+        }
+        if (t == 0)
+            for (int k = 0; k < 3; k++) partials[(size_t)k * npart + blockIdx.x] = sh[k];
+    }
+}
+
+__global__ void k_fill(int64_t n, double v, double *__restrict__ a)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) a[i] = v;
+}
+
+__global__ __launch_bounds__(1024) void k_reduce3(const double *__restrict__ partials, int np, double *out)
+{
+    __shared__ double sh[16];
+    for (int k = 0; k < 3; k++) {
+        double s = 0.0;
+        for (int i = threadIdx.x; i < np; i += 1024) s += partials[(size_t)k * np + i];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+        if (threadIdx.x % 64 == 0) sh[threadIdx.x / 64] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double tt = 0.0;
+            for (int i = 0; i < 16; i++) tt += sh[i];
+            out[k] = tt;
+        }
+        __syncthreads();
+    }
+}
+
+// MisesMatStatus::updateYourself misesmat.C:720-728 (and the stress of
+// StructuralMaterialStatus::updateYourself)
+__global__ void k_commit(int ngp_total, double *__restrict__ state)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ngp_total) return;
+    const size_t n = ngp_total;
+    for (int i = 0; i < 6; i++) {
+        state[i * n + g] = state[(7 + i) * n + g];
+        state[(32 + i) * n + g] = state[(20 + i) * n + g];
+    }
+    state[6 * n + g] = state[13 * n + g];
+}
+
+}  // namespace hipfem
+
+using namespace hipfem;
+
+struct hfem_model {
+    hipfem_ctx *ctx = nullptr;
+    int nnode = 0, etype = 0, nen = 0, ngp = 0, nelem = 0, nmat = 0, neq = 0, npeq = 0;
+    std::vector<int> h_conn, h_eqn;
+    double *d_coords = nullptr, *d_mat = nullptr, *d_state = nullptr;
+    int *d_conn = nullptr, *d_emat = nullptr, *d_eqn = nullptr, *d_bad = nullptr;
+    uint16_t *d_eoff = nullptr;
+    long pattern_id = -1;
+    int ncolors = 0;
+    std::vector<int> color_ptr;
+    int *d_colors = nullptr;
+    int ngp_total = 0;
+    // host-API staging
+    double *d_uf = nullptr, *d_up = nullptr, *d_ff = nullptr, *d_fp = nullptr, *d_out3 = nullptr;
+};
+
+static int model_free(hfem_model *md)
+{
+    void *ps[] = {md->d_coords, md->d_mat, md->d_state, md->d_conn, md->d_emat, md->d_eqn, md->d_bad,
+                  md->d_eoff,   md->d_colors, md->d_uf, md->d_up, md->d_ff, md->d_fp, md->d_out3};
+    for (void *p : ps)
+        if (p) HIPFEM_CHECK(hipFree(p));
+    md->d_coords = md->d_mat = md->d_state = md->d_uf = md->d_up = md->d_ff = md->d_fp = md->d_out3 = nullptr;
+    md->d_conn = md->d_emat = md->d_eqn = md->d_bad = md->d_colors = nullptr;
+    md->d_eoff = nullptr;
+    md->pattern_id = -1;
+    md->ncolors = 0;
+    return 0;
+}
+
+static int nen_of(int et) { return et == HIPFEM_LSPACE ? 8 : et == HIPFEM_LTRSPACE ? 4 : et == HIPFEM_QSPACE ? 20 : 0; }
+static int ngp_of(int et) { return et == HIPFEM_LSPACE ? 8 : et == HIPFEM_LTRSPACE ? 1 : et == HIPFEM_QSPACE ? 27 : 0; }
+
+extern "C" int hfem_model_create(hipfem_ctx *ctx, hfem_model **out)
+{
+    HIPFEM_REQUIRE(ctx && out, HIPFEM_EINVAL, "hfem_model_create: null argument");
+    hfem_model *m = new hfem_model();
+    m->ctx = ctx;
+    ctx_retain(ctx);
+    *out = m;
+    return 0;
+}
+
+extern "C" void hfem_model_destroy(hfem_model *md)
+{
+    if (!md) return;
+    hipfem_ctx *c = md->ctx;
+    {
+        DeviceGuard g(c->device);
+        (void)hipStreamSynchronize(c->stream);
+        model_free(md);
+        delete md;
+    }
+    ctx_release(c);
+}
+
+extern "C" int hfem_model_set_mesh(hfem_model *md, int nnode, const double *coords, int etype, int nelem,
+                                   const int *conn, const int *emat, int nmat, const double *mat, const int *eqn,
+                                   int neq, int npeq)
+{
+    HIPFEM_REQUIRE(md && coords && conn && emat && mat && eqn, HIPFEM_EINVAL, "set_mesh: null argument");
+    const int nen = nen_of(etype);
+    HIPFEM_REQUIRE(nen > 0, HIPFEM_EINVAL, "set_mesh: unknown element type %d", etype);
+    HIPFEM_REQUIRE(nnode > 0 && nelem >= 0 && nmat > 0 && neq >= 0 && npeq >= 0, HIPFEM_EINVAL, "set_mesh: bad sizes");
+    HIPFEM_REQUIRE((int64_t)nelem * ngp_of(etype) < INT32_MAX, HIPFEM_ETOOBIG, "set_mesh: too many Gauss points");
+    // host-side validation of every index the kernels will dereference
+    for (int64_t i = 0; i < (int64_t)nelem * nen; i++)
+        HIPFEM_REQUIRE(conn[i] >= 0 && conn[i] < nnode, HIPFEM_ERANGE, "set_mesh: conn[%lld]=%d outside [0,%d)",
+                       (long long)i, conn[i], nnode);
+    for (int e = 0; e < nelem; e++)
+        HIPFEM_REQUIRE(emat[e] >= 0 && emat[e] < nmat, HIPFEM_ERANGE, "set_mesh: emat[%d]=%d outside [0,%d)", e, emat[e], nmat);
+    for (int k = 0; k < nmat; k++) {
+        const int model = (int)mat[(size_t)k * HIPFEM_MATP];
+        HIPFEM_REQUIRE(model == HIPFEM_MAT_ISOLE || model == HIPFEM_MAT_MISES, HIPFEM_EINVAL,
+                       "set_mesh: material %d has unknown model %d", k, model);
+    }
+    for (int64_t i = 0; i < (int64_t)nnode * 3; i++)
+        HIPFEM_REQUIRE(eqn[i] <= neq && eqn[i] >= -npeq, HIPFEM_ERANGE, "set_mesh: eqn[%lld]=%d outside [-%d,%d]",
+                       (long long)i, eqn[i], npeq, neq);
+    DeviceGuard g(md->ctx->device);
+    int rc = model_free(md);
+    if (rc) return rc;
+    md->nnode = nnode;
+    md->etype = etype;
+    md->nen = nen;
+    md->ngp = ngp_of(etype);
+    md->nelem = nelem;
+    md->nmat = nmat;
+    md->neq = neq;
+    md->npeq = npeq;
+    md->ngp_total = nelem * md->ngp;
+    md->h_conn.assign(conn, conn + (size_t)nelem * nen);
+    md->h_eqn.assign(eqn, eqn + (size_t)nnode * 3);
+    hipStream_t st = md->ctx->stream;
+    HIPFEM_CHECK(hipMalloc(&md->d_coords, sizeof(double) * 3 * nnode));
+    HIPFEM_CHECK(hipMalloc(&md->d_conn, sizeof(int) * std::max<size_t>(1, (size_t)nelem * nen)));
+    HIPFEM_CHECK(hipMalloc(&md->d_emat, sizeof(int) * std::max(1, nelem)));
+    HIPFEM_CHECK(hipMalloc(&md->d_mat, sizeof(double) * HIPFEM_MATP * nmat));
+    HIPFEM_CHECK(hipMalloc(&md->d_eqn, sizeof(int) * 3 * nnode));
+    HIPFEM_CHECK(hipMalloc(&md->d_bad, sizeof(int)));
+    HIPFEM_CHECK(hipMalloc(&md->d_state, sizeof(double) * NST * std::max(1, md->ngp_total)));
+    HIPFEM_CHECK(hipMalloc(&md->d_out3, sizeof(double) * 4));
+    HIPFEM_CHECK(hipMalloc(&md->d_uf, sizeof(double) * std::max(1, neq)));
+    HIPFEM_CHECK(hipMalloc(&md->d_ff, sizeof(double) * std::max(1, neq)));
+    HIPFEM_CHECK(hipMalloc(&md->d_up, sizeof(double) * std::max(1, npeq)));
+    HIPFEM_CHECK(hipMalloc(&md->d_fp, sizeof(double) * std::max(1, npeq)));
+    HIPFEM_CHECK(hipMemcpyAsync(md->d_coords, coords, sizeof(double) * 3 * nnode, hipMemcpyHostToDevice, st));
+    if (nelem) {
+        HIPFEM_CHECK(hipMemcpyAsync(md->d_conn, conn, sizeof(int) * (size_t)nelem * nen, hipMemcpyHostToDevice, st));
+        HIPFEM_CHECK(hipMemcpyAsync(md->d_emat, emat, sizeof(int) * nelem, hipMemcpyHostToDevice, st));
+    }
+    HIPFEM_CHECK(hipMemcpyAsync(md->d_mat, mat, sizeof(double) * HIPFEM_MATP * nmat, hipMemcpyHostToDevice, st));
+    HIPFEM_CHECK(hipMemcpyAsync(md->d_eqn, eqn, sizeof(int) * 3 * nnode, hipMemcpyHostToDevice, st));
+    const int64_t ns = (int64_t)NST * std::max(1, md->ngp_total);
+    k_fill<<<(int)std::min<int64_t>((ns + 255) / 256, 8192), 256, 0, st>>>(ns, 0.0, md->d_state);
+    HIPFEM_CHECK(hipGetLastError());
+    HIPFEM_CHECK(hipStreamSynchronize(st));
+    return 0;
+}
+
+// Pattern + eoff map + coloring, built on the host from node adjacency.
+extern "C" int hfem_model_build_structure(hfem_model *md, hipcsr_mtx *K)
+{
+    HIPFEM_REQUIRE(md && K && md->d_conn, HIPFEM_EINVAL, "build_structure: model has no mesh");
+    HIPFEM_REQUIRE(K->ctx == md->ctx, HIPFEM_EINVAL, "build_structure: matrix of another context");
+    DeviceGuard g(md->ctx->device);
+    const int nen = md->nen, nnode = md->nnode, nelem = md->nelem, neq = md->neq;
+    const int *conn = md->h_conn.data(), *eqn = md->h_eqn.data();
+    // node -> elements (ascending)
+    std::vector<int> nptr((size_t)nnode + 1, 0), nlst((size_t)nelem * nen);
+    for (size_t i = 0; i < (size_t)nelem * nen; i++) nptr[conn[i] + 1]++;
+    for (int n = 0; n < nnode; n++) nptr[n + 1] += nptr[n];
+    {
+        std::vector<int> fill(nptr.begin(), nptr.end() - 1);
+        for (int e = 0; e < nelem; e++)
+            for (int a = 0; a < nen; a++) nlst[fill[conn[(size_t)e * nen + a]]++] = e;
+    }
+    auto nfree = [&](int n) { return (eqn[3 * n] > 0) + (eqn[3 * n + 1] > 0) + (eqn[3 * n + 2] > 0); };
+    auto firsteq = [&](int n) {
+        for (int d = 0; d < 3; d++)
+            if (eqn[3 * n + d] > 0) return eqn[3 * n + d];
+        return 0;
+    };
+    // per node: sorted neighbor list (free neighbors) and row length
+    std::vector<int> rowlen((size_t)neq, 0);
+    std::vector<int64_t> nbptr((size_t)nnode + 1, 0);
+    std::vector<int> mark((size_t)nnode, -1);
+    std::vector<int> nb;
+    std::vector<std::vector<int>> nbrs((size_t)nnode);
+    for (int a = 0; a < nnode; a++) {
+        if (!nfree(a)) continue;
+        nb.clear();
+        for (int k = nptr[a]; k < nptr[a + 1]; k++) {
+            const int e = nlst[k];
+            for (int b = 0; b < nen; b++) {
+                const int nbn = conn[(size_t)e * nen + b];
+                if (mark[nbn] == a || !nfree(nbn)) continue;
+                mark[nbn] = a;
+                nb.push_back(nbn);
+            }
+        }
+        std::sort(nb.begin(), nb.end(), [&](int x, int y) { return firsteq(x) < firsteq(y); });
+        nbrs[a] = nb;
+        int len = 0;
+        for (int x : nb) len += nfree(x);
+        for (int d = 0; d < 3; d++)
+            if (eqn[3 * a + d] > 0) rowlen[eqn[3 * a + d] - 1] = len;
+    }
+    std::vector<int32_t> rowptr((size_t)neq + 1, 0);
+    int64_t total = 0;
+    for (int r = 0; r < neq; r++) {
+        total += rowlen[r];
+        HIPFEM_REQUIRE(total < INT32_MAX, HIPFEM_ETOOBIG, "pattern exceeds int32 indices");
+        rowptr[r + 1] = (int32_t)total;
+    }
+    std::vector<int32_t> colind((size_t)total);
+    std::vector<uint16_t> eoff((size_t)nelem * nen * nen, 0xFFFF);
+    std::vector<int> offmark((size_t)nnode, -1);
+    for (int a = 0; a < nnode; a++) {
+        if (!nfree(a)) continue;
+        // column list of node a's rows and the offset of every neighbor
+        std::vector<int32_t> cols;
+        cols.reserve(81);
+        for (int x : nbrs[a]) {
+            HIPFEM_REQUIRE(cols.size() < 0xFFFF, HIPFEM_ETOOBIG, "row of node %d too long", a);
+            offmark[x] = (int)cols.size();
+            for (int d = 0; d < 3; d++)
+                if (eqn[3 * x + d] > 0) cols.push_back(eqn[3 * x + d] - 1);
+        }
+        for (int d = 0; d < 3; d++)
+            if (eqn[3 * a + d] > 0) std::copy(cols.begin(), cols.end(), colind.begin() + rowptr[eqn[3 * a + d] - 1]);
+        for (int k = nptr[a]; k < nptr[a + 1]; k++) {
+            const int e = nlst[k];
+            int la = -1;
+            for (int b = 0; b < nen; b++)
+                if (conn[(size_t)e * nen + b] == a) { la = b; break; }
+            for (int b = 0; b < nen; b++) {
+                const int nbn = conn[(size_t)e * nen + b];
+                if (!nfree(nbn)) continue;
+                eoff[((size_t)e * nen + la) * nen + b] = (uint16_t)offmark[nbn];
+            }
+        }
+    }
+    // greedy coloring in ascending element order: smallest color not used by
+    // an element sharing a node
+    std::vector<uint64_t> nodemask((size_t)nnode, 0);
+    std::vector<int> color((size_t)nelem, 0);
+    int ncol = 0;
+    for (int e = 0; e < nelem; e++) {
+        uint64_t used = 0;
+        for (int a = 0; a < nen; a++) used |= nodemask[conn[(size_t)e * nen + a]];
+        HIPFEM_REQUIRE(~used != 0, HIPFEM_ETOOBIG, "coloring needs more than 64 colors");
+        const int c = __builtin_ctzll(~used);
+        color[e] = c;
+        ncol = std::max(ncol, c + 1);
+        for (int a = 0; a < nen; a++) nodemask[conn[(size_t)e * nen + a]] |= (1ull << c);
+    }
+    std::vector<int> cptr((size_t)ncol + 1, 0), clist((size_t)nelem);
+    for (int e = 0; e < nelem; e++) cptr[color[e] + 1]++;
+    for (int c = 0; c < ncol; c++) cptr[c + 1] += cptr[c];
+    {
+        std::vector<int> fill(cptr.begin(), cptr.end() - 1);
+        for (int e = 0; e < nelem; e++) clist[fill[color[e]]++] = e;
+    }
+    int rc = csr_alloc(K, neq, rowptr, colind);
+    if (rc) return rc;
+    hipStream_t st = md->ctx->stream;
+    if (md->d_eoff) HIPFEM_CHECK(hipFree(md->d_eoff));
+    if (md->d_colors) HIPFEM_CHECK(hipFree(md->d_colors));
+    md->d_eoff = nullptr;
+    md->d_colors = nullptr;
+    HIPFEM_CHECK(hipMalloc(&md->d_eoff, sizeof(uint16_t) * std::max<size_t>(1, eoff.size())));
+    HIPFEM_CHECK(hipMalloc(&md->d_colors, sizeof(int) * std::max(1, nelem)));
+    if (!eoff.empty())
+        HIPFEM_CHECK(hipMemcpyAsync(md->d_eoff, eoff.data(), sizeof(uint16_t) * eoff.size(), hipMemcpyHostToDevice, st));
+    if (nelem) HIPFEM_CHECK(hipMemcpyAsync(md->d_colors, clist.data(), sizeof(int) * nelem, hipMemcpyHostToDevice, st));
+    HIPFEM_CHECK(hipStreamSynchronize(st));
+    md->color_ptr = cptr;
+    md->ncolors = ncol;
+    md->pattern_id = K->pattern_id;
+    return 0;
+}
+
+static int check_bad(hfem_model *md, const char *what)
+{
+    int bad = INT32_MAX;
+    HIPFEM_CHECK(hipMemcpyAsync(&bad, md->d_bad, sizeof(int), hipMemcpyDeviceToHost, md->ctx->stream));
+    HIPFEM_CHECK(hipStreamSynchronize(md->ctx->stream));
+    // FEI3dTetLin::evaldNdx aborts with "negative volume" (fei3dtetlin.C:146)
+    HIPFEM_REQUIRE(bad == INT32_MAX, HIPFEM_ENUMERIC, "%s: degenerate or inverted element %d", what, bad + 1);
+    return 0;
+}
+
+static int reset_bad(hfem_model *md)
+{
+    const int big = INT32_MAX;
+    HIPFEM_CHECK(hipMemcpyAsync(md->d_bad, &big, sizeof(int), hipMemcpyHostToDevice, md->ctx->stream));
+    return 0;
+}
+
+extern "C" int hfem_model_assemble_stiffness(hfem_model *md, hipcsr_mtx *K, int mode)
+{
+    HIPFEM_REQUIRE(md && K && K->has_structure, HIPFEM_EINVAL, "assemble: no structure");
+    HIPFEM_REQUIRE(md->pattern_id == K->pattern_id, HIPFEM_EINVAL,
+                   "assemble: matrix pattern was not built by hfem_model_build_structure for this model");
+    HIPFEM_REQUIRE(mode == 0 || mode == 1, HIPFEM_EINVAL, "assemble: mode must be 0 or 1");
+    DeviceGuard g(md->ctx->device);
+    int rc = hipcsr_zero(K);
+    if (rc) return rc;
+    rc = reset_bad(md);
+    if (rc) return rc;
+    hipStream_t st = md->ctx->stream;
+    for (int c = 0; c < md->ncolors; c++) {
+        const int n = md->color_ptr[c + 1] - md->color_ptr[c];
+        if (!n) continue;
+        const int *el = md->d_colors + md->color_ptr[c];
+#define LAUNCH_ASM(ET)                                                                                              \
+    k_assemble<ET><<<n, 64, 0, st>>>(el, n, md->d_coords, md->d_conn, md->d_eqn, md->d_emat, md->d_mat, md->d_eoff, \
+                                     md->d_state, md->ngp_total, mode, K->d_rowptr, K->d_val, md->d_bad)
+        if (md->etype == HIPFEM_LSPACE) LAUNCH_ASM(HIPFEM_LSPACE);
+        else if (md->etype == HIPFEM_LTRSPACE) LAUNCH_ASM(HIPFEM_LTRSPACE);
+        else LAUNCH_ASM(HIPFEM_QSPACE);
+#undef LAUNCH_ASM
+        HIPFEM_CHECK(hipGetLastError());
+    }
+    K->version++;
+    return check_bad(md, "assemble");
+}
+
+static int internal_dev(hfem_model *md, const double *uf, const double *up, double *ff, double *fp, double *ebe3,
+                        int update)
+{
+    hipStream_t st = md->ctx->stream;
+    HIPFEM_REQUIRE(md->ncolors > 0 || md->nelem == 0, HIPFEM_EINVAL,
+                   "internal forces: call hfem_model_build_structure first (it builds the element coloring)");
+    int epb = md->etype == HIPFEM_LSPACE ? IntCfg<HIPFEM_LSPACE>::EPB
+              : md->etype == HIPFEM_LTRSPACE ? IntCfg<HIPFEM_LTRSPACE>::EPB : IntCfg<HIPFEM_QSPACE>::EPB;
+    int npart = 0;
+    for (int c = 0; c < md->ncolors; c++) npart += (md->color_ptr[c + 1] - md->color_ptr[c] + epb - 1) / epb;
+    int rc = ensure_partials(md->ctx, 3 * (size_t)std::max(1, npart));
+    if (rc) return rc;
+    rc = reset_bad(md);
+    if (rc) return rc;
+    if (ff && md->neq) k_fill<<<std::min(8192, (md->neq + 255) / 256), 256, 0, st>>>(md->neq, 0.0, ff);
+    if (fp && md->npeq) k_fill<<<std::min(8192, (md->npeq + 255) / 256), 256, 0, st>>>(md->npeq, 0.0, fp);
+    double *part = md->ctx->d_partials;
+    int poff = 0;
+    for (int c = 0; c < md->ncolors; c++) {
+        const int n = md->color_ptr[c + 1] - md->color_ptr[c];
+        if (!n) continue;
+        const int nb = (n + epb - 1) / epb;
+        const int *el = md->d_colors + md->color_ptr[c];
+        // partial slots of this color: [poff, poff + nb) in each of the 3 planes
+#define LAUNCH_INT(ET)                                                                                          \
+    k_internal<ET><<<nb, 64, 0, st>>>(el, n, md->d_coords, md->d_conn, md->d_eqn, md->d_emat, md->d_mat, uf, up, \
+                                      md->d_state, md->ngp_total, update, ff, fp, part + poff, npart, md->d_bad)
+        if (md->etype == HIPFEM_LSPACE) LAUNCH_INT(HIPFEM_LSPACE);
+        else if (md->etype == HIPFEM_LTRSPACE) LAUNCH_INT(HIPFEM_LTRSPACE);
+        else LAUNCH_INT(HIPFEM_QSPACE);
+#undef LAUNCH_INT
+        HIPFEM_CHECK(hipGetLastError());
+        poff += nb;
+    }
+    if (ebe3) {
+        k_reduce3<<<1, 1024, 0, st>>>(part, npart, md->d_out3);
+        HIPFEM_CHECK(hipGetLastError());
+        HIPFEM_CHECK(hipMemcpyAsync(ebe3, md->d_out3, 3 * sizeof(double), hipMemcpyDeviceToHost, st));
+    }
+    return check_bad(md, "internal forces");
+}
+
+extern "C" int hfem_model_internal_forces_dev(hfem_model *md, const double *du_free, const double *du_presc,
+                                              double *df_free, double *df_presc, double *ebe3, int update_state)
+{
+    HIPFEM_REQUIRE(md && md->d_conn && (du_free || md->neq == 0), HIPFEM_EINVAL, "internal forces: bad arguments");
+    DeviceGuard g(md->ctx->device);
+    return internal_dev(md, du_free, du_presc, df_free, df_presc, ebe3, update_state);
+}
+
+extern "C" int hfem_model_internal_forces(hfem_model *md, const double *u_free, const double *u_presc, double *f_free,
+                                          double *f_presc, double *ebe3, int update_state)
+{
+    HIPFEM_REQUIRE(md && md->d_conn && (u_free || md->neq == 0), HIPFEM_EINVAL, "internal forces: bad arguments");
+    DeviceGuard g(md->ctx->device);
+    hipStream_t st = md->ctx->stream;
+    if (md->neq) HIPFEM_CHECK(hipMemcpyAsync(md->d_uf, u_free, sizeof(double) * md->neq, hipMemcpyHostToDevice, st));
+    if (u_presc && md->npeq)
+        HIPFEM_CHECK(hipMemcpyAsync(md->d_up, u_presc, sizeof(double) * md->npeq, hipMemcpyHostToDevice, st));
+    int rc = internal_dev(md, md->d_uf, (u_presc && md->npeq) ? md->d_up : nullptr, md->d_ff,
+                          md->npeq ? md->d_fp : nullptr, ebe3, update_state);
+    if (rc) return rc;
+    if (f_free && md->neq) HIPFEM_CHECK(hipMemcpyAsync(f_free, md->d_ff, sizeof(double) * md->neq, hipMemcpyDeviceToHost, st));
+    if (f_presc && md->npeq)
+        HIPFEM_CHECK(hipMemcpyAsync(f_presc, md->d_fp, sizeof(double) * md->npeq, hipMemcpyDeviceToHost, st));
+    HIPFEM_CHECK(hipStreamSynchronize(st));
+    return 0;
+}
+
+extern "C" int hfem_model_commit(hfem_model *md)
+{
+    HIPFEM_REQUIRE(md && md->d_state, HIPFEM_EINVAL, "commit: no mesh");
+    DeviceGuard g(md->ctx->device);
+    if (md->ngp_total) {
+        k_commit<<<(md->ngp_total + 255) / 256, 256, 0, md->ctx->stream>>>(md->ngp_total, md->d_state);
+        HIPFEM_CHECK(hipGetLastError());
+    }
+    return 0;
+}
+
+extern "C" int hfem_model_reset_state(hfem_model *md)
+{
+    HIPFEM_REQUIRE(md && md->d_state, HIPFEM_EINVAL, "reset: no mesh");
+    DeviceGuard g(md->ctx->device);
+    const int64_t ns = (int64_t)NST * std::max(1, md->ngp_total);
+    k_fill<<<(int)std::min<int64_t>((ns + 255) / 256, 8192), 256, 0, md->ctx->stream>>>(ns, 0.0, md->d_state);
+    HIPFEM_CHECK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int hfem_model_get_state(hfem_model *md, double *state)
+{
+    HIPFEM_REQUIRE(md && md->d_state && state, HIPFEM_EINVAL, "get_state: bad arguments");
+    DeviceGuard g(md->ctx->device);
+    const size_t n = md->ngp_total;
+    std::vector<double> soa(NST * std::max<size_t>(1, n));
+    HIPFEM_CHECK(hipStreamSynchronize(md->ctx->stream));
+    HIPFEM_CHECK(hipMemcpy(soa.data(), md->d_state, sizeof(double) * NST * n, hipMemcpyDeviceToHost));
+    for (size_t g2 = 0; g2 < n; g2++)
+        for (int k = 0; k < NST; k++) state[g2 * NST + k] = soa[k * n + g2];
+    return 0;
+}
+
+extern "C" int hfem_model_ngp_total(const hfem_model *md) { return md ? md->ngp_total : 0; }
+extern "C" int hfem_model_ncolors(const hfem_model *md) { return md ? md->ncolors : 0; }

@@ -1,0 +1,654 @@
+// hippcg: the SparseLinearSystemNM of the MI355X path (reference interface
+// src/core/sparselinsystemnm.h:56-97), reproducing IMLSolver + CG + the
+// diagonal preconditioner (imlsolver.C:102-146, iml/cg.h:22-72,
+// diagpre.C:41-68):
+//
+//   r = b - A x;  stop if |r|/|b| <= tol (|b| = 1 when b = 0)
+//   for i = 1..maxit:
+//       z = D^-1 r;  rho = r.z;  p = (i == 1) ? z : z + (rho/rho_1) p
+//       q = A p;  alpha = rho / p.q;  x += alpha p;  r -= alpha q
+//       stop if |r|/|b| <= tol
+//
+// One iteration is four launches on the context stream:
+//   k_pcg_spmv     p_new = z + beta p_old formed on the fly in the gather,
+//                  q = A p_new, per-block p.q partials
+//   k_reduce_alpha alpha = rho / sum(p.q)                 (one block)
+//   k_pcg_update   x += alpha p; r -= alpha q; z = D^-1 r; partials r.z, r.r
+//   k_reduce_beta  convergence test, beta, rho            (one block)
+// Dot products are block partials summed in a fixed order, so a solve is
+// bitwise reproducible.  Kernels return at once when the device-side "done"
+// flag is set, which lets the host enqueue iterations in chunks and read
+// the flag once per chunk.
+//
+// precision 1: FP32 matrix/vectors for an inner PCG (FP64 dot
+// accumulation) inside FP64 iterative refinement; the true residual
+// r = b - A x and the update x += d are FP64.
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <type_traits>
+
+#include "hipfem_internal.h"
+
+namespace hipfem {
+
+struct PcgState {
+    double rho, alpha, beta, normb, res, tol;
+    int iter, done, bad, pad;
+};
+
+constexpr int RED_THREADS = 1024;
+constexpr int MAX_GRID = 4096;
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+// block-wide sum of one value per thread; result valid in thread 0
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double *sh)
+{
+    v = wave_sum(v);
+    const int w = threadIdx.x / 64, l = threadIdx.x % 64;
+    if (l == 0) sh[w] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (threadIdx.x == 0)
+        for (int i = 0; i < NT / 64; i++) s += sh[i];
+    __syncthreads();
+    return s;
+}
+
+template <int LANES, typename VT, typename MT, bool FIRST>
+__global__ __launch_bounds__(256) void k_pcg_spmv(int n, const int32_t *__restrict__ rowptr,
+                                                  const int32_t *__restrict__ col, const MT *__restrict__ val,
+                                                  const VT *__restrict__ z, const VT *__restrict__ pold,
+                                                  VT *__restrict__ pnew, VT *__restrict__ q,
+                                                  const PcgState *__restrict__ st, double *__restrict__ partials)
+{
+    __shared__ double sh[4];
+    if (st->done) return;
+    const VT beta = FIRST ? VT(0) : (VT)st->beta;
+    const int rows_per_block = 256 / LANES;
+    const int lane = threadIdx.x % LANES;
+    double acc = 0.0;
+    for (int row = blockIdx.x * rows_per_block + threadIdx.x / LANES; row < n; row += gridDim.x * rows_per_block) {
+        const int s = rowptr[row], e = rowptr[row + 1];
+        VT sum = VT(0);
+        for (int k = s + lane; k < e; k += LANES) {
+            const int c = col[k];
+            const VT pc = FIRST ? z[c] : fma(beta, pold[c], z[c]);
+            sum += (VT)val[k] * pc;
+        }
+#pragma unroll
+        for (int off = LANES / 2; off > 0; off >>= 1) sum += __shfl_xor(sum, off, LANES);
+        if (lane == 0) {
+            const VT pi = FIRST ? z[row] : fma(beta, pold[row], z[row]);
+            pnew[row] = pi;
+            q[row] = sum;
+            acc += (double)pi * (double)sum;
+        }
+    }
+    const double t = block_sum<256>(acc, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+template <typename XT, typename VT>
+__global__ __launch_bounds__(256) void k_pcg_update(int n, XT *__restrict__ x, VT *__restrict__ r, VT *__restrict__ z,
+                                                    const VT *__restrict__ p, const VT *__restrict__ q,
+                                                    const VT *__restrict__ dinv, const PcgState *__restrict__ st,
+                                                    double *__restrict__ partials)
+{
+    __shared__ double sh[4];
+    if (st->done) return;
+    const double alpha = st->alpha;
+    const VT a = (VT)alpha;
+    double rz = 0.0, rr = 0.0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        x[i] = fma((XT)a, (XT)p[i], x[i]);
+        const VT ri = fma(-a, q[i], r[i]);
+        r[i] = ri;
+        const VT zi = dinv[i] * ri;
+        z[i] = zi;
+        rz += (double)ri * (double)zi;
+        rr += (double)ri * (double)ri;
+    }
+    const double s1 = block_sum<256>(rz, sh);
+    const double s2 = block_sum<256>(rr, sh);
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = s1;
+        partials[gridDim.x + blockIdx.x] = s2;
+    }
+}
+
+// r = b - q (q = A x, or x == 0), z = D^-1 r; partials b.b, r.r, r.z
+template <typename VT>
+__global__ __launch_bounds__(256) void k_pcg_init(int n, const VT *__restrict__ b, const VT *__restrict__ q, int x_zero,
+                                                  VT *__restrict__ r, VT *__restrict__ z, const VT *__restrict__ dinv,
+                                                  double *__restrict__ partials)
+{
+    __shared__ double sh[4];
+    double bb = 0.0, rr = 0.0, rz = 0.0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        const VT bi = b[i];
+        const VT ri = x_zero ? bi : bi - q[i];
+        r[i] = ri;
+        const VT zi = dinv[i] * ri;
+        z[i] = zi;
+        bb += (double)bi * (double)bi;
+        rr += (double)ri * (double)ri;
+        rz += (double)ri * (double)zi;
+    }
+    const double s1 = block_sum<256>(bb, sh);
+    const double s2 = block_sum<256>(rr, sh);
+    const double s3 = block_sum<256>(rz, sh);
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = s1;
+        partials[gridDim.x + blockIdx.x] = s2;
+        partials[2 * gridDim.x + blockIdx.x] = s3;
+    }
+}
+
+__device__ double reduce_partials(const double *__restrict__ p, int nb, double *sh)
+{
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nb; i += RED_THREADS) s += p[i];
+    return block_sum<RED_THREADS>(s, sh);
+}
+
+__global__ __launch_bounds__(RED_THREADS) void k_reduce_init(PcgState *st, const double *__restrict__ partials, int nb,
+                                                             double tol)
+{
+    __shared__ double sh[RED_THREADS / 64];
+    const double bb = reduce_partials(partials, nb, sh);
+    const double rr = reduce_partials(partials + nb, nb, sh);
+    const double rz = reduce_partials(partials + 2 * nb, nb, sh);
+    if (threadIdx.x == 0) {
+        double normb = sqrt(bb);
+        if (normb == 0.0) normb = 1;
+        st->normb = normb;
+        st->tol = tol;
+        st->res = sqrt(rr) / normb;
+        st->rho = rz;
+        st->alpha = 0.0;
+        st->beta = 0.0;
+        st->iter = 0;
+        st->bad = 0;
+        st->done = (st->res <= tol) ? 1 : 0;
+    }
+}
+
+__global__ __launch_bounds__(RED_THREADS) void k_reduce_alpha(PcgState *st, const double *__restrict__ partials, int nb)
+{
+    __shared__ double sh[RED_THREADS / 64];
+    if (st->done) return;
+    const double pq = reduce_partials(partials, nb, sh);
+    if (threadIdx.x == 0) st->alpha = st->rho / pq;
+}
+
+__global__ __launch_bounds__(RED_THREADS) void k_reduce_beta(PcgState *st, const double *__restrict__ partials, int nb,
+                                                             int iter)
+{
+    __shared__ double sh[RED_THREADS / 64];
+    if (st->done) return;
+    const double rz = reduce_partials(partials, nb, sh);
+    const double rr = reduce_partials(partials + nb, nb, sh);
+    if (threadIdx.x == 0) {
+        const double res = sqrt(rr) / st->normb;
+        st->iter = iter;
+        st->res = res;
+        if (res <= st->tol) {
+            st->done = 1;
+        } else if (!(res == res)) {
+            st->done = 1;
+            st->bad = 1;
+        } else {
+            st->beta = rz / st->rho;
+            st->rho = rz;
+        }
+    }
+}
+
+__global__ void k_dinv(int n, const int32_t *__restrict__ diag, const double *__restrict__ val, int use_diag,
+                       double *__restrict__ dinv, float *__restrict__ dinv32, int *__restrict__ zero_row)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double d = 1.0;
+    if (use_diag) {
+        d = diag[i] >= 0 ? val[diag[i]] : 0.0;
+        if (d == 0.0) atomicMin(zero_row, i);
+    }
+    dinv[i] = 1. / d;
+    dinv32[i] = (float)(1. / d);
+}
+
+__global__ void k_d2f(int n, const double *__restrict__ a, float *__restrict__ b)
+{
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) b[i] = (float)a[i];
+}
+
+__global__ void k_zero_f(int n, float *__restrict__ a)
+{
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) a[i] = 0.0f;
+}
+
+__global__ void k_add_f2d(int n, const float *__restrict__ d, double *__restrict__ x)
+{
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) x[i] += (double)d[i];
+}
+
+// r = b - q (FP64), partial r.r, b.b
+__global__ __launch_bounds__(256) void k_residual(int n, const double *__restrict__ b, const double *__restrict__ q,
+                                                  double *__restrict__ r, double *__restrict__ partials)
+{
+    __shared__ double sh[4];
+    double rr = 0.0, bb = 0.0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        const double ri = b[i] - q[i];
+        r[i] = ri;
+        rr += ri * ri;
+        bb += b[i] * b[i];
+    }
+    const double s1 = block_sum<256>(rr, sh);
+    const double s2 = block_sum<256>(bb, sh);
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = s1;
+        partials[gridDim.x + blockIdx.x] = s2;
+    }
+}
+
+__global__ __launch_bounds__(RED_THREADS) void k_reduce2(const double *__restrict__ partials, int nb, double *out)
+{
+    __shared__ double sh[RED_THREADS / 64];
+    const double a = reduce_partials(partials, nb, sh);
+    const double b = reduce_partials(partials + nb, nb, sh);
+    if (threadIdx.x == 0) {
+        out[0] = a;
+        out[1] = b;
+    }
+}
+
+static int vec_grid(int n) { return std::max(1, std::min((n + 255) / 256, MAX_GRID)); }
+static int spmv_grid(int n, int lanes) { return std::max(1, std::min((n + 256 / lanes - 1) / (256 / lanes), MAX_GRID)); }
+
+template <typename VT, typename MT, bool FIRST>
+static void launch_pcg_spmv(int lanes, int grid, hipStream_t s, int n, const int32_t *rp, const int32_t *ci, const MT *v,
+                            const VT *z, const VT *po, VT *pn, VT *q, const PcgState *st, double *part)
+{
+    switch (lanes) {
+    case 4: k_pcg_spmv<4, VT, MT, FIRST><<<grid, 256, 0, s>>>(n, rp, ci, v, z, po, pn, q, st, part); break;
+    case 8: k_pcg_spmv<8, VT, MT, FIRST><<<grid, 256, 0, s>>>(n, rp, ci, v, z, po, pn, q, st, part); break;
+    case 16: k_pcg_spmv<16, VT, MT, FIRST><<<grid, 256, 0, s>>>(n, rp, ci, v, z, po, pn, q, st, part); break;
+    case 32: k_pcg_spmv<32, VT, MT, FIRST><<<grid, 256, 0, s>>>(n, rp, ci, v, z, po, pn, q, st, part); break;
+    default: k_pcg_spmv<64, VT, MT, FIRST><<<grid, 256, 0, s>>>(n, rp, ci, v, z, po, pn, q, st, part); break;
+    }
+}
+
+}  // namespace hipfem
+
+using namespace hipfem;
+
+struct hippcg_solver {
+    hipfem_ctx *ctx = nullptr;
+    double tol = 1.e-5;
+    int maxit = 200;
+    int precond = 0;
+    int precision = 0;
+    double inner_tol = 1e-4;
+    int max_outer = 100;
+    int last_outer = 0;
+    int check_every = 16;
+    // preconditioner cache (IMLSolver re-inits on a new matrix or version, imlsolver.C:112-114)
+    const hipcsr_mtx *lhs = nullptr;
+    long lhs_version = -1;
+    int lhs_precond = -1;
+    int n = 0;
+    double *dinv = nullptr;
+    float *dinv32 = nullptr;
+    // work vectors
+    double *r = nullptr, *z = nullptr, *p0 = nullptr, *p1 = nullptr, *q = nullptr, *hx = nullptr, *hb = nullptr;
+    float *fr = nullptr, *fz = nullptr, *fp0 = nullptr, *fp1 = nullptr, *fq = nullptr, *fx = nullptr, *fb = nullptr;
+    PcgState *st = nullptr;
+    int *zero_row = nullptr;
+    double *scal = nullptr;     // two doubles of device scratch
+    // optional per-launch timing of the PCG SpMV (HIP events on the context stream)
+    int profiling = 0;
+    std::vector<hipEvent_t> ev;           // 2 events per iteration slot
+    int prof_n64 = 0, prof_n32 = 0;
+    double prof_ms64 = 0.0, prof_ms32 = 0.0;
+};
+
+static int ensure_events(hippcg_solver *s, size_t slots)
+{
+    while (s->ev.size() < 2 * slots) {
+        hipEvent_t e;
+        HIPFEM_CHECK(hipEventCreate(&e));
+        s->ev.push_back(e);
+    }
+    return 0;
+}
+
+static int solver_free(hippcg_solver *s)
+{
+    void *ps[] = {s->dinv, s->dinv32, s->r, s->z, s->p0, s->p1, s->q, s->hx, s->hb,
+                  s->fr, s->fz, s->fp0, s->fp1, s->fq, s->fx, s->fb, s->st, s->zero_row, s->scal};
+    for (void *p : ps)
+        if (p) HIPFEM_CHECK(hipFree(p));
+    s->dinv = nullptr; s->dinv32 = nullptr;
+    s->r = s->z = s->p0 = s->p1 = s->q = s->hx = s->hb = nullptr;
+    s->fr = s->fz = s->fp0 = s->fp1 = s->fq = s->fx = s->fb = nullptr;
+    s->st = nullptr;
+    s->zero_row = nullptr;
+    s->scal = nullptr;
+    s->n = 0;
+    s->lhs = nullptr;
+    s->lhs_version = -1;
+    return 0;
+}
+
+static int solver_alloc(hippcg_solver *s, int n)
+{
+    if (s->n == n && s->r) return 0;
+    int rc = solver_free(s);
+    if (rc) return rc;
+    const size_t nd = sizeof(double) * std::max(1, n), nf = sizeof(float) * std::max(1, n);
+    for (double **p : {&s->dinv, &s->r, &s->z, &s->p0, &s->p1, &s->q, &s->hx, &s->hb}) HIPFEM_CHECK(hipMalloc(p, nd));
+    for (float **p : {&s->dinv32, &s->fr, &s->fz, &s->fp0, &s->fp1, &s->fq, &s->fx, &s->fb}) HIPFEM_CHECK(hipMalloc(p, nf));
+    HIPFEM_CHECK(hipMalloc(&s->st, sizeof(PcgState)));
+    HIPFEM_CHECK(hipMalloc(&s->zero_row, sizeof(int)));
+    HIPFEM_CHECK(hipMalloc(&s->scal, 2 * sizeof(double)));
+    s->n = n;
+    return 0;
+}
+
+static int init_precond(hippcg_solver *s, hipcsr_mtx *A)
+{
+    if (s->lhs == A && s->lhs_version == A->version && s->lhs_precond == s->precond) return 0;
+    hipStream_t st = s->ctx->stream;
+    const int big = INT32_MAX;
+    HIPFEM_CHECK(hipMemcpyAsync(s->zero_row, &big, sizeof(int), hipMemcpyHostToDevice, st));
+    k_dinv<<<(A->n + 255) / 256, 256, 0, st>>>(A->n, A->d_diag, A->d_val, s->precond == 1, s->dinv, s->dinv32, s->zero_row);
+    HIPFEM_CHECK(hipGetLastError());
+    int zr = big;
+    HIPFEM_CHECK(hipMemcpyAsync(&zr, s->zero_row, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPFEM_CHECK(hipStreamSynchronize(st));
+    // DiagPreconditioner::init diagpre.C:49-55: zero diagonal is an error
+    HIPFEM_REQUIRE(zr == big, HIPFEM_ENUMERIC, "DiagPreconditioner: zero diagonal detected in equation %d", zr + 1);
+    s->lhs = A;
+    s->lhs_version = A->version;
+    s->lhs_precond = s->precond;
+    return 0;
+}
+
+static int read_state(hippcg_solver *s, PcgState *out)
+{
+    hipStream_t st = s->ctx->stream;
+    HIPFEM_CHECK(hipMemcpyAsync(s->ctx->h_pinned, s->st, sizeof(PcgState), hipMemcpyDeviceToHost, st));
+    HIPFEM_CHECK(hipStreamSynchronize(st));
+    memcpy(out, s->ctx->h_pinned, sizeof(PcgState));
+    return 0;
+}
+
+// Generic PCG on device vectors.  VT: vector/arith type, MT: matrix values.
+// b, x: length n; x is the initial guess (x_zero: known to be 0).
+template <typename VT, typename XT, typename MT>
+static int pcg_run(hippcg_solver *s, hipcsr_mtx *A, const MT *val, const VT *b, XT *x, int x_zero, double tol, int maxit,
+                   VT *r, VT *z, VT *p0, VT *p1, VT *q, const VT *dinv, int *iters, double *res, int *converged)
+{
+    hipStream_t st = s->ctx->stream;
+    const int n = A->n;
+    const int lanes = spmv_lanes(A);
+    const int gv = vec_grid(n), gs = spmv_grid(n, lanes);
+    int rc = ensure_partials(s->ctx, 3 * (size_t)std::max(gv, gs));
+    if (rc) return rc;
+    double *part = s->ctx->d_partials;
+    if (!x_zero) {
+        // q = A x with x in XT (FP64 only for the plain path)
+        if constexpr (std::is_same<VT, double>::value && std::is_same<XT, double>::value) {
+            rc = csr_spmv_dev(A, x, q);
+            if (rc) return rc;
+        } else {
+            set_error("pcg: non-zero initial guess needs the FP64 path");
+            return HIPFEM_EINVAL;
+        }
+    }
+    k_pcg_init<VT><<<gv, 256, 0, st>>>(n, b, q, x_zero, r, z, dinv, part);
+    k_reduce_init<<<1, RED_THREADS, 0, st>>>(s->st, part, gv, tol);
+    HIPFEM_CHECK(hipGetLastError());
+    PcgState h;
+    rc = read_state(s, &h);
+    if (rc) return rc;
+    if (h.done) {
+        *iters = 0;
+        *res = h.res;
+        *converged = 1;
+        return 0;
+    }
+    int it = 1;
+    VT *pold = p0, *pnew = p1;
+    int chunk = s->check_every;
+    const bool prof = s->profiling != 0;
+    while (it <= maxit) {
+        const int stop = std::min(maxit, it + chunk - 1);
+        if (prof) {
+            rc = ensure_events(s, (size_t)stop);
+            if (rc) return rc;
+        }
+        for (; it <= stop; it++) {
+            if (prof) HIPFEM_CHECK(hipEventRecord(s->ev[2 * (it - 1)], st));
+            if (it == 1)
+                launch_pcg_spmv<VT, MT, true>(lanes, gs, st, n, A->d_rowptr, A->d_colind, val, z, pold, pnew, q, s->st, part);
+            else
+                launch_pcg_spmv<VT, MT, false>(lanes, gs, st, n, A->d_rowptr, A->d_colind, val, z, pold, pnew, q, s->st, part);
+            if (prof) HIPFEM_CHECK(hipEventRecord(s->ev[2 * (it - 1) + 1], st));
+            k_reduce_alpha<<<1, RED_THREADS, 0, st>>>(s->st, part, gs);
+            k_pcg_update<XT, VT><<<gv, 256, 0, st>>>(n, x, r, z, pnew, q, dinv, s->st, part);
+            k_reduce_beta<<<1, RED_THREADS, 0, st>>>(s->st, part, gv, it);
+            std::swap(pold, pnew);
+        }
+        HIPFEM_CHECK(hipGetLastError());
+        rc = read_state(s, &h);
+        if (rc) return rc;
+        if (h.done) break;
+        chunk = std::min(chunk * 2, 64);
+    }
+    *iters = h.iter;
+    *res = h.res;
+    *converged = (h.done && !h.bad && h.res <= tol) ? 1 : 0;
+    if (prof) {
+        // launches after convergence returned at once: time iterations 1..h.iter only
+        HIPFEM_CHECK(hipStreamSynchronize(st));
+        double tot = 0.0;
+        for (int k = 0; k < h.iter; k++) {
+            float ms = 0.f;
+            HIPFEM_CHECK(hipEventElapsedTime(&ms, s->ev[2 * k], s->ev[2 * k + 1]));
+            tot += ms;
+        }
+        if (std::is_same<VT, double>::value) {
+            s->prof_n64 += h.iter;
+            s->prof_ms64 += tot;
+        } else {
+            s->prof_n32 += h.iter;
+            s->prof_ms32 += tot;
+        }
+    }
+    HIPFEM_REQUIRE(!h.bad, HIPFEM_ENUMERIC, "pcg: residual became NaN at iteration %d", h.iter);
+    return 0;
+}
+
+static int solve_dev(hippcg_solver *s, hipcsr_mtx *A, const double *db, double *dx, int *reason, int *iters,
+                     double *achieved)
+{
+    HIPFEM_REQUIRE(A->has_structure, HIPFEM_EINVAL, "solve: matrix has no structure");
+    HIPFEM_REQUIRE(!A->staged, HIPFEM_EINVAL, "solve: assemble() contributions not flushed (call assembleEnd)");
+    int rc = solver_alloc(s, A->n);
+    if (rc) return rc;
+    rc = init_precond(s, A);
+    if (rc) return rc;
+    int it = 0, conv = 0;
+    double res = 0.0;
+    s->last_outer = 0;
+    s->prof_n64 = s->prof_n32 = 0;
+    s->prof_ms64 = s->prof_ms32 = 0.0;
+    if (A->n == 0) {
+        *reason = HIPFEM_CR_CONVERGED;
+        *iters = 0;
+        *achieved = 0.0;
+        return 0;
+    }
+    if (s->precision == 0) {
+        rc = pcg_run<double, double, double>(s, A, A->d_val, db, dx, 0, s->tol, s->maxit, s->r, s->z, s->p0, s->p1,
+                                              s->q, s->dinv, &it, &res, &conv);
+        if (rc) return rc;
+    } else {
+        // FP64 iterative refinement around an FP32 inner PCG
+        rc = csr_refresh_fp32(A);
+        if (rc) return rc;
+        hipStream_t st = s->ctx->stream;
+        const int n = A->n, gv = vec_grid(n);
+        rc = ensure_partials(s->ctx, 3 * (size_t)std::max(gv, MAX_GRID));
+        if (rc) return rc;
+        double *out = s->scal;
+        int total = 0;
+        for (int outer = 0; outer < s->max_outer; outer++) {
+            rc = csr_spmv_dev(A, dx, s->q);
+            if (rc) return rc;
+            k_residual<<<gv, 256, 0, st>>>(n, db, s->q, s->r, s->ctx->d_partials);
+            k_reduce2<<<1, RED_THREADS, 0, st>>>(s->ctx->d_partials, gv, out);
+            HIPFEM_CHECK(hipGetLastError());
+            double h2[2];
+            HIPFEM_CHECK(hipMemcpyAsync(s->ctx->h_pinned, out, 2 * sizeof(double), hipMemcpyDeviceToHost, st));
+            HIPFEM_CHECK(hipStreamSynchronize(st));
+            memcpy(h2, s->ctx->h_pinned, sizeof h2);
+            double normb = sqrt(h2[1]);
+            if (normb == 0.0) normb = 1;
+            res = sqrt(h2[0]) / normb;
+            s->last_outer = outer;
+            if (res <= s->tol) { conv = 1; break; }
+            if (total >= s->maxit) break;
+            k_d2f<<<gv, 256, 0, st>>>(n, s->r, s->fb);
+            k_zero_f<<<gv, 256, 0, st>>>(n, s->fx);
+            // inner solve to inner_tol relative, but never below what the outer
+            // tolerance needs from this correction
+            const double itol = std::max(s->inner_tol, 0.5 * s->tol / res);
+            int iit = 0, iconv = 0;
+            double ires = 0;
+            rc = pcg_run<float, float, float>(s, A, A->d_val32, s->fb, s->fx, 1, itol, s->maxit - total, s->fr, s->fz,
+                                              s->fp0, s->fp1, s->fq, s->dinv32, &iit, &ires, &iconv);
+            if (rc) return rc;
+            total += iit;
+            k_add_f2d<<<gv, 256, 0, st>>>(n, s->fx, dx);
+            HIPFEM_CHECK(hipGetLastError());
+            s->last_outer = outer + 1;
+        }
+        it = total;
+    }
+    *reason = conv ? HIPFEM_CR_CONVERGED : HIPFEM_CR_DIVERGED_ITS;
+    *iters = it;
+    *achieved = res;
+    return 0;
+}
+
+extern "C" int hippcg_create(hipfem_ctx *ctx, hippcg_solver **out)
+{
+    HIPFEM_REQUIRE(ctx && out, HIPFEM_EINVAL, "hippcg_create: null argument");
+    hippcg_solver *s = new hippcg_solver();
+    s->ctx = ctx;
+    ctx_retain(ctx);
+    *out = s;
+    return 0;
+}
+
+extern "C" void hippcg_destroy(hippcg_solver *s)
+{
+    if (!s) return;
+    hipfem_ctx *c = s->ctx;
+    {
+        DeviceGuard g(c->device);
+        (void)hipStreamSynchronize(c->stream);
+        solver_free(s);
+        for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
+        delete s;
+    }
+    ctx_release(c);
+}
+
+extern "C" int hippcg_set_options(hippcg_solver *s, double lstol, int lsiter, int lsprecond)
+{
+    HIPFEM_REQUIRE(s, HIPFEM_EINVAL, "null solver");
+    HIPFEM_REQUIRE(lsprecond == 0 || lsprecond == 1, HIPFEM_EINVAL, "unknown preconditioner type %d", lsprecond);
+    HIPFEM_REQUIRE(lsiter >= 0, HIPFEM_EINVAL, "lsiter < 0");
+    s->tol = lstol;
+    s->maxit = lsiter;
+    s->precond = lsprecond;
+    return 0;
+}
+
+extern "C" int hippcg_set_precision(hippcg_solver *s, int precision, double inner_tol, int max_outer)
+{
+    HIPFEM_REQUIRE(s, HIPFEM_EINVAL, "null solver");
+    HIPFEM_REQUIRE(precision == 0 || precision == 1, HIPFEM_EINVAL, "precision must be 0 or 1");
+    s->precision = precision;
+    if (inner_tol > 0) s->inner_tol = inner_tol;
+    if (max_outer > 0) s->max_outer = max_outer;
+    return 0;
+}
+
+extern "C" int hippcg_last_outer(hippcg_solver *s) { return s ? s->last_outer : 0; }
+
+extern "C" int hippcg_set_profiling(hippcg_solver *s, int on)
+{
+    HIPFEM_REQUIRE(s, HIPFEM_EINVAL, "null solver");
+    s->profiling = on ? 1 : 0;
+    return 0;
+}
+
+extern "C" int hippcg_profile(hippcg_solver *s, int *n64, double *ms64, int *n32, double *ms32)
+{
+    HIPFEM_REQUIRE(s, HIPFEM_EINVAL, "null solver");
+    if (n64) *n64 = s->prof_n64;
+    if (ms64) *ms64 = s->prof_ms64;
+    if (n32) *n32 = s->prof_n32;
+    if (ms32) *ms32 = s->prof_ms32;
+    return 0;
+}
+
+extern "C" int hippcg_solve_dev(hippcg_solver *s, hipcsr_mtx *A, const double *db, double *dx, int *reason, int *iters,
+                                double *achieved)
+{
+    HIPFEM_REQUIRE(s && A && db && dx && reason && iters && achieved, HIPFEM_EINVAL, "solve: null argument");
+    HIPFEM_REQUIRE(s->ctx == A->ctx, HIPFEM_EINVAL, "solve: solver and matrix belong to different contexts");
+    DeviceGuard g(s->ctx->device);
+    return solve_dev(s, A, db, dx, reason, iters, achieved);
+}
+
+extern "C" int hippcg_solve(hippcg_solver *s, hipcsr_mtx *A, const double *b, double *x, int *reason, int *iters,
+                            double *achieved)
+{
+    HIPFEM_REQUIRE(s && A && b && x && reason && iters && achieved, HIPFEM_EINVAL, "solve: null argument");
+    HIPFEM_REQUIRE(s->ctx == A->ctx, HIPFEM_EINVAL, "solve: solver and matrix belong to different contexts");
+    DeviceGuard g(s->ctx->device);
+    int rc = solver_alloc(s, A->n);
+    if (rc) return rc;
+    hipStream_t st = s->ctx->stream;
+    if (A->n) {
+        HIPFEM_CHECK(hipMemcpyAsync(s->hb, b, sizeof(double) * A->n, hipMemcpyHostToDevice, st));
+        HIPFEM_CHECK(hipMemcpyAsync(s->hx, x, sizeof(double) * A->n, hipMemcpyHostToDevice, st));
+    }
+    rc = solve_dev(s, A, s->hb, s->hx, reason, iters, achieved);
+    if (!rc && A->n) {
+        hipError_t e = hipMemcpyAsync(x, s->hx, sizeof(double) * A->n, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
+        if (e != hipSuccess) {
+            set_error("solve: copy back failed: %s", hipGetErrorString(e));
+            rc = HIPFEM_EHIP;
+        }
+    }
+    return rc;
+}

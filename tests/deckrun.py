@@ -1,0 +1,55 @@
+"""Run a reference deck step by step (linear material) on the oracle or on the
+GPU, and collect the quantities its errorcheck block asks for."""
+import numpy as np
+
+from oofem_amd import meshgen
+
+
+def _collect(prob, u, up, reactions_presc, state):
+    d = prob.deck
+    u_node = np.zeros((d.nnode, 3))
+    m = d.eqn > 0
+    u_node[m] = u[d.eqn[m] - 1]
+    m = d.eqn < 0
+    u_node[m] = up[-d.eqn[m] - 1]
+    reac = {}
+    for n, k in zip(*np.nonzero(d.eqn < 0)):
+        reac[(n, k + 1)] = reactions_presc[-d.eqn[n, k] - 1]
+    ngp = meshgen.NGP[d.etype]
+    st = state.reshape(d.nelem, ngp, -1)
+    return dict(u_node=u_node, reactions=reac, gp_sig=st[:, :, 20:26], gp_eps=st[:, :, 26:32], u=u)
+
+
+def run_deck_oracle(prob, tol=1e-14):
+    from oracle import oracle
+    d = prob.deck
+    rp, ci, val = oracle.csr(d)
+    out = {}
+    for k in range(1, prob.nsteps + 1):
+        up, (f, fp) = prob.step_loading(k)
+        fint, _, _ = oracle.internal_forces(d, np.zeros(d.neq), up)
+        u, it, res, rc = oracle.pcg(rp, ci, val, f - fint, tol=tol, maxit=100000)
+        state = oracle.state_new(d)
+        _, fpi, _ = oracle.internal_forces(d, u, up, state=state, update_state=True)
+        out[k] = _collect(prob, u, up, fpi - fp, state)
+    return out
+
+
+def run_deck_gpu(ctx, prob, tol=1e-14):
+    from oofem_amd.hipfem import BatchedEvaluator, HipCSRSparseMtrx, HipPCGSolver, CR_CONVERGED
+    d = prob.deck
+    K = HipCSRSparseMtrx(ctx)
+    ev = BatchedEvaluator(ctx, d)
+    ev.build_structure(K)
+    ev.assemble_stiffness(K)
+    solver = HipPCGSolver(ctx, lstol=tol, lsiter=100000, lsprecond=1)
+    out = {}
+    for k in range(1, prob.nsteps + 1):
+        up, (f, fp) = prob.step_loading(k)
+        fint, _, _ = ev.internal_forces(np.zeros(d.neq), up)
+        u = np.zeros(d.neq)
+        reason, it, res = solver.solve(K, f - fint, u)
+        assert reason == CR_CONVERGED, (reason, it, res)
+        _, fpi, _ = ev.internal_forces(u, up, update_state=True)
+        out[k] = _collect(prob, u, up, fpi - fp, ev.get_state().reshape(-1))
+    return out

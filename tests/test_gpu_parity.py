@@ -1,0 +1,235 @@
+"""HIP path against the oracle, through the C ABI (needs the GPU).
+
+Bars: integer data (patterns) bit-exact; FP64 values within a few ulps of
+the largest entry (tolerances written per test); solutions within 1e-10
+relative of the oracle's, the bar north_star sets for the plain FP64 path.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from oofem_amd import meshgen
+from oofem_amd.deck import evaluate_checks, read_deck
+from oracle import oracle
+from tests.conftest import GOLDEN
+from tests.deckrun import run_deck_gpu, run_deck_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _small_decks():
+    hexd = meshgen.cantilever_hex(4, 3, 6, jitter=0.2, e_sigma=0.5, seed=11, name="hex_small")
+    coords, tets = meshgen.kuhn_tets(4, 3, 5, 1.0, 0.8, 1.3, jitter=0.15, seed=3)
+    bc = np.zeros((coords.shape[0], 3), bool)
+    bc[: 5 * 4] = True
+    mat = np.array([[meshgen.MAT_ISOLE, 210e3, 0.3, 0, 0, 0, 0, 0]])
+    tetd = meshgen.Deck("tet_small", meshgen.LTRSPACE, coords, tets, np.zeros(len(tets), np.int32), mat, bc,
+                        np.zeros_like(coords))
+    qd = read_deck(os.path.join(GOLDEN, "cantilever_Qspace.in")).deck
+    return {"lspace": hexd, "ltrspace": tetd, "qspace": qd}
+
+
+DECKS = _small_decks()
+
+
+@pytest.mark.parametrize("name", list(DECKS))
+def test_stiffness_matches_oracle(ctx, name):
+    from oofem_amd.hipfem import BatchedEvaluator, HipCSRSparseMtrx
+    d = DECKS[name]
+    rp, ci, val = oracle.csr(d)
+    K = HipCSRSparseMtrx(ctx)
+    ev = BatchedEvaluator(ctx, d)
+    ev.build_structure(K)
+    ev.assemble_stiffness(K)
+    grp, gci, gval = K.download()
+    np.testing.assert_array_equal(grp, rp)
+    np.testing.assert_array_equal(gci, ci)
+    scale = np.abs(val).max()
+    assert np.abs(gval - val).max() <= 1e-13 * scale
+
+
+@pytest.mark.parametrize("name", list(DECKS))
+def test_spmv_matches_oracle(ctx, name):
+    from oofem_amd.hipfem import HipCSRSparseMtrx
+    d = DECKS[name]
+    rp, ci, val = oracle.csr(d)
+    K = HipCSRSparseMtrx(ctx)
+    K.buildInternalStructureCSR(rp, ci)
+    K.upload_values(val)
+    x = np.random.default_rng(2).standard_normal(d.neq)
+    y = K.times(x)
+    yo = oracle.spmv(rp, ci, val, x)
+    assert np.abs(y - yo).max() <= 1e-14 * np.abs(val).max() * np.abs(x).max() * 100
+    yt = K.timesT(x)
+    assert np.abs(yt - yo).max() <= 1e-14 * np.abs(val).max() * np.abs(x).max() * 100
+
+
+def test_config0_linear_static_parity(ctx):
+    from oofem_amd.drivers import linear_static, reaction_sums
+    d = meshgen.config0()
+    ref = oracle.linear_static(d, tol=1e-10)
+    got = linear_static(ctx, d, lstol=1e-10)
+    assert abs(got["iters"] - ref["iters"]) <= 1, (got["iters"], ref["iters"])
+    du = np.linalg.norm(got["u"] - ref["u"]) / np.linalg.norm(ref["u"])
+    dr = np.linalg.norm(got["reactions"] - ref["reactions"]) / np.linalg.norm(ref["reactions"])
+    assert du <= 1e-10, du
+    assert dr <= 1e-10, dr
+    assert abs(reaction_sums(d, got["reactions"])[1] - 1.0) < 1e-9
+
+
+def test_config0_tight_solve_matches_direct(ctx):
+    """Both sides solved to 1e-13: displacements agree to 1e-12."""
+    from oofem_amd.drivers import linear_static
+    d = meshgen.config0()
+    ref = oracle.linear_static(d, tol=1e-13)
+    got = linear_static(ctx, d, lstol=1e-13)
+    assert np.linalg.norm(got["u"] - ref["u"]) / np.linalg.norm(ref["u"]) < 1e-12
+
+
+@pytest.mark.parametrize("name", ["patch301.in", "cantilever_Qspace.in"])
+def test_reference_decks_on_gpu(ctx, name):
+    prob = read_deck(os.path.join(GOLDEN, name))
+    got = run_deck_gpu(ctx, prob)
+    checks = evaluate_checks(prob, got)
+    bad = [(c.line, v) for c, v, ok in checks if not ok]
+    assert not bad, bad
+    ref = run_deck_oracle(prob)
+    for k in ref:
+        scale = max(np.abs(ref[k]["u"]).max(), 1e-300)
+        assert np.abs(got[k]["u"] - ref[k]["u"]).max() <= 1e-10 * scale
+
+
+def _mises_deck(etype):
+    if etype == meshgen.LTRSPACE:
+        coords, conn = meshgen.kuhn_tets(3, 3, 4, 1.0, 1.0, 1.3, jitter=0.1, seed=5)
+    else:
+        d0 = meshgen.cantilever_hex(3, 3, 4, lz=1.3, jitter=0.2, seed=5)
+        coords, conn = d0.coords, d0.conn
+    bc = np.zeros((coords.shape[0], 3), bool)
+    bc[:16] = True
+    mat = np.array([[meshgen.MAT_MISES, 210e3, 0.3, 250.0, 2e3, 0, 0, 0]])
+    return meshgen.Deck("mises", etype, coords, conn, np.zeros(len(conn), np.int32), mat, bc, np.zeros_like(coords))
+
+
+@pytest.mark.parametrize("etype", [meshgen.LSPACE, meshgen.LTRSPACE])
+def test_mises_internal_forces_state_and_tangent(ctx, etype):
+    from oofem_amd.hipfem import BatchedEvaluator, HipCSRSparseMtrx
+    d = _mises_deck(etype)
+    rng = np.random.default_rng(4)
+    up = np.zeros(d.npeq)
+    u1 = rng.standard_normal(d.neq) * 4e-3          # large enough to yield most points
+    ev = BatchedEvaluator(ctx, d)
+    K = HipCSRSparseMtrx(ctx)
+    ev.build_structure(K)
+    st = oracle.state_new(d)
+    for step, u in enumerate((u1, 1.6 * u1)):
+        f, fp, ebe = ev.internal_forces(u, up, update_state=True)
+        fo, fpo, ebeo = oracle.internal_forces(d, u, up, state=st, update_state=True)
+        scale = np.abs(fo).max()
+        assert np.abs(f - fo).max() <= 1e-12 * scale
+        assert np.abs(fp - fpo).max() <= 1e-12 * scale
+        np.testing.assert_allclose(ebe, ebeo, rtol=1e-12)
+        gs = ev.get_state()
+        os_ = st.reshape(-1, 38)
+        assert (os_[:, 13] > os_[:, 6]).mean() > 0.3      # many points plastic in this step
+        for lo, hi in ((7, 14), (14, 20), (20, 26)):
+            sc = max(np.abs(os_[:, lo:hi]).max(), 1e-300)
+            assert np.abs(gs[:, lo:hi] - os_[:, lo:hi]).max() <= 1e-11 * sc, (lo, hi)
+        ev.assemble_stiffness(K, tangent=True)
+        _, _, gval = K.download()
+        _, _, oval = oracle.csr(d, state=st, tangent=True)
+        assert np.abs(gval - oval).max() <= 1e-12 * np.abs(oval).max()
+        ev.commit()
+        oracle.state_commit(d, st)
+
+
+def test_hipcsr_sparsemtrx_api(ctx):
+    """buildInternalStructure from location arrays, assemble(loc, K_e),
+    at(), zero(), version counter, timesT, add, scale."""
+    from oofem_amd.hipfem import BatchedEvaluator, HipCSRSparseMtrx, HipFemError
+    d = DECKS["lspace"]
+    loc = np.where(d.eqn[d.conn] > 0, d.eqn[d.conn], 0).reshape(d.nelem, -1)
+    A = HipCSRSparseMtrx(ctx)
+    A.buildInternalStructure(d.neq, loc)
+    B = HipCSRSparseMtrx(ctx)
+    ev = BatchedEvaluator(ctx, d)
+    ev.build_structure(B)
+    ra, ca, _ = A.download()
+    rb, cb, _ = B.download()
+    np.testing.assert_array_equal(ra, rb)
+    np.testing.assert_array_equal(ca, cb)
+    # host-side element scatter through assemble(loc, mat)
+    v0 = A.giveVersion()
+    A.assembleBegin()
+    for e in range(d.nelem):
+        X = d.coords[d.conn[e]]
+        mp = d.mat[d.emat[e]]
+        Ke = oracle.element_stiffness(d.etype, X, oracle.isole_D(mp[1], mp[2]))
+        A.assemble(loc[e], Ke)
+    A.assembleEnd()
+    assert A.giveVersion() > v0
+    ev.assemble_stiffness(B)
+    _, _, va = A.download()
+    _, _, vb = B.download()
+    assert np.abs(va - vb).max() <= 1e-13 * np.abs(va).max()
+    i, j = int(loc[0][loc[0] > 0][0]), int(loc[0][loc[0] > 0][-1])
+    assert A.at(i, j) == va[np.searchsorted(ca[ra[i - 1]:ra[i]], j - 1) + ra[i - 1]]
+    A.set_at(i, j, 7.5)
+    assert A.at(i, j) == 7.5
+    assert not A.isAllocatedAt(1, d.neq)           # far-apart nodes share no element
+    assert A.at(1, d.neq) == 0.0                     # missing entries read as zero ...
+    with pytest.raises(HipFemError):
+        A.set_at(1, d.neq, 1.0)                      # ... and cannot be written
+    with pytest.raises(HipFemError):
+        A.at(0, 1)                                   # 1-based bounds (SparseMtrx::checkBounds)
+    x = np.random.default_rng(0).standard_normal(d.neq)
+    B.times_scalar(2.0)
+    np.testing.assert_allclose(B.times(x), 2.0 * (sp_from(vb, cb, rb) @ x), rtol=1e-12, atol=1e-9)
+    A.zero()
+    assert np.abs(A.download()[2]).max() == 0.0
+
+
+def sp_from(v, c, r):
+    import scipy.sparse as sp
+    return sp.csr_matrix((v, c, r))
+
+
+def test_pcg_edge_cases(ctx):
+    from oofem_amd.hipfem import CR_CONVERGED, CR_DIVERGED_ITS, HipCSRSparseMtrx, HipFemError, HipPCGSolver
+    d = DECKS["lspace"]
+    rp, ci, val = oracle.csr(d)
+    K = HipCSRSparseMtrx(ctx)
+    K.buildInternalStructureCSR(rp, ci)
+    K.upload_values(val)
+    s = HipPCGSolver(ctx, lstol=1e-10, lsiter=1000, lsprecond=1)
+    x = np.zeros(d.neq)
+    assert s.solve(K, np.zeros(d.neq), x) == (CR_CONVERGED, 0, 0.0)   # b = 0: |b| := 1, r = 0
+    assert not x.any()
+    b = np.random.default_rng(1).standard_normal(d.neq)
+    s2 = HipPCGSolver(ctx, lstol=1e-12, lsiter=3, lsprecond=1)
+    reason, it, res = s2.solve(K, b, np.zeros(d.neq))
+    assert reason == CR_DIVERGED_ITS and it == 3
+    # the oracle's CG after 3 iterations gives the same residual
+    xo, ito, reso, rco = oracle.pcg(rp, ci, val, b, tol=1e-12, maxit=3)
+    assert rco == 1 and abs(res - reso) <= 1e-12 * reso
+    # warm start: a converged x is returned at once
+    x = np.zeros(d.neq)
+    s.solve(K, b, x)
+    assert s.solve(K, b, x)[1] == 0
+    # zero diagonal -> DiagPreconditioner error
+    val2 = val.copy()
+    val2[np.searchsorted(ci[rp[0]:rp[1]], 0) + rp[0]] = 0.0
+    K.upload_values(val2)
+    with pytest.raises(HipFemError, match="zero diagonal"):
+        s.solve(K, b, np.zeros(d.neq))
+
+
+def test_mixed_precision_refinement(ctx):
+    from oofem_amd.drivers import linear_static
+    d = meshgen.cantilever_hex(8, 8, 24, jitter=0.2, e_sigma=0.5, seed=3)
+    ref = linear_static(ctx, d, lstol=1e-12)
+    got = linear_static(ctx, d, lstol=1e-10, precision=1, inner_tol=1e-4)
+    assert got["outer"] >= 2
+    assert got["resid"] <= 1e-10
+    assert np.linalg.norm(got["u"] - ref["u"]) / np.linalg.norm(ref["u"]) < 1e-8

@@ -43,6 +43,53 @@ def make_deck(args):
     return meshgen.config1(nx=args.nx, ny=args.ny, nz=args.nz)
 
 
+def apply_shape(K, spec):
+    """--shape lanes:L:U[:nt] or stream:R:U"""
+    if not spec:
+        return
+    p = spec.split(":")
+    if p[0] == "lanes":
+        K.set_spmv_shape(int(p[1]), int(p[2]), int(p[3]) if len(p) > 3 else 0)
+    elif p[0] == "stream":
+        K.set_spmv_stream(int(p[1]), int(p[2]))
+    else:
+        raise SystemExit(f"bad --shape {spec}")
+
+
+def run_kernels(args, deck):
+    """Kernel microbenchmark for counter runs: a copy-like calibration kernel
+    of known size (y = x over 2^27 doubles: 1 GiB read + 1 GiB written), then
+    --iters PCG iterations (FP64 and FP32-inner) on the assembled K."""
+    from oofem_amd.hipfem import BatchedEvaluator, Context, DeviceVector, HipCSRSparseMtrx, HipPCGSolver
+    ctx = Context(0)
+    n_copy = 1 << 27
+    a = DeviceVector(ctx, n_copy)
+    b = DeviceVector(ctx, n_copy)
+    for _ in range(3):
+        b.axpby(1.0, a, 0.0)
+    ctx.synchronize()
+    K = HipCSRSparseMtrx(ctx)
+    ev = BatchedEvaluator(ctx, deck)
+    ev.build_structure(K)
+    ev.assemble_stiffness(K)
+    apply_shape(K, args.shape)
+    f, _ = deck.external_load()
+    db = DeviceVector(ctx, deck.neq, f)
+    dx = DeviceVector(ctx, deck.neq)
+    out = {}
+    for prec in (0, 1):
+        s = HipPCGSolver(ctx, lstol=1e-30, lsiter=args.iters, lsprecond=1, precision=prec, inner_tol=1e-30)
+        s.set_profiling(True)
+        dx.fill(0.0)
+        s.solve_dev(K, db.addr, dx.addr)
+        p = s.profile()
+        out["fp64" if prec == 0 else "fp32"] = (p["ms64"] / max(p["n64"], 1)) if prec == 0 else (p["ms32"] / max(p["n32"], 1))
+        s.close()
+    print(json.dumps(dict(mode="kernels", neq=deck.neq, nnz=K.nnz(), shape=K.spmv_shape(), copy_doubles=n_copy,
+                          spmv_avg_ms=out)), flush=True)
+    return 0
+
+
 def run_gpu(args, deck, precision):
     import numpy as np
 
@@ -54,6 +101,7 @@ def run_gpu(args, deck, precision):
     t = time.perf_counter()
     ev.build_structure(K)
     t_struct = time.perf_counter() - t
+    apply_shape(K, args.shape)
     solver = HipPCGSolver(ctx, lstol=args.tol, lsiter=args.maxit, lsprecond=1, precision=precision,
                           inner_tol=args.inner_tol)
     solver.set_profiling(True)
@@ -150,6 +198,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-nz", type=int, default=16)
     ap.add_argument("--cpu-sample-iters", type=int, default=40)
+    ap.add_argument("--shape", default="", help="SpMV shape: lanes:L:U[:nt] or stream:R:U")
+    ap.add_argument("--mode", choices=["step", "kernels"], default="step")
+    ap.add_argument("--iters", type=int, default=50, help="PCG iterations in --mode kernels")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
@@ -160,6 +211,8 @@ def main():
     deck = make_deck(args)
     log(f"[bench] deck {deck.name}: nnode={deck.nnode} nelem={deck.nelem} neq={deck.neq} "
         f"({time.perf_counter() - t:.1f} s)")
+    if args.mode == "kernels":
+        return run_kernels(args, deck)
     modes = {"fp64": [0], "mixed": [1], "both": [0, 1]}[args.precision]
     res = {}
     for prec in modes:

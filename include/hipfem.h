@@ -125,6 +125,21 @@ long    hipcsr_version(const hipcsr_mtx *m);             /* giveVersion()  :82 *
 /* copies of the pattern / values for inspection (host arrays sized n+1, nnz) */
 int hipcsr_download(hipcsr_mtx *m, int64_t *rowptr, int32_t *colind, double *val);
 int hipcsr_upload_values(hipcsr_mtx *m, const double *val);
+/* SpMV kernel shape (no counterpart in the reference): lanes per row (8, 16,
+ * 32), loads in flight per lane and pass (1, 2, 4, 6), non-temporal value and
+ * column streams (0/1).  A default is chosen from the mean row length when
+ * the pattern is built. */
+int hipcsr_set_spmv_shape(hipcsr_mtx *m, int lanes, int unroll, int nontemporal);
+int hipcsr_get_spmv_shape(const hipcsr_mtx *m, int *lanes, int *unroll, int *nontemporal);
+/* streaming SpMV form: rows_per_block (8, 16, 32, 64; 0 = back to the lane
+ * form) consecutive rows per 256-thread block, their contiguous value and
+ * column range read with unit stride; unroll 2, 4 or 6. */
+int hipcsr_set_spmv_stream(hipcsr_mtx *m, int rows_per_block, int unroll);
+/* node-block SpMV form over the same CSR arrays, available when every free
+ * node is fully free and numbered consecutively (then CSR order is a 3x3
+ * block layout); on by default for such patterns. */
+int hipcsr_set_spmv_nodeblock(hipcsr_mtx *m, int on);
+int hipcsr_is_nodeblocked(const hipcsr_mtx *m);
 /* device pointers of the CSR arrays (row pointers and columns are int32) */
 int hipcsr_device_arrays(hipcsr_mtx *m, const int32_t **rowptr, const int32_t **colind, double **val);
 
@@ -154,6 +169,10 @@ int hippcg_last_outer(hippcg_solver *s);
  * (iterations up to convergence) and their summed duration in ms, and
  * the FP32 inner launches separately. */
 int hippcg_set_profiling(hippcg_solver *s, int on);
+/* Iteration layout (no counterpart in the reference; same arithmetic):
+ * fused 1 forms p = z + beta p inside the SpMV gather, fused 0 updates p in
+ * place in its own pass and gathers p once per nonzero. */
+int hippcg_set_fused(hippcg_solver *s, int fused);
 int hippcg_profile(hippcg_solver *s, int *n_spmv64, double *ms_spmv64, int *n_spmv32, double *ms_spmv32);
 
 /* --------------------------------------------------- batched evaluator */

@@ -43,7 +43,7 @@ int ensure_partials(hipfem_ctx *ctx, size_t n)
 }
 
 // ------------------------------------------------------------------ kernels
-template <int LANES, typename VT>
+template <int LANES, int U, bool NT, typename VT>
 __global__ __launch_bounds__(256) void k_spmv(int n, const int32_t *__restrict__ rowptr,
                                               const int32_t *__restrict__ col,
                                               const VT *__restrict__ val,
@@ -54,11 +54,25 @@ __global__ __launch_bounds__(256) void k_spmv(int n, const int32_t *__restrict__
     for (int row = blockIdx.x * rows_per_block + threadIdx.x / LANES; row < n;
          row += gridDim.x * rows_per_block) {
         const int s = rowptr[row], e = rowptr[row + 1];
-        double sum = 0.0;
-        for (int k = s + lane; k < e; k += LANES) sum += (double)val[k] * x[col[k]];
-#pragma unroll
-        for (int off = LANES / 2; off > 0; off >>= 1) sum += __shfl_xor(sum, off, LANES);
+        double sum = row_partial<LANES, U, NT, double>(s, e, lane, col, val, [&](int c) { return x[c]; });
+        sum = group_sum<LANES>(sum);
         if (lane == 0) y[row] = sum;
+    }
+}
+
+template <int RPB, int U, typename VT>
+__global__ __launch_bounds__(256) void k_spmv_stream(int n, const int32_t *__restrict__ rowptr,
+                                                     const int32_t *__restrict__ col, const VT *__restrict__ val,
+                                                     const double *__restrict__ x, double *__restrict__ y)
+{
+    __shared__ double prod[STREAM_CAP];
+    __shared__ int rp[RPB + 1];
+    constexpr int LPR = 256 / RPB;
+    for (int r0 = blockIdx.x * RPB; r0 < n; r0 += gridDim.x * RPB) {
+        const double sum = stream_block_row<RPB, U, double>(r0, n, rowptr, col, val, [&](int c) { return x[c]; },
+                                                            prod, rp);
+        const int row = r0 + threadIdx.x / LPR;
+        if (threadIdx.x % LPR == 0 && row < n) y[row] = sum;
     }
 }
 
@@ -121,48 +135,118 @@ static int grid_for(int64_t work, int block = 256, int64_t cap = 8192)
     return (int)std::max<int64_t>(1, std::min(g, cap));
 }
 
-int spmv_lanes(const hipcsr_mtx *m)
+int spmv_lanes(const hipcsr_mtx *m) { return m->shape.lanes; }
+
+// default shape from the mean row length: 16 lanes for FEM rows of 30-100
+// entries, unrolled so one pass covers a typical row
+static SpmvShape default_shape(double avg_row)
 {
-    const double a = m->avg_row;
-    if (a <= 6) return 4;
-    if (a <= 12) return 8;
-    if (a <= 40) return 16;
-    if (a <= 120) return 16;
-    return 32;
+    SpmvShape s;
+    s.lanes = avg_row <= 12 ? 8 : (avg_row <= 120 ? 16 : 32);
+    const int passes = (int)std::ceil(avg_row / s.lanes);
+    s.unroll = passes <= 1 ? 1 : passes <= 2 ? 2 : passes <= 4 ? 4 : 6;
+    s.nt = 0;  // non-temporal streams measured slower on gfx950 for this access pattern
+    return s;
+}
+
+// node-block SpMV: one 32-lane half wave per node row (see nb_row)
+template <typename MT>
+__global__ __launch_bounds__(256) void k_spmv_nb(int nbr, const int32_t *__restrict__ bptr,
+                                                 const int32_t *__restrict__ bcol, const MT *__restrict__ val,
+                                                 const double *__restrict__ x, double *__restrict__ y)
+{
+    const int lane = threadIdx.x % 32;
+    for (int I = blockIdx.x * 8 + threadIdx.x / 32; I < nbr; I += gridDim.x * 8) {
+        const int bo = bptr[I], nbk = bptr[I + 1] - bo;
+        double acc[3];
+        nb_row<double>(bo, nbk, 9 * bo, lane, bcol, val,
+                       [&](int bc, double &a, double &b, double &c) {
+                           const double *xp = x + 3 * bc;
+                           a = xp[0];
+                           b = xp[1];
+                           c = xp[2];
+                       },
+                       acc);
+        if (lane == 0) {
+            y[3 * I] = acc[0];
+            y[3 * I + 1] = acc[1];
+            y[3 * I + 2] = acc[2];
+        }
+    }
 }
 
 template <typename VT>
-static void launch_spmv(int lanes, hipStream_t st, int n, const int32_t *rp, const int32_t *ci, const VT *v,
-                        const double *x, double *y)
+static int launch_spmv(const hipcsr_mtx *m, const SpmvShape &sh, hipStream_t st, int n, const int32_t *rp,
+                       const int32_t *ci, const VT *v, const double *x, double *y)
 {
-    const int rows_per_block = 256 / lanes;
-    const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((n + rows_per_block - 1) / rows_per_block, 1 << 20));
-    switch (lanes) {
-    case 4: k_spmv<4, VT><<<grid, 256, 0, st>>>(n, rp, ci, v, x, y); break;
-    case 8: k_spmv<8, VT><<<grid, 256, 0, st>>>(n, rp, ci, v, x, y); break;
-    case 16: k_spmv<16, VT><<<grid, 256, 0, st>>>(n, rp, ci, v, x, y); break;
-    case 32: k_spmv<32, VT><<<grid, 256, 0, st>>>(n, rp, ci, v, x, y); break;
-    default: k_spmv<64, VT><<<grid, 256, 0, st>>>(n, rp, ci, v, x, y); break;
+    bool ok;
+    if (sh.nodeblock && m->nodeblocked) {
+        const int grid = std::max(1, std::min((m->nbr + 7) / 8, 1 << 20));
+        k_spmv_nb<VT><<<grid, 256, 0, st>>>(m->nbr, m->d_bptr, m->d_bcol, v, x, y);
+        ok = true;
+    } else if (sh.stream) {
+        const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((n + sh.rpb - 1) / sh.rpb, 1 << 20));
+        ok = dispatch_stream(sh, [&](auto R, auto U) {
+            k_spmv_stream<decltype(R)::value, decltype(U)::value, VT><<<grid, 256, 0, st>>>(n, rp, ci, v, x, y);
+        });
+    } else {
+        const int rows_per_block = 256 / sh.lanes;
+        const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((n + rows_per_block - 1) / rows_per_block, 1 << 20));
+        ok = dispatch_shape(sh, [&](auto L, auto U, auto NT) {
+            k_spmv<decltype(L)::value, decltype(U)::value, decltype(NT)::value, VT>
+                <<<grid, 256, 0, st>>>(n, rp, ci, v, x, y);
+        });
     }
+    HIPFEM_REQUIRE(ok, HIPFEM_EINVAL, "unsupported SpMV shape (%d lanes, unroll %d)", sh.lanes, sh.unroll);
+    HIPFEM_CHECK(hipGetLastError());
+    return 0;
 }
 
 int csr_spmv_dev(hipcsr_mtx *m, const double *dx, double *dy)
 {
-    launch_spmv<double>(spmv_lanes(m), m->ctx->stream, m->n, m->d_rowptr, m->d_colind, m->d_val, dx, dy);
-    HIPFEM_CHECK(hipGetLastError());
-    return 0;
+    return launch_spmv<double>(m, m->shape, m->ctx->stream, m->n, m->d_rowptr, m->d_colind, m->d_val, dx, dy);
 }
 
 int csr_free(hipcsr_mtx *m)
 {
     for (void *p : {(void *)m->d_rowptr, (void *)m->d_colind, (void *)m->d_diag, (void *)m->d_val,
-                    (void *)m->d_val32, (void *)m->d_tptr, (void *)m->d_tcol, (void *)m->d_tmap})
+                    (void *)m->d_val32, (void *)m->d_tptr, (void *)m->d_tcol, (void *)m->d_tmap,
+                    (void *)m->d_bptr, (void *)m->d_bcol})
         if (p) HIPFEM_CHECK(hipFree(p));
     m->d_rowptr = m->d_colind = m->d_diag = m->d_tptr = m->d_tcol = m->d_tmap = nullptr;
+    m->d_bptr = m->d_bcol = nullptr;
     m->d_val = nullptr;
     m->d_val32 = nullptr;
     m->has_structure = false;
+    m->nodeblocked = false;
     return 0;
+}
+
+// Node-blocked pattern test: n = 3 nbr; rows 3I..3I+2 share one column list
+// made of whole triples.  Fills bptr (node-row block offsets) and bcol.
+static bool node_blocked(int n, const std::vector<int32_t> &rp, const std::vector<int32_t> &ci,
+                         std::vector<int32_t> &bptr, std::vector<int32_t> &bcol)
+{
+    if (n == 0 || n % 3) return false;
+    const int nbr = n / 3;
+    bptr.assign((size_t)nbr + 1, 0);
+    for (int I = 0; I < nbr; I++) {
+        const int r0 = 3 * I, len = rp[r0 + 1] - rp[r0];
+        if (len % 3) return false;
+        for (int r = 1; r < 3; r++) {
+            if (rp[r0 + r + 1] - rp[r0 + r] != len) return false;
+            if (!std::equal(ci.begin() + rp[r0], ci.begin() + rp[r0 + 1], ci.begin() + rp[r0 + r])) return false;
+        }
+        for (int t = 0; t < len; t += 3) {
+            const int c = ci[rp[r0] + t];
+            if (c % 3 || ci[rp[r0] + t + 1] != c + 1 || ci[rp[r0] + t + 2] != c + 2) return false;
+        }
+        bptr[I + 1] = bptr[I] + len / 3;
+    }
+    bcol.resize((size_t)bptr[nbr]);
+    for (int I = 0; I < nbr; I++)
+        for (int k = 0; k < bptr[I + 1] - bptr[I]; k++) bcol[bptr[I] + k] = ci[rp[3 * I] + 3 * k] / 3;
+    return true;
 }
 
 int csr_alloc(hipcsr_mtx *m, int n, const std::vector<int32_t> &rowptr, const std::vector<int32_t> &colind)
@@ -174,7 +258,27 @@ int csr_alloc(hipcsr_mtx *m, int n, const std::vector<int32_t> &rowptr, const st
     m->h_rowptr = rowptr;
     m->h_colind = colind;
     m->avg_row = n ? (double)m->nnz / n : 0.0;
+    for (int k = 0; k < 4; k++) {
+        const int rpb = 8 << k;
+        int mx = 0;
+        for (int r0 = 0; r0 < n; r0 += rpb) mx = std::max(mx, rowptr[std::min(r0 + rpb, n)] - rowptr[r0]);
+        m->stream_max[k] = mx;
+    }
+    m->shape = default_shape(m->avg_row);
+    std::vector<int32_t> bptr, bcol;
+    m->nodeblocked = node_blocked(n, rowptr, colind, bptr, bcol);
+    m->nbr = m->nodeblocked ? n / 3 : 0;
+    m->shape.nodeblock = m->nodeblocked ? 1 : 0;
     HIPFEM_CHECK(hipSetDevice(m->ctx->device));
+    if (m->nodeblocked) {
+        HIPFEM_CHECK(hipMalloc(&m->d_bptr, sizeof(int32_t) * (m->nbr + 1)));
+        HIPFEM_CHECK(hipMalloc(&m->d_bcol, sizeof(int32_t) * std::max<size_t>(1, bcol.size())));
+        HIPFEM_CHECK(hipMemcpyAsync(m->d_bptr, bptr.data(), sizeof(int32_t) * (m->nbr + 1), hipMemcpyHostToDevice,
+                                    m->ctx->stream));
+        if (!bcol.empty())
+            HIPFEM_CHECK(hipMemcpyAsync(m->d_bcol, bcol.data(), sizeof(int32_t) * bcol.size(), hipMemcpyHostToDevice,
+                                        m->ctx->stream));
+    }
     HIPFEM_CHECK(hipMalloc(&m->d_rowptr, sizeof(int32_t) * (n + 1)));
     HIPFEM_CHECK(hipMalloc(&m->d_colind, sizeof(int32_t) * std::max<int64_t>(1, m->nnz)));
     HIPFEM_CHECK(hipMalloc(&m->d_diag, sizeof(int32_t) * std::max(1, n)));
@@ -209,9 +313,7 @@ int csr_refresh_fp32(hipcsr_mtx *m)
 
 int spmv_fp32_dev(hipcsr_mtx *m, const double *dx, double *dy)
 {
-    launch_spmv<float>(spmv_lanes(m), m->ctx->stream, m->n, m->d_rowptr, m->d_colind, m->d_val32, dx, dy);
-    HIPFEM_CHECK(hipGetLastError());
-    return 0;
+    return launch_spmv<float>(m, m->shape, m->ctx->stream, m->n, m->d_rowptr, m->d_colind, m->d_val32, dx, dy);
 }
 
 static int64_t find_pos(const hipcsr_mtx *m, int r, int c)
@@ -744,6 +846,63 @@ extern "C" int hipcsr_is_allocated(hipcsr_mtx *m, int i, int j)
 {
     if (!m || !m->has_structure || i < 1 || j < 1 || i > m->n || j > m->n) return 0;
     return find_pos(m, i - 1, j - 1) >= 0;
+}
+
+extern "C" int hipcsr_set_spmv_shape(hipcsr_mtx *m, int lanes, int unroll, int nontemporal)
+{
+    HIPFEM_REQUIRE(m, HIPFEM_EINVAL, "null matrix");
+    SpmvShape s;
+    s.lanes = lanes;
+    s.unroll = unroll;
+    s.nt = nontemporal ? 1 : 0;
+    s.stream = 0;
+    const bool ok = (lanes == 8 || lanes == 16 || lanes == 32) && (unroll == 1 || unroll == 2 || unroll == 4 || unroll == 6);
+    HIPFEM_REQUIRE(ok, HIPFEM_EINVAL, "unsupported SpMV shape (%d lanes, unroll %d)", lanes, unroll);
+    m->shape = s;
+    return 0;
+}
+
+extern "C" int hipcsr_set_spmv_stream(hipcsr_mtx *m, int rows_per_block, int unroll)
+{
+    HIPFEM_REQUIRE(m && m->has_structure, HIPFEM_EINVAL, "set_spmv_stream: no structure");
+    if (rows_per_block == 0) {
+        m->shape.stream = 0;
+        return 0;
+    }
+    int k = -1;
+    for (int i = 0; i < 4; i++)
+        if ((8 << i) == rows_per_block) k = i;
+    HIPFEM_REQUIRE(k >= 0 && (unroll == 2 || unroll == 4 || unroll == 6), HIPFEM_EINVAL,
+                   "unsupported streaming shape (%d rows per block, unroll %d)", rows_per_block, unroll);
+    HIPFEM_REQUIRE(m->stream_max[k] <= STREAM_CAP, HIPFEM_EINVAL,
+                   "streaming SpMV: %d rows per block hold up to %d entries (> %d)", rows_per_block, m->stream_max[k],
+                   STREAM_CAP);
+    m->shape.stream = 1;
+    m->shape.nodeblock = 0;
+    m->shape.rpb = rows_per_block;
+    m->shape.unroll = unroll;
+    return 0;
+}
+
+extern "C" int hipcsr_set_spmv_nodeblock(hipcsr_mtx *m, int on)
+{
+    HIPFEM_REQUIRE(m && m->has_structure, HIPFEM_EINVAL, "set_spmv_nodeblock: no structure");
+    HIPFEM_REQUIRE(!on || m->nodeblocked, HIPFEM_EINVAL,
+                   "node-block SpMV needs every free node fully free and numbered consecutively");
+    m->shape.nodeblock = on ? 1 : 0;
+    if (on) m->shape.stream = 0;
+    return 0;
+}
+
+extern "C" int hipcsr_is_nodeblocked(const hipcsr_mtx *m) { return m && m->nodeblocked ? 1 : 0; }
+
+extern "C" int hipcsr_get_spmv_shape(const hipcsr_mtx *m, int *lanes, int *unroll, int *nontemporal)
+{
+    HIPFEM_REQUIRE(m, HIPFEM_EINVAL, "null matrix");
+    if (lanes) *lanes = m->shape.lanes;
+    if (unroll) *unroll = m->shape.unroll;
+    if (nontemporal) *nontemporal = m->shape.nt;
+    return 0;
 }
 
 extern "C" int hipcsr_nrows(const hipcsr_mtx *m) { return m ? m->n : 0; }

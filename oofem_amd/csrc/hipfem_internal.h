@@ -8,6 +8,61 @@
 #include <vector>
 
 #include "../../include/hipfem.h"
+#include "hipfem_spmv.h"
+
+#include <type_traits>
+
+namespace hipfem {
+// Calls f(lanes, unroll, nt) with compile-time constants for the supported
+// SpMV shapes; returns false for an unsupported shape.
+template <typename F>
+bool dispatch_shape(const SpmvShape &s, F &&f)
+{
+    auto with_nt = [&](auto L, auto U) {
+        if (s.nt)
+            f(L, U, std::true_type{});
+        else
+            f(L, U, std::false_type{});
+        return true;
+    };
+    auto with_u = [&](auto L) {
+        switch (s.unroll) {
+        case 1: return with_nt(L, std::integral_constant<int, 1>{});
+        case 2: return with_nt(L, std::integral_constant<int, 2>{});
+        case 4: return with_nt(L, std::integral_constant<int, 4>{});
+        case 6: return with_nt(L, std::integral_constant<int, 6>{});
+        }
+        return false;
+    };
+    switch (s.lanes) {
+    case 8: return with_u(std::integral_constant<int, 8>{});
+    case 16: return with_u(std::integral_constant<int, 16>{});
+    case 32: return with_u(std::integral_constant<int, 32>{});
+    }
+    return false;
+}
+
+// Same for the streaming form: f(rows_per_block, unroll).
+template <typename F>
+bool dispatch_stream(const SpmvShape &s, F &&f)
+{
+    auto with_u = [&](auto R) {
+        switch (s.unroll) {
+        case 2: f(R, std::integral_constant<int, 2>{}); return true;
+        case 4: f(R, std::integral_constant<int, 4>{}); return true;
+        case 6: f(R, std::integral_constant<int, 6>{}); return true;
+        }
+        return false;
+    };
+    switch (s.rpb) {
+    case 8: return with_u(std::integral_constant<int, 8>{});
+    case 16: return with_u(std::integral_constant<int, 16>{});
+    case 32: return with_u(std::integral_constant<int, 32>{});
+    case 64: return with_u(std::integral_constant<int, 64>{});
+    }
+    return false;
+}
+}  // namespace hipfem
 
 namespace hipfem {
 
@@ -86,6 +141,12 @@ struct hipcsr_mtx {
     // transpose pattern for timesT, built on first use
     int32_t *d_tptr = nullptr, *d_tcol = nullptr, *d_tmap = nullptr;
     double avg_row = 0.0;
+    hipfem::SpmvShape shape;                   // SpMV kernel shape (hipcsr_set_spmv_shape)
+    int stream_max[4] = {0, 0, 0, 0};          // largest block range for 8/16/32/64 rows per block
+    // node-block view of the same CSR arrays (valid when the pattern is node-blocked)
+    bool nodeblocked = false;
+    int nbr = 0;                               // node rows = n / 3
+    int32_t *d_bptr = nullptr, *d_bcol = nullptr;
 };
 
 namespace hipfem {

@@ -64,31 +64,33 @@ __device__ __forceinline__ double block_sum(double v, double *sh)
     return s;
 }
 
-template <int LANES, typename VT, typename MT, bool FIRST>
-__global__ __launch_bounds__(256) void k_pcg_spmv(int n, const int32_t *__restrict__ rowptr,
-                                                  const int32_t *__restrict__ col, const MT *__restrict__ val,
-                                                  const VT *__restrict__ z, const VT *__restrict__ pold,
-                                                  VT *__restrict__ pnew, VT *__restrict__ q,
-                                                  const PcgState *__restrict__ st, double *__restrict__ partials)
+// fused form: p_new = z + beta p_old is formed inside the gather (two
+// gathers per nonzero) and written by the row owner; q = A p_new
+template <int LANES, int U, bool NT, typename VT, typename MT>
+__global__ __launch_bounds__(256) void k_pcg_spmv_fused(int n, const int32_t *__restrict__ rowptr,
+                                                        const int32_t *__restrict__ col, const MT *__restrict__ val,
+                                                        const VT *__restrict__ z, const VT *__restrict__ pold,
+                                                        VT *__restrict__ pnew, VT *__restrict__ q,
+                                                        const PcgState *__restrict__ st, double *__restrict__ partials,
+                                                        int first)
 {
     __shared__ double sh[4];
     if (st->done) return;
-    const VT beta = FIRST ? VT(0) : (VT)st->beta;
+    const VT beta = first ? VT(0) : (VT)st->beta;
     const int rows_per_block = 256 / LANES;
     const int lane = threadIdx.x % LANES;
     double acc = 0.0;
     for (int row = blockIdx.x * rows_per_block + threadIdx.x / LANES; row < n; row += gridDim.x * rows_per_block) {
         const int s = rowptr[row], e = rowptr[row + 1];
-        VT sum = VT(0);
-        for (int k = s + lane; k < e; k += LANES) {
-            const int c = col[k];
-            const VT pc = FIRST ? z[c] : fma(beta, pold[c], z[c]);
-            sum += (VT)val[k] * pc;
-        }
-#pragma unroll
-        for (int off = LANES / 2; off > 0; off >>= 1) sum += __shfl_xor(sum, off, LANES);
+        VT sum;
+        if (first)
+            sum = row_partial<LANES, U, NT, VT>(s, e, lane, col, val, [&](int c) { return z[c]; });
+        else
+            sum = row_partial<LANES, U, NT, VT>(s, e, lane, col, val,
+                                                [&](int c) { return fma(beta, pold[c], z[c]); });
+        sum = group_sum<LANES>(sum);
         if (lane == 0) {
-            const VT pi = FIRST ? z[row] : fma(beta, pold[row], z[row]);
+            const VT pi = first ? z[row] : fma(beta, pold[row], z[row]);
             pnew[row] = pi;
             q[row] = sum;
             acc += (double)pi * (double)sum;
@@ -96,6 +98,101 @@ __global__ __launch_bounds__(256) void k_pcg_spmv(int n, const int32_t *__restri
     }
     const double t = block_sum<256>(acc, sh);
     if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+// separate form: p was updated in place by k_pcg_pupdate; one gather per nonzero
+template <int LANES, int U, bool NT, typename VT, typename MT>
+__global__ __launch_bounds__(256) void k_pcg_spmv_p(int n, const int32_t *__restrict__ rowptr,
+                                                    const int32_t *__restrict__ col, const MT *__restrict__ val,
+                                                    const VT *__restrict__ p, VT *__restrict__ q,
+                                                    const PcgState *__restrict__ st, double *__restrict__ partials)
+{
+    __shared__ double sh[4];
+    if (st->done) return;
+    const int rows_per_block = 256 / LANES;
+    const int lane = threadIdx.x % LANES;
+    double acc = 0.0;
+    for (int row = blockIdx.x * rows_per_block + threadIdx.x / LANES; row < n; row += gridDim.x * rows_per_block) {
+        const int s = rowptr[row], e = rowptr[row + 1];
+        VT sum = row_partial<LANES, U, NT, VT>(s, e, lane, col, val, [&](int c) { return p[c]; });
+        sum = group_sum<LANES>(sum);
+        if (lane == 0) {
+            q[row] = sum;
+            acc += (double)p[row] * (double)sum;
+        }
+    }
+    const double t = block_sum<256>(acc, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+// streaming form of k_pcg_spmv_p (block-contiguous value/column streams)
+template <int RPB, int U, typename VT, typename MT>
+__global__ __launch_bounds__(256) void k_pcg_spmv_p_stream(int n, const int32_t *__restrict__ rowptr,
+                                                           const int32_t *__restrict__ col, const MT *__restrict__ val,
+                                                           const VT *__restrict__ p, VT *__restrict__ q,
+                                                           const PcgState *__restrict__ st, double *__restrict__ partials)
+{
+    __shared__ VT prod[STREAM_CAP];
+    __shared__ int rp[RPB + 1];
+    __shared__ double sh[4];
+    if (st->done) return;
+    constexpr int LPR = 256 / RPB;
+    double acc = 0.0;
+    for (int r0 = blockIdx.x * RPB; r0 < n; r0 += gridDim.x * RPB) {
+        const VT sum = stream_block_row<RPB, U, VT>(r0, n, rowptr, col, val, [&](int c) { return p[c]; }, prod, rp);
+        const int row = r0 + threadIdx.x / LPR;
+        if (threadIdx.x % LPR == 0 && row < n) {
+            q[row] = sum;
+            acc += (double)p[row] * (double)sum;
+        }
+    }
+    const double t = block_sum<256>(acc, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+// node-block form of k_pcg_spmv_p (bcol per 3x3 block, one gather per block)
+template <typename VT, typename MT>
+__global__ __launch_bounds__(256) void k_pcg_spmv_p_nb(int nbr, const int32_t *__restrict__ bptr,
+                                                       const int32_t *__restrict__ bcol, const MT *__restrict__ val,
+                                                       const VT *__restrict__ p, VT *__restrict__ q,
+                                                       const PcgState *__restrict__ st, double *__restrict__ partials)
+{
+    __shared__ double sh[4];
+    if (st->done) return;
+    const int lane = threadIdx.x % 32;
+    double acc = 0.0;
+    for (int I = blockIdx.x * 8 + threadIdx.x / 32; I < nbr; I += gridDim.x * 8) {
+        const int bo = bptr[I], nbk = bptr[I + 1] - bo;
+        VT a[3];
+        nb_row<VT>(bo, nbk, 9 * bo, lane, bcol, val,
+                   [&](int bc, VT &x0, VT &x1, VT &x2) {
+                       const VT *pp = p + 3 * bc;
+                       x0 = pp[0];
+                       x1 = pp[1];
+                       x2 = pp[2];
+                   },
+                   a);
+        if (lane == 0) {
+            q[3 * I] = a[0];
+            q[3 * I + 1] = a[1];
+            q[3 * I + 2] = a[2];
+            acc += (double)p[3 * I] * (double)a[0] + (double)p[3 * I + 1] * (double)a[1] +
+                   (double)p[3 * I + 2] * (double)a[2];
+        }
+    }
+    const double t = block_sum<256>(acc, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+// p = z (first iteration) or p = z + beta p, in place
+template <typename VT>
+__global__ __launch_bounds__(256) void k_pcg_pupdate(int n, const VT *__restrict__ z, VT *__restrict__ p,
+                                                     const PcgState *__restrict__ st, int first)
+{
+    if (st->done) return;
+    const VT beta = first ? VT(0) : (VT)st->beta;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
+        p[i] = first ? z[i] : fma(beta, p[i], z[i]);
 }
 
 template <typename XT, typename VT>
@@ -277,17 +374,31 @@ __global__ __launch_bounds__(RED_THREADS) void k_reduce2(const double *__restric
 static int vec_grid(int n) { return std::max(1, std::min((n + 255) / 256, MAX_GRID)); }
 static int spmv_grid(int n, int lanes) { return std::max(1, std::min((n + 256 / lanes - 1) / (256 / lanes), MAX_GRID)); }
 
-template <typename VT, typename MT, bool FIRST>
-static void launch_pcg_spmv(int lanes, int grid, hipStream_t s, int n, const int32_t *rp, const int32_t *ci, const MT *v,
-                            const VT *z, const VT *po, VT *pn, VT *q, const PcgState *st, double *part)
+template <typename VT, typename MT>
+static void launch_pcg_spmv_fused(const SpmvShape &sh, int grid, hipStream_t s, int n, const int32_t *rp,
+                                  const int32_t *ci, const MT *v, const VT *z, const VT *po, VT *pn, VT *q,
+                                  const PcgState *st, double *part, int first)
 {
-    switch (lanes) {
-    case 4: k_pcg_spmv<4, VT, MT, FIRST><<<grid, 256, 0, s>>>(n, rp, ci, v, z, po, pn, q, st, part); break;
-    case 8: k_pcg_spmv<8, VT, MT, FIRST><<<grid, 256, 0, s>>>(n, rp, ci, v, z, po, pn, q, st, part); break;
-    case 16: k_pcg_spmv<16, VT, MT, FIRST><<<grid, 256, 0, s>>>(n, rp, ci, v, z, po, pn, q, st, part); break;
-    case 32: k_pcg_spmv<32, VT, MT, FIRST><<<grid, 256, 0, s>>>(n, rp, ci, v, z, po, pn, q, st, part); break;
-    default: k_pcg_spmv<64, VT, MT, FIRST><<<grid, 256, 0, s>>>(n, rp, ci, v, z, po, pn, q, st, part); break;
-    }
+    dispatch_shape(sh, [&](auto L, auto U, auto NT) {
+        k_pcg_spmv_fused<decltype(L)::value, decltype(U)::value, decltype(NT)::value, VT, MT>
+            <<<grid, 256, 0, s>>>(n, rp, ci, v, z, po, pn, q, st, part, first);
+    });
+}
+
+template <typename VT, typename MT>
+static void launch_pcg_spmv_p(const SpmvShape &sh, int grid, hipStream_t s, int n, const int32_t *rp,
+                              const int32_t *ci, const MT *v, const VT *p, VT *q, const PcgState *st, double *part)
+{
+    if (sh.stream)
+        dispatch_stream(sh, [&](auto R, auto U) {
+            k_pcg_spmv_p_stream<decltype(R)::value, decltype(U)::value, VT, MT>
+                <<<grid, 256, 0, s>>>(n, rp, ci, v, p, q, st, part);
+        });
+    else
+        dispatch_shape(sh, [&](auto L, auto U, auto NT) {
+            k_pcg_spmv_p<decltype(L)::value, decltype(U)::value, decltype(NT)::value, VT, MT>
+                <<<grid, 256, 0, s>>>(n, rp, ci, v, p, q, st, part);
+        });
 }
 
 }  // namespace hipfem
@@ -304,6 +415,7 @@ struct hippcg_solver {
     int max_outer = 100;
     int last_outer = 0;
     int check_every = 16;
+    int fused = 0;   // 1: p formed in the SpMV gather; 0: separate in-place p update (faster on gfx950)
     // preconditioner cache (IMLSolver re-inits on a new matrix or version, imlsolver.C:112-114)
     const hipcsr_mtx *lhs = nullptr;
     long lhs_version = -1;
@@ -403,8 +515,13 @@ static int pcg_run(hippcg_solver *s, hipcsr_mtx *A, const MT *val, const VT *b, 
 {
     hipStream_t st = s->ctx->stream;
     const int n = A->n;
-    const int lanes = spmv_lanes(A);
-    const int gv = vec_grid(n), gs = spmv_grid(n, lanes);
+    const SpmvShape sh = A->shape;
+    const int gv = vec_grid(n);
+    const bool nodeblock = sh.nodeblock && A->nodeblocked;
+    const int gs = nodeblock    ? std::max(1, std::min((A->nbr + 7) / 8, MAX_GRID))
+                   : sh.stream ? std::max(1, std::min((n + sh.rpb - 1) / sh.rpb, MAX_GRID))
+                               : spmv_grid(n, sh.lanes);
+    const bool fused = s->fused && !sh.stream && !nodeblock;  // these forms gather a ready p
     int rc = ensure_partials(s->ctx, 3 * (size_t)std::max(gv, gs));
     if (rc) return rc;
     double *part = s->ctx->d_partials;
@@ -441,16 +558,21 @@ static int pcg_run(hippcg_solver *s, hipcsr_mtx *A, const MT *val, const VT *b, 
             if (rc) return rc;
         }
         for (; it <= stop; it++) {
+            const int first = (it == 1);
+            if (!fused) k_pcg_pupdate<VT><<<gv, 256, 0, st>>>(n, z, p0, s->st, first);
             if (prof) HIPFEM_CHECK(hipEventRecord(s->ev[2 * (it - 1)], st));
-            if (it == 1)
-                launch_pcg_spmv<VT, MT, true>(lanes, gs, st, n, A->d_rowptr, A->d_colind, val, z, pold, pnew, q, s->st, part);
+            if (nodeblock)
+                k_pcg_spmv_p_nb<VT, MT><<<gs, 256, 0, st>>>(A->nbr, A->d_bptr, A->d_bcol, val, p0, q, s->st, part);
+            else if (fused)
+                launch_pcg_spmv_fused<VT, MT>(sh, gs, st, n, A->d_rowptr, A->d_colind, val, z, pold, pnew, q, s->st,
+                                              part, first);
             else
-                launch_pcg_spmv<VT, MT, false>(lanes, gs, st, n, A->d_rowptr, A->d_colind, val, z, pold, pnew, q, s->st, part);
+                launch_pcg_spmv_p<VT, MT>(sh, gs, st, n, A->d_rowptr, A->d_colind, val, p0, q, s->st, part);
             if (prof) HIPFEM_CHECK(hipEventRecord(s->ev[2 * (it - 1) + 1], st));
             k_reduce_alpha<<<1, RED_THREADS, 0, st>>>(s->st, part, gs);
-            k_pcg_update<XT, VT><<<gv, 256, 0, st>>>(n, x, r, z, pnew, q, dinv, s->st, part);
+            k_pcg_update<XT, VT><<<gv, 256, 0, st>>>(n, x, r, z, fused ? pnew : p0, q, dinv, s->st, part);
             k_reduce_beta<<<1, RED_THREADS, 0, st>>>(s->st, part, gv, it);
-            std::swap(pold, pnew);
+            if (fused) std::swap(pold, pnew);
         }
         HIPFEM_CHECK(hipGetLastError());
         rc = read_state(s, &h);
@@ -601,6 +723,13 @@ extern "C" int hippcg_set_precision(hippcg_solver *s, int precision, double inne
 }
 
 extern "C" int hippcg_last_outer(hippcg_solver *s) { return s ? s->last_outer : 0; }
+
+extern "C" int hippcg_set_fused(hippcg_solver *s, int fused)
+{
+    HIPFEM_REQUIRE(s, HIPFEM_EINVAL, "null solver");
+    s->fused = fused ? 1 : 0;
+    return 0;
+}
 
 extern "C" int hippcg_set_profiling(hippcg_solver *s, int on)
 {

@@ -1,0 +1,147 @@
+// Row kernel shared by the plain SpMV and the PCG's fused SpMV.
+//
+// A group of LANES consecutive lanes owns one CSR row.  Each pass over the
+// row issues U independent (column, value) load pairs per lane before any
+// gather, then U gathers, so a lane keeps 2U streaming loads and U gathers in
+// flight instead of one dependent chain per nonzero.  With NT the value and
+// column streams are loaded non-temporally: they are read once per SpMV and
+// should not displace the gathered vectors from L2 / MALL.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace hipfem {
+
+template <bool NT, typename T>
+__device__ __forceinline__ T ld_stream(const T *p)
+{
+    if constexpr (NT)
+        return __builtin_nontemporal_load(p);
+    else
+        return *p;
+}
+
+// sum_k val[k] * g(col[k]) over k in [s, e), partial per lane (not reduced)
+template <int LANES, int U, bool NT, typename AT, typename MT, typename G>
+__device__ __forceinline__ AT row_partial(int s, int e, int lane, const int32_t *__restrict__ col,
+                                          const MT *__restrict__ val, G gather)
+{
+    AT sum = AT(0);
+    for (int base = s + lane; base < e; base += U * LANES) {
+        int c[U];
+        MT v[U];
+#pragma unroll
+        for (int j = 0; j < U; j++) {
+            const int k = base + j * LANES;
+            if (k < e) {
+                c[j] = ld_stream<NT>(col + k);
+                v[j] = ld_stream<NT>(val + k);
+            } else {
+                c[j] = -1;
+                v[j] = MT(0);
+            }
+        }
+        AT g[U];
+#pragma unroll
+        for (int j = 0; j < U; j++) g[j] = (c[j] >= 0) ? gather(c[j]) : AT(0);
+#pragma unroll
+        for (int j = 0; j < U; j++) sum += (AT)v[j] * g[j];
+    }
+    return sum;
+}
+
+template <int LANES, typename T>
+__device__ __forceinline__ T group_sum(T v)
+{
+#pragma unroll
+    for (int off = LANES / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, LANES);
+    return v;
+}
+
+// kernel shape of an SpMV launch; a property of the matrix (tunable)
+struct SpmvShape {
+    int lanes = 16;   // 8, 16, 32
+    int unroll = 4;   // 1, 2, 4, 6
+    int nt = 0;       // non-temporal streams
+    int stream = 0;   // 1: block-contiguous streaming form (rows_per_block below)
+    int rpb = 16;     // rows per block of the streaming form
+    int nodeblock = 0;  // 1: node-block form (needs a node-blocked pattern)
+};
+
+// Node-block form.  When every row triple (3I, 3I+1, 3I+2) shares one column
+// list made of whole triples (3J, 3J+1, 3J+2) -- all free nodes fully free
+// and numbered consecutively, as in the reference's node-order numbering --
+// the CSR value array of node row I is a sequence of 3x3 blocks laid out
+// [r][k][c]: val[rowptr[3I] + r*3*nbk + 3*k + c] with nbk = number of block
+// columns.  The kernel then reads one block column index per 9 values
+// (bcol) instead of one column per value, and gathers x once per block.
+// A 32-lane half wave owns one node row; lane k owns block column k.
+template <typename AT, typename MT, typename G>
+__device__ __forceinline__ void nb_row(int bo, int nbk, int vbase, int lane, const int32_t *__restrict__ bcol,
+                                       const MT *__restrict__ val, G gather3, AT acc[3])
+{
+    acc[0] = acc[1] = acc[2] = AT(0);
+    for (int k = lane; k < nbk; k += 32) {
+        const int bc = bcol[bo + k];
+        AT x0, x1, x2;
+        gather3(bc, x0, x1, x2);
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            const MT *vp = val + vbase + r * 3 * nbk + 3 * k;
+            acc[r] += (AT)vp[0] * x0 + (AT)vp[1] * x1 + (AT)vp[2] * x2;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 3; r++) acc[r] = group_sum<32>(acc[r]);
+}
+
+// Streaming form: a 256-thread block owns rows [r0, r0 + RPB).  Their values
+// and columns are one contiguous range of the CSR arrays, which the block
+// reads with unit stride across all 256 lanes (U loads in flight per lane),
+// multiplying each value by its gathered vector entry into LDS.  A second
+// phase sums each row's products with 256/RPB lanes per row.  The host
+// guarantees that no block's range exceeds STREAM_CAP entries.
+constexpr int STREAM_CAP = 4096;
+
+template <int RPB, int U, typename AT, typename MT, typename G>
+__device__ __forceinline__ AT stream_block_row(int r0, int n, const int32_t *__restrict__ rowptr,
+                                               const int32_t *__restrict__ col, const MT *__restrict__ val,
+                                               G gather, AT *prod, int *rp)
+{
+    constexpr int LPR = 256 / RPB;  // lanes per row in the reduction phase
+    const int t = threadIdx.x;
+    const int rend = min(r0 + RPB, n);
+    const int s0 = rowptr[r0];
+    const int len = rowptr[rend] - s0;
+    if (t <= RPB) rp[t] = rowptr[min(r0 + t, rend)] - s0;
+    for (int base = t; base < len; base += U * 256) {
+        int c[U];
+        MT v[U];
+#pragma unroll
+        for (int j = 0; j < U; j++) {
+            const int i = base + j * 256;
+            if (i < len) {
+                c[j] = col[s0 + i];
+                v[j] = val[s0 + i];
+            } else {
+                c[j] = -1;
+                v[j] = MT(0);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < U; j++)
+            if (c[j] >= 0) prod[base + j * 256] = (AT)v[j] * gather(c[j]);
+    }
+    __syncthreads();
+    const int r = t / LPR, lane = t % LPR;
+    AT sum = AT(0);
+    if (r0 + r < rend)
+        for (int i = rp[r] + lane; i < rp[r + 1]; i += LPR) sum += prod[i];
+    sum = group_sum<LPR>(sum);
+    __syncthreads();  // prod / rp are reused by the next row block
+    return sum;       // valid in lane 0 of each row group; row = r0 + t / LPR
+}
+
+}  // namespace hipfem

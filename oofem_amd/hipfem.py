@@ -285,6 +285,23 @@ class HipCSRSparseMtrx:
         _check(lib().hipcsr_clone(self.h, C.byref(out.h)), "clone")
         return out
 
+    def set_spmv_shape(self, lanes, unroll, nontemporal=True):
+        _check(lib().hipcsr_set_spmv_shape(self.h, int(lanes), int(unroll), int(bool(nontemporal))), "set_spmv_shape")
+
+    def set_spmv_stream(self, rows_per_block, unroll=4):
+        _check(lib().hipcsr_set_spmv_stream(self.h, int(rows_per_block), int(unroll)), "set_spmv_stream")
+
+    def set_spmv_nodeblock(self, on=True):
+        _check(lib().hipcsr_set_spmv_nodeblock(self.h, int(bool(on))), "set_spmv_nodeblock")
+
+    def is_nodeblocked(self):
+        return bool(lib().hipcsr_is_nodeblocked(self.h))
+
+    def spmv_shape(self):
+        a, b, c = C.c_int(0), C.c_int(0), C.c_int(0)
+        _check(lib().hipcsr_get_spmv_shape(self.h, C.byref(a), C.byref(b), C.byref(c)), "get_spmv_shape")
+        return a.value, b.value, c.value
+
     def giveClassName(self):
         return "HipCSRSparseMtrx"
 
@@ -332,6 +349,9 @@ class HipPCGSolver:
 
     def last_outer(self):
         return lib().hippcg_last_outer(self.h)
+
+    def set_fused(self, fused=True):
+        _check(lib().hippcg_set_fused(self.h, int(bool(fused))), "set_fused")
 
     def set_profiling(self, on=True):
         _check(lib().hippcg_set_profiling(self.h, int(bool(on))), "set_profiling")

@@ -65,6 +65,57 @@ def test_spmv_matches_oracle(ctx, name):
     assert np.abs(yt - yo).max() <= 1e-14 * np.abs(val).max() * np.abs(x).max() * 100
 
 
+SHAPES = [("lanes", 8, 6, 0), ("lanes", 16, 4, 1), ("lanes", 32, 2, 0), ("lanes", 16, 1, 0),
+          ("stream", 8, 2, None), ("stream", 16, 4, None), ("stream", 32, 6, None), ("stream", 64, 4, None),
+          ("nodeblock", 0, 0, None)]
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=[f"{s[0]}-{s[1]}-{s[2]}" for s in SHAPES])
+@pytest.mark.parametrize("name", ["lspace", "qspace"])
+def test_spmv_shapes_match_oracle(ctx, name, shape):
+    """Every SpMV kernel shape (plain and inside PCG) against the oracle."""
+    from oofem_amd.hipfem import HipCSRSparseMtrx, HipFemError, HipPCGSolver
+    d = DECKS[name]
+    rp, ci, val = oracle.csr(d)
+    K = HipCSRSparseMtrx(ctx)
+    K.buildInternalStructureCSR(rp, ci)
+    K.upload_values(val)
+    kind, a, b, c = shape
+    if kind == "lanes":
+        K.set_spmv_shape(a, b, c)
+    elif kind == "nodeblock":
+        # the QSpace beam prescribes single dofs (w only) on its loaded end:
+        # not node-blocked, so the form must be refused
+        if name == "qspace":
+            assert not K.is_nodeblocked()
+            with pytest.raises(HipFemError):
+                K.set_spmv_nodeblock(True)
+            return
+        assert K.is_nodeblocked()
+        K.set_spmv_nodeblock(True)
+    else:
+        max_block = max(rp[min(r + a, d.neq)] - rp[r] for r in range(0, d.neq, a))
+        if max_block > 4096:
+            with pytest.raises(HipFemError):
+                K.set_spmv_stream(a, b)
+            return
+        K.set_spmv_stream(a, b)
+    x = np.random.default_rng(2).standard_normal(d.neq)
+    y = K.times(x)
+    yo = oracle.spmv(rp, ci, val, x)
+    assert np.abs(y - yo).max() <= 1e-12 * np.abs(val).max() * np.abs(x).max()
+    b_ = np.random.default_rng(3).standard_normal(d.neq)
+    for fused in (0, 1):
+        s = HipPCGSolver(ctx, lstol=1e-12, lsiter=20000, lsprecond=1)
+        s.set_fused(fused)
+        u = np.zeros(d.neq)
+        reason, it, res = s.solve(K, b_, u)
+        uo, ito, reso, rco = oracle.pcg(rp, ci, val, b_, tol=1e-12)
+        # summation order differs from the oracle's: near 1e-12 the count may move a few percent
+        assert reason == 1 and abs(it - ito) <= max(2, 0.05 * ito)
+        assert np.linalg.norm(u - uo) <= 1e-9 * np.linalg.norm(uo)
+
+
 def test_config0_linear_static_parity(ctx):
     from oofem_amd.drivers import linear_static, reaction_sums
     d = meshgen.config0()

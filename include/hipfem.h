@@ -203,6 +203,10 @@ int hfem_model_internal_forces(hfem_model *md, const double *u_free, const doubl
                                double *f_free, double *f_presc, double *ebe3, int update_state);
 int hfem_model_internal_forces_dev(hfem_model *md, const double *du_free, const double *du_presc,
                                    double *df_free, double *df_presc, double *ebe3, int update_state);
+/* Newton residual on device vectors: rhs = RT - F (RT may be NULL = 0) and
+ * sums6 = {sum rhs^2 over u, v, w equations, sum RT^2 over u, v, w}, the
+ * per-dof-id quantities of NRSolver::checkConvergence (nrsolver.C:648-660). */
+int hfem_model_residual(hfem_model *md, const double *dRT, const double *dF, double *drhs, double *sums6);
 /* MisesMatStatus::updateYourself (misesmat.C:720-728) for every Gauss point */
 int hfem_model_commit(hfem_model *md);
 /* reset every Gauss point to the virgin state */

@@ -1,0 +1,57 @@
+/*
+ * hipfem_host.h -- C entry points of the C++ host layer (libhipfem_host.so).
+ *
+ * The host layer holds the reference-shaped C++ classes (HipCSRSparseMtrx,
+ * HipPCGSolver, the class factory and a StructuralProblem with the
+ * LinearStatic and StaticStructural + NRSolver drivers).  These functions
+ * expose one problem to callers that are not C++ (Python tests and the
+ * benchmark).  They have no single counterpart in the reference; each names
+ * the driver it runs.  Return 0 on success, -1 on failure with the message in
+ * hfem_problem_last_error().
+ */
+#ifndef HIPFEM_HOST_H
+#define HIPFEM_HOST_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hfem_problem hfem_problem;
+
+const char *hfem_problem_last_error(void);
+
+/* mesh as in hfem_model_set_mesh (include/hipfem.h) */
+int  hfem_problem_create(int nnode, const double *coords, int etype, int nelem, const int *conn, const int *emat,
+                         int nmat, const double *mat, const int *eqn, int neq, int npeq, hfem_problem **out);
+void hfem_problem_destroy(hfem_problem *p);
+
+/* smtype / lstype are the deck integers (SMT_HipCSR = 11, ST_HipPCG = 9);
+ * lstol, lsiter, lsprecond: IMLSolver keywords (imlsolver.h:48-52);
+ * mixed, irtol: hippcg precision keywords; rtolf, maxiter, manrmsteps:
+ * NRSolver keywords (nrsolver.h:54-73). */
+int hfem_problem_set_options(hfem_problem *p, int smtype, int lstype, double lstol, int lsiter, int lsprecond,
+                             int mixed, double irtol, double rtolf, int maxiter, int manrmsteps);
+/* prescribed values (npeq) and total external load (neq, npeq) of the next step */
+int hfem_problem_set_prescribed(hfem_problem *p, const double *up);
+int hfem_problem_set_load(hfem_problem *p, const double *f_free, const double *f_presc);
+
+/* LinearStatic::solveYourselfAt (linearstatic.C:177-252) */
+int hfem_problem_linear_static(hfem_problem *p, int *iterations, double *residual);
+/* StaticStructural::solveYourselfAt + NRSolver::solve (staticstructural.C:224-358,
+ * nrsolver.C:205-404): out3 = {Newton iterations, PCG iterations, ConvergedReason} */
+int hfem_problem_nonlinear_step(hfem_problem *p, int *out3, double *force_error);
+/* EngngModel::updateYourself: commit the Gauss-point state */
+int hfem_problem_update(hfem_problem *p);
+
+int hfem_problem_get_solution(hfem_problem *p, double *u);
+/* StructuralEngngModel::computeReaction (structengngmodel.C:183-201) */
+int hfem_problem_get_reactions(hfem_problem *p, double *r);
+int hfem_problem_get_state(hfem_problem *p, double *s);
+/* SparseMtrx::at / giveClassName of the factory-made stiffness matrix */
+int hfem_problem_matrix_at(hfem_problem *p, int i, int j, double *v);
+int hfem_problem_matrix_class(hfem_problem *p, char *buf, int len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

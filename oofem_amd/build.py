@@ -16,7 +16,7 @@ LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libhipfem.so")
 HOST_LIB = os.path.join(LIBDIR, "libhipfem_host.so")
 SOURCES = ["hipfem_csr.hip", "hipfem_pcg.hip", "hipfem_model.hip"]
-HOST_SOURCES = ["oofem_types.C", "hipcsr.C", "hippcg.C", "classfactory.C", "structmodel.C", "host_capi.C"]
+HOST_SOURCES = ["classfactory.C", "hipcsr.C", "hippcg.C", "structmodel.C", "host_capi.C"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",

@@ -550,6 +550,53 @@ __global__ __launch_bounds__(1024) void k_reduce3(const double *__restrict__ par
     }
 }
 
+// rhs = RT - F and the squared sums of rhs and RT per dof id (u, v, w), the
+// quantities NRSolver::checkConvergence groups by dof id (nrsolver.C:648-660)
+__global__ __launch_bounds__(256) void k_residual_groups(int neq, const int8_t *__restrict__ dofid,
+                                                         const double *__restrict__ RT, const double *__restrict__ F,
+                                                         double *__restrict__ rhs, double *__restrict__ partials,
+                                                         int np)
+{
+    __shared__ double sh[6][4];
+    double s[6] = {0, 0, 0, 0, 0, 0};
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < neq; i += gridDim.x * 256) {
+        const double rt = RT ? RT[i] : 0.0;
+        const double r = rt - F[i];
+        rhs[i] = r;
+        const int d = dofid[i];
+        s[d] += r * r;
+        s[3 + d] += rt * rt;
+    }
+    for (int k = 0; k < 6; k++) {
+        double v = s[k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if (threadIdx.x % 64 == 0) sh[k][threadIdx.x / 64] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) partials[(size_t)threadIdx.x * np + blockIdx.x] =
+        sh[threadIdx.x][0] + sh[threadIdx.x][1] + sh[threadIdx.x][2] + sh[threadIdx.x][3];
+}
+
+__global__ __launch_bounds__(1024) void k_reduce6(const double *__restrict__ partials, int np, double *out)
+{
+    __shared__ double sh[16];
+    for (int k = 0; k < 6; k++) {
+        double s = 0.0;
+        for (int i = threadIdx.x; i < np; i += 1024) s += partials[(size_t)k * np + i];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+        if (threadIdx.x % 64 == 0) sh[threadIdx.x / 64] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t = 0.0;
+            for (int i = 0; i < 16; i++) t += sh[i];
+            out[k] = t;
+        }
+        __syncthreads();
+    }
+}
+
 // MisesMatStatus::updateYourself misesmat.C:720-728 (and the stress of
 // StructuralMaterialStatus::updateYourself)
 __global__ void k_commit(int ngp_total, double *__restrict__ state)
@@ -582,16 +629,18 @@ struct hfem_model {
     int ngp_total = 0;
     // host-API staging
     double *d_uf = nullptr, *d_up = nullptr, *d_ff = nullptr, *d_fp = nullptr, *d_out3 = nullptr;
+    int8_t *d_dofid = nullptr;   // dof id (0 u, 1 v, 2 w) of every free equation
 };
 
 static int model_free(hfem_model *md)
 {
     void *ps[] = {md->d_coords, md->d_mat, md->d_state, md->d_conn, md->d_emat, md->d_eqn, md->d_bad,
-                  md->d_eoff,   md->d_colors, md->d_uf, md->d_up, md->d_ff, md->d_fp, md->d_out3};
+                  md->d_eoff,   md->d_colors, md->d_uf, md->d_up, md->d_ff, md->d_fp, md->d_out3, md->d_dofid};
     for (void *p : ps)
         if (p) HIPFEM_CHECK(hipFree(p));
     md->d_coords = md->d_mat = md->d_state = md->d_uf = md->d_up = md->d_ff = md->d_fp = md->d_out3 = nullptr;
     md->d_conn = md->d_emat = md->d_eqn = md->d_bad = md->d_colors = nullptr;
+    md->d_dofid = nullptr;
     md->d_eoff = nullptr;
     md->pattern_id = -1;
     md->ncolors = 0;
@@ -669,7 +718,7 @@ extern "C" int hfem_model_set_mesh(hfem_model *md, int nnode, const double *coor
     HIPFEM_CHECK(hipMalloc(&md->d_eqn, sizeof(int) * 3 * nnode));
     HIPFEM_CHECK(hipMalloc(&md->d_bad, sizeof(int)));
     HIPFEM_CHECK(hipMalloc(&md->d_state, sizeof(double) * NST * std::max(1, md->ngp_total)));
-    HIPFEM_CHECK(hipMalloc(&md->d_out3, sizeof(double) * 4));
+    HIPFEM_CHECK(hipMalloc(&md->d_out3, sizeof(double) * 8));   // k_reduce3 writes 3, k_reduce6 writes 6
     HIPFEM_CHECK(hipMalloc(&md->d_uf, sizeof(double) * std::max(1, neq)));
     HIPFEM_CHECK(hipMalloc(&md->d_ff, sizeof(double) * std::max(1, neq)));
     HIPFEM_CHECK(hipMalloc(&md->d_up, sizeof(double) * std::max(1, npeq)));
@@ -681,10 +730,36 @@ extern "C" int hfem_model_set_mesh(hfem_model *md, int nnode, const double *coor
     }
     HIPFEM_CHECK(hipMemcpyAsync(md->d_mat, mat, sizeof(double) * HIPFEM_MATP * nmat, hipMemcpyHostToDevice, st));
     HIPFEM_CHECK(hipMemcpyAsync(md->d_eqn, eqn, sizeof(int) * 3 * nnode, hipMemcpyHostToDevice, st));
+    std::vector<int8_t> dofid((size_t)std::max(1, neq), 0);
+    for (int64_t i = 0; i < (int64_t)nnode * 3; i++)
+        if (eqn[i] > 0) dofid[eqn[i] - 1] = (int8_t)(i % 3);
+    HIPFEM_CHECK(hipMalloc(&md->d_dofid, dofid.size()));
+    HIPFEM_CHECK(hipMemcpyAsync(md->d_dofid, dofid.data(), dofid.size(), hipMemcpyHostToDevice, st));
     const int64_t ns = (int64_t)NST * std::max(1, md->ngp_total);
     k_fill<<<(int)std::min<int64_t>((ns + 255) / 256, 8192), 256, 0, st>>>(ns, 0.0, md->d_state);
     HIPFEM_CHECK(hipGetLastError());
     HIPFEM_CHECK(hipStreamSynchronize(st));
+    return 0;
+}
+
+extern "C" int hfem_model_residual(hfem_model *md, const double *dRT, const double *dF, double *drhs, double *sums6)
+{
+    HIPFEM_REQUIRE(md && md->d_dofid && dF && drhs && sums6, HIPFEM_EINVAL, "residual: bad arguments");
+    DeviceGuard g(md->ctx->device);
+    hipStream_t st = md->ctx->stream;
+    const int np = std::max(1, std::min((md->neq + 255) / 256, 2048));
+    int rc = ensure_partials(md->ctx, 6 * (size_t)np);
+    if (rc) return rc;
+    if (md->neq) {
+        k_residual_groups<<<np, 256, 0, st>>>(md->neq, md->d_dofid, dRT, dF, drhs, md->ctx->d_partials, np);
+        k_reduce6<<<1, 1024, 0, st>>>(md->ctx->d_partials, np, md->d_out3);
+        HIPFEM_CHECK(hipGetLastError());
+        HIPFEM_CHECK(hipMemcpyAsync(md->ctx->h_pinned, md->d_out3, 6 * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPFEM_CHECK(hipStreamSynchronize(st));
+        for (int k = 0; k < 6; k++) sums6[k] = md->ctx->h_pinned[k];
+    } else {
+        for (int k = 0; k < 6; k++) sums6[k] = 0.0;
+    }
     return 0;
 }
 

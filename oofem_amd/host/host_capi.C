@@ -1,0 +1,131 @@
+// C entry points of the host layer (libhipfem_host.so), used by the Python
+// driver and tests: a StructuralProblem created from flat arrays, solved
+// through the class factory (smtype / lstype) and the mirrored drivers.
+#include <cstring>
+#include <string>
+
+#include "../../include/hipfem_host.h"
+#include "structmodel.h"
+
+using namespace oofem;
+
+namespace {
+thread_local std::string g_host_err;
+
+template <typename F>
+int guarded(F &&f)
+{
+    try {
+        f();
+        return 0;
+    } catch (const std::exception &e) {
+        g_host_err = e.what();
+        return -1;
+    }
+}
+}  // namespace
+
+extern "C" {
+
+const char *hfem_problem_last_error(void) { return g_host_err.c_str(); }
+
+int hfem_problem_create(int nnode, const double *coords, int etype, int nelem, const int *conn, const int *emat,
+                        int nmat, const double *mat, const int *eqn, int neq, int npeq, hfem_problem **out)
+{
+    return guarded([&] {
+        *out = reinterpret_cast<hfem_problem *>(
+            new StructuralProblem(nnode, coords, etype, nelem, conn, emat, nmat, mat, eqn, neq, npeq));
+    });
+}
+
+void hfem_problem_destroy(hfem_problem *p) { delete reinterpret_cast<StructuralProblem *>(p); }
+
+// smtype / lstype: deck integers (SMT_HipCSR = 11, ST_HipPCG = 9); the
+// remaining fields are the IMLSolver / hippcg keywords and NRSolver options
+int hfem_problem_set_options(hfem_problem *p, int smtype, int lstype, double lstol, int lsiter, int lsprecond,
+                             int mixed, double irtol, double rtolf, int maxiter, int manrmsteps)
+{
+    return guarded([&] {
+        auto *P = reinterpret_cast<StructuralProblem *>(p);
+        P->setSparseMtrxType(smtype);
+        P->setLinSystSolverType(lstype);
+        InputRecord &ir = P->giveSolverRecord();
+        ir.set("lstol", lstol);
+        ir.set("lsiter", lsiter);
+        ir.set("lsprecond", lsprecond);
+        ir.set("mixed", mixed);
+        ir.set("irtol", irtol);
+        P->nr.rtolf = rtolf;
+        if (maxiter > 0) P->nr.maxiter = maxiter;
+        P->nr.manrmsteps = manrmsteps;
+    });
+}
+
+int hfem_problem_set_prescribed(hfem_problem *p, const double *up)
+{
+    return guarded([&] { reinterpret_cast<StructuralProblem *>(p)->setPrescribedValues(up); });
+}
+
+int hfem_problem_set_load(hfem_problem *p, const double *f_free, const double *f_presc)
+{
+    return guarded([&] { reinterpret_cast<StructuralProblem *>(p)->setExternalLoad(f_free, f_presc); });
+}
+
+int hfem_problem_linear_static(hfem_problem *p, int *iterations, double *residual)
+{
+    return guarded([&] { reinterpret_cast<StructuralProblem *>(p)->solveLinearStatic(*iterations, *residual); });
+}
+
+// out[0] Newton iterations, out[1] PCG iterations, out[2] status; err = final force error
+int hfem_problem_nonlinear_step(hfem_problem *p, int *out3, double *force_error)
+{
+    return guarded([&] {
+        NRStepReport r = reinterpret_cast<StructuralProblem *>(p)->solveNonlinearStep();
+        out3[0] = r.iterations;
+        out3[1] = r.linearIterations;
+        out3[2] = (int)r.status;
+        *force_error = r.forceError;
+    });
+}
+
+int hfem_problem_update(hfem_problem *p)
+{
+    return guarded([&] { reinterpret_cast<StructuralProblem *>(p)->updateYourself(); });
+}
+
+int hfem_problem_get_solution(hfem_problem *p, double *u)
+{
+    return guarded([&] { reinterpret_cast<StructuralProblem *>(p)->giveSolution(u); });
+}
+
+int hfem_problem_get_reactions(hfem_problem *p, double *r)
+{
+    return guarded([&] { reinterpret_cast<StructuralProblem *>(p)->giveReactions(r); });
+}
+
+int hfem_problem_get_state(hfem_problem *p, double *s)
+{
+    return guarded([&] { reinterpret_cast<StructuralProblem *>(p)->giveGaussPointState(s); });
+}
+
+// SparseMtrx access through the factory-made matrix (at() / times())
+int hfem_problem_matrix_at(hfem_problem *p, int i, int j, double *v)
+{
+    return guarded([&] {
+        SparseMtrx *K = reinterpret_cast<StructuralProblem *>(p)->giveStiffnessMatrix();
+        if (!K) OOFEM_ERROR("no stiffness matrix yet");
+        *v = static_cast<const SparseMtrx *>(K)->at(i, j);
+    });
+}
+
+int hfem_problem_matrix_class(hfem_problem *p, char *buf, int len)
+{
+    return guarded([&] {
+        SparseMtrx *K = reinterpret_cast<StructuralProblem *>(p)->giveStiffnessMatrix();
+        const char *n = K ? K->giveClassName() : "";
+        std::strncpy(buf, n, (size_t)len - 1);
+        buf[len - 1] = 0;
+    });
+}
+
+}  // extern "C"

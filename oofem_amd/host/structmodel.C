@@ -1,0 +1,208 @@
+#include "structmodel.h"
+
+#include <cmath>
+
+#include "classfactory.h"
+
+namespace oofem {
+
+// ------------------------------------------------------------ DeviceVector
+DeviceVector::DeviceVector(hipfem_ctx *c, int64_t len) { allocate(c, len); }
+
+DeviceVector::~DeviceVector()
+{
+    if (p) hipfem_dev_free(ctx, p);
+}
+
+void DeviceVector::allocate(hipfem_ctx *c, int64_t len)
+{
+    if (p) hipfem_dev_free(ctx, p);
+    ctx = c;
+    n = len;
+    hipfemCheck(hipfem_dev_alloc(ctx, std::max<int64_t>(1, n), &p), "dev_alloc");
+    zero();
+}
+
+void DeviceVector::upload(const double *h) { hipfemCheck(hipfem_dev_upload(ctx, p, h, n), "dev_upload"); }
+void DeviceVector::download(double *h) const { hipfemCheck(hipfem_dev_download(ctx, h, p, n), "dev_download"); }
+
+void DeviceVector::axpby(double a, const DeviceVector *x, double b)
+{
+    hipfemCheck(hipfem_dev_axpby(ctx, n, a, x ? x->p : nullptr, b, p), "dev_axpby");
+}
+
+double DeviceVector::dot(const DeviceVector &o) const
+{
+    double r = 0.0;
+    hipfemCheck(hipfem_dev_dot(ctx, n, p, o.p, &r), "dev_dot");
+    return r;
+}
+
+// ------------------------------------------------------- StructuralProblem
+StructuralProblem::StructuralProblem(int nnode_, const double *coords, int etype_, int nelem_, const int *conn_,
+                                     const int *emat, int nmat_, const double *mat, const int *eqn_, int neq_,
+                                     int npeq_) :
+    ctx(hipfemDefaultContext()), nnode(nnode_), etype(etype_), nelem(nelem_), nmat(nmat_), neq(neq_), npeq(npeq_)
+{
+    const int nen = etype == HIPFEM_LSPACE ? 8 : etype == HIPFEM_LTRSPACE ? 4 : 20;
+    conn.assign(conn_, conn_ + (size_t)nelem * nen);
+    eqn.assign(eqn_, eqn_ + (size_t)nnode * 3);
+    hipfemCheck(hfem_model_create(ctx, &ev), "hfem_model_create");
+    hipfemCheck(hfem_model_set_mesh(ev, nnode, coords, etype, nelem, conn_, emat, nmat, mat, eqn_, neq, npeq),
+                "set_mesh");
+    for (DeviceVector *v : {&X, &RT, &F, &rhs, &ddX}) v->allocate(ctx, neq);
+    for (DeviceVector *v : {&up, &RTp, &Fp}) v->allocate(ctx, npeq);
+    // IMLSolver defaults unless the caller overrides them
+    solverRecord.set("lsprecond", 1);
+}
+
+StructuralProblem::~StructuralProblem()
+{
+    linSolver.reset();
+    K.reset();
+    if (ev) hfem_model_destroy(ev);
+}
+
+void StructuralProblem::giveElementLocationArrays(int di, std::vector<int> &loc, int &ne, int &ndofe)
+{
+    const int nen = (int)(conn.size() / std::max(1, nelem));
+    ne = nelem;
+    ndofe = 3 * nen;
+    loc.assign((size_t)nelem * ndofe, 0);
+    for (int e = 0; e < nelem; e++)
+        for (int a = 0; a < nen; a++)
+            for (int d = 0; d < 3; d++) {
+                const int q = eqn[3 * (size_t)conn[(size_t)e * nen + a] + d];
+                loc[(size_t)e * ndofe + 3 * a + d] = q > 0 ? q : 0;
+            }
+}
+
+void StructuralProblem::setPrescribedValues(const double *h) { up.upload(h); }
+
+void StructuralProblem::setExternalLoad(const double *f_free, const double *f_presc)
+{
+    if (f_free) RT.upload(f_free); else RT.zero();
+    if (f_presc) RTp.upload(f_presc); else RTp.zero();
+}
+
+void StructuralProblem::ensureStructure()
+{
+    if (structureBuilt) return;
+    K = GiveClassFactory().createSparseMtrx(smtype);
+    if (!K) OOFEM_ERROR("sparse matrix creation failed (smtype %d is not registered)", (int)smtype);
+    K->buildInternalStructure(this, 1, EModelDefaultEquationNumbering());
+    linSolver = GiveClassFactory().createSparseLinSolver(lstype, nullptr, this);
+    if (!linSolver) OOFEM_ERROR("linear solver creation failed (lstype %d is not registered)", (int)lstype);
+    linSolver->initializeFrom(solverRecord);
+    structureBuilt = true;
+}
+
+void StructuralProblem::assemble(SparseMtrx &answer, int mode)
+{
+    auto *H = dynamic_cast<HipCSRSparseMtrx *>(&answer);
+    if (!H) OOFEM_ERROR("the batched evaluator assembles into a HipCSRSparseMtrx only (got %s)", answer.giveClassName());
+    hipfemCheck(hfem_model_assemble_stiffness(ev, H->handle(), mode), "assemble");
+}
+
+void StructuralProblem::internalForces(const DeviceVector &x, bool updateState, double *ebe3)
+{
+    hipfemCheck(hfem_model_internal_forces_dev(ev, x.ptr(), npeq ? up.ptr() : nullptr, F.ptr(),
+                                               npeq ? Fp.ptr() : nullptr, ebe3, updateState ? 1 : 0),
+                "internal forces");
+}
+
+ConvergedReason StructuralProblem::linearSolve(const DeviceVector &b, DeviceVector &x)
+{
+    auto *H = dynamic_cast<HipCSRSparseMtrx *>(K.get());
+    auto *S = dynamic_cast<HipPCGSolver *>(linSolver.get());
+    if (H && S) {
+        ConvergedReason r = S->solveDevice(*H, b.ptr(), x.ptr());
+        lastLinearIterations = S->giveLastIterations();
+        return r;
+    }
+    // generic SparseLinearSystemNM path on host vectors
+    FloatArray bh(neq), xh(neq);
+    b.download(bh.givePointer());
+    x.download(xh.givePointer());
+    ConvergedReason r = linSolver->solve(*K, bh, xh);
+    x.upload(xh.givePointer());
+    lastLinearIterations = 0;
+    return r;
+}
+
+// LinearStatic::solveYourselfAt linearstatic.C:177-252
+ConvergedReason StructuralProblem::solveLinearStatic(int &iterations, double &residual)
+{
+    ensureStructure();
+    assemble(*K, 0);
+    X.zero();
+    internalForces(X, false, nullptr);            // f_int of the prescribed values
+    double sums[6];
+    hipfemCheck(hfem_model_residual(ev, RT.ptr(), F.ptr(), rhs.ptr(), sums), "residual");   // rhs = f_ext - f_int
+    ConvergedReason s = linearSolve(rhs, X);
+    if (s != CR_CONVERGED) OOFEM_ERROR("No success in solving system.");
+    auto *S = dynamic_cast<HipPCGSolver *>(linSolver.get());
+    iterations = S ? S->giveLastIterations() : 0;
+    residual = S ? S->giveLastResidual() : 0.0;
+    internalForces(X, true, nullptr);             // stresses and reaction forces
+    return s;
+}
+
+// StaticStructural::solveYourselfAt staticstructural.C:224-358 with the default
+// initial guess (none: the increment starts at zero) and NRSolver::solve
+// nrsolver.C:205-404.
+NRStepReport StructuralProblem::solveNonlinearStep()
+{
+    ensureStructure();
+    NRStepReport rep;
+    // idsInUse per dof id (nrsolver.C:648-660)
+    int idsInUse[3] = {0, 0, 0};
+    for (size_t i = 0; i < eqn.size(); i++)
+        if (eqn[i] > 0) idsInUse[i % 3]++;
+    ddX.zero();
+    // stiffness before the first residual (mCalcStiffBeforeRes, nrsolver.C:245-256)
+    assemble(*K, 1);
+    for (int nite = 0;; ++nite) {
+        double ebe[3], sums[6];
+        internalForces(X, true, ebe);               // updateComponent(InternalRhs)
+        hipfemCheck(hfem_model_residual(ev, RT.ptr(), F.ptr(), rhs.ptr(), sums), "residual");
+        // checkConvergence nrsolver.C:594-885, grouped by dof id; rtold is 0 (off)
+        bool converged = true, outOfRange = false;
+        double maxErr = 0.0;
+        for (int dg = 0; dg < 3; dg++) {
+            if (!idsInUse[dg]) continue;
+            const double denom = sums[3 + dg] + ebe[dg];
+            double forceErr = denom >= 1.e-6 ? std::sqrt(sums[dg] / denom) : std::sqrt(sums[dg]);
+            if (std::isnan(forceErr) || forceErr > nr.rtolf * 1.e20) outOfRange = true;
+            if (forceErr > nr.rtolf) converged = false;
+            if (forceErr > 0.0 && nite == 0) converged = false;
+            maxErr = std::max(maxErr, forceErr);
+        }
+        rep.forceError = maxErr;
+        rep.iterations = nite;
+        if (outOfRange) { rep.status = CR_DIVERGED_TOL; break; }
+        if (converged && nite >= nr.minIterations) { rep.status = CR_CONVERGED; break; }
+        if (nite >= nr.maxiter) { rep.status = CR_DIVERGED_ITS; break; }
+        if (nite > 0 || !nr.calcStiffBeforeRes) {
+            if (nr.manrmsteps > 0 && nite % nr.manrmsteps == 0) assemble(*K, 1);
+        }
+        ConvergedReason ls = linearSolve(rhs, ddX);  // ddX carries the previous correction
+        rep.linearIterations += lastLinearIterations;
+        if (ls != CR_CONVERGED) { rep.status = CR_FAILED; break; }
+        X.axpby(1.0, &ddX, 1.0);
+    }
+    return rep;
+}
+
+void StructuralProblem::updateYourself() { hipfemCheck(hfem_model_commit(ev), "commit"); }
+
+void StructuralProblem::giveReactions(double *r)
+{
+    // StructuralEngngModel::computeReaction: f_int (prescribed numbering) - f_ext (prescribed numbering)
+    DeviceVector R(ctx, npeq);
+    R.axpby(1.0, &Fp, 0.0);
+    R.axpby(-1.0, &RTp, 1.0);
+    R.download(r);
+}
+
+}  // namespace oofem

@@ -1,0 +1,116 @@
+// StructuralProblem: a small engineering model for meshes of one element type
+// (LSpace, LTRSpace or QSpace; IsoLE / MisesMat), with the two solution
+// procedures the configurations use:
+//   solveLinearStatic()   LinearStatic::solveYourselfAt (linearstatic.C:177-252)
+//                         + StructuralEngngModel::computeReaction
+//                         (structengngmodel.C:183-201)
+//   solveNonlinearStep()  StaticStructural::solveYourselfAt
+//                         (staticstructural.C:224-358) driving NRSolver::solve
+//                         (nrsolver.C:205-404) with checkConvergence
+//                         (nrsolver.C:594-885, per-dof-id force error)
+// The sparse matrix and the linear solver come from the class factory by
+// type (smtype / lstype), so the problem runs on hipcsr / hippcg exactly as a
+// deck selecting them would.  Assembly goes through the batched evaluator
+// (the EngngModel::assemble hook, engngm.C:889-1037).
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "hippcg.h"
+
+namespace oofem {
+
+// FP64 vector resident on the problem's GPU
+class DeviceVector {
+    hipfem_ctx *ctx = nullptr;
+    double *p = nullptr;
+    int64_t n = 0;
+
+public:
+    DeviceVector() {}
+    DeviceVector(hipfem_ctx *c, int64_t len);
+    ~DeviceVector();
+    DeviceVector(const DeviceVector &) = delete;
+    DeviceVector &operator=(const DeviceVector &) = delete;
+    void allocate(hipfem_ctx *c, int64_t len);
+    double *ptr() { return p; }
+    const double *ptr() const { return p; }
+    int64_t size() const { return n; }
+    void upload(const double *h);
+    void download(double *h) const;
+    void axpby(double a, const DeviceVector *x, double b);   // this = a x + b this
+    void zero() { axpby(0.0, nullptr, 0.0); }
+    double dot(const DeviceVector &o) const;
+};
+
+struct NRSolverOptions {
+    double rtolf = 1.e-3;      // nrsolver.C:137 default
+    int maxiter = 100000000;   // nsmax default 1e8 (nrsolver.C:106)
+    int minIterations = 0;
+    int manrmsteps = 0;        // 0: modified NR (default), k: tangent every k iterations
+    bool calcStiffBeforeRes = true;
+};
+
+struct NRStepReport {
+    int iterations = 0;        // Newton iterations
+    int linearIterations = 0;  // PCG iterations summed over the step
+    double forceError = 0.0;   // final max force error over dof groups
+    ConvergedReason status = CR_UNKNOWN;
+};
+
+class StructuralProblem : public EngngModel {
+    hipfem_ctx *ctx;
+    int nnode, etype, nelem, nmat, neq, npeq;
+    std::vector<int> conn, eqn;
+    hfem_model *ev = nullptr;
+    SparseMtrxType smtype = SMT_HipCSR;
+    LinSystSolverType lstype = ST_HipPCG;
+    InputRecord solverRecord;
+    std::unique_ptr<SparseMtrx> K;
+    std::unique_ptr<SparseLinearSystemNM> linSolver;
+    // state on the device
+    DeviceVector X, up, RT, RTp, F, Fp, rhs, ddX;
+    bool structureBuilt = false;
+    int lastLinearIterations = 0;
+
+    void ensureStructure();
+    void internalForces(const DeviceVector &x, bool updateState, double *ebe3);
+    ConvergedReason linearSolve(const DeviceVector &b, DeviceVector &x);
+
+public:
+    NRSolverOptions nr;
+
+    StructuralProblem(int nnode, const double *coords, int etype, int nelem, const int *conn, const int *emat,
+                      int nmat, const double *mat, const int *eqn, int neq, int npeq);
+    ~StructuralProblem() override;
+
+    // EngngModel hooks used by the plug-ins
+    int giveNumberOfDomainEquations(int di, const UnknownNumberingScheme &s) override { return neq; }
+    void giveElementLocationArrays(int di, std::vector<int> &loc, int &ne, int &ndofe) override;
+    hfem_model *giveBatchedEvaluator(int di) override { return ev; }
+
+    void setSparseMtrxType(int t) { smtype = (SparseMtrxType)t; }
+    void setLinSystSolverType(int t) { lstype = (LinSystSolverType)t; }
+    InputRecord &giveSolverRecord() { return solverRecord; }
+
+    void setPrescribedValues(const double *up_host);               // npeq values for the step
+    void setExternalLoad(const double *f_free, const double *f_presc);  // total external load
+
+    // EngngModel::assemble for the tangent (mode 1) or elastic (mode 0) stiffness
+    void assemble(SparseMtrx &answer, int mode);
+
+    ConvergedReason solveLinearStatic(int &iterations, double &residual);
+    NRStepReport solveNonlinearStep();
+    void updateYourself();   // commit Gauss-point state (EngngModel::updateYourself)
+
+    void giveSolution(double *u) const { X.download(u); }
+    void giveReactions(double *r);
+    void giveGaussPointState(double *s) { hipfemCheck(hfem_model_get_state(ev, s), "get_state"); }
+    int giveNeq() const { return neq; }
+    int giveNpeq() const { return npeq; }
+    int giveLastLinearIterations() const { return lastLinearIterations; }
+    SparseMtrx *giveStiffnessMatrix() { return K.get(); }
+};
+
+}  // namespace oofem

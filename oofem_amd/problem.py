@@ -1,0 +1,126 @@
+"""Python front end of the C++ host layer (libhipfem_host.so).
+
+``StructuralProblem`` wraps the host engineering model: the sparse matrix
+and the linear solver are created by the class factory from deck integers
+(``smtype`` 11 = hipcsr, ``lstype`` 9 = hippcg), assembly goes through the
+batched evaluator, and the two drivers are the C++ restatements of
+LinearStatic::solveYourselfAt and StaticStructural::solveYourselfAt +
+NRSolver::solve.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import build as _build
+from .hipfem import lib as _devlib
+
+SMT_HipCSR = 11
+ST_HipPCG = 9
+CR_CONVERGED = 1
+
+_hl = None
+
+
+class ProblemError(RuntimeError):
+    pass
+
+
+def hostlib():
+    global _hl
+    if _hl is None:
+        if not os.path.exists(_build.HOST_LIB):
+            raise ProblemError(f"{_build.HOST_LIB} is missing: build it with __graft_entry__.build()")
+        _devlib()  # libhipfem.so first (the host library links it)
+        L = C.CDLL(_build.HOST_LIB)
+        L.hfem_problem_last_error.restype = C.c_char_p
+        L.hfem_problem_destroy.restype = None
+        L.hfem_problem_destroy.argtypes = [C.c_void_p]
+        L.hfem_problem_set_options.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int,
+                                               C.c_double, C.c_double, C.c_int, C.c_int]
+        _hl = L
+    return _hl
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise ProblemError(f"{what}: {hostlib().hfem_problem_last_error().decode(errors='replace')}")
+
+
+def _p(a):
+    return C.c_void_p(a.ctypes.data) if a is not None else None
+
+
+class StructuralProblem:
+    def __init__(self, deck, smtype=SMT_HipCSR, lstype=ST_HipPCG, lstol=1e-10, lsiter=100000, lsprecond=1,
+                 mixed=0, irtol=1e-4, rtolf=1e-8, maxiter=50, manrmsteps=1):
+        self.deck = d = deck
+        self._keep = [np.ascontiguousarray(x) for x in (d.coords, d.conn, d.emat, d.mat, d.eqn)]
+        c, cn, em, mt, eq = self._keep
+        self.h = C.c_void_p()
+        _check(hostlib().hfem_problem_create(d.nnode, _p(c), d.etype, d.nelem, _p(cn), _p(em), mt.shape[0], _p(mt),
+                                             _p(eq), d.neq, d.npeq, C.byref(self.h)), "create")
+        _check(hostlib().hfem_problem_set_options(self.h, smtype, lstype, lstol, lsiter, lsprecond, mixed, irtol,
+                                                  rtolf, maxiter, manrmsteps), "set_options")
+
+    def close(self):
+        if self.h:
+            hostlib().hfem_problem_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_prescribed(self, up):
+        up = np.ascontiguousarray(up, dtype=np.float64)
+        _check(hostlib().hfem_problem_set_prescribed(self.h, _p(up) if up.size else None), "set_prescribed")
+
+    def set_load(self, f_free, f_presc=None):
+        f = np.ascontiguousarray(f_free, dtype=np.float64) if f_free is not None else None
+        fp = np.ascontiguousarray(f_presc, dtype=np.float64) if f_presc is not None and len(f_presc) else None
+        _check(hostlib().hfem_problem_set_load(self.h, _p(f), _p(fp)), "set_load")
+
+    def linear_static(self):
+        it, res = C.c_int(0), C.c_double(0)
+        _check(hostlib().hfem_problem_linear_static(self.h, C.byref(it), C.byref(res)), "linear_static")
+        return it.value, res.value
+
+    def nonlinear_step(self):
+        out = (C.c_int * 3)()
+        err = C.c_double(0)
+        _check(hostlib().hfem_problem_nonlinear_step(self.h, out, C.byref(err)), "nonlinear_step")
+        return dict(newton=out[0], pcg=out[1], status=out[2], force_error=err.value)
+
+    def update(self):
+        _check(hostlib().hfem_problem_update(self.h), "update")
+
+    def solution(self):
+        u = np.zeros(self.deck.neq)
+        _check(hostlib().hfem_problem_get_solution(self.h, _p(u)), "get_solution")
+        return u
+
+    def reactions(self):
+        r = np.zeros(max(self.deck.npeq, 1))
+        _check(hostlib().hfem_problem_get_reactions(self.h, _p(r)), "get_reactions")
+        return r[: self.deck.npeq]
+
+    def state(self):
+        n = self.deck.nelem * {1: 8, 2: 1, 3: 27}[self.deck.etype]
+        s = np.zeros(n * 38)
+        _check(hostlib().hfem_problem_get_state(self.h, _p(s)), "get_state")
+        return s.reshape(n, 38)
+
+    def matrix_at(self, i, j):
+        v = C.c_double(0)
+        _check(hostlib().hfem_problem_matrix_at(self.h, int(i), int(j), C.byref(v)), "matrix_at")
+        return v.value
+
+    def matrix_class(self):
+        buf = C.create_string_buffer(64)
+        _check(hostlib().hfem_problem_matrix_class(self.h, buf, 64), "matrix_class")
+        return buf.value.decode()

@@ -857,6 +857,78 @@ int orc_surface_load(const orc_mesh *m, int nface, const int *elem, const int *s
 }
 
 /* ------------------------------------------------------------------ */
+/* StaticStructural::solveYourselfAt staticstructural.C:224-358 (initial guess
+ * none) with NRSolver::solve nrsolver.C:205-404 and the per-dof-id force
+ * criterion of checkConvergence nrsolver.C:594-885 (rtold off):
+ *   K = tangent of the state at step start; for nite = 0..:
+ *     F = f_int(X, prescribed); rhs = RT - F;
+ *     err_d = sqrt(sum_{eq of dof d} rhs^2 / (sum RT^2 + eNorm_d)) (absolute
+ *       when the denominator < 1e-6); converged when all err_d <= rtolf and
+ *       not (nite == 0 and err > 0);
+ *     if nite > 0 and manrmsteps > 0 and nite % manrmsteps == 0: K = tangent;
+ *     K ddX = rhs (IML CG + Jacobi from the previous ddX, tolerance lstol);
+ *     X += ddX.
+ * X (neq) is updated in place; state holds the Gauss-point state (temp
+ * fields updated, committed fields untouched).  out[0] Newton iterations,
+ * out[1] CG iterations, out[2] status (1 converged, -1 maxiter, -2 out of
+ * range).  Returns the final max force error in *err. */
+int orc_nonlinear_step(const orc_mesh *m, double *state, double *X, const double *u_presc, const double *RT,
+                       double rtolf, int maxiter, int manrmsteps, double lstol, int *out, double *err)
+{
+    const long long nnz = orc_csr_count(m);
+    long long *rowptr = (long long *)malloc(sizeof(long long) * ((size_t)m->neq + 1));
+    int *colind = (int *)malloc(sizeof(int) * (size_t)nnz);
+    double *val = (double *)malloc(sizeof(double) * (size_t)nnz);
+    double *F = (double *)malloc(sizeof(double) * ((size_t)m->neq + 1));
+    double *rhs = (double *)malloc(sizeof(double) * ((size_t)m->neq + 1));
+    double *ddX = (double *)calloc((size_t)m->neq + 1, sizeof(double));
+    int *dofid = (int *)malloc(sizeof(int) * ((size_t)m->neq + 1));
+    int inuse[3] = {0, 0, 0}, rc = 0;
+    for (long long i = 0; i < 3LL * m->nnode; i++)
+        if (m->eqn[i] > 0) { dofid[m->eqn[i] - 1] = (int)(i % 3); inuse[i % 3]++; }
+    orc_csr_fill(m, rowptr, colind);
+    if (orc_assemble_stiffness(m, rowptr, colind, val, state, 1)) { rc = -10; goto out; }
+    out[0] = out[1] = out[2] = 0;
+    for (int nite = 0;; nite++) {
+        double ebe[3];
+        if (orc_internal_forces(m, X, u_presc, state, 1, F, NULL, ebe)) { rc = -11; goto out; }
+        double s_r[3] = {0, 0, 0}, s_t[3] = {0, 0, 0};
+        for (int i = 0; i < m->neq; i++) {
+            const double rt = RT ? RT[i] : 0.0;
+            rhs[i] = rt - F[i];
+            s_r[dofid[i]] += rhs[i] * rhs[i];
+            s_t[dofid[i]] += rt * rt;
+        }
+        int converged = 1, outofrange = 0;
+        double mx = 0.0;
+        for (int d = 0; d < 3; d++) {
+            if (!inuse[d]) continue;
+            const double den = s_t[d] + ebe[d];
+            const double fe = den >= 1.e-6 ? sqrt(s_r[d] / den) : sqrt(s_r[d]);
+            if (isnan(fe) || fe > rtolf * 1.e20) outofrange = 1;
+            if (fe > rtolf) converged = 0;
+            if (fe > 0.0 && nite == 0) converged = 0;
+            if (fe > mx) mx = fe;
+        }
+        *err = mx;
+        out[0] = nite;
+        if (outofrange) { out[2] = -2; break; }
+        if (converged) { out[2] = 1; break; }
+        if (nite >= maxiter) { out[2] = -1; break; }
+        if (nite > 0 && manrmsteps > 0 && nite % manrmsteps == 0)
+            if (orc_assemble_stiffness(m, rowptr, colind, val, state, 1)) { rc = -10; goto out; }
+        int it = 0;
+        double res = 0.0;
+        orc_pcg(m->neq, rowptr, colind, val, rhs, ddX, lstol, 1000000, 1, &it, &res);
+        out[1] += it;
+        for (int i = 0; i < m->neq; i++) X[i] += ddX[i];
+    }
+out:
+    free(rowptr); free(colind); free(val); free(F); free(rhs); free(ddX); free(dofid);
+    return rc;
+}
+
+/* ------------------------------------------------------------------ */
 /* LinearStatic::solveYourselfAt linearstatic.C:177-252: build K, assemble,
  * rhs = f_ext - f_int(prescribed values), CG from x=0; then reactions
  * StructuralEngngModel::computeReaction structengngmodel.C:183-201:

@@ -99,6 +99,10 @@ void orc_state_commit(const orc_mesh *m, double *state);
 int orc_surface_load(const orc_mesh *m, int nface, const int *elem, const int *surf,
                      const double *traction, double *f_free, double *f_presc);
 
+/* StaticStructural step with NRSolver (per-dof-id force criterion) */
+int orc_nonlinear_step(const orc_mesh *m, double *state, double *X, const double *u_presc, const double *RT,
+                       double rtolf, int maxiter, int manrmsteps, double lstol, int *out, double *err);
+
 /* LinearStatic::solveYourselfAt + StructuralEngngModel::computeReaction */
 int orc_linear_static(const orc_mesh *m, const double *f_ext, const double *f_ext_presc,
                       const double *u_presc, double tol, int maxit,

@@ -190,6 +190,22 @@ def surface_load(deck, factor=1.0):
     return f, fp
 
 
+def nonlinear_step(deck, state, X, u_presc, RT=None, rtolf=1e-8, maxiter=50, manrmsteps=1, lstol=1e-12):
+    """One StaticStructural step with NRSolver; X and state updated in place.
+    Returns (newton_iterations, cg_iterations, status, force_error)."""
+    m = Mesh(deck)
+    out = (C.c_int * 3)()
+    err = C.c_double(0)
+    rt = _d(RT) if RT is not None else None
+    rc = lib().orc_nonlinear_step(m.ref, state.ctypes.data_as(C.c_void_p), X.ctypes.data_as(C.c_void_p),
+                                  _d(u_presc).ctypes.data_as(C.c_void_p),
+                                  rt.ctypes.data_as(C.c_void_p) if rt is not None else None,
+                                  C.c_double(rtolf), maxiter, manrmsteps, C.c_double(lstol), out, C.byref(err))
+    if rc:
+        raise RuntimeError(f"orc_nonlinear_step rc={rc}")
+    return out[0], out[1], out[2], err.value
+
+
 def linear_static(deck, tol=1e-10, maxit=100000):
     m = Mesh(deck)
     f, fp = surface_load(deck)

@@ -10,11 +10,21 @@ from oofem_amd import build, meshgen
 from tests.conftest import ROOT
 
 
-def _declared():
-    txt = open(os.path.join(ROOT, "include", "hipfem.h")).read()
+def _declared(header="hipfem.h"):
+    txt = open(os.path.join(ROOT, "include", header)).read()
     txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
-    names = re.findall(r"\b((?:hipfem|hipcsr|hippcg|hfem_model)_[a-z0-9_]+)\s*\(", txt)
+    names = re.findall(r"\b((?:hipfem|hipcsr|hippcg|hfem_model|hfem_problem)_[a-z0-9_]+)\s*\(", txt)
     return sorted(set(names))
+
+
+def test_host_library_exports_every_symbol():
+    assert os.path.exists(build.HOST_LIB), "build libhipfem_host.so first (__graft_entry__.build())"
+    ctypes.CDLL(build.LIB, mode=ctypes.RTLD_GLOBAL)
+    lib = ctypes.CDLL(build.HOST_LIB)
+    names = _declared("hipfem_host.h")
+    assert len(names) >= 12
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
 
 
 def test_header_declares_entry_points():

@@ -225,6 +225,33 @@ def test_mises_tangent_vs_finite_differences(seed):
     assert np.abs(D - D.T).max() < 1e-9 * np.abs(D).max()
 
 
+@pytest.mark.parametrize("etype", [meshgen.LSPACE, meshgen.LTRSPACE])
+def test_newton_uniaxial_block_closed_form(etype):
+    """NRSolver + MisesMat + reactions on a homogeneous uniaxial stress state:
+    the reaction sum on the loaded face equals the closed-form stress times
+    the area at every load step, elastic and plastic."""
+    from tests import nlcases
+    d, top = nlcases.uniaxial_block(etype)
+    ey = nlcases.SIG0 / nlcases.E
+    st = oracle.state_new(d)
+    X = np.zeros(d.neq)
+    for k in range(1, 7):
+        eps = k * 0.6 * ey
+        up = d.prescribed_values(eps)
+        nit, cg, status, err = oracle.nonlinear_step(d, st, X, up, rtolf=1e-10, maxiter=30, manrmsteps=1)
+        assert status == 1 and err <= 1e-10
+        _, fp, _ = oracle.internal_forces(d, X, up, state=st.copy(), update_state=False)
+        oracle.state_commit(d, st)
+        Rz = nlcases.top_reaction_z(d, top, fp)
+        expect = nlcases.uniaxial_stress(eps)
+        assert abs(Rz - expect) <= 1e-8 * expect, (k, Rz, expect)
+        if k <= 1:
+            assert nit == 1          # linear: one Newton correction
+        s = st.reshape(-1, 38)
+        kap = s[:, 6]
+        assert np.ptp(kap) <= 1e-12 + 1e-9 * kap.max()   # homogeneous
+
+
 def test_uniaxial_bar_closed_form():
     """Uniaxial stress from the 3D return: a strain path with free lateral
     contraction found by iteration reproduces sigma = sig0 + H_t (eps - eps_y)

@@ -91,65 +91,80 @@ def run_kernels(args, deck):
 
 
 def run_gpu(args, deck, precision):
-    import numpy as np
-
-    from oofem_amd.hipfem import (CR_CONVERGED, BatchedEvaluator, Context, DeviceVector, HipCSRSparseMtrx,
-                                  HipPCGSolver)
-    ctx = Context(0)
-    K = HipCSRSparseMtrx(ctx)
-    ev = BatchedEvaluator(ctx, deck)
-    t = time.perf_counter()
-    ev.build_structure(K)
-    t_struct = time.perf_counter() - t
-    apply_shape(K, args.shape)
-    solver = HipPCGSolver(ctx, lstol=args.tol, lsiter=args.maxit, lsprecond=1, precision=precision,
-                          inner_tol=args.inner_tol)
-    solver.set_profiling(True)
+    """LinearStatic steps through the C++ host driver (class factory ->
+    hipcsr + hippcg, batched evaluator); data resident on the GPU."""
+    from oofem_amd.problem import StructuralProblem
+    P = StructuralProblem(deck, lstol=args.tol, lsiter=args.maxit, lsprecond=1, mixed=precision,
+                          irtol=args.inner_tol)
     f_ext, f_ext_p = deck.external_load()
-    up = deck.prescribed_values()
-    d_fext = DeviceVector(ctx, deck.neq, f_ext)
-    d_fextp = DeviceVector(ctx, max(deck.npeq, 1), f_ext_p if deck.npeq else np.zeros(1))
-    d_up = DeviceVector(ctx, max(deck.npeq, 1), up if deck.npeq else np.zeros(1))
-    d_u0 = DeviceVector(ctx, deck.neq)
-    d_rhs = DeviceVector(ctx, deck.neq)
-    d_u = DeviceVector(ctx, deck.neq)
-    d_f = DeviceVector(ctx, deck.neq)
-    d_R = DeviceVector(ctx, max(deck.npeq, 1))
-    ctx.synchronize()
-
-    def step():
-        ev.assemble_stiffness(K)
-        ev.internal_forces_dev(d_u0.addr, d_up.addr, d_rhs.addr, d_R.addr)
-        d_rhs.axpby(1.0, d_fext, -1.0)                     # rhs = f_ext - f_int
-        d_u.fill(0.0)
-        reason, it, res = solver.solve_dev(K, d_rhs.addr, d_u.addr)
-        ev.internal_forces_dev(d_u.addr, d_up.addr, d_f.addr, d_R.addr, update_state=True)
-        d_R.axpby(-1.0, d_fextp, 1.0)                      # reactions
-        if reason != CR_CONVERGED:
-            raise RuntimeError(f"PCG did not converge: {it} iterations, residual {res:.3e}")
-        return it, res
-
-    for i in range(args.warmup):
-        it, res = step()
-        log(f"[bench] {precision=} warmup {i}: iters={it} res={res:.3e}")
-    ctx.synchronize()
-    n_sp, ms_sp, n32, ms32, iters = 0, 0.0, 0, 0.0, []
+    P.set_prescribed(deck.prescribed_values())
+    P.set_load(f_ext, f_ext_p)
+    t = time.perf_counter()
+    for i in range(args.warmup):          # the first step also builds the sparse structure
+        it, res = P.linear_static()
+        log(f"[bench] {precision=} warmup {i}: iters={it} res={res:.3e} ({time.perf_counter() - t:.1f} s)")
+    P.set_profiling(True)
+    iters = []
     t0 = time.perf_counter()
     for i in range(args.steps):
-        it, res = step()
-        p = solver.profile()
-        n_sp += p["n64"]; ms_sp += p["ms64"]; n32 += p["n32"]; ms32 += p["ms32"]
+        it, res = P.linear_static()
         iters.append(it)
-    ctx.synchronize()
-    dt = time.perf_counter() - t0
-    u = d_u.download()
-    R = d_R.download()[: deck.npeq]
-    out = dict(dt=dt, iters=iters, res=res, t_struct=t_struct, nnz=K.nnz(), n_spmv=n_sp, ms_spmv=ms_sp,
-               n_spmv32=n32, ms_spmv32=ms32, outer=solver.last_outer(), u=u, reactions=R)
-    for v in (d_fext, d_fextp, d_up, d_u0, d_rhs, d_u, d_f, d_R):
-        v.close()
-    solver.close(); K.close(); ev.close(); ctx.close()
+    dt = time.perf_counter() - t0         # linear_static returns after the device work has finished
+    p = P.profile()
+    u = P.solution()
+    R = P.reactions()
+    out = dict(dt=dt, iters=iters, res=res, nnz=P.matrix_nnz(), n_spmv=p["n64"], ms_spmv=p["ms64"],
+               n_spmv32=p["n32"], ms_spmv32=p["ms32"], outer=0, u=u, reactions=R)
+    P.close()
     return out
+
+
+def run_newton(args, deck, precision):
+    """StaticStructural + NRSolver steps (full Newton, consistent tangent,
+    rtolf 1e-8) on the MisesMat tet bar, prescribed end displacement ramped
+    over --nl-total steps; the first step is warm-up."""
+    import numpy as np
+
+    from oofem_amd.problem import StructuralProblem
+    P = StructuralProblem(deck, lstol=args.nl_lstol, lsiter=args.maxit, lsprecond=1, mixed=precision,
+                          irtol=args.inner_tol, rtolf=1e-8, maxiter=60, manrmsteps=1)
+    P.set_load(np.zeros(deck.neq), np.zeros(deck.npeq))
+    reports = []
+    t0 = None
+    for k in range(1, 1 + 1 + args.nl_steps):
+        P.set_prescribed(deck.prescribed_values(k / args.nl_total))
+        if k == 2:
+            P.set_profiling(True)
+            t0 = time.perf_counter()
+        rep = P.nonlinear_step()
+        if rep["status"] != 1:
+            raise RuntimeError(f"Newton step {k} failed: {rep}")
+        P.update()
+        reports.append(rep)
+        log(f"[bench] newton step {k}: {rep}")
+    dt = time.perf_counter() - t0
+    p = P.profile()
+    P.close()
+    timed = reports[1:]
+    return dict(dt=dt, steps=len(timed), newton=[r["newton"] for r in timed], pcg=[r["pcg"] for r in timed],
+                n_spmv=p["n64"], ms_spmv=p["ms64"])
+
+
+def measured_traffic(kernel, workload, args):
+    """HBM bytes per launch of `kernel` on `workload` from the committed
+    counter summary (profiles/pmc_traffic.json: FETCH_SIZE and WRITE_SIZE per
+    dispatch, each scaled by its copy-kernel calibration factor), or None."""
+    if args.traffic_gb is not None:
+        return args.traffic_gb * 1e9
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        tab = json.load(fh)
+    for rec in tab.get("kernels", []):
+        if rec.get("kernel") == kernel and rec.get("workload") == workload:
+            return rec.get("bytes_per_launch")
+    return None
 
 
 def cpu_baseline(args, deck_full, gpu_iters, nnz_full):
@@ -201,6 +216,11 @@ def main():
     ap.add_argument("--shape", default="", help="SpMV shape: lanes:L:U[:nt] or stream:R:U")
     ap.add_argument("--mode", choices=["step", "kernels"], default="step")
     ap.add_argument("--iters", type=int, default=50, help="PCG iterations in --mode kernels")
+    ap.add_argument("--newton-steps", type=int, default=2, help="timed Newton steps of the tet plasticity variant")
+    ap.add_argument("--nl-total", type=int, default=20, help="load steps of the full ramp")
+    ap.add_argument("--nl-lstol", type=float, default=1e-6, help="PCG tolerance inside Newton")
+    ap.add_argument("--traffic-gb", type=float, default=None,
+                    help="HBM bytes per SpMV launch from the calibrated FETCH_SIZE+WRITE_SIZE runs (GB)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
@@ -219,8 +239,7 @@ def main():
         name = "fp64" if prec == 0 else "mixed"
         r = run_gpu(args, deck, prec)
         res[name] = r
-        log(f"[bench] {name}: {args.steps} steps in {r['dt']:.3f} s, iters {r['iters']}, outer {r['outer']}, "
-            f"structure build {r['t_struct']:.1f} s")
+        log(f"[bench] {name}: {args.steps} steps in {r['dt']:.3f} s, iters {r['iters']}")
     main_name = "fp64" if 0 in modes else "mixed"
     r = res[main_name]
     n, nnz = deck.neq, r["nnz"]
@@ -228,16 +247,31 @@ def main():
     value = n * args.steps / r["dt"]
     roof = None
     if r["n_spmv"]:
+        # node-block PCG SpMV: values 8 B/nnz, one int32 block column per 9
+        # values, row pointers 4 B per node row, gather p and write q (16 B/row)
         avg_ms = r["ms_spmv"] / r["n_spmv"]
-        bytes_per = 12.0 * nnz + 36.0 * n
+        bytes_per = 8.0 * nnz + 4.0 * nnz / 9.0 + 4.0 * n / 3.0 + 16.0 * n
         ach = bytes_per / (avg_ms * 1e-3) / 1e9
+        kname = "k_pcg_spmv_p_nb<double, double>"
         roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                    frac=round(ach / HBM_PEAK_GBS, 4), traffic=None, kernel="k_pcg_spmv<16,double,double>",
-                    avg_launch_ms=round(avg_ms, 4), launches=r["n_spmv"], algorithmic_bytes=int(bytes_per))
+                    frac=round(ach / HBM_PEAK_GBS, 4), traffic=measured_traffic(kname, deck.name, args),
+                    kernel=kname, avg_launch_ms=round(avg_ms, 4), launches=r["n_spmv"],
+                    algorithmic_bytes=int(bytes_per))
     variants = {}
     for name, rv in res.items():
         variants[name] = dict(dofs_per_s=n * args.steps / rv["dt"], ms_per_step=rv["dt"] / args.steps * 1e3,
-                              pcg_iters=rv["iters"], outer=rv["outer"], final_resid=rv["res"])
+                              pcg_iters=rv["iters"], final_resid=rv["res"])
+    if args.newton_steps > 0:
+        from oofem_amd import meshgen
+        nd = meshgen.config2()
+        log(f"[bench] newton deck {nd.name}: nelem={nd.nelem} neq={nd.neq}")
+        args.nl_steps = args.newton_steps
+        rn = run_newton(args, nd, 0)
+        variants["newton_fp64_tets"] = dict(workload=nd.name, free_dofs=nd.neq, steps=rn["steps"],
+                                            dofs_per_s=nd.neq * rn["steps"] / rn["dt"],
+                                            ms_per_step=rn["dt"] / rn["steps"] * 1e3, newton_iters=rn["newton"],
+                                            pcg_iters=rn["pcg"], lstol=args.nl_lstol,
+                                            spmv_avg_ms=rn["ms_spmv"] / max(rn["n_spmv"], 1))
     if "fp64" in res and "mixed" in res:
         import numpy as np
         a, b = res["fp64"], res["mixed"]

@@ -203,6 +203,15 @@ int hfem_model_internal_forces(hfem_model *md, const double *u_free, const doubl
                                double *f_free, double *f_presc, double *ebe3, int update_state);
 int hfem_model_internal_forces_dev(hfem_model *md, const double *du_free, const double *du_presc,
                                    double *df_free, double *df_presc, double *ebe3, int update_state);
+/* y = sum_e K_e x_e on device vectors (EngngModel::assembleExtrapolatedForces
+ * engngm.C:1536-1610, element matrix times element vector, assembled by
+ * equation number), evaluated as sum_g B^T D B x_e dV without forming K_e.
+ * mode 0: elastic D; 1: TangentStiffness D of the temp Gauss-point state
+ * (as hfem_model_assemble_stiffness).  dx_free / dx_presc may be NULL
+ * (zero); dy_free (neq) and dy_presc (npeq) are overwritten, either may be
+ * NULL.  The state is not modified. */
+int hfem_model_tangent_product_dev(hfem_model *md, int mode, const double *dx_free, const double *dx_presc,
+                                   double *dy_free, double *dy_presc);
 /* Newton residual on device vectors: rhs = RT - F (RT may be NULL = 0) and
  * sums6 = {sum rhs^2 over u, v, w equations, sum RT^2 over u, v, w}, the
  * per-dof-id quantities of NRSolver::checkConvergence (nrsolver.C:648-660). */

@@ -31,6 +31,9 @@ void hfem_problem_destroy(hfem_problem *p);
  * NRSolver keywords (nrsolver.h:54-73). */
 int hfem_problem_set_options(hfem_problem *p, int smtype, int lstype, double lstol, int lsiter, int lsprecond,
                              int mixed, double irtol, double rtolf, int maxiter, int manrmsteps);
+/* StaticStructural initial guess (_IFT_EngngModel_initialGuess,
+ * staticstructural.C:130,247-297): 0 = IG_None, 1 = IG_Tangent (default) */
+int hfem_problem_set_initial_guess(hfem_problem *p, int type);
 /* prescribed values (npeq) and total external load (neq, npeq) of the next step */
 int hfem_problem_set_prescribed(hfem_problem *p, const double *up);
 int hfem_problem_set_load(hfem_problem *p, const double *f_free, const double *f_presc);
@@ -47,9 +50,14 @@ int hfem_problem_get_solution(hfem_problem *p, double *u);
 /* StructuralEngngModel::computeReaction (structengngmodel.C:183-201) */
 int hfem_problem_get_reactions(hfem_problem *p, double *r);
 int hfem_problem_get_state(hfem_problem *p, double *s);
+/* HIP-event timing of the PCG SpMV launches, summed over the solves since
+ * profiling was switched on (instrumentation; no counterpart in the reference) */
+int hfem_problem_set_profiling(hfem_problem *p, int on);
+int hfem_problem_profile(hfem_problem *p, int *n64, double *ms64, int *n32, double *ms32);
 /* SparseMtrx::at / giveClassName of the factory-made stiffness matrix */
 int hfem_problem_matrix_at(hfem_problem *p, int i, int j, double *v);
 int hfem_problem_matrix_class(hfem_problem *p, char *buf, int len);
+int hfem_problem_matrix_nnz(hfem_problem *p, long long *nnz);
 
 #ifdef __cplusplus
 }

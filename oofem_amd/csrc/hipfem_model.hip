@@ -413,7 +413,7 @@ __global__ __launch_bounds__(64) void k_internal(const int *__restrict__ elems, 
                                                  const double *__restrict__ u_presc, double *__restrict__ state,
                                                  int ngp_total, int update_state, double *__restrict__ f_free,
                                                  double *__restrict__ f_presc, double *__restrict__ partials,
-                                                 int npart, int *__restrict__ bad)
+                                                 int npart, int *__restrict__ bad, int op)
 {
     constexpr int NEN = ElemT<ET>::NEN, NGP = ElemT<ET>::NGP, EPB = IntCfg<ET>::EPB, NC = 3 * NEN;
     __shared__ double sF[EPB][NGP][NC];
@@ -431,7 +431,7 @@ __global__ __launch_bounds__(64) void k_internal(const int *__restrict__ elems, 
             for (int d = 0; d < 3; d++) {
                 X[3 * a + d] = coords[3 * (size_t)node + d];
                 const int q = eqn[3 * (size_t)node + d];
-                ue[3 * a + d] = q > 0 ? u_free[q - 1] : ((q < 0 && u_presc) ? u_presc[-q - 1] : 0.0);
+                ue[3 * a + d] = (q > 0 && u_free) ? u_free[q - 1] : ((q < 0 && u_presc) ? u_presc[-q - 1] : 0.0);
             }
         }
         double xi[3], w, detJ, dNdx[NC];
@@ -454,7 +454,29 @@ __global__ __launch_bounds__(64) void k_internal(const int *__restrict__ elems, 
         const double E = mp[1], nu = mp[2];
         const size_t gp = (size_t)e * NGP + g;
         double sig[6];
-        if ((int)mp[0] == HIPFEM_MAT_MISES) {
+        if (op != 0) {
+            // tangent product: sigma = D eps with D the elastic (op 1) or the
+            // TangentStiffness (op 2) matrix of the current temp state, so that
+            // sum_g B^T D B u_e dV = K_e u_e
+            double D[36], s[6];
+            if ((int)mp[0] == HIPFEM_MAT_MISES && op == 2) {
+#pragma unroll
+                for (int i = 0; i < 6; i++) s[i] = state[(size_t)(14 + i) * ngp_total + gp];
+                mises_tangent(E, nu, mp[3], mp[4], state[(size_t)6 * ngp_total + gp],
+                              state[(size_t)13 * ngp_total + gp], s, D);
+            } else {
+#pragma unroll
+                for (int i = 0; i < 6; i++) s[i] = 0.0;
+                mises_tangent(E, nu, 0.0, 0.0, 0.0, 0.0, s, D);   // dKappa = 0: elastic D
+            }
+#pragma unroll
+            for (int r = 0; r < 6; r++) {
+                double v = 0.0;
+#pragma unroll
+                for (int c = 0; c < 6; c++) v += D[6 * r + c] * eps[c];
+                sig[r] = v;
+            }
+        } else if ((int)mp[0] == HIPFEM_MAT_MISES) {
             double epsp_n[6];
 #pragma unroll
             for (int i = 0; i < 6; i++) epsp_n[i] = state[(size_t)i * ngp_total + gp];
@@ -483,7 +505,7 @@ __global__ __launch_bounds__(64) void k_internal(const int *__restrict__ elems, 
             sig[4] = G * eps[4];
             sig[5] = G * eps[5];
         }
-        if (update_state) {
+        if (update_state && op == 0) {
 #pragma unroll
             for (int i = 0; i < 6; i++) {
                 state[(size_t)(20 + i) * ngp_total + gp] = sig[i];
@@ -933,7 +955,7 @@ extern "C" int hfem_model_assemble_stiffness(hfem_model *md, hipcsr_mtx *K, int 
 }
 
 static int internal_dev(hfem_model *md, const double *uf, const double *up, double *ff, double *fp, double *ebe3,
-                        int update)
+                        int update, int op = 0)
 {
     hipStream_t st = md->ctx->stream;
     HIPFEM_REQUIRE(md->ncolors > 0 || md->nelem == 0, HIPFEM_EINVAL,
@@ -958,7 +980,7 @@ static int internal_dev(hfem_model *md, const double *uf, const double *up, doub
         // partial slots of this color: [poff, poff + nb) in each of the 3 planes
 #define LAUNCH_INT(ET)                                                                                          \
     k_internal<ET><<<nb, 64, 0, st>>>(el, n, md->d_coords, md->d_conn, md->d_eqn, md->d_emat, md->d_mat, uf, up, \
-                                      md->d_state, md->ngp_total, update, ff, fp, part + poff, npart, md->d_bad)
+                                      md->d_state, md->ngp_total, update, ff, fp, part + poff, npart, md->d_bad, op)
         if (md->etype == HIPFEM_LSPACE) LAUNCH_INT(HIPFEM_LSPACE);
         else if (md->etype == HIPFEM_LTRSPACE) LAUNCH_INT(HIPFEM_LTRSPACE);
         else LAUNCH_INT(HIPFEM_QSPACE);
@@ -999,6 +1021,17 @@ extern "C" int hfem_model_internal_forces(hfem_model *md, const double *u_free, 
         HIPFEM_CHECK(hipMemcpyAsync(f_presc, md->d_fp, sizeof(double) * md->npeq, hipMemcpyDeviceToHost, st));
     HIPFEM_CHECK(hipStreamSynchronize(st));
     return 0;
+}
+
+extern "C" int hfem_model_tangent_product_dev(hfem_model *md, int mode, const double *dx_free, const double *dx_presc,
+                                              double *dy_free, double *dy_presc)
+{
+    HIPFEM_REQUIRE(md && md->d_conn, HIPFEM_EINVAL, "tangent product: no mesh");
+    HIPFEM_REQUIRE(mode == 0 || mode == 1, HIPFEM_EINVAL, "tangent product: mode must be 0 or 1");
+    HIPFEM_REQUIRE(dy_free || dy_presc, HIPFEM_EINVAL, "tangent product: no output");
+    DeviceGuard g(md->ctx->device);
+    return internal_dev(md, dx_free, md->npeq ? dx_presc : nullptr, dy_free, md->npeq ? dy_presc : nullptr, nullptr,
+                        0, mode + 1);
 }
 
 extern "C" int hfem_model_commit(hfem_model *md)

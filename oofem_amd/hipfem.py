@@ -420,6 +420,19 @@ class BatchedEvaluator:
                                                     int(bool(update_state))), "internal_forces_dev")
         return ebe
 
+    def tangent_product(self, x_free=None, x_presc=None, tangent=True):
+        """y = sum_e K_e x_e (free and prescribed rows) with the elastic or
+        the tangent D of the temp Gauss-point state, on host arrays."""
+        d = self.deck
+        xf = DeviceVector(self.ctx, max(d.neq, 1), x_free) if x_free is not None else None
+        xp = DeviceVector(self.ctx, max(d.npeq, 1), x_presc) if x_presc is not None and d.npeq else None
+        yf = DeviceVector(self.ctx, max(d.neq, 1))
+        yp = DeviceVector(self.ctx, max(d.npeq, 1)) if d.npeq else None
+        _check(lib().hfem_model_tangent_product_dev(self.h, int(bool(tangent)), C.c_void_p(xf.addr if xf else 0),
+                                                    C.c_void_p(xp.addr if xp else 0), C.c_void_p(yf.addr),
+                                                    C.c_void_p(yp.addr if yp else 0)), "tangent_product")
+        return yf.download()[: d.neq], (yp.download()[: d.npeq] if yp else np.zeros(0))
+
     def commit(self):
         _check(lib().hfem_model_commit(self.h), "commit")
 

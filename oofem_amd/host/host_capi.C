@@ -61,6 +61,14 @@ int hfem_problem_set_options(hfem_problem *p, int smtype, int lstype, double lst
     });
 }
 
+int hfem_problem_set_initial_guess(hfem_problem *p, int type)
+{
+    return guarded([&] {
+        if (type != IG_None && type != IG_Tangent) OOFEM_ERROR("Initial guess type: %d not supported", type);
+        reinterpret_cast<StructuralProblem *>(p)->initialGuessType = (InitialGuess)type;
+    });
+}
+
 int hfem_problem_set_prescribed(hfem_problem *p, const double *up)
 {
     return guarded([&] { reinterpret_cast<StructuralProblem *>(p)->setPrescribedValues(up); });
@@ -115,6 +123,24 @@ int hfem_problem_matrix_at(hfem_problem *p, int i, int j, double *v)
         SparseMtrx *K = reinterpret_cast<StructuralProblem *>(p)->giveStiffnessMatrix();
         if (!K) OOFEM_ERROR("no stiffness matrix yet");
         *v = static_cast<const SparseMtrx *>(K)->at(i, j);
+    });
+}
+
+int hfem_problem_set_profiling(hfem_problem *p, int on)
+{
+    return guarded([&] { reinterpret_cast<StructuralProblem *>(p)->setProfiling(on != 0); });
+}
+
+int hfem_problem_profile(hfem_problem *p, int *n64, double *ms64, int *n32, double *ms32)
+{
+    return guarded([&] { reinterpret_cast<StructuralProblem *>(p)->giveProfile(n64, ms64, n32, ms32); });
+}
+
+int hfem_problem_matrix_nnz(hfem_problem *p, long long *nnz)
+{
+    return guarded([&] {
+        auto *H = dynamic_cast<HipCSRSparseMtrx *>(reinterpret_cast<StructuralProblem *>(p)->giveStiffnessMatrix());
+        *nnz = H ? (long long)hipcsr_nnz(H->handle()) : 0;
     });
 }
 

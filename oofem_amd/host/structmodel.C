@@ -51,7 +51,7 @@ StructuralProblem::StructuralProblem(int nnode_, const double *coords, int etype
     hipfemCheck(hfem_model_set_mesh(ev, nnode, coords, etype, nelem, conn_, emat, nmat, mat, eqn_, neq, npeq),
                 "set_mesh");
     for (DeviceVector *v : {&X, &RT, &F, &rhs, &ddX}) v->allocate(ctx, neq);
-    for (DeviceVector *v : {&up, &RTp, &Fp}) v->allocate(ctx, npeq);
+    for (DeviceVector *v : {&up, &upPrev, &dup, &RTp, &Fp}) v->allocate(ctx, npeq);
     // IMLSolver defaults unless the caller overrides them
     solverRecord.set("lsprecond", 1);
 }
@@ -116,8 +116,18 @@ ConvergedReason StructuralProblem::linearSolve(const DeviceVector &b, DeviceVect
     auto *H = dynamic_cast<HipCSRSparseMtrx *>(K.get());
     auto *S = dynamic_cast<HipPCGSolver *>(linSolver.get());
     if (H && S) {
+        hipfemCheck(hippcg_set_profiling(S->handle(), profiling ? 1 : 0), "set_profiling");
         ConvergedReason r = S->solveDevice(*H, b.ptr(), x.ptr());
         lastLinearIterations = S->giveLastIterations();
+        if (profiling) {
+            int n64 = 0, n32 = 0;
+            double m64 = 0.0, m32 = 0.0;
+            hipfemCheck(hippcg_profile(S->handle(), &n64, &m64, &n32, &m32), "profile");
+            profN64 += n64;
+            profN32 += n32;
+            profMs64 += m64;
+            profMs32 += m32;
+        }
         return r;
     }
     // generic SparseLinearSystemNM path on host vectors
@@ -148,9 +158,8 @@ ConvergedReason StructuralProblem::solveLinearStatic(int &iterations, double &re
     return s;
 }
 
-// StaticStructural::solveYourselfAt staticstructural.C:224-358 with the default
-// initial guess (none: the increment starts at zero) and NRSolver::solve
-// nrsolver.C:205-404.
+// StaticStructural::solveYourselfAt staticstructural.C:224-358 and
+// NRSolver::solve nrsolver.C:205-404.
 NRStepReport StructuralProblem::solveNonlinearStep()
 {
     ensureStructure();
@@ -159,9 +168,33 @@ NRStepReport StructuralProblem::solveNonlinearStep()
     int idsInUse[3] = {0, 0, 0};
     for (size_t i = 0; i < eqn.size(); i++)
         if (eqn[i] > 0) idsInUse[i % 3]++;
+    bool tangentCurrent = false;
+    if (initialGuessType == IG_Tangent && npeq > 0) {
+        // staticstructural.C:247-297: extrapolated forces -sum_e K_e du_e of
+        // the prescribed increments (free entries of du_e are zero at step
+        // start), then K inc = f from zero and X += inc
+        dup.axpby(1.0, &up, 0.0);
+        dup.axpby(-1.0, &upPrev, 1.0);
+        hipfemCheck(hfem_model_tangent_product_dev(ev, 1, nullptr, dup.ptr(), rhs.ptr(), nullptr), "extrapolate");
+        rhs.axpby(0.0, nullptr, -1.0);
+        if (rhs.dot(rhs) > 0.0) {
+            assemble(*K, 1);                        // updateMatrix
+            tangentCurrent = true;
+            ddX.zero();
+            ConvergedReason ls = linearSolve(rhs, ddX);
+            rep.linearIterations += lastLinearIterations;
+            if (ls != CR_CONVERGED) {
+                rep.status = CR_FAILED;
+                return rep;
+            }
+            X.axpby(1.0, &ddX, 1.0);
+        }
+    }
     ddX.zero();
-    // stiffness before the first residual (mCalcStiffBeforeRes, nrsolver.C:245-256)
-    assemble(*K, 1);
+    // stiffness before the first residual (mCalcStiffBeforeRes, nrsolver.C:245-256);
+    // the matrix of the initial guess is that same tangent (no Gauss-point
+    // state changed since), so it is reused
+    if (!tangentCurrent) assemble(*K, 1);
     for (int nite = 0;; ++nite) {
         double ebe[3], sums[6];
         internalForces(X, true, ebe);               // updateComponent(InternalRhs)
@@ -194,7 +227,26 @@ NRStepReport StructuralProblem::solveNonlinearStep()
     return rep;
 }
 
-void StructuralProblem::updateYourself() { hipfemCheck(hfem_model_commit(ev), "commit"); }
+void StructuralProblem::updateYourself()
+{
+    hipfemCheck(hfem_model_commit(ev), "commit");
+    upPrev.axpby(1.0, &up, 0.0);   // prescribed values of the equilibrated step
+}
+
+void StructuralProblem::setProfiling(bool on)
+{
+    profiling = on;
+    profN64 = profN32 = 0;
+    profMs64 = profMs32 = 0.0;
+}
+
+void StructuralProblem::giveProfile(int *n64, double *ms64, int *n32, double *ms32) const
+{
+    *n64 = profN64;
+    *ms64 = profMs64;
+    *n32 = profN32;
+    *ms32 = profMs32;
+}
 
 void StructuralProblem::giveReactions(double *r)
 {

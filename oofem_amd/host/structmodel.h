@@ -5,7 +5,8 @@
 //                         + StructuralEngngModel::computeReaction
 //                         (structengngmodel.C:183-201)
 //   solveNonlinearStep()  StaticStructural::solveYourselfAt
-//                         (staticstructural.C:224-358) driving NRSolver::solve
+//                         (staticstructural.C:224-358: tangent initial guess
+//                         over the prescribed increments) driving NRSolver::solve
 //                         (nrsolver.C:205-404) with checkConvergence
 //                         (nrsolver.C:594-885, per-dof-id force error)
 // The sparse matrix and the linear solver come from the class factory by
@@ -52,6 +53,10 @@ struct NRSolverOptions {
     bool calcStiffBeforeRes = true;
 };
 
+// StaticStructural initial guess (_IFT_EngngModel_initialGuess): IG_None = 0,
+// IG_Tangent = 1 (the default, staticstructural.C:130)
+enum InitialGuess { IG_None = 0, IG_Tangent = 1 };
+
 struct NRStepReport {
     int iterations = 0;        // Newton iterations
     int linearIterations = 0;  // PCG iterations summed over the step
@@ -70,9 +75,12 @@ class StructuralProblem : public EngngModel {
     std::unique_ptr<SparseMtrx> K;
     std::unique_ptr<SparseLinearSystemNM> linSolver;
     // state on the device
-    DeviceVector X, up, RT, RTp, F, Fp, rhs, ddX;
+    DeviceVector X, up, upPrev, dup, RT, RTp, F, Fp, rhs, ddX;
     bool structureBuilt = false;
     int lastLinearIterations = 0;
+    bool profiling = false;
+    int profN64 = 0, profN32 = 0;
+    double profMs64 = 0.0, profMs32 = 0.0;
 
     void ensureStructure();
     void internalForces(const DeviceVector &x, bool updateState, double *ebe3);
@@ -80,6 +88,7 @@ class StructuralProblem : public EngngModel {
 
 public:
     NRSolverOptions nr;
+    InitialGuess initialGuessType = IG_Tangent;
 
     StructuralProblem(int nnode, const double *coords, int etype, int nelem, const int *conn, const int *emat,
                       int nmat, const double *mat, const int *eqn, int neq, int npeq);
@@ -110,6 +119,10 @@ public:
     int giveNeq() const { return neq; }
     int giveNpeq() const { return npeq; }
     int giveLastLinearIterations() const { return lastLinearIterations; }
+    // SpMV launch timing of the hippcg solver, summed over every solve since
+    // setProfiling(true)
+    void setProfiling(bool on);
+    void giveProfile(int *n64, double *ms64, int *n32, double *ms32) const;
     SparseMtrx *giveStiffnessMatrix() { return K.get(); }
 };
 

@@ -55,7 +55,7 @@ def _p(a):
 
 class StructuralProblem:
     def __init__(self, deck, smtype=SMT_HipCSR, lstype=ST_HipPCG, lstol=1e-10, lsiter=100000, lsprecond=1,
-                 mixed=0, irtol=1e-4, rtolf=1e-8, maxiter=50, manrmsteps=1):
+                 mixed=0, irtol=1e-4, rtolf=1e-8, maxiter=50, manrmsteps=1, initial_guess=1):
         self.deck = d = deck
         self._keep = [np.ascontiguousarray(x) for x in (d.coords, d.conn, d.emat, d.mat, d.eqn)]
         c, cn, em, mt, eq = self._keep
@@ -64,6 +64,7 @@ class StructuralProblem:
                                              _p(eq), d.neq, d.npeq, C.byref(self.h)), "create")
         _check(hostlib().hfem_problem_set_options(self.h, smtype, lstype, lstol, lsiter, lsprecond, mixed, irtol,
                                                   rtolf, maxiter, manrmsteps), "set_options")
+        _check(hostlib().hfem_problem_set_initial_guess(self.h, int(initial_guess)), "set_initial_guess")
 
     def close(self):
         if self.h:
@@ -115,9 +116,24 @@ class StructuralProblem:
         _check(hostlib().hfem_problem_get_state(self.h, _p(s)), "get_state")
         return s.reshape(n, 38)
 
+    def set_profiling(self, on=True):
+        _check(hostlib().hfem_problem_set_profiling(self.h, int(bool(on))), "set_profiling")
+
+    def profile(self):
+        n64, n32 = C.c_int(0), C.c_int(0)
+        m64, m32 = C.c_double(0), C.c_double(0)
+        _check(hostlib().hfem_problem_profile(self.h, C.byref(n64), C.byref(m64), C.byref(n32), C.byref(m32)),
+               "profile")
+        return dict(n64=n64.value, ms64=m64.value, n32=n32.value, ms32=m32.value)
+
     def matrix_at(self, i, j):
         v = C.c_double(0)
         _check(hostlib().hfem_problem_matrix_at(self.h, int(i), int(j), C.byref(v)), "matrix_at")
+        return v.value
+
+    def matrix_nnz(self):
+        v = C.c_int64(0)
+        _check(hostlib().hfem_problem_matrix_nnz(self.h, C.byref(v)), "matrix_nnz")
         return v.value
 
     def matrix_class(self):

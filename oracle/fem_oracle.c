@@ -857,10 +857,50 @@ int orc_surface_load(const orc_mesh *m, int nface, const int *elem, const int *s
 }
 
 /* ------------------------------------------------------------------ */
-/* StaticStructural::solveYourselfAt staticstructural.C:224-358 (initial guess
- * none) with NRSolver::solve nrsolver.C:205-404 and the per-dof-id force
- * criterion of checkConvergence nrsolver.C:594-885 (rtold off):
- *   K = tangent of the state at step start; for nite = 0..:
+/* EngngModel::assembleExtrapolatedForces engngm.C:1536-1610 with the
+ * TangentStiffnessMatrix of the current Gauss-point state: per element
+ * K_e * du_e, where du_e holds the prescribed increments (free entries are
+ * zero at step start), assembled into the free rows: f = sum_e K_e du_e.
+ * The caller negates it (staticstructural.C:267).                      */
+int orc_extrapolated_forces(const orc_mesh *m, const double *state, const double *du_presc, double *f)
+{
+    const int nen = orc_nen(m->etype), nc = 3 * nen, ngp = orc_ngp(m->etype);
+    double X[60], du[60], *Ke = (double *)malloc(sizeof(double) * nc * nc), *Dgp = (double *)malloc(sizeof(double) * 36 * ngp);
+    int loc[60], ploc[60], rc = 0;
+    memset(f, 0, sizeof(double) * (size_t)m->neq);
+    for (int e = 0; e < m->nelem; e++) {
+        location(m, e, loc, ploc);
+        int any = 0;
+        for (int i = 0; i < nc; i++) {
+            du[i] = ploc[i] ? du_presc[ploc[i] - 1] : 0.0;
+            if (du[i] != 0.0) any = 1;
+        }
+        if (!any) continue;  /* K_e * 0 contributes exact zeros */
+        int sym = 1;
+        gather_coords(m, e, X);
+        element_D(m, e, state, 1, Dgp, &sym);
+        if (orc_element_stiffness(m->etype, X, Dgp, 36, sym, Ke)) { rc = -1 - e; break; }
+        for (int i = 0; i < nc; i++) {   /* FloatArray::beProductOf */
+            if (!loc[i]) continue;
+            double s = 0.0;
+            for (int j = 0; j < nc; j++) s += Ke[i * nc + j] * du[j];
+            f[loc[i] - 1] += s;
+        }
+    }
+    free(Ke);
+    free(Dgp);
+    return rc;
+}
+
+/* StaticStructural::solveYourselfAt staticstructural.C:224-358 with
+ * NRSolver::solve nrsolver.C:205-404 and the per-dof-id force criterion of
+ * checkConvergence nrsolver.C:594-885 (rtold off):
+ *   initial guess (initial_guess = 1, IG_Tangent, the default at
+ *   staticstructural.C:130,247-297): f = -sum_e K_e du_e over the prescribed
+ *   increments du = u_presc - u_presc_prev; if |f| > 0: K = tangent,
+ *   K inc = f (CG from zero), X += inc;
+ *   K = tangent of the state at step start (NonLinearLhs before the first
+ *   residual); for nite = 0..:
  *     F = f_int(X, prescribed); rhs = RT - F;
  *     err_d = sqrt(sum_{eq of dof d} rhs^2 / (sum RT^2 + eNorm_d)) (absolute
  *       when the denominator < 1e-6); converged when all err_d <= rtolf and
@@ -870,9 +910,11 @@ int orc_surface_load(const orc_mesh *m, int nface, const int *elem, const int *s
  *     X += ddX.
  * X (neq) is updated in place; state holds the Gauss-point state (temp
  * fields updated, committed fields untouched).  out[0] Newton iterations,
- * out[1] CG iterations, out[2] status (1 converged, -1 maxiter, -2 out of
- * range).  Returns the final max force error in *err. */
-int orc_nonlinear_step(const orc_mesh *m, double *state, double *X, const double *u_presc, const double *RT,
+ * out[1] CG iterations (initial guess included), out[2] status (1
+ * converged, -1 maxiter, -2 out of range).  Returns the final max force
+ * error in *err.  u_presc_prev may be NULL (all zero, the first step). */
+int orc_nonlinear_step(const orc_mesh *m, double *state, double *X, const double *u_presc,
+                       const double *u_presc_prev, int initial_guess, const double *RT,
                        double rtolf, int maxiter, int manrmsteps, double lstol, int *out, double *err)
 {
     const long long nnz = orc_csr_count(m);
@@ -887,8 +929,25 @@ int orc_nonlinear_step(const orc_mesh *m, double *state, double *X, const double
     for (long long i = 0; i < 3LL * m->nnode; i++)
         if (m->eqn[i] > 0) { dofid[m->eqn[i] - 1] = (int)(i % 3); inuse[i % 3]++; }
     orc_csr_fill(m, rowptr, colind);
-    if (orc_assemble_stiffness(m, rowptr, colind, val, state, 1)) { rc = -10; goto out; }
     out[0] = out[1] = out[2] = 0;
+    if (initial_guess && m->npeq > 0) {
+        double *du = (double *)malloc(sizeof(double) * (size_t)m->npeq);
+        for (int i = 0; i < m->npeq; i++) du[i] = u_presc[i] - (u_presc_prev ? u_presc_prev[i] : 0.0);
+        rc = orc_extrapolated_forces(m, state, du, rhs);
+        free(du);
+        if (rc) { rc = -12; goto out; }
+        double nrm = 0.0;
+        for (int i = 0; i < m->neq; i++) { rhs[i] = -rhs[i]; nrm += rhs[i] * rhs[i]; }
+        if (nrm > 0.0) {
+            if (orc_assemble_stiffness(m, rowptr, colind, val, state, 1)) { rc = -10; goto out; }
+            int it = 0;
+            double res = 0.0;
+            orc_pcg(m->neq, rowptr, colind, val, rhs, ddX, lstol, 1000000, 1, &it, &res);
+            out[1] += it;
+            for (int i = 0; i < m->neq; i++) { X[i] += ddX[i]; ddX[i] = 0.0; }
+        }
+    }
+    if (orc_assemble_stiffness(m, rowptr, colind, val, state, 1)) { rc = -10; goto out; }
     for (int nite = 0;; nite++) {
         double ebe[3];
         if (orc_internal_forces(m, X, u_presc, state, 1, F, NULL, ebe)) { rc = -11; goto out; }

@@ -99,8 +99,13 @@ void orc_state_commit(const orc_mesh *m, double *state);
 int orc_surface_load(const orc_mesh *m, int nface, const int *elem, const int *surf,
                      const double *traction, double *f_free, double *f_presc);
 
-/* StaticStructural step with NRSolver (per-dof-id force criterion) */
-int orc_nonlinear_step(const orc_mesh *m, double *state, double *X, const double *u_presc, const double *RT,
+/* f = sum_e K_e(tangent) du_e over prescribed increments (free rows) */
+int orc_extrapolated_forces(const orc_mesh *m, const double *state, const double *du_presc, double *f);
+
+/* StaticStructural step (optional tangent initial guess) with NRSolver
+ * (per-dof-id force criterion) */
+int orc_nonlinear_step(const orc_mesh *m, double *state, double *X, const double *u_presc,
+                       const double *u_presc_prev, int initial_guess, const double *RT,
                        double rtolf, int maxiter, int manrmsteps, double lstol, int *out, double *err);
 
 /* LinearStatic::solveYourselfAt + StructuralEngngModel::computeReaction */

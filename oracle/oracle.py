@@ -190,20 +190,37 @@ def surface_load(deck, factor=1.0):
     return f, fp
 
 
-def nonlinear_step(deck, state, X, u_presc, RT=None, rtolf=1e-8, maxiter=50, manrmsteps=1, lstol=1e-12):
+def nonlinear_step(deck, state, X, u_presc, RT=None, rtolf=1e-8, maxiter=50, manrmsteps=1, lstol=1e-12,
+                   u_presc_prev=None, initial_guess=True):
     """One StaticStructural step with NRSolver; X and state updated in place.
+    initial_guess: the tangent predictor over the prescribed increments
+    u_presc - u_presc_prev (IG_Tangent, the reference's default).
     Returns (newton_iterations, cg_iterations, status, force_error)."""
     m = Mesh(deck)
     out = (C.c_int * 3)()
     err = C.c_double(0)
     rt = _d(RT) if RT is not None else None
+    prev = _d(u_presc_prev) if u_presc_prev is not None else None
     rc = lib().orc_nonlinear_step(m.ref, state.ctypes.data_as(C.c_void_p), X.ctypes.data_as(C.c_void_p),
                                   _d(u_presc).ctypes.data_as(C.c_void_p),
+                                  prev.ctypes.data_as(C.c_void_p) if prev is not None else None,
+                                  int(bool(initial_guess)),
                                   rt.ctypes.data_as(C.c_void_p) if rt is not None else None,
                                   C.c_double(rtolf), maxiter, manrmsteps, C.c_double(lstol), out, C.byref(err))
     if rc:
         raise RuntimeError(f"orc_nonlinear_step rc={rc}")
     return out[0], out[1], out[2], err.value
+
+
+def extrapolated_forces(deck, state, du_presc):
+    """sum_e K_e(tangent) du_e over prescribed increments, free rows."""
+    m = Mesh(deck)
+    f = np.zeros(deck.neq)
+    rc = lib().orc_extrapolated_forces(m.ref, state.ctypes.data_as(C.c_void_p),
+                                       _d(du_presc).ctypes.data_as(C.c_void_p), f.ctypes.data_as(C.c_void_p))
+    if rc:
+        raise RuntimeError(f"orc_extrapolated_forces rc={rc}")
+    return f
 
 
 def linear_static(deck, tol=1e-10, maxit=100000):

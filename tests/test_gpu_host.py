@@ -73,11 +73,14 @@ def test_newton_tet_bar_matches_oracle_step_by_step():
     st = oracle.state_new(d)
     X = np.zeros(d.neq)
     nsteps = 5
+    prev = np.zeros(d.npeq)
     for k in range(1, nsteps + 1):
         up = d.prescribed_values(k / nsteps)
         P.set_prescribed(up)
         rep = P.nonlinear_step()
-        nit, cg, status, err = oracle.nonlinear_step(d, st, X, up, rtolf=1e-8, maxiter=40, manrmsteps=1, lstol=1e-13)
+        nit, cg, status, err = oracle.nonlinear_step(d, st, X, up, rtolf=1e-8, maxiter=40, manrmsteps=1, lstol=1e-13,
+                                                     u_presc_prev=prev)
+        prev = up
         assert rep["status"] == 1 and status == 1
         assert rep["newton"] == nit, (k, rep, nit)
         u = P.solution()
@@ -88,3 +91,27 @@ def test_newton_tet_bar_matches_oracle_step_by_step():
         P.update()
         oracle.state_commit(d, st)
     assert (st.reshape(-1, 38)[:, 6] > 0).mean() > 0.2      # the bar has yielded
+
+
+@pytest.mark.parametrize("tangent", [False, True])
+def test_tangent_product_matches_oracle(ctx, tangent):
+    """hfem_model_tangent_product_dev (sum_g B^T D B x_e dV) against the
+    oracle's element-matrix product K_e du_e over prescribed increments,
+    for the elastic and the plastic temp state."""
+    from oofem_amd.hipfem import BatchedEvaluator, HipCSRSparseMtrx
+    d = nlcases.tet_bar()
+    st = oracle.state_new(d)
+    X = np.zeros(d.neq)
+    up = d.prescribed_values(0.6)
+    if tangent:
+        nit, cg, status, err = oracle.nonlinear_step(d, st, X, up, rtolf=1e-10, maxiter=40, manrmsteps=1)
+        assert status == 1
+    ev = BatchedEvaluator(ctx, d)
+    K = HipCSRSparseMtrx(ctx)
+    ev.build_structure(K)
+    if tangent:
+        ev.internal_forces(X, up, update_state=True)   # temp state on the device from the same point
+    du = d.prescribed_values(1.0)
+    y, yp = ev.tangent_product(None, du, tangent=True)
+    ref = oracle.extrapolated_forces(d, st, du)
+    assert np.abs(y - ref).max() <= 1e-10 * np.abs(ref).max()

@@ -140,6 +140,15 @@ int hipcsr_set_spmv_stream(hipcsr_mtx *m, int rows_per_block, int unroll);
  * block layout); on by default for such patterns. */
 int hipcsr_set_spmv_nodeblock(hipcsr_mtx *m, int on);
 int hipcsr_is_nodeblocked(const hipcsr_mtx *m);
+/* element-by-element operator (no counterpart in the reference): after
+ * hfem_model_assemble_stiffness on an LSpace mesh the matrix remembers the
+ * evaluator, and while its values stay unchanged the FP32 PCG of precision
+ * 2 applies A as sum_e K_e x_e recomputed in FP32 instead of reading the
+ * stored values; on by default, off falls back to the FP32 copy. */
+int hipcsr_set_matrix_free(hipcsr_mtx *m, int on);
+int hipcsr_is_matrix_free(const hipcsr_mtx *m);
+/* y = A x through the element-by-element FP32 operator (host arrays) */
+int hipcsr_times_matrix_free(hipcsr_mtx *m, const double *x, double *y);
 /* device pointers of the CSR arrays (row pointers and columns are int32) */
 int hipcsr_device_arrays(hipcsr_mtx *m, const int32_t **rowptr, const int32_t **colind, double **val);
 
@@ -150,9 +159,18 @@ int  hippcg_create(hipfem_ctx *ctx, hippcg_solver **out);
 void hippcg_destroy(hippcg_solver *s);
 int  hippcg_set_options(hippcg_solver *s, double lstol, int lsiter, int lsprecond);
 /* precision: 0 = FP64 PCG; 1 = FP32 inner PCG inside FP64 iterative
- * refinement (true residual, update and dot products in FP64).
- * inner_tol: relative reduction asked of each inner solve. */
+ * refinement (true residual, update and dot products in FP64), inner_tol:
+ * relative reduction asked of each inner solve; 2 = one PCG on FP32
+ * vectors and matrix with FP64 solution and dot accumulation, the
+ * recursive residual replaced by the FP64 true residual b - A x whenever it
+ * has dropped by the reliable-update factor (hippcg_set_reliable, default
+ * 0.1) and convergence decided on the true residual. */
 int  hippcg_set_precision(hippcg_solver *s, int precision, double inner_tol, int max_outer);
+int  hippcg_set_reliable(hippcg_solver *s, double delta);
+/* 1 when the last precision-2 solve applied A element by element (A was the
+ * unchanged result of hfem_model_assemble_stiffness on LSpace), 0 when it
+ * used the FP32 copy of the stored values */
+int  hippcg_last_matrix_free(hippcg_solver *s);
 /* solve(A, b, x) (sparselinsystemnm.h:81; imlsolver.C:102-146): x holds the
  * initial guess on entry.  *reason receives HIPFEM_CR_CONVERGED or
  * HIPFEM_CR_DIVERGED_ITS; *iters the (inner) iterations; *achieved the final
@@ -161,7 +179,8 @@ int hippcg_solve(hippcg_solver *s, hipcsr_mtx *A, const double *b, double *x,
                  int *reason, int *iters, double *achieved);
 int hippcg_solve_dev(hippcg_solver *s, hipcsr_mtx *A, const double *db, double *dx,
                      int *reason, int *iters, double *achieved);
-/* outer iterations of the last mixed-precision solve */
+/* outer iterations (precision 1) or reliable updates (precision 2) of the
+ * last mixed-precision solve */
 int hippcg_last_outer(hippcg_solver *s);
 /* Instrumentation (no counterpart in the reference): when on, every PCG
  * SpMV launch of a solve is bracketed by HIP events on the context stream;

@@ -536,10 +536,61 @@ extern "C" void hipcsr_destroy(hipcsr_mtx *m)
     {
         DeviceGuard g(c->device);
         (void)hipStreamSynchronize(c->stream);
+        if (m->mf_model) model_unlink(m->mf_model, m);
         csr_free(m);
         delete m;
     }
     ctx_release(c);
+}
+
+extern "C" int hipcsr_set_matrix_free(hipcsr_mtx *m, int on)
+{
+    HIPFEM_REQUIRE(m, HIPFEM_EINVAL, "null matrix");
+    m->mf_enabled = on != 0;
+    return 0;
+}
+
+extern "C" int hipcsr_is_matrix_free(const hipcsr_mtx *m) { return (m && model_mf_available(m)) ? 1 : 0; }
+
+__global__ void k_d2f_vec(int n, const double *__restrict__ a, float *__restrict__ b)
+{
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) b[i] = (float)a[i];
+}
+
+__global__ void k_f2d_vec(int n, const float *__restrict__ a, double *__restrict__ b)
+{
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) b[i] = (double)a[i];
+}
+
+extern "C" int hipcsr_times_matrix_free(hipcsr_mtx *m, const double *x, double *y)
+{
+    HIPFEM_REQUIRE(m && x && y, HIPFEM_EINVAL, "times_matrix_free: null argument");
+    HIPFEM_REQUIRE(model_mf_available(m), HIPFEM_EINVAL,
+                   "times_matrix_free: the matrix has no current element-by-element view");
+    DeviceGuard g(m->ctx->device);
+    hipStream_t st = m->ctx->stream;
+    const int n = m->n;
+    double *dd = nullptr;
+    float *fx = nullptr, *fy = nullptr;
+    HIPFEM_CHECK(hipMalloc(&dd, sizeof(double) * std::max(1, n)));
+    HIPFEM_CHECK(hipMalloc(&fx, sizeof(float) * std::max(1, n)));
+    HIPFEM_CHECK(hipMalloc(&fy, sizeof(float) * std::max(1, n)));
+    int rc = 0;
+    const int grid = std::max(1, std::min((n + 255) / 256, 4096));
+    if (n) {
+        HIPFEM_CHECK(hipMemcpyAsync(dd, x, sizeof(double) * n, hipMemcpyHostToDevice, st));
+        k_d2f_vec<<<grid, 256, 0, st>>>(n, dd, fx);
+        rc = model_mf_apply_f32(m->mf_model, fx, fy, nullptr);
+        if (!rc) {
+            k_f2d_vec<<<grid, 256, 0, st>>>(n, fy, dd);
+            HIPFEM_CHECK(hipMemcpyAsync(y, dd, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+            HIPFEM_CHECK(hipStreamSynchronize(st));
+        }
+    }
+    HIPFEM_CHECK(hipFree(dd));
+    HIPFEM_CHECK(hipFree(fx));
+    HIPFEM_CHECK(hipFree(fy));
+    return rc;
 }
 
 extern "C" int hipcsr_clone(const hipcsr_mtx *src, hipcsr_mtx **out)

@@ -124,6 +124,21 @@ void ctx_retain(hipfem_ctx *c);
 void ctx_release(hipfem_ctx *c);
 }  // namespace hipfem
 
+struct hfem_model;
+
+namespace hipfem {
+// Device-resident scalars of one PCG solve.
+struct PcgState {
+    double rho, alpha, beta, normb, res, tol;
+    double rmax, delta;   // reliable updates: residual at the last update, trigger factor
+    int iter, done, bad, pause;
+};
+
+// kernels of an iteration return at once once the solve is done, or while it
+// is paused for a reliable update (the host then recomputes the residual)
+__device__ __forceinline__ bool halted(const PcgState *st) { return (st->done | st->pause) != 0; }
+}  // namespace hipfem
+
 struct hipcsr_mtx {
     hipfem_ctx *ctx = nullptr;
     int n = 0;
@@ -147,6 +162,13 @@ struct hipcsr_mtx {
     bool nodeblocked = false;
     int nbr = 0;                               // node rows = n / 3
     int32_t *d_bptr = nullptr, *d_bcol = nullptr;
+    // element-by-element view: set when the values were assembled by a batched
+    // evaluator and not changed since (mf_version == version); the FP32 PCG
+    // of precision 2 then applies the element operator instead of the stored
+    // values
+    hfem_model *mf_model = nullptr;
+    long mf_version = -1;
+    bool mf_enabled = true;
 };
 
 namespace hipfem {
@@ -159,4 +181,10 @@ int csr_refresh_fp32(hipcsr_mtx *m);
 int spmv_lanes(const hipcsr_mtx *m);
 int spmv_fp32_dev(hipcsr_mtx *m, const double *dx, double *dy);
 long next_pattern_id();
+// element-by-element operator of a linked evaluator: y = K_ff x (FP32
+// arithmetic, free equations only); returns at once inside kernels when the
+// PCG state is halted
+bool model_mf_available(const hipcsr_mtx *m);
+int model_mf_apply_f32(hfem_model *md, const float *x, float *y, const PcgState *st);
+void model_unlink(hfem_model *md, hipcsr_mtx *m);
 }  // namespace hipfem

@@ -548,6 +548,188 @@ __global__ __launch_bounds__(64) void k_internal(const int *__restrict__ elems, 
     }
 }
 
+// ------------------------------------------------- element-by-element operator
+// Per-GP tangent descriptor (FP32, SoA, 8 fields): the TangentStiffness D of
+// misesmat.C:496-544 written as D = c I_dev + K 1x1 - m m^T with
+// c = 2G + factor2 and m = sqrt(-factor1) s_trial (factor1 <= 0), so that
+// D eps = c dev(eps) + K tr(eps) 1 - m (m . eps); elastic points have c = 2G,
+// m = 0.  Fields: 0 c, 1 K, 2-7 m.
+__global__ void k_tangent_desc(int ngp_total, int ngp, const int *__restrict__ emat, const double *__restrict__ mat,
+                               const double *__restrict__ state, int mode, float *__restrict__ desc)
+{
+    const int gp = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gp >= ngp_total) return;
+    const double *mp = mat + (size_t)HIPFEM_MATP * emat[gp / ngp];
+    const double E = mp[1], nu = mp[2];
+    const double G = E / (2.0 * (1. + nu));
+    const double K = E / (3.0 * (1. - 2. * nu));
+    double c = 2.0 * G, m[6] = {0, 0, 0, 0, 0, 0};
+    if (mode == 1 && (int)mp[0] == HIPFEM_MAT_MISES) {
+        const double kn = state[(size_t)6 * ngp_total + gp], kt = state[(size_t)13 * ngp_total + gp];
+        const double dk = kt - kn;
+        if (dk > 0.0) {
+            double s[6];
+#pragma unroll
+            for (int i = 0; i < 6; i++) s[i] = state[(size_t)(14 + i) * ngp_total + gp];
+            const double trialS = sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2] + 2. * s[3] * s[3] +
+                                       2. * s[4] * s[4] + 2. * s[5] * s[5]);
+            const double sigmaY = mp[3] + mp[4] * kn;
+            const double factor = -2. * sqrt(6.) * G * G / trialS;
+            const double f1 = factor * sigmaY / ((mp[4] + 3. * G) * trialS * trialS);
+            c += factor * dk;
+            const double sc = sqrt(-f1);
+#pragma unroll
+            for (int i = 0; i < 6; i++) m[i] = sc * s[i];
+        }
+    }
+    desc[gp] = (float)c;
+    desc[(size_t)ngp_total + gp] = (float)K;
+#pragma unroll
+    for (int i = 0; i < 6; i++) desc[(size_t)(2 + i) * ngp_total + gp] = (float)m[i];
+}
+
+__global__ void k_fill_f(int n, float *__restrict__ a, const PcgState *__restrict__ st)
+{
+    if (st && halted(st)) return;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) a[i] = 0.0f;
+}
+
+// y += K_e x_e over the free equations of one color of LSpace elements, FP32.
+// 8 threads per element (one per Gauss point), 32 elements per block.  Node
+// data are staged in LDS relative to node 1 (coordinates differenced in FP64,
+// displacements in FP32: a rigid translation adds nothing to the strain), the
+// Gauss-point forces B^T sigma dV are summed over the points in ascending
+// order and each of the 8 threads scatters 3 of the 24 components.
+constexpr int MF_EPB = 32;
+__global__ __launch_bounds__(256) void k_mf_hex(const int *__restrict__ elems, int nel,
+                                                const double *__restrict__ coords, const int *__restrict__ conn,
+                                                const int *__restrict__ eqn, const int *__restrict__ emat,
+                                                const double *__restrict__ mat, const float *__restrict__ desc,
+                                                int ngp_total, const float *__restrict__ x, float *__restrict__ y,
+                                                const PcgState *__restrict__ st)
+{
+    __shared__ float sX[MF_EPB][8][3];
+    __shared__ float sU[MF_EPB][8][3];
+    __shared__ int sQ[MF_EPB][8][3];
+    __shared__ float sF[MF_EPB][8][25];
+    if (st && halted(st)) return;
+    const int t = threadIdx.x, le = t >> 3, g = t & 7;
+    const int ei = blockIdx.x * MF_EPB + le;
+    const bool active = ei < nel;
+    int e = 0;
+    if (active) {
+        e = elems[ei];
+        const int node = conn[(size_t)e * 8 + g], node0 = conn[(size_t)e * 8];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            sX[le][g][d] = (float)(coords[3 * (size_t)node + d] - coords[3 * (size_t)node0 + d]);
+            const int q = eqn[3 * (size_t)node + d];
+            sQ[le][g][d] = q;
+            sU[le][g][d] = q > 0 ? x[q - 1] : 0.0f;
+        }
+    }
+    __syncthreads();
+    if (active) {
+        const float a = 0.577350269189626f;
+        const float xi0 = (g & 4) ? a : -a, xi1 = (g & 2) ? a : -a, xi2 = (g & 1) ? a : -a;
+        float dN[8][3];
+        float J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        float gu[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};   // du_i / dxi_j
+        const float u0[3] = {sU[le][0][0], sU[le][0][1], sU[le][0][2]};
+#pragma unroll
+        for (int n = 0; n < 8; n++) {
+            const float sx = (n == 2 || n == 3 || n == 6 || n == 7) ? 1.f : -1.f;
+            const float sy = (n == 1 || n == 2 || n == 5 || n == 6) ? 1.f : -1.f;
+            const float sz = (n < 4) ? 1.f : -1.f;
+            dN[n][0] = sx * 0.125f * (1.f + sy * xi1) * (1.f + sz * xi2);
+            dN[n][1] = sy * 0.125f * (1.f + sx * xi0) * (1.f + sz * xi2);
+            dN[n][2] = sz * 0.125f * (1.f + sx * xi0) * (1.f + sy * xi1);
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                const float xv = sX[le][n][i], uv = sU[le][n][i] - u0[i];
+#pragma unroll
+                for (int j = 0; j < 3; j++) {
+                    J[3 * i + j] = fmaf(xv, dN[n][j], J[3 * i + j]);
+                    gu[3 * i + j] = fmaf(uv, dN[n][j], gu[3 * i + j]);
+                }
+            }
+        }
+        const float det = J[0] * (J[4] * J[8] - J[5] * J[7]) - J[1] * (J[3] * J[8] - J[5] * J[6]) +
+                          J[2] * (J[3] * J[7] - J[4] * J[6]);
+        const float id = 1.f / det;
+        float I[9];   // J^-1
+        I[0] = (J[4] * J[8] - J[5] * J[7]) * id;
+        I[1] = (J[2] * J[7] - J[1] * J[8]) * id;
+        I[2] = (J[1] * J[5] - J[2] * J[4]) * id;
+        I[3] = (J[5] * J[6] - J[3] * J[8]) * id;
+        I[4] = (J[0] * J[8] - J[2] * J[6]) * id;
+        I[5] = (J[2] * J[3] - J[0] * J[5]) * id;
+        I[6] = (J[3] * J[7] - J[4] * J[6]) * id;
+        I[7] = (J[1] * J[6] - J[0] * J[7]) * id;
+        I[8] = (J[0] * J[4] - J[1] * J[3]) * id;
+        // H = du/dx = gu J^-1
+        float H[9];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++)
+                H[3 * i + j] = gu[3 * i] * I[j] + gu[3 * i + 1] * I[3 + j] + gu[3 * i + 2] * I[6 + j];
+        const float eps[6] = {H[0], H[4], H[8], H[5] + H[7], H[2] + H[6], H[1] + H[3]};
+        const size_t gp = (size_t)e * 8 + g;
+        float c, Kb, m[6];
+        if (desc) {
+            c = desc[gp];
+            Kb = desc[(size_t)ngp_total + gp];
+#pragma unroll
+            for (int i = 0; i < 6; i++) m[i] = desc[(size_t)(2 + i) * ngp_total + gp];
+        } else {
+            const double *mp = mat + (size_t)HIPFEM_MATP * emat[e];
+            const float E = (float)mp[1], nu = (float)mp[2];
+            c = E / (1.f + nu);
+            Kb = E / (3.f * (1.f - 2.f * nu));
+#pragma unroll
+            for (int i = 0; i < 6; i++) m[i] = 0.f;
+        }
+        const float tr = eps[0] + eps[1] + eps[2];
+        const float me = m[0] * eps[0] + m[1] * eps[1] + m[2] * eps[2] + m[3] * eps[3] + m[4] * eps[4] + m[5] * eps[5];
+        const float p = Kb * tr, t3 = tr * (1.f / 3.f);
+        float sig[6];
+        sig[0] = c * (eps[0] - t3) + p - m[0] * me;
+        sig[1] = c * (eps[1] - t3) + p - m[1] * me;
+        sig[2] = c * (eps[2] - t3) + p - m[2] * me;
+        sig[3] = 0.5f * c * eps[3] - m[3] * me;
+        sig[4] = 0.5f * c * eps[4] - m[4] * me;
+        sig[5] = 0.5f * c * eps[5] - m[5] * me;
+        // T = dV sigma J^-T: f_n = T dN_n (dN/dx = J^-T dN/dxi in column form)
+        const float dV = fabsf(det);
+        const float S[9] = {sig[0], sig[5], sig[4], sig[5], sig[1], sig[3], sig[4], sig[3], sig[2]};
+        float T[9];
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+                T[3 * i + k] = dV * (S[3 * i] * I[3 * k] + S[3 * i + 1] * I[3 * k + 1] + S[3 * i + 2] * I[3 * k + 2]);
+#pragma unroll
+        for (int n = 0; n < 8; n++)
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+                sF[le][g][3 * n + i] = T[3 * i] * dN[n][0] + T[3 * i + 1] * dN[n][1] + T[3 * i + 2] * dN[n][2];
+    }
+    __syncthreads();
+    if (active) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const int cidx = g + 8 * k;   // component 3 n + i
+            const int q = sQ[le][cidx / 3][cidx % 3];
+            if (q <= 0) continue;
+            float f = 0.f;
+#pragma unroll
+            for (int gg = 0; gg < 8; gg++) f += sF[le][gg][cidx];
+            y[q - 1] += f;
+        }
+    }
+}
+
 __global__ void k_fill(int64_t n, double v, double *__restrict__ a)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) a[i] = v;
@@ -652,14 +834,24 @@ struct hfem_model {
     // host-API staging
     double *d_uf = nullptr, *d_up = nullptr, *d_ff = nullptr, *d_fp = nullptr, *d_out3 = nullptr;
     int8_t *d_dofid = nullptr;   // dof id (0 u, 1 v, 2 w) of every free equation
+    // element-by-element operator: per-GP FP32 tangent descriptors of the last
+    // assembly and the matrices that assembly linked
+    float *d_tdesc = nullptr;
+    bool tdesc_used = false;     // false: every point elastic, D from the material table
+    bool has_mises = false;
+    std::vector<hipcsr_mtx *> linked;
 };
 
 static int model_free(hfem_model *md)
 {
+    for (hipcsr_mtx *k : md->linked) k->mf_model = nullptr;
+    md->linked.clear();
     void *ps[] = {md->d_coords, md->d_mat, md->d_state, md->d_conn, md->d_emat, md->d_eqn, md->d_bad,
-                  md->d_eoff,   md->d_colors, md->d_uf, md->d_up, md->d_ff, md->d_fp, md->d_out3, md->d_dofid};
+                  md->d_eoff,   md->d_colors, md->d_uf, md->d_up, md->d_ff, md->d_fp, md->d_out3, md->d_dofid,
+                  md->d_tdesc};
     for (void *p : ps)
         if (p) HIPFEM_CHECK(hipFree(p));
+    md->d_tdesc = nullptr;
     md->d_coords = md->d_mat = md->d_state = md->d_uf = md->d_up = md->d_ff = md->d_fp = md->d_out3 = nullptr;
     md->d_conn = md->d_emat = md->d_eqn = md->d_bad = md->d_colors = nullptr;
     md->d_dofid = nullptr;
@@ -732,6 +924,9 @@ extern "C" int hfem_model_set_mesh(hfem_model *md, int nnode, const double *coor
     md->ngp_total = nelem * md->ngp;
     md->h_conn.assign(conn, conn + (size_t)nelem * nen);
     md->h_eqn.assign(eqn, eqn + (size_t)nnode * 3);
+    md->has_mises = false;
+    for (int k = 0; k < nmat; k++)
+        if ((int)mat[(size_t)k * HIPFEM_MATP] == HIPFEM_MAT_MISES) md->has_mises = true;
     hipStream_t st = md->ctx->stream;
     HIPFEM_CHECK(hipMalloc(&md->d_coords, sizeof(double) * 3 * nnode));
     HIPFEM_CHECK(hipMalloc(&md->d_conn, sizeof(int) * std::max<size_t>(1, (size_t)nelem * nen)));
@@ -951,8 +1146,59 @@ extern "C" int hfem_model_assemble_stiffness(hfem_model *md, hipcsr_mtx *K, int 
         HIPFEM_CHECK(hipGetLastError());
     }
     K->version++;
-    return check_bad(md, "assemble");
+    rc = check_bad(md, "assemble");
+    if (rc) return rc;
+    // element-by-element view of what was just assembled (LSpace): the FP32
+    // tangent descriptors of this state and a link from K to the model
+    if (md->etype == HIPFEM_LSPACE && md->ngp_total > 0) {
+        md->tdesc_used = md->has_mises && mode == 1;
+        if (md->tdesc_used) {
+            if (!md->d_tdesc) HIPFEM_CHECK(hipMalloc(&md->d_tdesc, sizeof(float) * 8 * (size_t)md->ngp_total));
+            k_tangent_desc<<<(md->ngp_total + 255) / 256, 256, 0, st>>>(md->ngp_total, md->ngp, md->d_emat,
+                                                                          md->d_mat, md->d_state, mode, md->d_tdesc);
+            HIPFEM_CHECK(hipGetLastError());
+        }
+        if (K->mf_model && K->mf_model != md) model_unlink(K->mf_model, K);
+        // the descriptors now describe this assembly only
+        const std::vector<hipcsr_mtx *> others = md->linked;
+        for (hipcsr_mtx *k : others)
+            if (k != K) model_unlink(md, k);
+        if (md->linked.empty()) md->linked.push_back(K);
+        K->mf_model = md;
+        K->mf_version = K->version;
+    }
+    return 0;
 }
+
+namespace hipfem {
+bool model_mf_available(const hipcsr_mtx *m)
+{
+    return m->mf_enabled && m->mf_model && m->mf_version == m->version && m->mf_model->pattern_id == m->pattern_id;
+}
+
+void model_unlink(hfem_model *md, hipcsr_mtx *m)
+{
+    md->linked.erase(std::remove(md->linked.begin(), md->linked.end(), m), md->linked.end());
+    m->mf_model = nullptr;
+    m->mf_version = -1;
+}
+
+int model_mf_apply_f32(hfem_model *md, const float *x, float *y, const PcgState *st)
+{
+    hipStream_t s = md->ctx->stream;
+    if (md->neq) k_fill_f<<<std::min(4096, (md->neq + 255) / 256), 256, 0, s>>>(md->neq, y, st);
+    for (int c = 0; c < md->ncolors; c++) {
+        const int n = md->color_ptr[c + 1] - md->color_ptr[c];
+        if (!n) continue;
+        k_mf_hex<<<(n + MF_EPB - 1) / MF_EPB, 256, 0, s>>>(md->d_colors + md->color_ptr[c], n, md->d_coords,
+                                                           md->d_conn, md->d_eqn, md->d_emat, md->d_mat,
+                                                           md->tdesc_used ? md->d_tdesc : nullptr, md->ngp_total, x,
+                                                           y, st);
+    }
+    HIPFEM_CHECK(hipGetLastError());
+    return 0;
+}
+}  // namespace hipfem
 
 static int internal_dev(hfem_model *md, const double *uf, const double *up, double *ff, double *fp, double *ebe3,
                         int update, int op = 0)

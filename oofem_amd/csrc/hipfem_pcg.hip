@@ -23,6 +23,11 @@
 // precision 1: FP32 matrix/vectors for an inner PCG (FP64 dot
 // accumulation) inside FP64 iterative refinement; the true residual
 // r = b - A x and the update x += d are FP64.
+// precision 2: one CG on FP32 vectors and the FP32 matrix with FP64 x and
+// FP64 dot accumulation; the recursive residual is replaced by the FP64
+// true residual whenever it has dropped by a factor delta (reliable
+// updates), keeping the Krylov direction, and convergence is decided on
+// the true residual.
 
 #include <algorithm>
 #include <cmath>
@@ -33,10 +38,6 @@
 
 namespace hipfem {
 
-struct PcgState {
-    double rho, alpha, beta, normb, res, tol;
-    int iter, done, bad, pad;
-};
 
 constexpr int RED_THREADS = 1024;
 constexpr int MAX_GRID = 4096;
@@ -75,7 +76,7 @@ __global__ __launch_bounds__(256) void k_pcg_spmv_fused(int n, const int32_t *__
                                                         int first)
 {
     __shared__ double sh[4];
-    if (st->done) return;
+    if (halted(st)) return;
     const VT beta = first ? VT(0) : (VT)st->beta;
     const int rows_per_block = 256 / LANES;
     const int lane = threadIdx.x % LANES;
@@ -108,7 +109,7 @@ __global__ __launch_bounds__(256) void k_pcg_spmv_p(int n, const int32_t *__rest
                                                     const PcgState *__restrict__ st, double *__restrict__ partials)
 {
     __shared__ double sh[4];
-    if (st->done) return;
+    if (halted(st)) return;
     const int rows_per_block = 256 / LANES;
     const int lane = threadIdx.x % LANES;
     double acc = 0.0;
@@ -135,7 +136,7 @@ __global__ __launch_bounds__(256) void k_pcg_spmv_p_stream(int n, const int32_t 
     __shared__ VT prod[STREAM_CAP];
     __shared__ int rp[RPB + 1];
     __shared__ double sh[4];
-    if (st->done) return;
+    if (halted(st)) return;
     constexpr int LPR = 256 / RPB;
     double acc = 0.0;
     for (int r0 = blockIdx.x * RPB; r0 < n; r0 += gridDim.x * RPB) {
@@ -158,7 +159,7 @@ __global__ __launch_bounds__(256) void k_pcg_spmv_p_nb(int nbr, const int32_t *_
                                                        const PcgState *__restrict__ st, double *__restrict__ partials)
 {
     __shared__ double sh[4];
-    if (st->done) return;
+    if (halted(st)) return;
     const int lane = threadIdx.x % 32;
     double acc = 0.0;
     for (int I = blockIdx.x * 8 + threadIdx.x / 32; I < nbr; I += gridDim.x * 8) {
@@ -189,7 +190,7 @@ template <typename VT>
 __global__ __launch_bounds__(256) void k_pcg_pupdate(int n, const VT *__restrict__ z, VT *__restrict__ p,
                                                      const PcgState *__restrict__ st, int first)
 {
-    if (st->done) return;
+    if (halted(st)) return;
     const VT beta = first ? VT(0) : (VT)st->beta;
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256)
         p[i] = first ? z[i] : fma(beta, p[i], z[i]);
@@ -202,7 +203,7 @@ __global__ __launch_bounds__(256) void k_pcg_update(int n, XT *__restrict__ x, V
                                                     double *__restrict__ partials)
 {
     __shared__ double sh[4];
-    if (st->done) return;
+    if (halted(st)) return;
     const double alpha = st->alpha;
     const VT a = (VT)alpha;
     double rz = 0.0, rr = 0.0;
@@ -276,6 +277,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_reduce_init(PcgState *st, const
         st->beta = 0.0;
         st->iter = 0;
         st->bad = 0;
+        st->pause = 0;
         st->done = (st->res <= tol) ? 1 : 0;
     }
 }
@@ -283,7 +285,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_reduce_init(PcgState *st, const
 __global__ __launch_bounds__(RED_THREADS) void k_reduce_alpha(PcgState *st, const double *__restrict__ partials, int nb)
 {
     __shared__ double sh[RED_THREADS / 64];
-    if (st->done) return;
+    if (halted(st)) return;
     const double pq = reduce_partials(partials, nb, sh);
     if (threadIdx.x == 0) st->alpha = st->rho / pq;
 }
@@ -292,7 +294,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_reduce_beta(PcgState *st, const
                                                              int iter)
 {
     __shared__ double sh[RED_THREADS / 64];
-    if (st->done) return;
+    if (halted(st)) return;
     const double rz = reduce_partials(partials, nb, sh);
     const double rr = reduce_partials(partials + nb, nb, sh);
     if (threadIdx.x == 0) {
@@ -342,12 +344,12 @@ __global__ void k_add_f2d(int n, const float *__restrict__ d, double *__restrict
 
 // r = b - q (FP64), partial r.r, b.b
 __global__ __launch_bounds__(256) void k_residual(int n, const double *__restrict__ b, const double *__restrict__ q,
-                                                  double *__restrict__ r, double *__restrict__ partials)
+                                                  double *__restrict__ r, double *__restrict__ partials, int x_zero)
 {
     __shared__ double sh[4];
     double rr = 0.0, bb = 0.0;
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
-        const double ri = b[i] - q[i];
+        const double ri = x_zero ? b[i] : b[i] - q[i];
         r[i] = ri;
         rr += ri * ri;
         bb += b[i] * b[i];
@@ -368,6 +370,114 @@ __global__ __launch_bounds__(RED_THREADS) void k_reduce2(const double *__restric
     if (threadIdx.x == 0) {
         out[0] = a;
         out[1] = b;
+    }
+}
+
+// ------------------------------------------------------ reliable updates
+// Mixed-precision CG with reliable updates (Sleijpen & van der Vorst 1996;
+// the GPU form of Clark et al. 2010): the iteration runs on FP32 vectors and
+// an FP32 operator with FP64 dot accumulation and an FP64 solution x; when
+// the recursive residual has dropped by the factor delta since the last
+// update (or meets the tolerance), the FP64 true residual r = b - A x
+// replaces it and the search direction is kept.
+
+// true residual norm after k_residual; init also sets |b| and the counters
+__global__ __launch_bounds__(RED_THREADS) void k_reduce_true(PcgState *st, const double *__restrict__ partials, int nb,
+                                                             double tol, double delta, int init)
+{
+    __shared__ double sh[RED_THREADS / 64];
+    if (st->done && !init) return;
+    const double rr = reduce_partials(partials, nb, sh);
+    const double bb = reduce_partials(partials + nb, nb, sh);
+    if (threadIdx.x == 0) {
+        if (init) {
+            double normb = sqrt(bb);
+            if (normb == 0.0) normb = 1;
+            st->normb = normb;
+            st->tol = tol;
+            st->delta = delta;
+            st->alpha = st->beta = 0.0;
+            st->iter = 0;
+            st->bad = 0;
+            st->done = 0;
+        }
+        const double res = sqrt(rr) / st->normb;
+        st->res = res;
+        if (res <= st->tol) st->done = 1;
+        if (!(res == res)) {
+            st->done = 1;
+            st->bad = 1;
+        }
+    }
+}
+
+// r32 = (float) r, z32 = D^-1 r32; partials r32.z32
+__global__ __launch_bounds__(256) void k_rel_refresh(int n, const double *__restrict__ r, float *__restrict__ r32,
+                                                     float *__restrict__ z32, const float *__restrict__ dinv32,
+                                                     const PcgState *__restrict__ st, double *__restrict__ partials)
+{
+    __shared__ double sh[4];
+    if (st->done) return;
+    double rz = 0.0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        const float ri = (float)r[i];
+        const float zi = dinv32[i] * ri;
+        r32[i] = ri;
+        z32[i] = zi;
+        rz += (double)ri * (double)zi;
+    }
+    const double t = block_sum<256>(rz, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+// partials of p.q after an operator that does not produce them
+__global__ __launch_bounds__(256) void k_dot_pq(int n, const float *__restrict__ p, const float *__restrict__ q,
+                                                const PcgState *__restrict__ st, double *__restrict__ partials)
+{
+    __shared__ double sh[4];
+    if (halted(st)) return;
+    double acc = 0.0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) acc += (double)p[i] * (double)q[i];
+    const double t = block_sum<256>(acc, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+// after a refresh: beta from the true residual (the direction is kept), rmax
+__global__ __launch_bounds__(RED_THREADS) void k_reduce_refresh(PcgState *st, const double *__restrict__ partials,
+                                                                int nb, int first)
+{
+    __shared__ double sh[RED_THREADS / 64];
+    if (st->done) return;
+    const double rz = reduce_partials(partials, nb, sh);
+    if (threadIdx.x == 0) {
+        st->beta = first ? 0.0 : rz / st->rho;
+        st->rho = rz;
+        st->rmax = st->res;
+        st->pause = 0;
+    }
+}
+
+// convergence / reliable-update test of one FP32 iteration
+__global__ __launch_bounds__(RED_THREADS) void k_reduce_beta_rel(PcgState *st, const double *__restrict__ partials,
+                                                                 int nb, int iter)
+{
+    __shared__ double sh[RED_THREADS / 64];
+    if (halted(st)) return;
+    const double rz = reduce_partials(partials, nb, sh);
+    const double rr = reduce_partials(partials + nb, nb, sh);
+    if (threadIdx.x == 0) {
+        const double res = sqrt(rr) / st->normb;
+        st->iter = iter;
+        st->res = res;
+        if (!(res == res)) {
+            st->done = 1;
+            st->bad = 1;
+        } else if (res <= st->tol || res < st->delta * st->rmax) {
+            st->pause = 1;   // the host replaces r by the true residual
+        } else {
+            st->beta = rz / st->rho;
+            st->rho = rz;
+        }
     }
 }
 
@@ -416,6 +526,9 @@ struct hippcg_solver {
     int last_outer = 0;
     int check_every = 16;
     int fused = 0;   // 1: p formed in the SpMV gather; 0: separate in-place p update (faster on gfx950)
+    double delta = 0.1;   // precision 2: reliable update when |r| < delta |r at last update|
+    int last_updates = 0;
+    int last_mf = 0;      // the last precision-2 solve used the element-by-element operator
     // preconditioner cache (IMLSolver re-inits on a new matrix or version, imlsolver.C:112-114)
     const hipcsr_mtx *lhs = nullptr;
     long lhs_version = -1;
@@ -604,6 +717,110 @@ static int pcg_run(hippcg_solver *s, hipcsr_mtx *A, const MT *val, const VT *b, 
     return 0;
 }
 
+// precision 2: CG on FP32 vectors and the FP32 copy of A, FP64 x and dot
+// accumulation, reliable updates with the FP64 true residual b - A x.
+static int pcg_reliable(hippcg_solver *s, hipcsr_mtx *A, const double *db, double *dx, int maxit, int *iters,
+                        double *res, int *converged)
+{
+    hipStream_t st = s->ctx->stream;
+    const int n = A->n;
+    // inner operator: the element-by-element FP32 form of A when A is the
+    // unchanged result of a batched assembly, else the FP32 copy of A
+    const bool mf = model_mf_available(A);
+    int rc = mf ? 0 : csr_refresh_fp32(A);
+    if (rc) return rc;
+    s->last_mf = mf ? 1 : 0;
+    const SpmvShape sh = A->shape;
+    const int gv = vec_grid(n);
+    const bool nodeblock = sh.nodeblock && A->nodeblocked;
+    const int gs = nodeblock    ? std::max(1, std::min((A->nbr + 7) / 8, MAX_GRID))
+                   : sh.stream ? std::max(1, std::min((n + sh.rpb - 1) / sh.rpb, MAX_GRID))
+                               : spmv_grid(n, sh.lanes);
+    rc = ensure_partials(s->ctx, 3 * (size_t)std::max(gv, gs));
+    if (rc) return rc;
+    double *part = s->ctx->d_partials;
+    // r = b - A x in FP64, then the FP32 residual; init = 1 sets |b|
+    auto true_residual = [&](int init) -> int {
+        int e = csr_spmv_dev(A, dx, s->q);
+        if (e) return e;
+        k_residual<<<gv, 256, 0, st>>>(n, db, s->q, s->r, part, 0);
+        k_reduce_true<<<1, RED_THREADS, 0, st>>>(s->st, part, gv, s->tol, s->delta, init);
+        k_rel_refresh<<<gv, 256, 0, st>>>(n, s->r, s->fr, s->fz, s->dinv32, s->st, part);
+        k_reduce_refresh<<<1, RED_THREADS, 0, st>>>(s->st, part, gv, init);
+        HIPFEM_CHECK(hipGetLastError());
+        return 0;
+    };
+    rc = true_residual(1);
+    if (rc) return rc;
+    PcgState h;
+    rc = read_state(s, &h);
+    if (rc) return rc;
+    int it = 1, chunk = s->check_every;
+    bool first = true;
+    const bool prof = s->profiling != 0;
+    s->last_updates = 0;
+    while (!h.done && it <= maxit) {
+        const int stop = std::min(maxit, it + chunk - 1);
+        if (prof) {
+            rc = ensure_events(s, (size_t)stop);
+            if (rc) return rc;
+        }
+        for (int k = it; k <= stop; k++) {
+            k_pcg_pupdate<float><<<gv, 256, 0, st>>>(n, s->fz, s->fp0, s->st, first ? 1 : 0);
+            first = false;
+            if (prof) HIPFEM_CHECK(hipEventRecord(s->ev[2 * (k - 1)], st));
+            if (mf) {
+                rc = model_mf_apply_f32(A->mf_model, s->fp0, s->fq, s->st);
+                if (rc) return rc;
+                k_dot_pq<<<gv, 256, 0, st>>>(n, s->fp0, s->fq, s->st, part);
+            } else if (nodeblock) {
+                k_pcg_spmv_p_nb<float, float><<<gs, 256, 0, st>>>(A->nbr, A->d_bptr, A->d_bcol, A->d_val32, s->fp0,
+                                                                  s->fq, s->st, part);
+            } else {
+                launch_pcg_spmv_p<float, float>(sh, gs, st, n, A->d_rowptr, A->d_colind, A->d_val32, s->fp0, s->fq,
+                                                s->st, part);
+            }
+            if (prof) HIPFEM_CHECK(hipEventRecord(s->ev[2 * (k - 1) + 1], st));
+            k_reduce_alpha<<<1, RED_THREADS, 0, st>>>(s->st, part, mf ? gv : gs);
+            k_pcg_update<double, float><<<gv, 256, 0, st>>>(n, dx, s->fr, s->fz, s->fp0, s->fq, s->dinv32, s->st,
+                                                            part);
+            k_reduce_beta_rel<<<1, RED_THREADS, 0, st>>>(s->st, part, gv, k);
+        }
+        HIPFEM_CHECK(hipGetLastError());
+        rc = read_state(s, &h);
+        if (rc) return rc;
+        if (h.done) break;
+        if (h.pause) {
+            // iterations after h.iter in this chunk returned at once; resume after it
+            rc = true_residual(0);
+            if (rc) return rc;
+            rc = read_state(s, &h);
+            if (rc) return rc;
+            s->last_updates++;
+            it = h.iter + 1;
+            continue;
+        }
+        it = stop + 1;
+        chunk = std::min(chunk * 2, 64);
+    }
+    *iters = h.iter;
+    *res = h.res;
+    *converged = (h.done && !h.bad && h.res <= s->tol) ? 1 : 0;
+    if (prof) {
+        HIPFEM_CHECK(hipStreamSynchronize(st));
+        double tot = 0.0;
+        for (int k = 0; k < h.iter; k++) {
+            float ms = 0.f;
+            HIPFEM_CHECK(hipEventElapsedTime(&ms, s->ev[2 * k], s->ev[2 * k + 1]));
+            tot += ms;
+        }
+        s->prof_n32 += h.iter;
+        s->prof_ms32 += tot;
+    }
+    HIPFEM_REQUIRE(!h.bad, HIPFEM_ENUMERIC, "pcg: residual became NaN at iteration %d", h.iter);
+    return 0;
+}
+
 static int solve_dev(hippcg_solver *s, hipcsr_mtx *A, const double *db, double *dx, int *reason, int *iters,
                      double *achieved)
 {
@@ -628,6 +845,10 @@ static int solve_dev(hippcg_solver *s, hipcsr_mtx *A, const double *db, double *
         rc = pcg_run<double, double, double>(s, A, A->d_val, db, dx, 0, s->tol, s->maxit, s->r, s->z, s->p0, s->p1,
                                               s->q, s->dinv, &it, &res, &conv);
         if (rc) return rc;
+    } else if (s->precision == 2) {
+        rc = pcg_reliable(s, A, db, dx, s->maxit, &it, &res, &conv);
+        if (rc) return rc;
+        s->last_outer = s->last_updates;
     } else {
         // FP64 iterative refinement around an FP32 inner PCG
         rc = csr_refresh_fp32(A);
@@ -641,7 +862,7 @@ static int solve_dev(hippcg_solver *s, hipcsr_mtx *A, const double *db, double *
         for (int outer = 0; outer < s->max_outer; outer++) {
             rc = csr_spmv_dev(A, dx, s->q);
             if (rc) return rc;
-            k_residual<<<gv, 256, 0, st>>>(n, db, s->q, s->r, s->ctx->d_partials);
+            k_residual<<<gv, 256, 0, st>>>(n, db, s->q, s->r, s->ctx->d_partials, 0);
             k_reduce2<<<1, RED_THREADS, 0, st>>>(s->ctx->d_partials, gv, out);
             HIPFEM_CHECK(hipGetLastError());
             double h2[2];
@@ -715,7 +936,7 @@ extern "C" int hippcg_set_options(hippcg_solver *s, double lstol, int lsiter, in
 extern "C" int hippcg_set_precision(hippcg_solver *s, int precision, double inner_tol, int max_outer)
 {
     HIPFEM_REQUIRE(s, HIPFEM_EINVAL, "null solver");
-    HIPFEM_REQUIRE(precision == 0 || precision == 1, HIPFEM_EINVAL, "precision must be 0 or 1");
+    HIPFEM_REQUIRE(precision >= 0 && precision <= 2, HIPFEM_EINVAL, "precision must be 0, 1 or 2");
     s->precision = precision;
     if (inner_tol > 0) s->inner_tol = inner_tol;
     if (max_outer > 0) s->max_outer = max_outer;
@@ -723,6 +944,16 @@ extern "C" int hippcg_set_precision(hippcg_solver *s, int precision, double inne
 }
 
 extern "C" int hippcg_last_outer(hippcg_solver *s) { return s ? s->last_outer : 0; }
+
+extern "C" int hippcg_last_matrix_free(hippcg_solver *s) { return s ? s->last_mf : 0; }
+
+extern "C" int hippcg_set_reliable(hippcg_solver *s, double delta)
+{
+    HIPFEM_REQUIRE(s, HIPFEM_EINVAL, "null solver");
+    HIPFEM_REQUIRE(delta > 0.0 && delta < 1.0, HIPFEM_EINVAL, "reliable-update factor must be in (0, 1)");
+    s->delta = delta;
+    return 0;
+}
 
 extern "C" int hippcg_set_fused(hippcg_solver *s, int fused)
 {

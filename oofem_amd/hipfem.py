@@ -47,6 +47,7 @@ def lib():
         L.hipcsr_add_diagonal.argtypes = [vp, dp, vp]
         L.hippcg_set_options.argtypes = [vp, dp, ip, ip]
         L.hippcg_set_precision.argtypes = [vp, ip, dp, ip]
+        L.hippcg_set_reliable.argtypes = [vp, dp]
         for f in ("hipfem_ctx_destroy", "hipcsr_destroy", "hippcg_destroy", "hfem_model_destroy"):
             getattr(L, f).restype = None
             getattr(L, f).argtypes = [vp]
@@ -291,6 +292,18 @@ class HipCSRSparseMtrx:
     def set_spmv_stream(self, rows_per_block, unroll=4):
         _check(lib().hipcsr_set_spmv_stream(self.h, int(rows_per_block), int(unroll)), "set_spmv_stream")
 
+    def set_matrix_free(self, on=True):
+        _check(lib().hipcsr_set_matrix_free(self.h, int(bool(on))), "set_matrix_free")
+
+    def is_matrix_free(self):
+        return bool(lib().hipcsr_is_matrix_free(self.h))
+
+    def times_matrix_free(self, x):
+        x = _f64(x)
+        y = np.zeros(self.giveNumberOfRows())
+        _check(lib().hipcsr_times_matrix_free(self.h, _p(x), _p(y)), "times_matrix_free")
+        return y
+
     def set_spmv_nodeblock(self, on=True):
         _check(lib().hipcsr_set_spmv_nodeblock(self.h, int(bool(on))), "set_spmv_nodeblock")
 
@@ -309,16 +322,19 @@ class HipCSRSparseMtrx:
 class HipPCGSolver:
     """SparseLinearSystemNM 'hippcg' with IMLSolver options (imlsolver.C:67-101)."""
 
-    def __init__(self, ctx: Context, lstol=1e-5, lsiter=200, lsprecond=1, precision=0, inner_tol=1e-4, max_outer=100):
+    def __init__(self, ctx: Context, lstol=1e-5, lsiter=200, lsprecond=1, precision=0, inner_tol=1e-4, max_outer=100,
+                 reldelta=0.1):
         self.ctx = ctx
         self.h = C.c_void_p()
         _check(lib().hippcg_create(ctx.h, C.byref(self.h)), "hippcg_create")
         self.initializeFrom(lstol=lstol, lsiter=lsiter, lsprecond=lsprecond, precision=precision,
-                            inner_tol=inner_tol, max_outer=max_outer)
+                            inner_tol=inner_tol, max_outer=max_outer, reldelta=reldelta)
 
-    def initializeFrom(self, lstol=1e-5, lsiter=200, lsprecond=1, precision=0, inner_tol=1e-4, max_outer=100):
+    def initializeFrom(self, lstol=1e-5, lsiter=200, lsprecond=1, precision=0, inner_tol=1e-4, max_outer=100,
+                       reldelta=0.1):
         _check(lib().hippcg_set_options(self.h, float(lstol), int(lsiter), int(lsprecond)), "initializeFrom")
         _check(lib().hippcg_set_precision(self.h, int(precision), float(inner_tol), int(max_outer)), "initializeFrom")
+        _check(lib().hippcg_set_reliable(self.h, float(reldelta)), "initializeFrom")
 
     def close(self):
         if self.h:
@@ -349,6 +365,9 @@ class HipPCGSolver:
 
     def last_outer(self):
         return lib().hippcg_last_outer(self.h)
+
+    def last_matrix_free(self):
+        return bool(lib().hippcg_last_matrix_free(self.h))
 
     def set_fused(self, fused=True):
         _check(lib().hippcg_set_fused(self.h, int(bool(fused))), "set_fused")

@@ -21,6 +21,7 @@ void HipPCGSolver::applyOptions()
 {
     hipfemCheck(hippcg_set_options(s, tol, maxite, precond), "hippcg_set_options");
     hipfemCheck(hippcg_set_precision(s, mixed, irtol, 0), "hippcg_set_precision");
+    hipfemCheck(hippcg_set_reliable(s, reldelta), "hippcg_set_reliable");
 }
 
 // IMLSolver::initializeFrom imlsolver.C:67-101 keywords and defaults
@@ -34,6 +35,7 @@ void HipPCGSolver::initializeFrom(InputRecord &ir)
     if (precond != 0 && precond != 1) OOFEM_ERROR("unknown preconditioner type %d", precond);
     mixed = (int)ir.get("mixed", 0);
     irtol = ir.get("irtol", 1e-4);
+    reldelta = ir.get("reldelta", 0.1);
     applyOptions();
 }
 

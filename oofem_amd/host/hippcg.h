@@ -2,7 +2,9 @@
 // interface sparselinsystemnm.h:56-97).  Reads the IMLSolver keywords with
 // IML semantics (imlsolver.C:67-101: lstol default 1e-5, lsiter default 200,
 // lsprecond 0 none / 1 diagonal) plus `mixed` (0 FP64 PCG, 1 FP32 inner PCG
-// in FP64 iterative refinement) and `irtol` (inner relative tolerance).
+// in FP64 iterative refinement, 2 FP32 PCG with FP64 reliable updates),
+// `irtol` (inner relative tolerance of 1) and `reldelta` (reliable-update
+// factor of 2, default 0.1).
 #pragma once
 
 #include "hipcsr.h"
@@ -17,6 +19,7 @@ class HipPCGSolver : public SparseLinearSystemNM {
     int precond = 0;
     int mixed = 0;
     double irtol = 1e-4;
+    double reldelta = 0.1;
     int lastIterations = 0;
     double lastResidual = 0.0;
 

@@ -284,3 +284,67 @@ def test_mixed_precision_refinement(ctx):
     assert got["outer"] >= 2
     assert got["resid"] <= 1e-10
     assert np.linalg.norm(got["u"] - ref["u"]) / np.linalg.norm(ref["u"]) < 1e-8
+
+
+@pytest.mark.parametrize("operator", ["element", "nodeblock", "csr"])
+def test_mixed_precision_reliable_updates(ctx, operator):
+    """precision 2: FP32 PCG with FP64 reliable updates, with the element-by-
+    element FP32 operator or the FP32 copy of the matrix (node-block and
+    scalar CSR).  Convergence is decided on the FP64 true residual, so the
+    returned residual is the true one, and the solution agrees with the FP64
+    solve to the tolerance's level; iterations stay close to the FP64 count
+    because the Krylov direction is kept across updates."""
+    from oofem_amd.drivers import LinearStaticProblem
+    d = meshgen.cantilever_hex(8, 8, 24, jitter=0.2, e_sigma=0.5, seed=3)
+    ref = LinearStaticProblem(ctx, d, lstol=1e-12).solve()
+    P = LinearStaticProblem(ctx, d, lstol=1e-10, precision=2)
+    P.K.set_matrix_free(operator == "element")
+    P.K.set_spmv_nodeblock(operator == "nodeblock")
+    got = P.solve()
+    assert P.solver.last_matrix_free() == (operator == "element")
+    assert got["resid"] <= 1e-10
+    assert got["outer"] >= 2                       # reliable updates happened
+    # true residual of the returned solution, FP64, through the oracle's SpMV
+    rp, ci, val = oracle.csr(d)
+    fint, _, _ = oracle.internal_forces(d, np.zeros(d.neq), d.prescribed_values())
+    rhs = d.external_load()[0] - fint
+    r = rhs - oracle.spmv(rp, ci, val, got["u"])
+    assert np.linalg.norm(r) <= 1.0001e-10 * np.linalg.norm(rhs)
+    assert np.linalg.norm(got["u"] - ref["u"]) / np.linalg.norm(ref["u"]) < 1e-8
+    it64 = LinearStaticProblem(ctx, d, lstol=1e-10).solve()["iters"]
+    assert got["iters"] <= 1.25 * it64, (got["iters"], it64)
+
+
+@pytest.mark.parametrize("plastic", [False, True])
+def test_element_operator_matches_assembled_matrix(ctx, plastic):
+    """The FP32 element-by-element operator equals the assembled FP64 matrix
+    to FP32 accuracy: elastic (IsoLE, log-normal E) and the MisesMat
+    consistent tangent of a plastic state (descriptor form c I_dev + K 1x1 -
+    m m^T of the tangent)."""
+    from oofem_amd.hipfem import BatchedEvaluator, HipCSRSparseMtrx
+    from tests import nlcases
+    if plastic:
+        d = meshgen.cantilever_hex(4, 4, 6, jitter=0.2, seed=5)
+        d.mat = nlcases.MISES.copy()
+        d.emat[:] = 0
+    else:
+        d = meshgen.cantilever_hex(6, 6, 10, jitter=0.2, e_sigma=0.5, seed=4)
+    ev = BatchedEvaluator(ctx, d)
+    K = HipCSRSparseMtrx(ctx)
+    ev.build_structure(K)
+    if plastic:
+        rng = np.random.default_rng(0)
+        u = rng.standard_normal(d.neq) * 3e-3       # strains well past yield
+        ev.internal_forces(u, np.zeros(d.npeq), update_state=True)
+        s = ev.get_state()
+        assert (s[:, 13] > s[:, 6]).mean() > 0.3
+    ev.assemble_stiffness(K, tangent=plastic)
+    assert K.is_matrix_free()
+    rng = np.random.default_rng(1)
+    for _ in range(3):
+        x = rng.standard_normal(d.neq)
+        y64 = K.times(x)
+        y32 = K.times_matrix_free(x)
+        assert np.linalg.norm(y32 - y64) <= 2e-5 * np.linalg.norm(y64)
+    K.zero()                                          # any change of the values drops the view
+    assert not K.is_matrix_free()

@@ -77,13 +77,13 @@ def run_kernels(args, deck):
     db = DeviceVector(ctx, deck.neq, f)
     dx = DeviceVector(ctx, deck.neq)
     out = {}
-    for prec in (0, 1):
+    for prec, name in ((0, "fp64_nodeblock"), (1, "fp32_nodeblock"), (2, "fp32_element_operator")):
         s = HipPCGSolver(ctx, lstol=1e-30, lsiter=args.iters, lsprecond=1, precision=prec, inner_tol=1e-30)
         s.set_profiling(True)
         dx.fill(0.0)
         s.solve_dev(K, db.addr, dx.addr)
         p = s.profile()
-        out["fp64" if prec == 0 else "fp32"] = (p["ms64"] / max(p["n64"], 1)) if prec == 0 else (p["ms32"] / max(p["n32"], 1))
+        out[name] = (p["ms64"] / max(p["n64"], 1)) if prec == 0 else (p["ms32"] / max(p["n32"], 1))
         s.close()
     print(json.dumps(dict(mode="kernels", neq=deck.neq, nnz=K.nnz(), shape=K.spmv_shape(), copy_doubles=n_copy,
                           spmv_avg_ms=out)), flush=True)
@@ -113,8 +113,10 @@ def run_gpu(args, deck, precision):
     p = P.profile()
     u = P.solution()
     R = P.reactions()
+    info = P.solver_info()
     out = dict(dt=dt, iters=iters, res=res, nnz=P.matrix_nnz(), n_spmv=p["n64"], ms_spmv=p["ms64"],
-               n_spmv32=p["n32"], ms_spmv32=p["ms32"], outer=0, u=u, reactions=R)
+               n_spmv32=p["n32"], ms_spmv32=p["ms32"], outer=info["outer"], matrix_free=info["matrix_free"],
+               u=u, reactions=R)
     P.close()
     return out
 
@@ -147,7 +149,7 @@ def run_newton(args, deck, precision):
     P.close()
     timed = reports[1:]
     return dict(dt=dt, steps=len(timed), newton=[r["newton"] for r in timed], pcg=[r["pcg"] for r in timed],
-                n_spmv=p["n64"], ms_spmv=p["ms64"])
+                n_spmv=p["n64"], ms_spmv=p["ms64"], n_spmv32=p["n32"], ms_spmv32=p["ms32"])
 
 
 def measured_traffic(kernel, workload, args):
@@ -208,7 +210,9 @@ def main():
     ap.add_argument("--nz", type=int, default=256)
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--maxit", type=int, default=200000)
-    ap.add_argument("--precision", choices=["fp64", "mixed", "both"], default="both")
+    ap.add_argument("--precision", choices=["fp64", "mixed", "ir", "both", "all"], default="both",
+                    help="fp64: plain PCG; mixed: FP32 PCG with FP64 reliable updates (element-by-element "
+                         "operator); ir: FP32 inner PCG in FP64 iterative refinement")
     ap.add_argument("--inner-tol", type=float, default=1e-4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-sample-nz", type=int, default=16)
@@ -233,14 +237,16 @@ def main():
         f"({time.perf_counter() - t:.1f} s)")
     if args.mode == "kernels":
         return run_kernels(args, deck)
-    modes = {"fp64": [0], "mixed": [1], "both": [0, 1]}[args.precision]
+    modes = {"fp64": [0], "mixed": [2], "ir": [1], "both": [0, 2], "all": [0, 2, 1]}[args.precision]
+    names = {0: "fp64", 1: "mixed_ir", 2: "mixed"}
     res = {}
     for prec in modes:
-        name = "fp64" if prec == 0 else "mixed"
+        name = names[prec]
         r = run_gpu(args, deck, prec)
         res[name] = r
-        log(f"[bench] {name}: {args.steps} steps in {r['dt']:.3f} s, iters {r['iters']}")
-    main_name = "fp64" if 0 in modes else "mixed"
+        log(f"[bench] {name}: {args.steps} steps in {r['dt']:.3f} s, iters {r['iters']} "
+            f"(outer/updates {r['outer']}, element operator {r['matrix_free']})")
+    main_name = names[modes[0]]
     r = res[main_name]
     n, nnz = deck.neq, r["nnz"]
     ms_per_step = r["dt"] / args.steps * 1e3
@@ -261,24 +267,34 @@ def main():
     for name, rv in res.items():
         variants[name] = dict(dofs_per_s=n * args.steps / rv["dt"], ms_per_step=rv["dt"] / args.steps * 1e3,
                               pcg_iters=rv["iters"], final_resid=rv["res"])
+        if name != "fp64":
+            variants[name]["outer_or_updates"] = rv["outer"]
+            variants[name]["element_operator"] = rv["matrix_free"]
     if args.newton_steps > 0:
         from oofem_amd import meshgen
         nd = meshgen.config2()
         log(f"[bench] newton deck {nd.name}: nelem={nd.nelem} neq={nd.neq}")
         args.nl_steps = args.newton_steps
-        rn = run_newton(args, nd, 0)
-        variants["newton_fp64_tets"] = dict(workload=nd.name, free_dofs=nd.neq, steps=rn["steps"],
-                                            dofs_per_s=nd.neq * rn["steps"] / rn["dt"],
-                                            ms_per_step=rn["dt"] / rn["steps"] * 1e3, newton_iters=rn["newton"],
-                                            pcg_iters=rn["pcg"], lstol=args.nl_lstol,
-                                            spmv_avg_ms=rn["ms_spmv"] / max(rn["n_spmv"], 1))
-    if "fp64" in res and "mixed" in res:
-        import numpy as np
-        a, b = res["fp64"], res["mixed"]
-        variants["mixed"]["u_rel_err_vs_fp64"] = float(np.linalg.norm(b["u"] - a["u"]) / np.linalg.norm(a["u"]))
-        variants["mixed"]["speedup_vs_fp64"] = variants["mixed"]["dofs_per_s"] / variants["fp64"]["dofs_per_s"]
-        if b["n_spmv32"]:
-            variants["mixed"]["spmv32_avg_ms"] = b["ms_spmv32"] / b["n_spmv32"]
+        for prec in ([0, 2] if 2 in modes else [0]):
+            rn = run_newton(args, nd, prec)
+            variants["newton_tets_" + names[prec]] = dict(
+                workload=nd.name, free_dofs=nd.neq, steps=rn["steps"], dofs_per_s=nd.neq * rn["steps"] / rn["dt"],
+                ms_per_step=rn["dt"] / rn["steps"] * 1e3, newton_iters=rn["newton"], pcg_iters=rn["pcg"],
+                lstol=args.nl_lstol, spmv_avg_ms=rn["ms_spmv"] / max(rn["n_spmv"], 1) if rn["n_spmv"] else None,
+                fp32_operator_avg_ms=rn["ms_spmv32"] / max(rn["n_spmv32"], 1) if rn["n_spmv32"] else None)
+        if "newton_tets_mixed" in variants:
+            variants["newton_tets_mixed"]["speedup_vs_fp64"] = (variants["newton_tets_mixed"]["dofs_per_s"] /
+                                                                variants["newton_tets_fp64"]["dofs_per_s"])
+    import numpy as np
+    for name in ("mixed", "mixed_ir"):
+        if "fp64" in res and name in res:
+            a, b = res["fp64"], res[name]
+            variants[name]["u_rel_err_vs_fp64"] = float(np.linalg.norm(b["u"] - a["u"]) / np.linalg.norm(a["u"]))
+            variants[name]["reactions_rel_err_vs_fp64"] = float(
+                np.linalg.norm(b["reactions"] - a["reactions"]) / np.linalg.norm(a["reactions"]))
+            variants[name]["speedup_vs_fp64"] = variants[name]["dofs_per_s"] / variants["fp64"]["dofs_per_s"]
+            if b["n_spmv32"]:
+                variants[name]["fp32_operator_avg_ms"] = b["ms_spmv32"] / b["n_spmv32"]
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args, deck, int(r["iters"][-1]), nnz)
@@ -286,7 +302,7 @@ def main():
     out = {
         "metric": METRIC, "value": value, "unit": "DOFs/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64" if main_name == "fp64" else "f32-inner/f64-refinement",
+        "vs_baseline": None, "dtype": "f64" if main_name == "fp64" else "f32-krylov/f64-residual",
         "data": "synthetic",
         "config": {"workload": deck.name, "elements": deck.nelem, "free_dofs": n, "nnz": nnz,
                    "step": "assemble K + rhs + Jacobi-PCG to 1e-8 + reactions", "tol": args.tol,

@@ -54,6 +54,10 @@ int hfem_problem_get_state(hfem_problem *p, double *s);
  * profiling was switched on (instrumentation; no counterpart in the reference) */
 int hfem_problem_set_profiling(hfem_problem *p, int on);
 int hfem_problem_profile(hfem_problem *p, int *n64, double *ms64, int *n32, double *ms32);
+/* last linear solve of the factory-made hippcg: outer iterations (mixed 1)
+ * or reliable updates (mixed 2), and whether it applied the stiffness
+ * element by element (instrumentation; no counterpart in the reference) */
+int hfem_problem_solver_info(hfem_problem *p, int *outer, int *matrix_free);
 /* SparseMtrx::at / giveClassName of the factory-made stiffness matrix */
 int hfem_problem_matrix_at(hfem_problem *p, int i, int j, double *v);
 int hfem_problem_matrix_class(hfem_problem *p, char *buf, int len);

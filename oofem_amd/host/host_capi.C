@@ -136,6 +136,15 @@ int hfem_problem_profile(hfem_problem *p, int *n64, double *ms64, int *n32, doub
     return guarded([&] { reinterpret_cast<StructuralProblem *>(p)->giveProfile(n64, ms64, n32, ms32); });
 }
 
+int hfem_problem_solver_info(hfem_problem *p, int *outer, int *matrix_free)
+{
+    return guarded([&] {
+        auto *S = dynamic_cast<HipPCGSolver *>(reinterpret_cast<StructuralProblem *>(p)->giveLinearSolver());
+        *outer = S ? hippcg_last_outer(S->handle()) : 0;
+        *matrix_free = S ? hippcg_last_matrix_free(S->handle()) : 0;
+    });
+}
+
 int hfem_problem_matrix_nnz(hfem_problem *p, long long *nnz)
 {
     return guarded([&] {

@@ -124,6 +124,7 @@ public:
     void setProfiling(bool on);
     void giveProfile(int *n64, double *ms64, int *n32, double *ms32) const;
     SparseMtrx *giveStiffnessMatrix() { return K.get(); }
+    SparseLinearSystemNM *giveLinearSolver() { return linSolver.get(); }
 };
 
 }  // namespace oofem

@@ -126,6 +126,11 @@ class StructuralProblem:
                "profile")
         return dict(n64=n64.value, ms64=m64.value, n32=n32.value, ms32=m32.value)
 
+    def solver_info(self):
+        o, mf = C.c_int(0), C.c_int(0)
+        _check(hostlib().hfem_problem_solver_info(self.h, C.byref(o), C.byref(mf)), "solver_info")
+        return dict(outer=o.value, matrix_free=bool(mf.value))
+
     def matrix_at(self, i, j):
         v = C.c_double(0)
         _check(hostlib().hfem_problem_matrix_at(self.h, int(i), int(j), C.byref(v)), "matrix_at")

@@ -244,6 +244,54 @@ int hfem_model_get_state(hfem_model *md, double *state);
 int hfem_model_ngp_total(const hfem_model *md);
 int hfem_model_ncolors(const hfem_model *md);
 
+/* ------------------------------------------------------ MisesMat surrogate */
+/* A learned stand-in for MisesMat's return mapping and consistent tangent
+ * (misesmat.C:181-263, 496-544); no counterpart in the reference.  Network
+ * 13 -> 3 x 128 (SiLU) -> 27, FP32 master weights (flat, hfem_mlp_nparams()
+ * values: W1 128x13, b1, W2 128x128, b2, W3 128x128, b3, W4 27x128, b4).
+ * Inputs (eps - epsp_n, epsp_n, kappa_n) scaled by E/sig0; outputs stress
+ * scaled by 1/sig0 and the upper triangle of the tangent (21, row-major)
+ * scaled by 1/E (hfem_mlp_scales: input, stress and tangent factors).
+ * mat4 = {E, nu, sig0, H}. */
+typedef struct hfem_mlp hfem_mlp;
+int  hfem_mlp_nparams(void);
+int  hfem_mlp_create(hipfem_ctx *ctx, const double *mat4, uint64_t seed, hfem_mlp **out);
+void hfem_mlp_destroy(hfem_mlp *m);
+int  hfem_mlp_scales(const hfem_mlp *m, double *scales3);
+int  hfem_mlp_get_params(hfem_mlp *m, float *params);
+int  hfem_mlp_set_params(hfem_mlp *m, const float *params);
+/* training set on the device: random strain paths (1..maxsteps Gaussian
+ * increments of standard deviation stdev per Voigt component, from the
+ * virgin state) through the plain return mapping; one sample per step */
+int  hfem_mlp_generate(hfem_mlp *m, int64_t nsamples, uint64_t seed, double stdev, int maxsteps);
+int  hfem_mlp_get_samples(hfem_mlp *m, int64_t first, int64_t count, float *X, float *Y);
+/* Adam on the mean-squared error over batches drawn with replacement,
+ * learning rate decayed from lr to lr_final (cosine); *final_loss: loss of
+ * the last batch */
+int  hfem_mlp_train(hfem_mlp *m, int steps, int batch, double lr, double lr_final, uint64_t seed,
+                    double *final_loss);
+/* Use the network inside MisesMat's update (no counterpart in the
+ * reference): at every internal-force evaluation the Gauss points whose
+ * trial state lies more than band * sigma_y(kappa_n) outside the yield
+ * surface are inferred in one BF16 batch; a prediction is accepted when the
+ * return-mapping equations hold at it within tol * sigma_y (elastic
+ * volumetric part, deviator along the trial deviator, yield condition),
+ * and the accepted point's temp state and tangent come from it; every
+ * other point, and every rejected one, takes the plain return mapping.  A
+ * point rejected once keeps the plain update until the next commit, so the
+ * choice cannot flip back and forth inside one load step's Newton loop.
+ * net = NULL switches the surrogate off.  Every MisesMat material of the
+ * mesh must be the network's. */
+int  hfem_model_set_surrogate(hfem_model *md, hfem_mlp *net, double band, double tol);
+/* counts since the last reset: {queued, accepted, rejected} */
+int  hfem_model_surrogate_stats(hfem_model *md, int64_t *counts3, int reset);
+/* FP32 forward of n rows (scaled inputs / outputs, host arrays) */
+int  hfem_mlp_forward(hfem_mlp *m, int n, const float *X, float *Y);
+/* gradient of the mean-squared error of n rows w.r.t. the flat parameters */
+int  hfem_mlp_gradients(hfem_mlp *m, int n, const float *X, const float *Y, float *grad, double *loss);
+/* BF16-operand MFMA inference with FP32 accumulation (host arrays) */
+int  hfem_mlp_infer_bf16(hfem_mlp *m, int n, const float *X, float *Y);
+
 #ifdef __cplusplus
 }
 #endif

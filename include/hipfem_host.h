@@ -12,6 +12,8 @@
 #ifndef HIPFEM_HOST_H
 #define HIPFEM_HOST_H
 
+#include "hipfem.h"
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -58,6 +60,12 @@ int hfem_problem_profile(hfem_problem *p, int *n64, double *ms64, int *n32, doub
  * or reliable updates (mixed 2), and whether it applied the stiffness
  * element by element (instrumentation; no counterpart in the reference) */
 int hfem_problem_solver_info(hfem_problem *p, int *outer, int *matrix_free);
+/* the GPU context the problem runs on (to create a surrogate network on it) */
+int hfem_problem_context(hfem_problem *p, hipfem_ctx **ctx);
+/* hfem_model_set_surrogate / hfem_model_surrogate_stats on the problem's
+ * batched evaluator (no counterpart in the reference) */
+int hfem_problem_set_surrogate(hfem_problem *p, hfem_mlp *net, double band, double tol);
+int hfem_problem_surrogate_stats(hfem_problem *p, long long *counts3, int reset);
 /* SparseMtrx::at / giveClassName of the factory-made stiffness matrix */
 int hfem_problem_matrix_at(hfem_problem *p, int i, int j, double *v);
 int hfem_problem_matrix_class(hfem_problem *p, char *buf, int len);

@@ -557,11 +557,6 @@ __global__ void k_d2f_vec(int n, const double *__restrict__ a, float *__restrict
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) b[i] = (float)a[i];
 }
 
-__global__ void k_f2d_vec(int n, const float *__restrict__ a, double *__restrict__ b)
-{
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) b[i] = (double)a[i];
-}
-
 extern "C" int hipcsr_times_matrix_free(hipcsr_mtx *m, const double *x, double *y)
 {
     HIPFEM_REQUIRE(m && x && y, HIPFEM_EINVAL, "times_matrix_free: null argument");
@@ -571,25 +566,22 @@ extern "C" int hipcsr_times_matrix_free(hipcsr_mtx *m, const double *x, double *
     hipStream_t st = m->ctx->stream;
     const int n = m->n;
     double *dd = nullptr;
-    float *fx = nullptr, *fy = nullptr;
+    float *fx = nullptr;
     HIPFEM_CHECK(hipMalloc(&dd, sizeof(double) * std::max(1, n)));
     HIPFEM_CHECK(hipMalloc(&fx, sizeof(float) * std::max(1, n)));
-    HIPFEM_CHECK(hipMalloc(&fy, sizeof(float) * std::max(1, n)));
     int rc = 0;
     const int grid = std::max(1, std::min((n + 255) / 256, 4096));
     if (n) {
         HIPFEM_CHECK(hipMemcpyAsync(dd, x, sizeof(double) * n, hipMemcpyHostToDevice, st));
         k_d2f_vec<<<grid, 256, 0, st>>>(n, dd, fx);
-        rc = model_mf_apply_f32(m->mf_model, fx, fy, nullptr);
+        rc = model_mf_apply_f32(m->mf_model, fx, dd, nullptr);
         if (!rc) {
-            k_f2d_vec<<<grid, 256, 0, st>>>(n, fy, dd);
             HIPFEM_CHECK(hipMemcpyAsync(y, dd, sizeof(double) * n, hipMemcpyDeviceToHost, st));
             HIPFEM_CHECK(hipStreamSynchronize(st));
         }
     }
     HIPFEM_CHECK(hipFree(dd));
     HIPFEM_CHECK(hipFree(fx));
-    HIPFEM_CHECK(hipFree(fy));
     return rc;
 }
 

@@ -132,6 +132,7 @@ struct PcgState {
     double rho, alpha, beta, normb, res, tol;
     double rmax, delta;   // reliable updates: residual at the last update, trigger factor
     int iter, done, bad, pause;
+    int restart;          // reliable updates: restart the direction after the next update
 };
 
 // kernels of an iteration return at once once the solve is done, or while it
@@ -182,9 +183,18 @@ int spmv_lanes(const hipcsr_mtx *m);
 int spmv_fp32_dev(hipcsr_mtx *m, const double *dx, double *dy);
 long next_pattern_id();
 // element-by-element operator of a linked evaluator: y = K_ff x (FP32
-// arithmetic, free equations only); returns at once inside kernels when the
-// PCG state is halted
+// element arithmetic, FP64 sum over elements, free equations only); returns
+// at once inside kernels when the PCG state is halted
 bool model_mf_available(const hipcsr_mtx *m);
-int model_mf_apply_f32(hfem_model *md, const float *x, float *y, const PcgState *st);
+int model_mf_apply_f32(hfem_model *md, const float *x, double *y, const PcgState *st);
 void model_unlink(hfem_model *md, hipcsr_mtx *m);
+}  // namespace hipfem
+
+struct hfem_mlp;
+namespace hipfem {
+// surrogate network: owner context, material {E, nu, sig0, H} and the
+// input / stress / tangent scale factors
+void mlp_info(const hfem_mlp *m, hipfem_ctx **ctx, double mat4[4], float scales[3]);
+// BF16 MFMA inference of n scaled rows (device arrays) on the context stream
+int mlp_infer_bf16_dev(hfem_mlp *m, int n, const float *dX, float *dY);
 }  // namespace hipfem

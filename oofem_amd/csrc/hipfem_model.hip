@@ -39,6 +39,7 @@
 #include <numeric>
 
 #include "hipfem_internal.h"
+#include "hipfem_material.h"
 
 namespace hipfem {
 
@@ -211,87 +212,6 @@ __device__ bool gp_geometry(const double *X, const double xi[3], double *dNdx, d
     }
 }
 
-// ----------------------------------------------------------------- MisesMat
-// performPlasticityReturn misesmat.C:181-263 (3D branch :221-256), helpers
-// structuralmaterial.C:1489-1600; no damage (omega_crit = 0).
-struct MisesOut {
-    double sig[6], epsp[6], sdev[6], kappa;
-};
-
-__device__ __forceinline__ void mises_return(double E, double nu, double sig0, double H, const double eps[6],
-                                             const double epsp_n[6], double kappa_n, MisesOut &o)
-{
-    const double G = E / (2.0 * (1. + nu));
-    const double K = E / (3.0 * (1. - 2. * nu));
-    double el[6];
-#pragma unroll
-    for (int i = 0; i < 6; i++) {
-        o.epsp[i] = epsp_n[i];
-        el[i] = eps[i] - epsp_n[i];
-    }
-    const double mean = (el[0] + el[1] + el[2]) / 3.0;
-    el[0] -= mean;
-    el[1] -= mean;
-    el[2] -= mean;
-    double s[6] = {2. * G * el[0], 2. * G * el[1], 2. * G * el[2], G * el[3], G * el[4], G * el[5]};
-    const double pvol = 3 * K * mean;
-#pragma unroll
-    for (int i = 0; i < 6; i++) o.sdev[i] = s[i];
-    const double trialS = sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2] + 2. * s[3] * s[3] + 2. * s[4] * s[4] +
-                               2. * s[5] * s[5]);
-    double kappa = kappa_n;
-    const double yieldValue = sqrt(3. / 2.) * trialS - (sig0 + H * kappa);
-    if (yieldValue > 0.) {
-        const double dKappa = yieldValue / (H + 3. * G);
-        kappa += dKappa;
-        const double f = sqrt(3. / 2.) * dKappa / trialS;
-        o.epsp[0] += f * s[0];
-        o.epsp[1] += f * s[1];
-        o.epsp[2] += f * s[2];
-        o.epsp[3] += f * (2. * s[3]);
-        o.epsp[4] += f * (2. * s[4]);
-        o.epsp[5] += f * (2. * s[5]);
-        const double sc = 1. - sqrt(6.) * G * dKappa / trialS;
-#pragma unroll
-        for (int i = 0; i < 6; i++) s[i] *= sc;
-    }
-    s[0] += pvol;
-    s[1] += pvol;
-    s[2] += pvol;
-#pragma unroll
-    for (int i = 0; i < 6; i++) o.sig[i] = s[i];
-    o.kappa = kappa;
-}
-
-// give3dMaterialStiffnessMatrix misesmat.C:496-544 (TangentStiffness, no damage)
-__device__ __forceinline__ void mises_tangent(double E, double nu, double sig0, double H, double kappa_n,
-                                              double kappa_t, const double s[6], double D[36])
-{
-    const double G = E / (2.0 * (1. + nu));
-    const double K = E / (3.0 * (1. - 2. * nu));
-    const double idev[36] = {2. / 3.,  -1. / 3., -1. / 3., 0., 0.,  0.,  -1. / 3., 2. / 3., -1. / 3.,
-                             0.,       0.,       0.,       -1. / 3., -1. / 3., 2. / 3., 0., 0.,  0.,
-                             0.,       0.,       0.,       0.5,      0.,       0.,       0., 0.,  0.,
-                             0.,       0.5,      0.,       0.,       0.,       0.,       0., 0.,  0.5};
-#pragma unroll
-    for (int i = 0; i < 36; i++) {
-        const int r = i / 6, c = i % 6;
-        D[i] = 2 * G * idev[i] + ((r < 3 && c < 3) ? K : 0.0);
-    }
-    const double dKappa = kappa_t - kappa_n;
-    if (dKappa <= 0.0) return;
-    const double sigmaY = sig0 + H * kappa_n;
-    const double trialS = sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2] + 2. * s[3] * s[3] + 2. * s[4] * s[4] +
-                               2. * s[5] * s[5]);
-    const double factor = -2. * sqrt(6.) * G * G / trialS;
-    const double factor1 = factor * sigmaY / ((H + 3. * G) * trialS * trialS);
-    const double factor2 = factor * dKappa;
-#pragma unroll
-    for (int i = 0; i < 36; i++) D[i] += factor1 * (s[i / 6] * s[i % 6]);
-#pragma unroll
-    for (int i = 0; i < 36; i++) D[i] += factor2 * idev[i];
-}
-
 // ------------------------------------------------------------ assembly kernel
 template <int ET>
 __global__ __launch_bounds__(64) void k_assemble(const int *__restrict__ elems, int nelem_color,
@@ -300,7 +220,8 @@ __global__ __launch_bounds__(64) void k_assemble(const int *__restrict__ elems, 
                                                  const double *__restrict__ mat, const uint16_t *__restrict__ eoff,
                                                  const double *__restrict__ state, int ngp_total, int tangent,
                                                  const int32_t *__restrict__ rowptr, double *__restrict__ val,
-                                                 int *__restrict__ bad)
+                                                 int *__restrict__ bad, const int8_t *__restrict__ sflag,
+                                                 const float *__restrict__ dsur)
 {
     constexpr int NEN = ElemT<ET>::NEN, NGP = ElemT<ET>::NGP;
     __shared__ double sX[NEN * 3];
@@ -331,11 +252,22 @@ __global__ __launch_bounds__(64) void k_assemble(const int *__restrict__ elems, 
         sdV[g] = fabs(detJ) * w;
         if (sGeneral) {
             const size_t gp = (size_t)e * NGP + g;
-            double s[6];
+            if (sflag && sflag[gp]) {
+                // tangent predicted by the surrogate (upper triangle, row-major)
+                double *D = sD + 36 * g;
+                int k = 0;
+                for (int r = 0; r < 6; r++)
+                    for (int c = r; c < 6; c++) {
+                        D[6 * r + c] = D[6 * c + r] = (double)dsur[(size_t)k * ngp_total + gp];
+                        k++;
+                    }
+            } else {
+                double s[6];
 #pragma unroll
-            for (int i = 0; i < 6; i++) s[i] = state[(size_t)(14 + i) * ngp_total + gp];
-            mises_tangent(E, nu, mp[3], mp[4], state[(size_t)6 * ngp_total + gp], state[(size_t)13 * ngp_total + gp], s,
-                          sD + 36 * g);
+                for (int i = 0; i < 6; i++) s[i] = state[(size_t)(14 + i) * ngp_total + gp];
+                mises_tangent(E, nu, mp[3], mp[4], state[(size_t)6 * ngp_total + gp],
+                              state[(size_t)13 * ngp_total + gp], s, sD + 36 * g);
+            }
         }
     }
     __syncthreads();
@@ -398,6 +330,156 @@ __global__ __launch_bounds__(64) void k_assemble(const int *__restrict__ elems, 
     }
 }
 
+// ------------------------------------------------------ MisesMat surrogate
+// Per-GP surrogate data of one internal-force evaluation.
+struct SurArgs {
+    const int *slot = nullptr;   // queue slot of the GP's prediction, -1 none (nullptr: surrogate off)
+    const float *y = nullptr;    // predictions, 27 scaled values per slot
+    float *dsur = nullptr;       // accepted tangent per GP, 21 x ngp_total (SoA, upper triangle)
+    int8_t *flag = nullptr;      // 1: the GP's temp state and tangent come from the surrogate
+    int8_t *rejected = nullptr;  // 1: rejected once in this load step (stays on the plain update)
+    unsigned long long *counts = nullptr;   // [0] accepted, [1] rejected by the residual check
+    float sig_scale = 1.f, tan_scale = 1.f;
+    double tol = 0.0;
+};
+
+// The network's stress is accepted when the return-mapping equations of
+// misesmat.C:181-263 hold at it, each within tol * sigma_y(kappa_n):
+// the volumetric part is elastic (p = K tr(eps - epsp_n)), the deviator lies
+// along the trial deviator, and the yield condition holds with the implied
+// Delta kappa = (sigma_eq,trial - sigma_eq) / 3G > 0.  The temp state is
+// then the one consistent with that stress: kappa_n + Delta kappa and
+// epsp = eps - C^-1 sigma.
+__device__ bool surrogate_update(double E, double nu, double sig0, double H, const double eps[6],
+                                 const double epsp_n[6], double kappa_n, const float *y, float sig_scale, double tol,
+                                 MisesOut &o)
+{
+    const double G = E / (2.0 * (1. + nu));
+    const double K = E / (3.0 * (1. - 2. * nu));
+    double el[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) el[i] = eps[i] - epsp_n[i];
+    const double mean = (el[0] + el[1] + el[2]) / 3.0;
+    const double st[6] = {2. * G * (el[0] - mean), 2. * G * (el[1] - mean), 2. * G * (el[2] - mean),
+                          G * el[3], G * el[4], G * el[5]};
+    const double trialS = sqrt(st[0] * st[0] + st[1] * st[1] + st[2] * st[2] +
+                               2. * (st[3] * st[3] + st[4] * st[4] + st[5] * st[5]));
+    double sp[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) sp[i] = (double)y[i] / (double)sig_scale;
+    const double pp = (sp[0] + sp[1] + sp[2]) / 3.0;
+    const double dv[6] = {sp[0] - pp, sp[1] - pp, sp[2] - pp, sp[3], sp[4], sp[5]};
+    const double nrm = sqrt(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2] +
+                            2. * (dv[3] * dv[3] + dv[4] * dv[4] + dv[5] * dv[5]));
+    const double sy = sig0 + H * kappa_n;
+    if (!(trialS > 0.0)) return false;
+    const double dk = sqrt(1.5) * (trialS - nrm) / (3. * G);
+    if (!(dk > 0.0)) return false;
+    const double r1 = fabs(sqrt(1.5) * nrm - (sig0 + H * (kappa_n + dk)));
+    const double f = nrm / trialS;
+    double r2 = 0.0;
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        const double d = dv[i] - f * st[i];
+        r2 += (i < 3 ? 1.0 : 2.0) * d * d;
+    }
+    r2 = sqrt(r2);
+    const double r3 = fabs(pp - K * 3.0 * mean);
+    if (!(fmax(r1, fmax(r2, r3)) <= tol * sy)) return false;
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        o.sig[i] = sp[i];
+        o.sdev[i] = st[i];
+    }
+    const double ev = pp / (3. * K);
+#pragma unroll
+    for (int i = 0; i < 3; i++) o.epsp[i] = eps[i] - (dv[i] / (2. * G) + ev);
+#pragma unroll
+    for (int i = 3; i < 6; i++) o.epsp[i] = eps[i] - dv[i] / G;
+    o.kappa = kappa_n + dk;
+    return true;
+}
+
+// Queue the MisesMat Gauss points whose trial state lies more than band *
+// sigma_y(kappa_n) outside the yield surface (the others stay with the
+// plain update: elastic, or within the band around the surface) and write
+// their scaled network inputs.  One thread per Gauss point, all elements.
+template <int ET>
+__global__ __launch_bounds__(256) void k_sur_classify(int nelem, const double *__restrict__ coords,
+                                                      const int *__restrict__ conn, const int *__restrict__ eqn,
+                                                      const int *__restrict__ emat, const double *__restrict__ mat,
+                                                      const double *__restrict__ u_free,
+                                                      const double *__restrict__ u_presc,
+                                                      const double *__restrict__ state, int ngp_total, double band,
+                                                      float in_scale, const int8_t *__restrict__ rejected,
+                                                      int *__restrict__ slot, int *__restrict__ count,
+                                                      float *__restrict__ X)
+{
+    constexpr int NEN = ElemT<ET>::NEN, NGP = ElemT<ET>::NGP, NC = 3 * NEN;
+    const int gp = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gp >= ngp_total) return;
+    const int e = gp / NGP, g = gp % NGP;
+    const double *mp = mat + (size_t)HIPFEM_MATP * emat[e];
+    if ((int)mp[0] != HIPFEM_MAT_MISES || rejected[gp]) {
+        slot[gp] = -1;
+        return;
+    }
+    double X3[NC], ue[NC];
+    for (int a = 0; a < NEN; a++) {
+        const int node = conn[(size_t)e * NEN + a];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            X3[3 * a + d] = coords[3 * (size_t)node + d];
+            const int q = eqn[3 * (size_t)node + d];
+            ue[3 * a + d] = (q > 0 && u_free) ? u_free[q - 1] : ((q < 0 && u_presc) ? u_presc[-q - 1] : 0.0);
+        }
+    }
+    double xi[3], w, detJ, dNdx[NC];
+    gauss_point(ET, g, xi, w);
+    if (!gp_geometry<ET>(X3, xi, dNdx, detJ)) {
+        slot[gp] = -1;
+        return;
+    }
+    double eps[6] = {0, 0, 0, 0, 0, 0};
+    for (int a = 0; a < NEN; a++) {
+        const double nx = dNdx[3 * a], ny = dNdx[3 * a + 1], nz = dNdx[3 * a + 2];
+        const double ux = ue[3 * a], uy = ue[3 * a + 1], uz = ue[3 * a + 2];
+        eps[0] += nx * ux;
+        eps[1] += ny * uy;
+        eps[2] += nz * uz;
+        eps[3] += nz * uy + ny * uz;
+        eps[4] += nz * ux + nx * uz;
+        eps[5] += ny * ux + nx * uy;
+    }
+    const double E = mp[1], nu = mp[2], sig0 = mp[3], H = mp[4];
+    const double G = E / (2.0 * (1. + nu));
+    double el[6], epsp_n[6];
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        epsp_n[i] = state[(size_t)i * ngp_total + gp];
+        el[i] = eps[i] - epsp_n[i];
+    }
+    const double kappa_n = state[(size_t)6 * ngp_total + gp];
+    const double mean = (el[0] + el[1] + el[2]) / 3.0;
+    const double s0 = 2. * G * (el[0] - mean), s1 = 2. * G * (el[1] - mean), s2 = 2. * G * (el[2] - mean);
+    const double s3 = G * el[3], s4 = G * el[4], s5 = G * el[5];
+    const double trialS = sqrt(s0 * s0 + s1 * s1 + s2 * s2 + 2. * (s3 * s3 + s4 * s4 + s5 * s5));
+    const double sy = sig0 + H * kappa_n;
+    if (sqrt(1.5) * trialS - sy > band * sy) {
+        const int k = atomicAdd(count, 1);
+        slot[gp] = k;
+        float *x = X + (size_t)k * 13;
+#pragma unroll
+        for (int i = 0; i < 6; i++) {
+            x[i] = (float)(el[i] * in_scale);
+            x[6 + i] = (float)(epsp_n[i] * in_scale);
+        }
+        x[12] = (float)(kappa_n * in_scale);
+    } else {
+        slot[gp] = -1;
+    }
+}
+
 // -------------------------------------------------------- internal forces
 template <int ET>
 struct IntCfg {
@@ -413,11 +495,14 @@ __global__ __launch_bounds__(64) void k_internal(const int *__restrict__ elems, 
                                                  const double *__restrict__ u_presc, double *__restrict__ state,
                                                  int ngp_total, int update_state, double *__restrict__ f_free,
                                                  double *__restrict__ f_presc, double *__restrict__ partials,
-                                                 int npart, int *__restrict__ bad, int op)
+                                                 int npart, int *__restrict__ bad, int op, SurArgs sur)
 {
     constexpr int NEN = ElemT<ET>::NEN, NGP = ElemT<ET>::NGP, EPB = IntCfg<ET>::EPB, NC = 3 * NEN;
     __shared__ double sF[EPB][NGP][NC];
     __shared__ double sh[3];
+    __shared__ unsigned int scount[2];
+    if (threadIdx.x < 2) scount[threadIdx.x] = 0;
+    __syncthreads();
     const int t = threadIdx.x;
     const int el = t / NGP, g = t % NGP;
     const int ei = blockIdx.x * EPB + el;
@@ -480,8 +565,29 @@ __global__ __launch_bounds__(64) void k_internal(const int *__restrict__ elems, 
             double epsp_n[6];
 #pragma unroll
             for (int i = 0; i < 6; i++) epsp_n[i] = state[(size_t)i * ngp_total + gp];
+            const double kappa_n = state[(size_t)6 * ngp_total + gp];
             MisesOut o;
-            mises_return(E, nu, mp[3], mp[4], eps, epsp_n, state[(size_t)6 * ngp_total + gp], o);
+            bool used = false;
+            if (sur.slot) {
+                const int sl = sur.slot[gp];
+                if (sl >= 0) {
+                    used = surrogate_update(E, nu, mp[3], mp[4], eps, epsp_n, kappa_n, sur.y + (size_t)27 * sl,
+                                            sur.sig_scale, sur.tol, o);
+                    atomicAdd(&scount[used ? 0 : 1], 1u);
+                }
+                if (update_state) {
+                    sur.flag[gp] = used ? 1 : 0;
+                    // a point rejected once keeps the plain update for the rest of the
+                    // step, so accept/reject switches cannot repeat inside Newton
+                    if (sl >= 0 && !used) sur.rejected[gp] = 1;
+                    if (used) {
+                        const float *yy = sur.y + (size_t)27 * sl + 6;
+#pragma unroll
+                        for (int k = 0; k < 21; k++) sur.dsur[(size_t)k * ngp_total + gp] = yy[k] / sur.tan_scale;
+                    }
+                }
+            }
+            if (!used) mises_return(E, nu, mp[3], mp[4], eps, epsp_n, kappa_n, o);
 #pragma unroll
             for (int i = 0; i < 6; i++) sig[i] = o.sig[i];
             if (update_state) {
@@ -521,6 +627,10 @@ __global__ __launch_bounds__(64) void k_internal(const int *__restrict__ elems, 
         }
     }
     __syncthreads();
+    if (sur.counts && t == 0 && (scount[0] | scount[1])) {
+        atomicAdd(&sur.counts[0], (unsigned long long)scount[0]);
+        atomicAdd(&sur.counts[1], (unsigned long long)scount[1]);
+    }
     double n3[3] = {0, 0, 0};
     for (int idx = t; idx < EPB * NC; idx += 64) {
         const int l = idx / NC, c = idx % NC;
@@ -555,10 +665,15 @@ __global__ __launch_bounds__(64) void k_internal(const int *__restrict__ elems, 
 // D eps = c dev(eps) + K tr(eps) 1 - m (m . eps); elastic points have c = 2G,
 // m = 0.  Fields: 0 c, 1 K, 2-7 m.
 __global__ void k_tangent_desc(int ngp_total, int ngp, const int *__restrict__ emat, const double *__restrict__ mat,
-                               const double *__restrict__ state, int mode, float *__restrict__ desc)
+                               const double *__restrict__ state, int mode, float *__restrict__ desc,
+                               const int8_t *__restrict__ sflag)
 {
     const int gp = blockIdx.x * blockDim.x + threadIdx.x;
     if (gp >= ngp_total) return;
+    if (mode == 1 && sflag && sflag[gp]) {
+        desc[gp] = -1.0f;   // general D: the surrogate's tangent (read by the operator)
+        return;
+    }
     const double *mp = mat + (size_t)HIPFEM_MATP * emat[gp / ngp];
     const double E = mp[1], nu = mp[2];
     const double G = E / (2.0 * (1. + nu));
@@ -588,13 +703,16 @@ __global__ void k_tangent_desc(int ngp_total, int ngp, const int *__restrict__ e
     for (int i = 0; i < 6; i++) desc[(size_t)(2 + i) * ngp_total + gp] = (float)m[i];
 }
 
-__global__ void k_fill_f(int n, float *__restrict__ a, const PcgState *__restrict__ st)
+__global__ void k_fill_d(int n, double *__restrict__ a, const PcgState *__restrict__ st)
 {
     if (st && halted(st)) return;
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) a[i] = 0.0f;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) a[i] = 0.0;
 }
 
-// y += K_e x_e over the free equations of one color of LSpace elements, FP32.
+// y += K_e x_e over the free equations of one color of LSpace elements:
+// element arithmetic in FP32, the sum over elements in FP64 (neighbouring
+// element forces of a smooth field nearly cancel at a node, by a factor of
+// about h/L, so an FP32 sum would lose that many digits).
 // 8 threads per element (one per Gauss point), 32 elements per block.  Node
 // data are staged in LDS relative to node 1 (coordinates differenced in FP64,
 // displacements in FP32: a rigid translation adds nothing to the strain), the
@@ -605,7 +723,8 @@ __global__ __launch_bounds__(256) void k_mf_hex(const int *__restrict__ elems, i
                                                 const double *__restrict__ coords, const int *__restrict__ conn,
                                                 const int *__restrict__ eqn, const int *__restrict__ emat,
                                                 const double *__restrict__ mat, const float *__restrict__ desc,
-                                                int ngp_total, const float *__restrict__ x, float *__restrict__ y,
+                                                const float *__restrict__ dgen, int ngp_total,
+                                                const float *__restrict__ x, double *__restrict__ y,
                                                 const PcgState *__restrict__ st)
 {
     __shared__ float sX[MF_EPB][8][3];
@@ -690,16 +809,30 @@ __global__ __launch_bounds__(256) void k_mf_hex(const int *__restrict__ elems, i
 #pragma unroll
             for (int i = 0; i < 6; i++) m[i] = 0.f;
         }
-        const float tr = eps[0] + eps[1] + eps[2];
-        const float me = m[0] * eps[0] + m[1] * eps[1] + m[2] * eps[2] + m[3] * eps[3] + m[4] * eps[4] + m[5] * eps[5];
-        const float p = Kb * tr, t3 = tr * (1.f / 3.f);
         float sig[6];
-        sig[0] = c * (eps[0] - t3) + p - m[0] * me;
-        sig[1] = c * (eps[1] - t3) + p - m[1] * me;
-        sig[2] = c * (eps[2] - t3) + p - m[2] * me;
-        sig[3] = 0.5f * c * eps[3] - m[3] * me;
-        sig[4] = 0.5f * c * eps[4] - m[4] * me;
-        sig[5] = 0.5f * c * eps[5] - m[5] * me;
+        if (c >= 0.f) {
+            const float tr = eps[0] + eps[1] + eps[2];
+            const float me =
+                m[0] * eps[0] + m[1] * eps[1] + m[2] * eps[2] + m[3] * eps[3] + m[4] * eps[4] + m[5] * eps[5];
+            const float p = Kb * tr, t3 = tr * (1.f / 3.f);
+            sig[0] = c * (eps[0] - t3) + p - m[0] * me;
+            sig[1] = c * (eps[1] - t3) + p - m[1] * me;
+            sig[2] = c * (eps[2] - t3) + p - m[2] * me;
+            sig[3] = 0.5f * c * eps[3] - m[3] * me;
+            sig[4] = 0.5f * c * eps[4] - m[4] * me;
+            sig[5] = 0.5f * c * eps[5] - m[5] * me;
+        } else {
+            // general symmetric D (upper triangle, row-major) of a surrogate point
+            float D[21];
+#pragma unroll
+            for (int k = 0; k < 21; k++) D[k] = dgen[(size_t)k * ngp_total + gp];
+            sig[0] = D[0] * eps[0] + D[1] * eps[1] + D[2] * eps[2] + D[3] * eps[3] + D[4] * eps[4] + D[5] * eps[5];
+            sig[1] = D[1] * eps[0] + D[6] * eps[1] + D[7] * eps[2] + D[8] * eps[3] + D[9] * eps[4] + D[10] * eps[5];
+            sig[2] = D[2] * eps[0] + D[7] * eps[1] + D[11] * eps[2] + D[12] * eps[3] + D[13] * eps[4] + D[14] * eps[5];
+            sig[3] = D[3] * eps[0] + D[8] * eps[1] + D[12] * eps[2] + D[15] * eps[3] + D[16] * eps[4] + D[17] * eps[5];
+            sig[4] = D[4] * eps[0] + D[9] * eps[1] + D[13] * eps[2] + D[16] * eps[3] + D[18] * eps[4] + D[19] * eps[5];
+            sig[5] = D[5] * eps[0] + D[10] * eps[1] + D[14] * eps[2] + D[17] * eps[3] + D[19] * eps[4] + D[20] * eps[5];
+        }
         // T = dV sigma J^-T: f_n = T dN_n (dN/dx = J^-T dN/dxi in column form)
         const float dV = fabsf(det);
         const float S[9] = {sig[0], sig[5], sig[4], sig[5], sig[1], sig[3], sig[4], sig[3], sig[2]};
@@ -725,7 +858,7 @@ __global__ __launch_bounds__(256) void k_mf_hex(const int *__restrict__ elems, i
             float f = 0.f;
 #pragma unroll
             for (int gg = 0; gg < 8; gg++) f += sF[le][gg][cidx];
-            y[q - 1] += f;
+            y[q - 1] += (double)f;
         }
     }
 }
@@ -840,12 +973,32 @@ struct hfem_model {
     bool tdesc_used = false;     // false: every point elastic, D from the material table
     bool has_mises = false;
     std::vector<hipcsr_mtx *> linked;
+    // MisesMat surrogate (hfem_model_set_surrogate)
+    hfem_mlp *sur = nullptr;
+    double sur_band = 0.05, sur_tol = 1e-4;
+    float sur_scales[3] = {1, 1, 1};
+    int *d_slot = nullptr, *d_qcount = nullptr;
+    float *d_sx = nullptr, *d_sy = nullptr, *d_dsur = nullptr;
+    int8_t *d_sflag = nullptr, *d_srej = nullptr;
+    unsigned long long *d_scounts = nullptr;   // accepted, rejected
+    unsigned long long queued_total = 0;
 };
 
 static int model_free(hfem_model *md)
 {
     for (hipcsr_mtx *k : md->linked) k->mf_model = nullptr;
     md->linked.clear();
+    {
+        void *sp[] = {md->d_slot, md->d_qcount, md->d_sx, md->d_sy, md->d_dsur, md->d_sflag, md->d_srej,
+                      md->d_scounts};
+        for (void *p : sp)
+            if (p) HIPFEM_CHECK(hipFree(p));
+        md->d_slot = md->d_qcount = nullptr;
+        md->d_sx = md->d_sy = md->d_dsur = nullptr;
+        md->d_sflag = md->d_srej = nullptr;
+        md->d_scounts = nullptr;
+        md->sur = nullptr;
+    }
     void *ps[] = {md->d_coords, md->d_mat, md->d_state, md->d_conn, md->d_emat, md->d_eqn, md->d_bad,
                   md->d_eoff,   md->d_colors, md->d_uf, md->d_up, md->d_ff, md->d_fp, md->d_out3, md->d_dofid,
                   md->d_tdesc};
@@ -1138,7 +1291,8 @@ extern "C" int hfem_model_assemble_stiffness(hfem_model *md, hipcsr_mtx *K, int 
         const int *el = md->d_colors + md->color_ptr[c];
 #define LAUNCH_ASM(ET)                                                                                              \
     k_assemble<ET><<<n, 64, 0, st>>>(el, n, md->d_coords, md->d_conn, md->d_eqn, md->d_emat, md->d_mat, md->d_eoff, \
-                                     md->d_state, md->ngp_total, mode, K->d_rowptr, K->d_val, md->d_bad)
+                                     md->d_state, md->ngp_total, mode, K->d_rowptr, K->d_val, md->d_bad,     \
+                                     md->sur ? md->d_sflag : nullptr, md->d_dsur)
         if (md->etype == HIPFEM_LSPACE) LAUNCH_ASM(HIPFEM_LSPACE);
         else if (md->etype == HIPFEM_LTRSPACE) LAUNCH_ASM(HIPFEM_LTRSPACE);
         else LAUNCH_ASM(HIPFEM_QSPACE);
@@ -1155,7 +1309,8 @@ extern "C" int hfem_model_assemble_stiffness(hfem_model *md, hipcsr_mtx *K, int 
         if (md->tdesc_used) {
             if (!md->d_tdesc) HIPFEM_CHECK(hipMalloc(&md->d_tdesc, sizeof(float) * 8 * (size_t)md->ngp_total));
             k_tangent_desc<<<(md->ngp_total + 255) / 256, 256, 0, st>>>(md->ngp_total, md->ngp, md->d_emat,
-                                                                          md->d_mat, md->d_state, mode, md->d_tdesc);
+                                                                          md->d_mat, md->d_state, mode, md->d_tdesc,
+                                                                          md->sur ? md->d_sflag : nullptr);
             HIPFEM_CHECK(hipGetLastError());
         }
         if (K->mf_model && K->mf_model != md) model_unlink(K->mf_model, K);
@@ -1183,17 +1338,17 @@ void model_unlink(hfem_model *md, hipcsr_mtx *m)
     m->mf_version = -1;
 }
 
-int model_mf_apply_f32(hfem_model *md, const float *x, float *y, const PcgState *st)
+int model_mf_apply_f32(hfem_model *md, const float *x, double *y, const PcgState *st)
 {
     hipStream_t s = md->ctx->stream;
-    if (md->neq) k_fill_f<<<std::min(4096, (md->neq + 255) / 256), 256, 0, s>>>(md->neq, y, st);
+    if (md->neq) k_fill_d<<<std::min(4096, (md->neq + 255) / 256), 256, 0, s>>>(md->neq, y, st);
     for (int c = 0; c < md->ncolors; c++) {
         const int n = md->color_ptr[c + 1] - md->color_ptr[c];
         if (!n) continue;
         k_mf_hex<<<(n + MF_EPB - 1) / MF_EPB, 256, 0, s>>>(md->d_colors + md->color_ptr[c], n, md->d_coords,
                                                            md->d_conn, md->d_eqn, md->d_emat, md->d_mat,
-                                                           md->tdesc_used ? md->d_tdesc : nullptr, md->ngp_total, x,
-                                                           y, st);
+                                                           md->tdesc_used ? md->d_tdesc : nullptr, md->d_dsur,
+                                                           md->ngp_total, x, y, st);
     }
     HIPFEM_CHECK(hipGetLastError());
     return 0;
@@ -1217,6 +1372,38 @@ static int internal_dev(hfem_model *md, const double *uf, const double *up, doub
     if (ff && md->neq) k_fill<<<std::min(8192, (md->neq + 255) / 256), 256, 0, st>>>(md->neq, 0.0, ff);
     if (fp && md->npeq) k_fill<<<std::min(8192, (md->npeq + 255) / 256), 256, 0, st>>>(md->npeq, 0.0, fp);
     double *part = md->ctx->d_partials;
+    SurArgs sur;
+    if (md->sur && op == 0 && md->has_mises && md->ngp_total > 0) {
+        // queue the points beyond the band, infer them in one BF16 batch
+        HIPFEM_CHECK(hipMemsetAsync(md->d_qcount, 0, sizeof(int), st));
+        const int g2 = (md->ngp_total + 255) / 256;
+#define LAUNCH_CLS(ET)                                                                                           \
+    k_sur_classify<ET><<<g2, 256, 0, st>>>(md->nelem, md->d_coords, md->d_conn, md->d_eqn, md->d_emat, md->d_mat, \
+                                           uf, up, md->d_state, md->ngp_total, md->sur_band, md->sur_scales[0],   \
+                                           md->d_srej, md->d_slot, md->d_qcount, md->d_sx)
+        if (md->etype == HIPFEM_LSPACE) LAUNCH_CLS(HIPFEM_LSPACE);
+        else if (md->etype == HIPFEM_LTRSPACE) LAUNCH_CLS(HIPFEM_LTRSPACE);
+        else LAUNCH_CLS(HIPFEM_QSPACE);
+#undef LAUNCH_CLS
+        HIPFEM_CHECK(hipGetLastError());
+        int nq = 0;
+        HIPFEM_CHECK(hipMemcpyAsync(md->ctx->h_pinned, md->d_qcount, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPFEM_CHECK(hipStreamSynchronize(st));
+        memcpy(&nq, md->ctx->h_pinned, sizeof(int));
+        HIPFEM_REQUIRE(nq >= 0 && nq <= md->ngp_total, HIPFEM_EHIP, "surrogate queue count %d out of range", nq);
+        md->queued_total += (unsigned long long)nq;
+        rc = mlp_infer_bf16_dev(md->sur, nq, md->d_sx, md->d_sy);
+        if (rc) return rc;
+        sur.slot = md->d_slot;
+        sur.y = md->d_sy;
+        sur.dsur = md->d_dsur;
+        sur.flag = md->d_sflag;
+        sur.rejected = md->d_srej;
+        sur.counts = md->d_scounts;
+        sur.sig_scale = md->sur_scales[1];
+        sur.tan_scale = md->sur_scales[2];
+        sur.tol = md->sur_tol;
+    }
     int poff = 0;
     for (int c = 0; c < md->ncolors; c++) {
         const int n = md->color_ptr[c + 1] - md->color_ptr[c];
@@ -1226,7 +1413,8 @@ static int internal_dev(hfem_model *md, const double *uf, const double *up, doub
         // partial slots of this color: [poff, poff + nb) in each of the 3 planes
 #define LAUNCH_INT(ET)                                                                                          \
     k_internal<ET><<<nb, 64, 0, st>>>(el, n, md->d_coords, md->d_conn, md->d_eqn, md->d_emat, md->d_mat, uf, up, \
-                                      md->d_state, md->ngp_total, update, ff, fp, part + poff, npart, md->d_bad, op)
+                                      md->d_state, md->ngp_total, update, ff, fp, part + poff, npart, md->d_bad, op, \
+                                      sur)
         if (md->etype == HIPFEM_LSPACE) LAUNCH_INT(HIPFEM_LSPACE);
         else if (md->etype == HIPFEM_LTRSPACE) LAUNCH_INT(HIPFEM_LTRSPACE);
         else LAUNCH_INT(HIPFEM_QSPACE);
@@ -1269,6 +1457,69 @@ extern "C" int hfem_model_internal_forces(hfem_model *md, const double *u_free, 
     return 0;
 }
 
+extern "C" int hfem_model_set_surrogate(hfem_model *md, hfem_mlp *net, double band, double tol)
+{
+    HIPFEM_REQUIRE(md && md->d_conn, HIPFEM_EINVAL, "set_surrogate: no mesh");
+    DeviceGuard g(md->ctx->device);
+    if (!net) {
+        md->sur = nullptr;
+        return 0;
+    }
+    HIPFEM_REQUIRE(band >= 0.0 && tol > 0.0, HIPFEM_EINVAL, "set_surrogate: band must be >= 0 and tol > 0");
+    hipfem_ctx *nc = nullptr;
+    double m4[4];
+    float sc[3];
+    mlp_info(net, &nc, m4, sc);
+    HIPFEM_REQUIRE(nc == md->ctx, HIPFEM_EINVAL, "set_surrogate: network and model belong to different contexts");
+    // every MisesMat row of the mesh must be the material the network learned
+    std::vector<double> mat((size_t)HIPFEM_MATP * md->nmat);
+    HIPFEM_CHECK(hipStreamSynchronize(md->ctx->stream));
+    HIPFEM_CHECK(hipMemcpy(mat.data(), md->d_mat, sizeof(double) * mat.size(), hipMemcpyDeviceToHost));
+    for (int k = 0; k < md->nmat; k++) {
+        const double *r = mat.data() + (size_t)HIPFEM_MATP * k;
+        if ((int)r[0] != HIPFEM_MAT_MISES) continue;
+        for (int i = 0; i < 4; i++)
+            HIPFEM_REQUIRE(std::fabs(r[1 + i] - m4[i]) <= 1e-12 * std::max(1.0, std::fabs(m4[i])), HIPFEM_EINVAL,
+                           "set_surrogate: material %d differs from the network's (E nu sig0 H)", k);
+    }
+    const size_t n = (size_t)std::max(1, md->ngp_total);
+    if (!md->d_slot) {
+        HIPFEM_CHECK(hipMalloc(&md->d_slot, sizeof(int) * n));
+        HIPFEM_CHECK(hipMalloc(&md->d_qcount, sizeof(int)));
+        HIPFEM_CHECK(hipMalloc(&md->d_sx, sizeof(float) * n * 13));
+        HIPFEM_CHECK(hipMalloc(&md->d_sy, sizeof(float) * n * 27));
+        HIPFEM_CHECK(hipMalloc(&md->d_dsur, sizeof(float) * n * 21));
+        HIPFEM_CHECK(hipMalloc(&md->d_sflag, n));
+        HIPFEM_CHECK(hipMalloc(&md->d_srej, n));
+        HIPFEM_CHECK(hipMalloc(&md->d_scounts, 2 * sizeof(unsigned long long)));
+        HIPFEM_CHECK(hipMemset(md->d_sflag, 0, n));
+        HIPFEM_CHECK(hipMemset(md->d_srej, 0, n));
+        HIPFEM_CHECK(hipMemset(md->d_scounts, 0, 2 * sizeof(unsigned long long)));
+    }
+    md->sur = net;
+    md->sur_band = band;
+    md->sur_tol = tol;
+    for (int i = 0; i < 3; i++) md->sur_scales[i] = sc[i];
+    return 0;
+}
+
+extern "C" int hfem_model_surrogate_stats(hfem_model *md, int64_t *counts3, int reset)
+{
+    HIPFEM_REQUIRE(md && counts3, HIPFEM_EINVAL, "surrogate_stats: null argument");
+    DeviceGuard g(md->ctx->device);
+    unsigned long long c[2] = {0, 0};
+    HIPFEM_CHECK(hipStreamSynchronize(md->ctx->stream));
+    if (md->d_scounts) HIPFEM_CHECK(hipMemcpy(c, md->d_scounts, sizeof c, hipMemcpyDeviceToHost));
+    counts3[0] = (int64_t)md->queued_total;
+    counts3[1] = (int64_t)c[0];
+    counts3[2] = (int64_t)c[1];
+    if (reset) {
+        md->queued_total = 0;
+        if (md->d_scounts) HIPFEM_CHECK(hipMemset(md->d_scounts, 0, sizeof c));
+    }
+    return 0;
+}
+
 extern "C" int hfem_model_tangent_product_dev(hfem_model *md, int mode, const double *dx_free, const double *dx_presc,
                                               double *dy_free, double *dy_presc)
 {
@@ -1287,6 +1538,9 @@ extern "C" int hfem_model_commit(hfem_model *md)
     if (md->ngp_total) {
         k_commit<<<(md->ngp_total + 255) / 256, 256, 0, md->ctx->stream>>>(md->ngp_total, md->d_state);
         HIPFEM_CHECK(hipGetLastError());
+        // the committed state's tangent is MisesMat's own (temp = committed)
+        if (md->d_sflag) HIPFEM_CHECK(hipMemsetAsync(md->d_sflag, 0, (size_t)md->ngp_total, md->ctx->stream));
+        if (md->d_srej) HIPFEM_CHECK(hipMemsetAsync(md->d_srej, 0, (size_t)md->ngp_total, md->ctx->stream));
     }
     return 0;
 }
@@ -1298,6 +1552,8 @@ extern "C" int hfem_model_reset_state(hfem_model *md)
     const int64_t ns = (int64_t)NST * std::max(1, md->ngp_total);
     k_fill<<<(int)std::min<int64_t>((ns + 255) / 256, 8192), 256, 0, md->ctx->stream>>>(ns, 0.0, md->d_state);
     HIPFEM_CHECK(hipGetLastError());
+    if (md->d_sflag) HIPFEM_CHECK(hipMemsetAsync(md->d_sflag, 0, (size_t)std::max(1, md->ngp_total), md->ctx->stream));
+    if (md->d_srej) HIPFEM_CHECK(hipMemsetAsync(md->d_srej, 0, (size_t)std::max(1, md->ngp_total), md->ctx->stream));
     return 0;
 }
 

@@ -400,6 +400,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_reduce_true(PcgState *st, const
             st->iter = 0;
             st->bad = 0;
             st->done = 0;
+            st->restart = 0;
         }
         const double res = sqrt(rr) / st->normb;
         st->res = res;
@@ -430,14 +431,19 @@ __global__ __launch_bounds__(256) void k_rel_refresh(int n, const double *__rest
     if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
-// partials of p.q after an operator that does not produce them
-__global__ __launch_bounds__(256) void k_dot_pq(int n, const float *__restrict__ p, const float *__restrict__ q,
-                                                const PcgState *__restrict__ st, double *__restrict__ partials)
+// q32 = (float) q64 and the partials of p.q after the element operator
+__global__ __launch_bounds__(256) void k_dot_pq(int n, const float *__restrict__ p, const double *__restrict__ q64,
+                                                float *__restrict__ q, const PcgState *__restrict__ st,
+                                                double *__restrict__ partials)
 {
     __shared__ double sh[4];
     if (halted(st)) return;
     double acc = 0.0;
-    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) acc += (double)p[i] * (double)q[i];
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        const double qi = q64[i];
+        q[i] = (float)qi;
+        acc += (double)p[i] * qi;
+    }
     const double t = block_sum<256>(acc, sh);
     if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
@@ -450,10 +456,31 @@ __global__ __launch_bounds__(RED_THREADS) void k_reduce_refresh(PcgState *st, co
     if (st->done) return;
     const double rz = reduce_partials(partials, nb, sh);
     if (threadIdx.x == 0) {
-        st->beta = first ? 0.0 : rz / st->rho;
+        st->beta = (first || st->restart) ? 0.0 : rz / st->rho;
         st->rho = rz;
         st->rmax = st->res;
         st->pause = 0;
+        st->restart = 0;
+    }
+}
+
+// alpha of one FP32 iteration; a non-positive curvature p.q (the FP32
+// operator has lost the definiteness of A on this direction) pauses the
+// iteration for an update that restarts the direction
+__global__ __launch_bounds__(RED_THREADS) void k_reduce_alpha_rel(PcgState *st, const double *__restrict__ partials,
+                                                                  int nb)
+{
+    __shared__ double sh[RED_THREADS / 64];
+    if (halted(st)) return;
+    const double pq = reduce_partials(partials, nb, sh);
+    if (threadIdx.x == 0) {
+        if (pq > 0.0) {
+            st->alpha = st->rho / pq;
+        } else {
+            st->alpha = 0.0;
+            st->pause = 1;
+            st->restart = 1;
+        }
     }
 }
 
@@ -474,6 +501,9 @@ __global__ __launch_bounds__(RED_THREADS) void k_reduce_beta_rel(PcgState *st, c
             st->bad = 1;
         } else if (res <= st->tol || res < st->delta * st->rmax) {
             st->pause = 1;   // the host replaces r by the true residual
+        } else if (res > 100.0 * st->rmax) {
+            st->pause = 1;   // the recursion has drifted: update and restart the direction
+            st->restart = 1;
         } else {
             st->beta = rz / st->rho;
             st->rho = rz;
@@ -529,6 +559,7 @@ struct hippcg_solver {
     double delta = 0.1;   // precision 2: reliable update when |r| < delta |r at last update|
     int last_updates = 0;
     int last_mf = 0;      // the last precision-2 solve used the element-by-element operator
+    int last_restarts = 0;
     // preconditioner cache (IMLSolver re-inits on a new matrix or version, imlsolver.C:112-114)
     const hipcsr_mtx *lhs = nullptr;
     long lhs_version = -1;
@@ -759,6 +790,7 @@ static int pcg_reliable(hippcg_solver *s, hipcsr_mtx *A, const double *db, doubl
     bool first = true;
     const bool prof = s->profiling != 0;
     s->last_updates = 0;
+    s->last_restarts = 0;
     while (!h.done && it <= maxit) {
         const int stop = std::min(maxit, it + chunk - 1);
         if (prof) {
@@ -770,9 +802,9 @@ static int pcg_reliable(hippcg_solver *s, hipcsr_mtx *A, const double *db, doubl
             first = false;
             if (prof) HIPFEM_CHECK(hipEventRecord(s->ev[2 * (k - 1)], st));
             if (mf) {
-                rc = model_mf_apply_f32(A->mf_model, s->fp0, s->fq, s->st);
+                rc = model_mf_apply_f32(A->mf_model, s->fp0, s->q, s->st);
                 if (rc) return rc;
-                k_dot_pq<<<gv, 256, 0, st>>>(n, s->fp0, s->fq, s->st, part);
+                k_dot_pq<<<gv, 256, 0, st>>>(n, s->fp0, s->q, s->fq, s->st, part);
             } else if (nodeblock) {
                 k_pcg_spmv_p_nb<float, float><<<gs, 256, 0, st>>>(A->nbr, A->d_bptr, A->d_bcol, A->d_val32, s->fp0,
                                                                   s->fq, s->st, part);
@@ -781,7 +813,7 @@ static int pcg_reliable(hippcg_solver *s, hipcsr_mtx *A, const double *db, doubl
                                                 s->st, part);
             }
             if (prof) HIPFEM_CHECK(hipEventRecord(s->ev[2 * (k - 1) + 1], st));
-            k_reduce_alpha<<<1, RED_THREADS, 0, st>>>(s->st, part, mf ? gv : gs);
+            k_reduce_alpha_rel<<<1, RED_THREADS, 0, st>>>(s->st, part, mf ? gv : gs);
             k_pcg_update<double, float><<<gv, 256, 0, st>>>(n, dx, s->fr, s->fz, s->fp0, s->fq, s->dinv32, s->st,
                                                             part);
             k_reduce_beta_rel<<<1, RED_THREADS, 0, st>>>(s->st, part, gv, k);
@@ -791,12 +823,19 @@ static int pcg_reliable(hippcg_solver *s, hipcsr_mtx *A, const double *db, doubl
         if (rc) return rc;
         if (h.done) break;
         if (h.pause) {
-            // iterations after h.iter in this chunk returned at once; resume after it
+            // iterations after the pause returned at once; resume after the
+            // last completed one (a pause inside k_reduce_alpha_rel leaves
+            // that iteration without an update: it is done again)
+            const bool restart = h.restart != 0;
             rc = true_residual(0);
             if (rc) return rc;
             rc = read_state(s, &h);
             if (rc) return rc;
             s->last_updates++;
+            if (restart) {
+                first = true;
+                s->last_restarts++;
+            }
             it = h.iter + 1;
             continue;
         }

@@ -76,9 +76,13 @@ def _i32(a):
 class Context:
     """One GPU + one HIP stream (hipfem_ctx)."""
 
-    def __init__(self, device: int = 0):
+    def __init__(self, device: int = 0, handle=None):
         self.h = C.c_void_p()
-        _check(lib().hipfem_ctx_create(device, C.byref(self.h)), "hipfem_ctx_create")
+        self.owned = handle is None
+        if handle is None:
+            _check(lib().hipfem_ctx_create(device, C.byref(self.h)), "hipfem_ctx_create")
+        else:
+            self.h = C.c_void_p(handle)   # borrowed (e.g. a problem's context): never destroyed here
         self.device = device
 
     def synchronize(self):
@@ -89,9 +93,9 @@ class Context:
         return lib().hipfem_ctx_stream(self.h)
 
     def close(self):
-        if self.h:
+        if self.h and self.owned:
             lib().hipfem_ctx_destroy(self.h)
-            self.h = C.c_void_p()
+        self.h = C.c_void_p()
 
     def __del__(self):
         try:
@@ -452,6 +456,15 @@ class BatchedEvaluator:
                                                     C.c_void_p(yp.addr if yp else 0)), "tangent_product")
         return yf.download()[: d.neq], (yp.download()[: d.npeq] if yp else np.zeros(0))
 
+    def set_surrogate(self, net, band=0.05, tol=1e-4):
+        _check(lib().hfem_model_set_surrogate(self.h, net.h if net is not None else None, C.c_double(band),
+                                              C.c_double(tol)), "set_surrogate")
+
+    def surrogate_stats(self, reset=False):
+        c = np.zeros(3, dtype=np.int64)
+        _check(lib().hfem_model_surrogate_stats(self.h, _p(c), int(bool(reset))), "surrogate_stats")
+        return dict(queued=int(c[0]), accepted=int(c[1]), rejected=int(c[2]))
+
     def commit(self):
         _check(lib().hfem_model_commit(self.h), "commit")
 
@@ -466,3 +479,79 @@ class BatchedEvaluator:
 
     def ncolors(self):
         return lib().hfem_model_ncolors(self.h)
+
+
+class MisesSurrogate:
+    """Learned MisesMat surrogate (hfem_mlp): 13 -> 3 x 128 SiLU -> 27, FP32
+    master weights, HIP training kernels, BF16 MFMA inference."""
+
+    def __init__(self, ctx: Context, E, nu, sig0, H, seed=1):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        mat = np.array([E, nu, sig0, H], dtype=np.float64)
+        lib().hfem_mlp_create.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]
+        _check(lib().hfem_mlp_create(ctx.h, _p(mat), C.c_uint64(seed), C.byref(self.h)), "hfem_mlp_create")
+        self.nparams = lib().hfem_mlp_nparams()
+        s = np.zeros(3)
+        _check(lib().hfem_mlp_scales(self.h, _p(s)), "scales")
+        self.in_scale, self.sig_scale, self.tan_scale = s
+
+    def close(self):
+        if self.h:
+            lib().hfem_mlp_destroy.argtypes = [C.c_void_p]
+            lib().hfem_mlp_destroy.restype = None
+            lib().hfem_mlp_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self):
+        p = np.zeros(self.nparams, dtype=np.float32)
+        _check(lib().hfem_mlp_get_params(self.h, _p(p)), "get_params")
+        return p
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        _check(lib().hfem_mlp_set_params(self.h, _p(p)), "set_params")
+
+    def generate(self, nsamples, seed=7, stdev=2e-4, maxsteps=50):
+        _check(lib().hfem_mlp_generate(self.h, C.c_int64(nsamples), C.c_uint64(seed), C.c_double(stdev),
+                                       C.c_int(maxsteps)), "generate")
+        self.nsamples = int(nsamples)
+
+    def samples(self, first, count):
+        X = np.zeros((count, 13), dtype=np.float32)
+        Y = np.zeros((count, 27), dtype=np.float32)
+        _check(lib().hfem_mlp_get_samples(self.h, C.c_int64(first), C.c_int64(count), _p(X), _p(Y)), "get_samples")
+        return X, Y
+
+    def train(self, steps, batch=4096, lr=1e-3, lr_final=1e-5, seed=11):
+        loss = C.c_double(0)
+        _check(lib().hfem_mlp_train(self.h, C.c_int(steps), C.c_int(batch), C.c_double(lr), C.c_double(lr_final),
+                                    C.c_uint64(seed), C.byref(loss)), "train")
+        return loss.value
+
+    def forward(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.zeros((X.shape[0], 27), dtype=np.float32)
+        _check(lib().hfem_mlp_forward(self.h, C.c_int(X.shape[0]), _p(X), _p(Y)), "forward")
+        return Y
+
+    def gradients(self, X, Y):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.ascontiguousarray(Y, dtype=np.float32)
+        g = np.zeros(self.nparams, dtype=np.float32)
+        loss = C.c_double(0)
+        _check(lib().hfem_mlp_gradients(self.h, C.c_int(X.shape[0]), _p(X), _p(Y), _p(g), C.byref(loss)),
+               "gradients")
+        return loss.value, g
+
+    def infer_bf16(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        Y = np.zeros((X.shape[0], 27), dtype=np.float32)
+        _check(lib().hfem_mlp_infer_bf16(self.h, C.c_int(X.shape[0]), _p(X), _p(Y)), "infer_bf16")
+        return Y

@@ -145,6 +145,31 @@ int hfem_problem_solver_info(hfem_problem *p, int *outer, int *matrix_free)
     });
 }
 
+int hfem_problem_context(hfem_problem *p, hipfem_ctx **ctx)
+{
+    return guarded([&] { *ctx = reinterpret_cast<StructuralProblem *>(p)->giveContext(); });
+}
+
+int hfem_problem_set_surrogate(hfem_problem *p, hfem_mlp *net, double band, double tol)
+{
+    return guarded([&] {
+        hipfemCheck(hfem_model_set_surrogate(reinterpret_cast<StructuralProblem *>(p)->giveBatchedEvaluator(1), net,
+                                             band, tol),
+                    "set_surrogate");
+    });
+}
+
+int hfem_problem_surrogate_stats(hfem_problem *p, long long *counts3, int reset)
+{
+    return guarded([&] {
+        int64_t c[3];
+        hipfemCheck(hfem_model_surrogate_stats(reinterpret_cast<StructuralProblem *>(p)->giveBatchedEvaluator(1), c,
+                                               reset),
+                    "surrogate_stats");
+        for (int i = 0; i < 3; i++) counts3[i] = c[i];
+    });
+}
+
 int hfem_problem_matrix_nnz(hfem_problem *p, long long *nnz)
 {
     return guarded([&] {

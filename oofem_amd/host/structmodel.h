@@ -125,6 +125,7 @@ public:
     void giveProfile(int *n64, double *ms64, int *n32, double *ms32) const;
     SparseMtrx *giveStiffnessMatrix() { return K.get(); }
     SparseLinearSystemNM *giveLinearSolver() { return linSolver.get(); }
+    hipfem_ctx *giveContext() { return ctx; }
 };
 
 }  // namespace oofem

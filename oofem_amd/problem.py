@@ -126,6 +126,22 @@ class StructuralProblem:
                "profile")
         return dict(n64=n64.value, ms64=m64.value, n32=n32.value, ms32=m32.value)
 
+    def context(self):
+        """The problem's GPU context (borrowed), e.g. to create a surrogate on it."""
+        from .hipfem import Context
+        h = C.c_void_p()
+        _check(hostlib().hfem_problem_context(self.h, C.byref(h)), "context")
+        return Context(handle=h.value)
+
+    def set_surrogate(self, net, band=0.05, tol=1e-4):
+        _check(hostlib().hfem_problem_set_surrogate(self.h, net.h if net is not None else None, C.c_double(band),
+                                                    C.c_double(tol)), "set_surrogate")
+
+    def surrogate_stats(self, reset=False):
+        c = (C.c_longlong * 3)()
+        _check(hostlib().hfem_problem_surrogate_stats(self.h, c, int(bool(reset))), "surrogate_stats")
+        return dict(queued=c[0], accepted=c[1], rejected=c[2])
+
     def solver_info(self):
         o, mf = C.c_int(0), C.c_int(0)
         _check(hostlib().hfem_problem_solver_info(self.h, C.byref(o), C.byref(mf)), "solver_info")

@@ -1,0 +1,208 @@
+"""GPU tests of the MisesMat surrogate (hfem_mlp): training data against
+the oracle's return mapping, FP32 forward and backward against the FP64
+restatement in oracle/mlp_oracle.py, training progress, and BF16 MFMA
+inference against the FP32 forward."""
+import numpy as np
+import pytest
+
+from oracle import mlp_oracle as M
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+E, NU, SIG0, H = 210e3, 0.3, 250.0, 2e3
+MP = np.array([1, E, NU, SIG0, H, 0, 0, 0], dtype=np.float64)
+
+
+@pytest.fixture(scope="module")
+def net(ctx):
+    from oofem_amd.hipfem import MisesSurrogate
+    n = MisesSurrogate(ctx, E, NU, SIG0, H, seed=3)
+    yield n
+    n.close()
+
+
+def test_forward_matches_oracle(net):
+    rng = np.random.default_rng(0)
+    X = rng.standard_normal((300, 13)).astype(np.float32)
+    p = net.params().astype(np.float64)
+    got = net.forward(X)
+    ref = M.forward(p, X.astype(np.float64))
+    assert np.abs(got - ref).max() <= 1e-5 * max(1.0, np.abs(ref).max())
+
+
+def test_gradients_match_oracle(net):
+    """Backward kernels (tiled GEMMs, SiLU derivative, column sums) against
+    the FP64 gradient, per parameter block."""
+    rng = np.random.default_rng(1)
+    X = rng.standard_normal((97, 13)).astype(np.float32)       # not a multiple of the 64-row tile
+    Y = rng.standard_normal((97, 27)).astype(np.float32)
+    loss, g = net.gradients(X, Y)
+    lref, gref = M.loss_and_grad(net.params().astype(np.float64), X.astype(np.float64), Y.astype(np.float64))
+    assert abs(loss - lref) <= 1e-5 * lref
+    offs = np.cumsum([0] + [int(np.prod(s)) for s in M.SHAPES])
+    for a, b in zip(offs[:-1], offs[1:]):
+        assert np.linalg.norm(g[a:b] - gref[a:b]) <= 1e-4 * np.linalg.norm(gref[a:b]), (a, b)
+
+
+def test_training_samples_follow_the_return_mapping(net):
+    """Every generated sample is the plain return mapping (oracle) of its own
+    inputs: stress and consistent tangent at (eps, epsp_n, kappa_n)."""
+    net.generate(50000, seed=5)
+    X, Y = net.samples(0, 50000)
+    rng = np.random.default_rng(2)
+    rows = rng.choice(50000, 300, replace=False)
+    s_in = net.in_scale
+    nplastic = 0
+    for r in rows:
+        x = X[r].astype(np.float64) / s_in
+        epsp_n, kap_n = x[6:12], x[12]
+        eps = x[:6] + epsp_n
+        sig, epsp, kap, sdev = oracle.mises_return(MP, eps, epsp_n, kap_n)
+        D = oracle.mises_tangent(MP, kap_n, kap, sdev)
+        nplastic += kap > kap_n
+        assert np.abs(Y[r, :6] / net.sig_scale - sig).max() <= 2e-4 * SIG0
+        iu = np.triu_indices(6)
+        assert np.abs(Y[r, 6:] / net.tan_scale - D[iu]).max() <= 1e-4 * E
+    assert 0.05 * len(rows) < nplastic < 0.95 * len(rows)      # both branches are sampled
+    # path starts: virgin state
+    assert np.any((X[:, 6:] == 0).all(axis=1))
+
+
+def test_training_reduces_loss(ctx):
+    from oofem_amd.hipfem import MisesSurrogate
+    n = MisesSurrogate(ctx, E, NU, SIG0, H, seed=9)
+    n.generate(400000, seed=6)
+    X, Y = n.samples(0, 4096)
+    l0, _ = n.gradients(X, Y)
+    l1 = n.train(1500, batch=2048, lr=2e-3, lr_final=1e-4)
+    l1b, _ = n.gradients(X, Y)
+    assert l1b < 0.05 * l0, (l0, l1, l1b)
+    n.close()
+
+
+def test_bf16_inference_matches_fp32(net):
+    """MFMA inference (BF16 operands, FP32 accumulation) against the FP32
+    forward, including a ragged last tile."""
+    X, _ = net.samples(0, 5000 + 17) if getattr(net, "nsamples", 0) >= 5017 else (None, None)
+    if X is None:
+        net.generate(6000, seed=8)
+        X, _ = net.samples(0, 5017)
+    y32 = net.forward(X)
+    y16 = net.infer_bf16(X)
+    err = np.linalg.norm(y16 - y32, axis=0) / np.maximum(np.linalg.norm(y32, axis=0), 1e-3 * np.sqrt(len(X)))
+    assert err.max() <= 3e-2, err
+    assert np.abs(y16).max() > 0
+
+
+@pytest.fixture(scope="module")
+def trained(ctx):
+    """A network trained briefly on paths through the plain return mapping
+    (a reduced training budget; the benchmark trains the full one)."""
+    from oofem_amd.hipfem import MisesSurrogate
+    n = MisesSurrogate(ctx, E, NU, SIG0, H, seed=21)
+    n.generate(1_000_000, seed=22)
+    n.loss = n.train(4000, batch=4096, lr=2e-3, lr_final=2e-5, seed=23)
+    yield n
+    n.close()
+
+
+def _plastic_block(ctx, amp):
+    from oofem_amd import meshgen
+    from oofem_amd.hipfem import BatchedEvaluator, HipCSRSparseMtrx
+    d = meshgen.cantilever_hex(4, 4, 8, jitter=0.2, seed=5)
+    d.mat = np.array([MP])
+    d.emat[:] = 0
+    ev = BatchedEvaluator(ctx, d)
+    K = HipCSRSparseMtrx(ctx)
+    ev.build_structure(K)
+    rng = np.random.default_rng(4)
+    u = amp * SIG0 / E * rng.standard_normal(d.neq)
+    return d, ev, K, u
+
+
+@pytest.mark.parametrize("tol", [1e-4, 1e-2])
+def test_surrogate_update_is_checked_and_falls_back(ctx, trained, tol):
+    """Internal forces with the surrogate: queued points either pass the
+    return-mapping residual check (then their stress is within a few tol *
+    sigma_y of the plain return mapping, the bound the check implies) or
+    take the plain update; points outside the queue are bitwise the plain
+    path's."""
+    d, ev, K, u = _plastic_block(ctx, 0.6)
+    up = np.zeros(d.npeq)
+    f_ref, _, _ = ev.internal_forces(u, up, update_state=True)
+    s_ref = ev.get_state()
+    ev.reset_state()
+    ev.set_surrogate(trained, band=0.05, tol=tol)
+    ev.surrogate_stats(reset=True)
+    f_sur, _, _ = ev.internal_forces(u, up, update_state=True)
+    s_sur = ev.get_state()
+    st = ev.surrogate_stats()
+    assert st["queued"] > 0.2 * len(s_ref)
+    assert st["accepted"] + st["rejected"] == st["queued"]
+    sy = SIG0 + H * s_ref[:, 6]
+    err = np.abs(s_sur[:, 20:26] - s_ref[:, 20:26]).max(axis=1) / sy
+    assert err.max() <= 4 * tol, err.max()
+    same = err == 0
+    assert same.sum() >= len(s_ref) - st["accepted"]        # everything not accepted is the plain result
+    assert np.linalg.norm(f_sur - f_ref) <= 4 * tol * np.linalg.norm(f_ref)
+    ev.set_surrogate(None)
+
+
+def test_trained_network_gains_over_untrained(ctx, trained):
+    """What training buys: at a loose acceptance tolerance the trained network
+    serves a large share of the plastic points, the untrained one (same
+    architecture, initial weights) essentially none."""
+    from oofem_amd.hipfem import MisesSurrogate
+    d, ev, K, u = _plastic_block(ctx, 0.6)
+    up = np.zeros(d.npeq)
+    fresh = MisesSurrogate(ctx, E, NU, SIG0, H, seed=21)
+    share = {}
+    for name, net in (("trained", trained), ("untrained", fresh)):
+        ev.reset_state()
+        ev.set_surrogate(net, band=0.05, tol=2e-2)
+        ev.surrogate_stats(reset=True)
+        ev.internal_forces(u, up, update_state=True)
+        st = ev.surrogate_stats()
+        share[name] = st["accepted"] / max(st["queued"], 1)
+    ev.set_surrogate(None)
+    fresh.close()
+    assert share["trained"] >= 0.5 and share["untrained"] <= 0.05, share
+
+
+def test_newton_with_surrogate_converges_to_plain_solution(ctx, trained):
+    """StaticStructural + NRSolver steps with the surrogate inside the
+    material update: every step converges (rtolf 1e-8), and displacements
+    stay within the acceptance tolerance's reach of the plain run."""
+    from oofem_amd import meshgen
+    from oofem_amd.problem import StructuralProblem
+    from tests import nlcases
+    d = meshgen.cantilever_hex(3, 3, 8, jitter=0.1, seed=2)
+    d.mat = np.array([MP])
+    d.emat[:] = 0
+    top = np.arange(d.nnode - 16, d.nnode)
+    d.bc_mask[top, 2] = True
+    d.bc_value[top, 2] = 4 * SIG0 / E * d.coords[:, 2].max()
+    d2 = meshgen.Deck(d.name, d.etype, d.coords, d.conn, d.emat, d.mat, d.bc_mask, d.bc_value)
+    res = {}
+    for use in (False, True):
+        P = StructuralProblem(d2, lstol=1e-12, rtolf=1e-8, maxiter=40, manrmsteps=1)
+        P.set_load(np.zeros(d2.neq), np.zeros(d2.npeq))
+        if use:
+            from oofem_amd.hipfem import MisesSurrogate
+            net = MisesSurrogate(P.context(), E, NU, SIG0, H, seed=21)
+            net.set_params(trained.params())
+            P.set_surrogate(net, band=0.05, tol=1e-3)
+        for k in range(1, 6):
+            P.set_prescribed(d2.prescribed_values(k / 5))
+            rep = P.nonlinear_step()
+            assert rep["status"] == 1, (use, k, rep)
+            P.update()
+        res[use] = P.solution()
+        if use:
+            res["stats"] = P.surrogate_stats()
+            P.set_surrogate(None)
+            net.close()
+        P.close()
+    assert res["stats"]["accepted"] > 0
+    assert np.linalg.norm(res[True] - res[False]) <= 1e-2 * np.linalg.norm(res[False])

@@ -321,3 +321,27 @@ def test_initial_guess_makes_tet_bar_step_converge():
     assert status == 1 and nit == 1 and err <= 1e-8
     s = st.reshape(-1, 38)
     assert (s[:, 13] > s[:, 6]).sum() == 0              # elastic, as the ramp prescribes
+
+
+def test_mlp_oracle_gradient_matches_finite_differences():
+    """The FP64 restatement of the surrogate's loss gradient (which checks
+    the HIP backward kernels) against central differences of its loss, on
+    parameters of every layer."""
+    from oracle import mlp_oracle as M
+    rng = np.random.default_rng(0)
+    p = rng.uniform(-0.2, 0.2, M.NPARAM)
+    X = rng.standard_normal((7, M.NIN))
+    Y = rng.standard_normal((7, M.NOUT))
+    loss, g = M.loss_and_grad(p, X, Y)
+    offs = np.cumsum([0] + [int(np.prod(s)) for s in M.SHAPES])
+    idx = [o + k for o in offs[:-1] for k in (0, 5)]
+    h = 1e-6
+    for i in idx:
+        e = np.zeros(M.NPARAM)
+        e[i] = h
+        fd = (M.loss_and_grad(p + e, X, Y)[0] - M.loss_and_grad(p - e, X, Y)[0]) / (2 * h)
+        assert abs(fd - g[i]) <= 1e-6 * max(1.0, abs(g[i])) + 1e-9, (i, fd, g[i])
+    # the network is the composition it claims to be: a zero last layer gives the bias
+    p2 = p.copy()
+    p2[offs[6]:offs[7]] = 0.0
+    assert np.allclose(M.forward(p2, X), np.broadcast_to(p2[offs[7]:], (7, M.NOUT)))

@@ -121,35 +121,82 @@ def run_gpu(args, deck, precision):
     return out
 
 
-def run_newton(args, deck, precision):
-    """StaticStructural + NRSolver steps (full Newton, consistent tangent,
-    rtolf 1e-8) on the MisesMat tet bar, prescribed end displacement ramped
-    over --nl-total steps; the first step is warm-up."""
+def run_newton(args, deck, precision, warm=1, steps=None, total=None, surrogate=None, keep_state=False):
+    """StaticStructural + NRSolver load steps (full Newton, consistent tangent,
+    rtolf 1e-8, tangent initial guess) with the prescribed end displacement
+    ramped over `total` steps: `warm` untimed steps, then `steps` timed ones.
+    surrogate: (params, band, tol) of a trained MisesMat network, or None."""
     import numpy as np
 
     from oofem_amd.problem import StructuralProblem
+    steps = args.nl_steps if steps is None else steps
+    total = args.nl_total if total is None else total
     P = StructuralProblem(deck, lstol=args.nl_lstol, lsiter=args.maxit, lsprecond=1, mixed=precision,
                           irtol=args.inner_tol, rtolf=1e-8, maxiter=60, manrmsteps=1)
     P.set_load(np.zeros(deck.neq), np.zeros(deck.npeq))
+    net = None
+    if surrogate is not None:
+        from oofem_amd.hipfem import MisesSurrogate
+        m = deck.mat[0]
+        net = MisesSurrogate(P.context(), m[1], m[2], m[3], m[4])
+        net.set_params(surrogate[0])
+        P.set_surrogate(net, band=surrogate[1], tol=surrogate[2])
     reports = []
     t0 = None
-    for k in range(1, 1 + 1 + args.nl_steps):
-        P.set_prescribed(deck.prescribed_values(k / args.nl_total))
-        if k == 2:
+    for k in range(1, warm + steps + 1):
+        P.set_prescribed(deck.prescribed_values(k / total))
+        if k == warm + 1:
             P.set_profiling(True)
+            if net is not None:
+                P.surrogate_stats(reset=True)
             t0 = time.perf_counter()
         rep = P.nonlinear_step()
         if rep["status"] != 1:
             raise RuntimeError(f"Newton step {k} failed: {rep}")
         P.update()
         reports.append(rep)
-        log(f"[bench] newton step {k}: {rep}")
+        log(f"[bench] precision {precision}{' +surrogate' if net else ''} newton step {k}: {rep}")
     dt = time.perf_counter() - t0
     p = P.profile()
+    out = dict(dt=dt, steps=steps, newton=[r["newton"] for r in reports[warm:]],
+               pcg=[r["pcg"] for r in reports[warm:]], n_spmv=p["n64"], ms_spmv=p["ms64"], n_spmv32=p["n32"],
+               ms_spmv32=p["ms32"], nnz=P.matrix_nnz(), u=P.solution(), info=P.solver_info())
+    if keep_state:
+        gs = P.state()
+        out["stress"] = gs[:, 20:26].copy()
+        out["kappa"] = gs[:, 6].copy()
+        del gs
+    if net is not None:
+        out["surrogate"] = P.surrogate_stats()
+        P.set_surrogate(None)
+        net.close()
     P.close()
-    timed = reports[1:]
-    return dict(dt=dt, steps=len(timed), newton=[r["newton"] for r in timed], pcg=[r["pcg"] for r in timed],
-                n_spmv=p["n64"], ms_spmv=p["ms64"], n_spmv32=p["n32"], ms_spmv32=p["ms32"])
+    return out
+
+
+def train_surrogate(args, deck):
+    """Training set and training on the GPU (untimed setup of the surrogate
+    variant): --sur-samples strain paths samples through the plain return
+    mapping, --sur-steps Adam steps."""
+    from oofem_amd.hipfem import Context, MisesSurrogate
+    m = deck.mat[0]
+    ctx = Context(0)
+    net = MisesSurrogate(ctx, m[1], m[2], m[3], m[4], seed=args.sur_seed)
+    t = time.perf_counter()
+    net.generate(args.sur_samples, seed=args.sur_seed + 1)
+    ctx.synchronize()
+    t_gen = time.perf_counter() - t
+    t = time.perf_counter()
+    loss = net.train(args.sur_steps, batch=args.sur_batch, lr=args.sur_lr, lr_final=args.sur_lr * 1e-2,
+                     seed=args.sur_seed + 2)
+    t_train = time.perf_counter() - t
+    params = net.params()
+    net.close()
+    ctx.close()
+    log(f"[bench] surrogate: {args.sur_samples} samples in {t_gen:.1f} s, {args.sur_steps} steps of "
+        f"{args.sur_batch} in {t_train:.1f} s, final batch loss {loss:.3e}")
+    return params, dict(samples=args.sur_samples, steps=args.sur_steps, batch=args.sur_batch, gen_s=t_gen,
+                        train_s=t_train, final_loss=loss)
 
 
 def measured_traffic(kernel, workload, args):
@@ -199,12 +246,127 @@ def cpu_baseline(args, deck_full, gpu_iters, nnz_full):
                         f"{gpu_iters} iterations (estimated step {t_step:.1f} s)"))
 
 
+def cpu_baseline_newton(args, deck, rn):
+    """Oracle (one thread) on a slab of the same plasticity problem: tangent
+    assembly, f_int with the return mapping and CG iterations timed there,
+    scaled to the full step by element count, nnz and the GPU run's Newton
+    and PCG iteration counts."""
+    import numpy as np
+
+    from oofem_amd import meshgen
+    from oracle import oracle
+    s = meshgen.config3(nx=args.nx, ny=args.ny, nz=args.cpu_sample_nz)
+    st = oracle.state_new(s)
+    up = s.prescribed_values(3.0 / args.c3_total)
+    t = time.perf_counter()
+    _, _, _ = oracle.internal_forces(s, np.zeros(s.neq), up, state=st, update_state=True)
+    t_fint = time.perf_counter() - t
+    t = time.perf_counter()
+    rp, ci, val = oracle.csr(s, st, tangent=True)
+    t_asm = time.perf_counter() - t
+    f = np.ones(s.neq)
+    nit = args.cpu_sample_iters
+    t = time.perf_counter()
+    oracle.pcg(rp, ci, val, f, tol=1e-30, maxit=nit)
+    t_cg = (time.perf_counter() - t) / nit
+    scale_e = deck.nelem / s.nelem
+    scale_nnz = rn["nnz"] / len(val)
+    newton = sum(rn["newton"]) / rn["steps"]
+    pcg = sum(rn["pcg"]) / rn["steps"]
+    # per Newton iteration: one tangent assembly and one f_int (+1 f_int per step)
+    t_step = ((newton + 1) * t_fint + newton * t_asm) * scale_e + pcg * t_cg * scale_nnz
+    return dict(value=deck.neq / t_step, unit="DOFs/s", cores=1, kind="port",
+                sample=(f"oracle (C restatement, 1 thread) on a {args.nx}x{args.ny}x{args.cpu_sample_nz} slab of the "
+                        f"same problem ({s.nelem} elements, {len(val)} nnz): f_int with return mapping {t_fint:.2f} s, "
+                        f"tangent assembly {t_asm:.2f} s, {nit} CG iterations at {t_cg * 1e3:.1f} ms; scaled by "
+                        f"elements x{scale_e:.1f}, nnz x{scale_nnz:.1f} and the GPU's {newton:.1f} Newton / "
+                        f"{pcg:.0f} PCG iterations per step (estimated step {t_step:.1f} s)"))
+
+
+def main_newton(args):
+    """Newton-step throughput on the LSpace plasticity block (config 3):
+    plain FP64 (hipcsr + hippcg FP64 + exact MisesMat) against the mixed
+    solver and the mixed solver with the trained surrogate."""
+    import numpy as np
+
+    from oofem_amd import meshgen
+    t = time.perf_counter()
+    deck = meshgen.config3(nx=args.nx, ny=args.ny, nz=args.nz)
+    log(f"[bench] deck {deck.name}: nelem={deck.nelem} neq={deck.neq} ({time.perf_counter() - t:.1f} s)")
+    kw = dict(warm=args.c3_warm, steps=args.c3_steps, total=args.c3_total)
+    runs = {"fp64": run_newton(args, deck, 0, keep_state=True, **kw)}
+    if args.precision in ("both", "mixed", "all"):
+        runs["mixed"] = run_newton(args, deck, 2, **kw)
+    sur_info = None
+    if args.precision in ("both", "all") and args.sur_steps > 0:
+        params, sur_info = train_surrogate(args, deck)
+        runs["mixed_surrogate"] = run_newton(args, deck, 2, surrogate=(params, args.sur_band, args.sur_tol),
+                                             keep_state=True, **kw)
+    base = runs["fp64"]
+    n = deck.neq
+    variants = {}
+    for name, r in runs.items():
+        v = dict(dofs_per_s=n * r["steps"] / r["dt"], ms_per_step=r["dt"] / r["steps"] * 1e3,
+                 newton_iters=r["newton"], pcg_iters=r["pcg"])
+        if name != "fp64":
+            v["speedup_vs_fp64"] = v["dofs_per_s"] / (n * base["steps"] / base["dt"])
+            v["u_rel_err_vs_fp64"] = float(np.linalg.norm(r["u"] - base["u"]) / np.linalg.norm(base["u"]))
+            v["reliable_updates_last_solve"] = r["info"]["outer"]
+            v["element_operator"] = r["info"]["matrix_free"]
+            if r["n_spmv32"]:
+                v["fp32_operator_avg_ms"] = r["ms_spmv32"] / r["n_spmv32"]
+        if "surrogate" in r:
+            v["surrogate"] = dict(r["surrogate"], band=args.sur_band, tol=args.sur_tol, training=sur_info)
+        if "stress" in r and name != "fp64":
+            sy = 250.0 + 2e3 * base["kappa"]
+            e = np.abs(r["stress"] - base["stress"]).max(axis=1) / sy
+            v["stress_err_vs_fp64"] = dict(max_rel=float(e.max()), rms_rel=float(np.sqrt((e * e).mean())))
+        variants[name] = v
+        log(f"[bench] {name}: {v}")
+    r = base
+    roof = None
+    if r["n_spmv"]:
+        avg_ms = r["ms_spmv"] / r["n_spmv"]
+        # scalar CSR PCG SpMV: value + column per nonzero, row pointer, gather p, write q
+        bytes_per = 12.0 * r["nnz"] + 20.0 * n
+        ach = bytes_per / (avg_ms * 1e-3) / 1e9
+        kname = "k_pcg_spmv_p<double, double> (lane form)"
+        roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 4),
+                    traffic=measured_traffic(kname, deck.name, args), kernel=kname, avg_launch_ms=round(avg_ms, 4),
+                    launches=r["n_spmv"], algorithmic_bytes=int(bytes_per))
+    cpu = None if args.no_cpu_baseline else cpu_baseline_newton(args, deck, r)
+    out = {
+        "metric": METRIC, "value": n * r["steps"] / r["dt"], "unit": "DOFs/s", "n_gpus": 1, "steps": r["steps"],
+        "warmup": args.c3_warm, "ms_per_step": r["dt"] / r["steps"] * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": deck.name, "elements": deck.nelem, "free_dofs": n, "nnz": r["nnz"],
+                   "step": (f"StaticStructural load step {args.c3_warm + 1}..{args.c3_warm + args.c3_steps} of "
+                            f"{args.c3_total} (prescribed end displacement, 5x yield strain in total): tangent "
+                            "initial guess, full Newton with consistent tangent, rtolf 1e-8, Jacobi-PCG to "
+                            f"{args.nl_lstol:g}"),
+                   "precision": "fp64", "variants": variants},
+        "roofline": roof, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["cfg1", "cfg0"], default="cfg1")
+    ap.add_argument("--workload", choices=["cfg1", "cfg0", "cfg3"], default="cfg1")
+    ap.add_argument("--c3-warm", type=int, default=2, help="untimed load steps before the timed ones (cfg3)")
+    ap.add_argument("--c3-steps", type=int, default=2, help="timed load steps (cfg3)")
+    ap.add_argument("--c3-total", type=int, default=8, help="load steps of the full ramp (cfg3)")
+    ap.add_argument("--sur-samples", type=int, default=20_000_000)
+    ap.add_argument("--sur-steps", type=int, default=20000)
+    ap.add_argument("--sur-batch", type=int, default=4096)
+    ap.add_argument("--sur-lr", type=float, default=2e-3)
+    ap.add_argument("--sur-seed", type=int, default=31)
+    ap.add_argument("--sur-band", type=float, default=0.05)
+    ap.add_argument("--sur-tol", type=float, default=1e-4)
     ap.add_argument("--nx", type=int, default=64)
     ap.add_argument("--ny", type=int, default=64)
     ap.add_argument("--nz", type=int, default=256)
@@ -231,6 +393,8 @@ def main():
               "(the multi-GPU slab decomposition is a design, not code).", file=sys.stderr)
         return 2
 
+    if args.workload == "cfg3":
+        return main_newton(args)
     t = time.perf_counter()
     deck = make_deck(args)
     log(f"[bench] deck {deck.name}: nnode={deck.nnode} nelem={deck.nelem} neq={deck.neq} "

@@ -135,9 +135,11 @@ int hipcsr_get_spmv_shape(const hipcsr_mtx *m, int *lanes, int *unroll, int *non
  * form) consecutive rows per 256-thread block, their contiguous value and
  * column range read with unit stride; unroll 2, 4 or 6. */
 int hipcsr_set_spmv_stream(hipcsr_mtx *m, int rows_per_block, int unroll);
-/* node-block SpMV form over the same CSR arrays, available when every free
- * node is fully free and numbered consecutively (then CSR order is a 3x3
- * block layout); on by default for such patterns. */
+/* node-block SpMV forms over the same CSR arrays: the 3x3 block form when
+ * every free node is fully free and numbered consecutively (then CSR order
+ * is a 3x3 block layout), else the row-group form when rows of a node share
+ * one column list (partially constrained nodes); on by default where one
+ * applies.  hipcsr_is_nodeblocked: 1 block form, 2 row-group form, 0 none. */
 int hipcsr_set_spmv_nodeblock(hipcsr_mtx *m, int on);
 int hipcsr_is_nodeblocked(const hipcsr_mtx *m);
 /* element-by-element operator (no counterpart in the reference): after

@@ -175,6 +175,24 @@ __global__ __launch_bounds__(256) void k_spmv_nb(int nbr, const int32_t *__restr
     }
 }
 
+// row-group SpMV: one 32-lane half wave per row group (see grp_row)
+template <typename MT>
+__global__ __launch_bounds__(256) void k_spmv_grp(int ngrp, const int32_t *__restrict__ grow,
+                                                  const int32_t *__restrict__ gch, const uint32_t *__restrict__ ccol,
+                                                  const int32_t *__restrict__ coff, const int32_t *__restrict__ rowptr,
+                                                  const MT *__restrict__ val, const double *__restrict__ x,
+                                                  double *__restrict__ y)
+{
+    const int lane = threadIdx.x % 32;
+    for (int g = blockIdx.x * 8 + threadIdx.x / 32; g < ngrp; g += gridDim.x * 8) {
+        const int r0 = grow[g], nr = grow[g + 1] - r0, c0 = gch[g], nch = gch[g + 1] - c0;
+        double acc[3];
+        grp_row<double>(r0, nr, c0, nch, lane, rowptr, ccol, coff, val, [&](int c) { return x[c]; }, acc);
+        if (lane == 0)
+            for (int r = 0; r < nr; r++) y[r0 + r] = acc[r];
+    }
+}
+
 template <typename VT>
 static int launch_spmv(const hipcsr_mtx *m, const SpmvShape &sh, hipStream_t st, int n, const int32_t *rp,
                        const int32_t *ci, const VT *v, const double *x, double *y)
@@ -183,6 +201,10 @@ static int launch_spmv(const hipcsr_mtx *m, const SpmvShape &sh, hipStream_t st,
     if (sh.nodeblock && m->nodeblocked) {
         const int grid = std::max(1, std::min((m->nbr + 7) / 8, 1 << 20));
         k_spmv_nb<VT><<<grid, 256, 0, st>>>(m->nbr, m->d_bptr, m->d_bcol, v, x, y);
+        ok = true;
+    } else if (sh.nodeblock && m->grouped) {
+        const int grid = std::max(1, std::min((m->ngrp + 7) / 8, 1 << 20));
+        k_spmv_grp<VT><<<grid, 256, 0, st>>>(m->ngrp, m->d_grow, m->d_gch, m->d_ccol, m->d_coff, rp, v, x, y);
         ok = true;
     } else if (sh.stream) {
         const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((n + sh.rpb - 1) / sh.rpb, 1 << 20));
@@ -211,10 +233,15 @@ int csr_free(hipcsr_mtx *m)
 {
     for (void *p : {(void *)m->d_rowptr, (void *)m->d_colind, (void *)m->d_diag, (void *)m->d_val,
                     (void *)m->d_val32, (void *)m->d_tptr, (void *)m->d_tcol, (void *)m->d_tmap,
-                    (void *)m->d_bptr, (void *)m->d_bcol})
+                    (void *)m->d_bptr, (void *)m->d_bcol, (void *)m->d_grow, (void *)m->d_gch,
+                    (void *)m->d_ccol, (void *)m->d_coff})
         if (p) HIPFEM_CHECK(hipFree(p));
     m->d_rowptr = m->d_colind = m->d_diag = m->d_tptr = m->d_tcol = m->d_tmap = nullptr;
     m->d_bptr = m->d_bcol = nullptr;
+    m->d_grow = m->d_gch = m->d_coff = nullptr;
+    m->d_ccol = nullptr;
+    m->grouped = false;
+    m->ngrp = 0;
     m->d_val = nullptr;
     m->d_val32 = nullptr;
     m->has_structure = false;
@@ -249,6 +276,45 @@ static bool node_blocked(int n, const std::vector<int32_t> &rp, const std::vecto
     return true;
 }
 
+// Row-group view: consecutive rows with identical column lists (at most 3)
+// form a group; each group's column list is cut into chunks of up to 3
+// consecutive columns.  Worth using when groups and chunks are mostly full
+// (FEM patterns with some partially constrained nodes).
+static bool row_groups(int n, const std::vector<int32_t> &rp, const std::vector<int32_t> &ci,
+                       std::vector<int32_t> &grow, std::vector<int32_t> &gch, std::vector<uint32_t> &ccol,
+                       std::vector<int32_t> &coff)
+{
+    if (n == 0 || n >= (int)GRP_COL_MASK) return false;
+    grow.assign(1, 0);
+    gch.assign(1, 0);
+    ccol.clear();
+    coff.clear();
+    int r = 0;
+    int64_t group_cols = 0;
+    while (r < n) {
+        const int len = rp[r + 1] - rp[r];
+        group_cols += len;
+        int nr = 1;
+        while (nr < 3 && r + nr < n && rp[r + nr + 1] - rp[r + nr] == len &&
+               std::equal(ci.begin() + rp[r], ci.begin() + rp[r + 1], ci.begin() + rp[r + nr]))
+            nr++;
+        const int32_t *c = ci.data() + rp[r];
+        for (int i = 0; i < len;) {
+            int L = 1;
+            while (L < 3 && i + L < len && c[i + L] == c[i] + L) L++;
+            ccol.push_back((uint32_t)c[i] | ((uint32_t)(L - 1) << 30));
+            coff.push_back(i);
+            i += L;
+        }
+        r += nr;
+        grow.push_back(r);
+        gch.push_back((int32_t)ccol.size());
+    }
+    const double rows_per_group = (double)n / (double)(grow.size() - 1);
+    const double cols_per_chunk = ccol.empty() ? 0.0 : (double)group_cols / (double)ccol.size();
+    return rows_per_group >= 2.5 && cols_per_chunk >= 2.5;
+}
+
 int csr_alloc(hipcsr_mtx *m, int n, const std::vector<int32_t> &rowptr, const std::vector<int32_t> &colind)
 {
     int rc = csr_free(m);
@@ -268,8 +334,27 @@ int csr_alloc(hipcsr_mtx *m, int n, const std::vector<int32_t> &rowptr, const st
     std::vector<int32_t> bptr, bcol;
     m->nodeblocked = node_blocked(n, rowptr, colind, bptr, bcol);
     m->nbr = m->nodeblocked ? n / 3 : 0;
-    m->shape.nodeblock = m->nodeblocked ? 1 : 0;
+    std::vector<int32_t> grow, gch, coff;
+    std::vector<uint32_t> ccol;
+    m->grouped = !m->nodeblocked && row_groups(n, rowptr, colind, grow, gch, ccol, coff);
+    m->ngrp = m->grouped ? (int)grow.size() - 1 : 0;
+    m->shape.nodeblock = (m->nodeblocked || m->grouped) ? 1 : 0;
     HIPFEM_CHECK(hipSetDevice(m->ctx->device));
+    if (m->grouped) {
+        HIPFEM_CHECK(hipMalloc(&m->d_grow, sizeof(int32_t) * grow.size()));
+        HIPFEM_CHECK(hipMalloc(&m->d_gch, sizeof(int32_t) * gch.size()));
+        HIPFEM_CHECK(hipMalloc(&m->d_ccol, sizeof(uint32_t) * std::max<size_t>(1, ccol.size())));
+        HIPFEM_CHECK(hipMalloc(&m->d_coff, sizeof(int32_t) * std::max<size_t>(1, coff.size())));
+        hipStream_t s0 = m->ctx->stream;
+        HIPFEM_CHECK(hipMemcpyAsync(m->d_grow, grow.data(), sizeof(int32_t) * grow.size(), hipMemcpyHostToDevice, s0));
+        HIPFEM_CHECK(hipMemcpyAsync(m->d_gch, gch.data(), sizeof(int32_t) * gch.size(), hipMemcpyHostToDevice, s0));
+        if (!ccol.empty()) {
+            HIPFEM_CHECK(hipMemcpyAsync(m->d_ccol, ccol.data(), sizeof(uint32_t) * ccol.size(), hipMemcpyHostToDevice, s0));
+            HIPFEM_CHECK(hipMemcpyAsync(m->d_coff, coff.data(), sizeof(int32_t) * coff.size(), hipMemcpyHostToDevice, s0));
+        }
+        // the host vectors die with this scope: finish the copies first
+        HIPFEM_CHECK(hipStreamSynchronize(s0));
+    }
     if (m->nodeblocked) {
         HIPFEM_CHECK(hipMalloc(&m->d_bptr, sizeof(int32_t) * (m->nbr + 1)));
         HIPFEM_CHECK(hipMalloc(&m->d_bcol, sizeof(int32_t) * std::max<size_t>(1, bcol.size())));
@@ -930,14 +1015,17 @@ extern "C" int hipcsr_set_spmv_stream(hipcsr_mtx *m, int rows_per_block, int unr
 extern "C" int hipcsr_set_spmv_nodeblock(hipcsr_mtx *m, int on)
 {
     HIPFEM_REQUIRE(m && m->has_structure, HIPFEM_EINVAL, "set_spmv_nodeblock: no structure");
-    HIPFEM_REQUIRE(!on || m->nodeblocked, HIPFEM_EINVAL,
-                   "node-block SpMV needs every free node fully free and numbered consecutively");
+    HIPFEM_REQUIRE(!on || m->nodeblocked || m->grouped, HIPFEM_EINVAL,
+                   "node-block SpMV needs a node-structured pattern (rows of a node sharing one column list)");
     m->shape.nodeblock = on ? 1 : 0;
     if (on) m->shape.stream = 0;
     return 0;
 }
 
-extern "C" int hipcsr_is_nodeblocked(const hipcsr_mtx *m) { return m && m->nodeblocked ? 1 : 0; }
+extern "C" int hipcsr_is_nodeblocked(const hipcsr_mtx *m)
+{
+    return !m ? 0 : m->nodeblocked ? 1 : m->grouped ? 2 : 0;
+}
 
 extern "C" int hipcsr_get_spmv_shape(const hipcsr_mtx *m, int *lanes, int *unroll, int *nontemporal)
 {

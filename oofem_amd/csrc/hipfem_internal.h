@@ -163,6 +163,11 @@ struct hipcsr_mtx {
     bool nodeblocked = false;
     int nbr = 0;                               // node rows = n / 3
     int32_t *d_bptr = nullptr, *d_bcol = nullptr;
+    // row-group view (patterns with partially constrained nodes), see grp_row
+    bool grouped = false;
+    int ngrp = 0;
+    int32_t *d_grow = nullptr, *d_gch = nullptr, *d_coff = nullptr;
+    uint32_t *d_ccol = nullptr;
     // element-by-element view: set when the values were assembled by a batched
     // evaluator and not changed since (mf_version == version); the FP32 PCG
     // of precision 2 then applies the element operator instead of the stored

@@ -185,6 +185,34 @@ __global__ __launch_bounds__(256) void k_pcg_spmv_p_nb(int nbr, const int32_t *_
     if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
+// row-group form of k_pcg_spmv_p (partially constrained node patterns)
+template <typename VT, typename MT>
+__global__ __launch_bounds__(256) void k_pcg_spmv_p_grp(int ngrp, const int32_t *__restrict__ grow,
+                                                        const int32_t *__restrict__ gch,
+                                                        const uint32_t *__restrict__ ccol,
+                                                        const int32_t *__restrict__ coff,
+                                                        const int32_t *__restrict__ rowptr, const MT *__restrict__ val,
+                                                        const VT *__restrict__ p, VT *__restrict__ q,
+                                                        const PcgState *__restrict__ st, double *__restrict__ partials)
+{
+    __shared__ double sh[4];
+    if (halted(st)) return;
+    const int lane = threadIdx.x % 32;
+    double acc = 0.0;
+    for (int g = blockIdx.x * 8 + threadIdx.x / 32; g < ngrp; g += gridDim.x * 8) {
+        const int r0 = grow[g], nr = grow[g + 1] - r0, c0 = gch[g], nch = gch[g + 1] - c0;
+        VT a[3];
+        grp_row<VT>(r0, nr, c0, nch, lane, rowptr, ccol, coff, val, [&](int c) { return p[c]; }, a);
+        if (lane == 0)
+            for (int r = 0; r < nr; r++) {
+                q[r0 + r] = a[r];
+                acc += (double)p[r0 + r] * (double)a[r];
+            }
+    }
+    const double t = block_sum<256>(acc, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
 // p = z (first iteration) or p = z + beta p, in place
 template <typename VT>
 __global__ __launch_bounds__(256) void k_pcg_pupdate(int n, const VT *__restrict__ z, VT *__restrict__ p,
@@ -662,10 +690,12 @@ static int pcg_run(hippcg_solver *s, hipcsr_mtx *A, const MT *val, const VT *b, 
     const SpmvShape sh = A->shape;
     const int gv = vec_grid(n);
     const bool nodeblock = sh.nodeblock && A->nodeblocked;
+    const bool grouped = sh.nodeblock && !A->nodeblocked && A->grouped;
     const int gs = nodeblock    ? std::max(1, std::min((A->nbr + 7) / 8, MAX_GRID))
+                   : grouped   ? std::max(1, std::min((A->ngrp + 7) / 8, MAX_GRID))
                    : sh.stream ? std::max(1, std::min((n + sh.rpb - 1) / sh.rpb, MAX_GRID))
                                : spmv_grid(n, sh.lanes);
-    const bool fused = s->fused && !sh.stream && !nodeblock;  // these forms gather a ready p
+    const bool fused = s->fused && !sh.stream && !nodeblock && !grouped;  // these forms gather a ready p
     int rc = ensure_partials(s->ctx, 3 * (size_t)std::max(gv, gs));
     if (rc) return rc;
     double *part = s->ctx->d_partials;
@@ -707,6 +737,9 @@ static int pcg_run(hippcg_solver *s, hipcsr_mtx *A, const MT *val, const VT *b, 
             if (prof) HIPFEM_CHECK(hipEventRecord(s->ev[2 * (it - 1)], st));
             if (nodeblock)
                 k_pcg_spmv_p_nb<VT, MT><<<gs, 256, 0, st>>>(A->nbr, A->d_bptr, A->d_bcol, val, p0, q, s->st, part);
+            else if (grouped)
+                k_pcg_spmv_p_grp<VT, MT><<<gs, 256, 0, st>>>(A->ngrp, A->d_grow, A->d_gch, A->d_ccol, A->d_coff,
+                                                             A->d_rowptr, val, p0, q, s->st, part);
             else if (fused)
                 launch_pcg_spmv_fused<VT, MT>(sh, gs, st, n, A->d_rowptr, A->d_colind, val, z, pold, pnew, q, s->st,
                                               part, first);
@@ -764,7 +797,9 @@ static int pcg_reliable(hippcg_solver *s, hipcsr_mtx *A, const double *db, doubl
     const SpmvShape sh = A->shape;
     const int gv = vec_grid(n);
     const bool nodeblock = sh.nodeblock && A->nodeblocked;
+    const bool grouped = sh.nodeblock && !A->nodeblocked && A->grouped;
     const int gs = nodeblock    ? std::max(1, std::min((A->nbr + 7) / 8, MAX_GRID))
+                   : grouped   ? std::max(1, std::min((A->ngrp + 7) / 8, MAX_GRID))
                    : sh.stream ? std::max(1, std::min((n + sh.rpb - 1) / sh.rpb, MAX_GRID))
                                : spmv_grid(n, sh.lanes);
     rc = ensure_partials(s->ctx, 3 * (size_t)std::max(gv, gs));
@@ -808,6 +843,10 @@ static int pcg_reliable(hippcg_solver *s, hipcsr_mtx *A, const double *db, doubl
             } else if (nodeblock) {
                 k_pcg_spmv_p_nb<float, float><<<gs, 256, 0, st>>>(A->nbr, A->d_bptr, A->d_bcol, A->d_val32, s->fp0,
                                                                   s->fq, s->st, part);
+            } else if (grouped) {
+                k_pcg_spmv_p_grp<float, float><<<gs, 256, 0, st>>>(A->ngrp, A->d_grow, A->d_gch, A->d_ccol,
+                                                                   A->d_coff, A->d_rowptr, A->d_val32, s->fp0,
+                                                                   s->fq, s->st, part);
             } else {
                 launch_pcg_spmv_p<float, float>(sh, gs, st, n, A->d_rowptr, A->d_colind, A->d_val32, s->fp0, s->fq,
                                                 s->st, part);

@@ -97,6 +97,42 @@ __device__ __forceinline__ void nb_row(int bo, int nbk, int vbase, int lane, con
     for (int r = 0; r < 3; r++) acc[r] = group_sum<32>(acc[r]);
 }
 
+// Row-group form, for patterns with partially constrained nodes: rows
+// [r0, r0 + nr) (nr <= 3) share one column list, which is cut into chunks of
+// up to 3 consecutive columns.  Chunk k of the group starts at column
+// ccol & GRP_COL_MASK, holds (ccol >> 30) + 1 columns and sits at offset coff
+// inside every row of the group.  Lane k of a 32-lane half wave takes chunks
+// k, k + 32, ...; for a fully free node row this is the node-block access
+// pattern with one more int32 per chunk.
+constexpr uint32_t GRP_COL_MASK = 0x3FFFFFFFu;
+
+template <typename AT, typename MT, typename G>
+__device__ __forceinline__ void grp_row(int r0, int nr, int c0, int nch, int lane, const int32_t *__restrict__ rowptr,
+                                        const uint32_t *__restrict__ ccol, const int32_t *__restrict__ coff,
+                                        const MT *__restrict__ val, G gather, AT acc[3])
+{
+    acc[0] = acc[1] = acc[2] = AT(0);
+    const int rp0 = rowptr[r0], rp1 = nr > 1 ? rowptr[r0 + 1] : 0, rp2 = nr > 2 ? rowptr[r0 + 2] : 0;
+    for (int k = lane; k < nch; k += 32) {
+        const uint32_t cc = ccol[c0 + k];
+        const int col = (int)(cc & GRP_COL_MASK), L = (int)(cc >> 30) + 1;
+        const int off = coff[c0 + k];
+        const AT x0 = gather(col), x1 = L > 1 ? gather(col + 1) : AT(0), x2 = L > 2 ? gather(col + 2) : AT(0);
+        const int rps[3] = {rp0, rp1, rp2};
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            if (r >= nr) break;
+            const MT *vp = val + rps[r] + off;
+            AT a = (AT)vp[0] * x0;
+            if (L > 1) a += (AT)vp[1] * x1;
+            if (L > 2) a += (AT)vp[2] * x2;
+            acc[r] += a;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 3; r++) acc[r] = group_sum<32>(acc[r]);
+}
+
 // Streaming form: a 256-thread block owns rows [r0, r0 + RPB).  Their values
 // and columns are one contiguous range of the CSR arrays, which the block
 // reads with unit stride across all 256 lanes (U loads in flight per lane),

@@ -169,6 +169,27 @@ def config1(nx=64, ny=64, nz=256, seed=20261203):
                           name=f"cantilever_{nx}x{ny}x{nz}_jitter_lognormalE")
 
 
+def config3(nx=64, ny=64, nz=256, seed=20261205, u_end=None):
+    """The plasticity problem of config2 on the config1 LSpace mesh: block
+    1 x 1 x 4 of nx x ny x nz hexes (jitter +-0.2 h), MisesMat E=210e3
+    nu=0.3 sig0=250 H=2e3, face z=0 clamped, u_z prescribed on z=4 (ramped
+    by the driver; total 5x the yield strain over the length)."""
+    base = cantilever_hex(nx, ny, nz, jitter=0.2, seed=seed)
+    lz = base.meta["lz"]
+    if u_end is None:
+        u_end = 5.0 * 250.0 / 210e3 * lz
+    nn = base.coords.shape[0]
+    bc = base.bc_mask.copy()
+    val = np.zeros((nn, 3))
+    top = np.arange(nn - (nx + 1) * (ny + 1), nn)
+    bc[top, 2] = True
+    val[top, 2] = u_end
+    mat = np.array([[MAT_MISES, 210e3, 0.3, 250.0, 2e3, 0, 0, 0]], dtype=np.float64)
+    return Deck(f"block_{nx}x{ny}x{nz}_hex_mises", LSPACE, base.coords, base.conn,
+                np.zeros(base.conn.shape[0], dtype=np.int32), mat, bc, val,
+                meta=dict(nx=nx, ny=ny, nz=nz, seed=seed, u_end=u_end))
+
+
 # ---------------------------------------------------------------- tetrahedra
 # Kuhn (Freudenthal) split of a cube into 6 tets sharing the main diagonal
 # (0,0,0)-(1,1,1); every tet is ordered so that the reference's detJ

@@ -1018,3 +1018,75 @@ out:
     free(rowptr); free(colind); free(val); free(rhs); free(fp);
     return rc;
 }
+
+/* ------------------------------------------------------------------ */
+/* Profile (skyline) U^T D U factorization and solve, the reference's
+ * direct solver: Skyline::factorized skyline.C:429-497 (Crout, column by
+ * column) and Skyline::backSubstitutionWith skyline.C:248-290, restated on
+ * 0-based column profiles: column j holds rows first[j]..j, stored from the
+ * first row down to the diagonal.  Input is the full symmetric CSR.  The
+ * inner sums run over ascending rows as in the reference.  Returns 0, or
+ * -1 on a zero pivot, -2 on allocation failure.                        */
+/* ------------------------------------------------------------------ */
+int orc_skyline_solve(int n, const long long *rowptr, const int *colind, const double *val, const double *b,
+                      double *x, long long *nwk_out)
+{
+    int *first = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+    long long *cs = (long long *)malloc(sizeof(long long) * ((size_t)n + 1));
+    if (!first || !cs) { free(first); free(cs); return -2; }
+    for (int j = 0; j < n; j++) {
+        int f = j;
+        for (long long k = rowptr[j]; k < rowptr[j + 1]; k++)
+            if (colind[k] < f) f = colind[k];
+        first[j] = f;
+    }
+    cs[0] = 0;
+    for (int j = 0; j < n; j++) cs[j + 1] = cs[j] + (j - first[j] + 1);
+    if (nwk_out) *nwk_out = cs[n];
+    double *a = (double *)calloc((size_t)cs[n] + 1, sizeof(double));
+    double *y = (double *)malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+    if (!a || !y) { free(first); free(cs); free(a); free(y); return -2; }
+#define SKY(i, j) a[cs[(j)] + ((i) - first[(j)])]   /* entry (i, j), first[j] <= i <= j */
+    /* upper triangle of row i = column entries (i, j), j >= i */
+    for (int i = 0; i < n; i++)
+        for (long long k = rowptr[i]; k < rowptr[i + 1]; k++) {
+            const int j = colind[k];
+            if (j >= i) SKY(i, j) = val[k];
+        }
+    int rc = 0;
+    for (int k = 1; k < n; k++) {
+        const int fk = first[k];
+        for (int i = fk + 1; i < k; i++) {
+            const int fi = first[i];
+            const int lo = fi > fk ? fi : fk;
+            double s = 0.0;
+            for (int m = lo; m < i; m++) s += SKY(m, k) * SKY(m, i);
+            SKY(i, k) -= s;
+        }
+        double s = 0.0;
+        for (int i = fk; i < k; i++) {
+            const double g = SKY(i, k);
+            SKY(i, k) /= SKY(i, i);
+            s += SKY(i, k) * g;
+        }
+        SKY(k, k) -= s;
+        if (SKY(k, k) == 0.0) { rc = -1; break; }
+    }
+    if (!rc && n > 0 && SKY(0, 0) == 0.0) rc = -1;
+    if (!rc) {
+        for (int i = 0; i < n; i++) y[i] = b[i];
+        for (int k = 1; k < n; k++) {
+            double s = 0.0;
+            for (int m = first[k]; m < k; m++) s += SKY(m, k) * y[m];
+            y[k] -= s;
+        }
+        for (int k = 0; k < n; k++) y[k] /= SKY(k, k);
+        for (int k = n - 1; k >= 0; k--) {
+            x[k] = y[k];
+            for (int m = first[k]; m < k; m++) y[m] -= SKY(m, k) * x[k];
+        }
+    }
+#undef SKY
+    free(first); free(cs); free(a); free(y);
+    return rc;
+}

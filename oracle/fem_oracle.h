@@ -108,6 +108,11 @@ int orc_nonlinear_step(const orc_mesh *m, double *state, double *X, const double
                        const double *u_presc_prev, int initial_guess, const double *RT,
                        double rtolf, int maxiter, int manrmsteps, double lstol, int *out, double *err);
 
+/* Skyline U^T D U factorization + solve of a full symmetric CSR (direct
+ * solver of the reference; CPU baseline only).  *nwk_out: profile size. */
+int orc_skyline_solve(int n, const long long *rowptr, const int *colind, const double *val, const double *b,
+                      double *x, long long *nwk_out);
+
 /* LinearStatic::solveYourselfAt + StructuralEngngModel::computeReaction */
 int orc_linear_static(const orc_mesh *m, const double *f_ext, const double *f_ext_presc,
                       const double *u_presc, double tol, int maxit,

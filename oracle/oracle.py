@@ -152,6 +152,20 @@ def pcg(rowptr, colind, val, b, x0=None, tol=1e-10, maxit=100000, precond=1):
     return x, it.value, res.value, rc
 
 
+def skyline_solve(rowptr, colind, val, b):
+    """Profile U^T D U direct solve (Skyline, skyline.C:248-290,429-497).
+    Returns (x, nwk)."""
+    n = rowptr.shape[0] - 1
+    x = np.zeros(n)
+    nwk = C.c_longlong(0)
+    rc = lib().orc_skyline_solve(n, rowptr.ctypes.data_as(C.c_void_p), colind.ctypes.data_as(C.c_void_p),
+                                 val.ctypes.data_as(C.c_void_p), _d(b).ctypes.data_as(C.c_void_p),
+                                 x.ctypes.data_as(C.c_void_p), C.byref(nwk))
+    if rc:
+        raise RuntimeError(f"orc_skyline_solve rc={rc}")
+    return x, nwk.value
+
+
 def state_new(deck):
     return np.zeros(GPSTATE * deck.nelem * {1: 8, 2: 1, 3: 27}[deck.etype])
 

@@ -31,6 +31,7 @@ def _small_decks():
 
 
 DECKS = _small_decks()
+SHAPE_DECKS = dict(DECKS, tetbar=meshgen.config2(nx=4, ny=3, nz=6, seed=5, jitter=0.12))
 
 
 @pytest.mark.parametrize("name", list(DECKS))
@@ -71,11 +72,11 @@ SHAPES = [("lanes", 8, 6, 0), ("lanes", 16, 4, 1), ("lanes", 32, 2, 0), ("lanes"
 
 
 @pytest.mark.parametrize("shape", SHAPES, ids=[f"{s[0]}-{s[1]}-{s[2]}" for s in SHAPES])
-@pytest.mark.parametrize("name", ["lspace", "qspace"])
+@pytest.mark.parametrize("name", ["lspace", "qspace", "tetbar"])
 def test_spmv_shapes_match_oracle(ctx, name, shape):
     """Every SpMV kernel shape (plain and inside PCG) against the oracle."""
     from oofem_amd.hipfem import HipCSRSparseMtrx, HipFemError, HipPCGSolver
-    d = DECKS[name]
+    d = SHAPE_DECKS[name]
     rp, ci, val = oracle.csr(d)
     K = HipCSRSparseMtrx(ctx)
     K.buildInternalStructureCSR(rp, ci)
@@ -84,14 +85,9 @@ def test_spmv_shapes_match_oracle(ctx, name, shape):
     if kind == "lanes":
         K.set_spmv_shape(a, b, c)
     elif kind == "nodeblock":
-        # the QSpace beam prescribes single dofs (w only) on its loaded end:
-        # not node-blocked, so the form must be refused
-        if name == "qspace":
-            assert not K.is_nodeblocked()
-            with pytest.raises(HipFemError):
-                K.set_spmv_nodeblock(True)
-            return
-        assert K.is_nodeblocked()
+        # fully free nodes: 3x3 block form; the QSpace beam and the tet bar
+        # prescribe single dofs (w only) on their loaded end: row-group form
+        assert K.is_nodeblocked() == (1 if name == "lspace" else 2)
         K.set_spmv_nodeblock(True)
     else:
         max_block = max(rp[min(r + a, d.neq)] - rp[r] for r in range(0, d.neq, a))

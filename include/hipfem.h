@@ -246,6 +246,29 @@ int hfem_model_get_state(hfem_model *md, double *state);
 int hfem_model_ngp_total(const hfem_model *md);
 int hfem_model_ncolors(const hfem_model *md);
 
+/* ---------------------------------------------------- batched block solves */
+/* Many independent SPD systems in one persistent launch (the RVE batch; no
+ * single counterpart in the reference, whose RVE solves go one by one
+ * through PrescribedGradient::computeField, prescribedgradient.C:161-181).
+ * A is block diagonal: block b is rows [rows[b], rows[b+1]) and no entry
+ * leaves its block (checked).  Six right-hand sides per block ([row][6]
+ * interleaved, FP64 device arrays).  Jacobi PCG on FP32 values/vectors with
+ * FP64 solution and dots and FP64 reliable updates (hippcg precision 2),
+ * one workgroup per block, blocks taken largest first. */
+typedef struct hfem_batch hfem_batch;
+int  hfem_batch_create(hipcsr_mtx *A, int nblocks, const int32_t *rows, hfem_batch **out);
+void hfem_batch_destroy(hfem_batch *bt);
+int  hfem_batch_set_options(hfem_batch *bt, double tol, double delta, int maxit);
+/* kernel shapes (workgroup size, lanes per row) and the one to use */
+int  hfem_batch_nshapes(void);
+int  hfem_batch_shape_info(int shape, int *block_size, int *lanes_per_row);
+int  hfem_batch_set_shape(hfem_batch *bt, int shape, int grid);
+/* per block: iterations, reliable updates, final max relative true
+ * residual of its six columns (negative: not converged within maxit) */
+int  hfem_batch_solve_dev(hfem_batch *bt, const double *db, double *dx, int *iters, int *updates, double *resid);
+/* time every shape on the `sample` largest blocks, keep the fastest */
+int  hfem_batch_autotune(hfem_batch *bt, const double *db, double *dx, int sample, double *ms, int *best);
+
 /* ------------------------------------------------------ MisesMat surrogate */
 /* A learned stand-in for MisesMat's return mapping and consistent tangent
  * (misesmat.C:181-263, 496-544); no counterpart in the reference.  Network

@@ -15,7 +15,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIBDIR = os.path.join(HERE, "lib")
 LIB = os.path.join(LIBDIR, "libhipfem.so")
 HOST_LIB = os.path.join(LIBDIR, "libhipfem_host.so")
-SOURCES = ["hipfem_csr.hip", "hipfem_pcg.hip", "hipfem_model.hip", "hipfem_mlp.hip"]
+SOURCES = ["hipfem_csr.hip", "hipfem_pcg.hip", "hipfem_model.hip", "hipfem_mlp.hip", "hipfem_batch.hip"]
 HOST_SOURCES = ["classfactory.C", "hipcsr.C", "hippcg.C", "structmodel.C", "host_capi.C"]
 
 HIPCC_FLAGS = [

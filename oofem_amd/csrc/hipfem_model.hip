@@ -347,12 +347,14 @@ struct SurArgs {
 // misesmat.C:181-263 hold at it, each within tol * sigma_y(kappa_n):
 // the volumetric part is elastic (p = K tr(eps - epsp_n)), the deviator lies
 // along the trial deviator, and the yield condition holds with the implied
-// Delta kappa = (sigma_eq,trial - sigma_eq) / 3G > 0.  The temp state is
-// then the one consistent with that stress: kappa_n + Delta kappa and
-// epsp = eps - C^-1 sigma.
+// Delta kappa = (sigma_eq,trial - sigma_eq) / 3G > 0; and its tangent when
+// every entry is within tol * E of the consistent tangent at that state
+// (misesmat.C:496-544; an indefinite prediction would break the linear
+// solver).  The temp state is then the one consistent with that stress:
+// kappa_n + Delta kappa and epsp = eps - C^-1 sigma.
 __device__ bool surrogate_update(double E, double nu, double sig0, double H, const double eps[6],
-                                 const double epsp_n[6], double kappa_n, const float *y, float sig_scale, double tol,
-                                 MisesOut &o)
+                                 const double epsp_n[6], double kappa_n, const float *y, float sig_scale,
+                                 float tan_scale, double tol, MisesOut &o)
 {
     const double G = E / (2.0 * (1. + nu));
     const double K = E / (3.0 * (1. - 2. * nu));
@@ -386,6 +388,14 @@ __device__ bool surrogate_update(double E, double nu, double sig0, double H, con
     r2 = sqrt(r2);
     const double r3 = fabs(pp - K * 3.0 * mean);
     if (!(fmax(r1, fmax(r2, r3)) <= tol * sy)) return false;
+    // the predicted tangent must match MisesMat's consistent tangent at the
+    // accepted state (kappa_n + Delta kappa, trial deviator) within tol * E
+    double D[36];
+    mises_tangent(E, nu, sig0, H, kappa_n, kappa_n + dk, st, D);
+    int k = 0;
+    for (int r = 0; r < 6; r++)
+        for (int c = r; c < 6; c++, k++)
+            if (!(fabs((double)y[6 + k] / (double)tan_scale - D[6 * r + c]) <= tol * E)) return false;
 #pragma unroll
     for (int i = 0; i < 6; i++) {
         o.sig[i] = sp[i];
@@ -572,7 +582,7 @@ __global__ __launch_bounds__(64) void k_internal(const int *__restrict__ elems, 
                 const int sl = sur.slot[gp];
                 if (sl >= 0) {
                     used = surrogate_update(E, nu, mp[3], mp[4], eps, epsp_n, kappa_n, sur.y + (size_t)27 * sl,
-                                            sur.sig_scale, sur.tol, o);
+                                            sur.sig_scale, sur.tan_scale, sur.tol, o);
                     atomicAdd(&scount[used ? 0 : 1], 1u);
                 }
                 if (update_state) {

@@ -826,6 +826,7 @@ static int pcg_reliable(hippcg_solver *s, hipcsr_mtx *A, const double *db, doubl
     const bool prof = s->profiling != 0;
     s->last_updates = 0;
     s->last_restarts = 0;
+    int last_restart_iter = -1;
     while (!h.done && it <= maxit) {
         const int stop = std::min(maxit, it + chunk - 1);
         if (prof) {
@@ -866,6 +867,11 @@ static int pcg_reliable(hippcg_solver *s, hipcsr_mtx *A, const double *db, doubl
             // last completed one (a pause inside k_reduce_alpha_rel leaves
             // that iteration without an update: it is done again)
             const bool restart = h.restart != 0;
+            // a restart that makes no progress (the FP32 operator is not
+            // positive on the preconditioned residual) ends the solve
+            if (restart && h.iter == last_restart_iter) break;
+            if (restart) last_restart_iter = h.iter;
+            if (s->last_updates > maxit) break;
             rc = true_residual(0);
             if (rc) return rc;
             rc = read_state(s, &h);

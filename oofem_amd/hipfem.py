@@ -312,7 +312,8 @@ class HipCSRSparseMtrx:
         _check(lib().hipcsr_set_spmv_nodeblock(self.h, int(bool(on))), "set_spmv_nodeblock")
 
     def is_nodeblocked(self):
-        return bool(lib().hipcsr_is_nodeblocked(self.h))
+        """1: 3x3 node-block form, 2: row-group form, 0: neither"""
+        return int(lib().hipcsr_is_nodeblocked(self.h))
 
     def spmv_shape(self):
         a, b, c = C.c_int(0), C.c_int(0), C.c_int(0)

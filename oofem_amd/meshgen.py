@@ -285,3 +285,120 @@ def surface_load_vector(deck: Deck, factor=1.0):
     m = q < 0
     np.add.at(f_presc, -q[m] - 1, fe[m])
     return f_free, f_presc
+
+
+# ------------------------------------------------------------------- RVEs
+# QSpace local node positions in (xi, eta, zeta) (fei3dhexaquad.C node
+# order: 8 corners as LSpace, 4 mid-edge nodes of the zeta = +1 face, 4 of
+# the zeta = -1 face, 4 mid-height edges); xi, eta, zeta run along x, y, z.
+_QS_POS = np.array([(-1, -1, 1), (-1, 1, 1), (1, 1, 1), (1, -1, 1), (-1, -1, -1), (-1, 1, -1), (1, 1, -1), (1, -1, -1),
+                    (-1, 0, 1), (0, 1, 1), (1, 0, 1), (0, -1, 1), (-1, 0, -1), (0, 1, -1), (1, 0, -1), (0, -1, -1),
+                    (-1, -1, 0), (-1, 1, 0), (1, 1, 0), (1, -1, 0)], dtype=np.int64)
+
+
+def _rve_mesh(n, etype):
+    """Unit cube of n^3 LSpace or QSpace cells: coordinates and connectivity."""
+    if etype == LSPACE:
+        return _grid_nodes(n, n, n, 1.0, 1.0, 1.0), _hex_conn(n, n, n)
+    m = 2 * n + 1   # doubled lattice: corner and mid-edge points only
+    I, J, K = np.meshgrid(np.arange(m), np.arange(m), np.arange(m), indexing="ij")
+    I, J, K = I.ravel(), J.ravel(), K.ravel()
+    keep = ((I % 2) + (J % 2) + (K % 2)) <= 1
+    lat = np.full(m ** 3, -1, dtype=np.int64)
+    order = np.nonzero(keep)[0]
+    lat[order] = np.arange(order.size)
+    # node order: z-major like the LSpace grid
+    key = K[order] * m * m + J[order] * m + I[order]
+    perm = np.argsort(key, kind="stable")
+    renum = np.empty(order.size, dtype=np.int64)
+    renum[perm] = np.arange(order.size)
+    coords = np.round(np.stack([I[order][perm], J[order][perm], K[order][perm]], axis=1) / (2.0 * n), 12)
+    ez, ey, ex = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    ex, ey, ez = ex.ravel(), ey.ravel(), ez.ravel()
+    conn = np.empty((ex.size, 20), dtype=np.int64)
+    for a, (u, v, w) in enumerate(_QS_POS):
+        li = 2 * ex + 1 + u
+        lj = 2 * ey + 1 + v
+        lk = 2 * ez + 1 + w
+        conn[:, a] = renum[lat[li * m * m + lj * m + lk]]
+    return coords, conn.astype(np.int32)
+
+
+def rve(n, etype, vf, seed, E_matrix=10e3, E_incl=100e3, nu=0.25):
+    """Two-phase RVE: unit cube of n^3 cells (LSpace or QSpace), seeded
+    spherical inclusions (radius 0.08..0.2) placed until the element volume
+    fraction reaches vf (an element is inclusion when its centre is inside a
+    sphere); every boundary node prescribed (affine displacement of a macro
+    strain, set per load case with affine_values)."""
+    coords, conn = _rve_mesh(n, etype)
+    rng = np.random.default_rng(seed)
+    centres = coords[conn[:, :8]].mean(axis=1)
+    incl = np.zeros(conn.shape[0], dtype=bool)
+    for _ in range(10000):
+        if incl.mean() >= vf:
+            break
+        c = rng.uniform(0.0, 1.0, 3)
+        r = rng.uniform(0.08, 0.2)
+        incl |= ((centres - c) ** 2).sum(axis=1) <= r * r
+    mat = np.array([[MAT_ISOLE, E_matrix, nu, 0, 0, 0, 0, 0], [MAT_ISOLE, E_incl, nu, 0, 0, 0, 0, 0]])
+    on_bnd = np.any((coords <= 1e-12) | (coords >= 1.0 - 1e-12), axis=1)
+    bc = np.repeat(on_bnd[:, None], 3, axis=1)
+    d = Deck(f"rve_{'q' if etype == QSPACE else 'l'}{n}_vf{vf:.2f}_s{seed}", etype, coords, conn,
+             incl.astype(np.int32), mat, bc, np.zeros_like(coords),
+             meta=dict(n=n, vf_target=vf, vf=float(incl.mean()), seed=seed))
+    return d
+
+
+VOIGT_PAIRS = [(0, 0), (1, 1), (2, 2), (1, 2), (0, 2), (0, 1)]
+
+
+def macro_gradient(eps_voigt):
+    """Symmetric displacement gradient of a Voigt macro strain (engineering
+    shear split over the two off-diagonal entries)."""
+    g = np.zeros((3, 3))
+    for k, (i, j) in enumerate(VOIGT_PAIRS):
+        if i == j:
+            g[i, i] = eps_voigt[k]
+        else:
+            g[i, j] = g[j, i] = 0.5 * eps_voigt[k]
+    return g
+
+
+def affine_values(deck, eps_voigt, centre=(0.5, 0.5, 0.5)):
+    """Prescribed values u = grad . (x - centre) of PrescribedGradient::give
+    (prescribedgradient.C:59-93) at the prescribed equations."""
+    g = macro_gradient(eps_voigt)
+    u = (deck.coords - np.asarray(centre)[None, :]) @ g.T
+    m = deck.eqn < 0
+    out = np.zeros(deck.npeq)
+    out[-deck.eqn[m] - 1] = u[m]
+    return out
+
+
+def homogenized_stress(deck, reactions, centre=(0.5, 0.5, 0.5)):
+    """Volume-averaged stress of an RVE under affine boundary displacements
+    from its boundary reactions: sigma = (1/V) sum_b R_b (x) (x_b - centre),
+    symmetrised (V = 1), in Voigt order."""
+    m = deck.eqn < 0
+    node, comp = np.nonzero(m)
+    R = np.zeros_like(deck.coords)
+    R[node, comp] = reactions[-deck.eqn[m] - 1]
+    X = deck.coords - np.asarray(centre)[None, :]
+    S = R.T @ X
+    S = 0.5 * (S + S.T)
+    return np.array([S[i, j] for (i, j) in VOIGT_PAIRS])
+
+
+def config4(count=512, seed=20261206, nmin=8, nmax=32, q_share=0.3):
+    """The RVE batch: sizes n ~ U{nmin..nmax}, QSpace in a seeded q_share of
+    the RVEs, volume fractions U(0.1, 0.4), and one seeded random macro strain
+    of magnitude 1e-3 per RVE.  Returns (specs, macro strains); decks are
+    built on demand with rve(*spec)."""
+    rng = np.random.default_rng(seed)
+    ns = rng.integers(nmin, nmax + 1, count)
+    q = rng.random(count) < q_share
+    vfs = rng.uniform(0.1, 0.4, count)
+    specs = [(int(ns[i]), QSPACE if q[i] else LSPACE, float(vfs[i]), int(seed + 1 + i)) for i in range(count)]
+    e = rng.standard_normal((count, 6))
+    e *= 1e-3 / np.linalg.norm(e, axis=1, keepdims=True)
+    return specs, e

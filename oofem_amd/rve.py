@@ -1,0 +1,192 @@
+"""Batched RVE homogenisation on the GPU (the configuration-4 workload).
+
+Each RVE is a unit cube of LSpace or QSpace cells with two isotropic phases
+under affine boundary displacements u = grad(eps) . (x - x_c)
+(PrescribedGradient::give, prescribedgradient.C:59-93).  Its homogenised
+stiffness comes from six unit-strain solves (PrescribedGradient::computeField
+style, prescribedgradient.C:161-181, without the PETSc-only computeTangent)
+and the boundary-reaction average sigma = (1/V) sum_b R_b (x) (x_b - x_c).
+
+GPU layout: the RVEs of one element type are concatenated into one mesh
+(disjoint node ranges), so the batched evaluator assembles one
+block-diagonal stiffness; chunks keep every matrix under the int32 index
+range.  All blocks and all six right-hand sides are solved in one persistent
+launch per chunk (hfem_batch: mixed-precision PCG with FP64 reliable
+updates, largest blocks first, kernel shape from the autotuner).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import time
+
+import numpy as np
+
+from . import meshgen
+from .hipfem import BatchedEvaluator, DeviceVector, HipCSRSparseMtrx, _check, _p, lib
+
+NNZ_ROW = {meshgen.LSPACE: 81, meshgen.QSPACE: 200}
+
+
+def _concat(decks):
+    """One deck holding the RVEs side by side (node and element offsets; the
+    shared two-phase material table)."""
+    coords = np.concatenate([d.coords for d in decks])
+    off = np.cumsum([0] + [d.nnode for d in decks])
+    conn = np.concatenate([d.conn + o for d, o in zip(decks, off[:-1])])
+    emat = np.concatenate([d.emat for d in decks])
+    bc = np.concatenate([d.bc_mask for d in decks])
+    big = meshgen.Deck("rve_chunk", decks[0].etype, coords, conn, emat, decks[0].mat, bc, np.zeros_like(coords))
+    return big, off
+
+
+def block_rows(deck, node_off):
+    """First free equation (0-based) of every RVE of a concatenated deck, and
+    the total: equations are numbered node by node, so each RVE's free
+    equations form one contiguous range."""
+    rows = np.zeros(len(node_off), dtype=np.int32)
+    for i in range(len(node_off) - 1):
+        e = deck.eqn[node_off[i]:node_off[i + 1]]
+        free = e[e > 0]
+        if free.size == 0 or free.max() - free.min() + 1 != free.size or free.min() - 1 != rows[i]:
+            raise ValueError(f"RVE {i}: free equations are not one contiguous range after the previous RVE")
+        rows[i + 1] = free.max()
+    return rows
+
+
+class _Chunk:
+    def __init__(self, ctx, decks, ids):
+        self.ctx = ctx
+        self.ids = ids
+        self.deck, self.node_off = _concat(decks)
+        d = self.deck
+        self.ev = BatchedEvaluator(ctx, d)
+        self.K = HipCSRSparseMtrx(ctx)
+        self.ev.build_structure(self.K)
+        self.ev.assemble_stiffness(self.K)
+        rows = block_rows(d, self.node_off)
+        self.rows = rows
+        self.h = C.c_void_p()
+        _check(lib().hfem_batch_create(self.K.h, len(decks), _p(rows), C.byref(self.h)), "hfem_batch_create")
+        self.n = d.neq
+        self.b = DeviceVector(ctx, max(1, 6 * self.n))
+        self.x = DeviceVector(ctx, max(1, 6 * self.n))
+        self.up = [meshgen.affine_values(d, e) for e in np.eye(6)]
+
+    def rhs(self):
+        """b_k = -f_int(free 0, prescribed affine_k) for the six unit strains, interleaved."""
+        B = np.zeros((self.n, 6))
+        for k in range(6):
+            f, _, _ = self.ev.internal_forces(np.zeros(self.n), self.up[k])
+            B[:, k] = -f
+        self.b.upload(B.ravel())
+
+    def solve(self):
+        nb = len(self.ids)
+        it = np.zeros(nb, dtype=np.int32)
+        upd = np.zeros(nb, dtype=np.int32)
+        res = np.zeros(nb)
+        _check(lib().hfem_batch_solve_dev(self.h, C.c_void_p(self.b.addr), C.c_void_p(self.x.addr), _p(it), _p(upd),
+                                          _p(res)), "hfem_batch_solve")
+        return it, upd, res
+
+    def homogenized(self):
+        """6x6 stiffness per RVE from the reactions of the six solves."""
+        X = self.x.download().reshape(self.n, 6)
+        d = self.deck
+        m = d.eqn < 0
+        node, comp = np.nonzero(m)
+        Xc = d.coords - 0.5
+        out = np.zeros((len(self.ids), 6, 6))
+        for k in range(6):
+            _, fp, _ = self.ev.internal_forces(X[:, k], self.up[k])
+            R = np.zeros_like(d.coords)
+            R[node, comp] = fp[-d.eqn[m] - 1]
+            # per-RVE sum over its node range of R (x) (x - x_c)
+            P = R[:, :, None] * Xc[:, None, :]
+            S = np.add.reduceat(P, self.node_off[:-1], axis=0)
+            S = 0.5 * (S + np.transpose(S, (0, 2, 1)))
+            out[:, :, k] = np.stack([S[:, i, j] for (i, j) in meshgen.VOIGT_PAIRS], axis=1)
+        return out
+
+    def close(self):
+        if self.h:
+            lib().hfem_batch_destroy.argtypes = [C.c_void_p]
+            lib().hfem_batch_destroy.restype = None
+            lib().hfem_batch_destroy(self.h)
+            self.h = C.c_void_p()
+        for o in (self.b, self.x, self.K, self.ev):
+            o.close()
+
+
+class RVEBatch:
+    """All RVEs of a batch on one GPU, grouped by element type into chunks of
+    at most nnz_budget stiffness entries."""
+
+    def __init__(self, ctx, specs, nnz_budget=1.2e9, log=None):
+        self.ctx = ctx
+        self.specs = specs
+        self.chunks = []
+        t = time.perf_counter()
+        for et in (meshgen.QSPACE, meshgen.LSPACE):
+            ids = [i for i, s in enumerate(specs) if s[1] == et]
+            cur, cur_nnz = [], 0.0
+            for i in ids + [None]:
+                if i is not None:
+                    n = specs[i][0]
+                    est = 3.0 * (n + 1) ** 3 * (4 if et == meshgen.QSPACE else 1) * NNZ_ROW[et]
+                if cur and (i is None or cur_nnz + est > nnz_budget):
+                    decks = [meshgen.rve(*specs[j]) for j in cur]
+                    self.chunks.append(_Chunk(ctx, decks, cur))
+                    if log:
+                        log(f"[rve] chunk of {len(cur)} {'QSpace' if et == meshgen.QSPACE else 'LSpace'} RVEs: "
+                            f"neq {self.chunks[-1].n}, nnz {self.chunks[-1].K.nnz()}")
+                    cur, cur_nnz = [], 0.0
+                if i is not None:
+                    cur.append(i)
+                    cur_nnz += est
+        self.t_setup = time.perf_counter() - t
+
+    def set_options(self, tol=1e-8, delta=0.1, maxit=20000):
+        for c in self.chunks:
+            _check(lib().hfem_batch_set_options(c.h, C.c_double(tol), C.c_double(delta), C.c_int(maxit)), "options")
+
+    def autotune(self, sample=8):
+        nshape = lib().hfem_batch_nshapes()
+        out = []
+        for c in self.chunks:
+            c.rhs()
+            ms = np.zeros(nshape)
+            best = C.c_int(0)
+            _check(lib().hfem_batch_autotune(c.h, C.c_void_p(c.b.addr), C.c_void_p(c.x.addr), C.c_int(sample),
+                                             _p(ms), C.byref(best)), "autotune")
+            bs, lpr = C.c_int(0), C.c_int(0)
+            lib().hfem_batch_shape_info(best.value, C.byref(bs), C.byref(lpr))
+            out.append(dict(etype=int(c.deck.etype), best=best.value, block=bs.value, lanes_per_row=lpr.value,
+                            ms=[round(float(v), 3) for v in ms]))
+        return out
+
+    def run(self):
+        """Six unit-strain solves per RVE; returns per-RVE 6x6 stiffness,
+        iterations, reliable updates, residuals and the solve time (the
+        persistent launches only; right-hand sides and reactions apart)."""
+        nrve = len(self.specs)
+        C6 = np.zeros((nrve, 6, 6))
+        it = np.zeros(nrve, dtype=np.int32)
+        upd = np.zeros(nrve, dtype=np.int32)
+        res = np.zeros(nrve)
+        t_solve = 0.0
+        for c in self.chunks:
+            c.rhs()
+            self.ctx.synchronize()
+            t = time.perf_counter()
+            i_, u_, r_ = c.solve()
+            t_solve += time.perf_counter() - t
+            ids = np.asarray(c.ids)
+            it[ids], upd[ids], res[ids] = i_, u_, r_
+            C6[ids] = c.homogenized()
+        return dict(C=C6, iters=it, updates=upd, resid=res, t_solve=t_solve)
+
+    def close(self):
+        for c in self.chunks:
+            c.close()
+        self.chunks = []

@@ -237,10 +237,10 @@ def extrapolated_forces(deck, state, du_presc):
     return f
 
 
-def linear_static(deck, tol=1e-10, maxit=100000):
+def linear_static(deck, tol=1e-10, maxit=100000, u_presc=None):
     m = Mesh(deck)
     f, fp = surface_load(deck)
-    up = deck.prescribed_values()
+    up = deck.prescribed_values() if u_presc is None else _d(u_presc)
     u = np.zeros(deck.neq)
     R = np.zeros(deck.npeq)
     it = C.c_int(0)

@@ -1,0 +1,48 @@
+"""GPU tests of the batched RVE solves (hfem_batch): homogenised stiffness
+of a mixed LSpace / QSpace batch against the oracle's six FP64 solves per
+RVE, and the autotuner's choice."""
+import numpy as np
+import pytest
+
+from oofem_amd import meshgen
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+SPECS = [(3, meshgen.LSPACE, 0.3, 1), (4, meshgen.QSPACE, 0.2, 2), (6, meshgen.LSPACE, 0.15, 3),
+         (3, meshgen.QSPACE, 0.35, 4), (5, meshgen.LSPACE, 0.25, 5), (2, meshgen.QSPACE, 0.1, 6)]
+
+
+def _oracle_C(spec):
+    d = meshgen.rve(*spec)
+    C = np.zeros((6, 6))
+    for k in range(6):
+        e = np.zeros(6)
+        e[k] = 1.0
+        r = oracle.linear_static(d, tol=1e-13, u_presc=meshgen.affine_values(d, e))
+        C[:, k] = meshgen.homogenized_stress(d, r["reactions"])
+    return C
+
+
+def test_batch_homogenized_stiffness_matches_oracle(ctx):
+    from oofem_amd.rve import RVEBatch
+    B = RVEBatch(ctx, SPECS, nnz_budget=2e5)        # small budget: several chunks per element type
+    assert len(B.chunks) >= 3
+    B.set_options(tol=1e-10)
+    out = B.run()
+    assert (out["resid"] > 0).all() and (out["resid"] <= 1e-10).all(), out["resid"]
+    for i, spec in enumerate(SPECS):
+        ref = _oracle_C(spec)
+        err = np.abs(out["C"][i] - ref).max() / np.abs(ref).max()
+        assert err <= 1e-7, (spec, err)
+    B.close()
+
+
+def test_batch_autotune_picks_a_valid_shape(ctx):
+    from oofem_amd.rve import RVEBatch
+    B = RVEBatch(ctx, SPECS[:4])
+    tuned = B.autotune(sample=2)
+    assert all(0 <= t["best"] < len(t["ms"]) and t["ms"][t["best"]] == min(t["ms"]) for t in tuned)
+    out = B.run()
+    assert (out["resid"] > 0).all()
+    B.close()

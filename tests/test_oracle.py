@@ -368,3 +368,60 @@ def test_skyline_direct_solve():
     x, _ = oracle.skyline_solve(rp, ci, val, f)
     u, it, res, rc = oracle.pcg(rp, ci, val, f, tol=1e-12)
     assert np.linalg.norm(x - u) <= 1e-10 * np.linalg.norm(x)
+
+
+@pytest.mark.parametrize("etype", [meshgen.LSPACE, meshgen.QSPACE])
+def test_rve_homogenization_reproduces_isotropic_stiffness(etype):
+    """A one-phase RVE under the six affine unit-strain boundary conditions
+    (PrescribedGradient) reproduces the isotropic stiffness exactly: pins the
+    RVE meshes (incl. the 20-node node order), the affine values and the
+    boundary-reaction homogenisation."""
+    d = meshgen.rve(3, etype, 0.0, 3)
+    C = np.zeros((6, 6))
+    for k in range(6):
+        e = np.zeros(6)
+        e[k] = 1e-3
+        r = oracle.linear_static(d, tol=1e-13, u_presc=meshgen.affine_values(d, e))
+        C[:, k] = meshgen.homogenized_stress(d, r["reactions"]) / 1e-3
+    D = oracle.isole_D(10e3, 0.25)
+    assert np.abs(C - D).max() <= 1e-12 * np.abs(D).max()
+
+
+def test_rve_two_phase_stiffness_between_bounds():
+    """Two-phase RVE: the homogenised stiffness is symmetric and lies between
+    the Reuss and Voigt bounds of the phases in the element volume fraction."""
+    d = meshgen.rve(5, meshgen.LSPACE, 0.3, 8)
+    C = np.zeros((6, 6))
+    for k in range(6):
+        e = np.zeros(6)
+        e[k] = 1e-3
+        r = oracle.linear_static(d, tol=1e-13, u_presc=meshgen.affine_values(d, e))
+        C[:, k] = meshgen.homogenized_stress(d, r["reactions"]) / 1e-3
+    assert np.abs(C - C.T).max() <= 1e-9 * np.abs(C).max()
+    f = d.meta["vf"]
+    D1, D2 = oracle.isole_D(10e3, 0.25), oracle.isole_D(100e3, 0.25)
+    voigt = (1 - f) * D1 + f * D2
+    reuss = np.linalg.inv((1 - f) * np.linalg.inv(D1) + f * np.linalg.inv(D2))
+    w = np.linalg.eigvalsh
+    assert w(voigt - C).min() >= -1e-9 * np.abs(C).max()
+    assert w(C - reuss).min() >= -1e-9 * np.abs(C).max()
+
+
+def test_rve_batch_block_rows_and_config4_specs():
+    """Host side of the RVE batch: concatenated RVEs keep contiguous equation
+    ranges (the block-diagonal layout the batched solver needs), and the
+    configuration-4 draw has the specified ranges."""
+    from oofem_amd.rve import _concat, block_rows
+    decks = [meshgen.rve(3, meshgen.LSPACE, 0.2, 1), meshgen.rve(4, meshgen.LSPACE, 0.3, 2),
+             meshgen.rve(2, meshgen.LSPACE, 0.1, 3)]
+    big, off = _concat(decks)
+    rows = block_rows(big, off)
+    assert list(np.diff(rows)) == [d.neq for d in decks]
+    assert rows[-1] == big.neq
+    specs, macro = meshgen.config4()
+    assert len(specs) == 512
+    ns = np.array([s[0] for s in specs])
+    assert ns.min() >= 8 and ns.max() <= 32
+    q = np.mean([s[1] == meshgen.QSPACE for s in specs])
+    assert 0.25 <= q <= 0.35
+    assert np.allclose(np.linalg.norm(macro, axis=1), 1e-3)

@@ -274,9 +274,10 @@ int  hfem_batch_autotune(hfem_batch *bt, const double *db, double *dx, int sampl
  * (misesmat.C:181-263, 496-544); no counterpart in the reference.  Network
  * 13 -> 3 x 128 (SiLU) -> 27, FP32 master weights (flat, hfem_mlp_nparams()
  * values: W1 128x13, b1, W2 128x128, b2, W3 128x128, b3, W4 27x128, b4).
- * Inputs (eps - epsp_n, epsp_n, kappa_n) scaled by E/sig0; outputs stress
- * scaled by 1/sig0 and the upper triangle of the tangent (21, row-major)
- * scaled by 1/E (hfem_mlp_scales: input, stress and tangent factors).
+ * Inputs (eps - epsp_n, epsp_n, kappa_n) scaled by E/sig0; outputs the
+ * stress and the upper triangle of the tangent (21, row-major) as their
+ * departures from the elastic trial, (D_e (eps - epsp_n) - sigma) / sig0 and
+ * (D_e - D) / E (hfem_mlp_scales: input factor E/sig0, 1/sig0, 1/E).
  * mat4 = {E, nu, sig0, H}. */
 typedef struct hfem_mlp hfem_mlp;
 int  hfem_mlp_nparams(void);

@@ -90,4 +90,31 @@ __device__ __forceinline__ void mises_tangent(double E, double nu, double sig0, 
     for (int i = 0; i < 36; i++) D[i] += factor2 * idev[i];
 }
 
+// Elastic trial stress D_e (eps - epsp_n) (Voigt, engineering shear) and
+// the entries of the upper triangle of D_e (row-major, k = 0..20).
+__device__ __forceinline__ void elastic_trial(double E, double nu, const double el[6], double sig[6])
+{
+    const double G = E / (2.0 * (1. + nu));
+    const double K = E / (3.0 * (1. - 2. * nu));
+    const double mean = (el[0] + el[1] + el[2]) / 3.0;
+    for (int i = 0; i < 3; i++) sig[i] = 2. * G * (el[i] - mean) + 3. * K * mean;
+    for (int i = 3; i < 6; i++) sig[i] = G * el[i];
+}
+
+__device__ __forceinline__ double elastic_upper(double E, double nu, int k)
+{
+    const double G = E / (2.0 * (1. + nu));
+    const double K = E / (3.0 * (1. - 2. * nu));
+    int r = 0, c = 0, t = k;
+    for (r = 0; r < 6; r++) {
+        if (t < 6 - r) {
+            c = r + t;
+            break;
+        }
+        t -= 6 - r;
+    }
+    if (r < 3 && c < 3) return (r == c ? 2. * G * 2. / 3. : -2. * G / 3.) + K;
+    return r == c ? G : 0.0;
+}
+
 }  // namespace hipfem

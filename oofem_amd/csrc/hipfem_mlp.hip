@@ -7,8 +7,11 @@
 //           map of misesmat.C:181-263 depends on the strain only through
 //           eps_{n+1} - epsp_n, so this difference is the strain input;
 //           all scaled by E / sig0 (the yield strain is 1)
-//   outputs sigma_{n+1} (6, scaled by 1 / sig0) and the upper triangle of
-//           the consistent tangent (21, row-major, scaled by 1 / E)
+//   outputs sigma_{n+1} (6) and the upper triangle of the consistent
+//           tangent (21, row-major), each as its departure from the elastic
+//           trial: o_sigma = (D_e (eps - epsp_n) - sigma) / sig0 and
+//           o_D = (D_e - D) / E, so both vanish on the elastic branch and the
+//           network learns only the plastic correction
 // Training data: random strain paths driven through the plain return
 // mapping (mises_return / mises_tangent, the device form of MisesMat):
 // each path starts virgin and takes 1..maxsteps Gaussian increments of
@@ -39,6 +42,7 @@ constexpr int OFF_B3 = OFF_W3 + MLP_H * MLP_H;
 constexpr int OFF_W4 = OFF_B3 + MLP_H;
 constexpr int OFF_B4 = OFF_W4 + MLP_OUT * MLP_H;
 constexpr int MLP_NPARAM = OFF_B4 + MLP_OUT;
+constexpr int MLP_SPLITK = 32;   // weight-gradient slabs (workspace MLP_SPLITK x 128 x 128)
 
 // --------------------------------------------------------------- hashing RNG
 __host__ __device__ inline uint64_t mix64(uint64_t x)
@@ -80,16 +84,20 @@ __global__ void k_mlp_paths(int64_t npaths, const int64_t *__restrict__ off, int
         mises_tangent(E, nu, sig0, H, kappa, m.kappa, m.sdev, D);
         float *x = X + (o + s) * MLP_IN;
         float *y = Y + (o + s) * MLP_OUT;
+        double el[6], st[6];
+#pragma unroll
+        for (int i = 0; i < 6; i++) el[i] = eps[i] - epsp[i];
+        elastic_trial(E, nu, el, st);
 #pragma unroll
         for (int i = 0; i < 6; i++) {
-            x[i] = (float)((eps[i] - epsp[i]) * in_scale);
+            x[i] = (float)(el[i] * in_scale);
             x[6 + i] = (float)(epsp[i] * in_scale);
-            y[i] = (float)(m.sig[i] * sig_scale);
+            y[i] = (float)((st[i] - m.sig[i]) * sig_scale);
         }
         x[12] = (float)(kappa * in_scale);
-        int k = 6;
+        int k = 0;
         for (int r = 0; r < 6; r++)
-            for (int c = r; c < 6; c++) y[k++] = (float)(D[6 * r + c] * tan_scale);
+            for (int c = r; c < 6; c++, k++) y[6 + k] = (float)((elastic_upper(E, nu, k) - D[6 * r + c]) * tan_scale);
 #pragma unroll
         for (int i = 0; i < 6; i++) epsp[i] = m.epsp[i];
         kappa = m.kappa;
@@ -113,23 +121,27 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const float *
     __shared__ float Bs[16][65];
     const int tm = threadIdx.x / 16, tn = threadIdx.x % 16;
     const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+    // split K: slice z of gridDim.z sums k in [kb, ke) into its own C slab
+    const int kc = (((K + gridDim.z - 1) / gridDim.z) + 15) / 16 * 16;
+    const int kb = blockIdx.z * kc, ke = min(K, kb + kc);
+    C += (int64_t)blockIdx.z * M * ldc;
     float acc[4][4];
 #pragma unroll
     for (int i = 0; i < 4; i++)
 #pragma unroll
         for (int j = 0; j < 4; j++) acc[i][j] = 0.f;
-    for (int k0 = 0; k0 < K; k0 += 16) {
+    for (int k0 = kb; k0 < ke; k0 += 16) {
         for (int t = threadIdx.x; t < 1024; t += 256) {
             int mm, kk;
             if (sam == 1) { mm = t % 64; kk = t / 64; } else { kk = t % 16; mm = t / 16; }
             const int m = m0 + mm, k = k0 + kk;
-            As[kk][mm] = (m < M && k < K) ? A[m * sam + k * sak] : 0.f;
+            As[kk][mm] = (m < M && k < ke) ? A[m * sam + k * sak] : 0.f;
         }
         for (int t = threadIdx.x; t < 1024; t += 256) {
             int nn, kk;
             if (sbn == 1) { nn = t % 64; kk = t / 64; } else { kk = t % 16; nn = t / 16; }
             const int n = n0 + nn, k = k0 + kk;
-            Bs[kk][nn] = (n < N && k < K) ? B[k * sbk + n * sbn] : 0.f;
+            Bs[kk][nn] = (n < N && k < ke) ? B[k * sbk + n * sbn] : 0.f;
         }
         __syncthreads();
 #pragma unroll
@@ -164,6 +176,16 @@ __global__ __launch_bounds__(256) void k_gemm(int M, int N, int K, const float *
             }
             C[m * ldc + n] = v;
         }
+    }
+}
+
+// out = sum over the S split-K slabs of ws (fixed order)
+__global__ void k_sum_slabs(int S, int64_t n, const float *__restrict__ ws, float *__restrict__ out)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        float s = 0.f;
+        for (int z = 0; z < S; z++) s += ws[(int64_t)z * n + i];
+        out[i] = s;
     }
 }
 
@@ -391,6 +413,7 @@ struct hfem_mlp {
     int cap = 0;
     float *d_xb = nullptr, *d_yb = nullptr, *d_z[3] = {nullptr, nullptr, nullptr}, *d_h[3] = {nullptr, nullptr, nullptr};
     float *d_yp = nullptr, *d_dy = nullptr, *d_dh = nullptr;
+    float *d_ws = nullptr;       // split-K workspace of the weight gradients
     double *d_loss = nullptr;
     // BF16 operand-order weights (valid when packed_version == param_version)
     uint16_t *d_wpk = nullptr;
@@ -414,6 +437,7 @@ static int mlp_ensure_batch(hfem_mlp *m, int B)
     HIPFEM_CHECK(hipMalloc(&m->d_yp, sizeof(float) * b * MLP_OUT));
     HIPFEM_CHECK(hipMalloc(&m->d_dy, sizeof(float) * b * MLP_OUT));
     HIPFEM_CHECK(hipMalloc(&m->d_dh, sizeof(float) * b * MLP_H));
+    if (!m->d_ws) HIPFEM_CHECK(hipMalloc(&m->d_ws, sizeof(float) * MLP_SPLITK * MLP_H * MLP_H));
     m->cap = B;
     return 0;
 }
@@ -449,9 +473,19 @@ static int mlp_backward(hfem_mlp *m, int B, const float *xb, const float *yb)
     const int nbm = (int)std::min<int64_t>(1024, (ny + 255) / 256);
     k_mse<<<nbm, 256, 0, s>>>(ny, m->d_yp, yb, 1.f / (float)ny, m->d_dy, m->ctx->d_partials);
     k_sum_partials<<<1, 64, 0, s>>>(nbm, m->ctx->d_partials, 1.0 / (double)ny, m->d_loss);
+    // weight gradients reduce over the batch: split K into S slabs of at
+    // least 256 rows, then sum the slabs in order
+    const int S = std::max(1, std::min(MLP_SPLITK, B / 256));
+    auto wgrad = [&](int M, int N, const float *A, int64_t sam, int64_t sak, const float *Bm, int64_t sbk,
+                     int64_t sbn, float *out) {
+        dim3 g = gemm_grid(M, N);
+        g.z = S;
+        k_gemm<0><<<g, 256, 0, s>>>(M, N, B, A, sam, sak, Bm, sbk, sbn, nullptr, m->d_ws, N, nullptr);
+        const int64_t mn = (int64_t)M * N;
+        k_sum_slabs<<<(int)std::min<int64_t>(1024, (mn + 255) / 256), 256, 0, s>>>(S, mn, m->d_ws, out);
+    };
     // layer 4: dW4 = dY^T H3, db4 = colsum dY, dH3 = dY W4
-    k_gemm<0><<<gemm_grid(MLP_OUT, MLP_H), 256, 0, s>>>(MLP_OUT, MLP_H, B, m->d_dy, 1, MLP_OUT, m->d_h[2], MLP_H, 1,
-                                                        nullptr, G + OFF_W4, MLP_H, nullptr);
+    wgrad(MLP_OUT, MLP_H, m->d_dy, 1, MLP_OUT, m->d_h[2], MLP_H, 1, G + OFF_W4);
     k_colsum<<<MLP_OUT, 256, 0, s>>>(B, MLP_OUT, m->d_dy, G + OFF_B4);
     k_gemm<0><<<gemm_grid(B, MLP_H), 256, 0, s>>>(B, MLP_H, MLP_OUT, m->d_dy, MLP_OUT, 1, P + OFF_W4, MLP_H, 1,
                                                   nullptr, m->d_dh, MLP_H, nullptr);
@@ -462,8 +496,7 @@ static int mlp_backward(hfem_mlp *m, int B, const float *xb, const float *yb)
         k_silu_back<<<nbe, 256, 0, s>>>(nh, m->d_z[L], m->d_dh);   // dZ_L
         const float *in = L == 0 ? xb : m->d_h[L - 1];
         const int nin = L == 0 ? MLP_IN : MLP_H;
-        k_gemm<0><<<gemm_grid(MLP_H, nin), 256, 0, s>>>(MLP_H, nin, B, m->d_dh, 1, MLP_H, in, nin, 1, nullptr,
-                                                        G + offW[L], nin, nullptr);
+        wgrad(MLP_H, nin, m->d_dh, 1, MLP_H, in, nin, 1, G + offW[L]);
         k_colsum<<<MLP_H, 256, 0, s>>>(B, MLP_H, m->d_dh, G + offB[L]);
         if (L > 0) {
             // dH_{L-1} = dZ_L W_L: uses d_dh as input, so write into the z
@@ -480,7 +513,8 @@ static int mlp_backward(hfem_mlp *m, int B, const float *xb, const float *yb)
 static int mlp_free(hfem_mlp *m)
 {
     void *ps[] = {m->d_p, m->d_g, m->d_m, m->d_v, m->d_X, m->d_Y, m->d_xb, m->d_yb, m->d_z[0], m->d_z[1],
-                  m->d_z[2], m->d_h[0], m->d_h[1], m->d_h[2], m->d_yp, m->d_dy, m->d_dh, m->d_loss, m->d_wpk};
+                  m->d_z[2], m->d_h[0], m->d_h[1], m->d_h[2], m->d_yp, m->d_dy, m->d_dh, m->d_loss, m->d_wpk,
+                  m->d_ws};
     for (void *p : ps)
         if (p) HIPFEM_CHECK(hipFree(p));
     return 0;

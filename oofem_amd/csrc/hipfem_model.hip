@@ -366,9 +366,12 @@ __device__ bool surrogate_update(double E, double nu, double sig0, double H, con
                           G * el[3], G * el[4], G * el[5]};
     const double trialS = sqrt(st[0] * st[0] + st[1] * st[1] + st[2] * st[2] +
                                2. * (st[3] * st[3] + st[4] * st[4] + st[5] * st[5]));
+    // network outputs are departures from the elastic trial
     double sp[6];
 #pragma unroll
-    for (int i = 0; i < 6; i++) sp[i] = (double)y[i] / (double)sig_scale;
+    for (int i = 0; i < 3; i++) sp[i] = st[i] + 3. * K * mean - (double)y[i] / (double)sig_scale;
+#pragma unroll
+    for (int i = 3; i < 6; i++) sp[i] = st[i] - (double)y[i] / (double)sig_scale;
     const double pp = (sp[0] + sp[1] + sp[2]) / 3.0;
     const double dv[6] = {sp[0] - pp, sp[1] - pp, sp[2] - pp, sp[3], sp[4], sp[5]};
     const double nrm = sqrt(dv[0] * dv[0] + dv[1] * dv[1] + dv[2] * dv[2] +
@@ -395,7 +398,8 @@ __device__ bool surrogate_update(double E, double nu, double sig0, double H, con
     int k = 0;
     for (int r = 0; r < 6; r++)
         for (int c = r; c < 6; c++, k++)
-            if (!(fabs((double)y[6 + k] / (double)tan_scale - D[6 * r + c]) <= tol * E)) return false;
+            if (!(fabs(elastic_upper(E, nu, k) - (double)y[6 + k] / (double)tan_scale - D[6 * r + c]) <= tol * E))
+                return false;
 #pragma unroll
     for (int i = 0; i < 6; i++) {
         o.sig[i] = sp[i];
@@ -593,7 +597,9 @@ __global__ __launch_bounds__(64) void k_internal(const int *__restrict__ elems, 
                     if (used) {
                         const float *yy = sur.y + (size_t)27 * sl + 6;
 #pragma unroll
-                        for (int k = 0; k < 21; k++) sur.dsur[(size_t)k * ngp_total + gp] = yy[k] / sur.tan_scale;
+                        for (int k = 0; k < 21; k++)
+                            sur.dsur[(size_t)k * ngp_total + gp] =
+                                (float)(elastic_upper(E, nu, k) - (double)yy[k] / (double)sur.tan_scale);
                     }
                 }
             }

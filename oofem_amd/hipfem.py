@@ -490,6 +490,7 @@ class MisesSurrogate:
         self.ctx = ctx
         self.h = C.c_void_p()
         mat = np.array([E, nu, sig0, H], dtype=np.float64)
+        self.E, self.nu, self.sig0, self.H = float(E), float(nu), float(sig0), float(H)
         lib().hfem_mlp_create.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_void_p]
         _check(lib().hfem_mlp_create(ctx.h, _p(mat), C.c_uint64(seed), C.byref(self.h)), "hfem_mlp_create")
         self.nparams = lib().hfem_mlp_nparams()
@@ -550,6 +551,24 @@ class MisesSurrogate:
         _check(lib().hfem_mlp_gradients(self.h, C.c_int(X.shape[0]), _p(X), _p(Y), _p(g), C.byref(loss)),
                "gradients")
         return loss.value, g
+
+    def decode(self, X, Y):
+        """Physical stress (n x 6) and tangent upper triangle (n x 21) from
+        scaled inputs and outputs: the outputs are departures from the
+        elastic trial, sigma = D_e (eps - epsp_n) - sig0 o_sigma and
+        D = D_e - E o_D."""
+        E, nu = self.E, self.nu
+        G, K = E / (2 * (1 + nu)), E / (3 * (1 - 2 * nu))
+        De = np.zeros((6, 6))
+        De[:3, :3] = K - 2 * G / 3
+        De[np.arange(3), np.arange(3)] += 2 * G
+        De[np.arange(3, 6), np.arange(3, 6)] = G
+        el = np.asarray(X, dtype=np.float64)[:, :6] / self.in_scale
+        Y = np.asarray(Y, dtype=np.float64)
+        sig = el @ De.T - Y[:, :6] / self.sig_scale
+        iu = np.triu_indices(6)
+        D = De[iu][None, :] - Y[:, 6:] / self.tan_scale
+        return sig, D
 
     def infer_bf16(self, X):
         X = np.ascontiguousarray(X, dtype=np.float32)

@@ -1,23 +1,29 @@
 #!/usr/bin/env python3
-"""Benchmark of the assembly-and-solve step on one MI355X.
+"""Benchmark of the assembly-and-solve path on one MI355X.
 
-Workload (default): the 64x64x256 LSpace cantilever (1,048,576 elements,
-3,244,800 free equations), jittered +-0.2 h, log-normal E (median 30e3,
-sigma 0.5), nu = 0.2, clamped at z = 0, 1 MPa traction in -y at z = 4.
-One step = LinearStatic::solveYourselfAt on resident data: assemble K
-(batched evaluator), rhs = f_ext - f_int(prescribed), PCG (Jacobi) from x = 0
-to 1e-8 relative residual, reactions.  value = free DOFs x steps / wall time.
+Default workload (config 3): the 64x64x256 LSpace block (1,048,576 elements,
+3,240,575 free equations, jitter +-0.2 h) with MisesMat (E 210e3, nu 0.3,
+sig0 250, H 2e3), clamped at z = 0, u_z prescribed on z = 4 and ramped to
+5x the yield strain over 8 load steps.  One step = one StaticStructural load
+step: tangent initial guess, full Newton with the consistent tangent to the
+1e-8 force criterion, Jacobi-PCG to 1e-6 per correction.  After --c3-warm
+untimed steps, --c3-steps are timed for three forms:
+  fp64             hipcsr + hippcg FP64 + plain return mapping (value)
+  mixed            hippcg mixed 2: FP32 PCG with FP64 reliable updates and
+                   the FP32 element-by-element operator
+  mixed_surrogate  the same with the trained MisesMat surrogate (BF16 MFMA
+                   inference, residual-checked, plain fallback)
+value = free DOFs x timed steps / wall time of the FP64 form; the others,
+their speed-ups and their displacement / stress errors against it are in
+config.variants.
 
-The roofline figure is the PCG SpMV kernel (the step's dominant kernel):
-algorithmic bytes per launch = 12 nnz (FP64 value + int32 column) + 36 N
-(row pointer, gathers of z and p, writes of p and q), divided by the mean
-launch duration measured with HIP events on the library's stream over the
-timed steps.
+roofline: the FP64 PCG SpMV (row-group form on this pattern: the top face
+constrains u_z only), algorithmic bytes per launch over its mean HIP-event
+duration.  cpu_baseline: the oracle (one thread) on a 64x64x16 slab of the
+same problem, scaled by elements, nnz and the GPU's iteration counts.
 
-cpu_baseline: the oracle (C restatement of the reference algorithm, one
-thread) on a 64x64x16 slab of the same mesh family: element assembly,
-f_int and CG iterations are timed there and scaled to the full workload by
-element count, nnz and the GPU run's iteration count.
+Other workloads: --workload cfg1 (LinearStatic cantilever, FP64 vs mixed),
+cfg0, cfg4 (512-RVE batch, solves/s); --mode kernels (counter runs).
 """
 import argparse
 import json
@@ -40,6 +46,8 @@ def make_deck(args):
     from oofem_amd import meshgen
     if args.workload == "cfg0":
         return meshgen.config0()
+    if args.workload == "cfg3":
+        return meshgen.config3(nx=args.nx, ny=args.ny, nz=args.nz)
     return meshgen.config1(nx=args.nx, ny=args.ny, nz=args.nz)
 
 
@@ -160,7 +168,8 @@ def run_newton(args, deck, precision, warm=1, steps=None, total=None, surrogate=
     p = P.profile()
     out = dict(dt=dt, steps=steps, newton=[r["newton"] for r in reports[warm:]],
                pcg=[r["pcg"] for r in reports[warm:]], n_spmv=p["n64"], ms_spmv=p["ms64"], n_spmv32=p["n32"],
-               ms_spmv32=p["ms32"], nnz=P.matrix_nnz(), u=P.solution(), info=P.solver_info())
+               ms_spmv32=p["ms32"], nnz=P.matrix_nnz(), u=P.solution(), info=P.solver_info(),
+               layout=P.matrix_layout())
     if keep_state:
         gs = P.state()
         out["stress"] = gs[:, 20:26].copy()
@@ -327,10 +336,19 @@ def main_newton(args):
     roof = None
     if r["n_spmv"]:
         avg_ms = r["ms_spmv"] / r["n_spmv"]
-        # scalar CSR PCG SpMV: value + column per nonzero, row pointer, gather p, write q
-        bytes_per = 12.0 * r["nnz"] + 20.0 * n
+        lay = r["layout"]
+        if lay["form"] == 2:
+            # row-group SpMV: values, one int32 column and one int32 offset per
+            # chunk, group bounds (2 int32), row pointers, gather p, write q
+            bytes_per = 8.0 * r["nnz"] + 8.0 * lay["blocks"] + 8.0 * lay["groups"] + 4.0 * n + 16.0 * n
+            kname = "k_pcg_spmv_p_grp<double, double>"
+        elif lay["form"] == 1:
+            bytes_per = 8.0 * r["nnz"] + 4.0 * r["nnz"] / 9.0 + 4.0 * n / 3.0 + 16.0 * n
+            kname = "k_pcg_spmv_p_nb<double, double>"
+        else:
+            bytes_per = 12.0 * r["nnz"] + 20.0 * n
+            kname = "k_pcg_spmv_p<double, double> (lane form)"
         ach = bytes_per / (avg_ms * 1e-3) / 1e9
-        kname = "k_pcg_spmv_p<double, double> (lane form)"
         roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 4),
                     traffic=measured_traffic(kname, deck.name, args), kernel=kname, avg_launch_ms=round(avg_ms, 4),
                     launches=r["n_spmv"], algorithmic_bytes=int(bytes_per))
@@ -418,13 +436,13 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["cfg1", "cfg0", "cfg3", "cfg4"], default="cfg1")
+    ap.add_argument("--workload", choices=["cfg1", "cfg0", "cfg3", "cfg4"], default="cfg3")
     ap.add_argument("--rve-count", type=int, default=512)
     ap.add_argument("--rve-tol", type=float, default=1e-8)
     ap.add_argument("--rve-tune-sample", type=int, default=16)
     ap.add_argument("--rve-check", type=int, default=8)
-    ap.add_argument("--c3-warm", type=int, default=2, help="untimed load steps before the timed ones (cfg3)")
-    ap.add_argument("--c3-steps", type=int, default=2, help="timed load steps (cfg3)")
+    ap.add_argument("--c3-warm", type=int, default=1, help="untimed load steps before the timed ones (cfg3)")
+    ap.add_argument("--c3-steps", type=int, default=1, help="timed load steps (cfg3)")
     ap.add_argument("--c3-total", type=int, default=8, help="load steps of the full ramp (cfg3)")
     ap.add_argument("--sur-samples", type=int, default=20_000_000)
     ap.add_argument("--sur-steps", type=int, default=20000)
@@ -459,7 +477,7 @@ def main():
               "(the multi-GPU slab decomposition is a design, not code).", file=sys.stderr)
         return 2
 
-    if args.workload == "cfg3":
+    if args.workload == "cfg3" and args.mode == "step":
         return main_newton(args)
     if args.workload == "cfg4":
         return main_rve(args)

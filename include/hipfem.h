@@ -142,6 +142,8 @@ int hipcsr_set_spmv_stream(hipcsr_mtx *m, int rows_per_block, int unroll);
  * applies.  hipcsr_is_nodeblocked: 1 block form, 2 row-group form, 0 none. */
 int hipcsr_set_spmv_nodeblock(hipcsr_mtx *m, int on);
 int hipcsr_is_nodeblocked(const hipcsr_mtx *m);
+/* node rows (block form) or row groups, and 3x3 blocks or column chunks */
+int hipcsr_nodeblock_sizes(const hipcsr_mtx *m, int64_t *groups, int64_t *blocks);
 /* element-by-element operator (no counterpart in the reference): after
  * hfem_model_assemble_stiffness on an LSpace mesh the matrix remembers the
  * evaluator, and while its values stay unchanged the FP32 PCG of precision

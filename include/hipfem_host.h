@@ -66,6 +66,9 @@ int hfem_problem_context(hfem_problem *p, hipfem_ctx **ctx);
  * batched evaluator (no counterpart in the reference) */
 int hfem_problem_set_surrogate(hfem_problem *p, hfem_mlp *net, double band, double tol);
 int hfem_problem_surrogate_stats(hfem_problem *p, long long *counts3, int reset);
+/* SpMV layout of the factory-made matrix: form (1 node-block, 2 row-group,
+ * 0 scalar CSR), node rows or row groups, 3x3 blocks or column chunks */
+int hfem_problem_matrix_layout(hfem_problem *p, int *form, long long *groups, long long *blocks);
 /* SparseMtrx::at / giveClassName of the factory-made stiffness matrix */
 int hfem_problem_matrix_at(hfem_problem *p, int i, int j, double *v);
 int hfem_problem_matrix_class(hfem_problem *p, char *buf, int len);

@@ -1022,6 +1022,20 @@ extern "C" int hipcsr_set_spmv_nodeblock(hipcsr_mtx *m, int on)
     return 0;
 }
 
+extern "C" int hipcsr_nodeblock_sizes(const hipcsr_mtx *m, int64_t *groups, int64_t *blocks)
+{
+    HIPFEM_REQUIRE(m && groups && blocks, HIPFEM_EINVAL, "null argument");
+    *groups = m->nodeblocked ? m->nbr : m->grouped ? m->ngrp : 0;
+    *blocks = 0;
+    if (m->nodeblocked) *blocks = m->nnz / 9;
+    if (m->grouped) {
+        int32_t nch = 0;
+        HIPFEM_CHECK(hipMemcpy(&nch, m->d_gch + m->ngrp, sizeof(int32_t), hipMemcpyDeviceToHost));
+        *blocks = nch;
+    }
+    return 0;
+}
+
 extern "C" int hipcsr_is_nodeblocked(const hipcsr_mtx *m)
 {
     return !m ? 0 : m->nodeblocked ? 1 : m->grouped ? 2 : 0;

@@ -729,28 +729,12 @@ __global__ void k_fill_d(int n, double *__restrict__ a, const PcgState *__restri
 // element arithmetic in FP32, the sum over elements in FP64 (neighbouring
 // element forces of a smooth field nearly cancel at a node, by a factor of
 // about h/L, so an FP32 sum would lose that many digits).
-// Modal form of the trilinear hexahedron: a nodal field f_a is
-// sum_m c_m phi_m with phi = (1, xi, eta, zeta, xi eta, eta zeta, xi zeta,
-// xi eta zeta) and c_m = 1/8 sum_a s_m(a) f_a (s_m the products of the
-// node's coordinate signs), so a reference gradient at a Gauss point costs 4
-// terms per entry instead of 8, and the forces B^T sigma dV are gathered as
-// 7 modal vectors F_m = sum_k T_ik dphi_m/dxi_k and mapped back to the
-// nodes once per element.
-// 8 threads per element and 32 elements per block; thread g owns node g
-// (gathers, modal coefficient g, the final force of node g) and Gauss point g.
+// 8 threads per element (one per Gauss point), 32 elements per block.  Node
+// data are staged in LDS relative to node 1 (coordinates differenced in FP64,
+// displacements in FP32: a rigid translation adds nothing to the strain), the
+// Gauss-point forces B^T sigma dV are summed over the points in ascending
+// order and each of the 8 threads scatters 3 of the 24 components.
 constexpr int MF_EPB = 32;
-__constant__ float kModeSign[8][8] = {
-    // node:  1    2    3    4    5    6    7    8  (LSpace local order)
-    {1, 1, 1, 1, 1, 1, 1, 1},            // 1
-    {-1, -1, 1, 1, -1, -1, 1, 1},        // xi
-    {-1, 1, 1, -1, -1, 1, 1, -1},        // eta
-    {1, 1, 1, 1, -1, -1, -1, -1},        // zeta
-    {1, -1, 1, -1, 1, -1, 1, -1},        // xi eta
-    {-1, 1, 1, -1, 1, -1, -1, 1},        // eta zeta
-    {-1, -1, 1, 1, 1, 1, -1, -1},        // xi zeta
-    {1, -1, 1, -1, -1, 1, -1, 1},        // xi eta zeta
-};
-
 __global__ __launch_bounds__(256) void k_mf_hex(const int *__restrict__ elems, int nel,
                                                 const double *__restrict__ coords, const int *__restrict__ conn,
                                                 const int *__restrict__ eqn, const int *__restrict__ emat,
@@ -759,13 +743,10 @@ __global__ __launch_bounds__(256) void k_mf_hex(const int *__restrict__ elems, i
                                                 const float *__restrict__ x, double *__restrict__ y,
                                                 const PcgState *__restrict__ st)
 {
-    __shared__ double sXd[MF_EPB][8][3];
+    __shared__ float sX[MF_EPB][8][3];
     __shared__ float sU[MF_EPB][8][3];
-    __shared__ float sCX[MF_EPB][8][3];
-    __shared__ float sCU[MF_EPB][8][3];
     __shared__ int sQ[MF_EPB][8][3];
-    __shared__ float sF[MF_EPB][8][22];
-    __shared__ float sFs[MF_EPB][22];
+    __shared__ float sF[MF_EPB][8][25];
     if (st && halted(st)) return;
     const int t = threadIdx.x, le = t >> 3, g = t & 7;
     const int ei = blockIdx.x * MF_EPB + le;
@@ -773,50 +754,40 @@ __global__ __launch_bounds__(256) void k_mf_hex(const int *__restrict__ elems, i
     int e = 0;
     if (active) {
         e = elems[ei];
-        const int node = conn[(size_t)e * 8 + g];
+        const int node = conn[(size_t)e * 8 + g], node0 = conn[(size_t)e * 8];
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-            sXd[le][g][d] = coords[3 * (size_t)node + d];
+            sX[le][g][d] = (float)(coords[3 * (size_t)node + d] - coords[3 * (size_t)node0 + d]);
             const int q = eqn[3 * (size_t)node + d];
             sQ[le][g][d] = q;
             sU[le][g][d] = q > 0 ? x[q - 1] : 0.0f;
         }
     }
     __syncthreads();
-    if (active && g > 0) {
-        // modal coefficient g of the coordinates (FP64 sums: differences of
-        // nearby points) and of the displacement
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-            double cx = 0.0;
-            float cu = 0.f;
-#pragma unroll
-            for (int a = 0; a < 8; a++) {
-                cx += (double)kModeSign[g][a] * sXd[le][a][d];
-                cu += kModeSign[g][a] * sU[le][a][d];
-            }
-            sCX[le][g][d] = (float)(0.125 * cx);
-            sCU[le][g][d] = 0.125f * cu;
-        }
-    }
-    __syncthreads();
     if (active) {
-        const float q3 = 0.577350269189626f;
-        const float x0 = (g & 4) ? q3 : -q3, x1 = (g & 2) ? q3 : -q3, x2 = (g & 1) ? q3 : -q3;
-        // reference gradients: column k = d/dxi_k of (c_1..c_7 . phi)
-        float J[9], gu[9];
+        const float a = 0.577350269189626f;
+        const float xi0 = (g & 4) ? a : -a, xi1 = (g & 2) ? a : -a, xi2 = (g & 1) ? a : -a;
+        float dN[8][3];
+        float J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        float gu[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};   // du_i / dxi_j
+        const float u0[3] = {sU[le][0][0], sU[le][0][1], sU[le][0][2]};
 #pragma unroll
-        for (int i = 0; i < 3; i++) {
-            const float cx1 = sCX[le][1][i], cx2 = sCX[le][2][i], cx3 = sCX[le][3][i], cx4 = sCX[le][4][i];
-            const float cx5 = sCX[le][5][i], cx6 = sCX[le][6][i], cx7 = sCX[le][7][i];
-            J[3 * i + 0] = cx1 + cx4 * x1 + cx6 * x2 + cx7 * x1 * x2;
-            J[3 * i + 1] = cx2 + cx4 * x0 + cx5 * x2 + cx7 * x0 * x2;
-            J[3 * i + 2] = cx3 + cx5 * x1 + cx6 * x0 + cx7 * x0 * x1;
-            const float cu1 = sCU[le][1][i], cu2 = sCU[le][2][i], cu3 = sCU[le][3][i], cu4 = sCU[le][4][i];
-            const float cu5 = sCU[le][5][i], cu6 = sCU[le][6][i], cu7 = sCU[le][7][i];
-            gu[3 * i + 0] = cu1 + cu4 * x1 + cu6 * x2 + cu7 * x1 * x2;
-            gu[3 * i + 1] = cu2 + cu4 * x0 + cu5 * x2 + cu7 * x0 * x2;
-            gu[3 * i + 2] = cu3 + cu5 * x1 + cu6 * x0 + cu7 * x0 * x1;
+        for (int n = 0; n < 8; n++) {
+            const float sx = (n == 2 || n == 3 || n == 6 || n == 7) ? 1.f : -1.f;
+            const float sy = (n == 1 || n == 2 || n == 5 || n == 6) ? 1.f : -1.f;
+            const float sz = (n < 4) ? 1.f : -1.f;
+            dN[n][0] = sx * 0.125f * (1.f + sy * xi1) * (1.f + sz * xi2);
+            dN[n][1] = sy * 0.125f * (1.f + sx * xi0) * (1.f + sz * xi2);
+            dN[n][2] = sz * 0.125f * (1.f + sx * xi0) * (1.f + sy * xi1);
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                const float xv = sX[le][n][i], uv = sU[le][n][i] - u0[i];
+#pragma unroll
+                for (int j = 0; j < 3; j++) {
+                    J[3 * i + j] = fmaf(xv, dN[n][j], J[3 * i + j]);
+                    gu[3 * i + j] = fmaf(uv, dN[n][j], gu[3 * i + j]);
+                }
+            }
         }
         const float det = J[0] * (J[4] * J[8] - J[5] * J[7]) - J[1] * (J[3] * J[8] - J[5] * J[6]) +
                           J[2] * (J[3] * J[7] - J[4] * J[6]);
@@ -878,45 +849,32 @@ __global__ __launch_bounds__(256) void k_mf_hex(const int *__restrict__ elems, i
             sig[4] = D[4] * eps[0] + D[9] * eps[1] + D[13] * eps[2] + D[16] * eps[3] + D[18] * eps[4] + D[19] * eps[5];
             sig[5] = D[5] * eps[0] + D[10] * eps[1] + D[14] * eps[2] + D[17] * eps[3] + D[19] * eps[4] + D[20] * eps[5];
         }
-        // T = dV sigma J^-T, then the modal force vectors F_1..F_7
+        // T = dV sigma J^-T: f_n = T dN_n (dN/dx = J^-T dN/dxi in column form)
         const float dV = fabsf(det);
         const float S[9] = {sig[0], sig[5], sig[4], sig[5], sig[1], sig[3], sig[4], sig[3], sig[2]};
+        float T[9];
 #pragma unroll
-        for (int i = 0; i < 3; i++) {
-            const float t0 = dV * (S[3 * i] * I[0] + S[3 * i + 1] * I[1] + S[3 * i + 2] * I[2]);
-            const float t1 = dV * (S[3 * i] * I[3] + S[3 * i + 1] * I[4] + S[3 * i + 2] * I[5]);
-            const float t2 = dV * (S[3 * i] * I[6] + S[3 * i + 1] * I[7] + S[3 * i + 2] * I[8]);
-            float *F = sF[le][g];
-            F[0 * 3 + i] = t0;
-            F[1 * 3 + i] = t1;
-            F[2 * 3 + i] = t2;
-            F[3 * 3 + i] = t0 * x1 + t1 * x0;
-            F[4 * 3 + i] = t1 * x2 + t2 * x1;
-            F[5 * 3 + i] = t0 * x2 + t2 * x0;
-            F[6 * 3 + i] = t0 * x1 * x2 + t1 * x0 * x2 + t2 * x0 * x1;
-        }
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int k = 0; k < 3; k++)
+                T[3 * i + k] = dV * (S[3 * i] * I[3 * k] + S[3 * i + 1] * I[3 * k + 1] + S[3 * i + 2] * I[3 * k + 2]);
+#pragma unroll
+        for (int n = 0; n < 8; n++)
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+                sF[le][g][3 * n + i] = T[3 * i] * dN[n][0] + T[3 * i + 1] * dN[n][1] + T[3 * i + 2] * dN[n][2];
     }
     __syncthreads();
     if (active) {
-        // sum over the Gauss points (ascending), 21 entries shared by 8 threads
-        for (int k = g; k < 21; k += 8) {
-            float f = 0.f;
 #pragma unroll
-            for (int gg = 0; gg < 8; gg++) f += sF[le][gg][k];
-            sFs[le][k] = f;
-        }
-    }
-    __syncthreads();
-    if (active) {
-        // node g: f_g = 1/8 sum_m s_m(g) F_m
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-            const int q = sQ[le][g][i];
+        for (int k = 0; k < 3; k++) {
+            const int cidx = g + 8 * k;   // component 3 n + i
+            const int q = sQ[le][cidx / 3][cidx % 3];
             if (q <= 0) continue;
             float f = 0.f;
 #pragma unroll
-            for (int mm = 1; mm < 8; mm++) f += kModeSign[mm][g] * sFs[le][3 * (mm - 1) + i];
-            y[q - 1] += (double)(0.125f * f);
+            for (int gg = 0; gg < 8; gg++) f += sF[le][gg][cidx];
+            y[q - 1] += (double)f;
         }
     }
 }

@@ -170,6 +170,19 @@ int hfem_problem_surrogate_stats(hfem_problem *p, long long *counts3, int reset)
     });
 }
 
+int hfem_problem_matrix_layout(hfem_problem *p, int *form, long long *groups, long long *blocks)
+{
+    return guarded([&] {
+        auto *H = dynamic_cast<HipCSRSparseMtrx *>(reinterpret_cast<StructuralProblem *>(p)->giveStiffnessMatrix());
+        if (!H) OOFEM_ERROR("no hipcsr stiffness matrix yet");
+        int64_t g = 0, b = 0;
+        hipfemCheck(hipcsr_nodeblock_sizes(H->handle(), &g, &b), "nodeblock_sizes");
+        *form = hipcsr_is_nodeblocked(H->handle());
+        *groups = g;
+        *blocks = b;
+    });
+}
+
 int hfem_problem_matrix_nnz(hfem_problem *p, long long *nnz)
 {
     return guarded([&] {

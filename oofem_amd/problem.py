@@ -152,6 +152,12 @@ class StructuralProblem:
         _check(hostlib().hfem_problem_matrix_at(self.h, int(i), int(j), C.byref(v)), "matrix_at")
         return v.value
 
+    def matrix_layout(self):
+        f = C.c_int(0)
+        g, b = C.c_longlong(0), C.c_longlong(0)
+        _check(hostlib().hfem_problem_matrix_layout(self.h, C.byref(f), C.byref(g), C.byref(b)), "matrix_layout")
+        return dict(form=f.value, groups=g.value, blocks=b.value)
+
     def matrix_nnz(self):
         v = C.c_int64(0)
         _check(hostlib().hfem_problem_matrix_nnz(self.h, C.byref(v)), "matrix_nnz")

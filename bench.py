@@ -68,6 +68,8 @@ def run_kernels(args, deck):
     """Kernel microbenchmark for counter runs: a copy-like calibration kernel
     of known size (y = x over 2^27 doubles: 1 GiB read + 1 GiB written), then
     --iters PCG iterations (FP64 and FP32-inner) on the assembled K."""
+    import numpy as np
+
     from oofem_amd.hipfem import BatchedEvaluator, Context, DeviceVector, HipCSRSparseMtrx, HipPCGSolver
     ctx = Context(0)
     n_copy = 1 << 27
@@ -82,6 +84,8 @@ def run_kernels(args, deck):
     ev.assemble_stiffness(K)
     apply_shape(K, args.shape)
     f, _ = deck.external_load()
+    if not f.any():
+        f = np.ones(deck.neq)          # displacement-driven decks carry no load: any non-zero b will do
     db = DeviceVector(ctx, deck.neq, f)
     dx = DeviceVector(ctx, deck.neq)
     out = {}

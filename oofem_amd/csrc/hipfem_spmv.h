@@ -117,6 +117,17 @@ __device__ __forceinline__ void grp_row(int r0, int nr, int c0, int nch, int lan
         const uint32_t cc = ccol[c0 + k];
         const int col = (int)(cc & GRP_COL_MASK), L = (int)(cc >> 30) + 1;
         const int off = coff[c0 + k];
+        if (nr == 3 && L == 3) {
+            // full 3x3 block (the common case): nine independent loads
+            const AT x0 = gather(col), x1 = gather(col + 1), x2 = gather(col + 2);
+            const MT *v0 = val + rp0 + off, *v1 = val + rp1 + off, *v2 = val + rp2 + off;
+            const MT a0 = v0[0], a1 = v0[1], a2 = v0[2], b0 = v1[0], b1 = v1[1], b2 = v1[2];
+            const MT c0v = v2[0], c1v = v2[1], c2v = v2[2];
+            acc[0] += (AT)a0 * x0 + (AT)a1 * x1 + (AT)a2 * x2;
+            acc[1] += (AT)b0 * x0 + (AT)b1 * x1 + (AT)b2 * x2;
+            acc[2] += (AT)c0v * x0 + (AT)c1v * x1 + (AT)c2v * x2;
+            continue;
+        }
         const AT x0 = gather(col), x1 = L > 1 ? gather(col + 1) : AT(0), x2 = L > 2 ? gather(col + 2) : AT(0);
         const int rps[3] = {rp0, rp1, rp2};
 #pragma unroll

@@ -241,6 +241,11 @@ int hfem_model_tangent_product_dev(hfem_model *md, int mode, const double *dx_fr
 int hfem_model_residual(hfem_model *md, const double *dRT, const double *dF, double *drhs, double *sums6);
 /* MisesMatStatus::updateYourself (misesmat.C:720-728) for every Gauss point */
 int hfem_model_commit(hfem_model *md);
+/* form of the FP32 element-by-element operator used by hippcg precision 2
+ * (no counterpart in the reference): 1 = every element in one launch, its
+ * forces stored and summed per node in FP64 (default); 0 = one launch per
+ * element color adding into the FP64 result */
+int hfem_model_set_element_operator(hfem_model *md, int form);
 /* reset every Gauss point to the virgin state */
 int hfem_model_reset_state(hfem_model *md);
 /* Gauss point state, ngp_total x 38 doubles in the oracle's layout */

@@ -725,15 +725,18 @@ __global__ void k_fill_d(int n, double *__restrict__ a, const PcgState *__restri
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) a[i] = 0.0;
 }
 
-// y += K_e x_e over the free equations of one color of LSpace elements:
-// element arithmetic in FP32, the sum over elements in FP64 (neighbouring
-// element forces of a smooth field nearly cancel at a node, by a factor of
-// about h/L, so an FP32 sum would lose that many digits).
+// K_e x_e of LSpace elements: element arithmetic in FP32, the sum over
+// elements in FP64 (neighbouring element forces of a smooth field nearly
+// cancel at a node, by a factor of about h/L, so an FP32 sum would lose that
+// many digits).
 // 8 threads per element (one per Gauss point), 32 elements per block.  Node
 // data are staged in LDS relative to node 1 (coordinates differenced in FP64,
 // displacements in FP32: a rigid translation adds nothing to the strain), the
 // Gauss-point forces B^T sigma dV are summed over the points in ascending
-// order and each of the 8 threads scatters 3 of the 24 components.
+// order and each of the 8 threads handles 3 of the 24 components: either
+// y += f over one color (elems = the color's list), or, with ebuf, the
+// element's 24 forces are stored (elems = NULL: every element, one launch)
+// for k_mf_gather to sum per node.
 constexpr int MF_EPB = 32;
 __global__ __launch_bounds__(256) void k_mf_hex(const int *__restrict__ elems, int nel,
                                                 const double *__restrict__ coords, const int *__restrict__ conn,
@@ -741,7 +744,7 @@ __global__ __launch_bounds__(256) void k_mf_hex(const int *__restrict__ elems, i
                                                 const double *__restrict__ mat, const float *__restrict__ desc,
                                                 const float *__restrict__ dgen, int ngp_total,
                                                 const float *__restrict__ x, double *__restrict__ y,
-                                                const PcgState *__restrict__ st)
+                                                float *__restrict__ ebuf, const PcgState *__restrict__ st)
 {
     __shared__ float sX[MF_EPB][8][3];
     __shared__ float sU[MF_EPB][8][3];
@@ -753,7 +756,7 @@ __global__ __launch_bounds__(256) void k_mf_hex(const int *__restrict__ elems, i
     const bool active = ei < nel;
     int e = 0;
     if (active) {
-        e = elems[ei];
+        e = elems ? elems[ei] : ei;
         const int node = conn[(size_t)e * 8 + g], node0 = conn[(size_t)e * 8];
 #pragma unroll
         for (int d = 0; d < 3; d++) {
@@ -870,12 +873,39 @@ __global__ __launch_bounds__(256) void k_mf_hex(const int *__restrict__ elems, i
         for (int k = 0; k < 3; k++) {
             const int cidx = g + 8 * k;   // component 3 n + i
             const int q = sQ[le][cidx / 3][cidx % 3];
-            if (q <= 0) continue;
             float f = 0.f;
 #pragma unroll
             for (int gg = 0; gg < 8; gg++) f += sF[le][gg][cidx];
-            y[q - 1] += (double)f;
+            if (ebuf)
+                ebuf[(size_t)e * 24 + cidx] = f;
+            else if (q > 0)
+                y[q - 1] += (double)f;
         }
+    }
+}
+
+// y = sum of the stored element forces per free equation, FP64, in the
+// fixed order of the node's element list (nptr / nadj: element * 8 + local
+// node, ascending element); one thread per node writes all its free dofs.
+__global__ void k_mf_gather(int nnode, const int *__restrict__ nptr, const int *__restrict__ nadj,
+                            const int *__restrict__ eqn, const float *__restrict__ ebuf, double *__restrict__ y,
+                            const PcgState *__restrict__ st)
+{
+    if (st && halted(st)) return;
+    for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < nnode; n += gridDim.x * blockDim.x) {
+        const int q0 = eqn[3 * (size_t)n], q1 = eqn[3 * (size_t)n + 1], q2 = eqn[3 * (size_t)n + 2];
+        if (q0 <= 0 && q1 <= 0 && q2 <= 0) continue;
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+        for (int j = nptr[n]; j < nptr[n + 1]; j++) {
+            const int ea = nadj[j];
+            const float *f = ebuf + (size_t)(ea >> 3) * 24 + 3 * (ea & 7);
+            s0 += (double)f[0];
+            s1 += (double)f[1];
+            s2 += (double)f[2];
+        }
+        if (q0 > 0) y[q0 - 1] = s0;
+        if (q1 > 0) y[q1 - 1] = s1;
+        if (q2 > 0) y[q2 - 1] = s2;
     }
 }
 
@@ -986,8 +1016,11 @@ struct hfem_model {
     // element-by-element operator: per-GP FP32 tangent descriptors of the last
     // assembly and the matrices that assembly linked
     float *d_tdesc = nullptr;
+    int *d_nptr = nullptr, *d_nadj = nullptr;   // node -> (element * 8 + local node), ascending element
+    float *d_ebuf = nullptr;                     // element forces of the gather form (24 per element)
     bool tdesc_used = false;     // false: every point elastic, D from the material table
     bool has_mises = false;
+    bool use_gather = true;      // element operator: gather form (else colored scatter)
     std::vector<hipcsr_mtx *> linked;
     // MisesMat surrogate (hfem_model_set_surrogate)
     hfem_mlp *sur = nullptr;
@@ -1017,10 +1050,12 @@ static int model_free(hfem_model *md)
     }
     void *ps[] = {md->d_coords, md->d_mat, md->d_state, md->d_conn, md->d_emat, md->d_eqn, md->d_bad,
                   md->d_eoff,   md->d_colors, md->d_uf, md->d_up, md->d_ff, md->d_fp, md->d_out3, md->d_dofid,
-                  md->d_tdesc};
+                  md->d_tdesc, md->d_nptr, md->d_nadj, md->d_ebuf};
     for (void *p : ps)
         if (p) HIPFEM_CHECK(hipFree(p));
     md->d_tdesc = nullptr;
+    md->d_nptr = md->d_nadj = nullptr;
+    md->d_ebuf = nullptr;
     md->d_coords = md->d_mat = md->d_state = md->d_uf = md->d_up = md->d_ff = md->d_fp = md->d_out3 = nullptr;
     md->d_conn = md->d_emat = md->d_eqn = md->d_bad = md->d_colors = nullptr;
     md->d_dofid = nullptr;
@@ -1329,6 +1364,21 @@ extern "C" int hfem_model_assemble_stiffness(hfem_model *md, hipcsr_mtx *K, int 
                                                                           md->sur ? md->d_sflag : nullptr);
             HIPFEM_CHECK(hipGetLastError());
         }
+        if (!md->d_nptr) {
+            // node -> element adjacency of the gather form (built once per mesh)
+            std::vector<int> cnt((size_t)md->nnode + 1, 0), adj((size_t)md->nelem * 8);
+            for (size_t i = 0; i < (size_t)md->nelem * 8; i++) cnt[md->h_conn[i] + 1]++;
+            for (int n = 0; n < md->nnode; n++) cnt[n + 1] += cnt[n];
+            std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+            for (int e = 0; e < md->nelem; e++)
+                for (int a = 0; a < 8; a++) adj[fill[md->h_conn[(size_t)e * 8 + a]]++] = e * 8 + a;
+            HIPFEM_CHECK(hipMalloc(&md->d_nptr, sizeof(int) * cnt.size()));
+            HIPFEM_CHECK(hipMalloc(&md->d_nadj, sizeof(int) * std::max<size_t>(1, adj.size())));
+            HIPFEM_CHECK(hipMalloc(&md->d_ebuf, sizeof(float) * 24 * std::max<size_t>(1, (size_t)md->nelem)));
+            HIPFEM_CHECK(hipMemcpy(md->d_nptr, cnt.data(), sizeof(int) * cnt.size(), hipMemcpyHostToDevice));
+            if (!adj.empty())
+                HIPFEM_CHECK(hipMemcpy(md->d_nadj, adj.data(), sizeof(int) * adj.size(), hipMemcpyHostToDevice));
+        }
         if (K->mf_model && K->mf_model != md) model_unlink(K->mf_model, K);
         // the descriptors now describe this assembly only
         const std::vector<hipcsr_mtx *> others = md->linked;
@@ -1357,6 +1407,17 @@ void model_unlink(hfem_model *md, hipcsr_mtx *m)
 int model_mf_apply_f32(hfem_model *md, const float *x, double *y, const PcgState *st)
 {
     hipStream_t s = md->ctx->stream;
+    if (md->d_nptr && md->use_gather) {
+        // gather form: every element in one launch, then one FP64 sum per node
+        if (md->nelem)
+            k_mf_hex<<<(md->nelem + MF_EPB - 1) / MF_EPB, 256, 0, s>>>(
+                nullptr, md->nelem, md->d_coords, md->d_conn, md->d_eqn, md->d_emat, md->d_mat,
+                md->tdesc_used ? md->d_tdesc : nullptr, md->d_dsur, md->ngp_total, x, y, md->d_ebuf, st);
+        k_mf_gather<<<std::min(8192, (md->nnode + 255) / 256), 256, 0, s>>>(md->nnode, md->d_nptr, md->d_nadj,
+                                                                              md->d_eqn, md->d_ebuf, y, st);
+        HIPFEM_CHECK(hipGetLastError());
+        return 0;
+    }
     if (md->neq) k_fill_d<<<std::min(4096, (md->neq + 255) / 256), 256, 0, s>>>(md->neq, y, st);
     for (int c = 0; c < md->ncolors; c++) {
         const int n = md->color_ptr[c + 1] - md->color_ptr[c];
@@ -1364,7 +1425,7 @@ int model_mf_apply_f32(hfem_model *md, const float *x, double *y, const PcgState
         k_mf_hex<<<(n + MF_EPB - 1) / MF_EPB, 256, 0, s>>>(md->d_colors + md->color_ptr[c], n, md->d_coords,
                                                            md->d_conn, md->d_eqn, md->d_emat, md->d_mat,
                                                            md->tdesc_used ? md->d_tdesc : nullptr, md->d_dsur,
-                                                           md->ngp_total, x, y, st);
+                                                           md->ngp_total, x, y, nullptr, st);
     }
     HIPFEM_CHECK(hipGetLastError());
     return 0;
@@ -1516,6 +1577,13 @@ extern "C" int hfem_model_set_surrogate(hfem_model *md, hfem_mlp *net, double ba
     md->sur_band = band;
     md->sur_tol = tol;
     for (int i = 0; i < 3; i++) md->sur_scales[i] = sc[i];
+    return 0;
+}
+
+extern "C" int hfem_model_set_element_operator(hfem_model *md, int form)
+{
+    HIPFEM_REQUIRE(md && (form == 0 || form == 1), HIPFEM_EINVAL, "element operator form must be 0 or 1");
+    md->use_gather = form == 1;
     return 0;
 }
 

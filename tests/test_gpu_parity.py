@@ -314,8 +314,9 @@ def test_mixed_precision_reliable_updates(ctx, operator):
     assert got["iters"] <= (2.0 if operator == "element" else 2.5) * it64, (got["iters"], it64)
 
 
+@pytest.mark.parametrize("gather", [True, False])
 @pytest.mark.parametrize("plastic", [False, True])
-def test_element_operator_matches_assembled_matrix(ctx, plastic):
+def test_element_operator_matches_assembled_matrix(ctx, plastic, gather):
     """The FP32 element-by-element operator equals the assembled FP64 matrix
     to FP32 accuracy: elastic (IsoLE, log-normal E) and the MisesMat
     consistent tangent of a plastic state (descriptor form c I_dev + K 1x1 -
@@ -338,6 +339,7 @@ def test_element_operator_matches_assembled_matrix(ctx, plastic):
         s = ev.get_state()
         assert (s[:, 13] > s[:, 6]).mean() > 0.3
     ev.assemble_stiffness(K, tangent=plastic)
+    ev.set_element_operator(gather)
     assert K.is_matrix_free()
     rng = np.random.default_rng(1)
     for _ in range(3):

@@ -75,9 +75,22 @@ def run_kernels(args, deck):
     n_copy = 1 << 27
     a = DeviceVector(ctx, n_copy)
     b = DeviceVector(ctx, n_copy)
+    from oofem_amd.hipfem import lib as _lib
+    import ctypes as _C
     for _ in range(3):
         b.axpby(1.0, a, 0.0)
     ctx.synchronize()
+    copy_gbs = {}
+    for name, fn in (("axpby", lambda: b.axpby(1.0, a, 0.0)),
+                     ("hipMemcpy", lambda: _lib().hipfem_dev_copy(ctx.h, _C.c_int64(n_copy), a.ptr, b.ptr, 0)),
+                     ("wide_kernel", lambda: _lib().hipfem_dev_copy(ctx.h, _C.c_int64(n_copy), a.ptr, b.ptr, 1))):
+        fn()
+        ctx.synchronize()
+        t = time.perf_counter()
+        for _ in range(5):
+            fn()
+        ctx.synchronize()
+        copy_gbs[name] = round(5 * 2 * 8 * n_copy / (time.perf_counter() - t) / 1e9, 1)
     K = HipCSRSparseMtrx(ctx)
     ev = BatchedEvaluator(ctx, deck)
     ev.build_structure(K)
@@ -98,7 +111,7 @@ def run_kernels(args, deck):
         out[name] = (p["ms64"] / max(p["n64"], 1)) if prec == 0 else (p["ms32"] / max(p["n32"], 1))
         s.close()
     print(json.dumps(dict(mode="kernels", neq=deck.neq, nnz=K.nnz(), shape=K.spmv_shape(), copy_doubles=n_copy,
-                          spmv_avg_ms=out)), flush=True)
+                          copy_gbs=copy_gbs, spmv_avg_ms=out)), flush=True)
     return 0
 
 

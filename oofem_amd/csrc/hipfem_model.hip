@@ -1020,7 +1020,7 @@ struct hfem_model {
     float *d_ebuf = nullptr;                     // element forces of the gather form (24 per element)
     bool tdesc_used = false;     // false: every point elastic, D from the material table
     bool has_mises = false;
-    bool use_gather = true;      // element operator: gather form (else colored scatter)
+    bool use_gather = false;     // element operator: gather form, else colored scatter (measured faster)
     std::vector<hipcsr_mtx *> linked;
     // MisesMat surrogate (hfem_model_set_surrogate)
     hfem_mlp *sur = nullptr;

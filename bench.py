@@ -401,12 +401,12 @@ def main_rve(args):
     log(f"[bench] {len(specs)} RVEs in {len(B.chunks)} chunks, setup {B.t_setup:.1f} s")
     B.set_options(tol=args.rve_tol)
     t = time.perf_counter()
-    tuned = B.autotune(sample=args.rve_tune_sample)
+    tuned = B.autotune(sample=args.rve_tune_sample, log=log)
     t_tune = time.perf_counter() - t
     log(f"[bench] autotune ({t_tune:.1f} s): {tuned}")
     for i in range(args.warmup):
-        B.run()
-    runs = [B.run() for _ in range(args.steps)]
+        B.run(log=log)
+    runs = [B.run(log=log) for _ in range(args.steps)]
     t_solve = sum(r["t_solve"] for r in runs) / len(runs)
     out = runs[-1]
     nsolve = 6 * len(specs)

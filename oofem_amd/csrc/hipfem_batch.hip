@@ -421,7 +421,7 @@ extern "C" int hfem_batch_set_shape(hfem_batch *bt, int shape, int grid)
 // updates and the final max relative true residual (negative when a block
 // did not converge within maxit)
 static int batch_solve(hfem_batch *bt, const double *db, double *dx, int *iters, int *updates, double *resid,
-                       int nb_limit)
+                       int nb_limit, int qfirst = 0)
 {
     hipcsr_mtx *A = bt->A;
     hipStream_t st = bt->ctx->stream;
@@ -435,14 +435,17 @@ static int batch_solve(hfem_batch *bt, const double *db, double *dx, int *iters,
     HIPFEM_CHECK(hipMemcpyAsync(&zr, bt->d_zero, sizeof(int), hipMemcpyDeviceToHost, st));
     HIPFEM_CHECK(hipStreamSynchronize(st));
     HIPFEM_REQUIRE(zr == big, HIPFEM_ENUMERIC, "batch: zero diagonal in equation %d", zr + 1);
-    HIPFEM_CHECK(hipMemsetAsync(bt->d_ctl, 0, sizeof(BatchCtl), st));
+    BatchCtl c0;
+    memset(&c0, 0, sizeof c0);
+    c0.next = qfirst;
+    HIPFEM_CHECK(hipMemcpyAsync(bt->d_ctl, &c0, sizeof(BatchCtl), hipMemcpyHostToDevice, st));
     const BatchShape &sh = kShapes[bt->shape];
-    const int nb = nb_limit > 0 ? std::min(nb_limit, bt->nblocks) : bt->nblocks;
+    const int nb = nb_limit > 0 ? std::min(qfirst + nb_limit, bt->nblocks) : bt->nblocks;
     int dev = 0, ncu = 0;
     HIPFEM_CHECK(hipGetDevice(&dev));
     HIPFEM_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     const int per_cu = std::max(1, 2048 / sh.bs);
-    const int grid = bt->grid > 0 ? bt->grid : std::max(1, std::min(nb, ncu * per_cu));
+    const int grid = bt->grid > 0 ? bt->grid : std::max(1, std::min(nb - qfirst, ncu * per_cu));
     hipLaunchKernelGGL(sh.fn, dim3(grid), dim3(sh.bs), 0, st, nb, bt->d_order, bt->d_rows, A->d_rowptr, A->d_colind,
                        A->d_val, A->d_val32, bt->d_dinv, db, dx, bt->d_r, bt->d_z, bt->d_p, bt->d_q, bt->d_r64, bt->tol,
                        bt->delta, bt->maxit, bt->d_ctl, bt->d_iters, bt->d_res, bt->d_upd);
@@ -463,8 +466,10 @@ extern "C" int hfem_batch_solve_dev(hfem_batch *bt, const double *db, double *dx
     return batch_solve(bt, db, dx, iters, updates, resid, 0);
 }
 
-// time every shape on the first `sample` blocks of the queue (the largest)
-// and keep the fastest; ms[NSHAPES] receives the timings
+// time every shape on `sample` blocks of median size (a contiguous stretch
+// of the size-ordered queue around its middle), each run capped at
+// TUNE_ITERS iterations, and keep the fastest; ms[NSHAPES] receives the
+// timings
 extern "C" int hfem_batch_autotune(hfem_batch *bt, const double *db, double *dx, int sample, double *ms, int *best)
 {
     HIPFEM_REQUIRE(bt && db && dx && sample > 0, HIPFEM_EINVAL, "autotune: bad arguments");
@@ -475,14 +480,20 @@ extern "C" int hfem_batch_autotune(hfem_batch *bt, const double *db, double *dx,
     const int keep_shape = bt->shape, keep_grid = bt->grid;
     int bi = keep_shape;
     float bms = 1e30f;
+    const int ns = std::min(sample, bt->nblocks);
+    const int q0 = std::max(0, bt->nblocks / 2 - ns / 2);
+    constexpr int TUNE_ITERS = 40;
+    const int keep_maxit = bt->maxit;
+    bt->maxit = std::min(keep_maxit, TUNE_ITERS);
     for (int s = 0; s < NSHAPES; s++) {
         bt->shape = s;
         bt->grid = 0;
-        int rc = batch_solve(bt, db, dx, nullptr, nullptr, nullptr, sample);   // warm
-        if (rc) return rc;
         HIPFEM_CHECK(hipEventRecord(e0, bt->ctx->stream));
-        rc = batch_solve(bt, db, dx, nullptr, nullptr, nullptr, sample);
-        if (rc) return rc;
+        int rc = batch_solve(bt, db, dx, nullptr, nullptr, nullptr, ns, q0);
+        if (rc) {
+            bt->maxit = keep_maxit;
+            return rc;
+        }
         HIPFEM_CHECK(hipEventRecord(e1, bt->ctx->stream));
         HIPFEM_CHECK(hipEventSynchronize(e1));
         float t = 0.f;
@@ -494,6 +505,7 @@ extern "C" int hfem_batch_autotune(hfem_batch *bt, const double *db, double *dx,
         }
     }
     (void)keep_grid;
+    bt->maxit = keep_maxit;
     bt->shape = bi;
     bt->grid = 0;
     if (best) *best = bi;

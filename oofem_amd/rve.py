@@ -122,7 +122,7 @@ class RVEBatch:
     """All RVEs of a batch on one GPU, grouped by element type into chunks of
     at most nnz_budget stiffness entries."""
 
-    def __init__(self, ctx, specs, nnz_budget=1.2e9, log=None):
+    def __init__(self, ctx, specs, nnz_budget=4e8, log=None):
         self.ctx = ctx
         self.specs = specs
         self.chunks = []
@@ -135,6 +135,9 @@ class RVEBatch:
                     n = specs[i][0]
                     est = 3.0 * (n + 1) ** 3 * (4 if et == meshgen.QSPACE else 1) * NNZ_ROW[et]
                 if cur and (i is None or cur_nnz + est > nnz_budget):
+                    if log:
+                        log(f"[rve] building {len(cur)} {'QSpace' if et == meshgen.QSPACE else 'LSpace'} RVEs "
+                            f"(about {cur_nnz:.2e} entries)")
                     decks = [meshgen.rve(*specs[j]) for j in cur]
                     self.chunks.append(_Chunk(ctx, decks, cur))
                     if log:
@@ -150,10 +153,12 @@ class RVEBatch:
         for c in self.chunks:
             _check(lib().hfem_batch_set_options(c.h, C.c_double(tol), C.c_double(delta), C.c_int(maxit)), "options")
 
-    def autotune(self, sample=8):
+    def autotune(self, sample=8, log=None):
         nshape = lib().hfem_batch_nshapes()
         out = []
         for c in self.chunks:
+            if log:
+                log(f"[rve] autotune on {sample} median blocks of a chunk of {len(c.ids)}")
             c.rhs()
             ms = np.zeros(nshape)
             best = C.c_int(0)
@@ -165,7 +170,7 @@ class RVEBatch:
                             ms=[round(float(v), 3) for v in ms]))
         return out
 
-    def run(self):
+    def run(self, log=None):
         """Six unit-strain solves per RVE; returns per-RVE 6x6 stiffness,
         iterations, reliable updates, residuals and the solve time (the
         persistent launches only; right-hand sides and reactions apart)."""
@@ -181,6 +186,8 @@ class RVEBatch:
             t = time.perf_counter()
             i_, u_, r_ = c.solve()
             t_solve += time.perf_counter() - t
+            if log:
+                log(f"[rve] chunk of {len(c.ids)} solved in {time.perf_counter() - t:.2f} s, iterations max {i_.max()}")
             ids = np.asarray(c.ids)
             it[ids], upd[ids], res[ids] = i_, u_, r_
             C6[ids] = c.homogenized()

@@ -399,7 +399,7 @@ def main_rve(args):
     ctx = Context(0)
     B = RVEBatch(ctx, specs, log=log)
     log(f"[bench] {len(specs)} RVEs in {len(B.chunks)} chunks, setup {B.t_setup:.1f} s")
-    B.set_options(tol=args.rve_tol)
+    B.set_options(tol=args.rve_tol, split=args.rve_split)
     t = time.perf_counter()
     tuned = B.autotune(sample=args.rve_tune_sample, log=log)
     t_tune = time.perf_counter() - t
@@ -408,6 +408,8 @@ def main_rve(args):
         B.run(log=log)
     runs = [B.run(log=log) for _ in range(args.steps)]
     t_solve = sum(r["t_solve"] for r in runs) / len(runs)
+    t_rhs = sum(r["t_rhs"] for r in runs) / len(runs)
+    t_hom = sum(r["t_homogenize"] for r in runs) / len(runs)
     out = runs[-1]
     nsolve = 6 * len(specs)
     # oracle FP64 on the smallest RVEs of the batch
@@ -435,6 +437,8 @@ def main_rve(args):
         "config": {"workload": f"rve_batch_{len(specs)}", "rves": len(specs), "qspace_rves": nq,
                    "free_dofs_total": dofs, "solves_per_step": nsolve, "tol": args.rve_tol,
                    "chunks": len(B.chunks), "setup_s": B.t_setup, "autotune": tuned, "autotune_s": t_tune,
+                   "split": args.rve_split, "rhs_s": t_rhs, "reactions_s": t_hom,
+                   "solves_per_s_with_rhs_and_reactions": nsolve / (t_solve + t_rhs + t_hom),
                    "iters_max": int(out["iters"].max()), "iters_mean": float(out["iters"].mean()),
                    "reliable_updates_mean": float(out["updates"].mean()),
                    "resid_max": float(np.abs(out["resid"]).max()), "all_converged": bool((out["resid"] > 0).all()),
@@ -457,6 +461,8 @@ def main():
     ap.add_argument("--rve-count", type=int, default=512)
     ap.add_argument("--rve-tol", type=float, default=1e-8)
     ap.add_argument("--rve-tune-sample", type=int, default=16)
+    ap.add_argument("--rve-split", type=int, default=-1,
+                    help="stored entries above which an RVE goes to the bulk phase (-1 automatic, 0 all)")
     ap.add_argument("--rve-check", type=int, default=8)
     ap.add_argument("--c3-warm", type=int, default=1, help="untimed load steps before the timed ones (cfg3)")
     ap.add_argument("--c3-steps", type=int, default=1, help="timed load steps (cfg3)")

@@ -264,8 +264,11 @@ int hfem_model_ncolors(const hfem_model *md);
  * A is block diagonal: block b is rows [rows[b], rows[b+1]) and no entry
  * leaves its block (checked).  Six right-hand sides per block ([row][6]
  * interleaved, FP64 device arrays).  Jacobi PCG on FP32 values/vectors with
- * FP64 solution and dots and FP64 reliable updates (hippcg precision 2),
- * one workgroup per block, blocks taken largest first. */
+ * FP64 solution and dots and FP64 reliable updates (hippcg precision 2).
+ * Blocks with more stored entries than the split are solved first, all
+ * together, by a bulk-synchronous phase that spreads each block's rows
+ * over many workgroups; the rest by one persistent launch, one workgroup
+ * per block, blocks taken largest first. */
 typedef struct hfem_batch hfem_batch;
 int  hfem_batch_create(hipcsr_mtx *A, int nblocks, const int32_t *rows, hfem_batch **out);
 void hfem_batch_destroy(hfem_batch *bt);
@@ -277,8 +280,21 @@ int  hfem_batch_set_shape(hfem_batch *bt, int shape, int grid);
 /* per block: iterations, reliable updates, final max relative true
  * residual of its six columns (negative: not converged within maxit) */
 int  hfem_batch_solve_dev(hfem_batch *bt, const double *db, double *dx, int *iters, int *updates, double *resid);
-/* time every shape on the `sample` largest blocks, keep the fastest */
+/* time every persistent-kernel shape on `sample` median blocks at or below
+ * the split (40 iterations each), keep the fastest */
 int  hfem_batch_autotune(hfem_batch *bt, const double *db, double *dx, int sample, double *ms, int *best);
+/* bulk-phase shapes (lanes per row of its SpMV) and the one to use */
+int  hfem_batch_nbulk_shapes(void);
+int  hfem_batch_bulk_shape_info(int shape, int *lanes_per_row);
+int  hfem_batch_set_bulk_shape(hfem_batch *bt, int shape);
+/* split: blocks with more stored entries go to the bulk phase; -1 (the
+ * default) = the batch's entries divided by the workgroups the persistent
+ * kernel keeps resident; 0 = every block */
+int  hfem_batch_set_split(hfem_batch *bt, long long split);
+int  hfem_batch_partition_info(hfem_batch *bt, int *nbulk, long long *split_used);
+/* time every bulk shape on the blocks above the split (40 iterations
+ * each), keep the fastest */
+int  hfem_batch_autotune_bulk(hfem_batch *bt, const double *db, double *dx, double *ms, int *best);
 
 /* ------------------------------------------------------ MisesMat surrogate */
 /* A learned stand-in for MisesMat's return mapping and consistent tangent

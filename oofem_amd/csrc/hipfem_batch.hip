@@ -18,7 +18,18 @@
 //
 // Shape: LPR lanes per row (8, 16, 32 or 64) and the workgroup size (256,
 // 512 or 1024) are template parameters; hfem_batch_autotune times them on
-// the largest blocks and keeps the fastest.
+// median blocks and keeps the fastest.
+//
+// Large blocks: one workgroup streams a block at a fraction of one CU's
+// bandwidth, so a block much larger than the batch's mean share would set
+// the whole launch's length.  Blocks above a split size (stored entries)
+// are therefore solved first, together, in a bulk-synchronous phase: every
+// launch covers all their rows in tiles of BULK_TILE rows (one workgroup
+// each, the whole GPU), per-tile partial dots are summed per block by a
+// reduction launch, and the same per-column scalars, reliable updates and
+// stopping rule as the persistent kernel live in a BulkBlock record per
+// block.  Finished blocks drop out of the tile list when the host next looks
+// (every BULK_CHECK iterations); until then their tiles return at once.
 
 #include <algorithm>
 #include <cmath>
@@ -266,6 +277,308 @@ __global__ __launch_bounds__(BS) void k_batch_pcg(int nblocks, const int *__rest
     }
 }
 
+// ------------------------------------------------------------ bulk phase
+constexpr int BULK_TILE = 256;   // rows per tile = threads per workgroup
+constexpr int BULK_CHECK = 16;   // iterations between host looks at the done flags
+constexpr int NPART = 2 * NRHS;  // partial sums per tile
+
+struct BulkBlock {
+    double rho[NRHS], normb[NRHS], rmax[NRHS], res[NRHS];
+    float alpha[NRHS], beta[NRHS];
+    int conv;      // bit k: column k meets the tolerance
+    int it, nupd;
+    int refresh;   // 1: this iteration replaces the residuals by b - A x
+    int done;      // 1: every column converged, or maxit reached
+    int blk;       // block number
+};
+
+struct BulkArgs {
+    const int4 *tiles;       // {first row, end row, bulk index, 0}
+    const int *act;          // bulk indices of the blocks in the tile list
+    const int *act_t0;       // their tile ranges [act_t0[a], act_t0[a + 1])
+    BulkBlock *st;
+    const int32_t *rowptr, *col;
+    const double *val64;
+    const float *val32, *dinv;
+    const double *b;
+    double *x;
+    float *r, *z, *p, *q;
+    double *part;            // [tile][NPART]
+    double tol, delta;
+    int maxit;
+    int *iters, *updates;
+    double *resid;
+};
+
+// x = 0, r = b, z = p = D^-1 r; partials r.z and b.b
+__global__ __launch_bounds__(BULK_TILE) void k_bulk_init(BulkArgs A)
+{
+    __shared__ double sh[NPART * (BULK_TILE / 64)];
+    const int4 t = A.tiles[blockIdx.x];
+    double v[NPART];
+#pragma unroll
+    for (int k = 0; k < NPART; k++) v[k] = 0.0;
+    const int i = t.x + threadIdx.x;
+    if (i < t.y) {
+        const float d = A.dinv[i];
+#pragma unroll
+        for (int k = 0; k < NRHS; k++) {
+            const size_t o = (size_t)i * NRHS + k;
+            const double bi = A.b[o];
+            A.x[o] = 0.0;
+            const float ri = (float)bi;
+            A.r[o] = ri;
+            const float zi = d * ri;
+            A.z[o] = zi;
+            A.p[o] = zi;
+            v[k] = (double)ri * (double)zi;
+            v[NRHS + k] = bi * bi;
+        }
+    }
+    block_sums<BULK_TILE, NPART>(v, sh, A.part + (size_t)blockIdx.x * NPART);
+}
+
+// q = A p (FP32) for the tile's rows, LPR lanes per row; partials p.q
+template <int LPR>
+__global__ __launch_bounds__(BULK_TILE) void k_bulk_spmv(BulkArgs A)
+{
+    __shared__ double sh[NRHS * (BULK_TILE / 64)];
+    const int4 t = A.tiles[blockIdx.x];
+    if (A.st[t.z].done) return;
+    double v[NRHS];
+#pragma unroll
+    for (int k = 0; k < NRHS; k++) v[k] = 0.0;
+    const int lane = threadIdx.x % LPR;
+    for (int row = t.x + threadIdx.x / LPR; row < t.y; row += BULK_TILE / LPR) {
+        const int s = A.rowptr[row], e = A.rowptr[row + 1];
+        float acc[NRHS];
+#pragma unroll
+        for (int k = 0; k < NRHS; k++) acc[k] = 0.f;
+        for (int i = s + lane; i < e; i += LPR) {
+            const float a = A.val32[i];
+            const float2 *xc = reinterpret_cast<const float2 *>(A.p + (size_t)A.col[i] * NRHS);
+            const float2 x0 = xc[0], x1 = xc[1], x2 = xc[2];
+            acc[0] = fmaf(a, x0.x, acc[0]);
+            acc[1] = fmaf(a, x0.y, acc[1]);
+            acc[2] = fmaf(a, x1.x, acc[2]);
+            acc[3] = fmaf(a, x1.y, acc[3]);
+            acc[4] = fmaf(a, x2.x, acc[4]);
+            acc[5] = fmaf(a, x2.y, acc[5]);
+        }
+#pragma unroll
+        for (int k = 0; k < NRHS; k++) {
+            float w = acc[k];
+#pragma unroll
+            for (int off = LPR / 2; off > 0; off >>= 1) w += __shfl_xor(w, off, 64);
+            acc[k] = w;
+        }
+        if (lane == 0) {
+#pragma unroll
+            for (int k = 0; k < NRHS; k++) {
+                const size_t o = (size_t)row * NRHS + k;
+                A.q[o] = acc[k];
+                v[k] += (double)A.p[o] * (double)acc[k];
+            }
+        }
+    }
+    block_sums<BULK_TILE, NRHS>(v, sh, A.part + (size_t)blockIdx.x * NPART);
+}
+
+// x += alpha p (FP64), r -= alpha q, z = D^-1 r; partials r.z and r.r
+__global__ __launch_bounds__(BULK_TILE) void k_bulk_update(BulkArgs A)
+{
+    __shared__ double sh[NPART * (BULK_TILE / 64)];
+    const int4 t = A.tiles[blockIdx.x];
+    const BulkBlock &S = A.st[t.z];
+    if (S.done) return;
+    double v[NPART];
+#pragma unroll
+    for (int k = 0; k < NPART; k++) v[k] = 0.0;
+    const int i = t.x + threadIdx.x;
+    if (i < t.y) {
+        const float d = A.dinv[i];
+#pragma unroll
+        for (int k = 0; k < NRHS; k++) {
+            const size_t o = (size_t)i * NRHS + k;
+            const float al = S.alpha[k];
+            A.x[o] += (double)al * (double)A.p[o];
+            const float ri = fmaf(-al, A.q[o], A.r[o]);
+            A.r[o] = ri;
+            const float zi = d * ri;
+            A.z[o] = zi;
+            v[k] = (double)ri * (double)zi;
+            v[NRHS + k] = (double)ri * (double)ri;
+        }
+    }
+    block_sums<BULK_TILE, NPART>(v, sh, A.part + (size_t)blockIdx.x * NPART);
+}
+
+// reliable update for the tiles of blocks that asked for one:
+// r = b - A x (FP64 values and solution), z = D^-1 r; partials r.z and |r|^2
+template <int LPR>
+__global__ __launch_bounds__(BULK_TILE) void k_bulk_refresh(BulkArgs A)
+{
+    __shared__ double sh[NPART * (BULK_TILE / 64)];
+    const int4 t = A.tiles[blockIdx.x];
+    const BulkBlock &S = A.st[t.z];
+    if (S.done || !S.refresh) return;
+    double v[NPART];
+#pragma unroll
+    for (int k = 0; k < NPART; k++) v[k] = 0.0;
+    const int lane = threadIdx.x % LPR;
+    for (int row = t.x + threadIdx.x / LPR; row < t.y; row += BULK_TILE / LPR) {
+        const int s = A.rowptr[row], e = A.rowptr[row + 1];
+        double acc[NRHS];
+#pragma unroll
+        for (int k = 0; k < NRHS; k++) acc[k] = 0.0;
+        for (int i = s + lane; i < e; i += LPR) {
+            const double a = A.val64[i];
+            const double2 *xc = reinterpret_cast<const double2 *>(A.x + (size_t)A.col[i] * NRHS);
+            const double2 x0 = xc[0], x1 = xc[1], x2 = xc[2];
+            acc[0] = fma(a, x0.x, acc[0]);
+            acc[1] = fma(a, x0.y, acc[1]);
+            acc[2] = fma(a, x1.x, acc[2]);
+            acc[3] = fma(a, x1.y, acc[3]);
+            acc[4] = fma(a, x2.x, acc[4]);
+            acc[5] = fma(a, x2.y, acc[5]);
+        }
+#pragma unroll
+        for (int k = 0; k < NRHS; k++) {
+            double w = acc[k];
+#pragma unroll
+            for (int off = LPR / 2; off > 0; off >>= 1) w += __shfl_xor(w, off, 64);
+            acc[k] = w;
+        }
+        if (lane == 0) {
+            const float d = A.dinv[row];
+#pragma unroll
+            for (int k = 0; k < NRHS; k++) {
+                const size_t o = (size_t)row * NRHS + k;
+                const double rt = A.b[o] - acc[k];
+                const float ri = (float)rt;
+                A.r[o] = ri;
+                const float zi = d * ri;
+                A.z[o] = zi;
+                v[k] += (double)ri * (double)zi;
+                v[NRHS + k] += rt * rt;
+            }
+        }
+    }
+    block_sums<BULK_TILE, NPART>(v, sh, A.part + (size_t)blockIdx.x * NPART);
+}
+
+// p = z + beta p
+__global__ __launch_bounds__(BULK_TILE) void k_bulk_pupdate(BulkArgs A)
+{
+    const int4 t = A.tiles[blockIdx.x];
+    const BulkBlock &S = A.st[t.z];
+    if (S.done) return;
+    const int i = t.x + threadIdx.x;
+    if (i < t.y) {
+#pragma unroll
+        for (int k = 0; k < NRHS; k++) {
+            const size_t o = (size_t)i * NRHS + k;
+            A.p[o] = fmaf(S.beta[k], A.p[o], A.z[o]);
+        }
+    }
+}
+
+enum BulkPhase { BP_INIT, BP_ALPHA, BP_BETA, BP_REFRESH };
+
+__device__ void bulk_finish(const BulkArgs &A, BulkBlock &S, bool all)
+{
+    double mx = 0.0;
+    for (int k = 0; k < NRHS; k++) mx = fmax(mx, S.res[k]);
+    A.iters[S.blk] = S.it;
+    A.updates[S.blk] = S.nupd;
+    A.resid[S.blk] = all ? mx : -mx;
+    S.done = 1;
+}
+
+// one workgroup per block of the tile list: sum the block's tile partials
+// and advance its scalars (the same rules as k_batch_pcg)
+template <int PHASE>
+__global__ __launch_bounds__(256) void k_bulk_reduce(BulkArgs A)
+{
+    __shared__ double sh[NPART * 4];
+    __shared__ double red[NPART];
+    BulkBlock &S = A.st[A.act[blockIdx.x]];
+    if (S.done) return;
+    if (PHASE == BP_REFRESH && !S.refresh) return;
+    constexpr int NV = PHASE == BP_ALPHA ? NRHS : NPART;
+    double v[NV];
+#pragma unroll
+    for (int k = 0; k < NV; k++) v[k] = 0.0;
+    for (int ti = A.act_t0[blockIdx.x] + threadIdx.x; ti < A.act_t0[blockIdx.x + 1]; ti += 256)
+#pragma unroll
+        for (int k = 0; k < NV; k++) v[k] += A.part[(size_t)ti * NPART + k];
+    block_sums<256, NV>(v, sh, red);
+    if (threadIdx.x != 0) return;
+    if (PHASE == BP_INIT) {
+        bool all = true;
+        S.conv = 0;
+        for (int k = 0; k < NRHS; k++) {
+            S.rho[k] = red[k];
+            const double nb = sqrt(red[NRHS + k]);
+            S.normb[k] = nb == 0.0 ? 1.0 : nb;
+            S.res[k] = nb / S.normb[k];
+            S.rmax[k] = S.res[k];
+            if (S.res[k] <= A.tol) S.conv |= 1 << k;
+            all = all && (S.conv >> k & 1);
+        }
+        S.it = 0;
+        S.nupd = 0;
+        S.refresh = 0;
+        if (all || A.maxit <= 0) bulk_finish(A, S, all);
+    } else if (PHASE == BP_ALPHA) {
+        S.it++;
+        for (int k = 0; k < NRHS; k++)
+            S.alpha[k] = (!(S.conv >> k & 1) && red[k] > 0.0) ? (float)(S.rho[k] / red[k]) : 0.f;
+    } else if (PHASE == BP_BETA) {
+        bool refresh = false;
+        for (int k = 0; k < NRHS; k++) {
+            S.res[k] = sqrt(red[NRHS + k]) / S.normb[k];
+            if (!(S.conv >> k & 1) && (S.res[k] <= A.tol || S.res[k] < A.delta * S.rmax[k])) refresh = true;
+        }
+        S.refresh = refresh;
+        if (!refresh) {
+            for (int k = 0; k < NRHS; k++) {
+                const bool c = S.conv >> k & 1;
+                S.beta[k] = c ? 0.f : (float)(red[k] / S.rho[k]);
+                if (!c) S.rho[k] = red[k];
+            }
+            if (S.it >= A.maxit) bulk_finish(A, S, false);
+        }
+    } else {
+        S.nupd++;
+        bool all = true;
+        S.conv = 0;
+        for (int k = 0; k < NRHS; k++) {
+            S.res[k] = sqrt(red[NRHS + k]) / S.normb[k];
+            const bool c = S.res[k] <= A.tol;
+            if (c) S.conv |= 1 << k;
+            all = all && c;
+            S.beta[k] = c ? 0.f : (float)(red[k] / S.rho[k]);
+            S.rho[k] = red[k];
+            S.rmax[k] = S.res[k];
+        }
+        S.refresh = 0;
+        if (all || S.it >= A.maxit) bulk_finish(A, S, all);
+    }
+}
+
+typedef void (*bulk_kernel_t)(BulkArgs);
+struct BulkShape {
+    int lpr;
+    bulk_kernel_t spmv, refresh;
+};
+static const BulkShape kBulkShapes[] = {
+    {4, k_bulk_spmv<4>, k_bulk_refresh<4>},
+    {8, k_bulk_spmv<8>, k_bulk_refresh<8>},
+    {16, k_bulk_spmv<16>, k_bulk_refresh<16>},
+};
+constexpr int NBULK = sizeof(kBulkShapes) / sizeof(kBulkShapes[0]);
+
 __global__ void k_batch_dinv(int n, const int32_t *__restrict__ diag, const double *__restrict__ val,
                              float *__restrict__ dinv, int *__restrict__ zero_row)
 {
@@ -301,8 +614,10 @@ struct hfem_batch {
     hipcsr_mtx *A = nullptr;
     int nblocks = 0, n = 0;
     std::vector<int32_t> h_rows;
+    std::vector<long long> h_nnz;   // stored entries per block
+    std::vector<int> h_sorted;      // blocks by descending size
     int32_t *d_rows = nullptr;
-    int *d_order = nullptr;
+    int *d_order = nullptr;         // queue of the persistent kernel (blocks at or below the split)
     float *d_dinv = nullptr;
     double *d_x = nullptr, *d_r64 = nullptr;
     float *d_r = nullptr, *d_z = nullptr, *d_p = nullptr, *d_q = nullptr;
@@ -313,12 +628,30 @@ struct hfem_batch {
     int grid = 0;               // persistent workgroups (0: one per block up to the device's capacity)
     double tol = 1e-8, delta = 0.1;
     int maxit = 20000;
+    // partition between the bulk phase and the persistent kernel
+    long long split = -1;       // stored entries above which a block goes to the bulk phase (-1: automatic)
+    long long split_used = -1;
+    int part_shape = -1;        // shape the automatic split was computed for
+    bool part_dirty = true;
+    std::vector<int> h_bulk;    // bulk blocks (descending size)
+    int nsmall = 0;
+    // bulk phase
+    int bulk_shape = 1;         // index into kBulkShapes
+    int tiles_cap = 0;
+    BulkBlock *d_bst = nullptr;
+    int4 *d_tiles = nullptr;
+    int *d_act = nullptr, *d_act_t0 = nullptr;
+    double *d_part = nullptr;
+    std::vector<BulkBlock> h_bst;
+    std::vector<int4> h_tiles;
+    std::vector<int> h_act, h_act_t0;
 };
 
 static int batch_free(hfem_batch *bt)
 {
     void *ps[] = {bt->d_rows, bt->d_order, bt->d_dinv, bt->d_x, bt->d_r64, bt->d_r, bt->d_z, bt->d_p, bt->d_q,
-                  bt->d_ctl, bt->d_iters, bt->d_upd, bt->d_zero, bt->d_res};
+                  bt->d_ctl, bt->d_iters, bt->d_upd, bt->d_zero, bt->d_res, bt->d_bst, bt->d_tiles, bt->d_act,
+                  bt->d_act_t0, bt->d_part};
     for (void *p : ps)
         if (p) HIPFEM_CHECK(hipFree(p));
     return 0;
@@ -344,13 +677,16 @@ extern "C" int hfem_batch_create(hipcsr_mtx *A, int nblocks, const int32_t *rows
     bt->nblocks = nblocks;
     bt->n = A->n;
     bt->h_rows.assign(rows, rows + nblocks + 1);
+    bt->h_nnz.resize(nblocks);
+    for (int b = 0; b < nblocks; b++) bt->h_nnz[b] = (long long)A->h_rowptr[rows[b + 1]] - A->h_rowptr[rows[b]];
     // descending size: the largest blocks are taken first
-    std::vector<int> order(nblocks);
-    for (int b = 0; b < nblocks; b++) order[b] = b;
-    std::stable_sort(order.begin(), order.end(), [&](int a, int c) {
-        return A->h_rowptr[rows[a + 1]] - A->h_rowptr[rows[a]] > A->h_rowptr[rows[c + 1]] - A->h_rowptr[rows[c]];
-    });
+    bt->h_sorted.resize(nblocks);
+    for (int b = 0; b < nblocks; b++) bt->h_sorted[b] = b;
+    std::stable_sort(bt->h_sorted.begin(), bt->h_sorted.end(),
+                     [&](int a, int c) { return bt->h_nnz[a] > bt->h_nnz[c]; });
     const size_t nv = (size_t)std::max(1, bt->n) * NRHS;
+    // tiles of BULK_TILE rows never straddle blocks: at most n / BULK_TILE + nblocks
+    bt->tiles_cap = bt->n / BULK_TILE + nblocks + 1;
     auto fail = [&](int rc) { batch_free(bt); delete bt; return rc; };
     if (hipMalloc(&bt->d_rows, sizeof(int32_t) * (nblocks + 1)) != hipSuccess ||
         hipMalloc(&bt->d_order, sizeof(int) * nblocks) != hipSuccess ||
@@ -360,13 +696,17 @@ extern "C" int hfem_batch_create(hipcsr_mtx *A, int nblocks, const int32_t *rows
         hipMalloc(&bt->d_p, sizeof(float) * nv) != hipSuccess || hipMalloc(&bt->d_q, sizeof(float) * nv) != hipSuccess ||
         hipMalloc(&bt->d_ctl, sizeof(BatchCtl)) != hipSuccess || hipMalloc(&bt->d_iters, sizeof(int) * nblocks) != hipSuccess ||
         hipMalloc(&bt->d_upd, sizeof(int) * nblocks) != hipSuccess || hipMalloc(&bt->d_zero, sizeof(int)) != hipSuccess ||
-        hipMalloc(&bt->d_res, sizeof(double) * nblocks) != hipSuccess) {
+        hipMalloc(&bt->d_res, sizeof(double) * nblocks) != hipSuccess ||
+        hipMalloc(&bt->d_bst, sizeof(BulkBlock) * nblocks) != hipSuccess ||
+        hipMalloc(&bt->d_tiles, sizeof(int4) * bt->tiles_cap) != hipSuccess ||
+        hipMalloc(&bt->d_act, sizeof(int) * nblocks) != hipSuccess ||
+        hipMalloc(&bt->d_act_t0, sizeof(int) * (nblocks + 1)) != hipSuccess ||
+        hipMalloc(&bt->d_part, sizeof(double) * NPART * bt->tiles_cap) != hipSuccess) {
         set_error("hfem_batch_create: out of device memory");
         return fail(HIPFEM_ENOMEM);
     }
     hipStream_t st = bt->ctx->stream;
     if (hipMemcpyAsync(bt->d_rows, rows, sizeof(int32_t) * (nblocks + 1), hipMemcpyHostToDevice, st) != hipSuccess ||
-        hipMemcpyAsync(bt->d_order, order.data(), sizeof(int) * nblocks, hipMemcpyHostToDevice, st) != hipSuccess ||
         hipStreamSynchronize(st) != hipSuccess) {
         set_error("hfem_batch_create: upload failed");
         return fail(HIPFEM_EHIP);
@@ -416,12 +756,79 @@ extern "C" int hfem_batch_set_shape(hfem_batch *bt, int shape, int grid)
     return 0;
 }
 
-// solve every block for the six right-hand sides b ([row][6], device, FP64);
-// solutions in x ([row][6], device); per block: iterations, reliable
-// updates and the final max relative true residual (negative when a block
-// did not converge within maxit)
-static int batch_solve(hfem_batch *bt, const double *db, double *dx, int *iters, int *updates, double *resid,
-                       int nb_limit, int qfirst = 0)
+extern "C" int hfem_batch_nbulk_shapes(void) { return NBULK; }
+
+extern "C" int hfem_batch_bulk_shape_info(int shape, int *lanes_per_row)
+{
+    HIPFEM_REQUIRE(shape >= 0 && shape < NBULK, HIPFEM_ERANGE, "bulk shape %d out of range", shape);
+    *lanes_per_row = kBulkShapes[shape].lpr;
+    return 0;
+}
+
+extern "C" int hfem_batch_set_bulk_shape(hfem_batch *bt, int shape)
+{
+    HIPFEM_REQUIRE(bt && shape >= 0 && shape < NBULK, HIPFEM_EINVAL, "bulk shape: bad values");
+    bt->bulk_shape = shape;
+    return 0;
+}
+
+extern "C" int hfem_batch_set_split(hfem_batch *bt, long long split)
+{
+    HIPFEM_REQUIRE(bt && split >= -1, HIPFEM_EINVAL, "batch split: bad value");
+    bt->split = split;
+    bt->part_dirty = true;
+    return 0;
+}
+
+static int device_cus(int *ncu)
+{
+    int dev = 0;
+    HIPFEM_CHECK(hipGetDevice(&dev));
+    HIPFEM_CHECK(hipDeviceGetAttribute(ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    return 0;
+}
+
+// blocks above the split go to the bulk phase; the automatic split is an
+// even share of the batch's entries over the workgroups the persistent
+// kernel keeps resident (a block above it would outlast the rest)
+static int batch_partition(hfem_batch *bt)
+{
+    if (!bt->part_dirty && (bt->split >= 0 || bt->part_shape == bt->shape)) return 0;
+    long long thr = bt->split;
+    if (thr < 0) {
+        int ncu = 0;
+        int rc = device_cus(&ncu);
+        if (rc) return rc;
+        long long total = 0;
+        for (long long v : bt->h_nnz) total += v;
+        const long long resident = (long long)ncu * std::max(1, 2048 / kShapes[bt->shape].bs);
+        thr = total / resident;
+    }
+    bt->h_bulk.clear();
+    std::vector<int> small;
+    for (int b : bt->h_sorted) (bt->h_nnz[b] > thr ? bt->h_bulk : small).push_back(b);
+    bt->nsmall = (int)small.size();
+    if (!small.empty())
+        HIPFEM_CHECK(hipMemcpy(bt->d_order, small.data(), sizeof(int) * small.size(), hipMemcpyHostToDevice));
+    bt->split_used = thr;
+    bt->part_shape = bt->shape;
+    bt->part_dirty = false;
+    return 0;
+}
+
+extern "C" int hfem_batch_partition_info(hfem_batch *bt, int *nbulk, long long *split_used)
+{
+    HIPFEM_REQUIRE(bt, HIPFEM_EINVAL, "batch partition: null handle");
+    DeviceGuard g(bt->ctx->device);
+    int rc = batch_partition(bt);
+    if (rc) return rc;
+    if (nbulk) *nbulk = (int)bt->h_bulk.size();
+    if (split_used) *split_used = bt->split_used;
+    return 0;
+}
+
+// dinv, FP32 values, partition; checks for zero diagonals
+static int batch_prepare(hfem_batch *bt)
 {
     hipcsr_mtx *A = bt->A;
     hipStream_t st = bt->ctx->stream;
@@ -435,21 +842,149 @@ static int batch_solve(hfem_batch *bt, const double *db, double *dx, int *iters,
     HIPFEM_CHECK(hipMemcpyAsync(&zr, bt->d_zero, sizeof(int), hipMemcpyDeviceToHost, st));
     HIPFEM_CHECK(hipStreamSynchronize(st));
     HIPFEM_REQUIRE(zr == big, HIPFEM_ENUMERIC, "batch: zero diagonal in equation %d", zr + 1);
+    return batch_partition(bt);
+}
+
+// tile list for the bulk blocks whose index is in `act` (the stream must be
+// idle: the host arrays are reused)
+static int bulk_tiles(hfem_batch *bt, const std::vector<int> &act)
+{
+    bt->h_tiles.clear();
+    bt->h_act = act;
+    bt->h_act_t0.clear();
+    for (int a : act) {
+        const int blk = bt->h_bulk[a];
+        bt->h_act_t0.push_back((int)bt->h_tiles.size());
+        for (int r = bt->h_rows[blk]; r < bt->h_rows[blk + 1]; r += BULK_TILE)
+            bt->h_tiles.push_back(make_int4(r, std::min(r + BULK_TILE, bt->h_rows[blk + 1]), a, 0));
+    }
+    bt->h_act_t0.push_back((int)bt->h_tiles.size());
+    HIPFEM_REQUIRE((int)bt->h_tiles.size() <= bt->tiles_cap, HIPFEM_EINVAL, "bulk tiles exceed their capacity");
+    hipStream_t st = bt->ctx->stream;
+    HIPFEM_CHECK(hipMemcpyAsync(bt->d_tiles, bt->h_tiles.data(), sizeof(int4) * bt->h_tiles.size(),
+                                hipMemcpyHostToDevice, st));
+    HIPFEM_CHECK(hipMemcpyAsync(bt->d_act, bt->h_act.data(), sizeof(int) * act.size(), hipMemcpyHostToDevice, st));
+    HIPFEM_CHECK(hipMemcpyAsync(bt->d_act_t0, bt->h_act_t0.data(), sizeof(int) * bt->h_act_t0.size(),
+                                hipMemcpyHostToDevice, st));
+    return 0;
+}
+
+// the bulk phase: every bulk block to convergence (or maxit)
+static int bulk_solve(hfem_batch *bt, const double *db, double *dx, int maxit)
+{
+    const int nbulk = (int)bt->h_bulk.size();
+    if (nbulk == 0) return 0;
+    hipcsr_mtx *M = bt->A;
+    hipStream_t st = bt->ctx->stream;
+    bt->h_bst.assign(nbulk, BulkBlock{});
+    for (int a = 0; a < nbulk; a++) bt->h_bst[a].blk = bt->h_bulk[a];
+    HIPFEM_CHECK(hipMemcpyAsync(bt->d_bst, bt->h_bst.data(), sizeof(BulkBlock) * nbulk, hipMemcpyHostToDevice, st));
+    std::vector<int> act(nbulk);
+    for (int a = 0; a < nbulk; a++) act[a] = a;
+    int rc = bulk_tiles(bt, act);
+    if (rc) return rc;
+    BulkArgs A;
+    A.tiles = bt->d_tiles;
+    A.act = bt->d_act;
+    A.act_t0 = bt->d_act_t0;
+    A.st = bt->d_bst;
+    A.rowptr = M->d_rowptr;
+    A.col = M->d_colind;
+    A.val64 = M->d_val;
+    A.val32 = M->d_val32;
+    A.dinv = bt->d_dinv;
+    A.b = db;
+    A.x = dx;
+    A.r = bt->d_r;
+    A.z = bt->d_z;
+    A.p = bt->d_p;
+    A.q = bt->d_q;
+    A.part = bt->d_part;
+    A.tol = bt->tol;
+    A.delta = bt->delta;
+    A.maxit = maxit;
+    A.iters = bt->d_iters;
+    A.updates = bt->d_upd;
+    A.resid = bt->d_res;
+    const BulkShape &sh = kBulkShapes[bt->bulk_shape];
+    int ntiles = (int)bt->h_tiles.size(), nact = nbulk;
+    k_bulk_init<<<ntiles, BULK_TILE, 0, st>>>(A);
+    k_bulk_reduce<BP_INIT><<<nact, 256, 0, st>>>(A);
+    HIPFEM_CHECK(hipGetLastError());
+    // every block is done once maxit iterations have run: the loop ends by then
+    for (int it = 0;; it++) {
+        if (it % BULK_CHECK == 0 || it >= maxit) {
+            HIPFEM_CHECK(hipMemcpyAsync(bt->h_bst.data(), bt->d_bst, sizeof(BulkBlock) * nbulk,
+                                        hipMemcpyDeviceToHost, st));
+            HIPFEM_CHECK(hipStreamSynchronize(st));
+            std::vector<int> still;
+            for (int a : act)
+                if (!bt->h_bst[a].done) still.push_back(a);
+            if (still.empty()) break;
+            HIPFEM_REQUIRE(it <= maxit, HIPFEM_ENUMERIC, "bulk phase: %d blocks not done after maxit",
+                           (int)still.size());
+            if (still.size() != act.size()) {
+                act.swap(still);
+                rc = bulk_tiles(bt, act);
+                if (rc) return rc;
+                ntiles = (int)bt->h_tiles.size();
+                nact = (int)act.size();
+            }
+        }
+        hipLaunchKernelGGL(sh.spmv, dim3(ntiles), dim3(BULK_TILE), 0, st, A);
+        k_bulk_reduce<BP_ALPHA><<<nact, 256, 0, st>>>(A);
+        k_bulk_update<<<ntiles, BULK_TILE, 0, st>>>(A);
+        k_bulk_reduce<BP_BETA><<<nact, 256, 0, st>>>(A);
+        hipLaunchKernelGGL(sh.refresh, dim3(ntiles), dim3(BULK_TILE), 0, st, A);
+        k_bulk_reduce<BP_REFRESH><<<nact, 256, 0, st>>>(A);
+        k_bulk_pupdate<<<ntiles, BULK_TILE, 0, st>>>(A);
+        HIPFEM_CHECK(hipGetLastError());
+    }
+    return 0;
+}
+
+// the persistent kernel over `count` queue entries starting at `qfirst`
+static int persistent_solve(hfem_batch *bt, const double *db, double *dx, int qfirst, int count, int maxit)
+{
+    if (count <= 0) return 0;
+    hipcsr_mtx *A = bt->A;
+    hipStream_t st = bt->ctx->stream;
     BatchCtl c0;
     memset(&c0, 0, sizeof c0);
     c0.next = qfirst;
     HIPFEM_CHECK(hipMemcpyAsync(bt->d_ctl, &c0, sizeof(BatchCtl), hipMemcpyHostToDevice, st));
+    HIPFEM_CHECK(hipStreamSynchronize(st));
     const BatchShape &sh = kShapes[bt->shape];
-    const int nb = nb_limit > 0 ? std::min(qfirst + nb_limit, bt->nblocks) : bt->nblocks;
-    int dev = 0, ncu = 0;
-    HIPFEM_CHECK(hipGetDevice(&dev));
-    HIPFEM_CHECK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    int ncu = 0;
+    int rc = device_cus(&ncu);
+    if (rc) return rc;
     const int per_cu = std::max(1, 2048 / sh.bs);
-    const int grid = bt->grid > 0 ? bt->grid : std::max(1, std::min(nb - qfirst, ncu * per_cu));
-    hipLaunchKernelGGL(sh.fn, dim3(grid), dim3(sh.bs), 0, st, nb, bt->d_order, bt->d_rows, A->d_rowptr, A->d_colind,
-                       A->d_val, A->d_val32, bt->d_dinv, db, dx, bt->d_r, bt->d_z, bt->d_p, bt->d_q, bt->d_r64, bt->tol,
-                       bt->delta, bt->maxit, bt->d_ctl, bt->d_iters, bt->d_res, bt->d_upd);
+    const int grid = bt->grid > 0 ? bt->grid : std::max(1, std::min(count, ncu * per_cu));
+    hipLaunchKernelGGL(sh.fn, dim3(grid), dim3(sh.bs), 0, st, qfirst + count, bt->d_order, bt->d_rows, A->d_rowptr,
+                       A->d_colind, A->d_val, A->d_val32, bt->d_dinv, db, dx, bt->d_r, bt->d_z, bt->d_p, bt->d_q,
+                       bt->d_r64, bt->tol, bt->delta, maxit, bt->d_ctl, bt->d_iters, bt->d_res, bt->d_upd);
     HIPFEM_CHECK(hipGetLastError());
+    return 0;
+}
+
+// solve every block for the six right-hand sides b ([row][6], device, FP64):
+// the bulk phase for the blocks above the split, then the persistent kernel
+// for the rest; solutions in x ([row][6], device); per block: iterations,
+// reliable updates and the final max relative true residual (negative when
+// a block did not converge within maxit)
+extern "C" int hfem_batch_solve_dev(hfem_batch *bt, const double *db, double *dx, int *iters, int *updates,
+                                    double *resid)
+{
+    HIPFEM_REQUIRE(bt && db && dx, HIPFEM_EINVAL, "batch solve: null argument");
+    HIPFEM_REQUIRE(bt->A->has_structure && bt->A->n == bt->n, HIPFEM_EINVAL, "batch solve: matrix changed size");
+    DeviceGuard g(bt->ctx->device);
+    int rc = batch_prepare(bt);
+    if (rc) return rc;
+    rc = bulk_solve(bt, db, dx, bt->maxit);
+    if (rc) return rc;
+    rc = persistent_solve(bt, db, dx, 0, bt->nsmall, bt->maxit);
+    if (rc) return rc;
+    hipStream_t st = bt->ctx->stream;
     if (iters) HIPFEM_CHECK(hipMemcpyAsync(iters, bt->d_iters, sizeof(int) * bt->nblocks, hipMemcpyDeviceToHost, st));
     if (updates) HIPFEM_CHECK(hipMemcpyAsync(updates, bt->d_upd, sizeof(int) * bt->nblocks, hipMemcpyDeviceToHost, st));
     if (resid) HIPFEM_CHECK(hipMemcpyAsync(resid, bt->d_res, sizeof(double) * bt->nblocks, hipMemcpyDeviceToHost, st));
@@ -457,60 +992,97 @@ static int batch_solve(hfem_batch *bt, const double *db, double *dx, int *iters,
     return 0;
 }
 
-extern "C" int hfem_batch_solve_dev(hfem_batch *bt, const double *db, double *dx, int *iters, int *updates,
-                                    double *resid)
-{
-    HIPFEM_REQUIRE(bt && db && dx, HIPFEM_EINVAL, "batch solve: null argument");
-    HIPFEM_REQUIRE(bt->A->has_structure && bt->A->n == bt->n, HIPFEM_EINVAL, "batch solve: matrix changed size");
-    DeviceGuard g(bt->ctx->device);
-    return batch_solve(bt, db, dx, iters, updates, resid, 0);
-}
+constexpr int TUNE_ITERS = 40;
 
-// time every shape on `sample` blocks of median size (a contiguous stretch
-// of the size-ordered queue around its middle), each run capped at
-// TUNE_ITERS iterations, and keep the fastest; ms[NSHAPES] receives the
-// timings
+// time every persistent-kernel shape on `sample` blocks of median size
+// among those at or below the split (a contiguous stretch of the queue
+// around its middle), each run capped at TUNE_ITERS iterations, and keep
+// the fastest; ms[NSHAPES] receives the timings (all 0 when no block is at
+// or below the split)
 extern "C" int hfem_batch_autotune(hfem_batch *bt, const double *db, double *dx, int sample, double *ms, int *best)
 {
     HIPFEM_REQUIRE(bt && db && dx && sample > 0, HIPFEM_EINVAL, "autotune: bad arguments");
     DeviceGuard g(bt->ctx->device);
+    int rc = batch_prepare(bt);
+    if (rc) return rc;
+    if (ms)
+        for (int s = 0; s < NSHAPES; s++) ms[s] = 0.0;
+    if (best) *best = bt->shape;
+    if (bt->nsmall == 0) return 0;
     hipEvent_t e0, e1;
     HIPFEM_CHECK(hipEventCreate(&e0));
     HIPFEM_CHECK(hipEventCreate(&e1));
-    const int keep_shape = bt->shape, keep_grid = bt->grid;
-    int bi = keep_shape;
+    int bi = bt->shape;
     float bms = 1e30f;
-    const int ns = std::min(sample, bt->nblocks);
-    const int q0 = std::max(0, bt->nblocks / 2 - ns / 2);
-    constexpr int TUNE_ITERS = 40;
-    const int keep_maxit = bt->maxit;
-    bt->maxit = std::min(keep_maxit, TUNE_ITERS);
-    for (int s = 0; s < NSHAPES; s++) {
+    const int ns = std::min(sample, bt->nsmall);
+    const int q0 = std::max(0, bt->nsmall / 2 - ns / 2);
+    const int cap = std::min(bt->maxit, TUNE_ITERS);
+    for (int s = 0; s < NSHAPES && rc == 0; s++) {
         bt->shape = s;
-        bt->grid = 0;
-        HIPFEM_CHECK(hipEventRecord(e0, bt->ctx->stream));
-        int rc = batch_solve(bt, db, dx, nullptr, nullptr, nullptr, ns, q0);
-        if (rc) {
-            bt->maxit = keep_maxit;
-            return rc;
-        }
-        HIPFEM_CHECK(hipEventRecord(e1, bt->ctx->stream));
-        HIPFEM_CHECK(hipEventSynchronize(e1));
+        rc = (hipEventRecord(e0, bt->ctx->stream) == hipSuccess) ? 0 : HIPFEM_EHIP;
+        if (!rc) rc = persistent_solve(bt, db, dx, q0, ns, cap);
+        if (!rc) rc = (hipEventRecord(e1, bt->ctx->stream) == hipSuccess) ? 0 : HIPFEM_EHIP;
+        if (!rc) rc = (hipEventSynchronize(e1) == hipSuccess) ? 0 : HIPFEM_EHIP;
         float t = 0.f;
-        HIPFEM_CHECK(hipEventElapsedTime(&t, e0, e1));
+        if (!rc) rc = (hipEventElapsedTime(&t, e0, e1) == hipSuccess) ? 0 : HIPFEM_EHIP;
         if (ms) ms[s] = t;
         if (t < bms) {
             bms = t;
             bi = s;
         }
     }
-    (void)keep_grid;
-    bt->maxit = keep_maxit;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) {
+        set_error("autotune: a timed run failed");
+        return rc;
+    }
     bt->shape = bi;
     bt->grid = 0;
     if (best) *best = bi;
-    HIPFEM_CHECK(hipEventDestroy(e0));
-    HIPFEM_CHECK(hipEventDestroy(e1));
-    (void)keep_shape;
+    return 0;
+}
+
+// time every bulk shape on the blocks above the split, each run capped at
+// TUNE_ITERS iterations, and keep the fastest; ms[NBULK] receives the
+// timings (all 0 when no block is above the split)
+extern "C" int hfem_batch_autotune_bulk(hfem_batch *bt, const double *db, double *dx, double *ms, int *best)
+{
+    HIPFEM_REQUIRE(bt && db && dx, HIPFEM_EINVAL, "bulk autotune: bad arguments");
+    DeviceGuard g(bt->ctx->device);
+    int rc = batch_prepare(bt);
+    if (rc) return rc;
+    if (ms)
+        for (int s = 0; s < NBULK; s++) ms[s] = 0.0;
+    if (best) *best = bt->bulk_shape;
+    if (bt->h_bulk.empty()) return 0;
+    hipEvent_t e0, e1;
+    HIPFEM_CHECK(hipEventCreate(&e0));
+    HIPFEM_CHECK(hipEventCreate(&e1));
+    int bi = bt->bulk_shape;
+    float bms = 1e30f;
+    const int cap = std::min(bt->maxit, TUNE_ITERS);
+    for (int s = 0; s < NBULK && rc == 0; s++) {
+        bt->bulk_shape = s;
+        rc = (hipEventRecord(e0, bt->ctx->stream) == hipSuccess) ? 0 : HIPFEM_EHIP;
+        if (!rc) rc = bulk_solve(bt, db, dx, cap);
+        if (!rc) rc = (hipEventRecord(e1, bt->ctx->stream) == hipSuccess) ? 0 : HIPFEM_EHIP;
+        if (!rc) rc = (hipEventSynchronize(e1) == hipSuccess) ? 0 : HIPFEM_EHIP;
+        float t = 0.f;
+        if (!rc) rc = (hipEventElapsedTime(&t, e0, e1) == hipSuccess) ? 0 : HIPFEM_EHIP;
+        if (ms) ms[s] = t;
+        if (t < bms) {
+            bms = t;
+            bi = s;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) {
+        set_error("bulk autotune: a timed run failed");
+        return rc;
+    }
+    bt->bulk_shape = bi;
+    if (best) *best = bi;
     return 0;
 }

@@ -10,9 +10,10 @@ and the boundary-reaction average sigma = (1/V) sum_b R_b (x) (x_b - x_c).
 GPU layout: the RVEs of one element type are concatenated into one mesh
 (disjoint node ranges), so the batched evaluator assembles one
 block-diagonal stiffness; chunks keep every matrix under the int32 index
-range.  All blocks and all six right-hand sides are solved in one persistent
-launch per chunk (hfem_batch: mixed-precision PCG with FP64 reliable
-updates, largest blocks first, kernel shape from the autotuner).
+range.  All blocks and all six right-hand sides of a chunk are solved by
+hfem_batch (mixed-precision PCG with FP64 reliable updates): blocks above
+the split by a bulk-synchronous phase over the whole GPU, the rest by one
+persistent launch, largest blocks first; kernel shapes from the autotuner.
 """
 from __future__ import annotations
 
@@ -149,13 +150,26 @@ class RVEBatch:
                     cur_nnz += est
         self.t_setup = time.perf_counter() - t
 
-    def set_options(self, tol=1e-8, delta=0.1, maxit=20000):
+    def set_options(self, tol=1e-8, delta=0.1, maxit=20000, split=-1):
+        """split: blocks with more stored entries than this are solved by the
+        bulk-synchronous phase (-1: automatic, 0: every block)."""
         for c in self.chunks:
             _check(lib().hfem_batch_set_options(c.h, C.c_double(tol), C.c_double(delta), C.c_int(maxit)), "options")
+            _check(lib().hfem_batch_set_split(c.h, C.c_longlong(split)), "split")
+
+    def partition(self):
+        """(bulk blocks, split used) per chunk."""
+        out = []
+        for c in self.chunks:
+            nb, thr = C.c_int(0), C.c_longlong(0)
+            _check(lib().hfem_batch_partition_info(c.h, C.byref(nb), C.byref(thr)), "partition")
+            out.append((nb.value, thr.value))
+        return out
 
     def autotune(self, sample=8, log=None):
         nshape = lib().hfem_batch_nshapes()
         out = []
+        nbulk = lib().hfem_batch_nbulk_shapes()
         for c in self.chunks:
             if log:
                 log(f"[rve] autotune on {sample} median blocks of a chunk of {len(c.ids)}")
@@ -166,8 +180,18 @@ class RVEBatch:
                                              _p(ms), C.byref(best)), "autotune")
             bs, lpr = C.c_int(0), C.c_int(0)
             lib().hfem_batch_shape_info(best.value, C.byref(bs), C.byref(lpr))
-            out.append(dict(etype=int(c.deck.etype), best=best.value, block=bs.value, lanes_per_row=lpr.value,
-                            ms=[round(float(v), 3) for v in ms]))
+            msb = np.zeros(nbulk)
+            bestb = C.c_int(0)
+            _check(lib().hfem_batch_autotune_bulk(c.h, C.c_void_p(c.b.addr), C.c_void_p(c.x.addr), _p(msb),
+                                                  C.byref(bestb)), "bulk autotune")
+            blpr = C.c_int(0)
+            lib().hfem_batch_bulk_shape_info(bestb.value, C.byref(blpr))
+            nb, thr = C.c_int(0), C.c_longlong(0)
+            _check(lib().hfem_batch_partition_info(c.h, C.byref(nb), C.byref(thr)), "partition")
+            out.append(dict(etype=int(c.deck.etype), blocks=len(c.ids), bulk_blocks=nb.value, split=thr.value,
+                            best=best.value, block=bs.value, lanes_per_row=lpr.value,
+                            ms=[round(float(v), 3) for v in ms], bulk_lanes_per_row=blpr.value,
+                            bulk_ms=[round(float(v), 3) for v in msb]))
         return out
 
     def run(self, log=None):
@@ -179,10 +203,12 @@ class RVEBatch:
         it = np.zeros(nrve, dtype=np.int32)
         upd = np.zeros(nrve, dtype=np.int32)
         res = np.zeros(nrve)
-        t_solve = 0.0
+        t_solve = t_rhs = t_hom = 0.0
         for c in self.chunks:
+            t = time.perf_counter()
             c.rhs()
             self.ctx.synchronize()
+            t_rhs += time.perf_counter() - t
             t = time.perf_counter()
             i_, u_, r_ = c.solve()
             t_solve += time.perf_counter() - t
@@ -190,8 +216,10 @@ class RVEBatch:
                 log(f"[rve] chunk of {len(c.ids)} solved in {time.perf_counter() - t:.2f} s, iterations max {i_.max()}")
             ids = np.asarray(c.ids)
             it[ids], upd[ids], res[ids] = i_, u_, r_
+            t = time.perf_counter()
             C6[ids] = c.homogenized()
-        return dict(C=C6, iters=it, updates=upd, resid=res, t_solve=t_solve)
+            t_hom += time.perf_counter() - t
+        return dict(C=C6, iters=it, updates=upd, resid=res, t_solve=t_solve, t_rhs=t_rhs, t_homogenize=t_hom)
 
     def close(self):
         for c in self.chunks:

@@ -24,11 +24,22 @@ def _oracle_C(spec):
     return C
 
 
-def test_batch_homogenized_stiffness_matches_oracle(ctx):
+# stored entries per block range from about 400 (n = 2, 3) to 60,000 (QSpace
+# n = 4): 0 sends every block to the bulk phase, 5000 splits the batch,
+# 10**12 keeps every block in the persistent kernel
+SPLITS = {"bulk": 0, "mixed": 5000, "persistent": 10**12}
+
+
+@pytest.mark.parametrize("mode", sorted(SPLITS))
+def test_batch_homogenized_stiffness_matches_oracle(ctx, mode):
     from oofem_amd.rve import RVEBatch
     B = RVEBatch(ctx, SPECS, nnz_budget=2e5)        # small budget: several chunks per element type
     assert len(B.chunks) >= 3
-    B.set_options(tol=1e-10)
+    B.set_options(tol=1e-10, split=SPLITS[mode])
+    nbulk = sum(nb for nb, _ in B.partition())
+    expect = {"bulk": lambda v: v == len(SPECS), "mixed": lambda v: 0 < v < len(SPECS),
+              "persistent": lambda v: v == 0}[mode]
+    assert expect(nbulk), (mode, nbulk, B.partition())
     out = B.run()
     assert (out["resid"] > 0).all() and (out["resid"] <= 1e-10).all(), out["resid"]
     for i, spec in enumerate(SPECS):
@@ -41,8 +52,24 @@ def test_batch_homogenized_stiffness_matches_oracle(ctx):
 def test_batch_autotune_picks_a_valid_shape(ctx):
     from oofem_amd.rve import RVEBatch
     B = RVEBatch(ctx, SPECS[:4])
+    B.set_options(tol=1e-8, split=5000)
     tuned = B.autotune(sample=2)
-    assert all(0 <= t["best"] < len(t["ms"]) and t["ms"][t["best"]] == min(t["ms"]) for t in tuned)
+    for t in tuned:
+        if t["bulk_blocks"] < t["blocks"]:
+            assert 0 <= t["best"] < len(t["ms"]) and t["ms"][t["best"]] == min(t["ms"]), t
+        if t["bulk_blocks"] > 0:
+            assert t["bulk_ms"][t["bulk_lanes_per_row"].bit_length() - 3] == min(t["bulk_ms"]), t
     out = B.run()
     assert (out["resid"] > 0).all()
+    B.close()
+
+
+def test_batch_bulk_maxit_reports_unconverged(ctx):
+    """A cap below the iterations needed ends every block of both phases
+    with a negative residual and exactly the capped iteration count."""
+    from oofem_amd.rve import RVEBatch
+    B = RVEBatch(ctx, SPECS[:3])
+    B.set_options(tol=1e-12, maxit=5, split=5000)
+    out = B.run()
+    assert (out["resid"] < 0).all() and (out["iters"] == 5).all(), out
     B.close()

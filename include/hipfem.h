@@ -287,6 +287,11 @@ int  hfem_batch_autotune(hfem_batch *bt, const double *db, double *dx, int sampl
 int  hfem_batch_nbulk_shapes(void);
 int  hfem_batch_bulk_shape_info(int shape, int *lanes_per_row);
 int  hfem_batch_set_bulk_shape(hfem_batch *bt, int shape);
+/* bulk products in the 3x3 node-block form (1, the default) when the
+ * pattern is node-blocked and every block starts on a node, else CSR rows;
+ * hfem_batch_bulk_form reports the form in use (1 node-block, 0 CSR) */
+int  hfem_batch_set_bulk_form(hfem_batch *bt, int nodeblock);
+int  hfem_batch_bulk_form(const hfem_batch *bt);
 /* split: blocks with more stored entries go to the bulk phase; -1 (the
  * default) = the batch's entries divided by the workgroups the persistent
  * kernel keeps resident; 0 = every block */

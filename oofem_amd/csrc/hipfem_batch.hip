@@ -24,8 +24,9 @@
 // bandwidth, so a block much larger than the batch's mean share would set
 // the whole launch's length.  Blocks above a split size (stored entries)
 // are therefore solved first, together, in a bulk-synchronous phase: every
-// launch covers all their rows in tiles of BULK_TILE rows (one workgroup
-// each, the whole GPU), per-tile partial dots are summed per block by a
+// launch covers all their rows in tiles of TILE_ROWS rows (one workgroup
+// each, the whole GPU; the products use a 3x3 node-block form when the
+// pattern has one), per-tile partial dots are summed per block by a
 // reduction launch, and the same per-column scalars, reliable updates and
 // stopping rule as the persistent kernel live in a BulkBlock record per
 // block.  Finished blocks drop out of the tile list when the host next looks
@@ -278,7 +279,8 @@ __global__ __launch_bounds__(BS) void k_batch_pcg(int nblocks, const int *__rest
 }
 
 // ------------------------------------------------------------ bulk phase
-constexpr int BULK_TILE = 256;   // rows per tile = threads per workgroup
+constexpr int BULK_TILE = 256;   // threads per workgroup
+constexpr int TILE_ROWS = 255;   // rows per tile: 85 whole nodes, one row per thread
 constexpr int BULK_CHECK = 16;   // iterations between host looks at the done flags
 constexpr int NPART = 2 * NRHS;  // partial sums per tile
 
@@ -293,11 +295,12 @@ struct BulkBlock {
 };
 
 struct BulkArgs {
-    const int4 *tiles;       // {first row, end row, bulk index, 0}
+    const int4 *tiles;       // {first row, end row, bulk index, 0}; node-aligned in the node-block form
     const int *act;          // bulk indices of the blocks in the tile list
     const int *act_t0;       // their tile ranges [act_t0[a], act_t0[a + 1])
     BulkBlock *st;
     const int32_t *rowptr, *col;
+    const int32_t *bptr, *bcol;  // node-block view (node rows, block columns) when the pattern has one
     const double *val64;
     const float *val32, *dinv;
     const double *b;
@@ -467,6 +470,135 @@ __global__ __launch_bounds__(BULK_TILE) void k_bulk_refresh(BulkArgs A)
     block_sums<BULK_TILE, NPART>(v, sh, A.part + (size_t)blockIdx.x * NPART);
 }
 
+// Node-block forms of the two products, for patterns whose rows come in
+// fully free node triples (every RVE: boundary nodes are prescribed whole).
+// Node row I holds nbk 3x3 blocks; block k's column node is bcol[bo + k] and
+// its row r values sit at val[9 bo + 3 r nbk + 3 k + (0..2)] (the CSR arrays
+// themselves).  The six right-hand sides of a node are 18 consecutive
+// entries of the interleaved vectors, so a block gathers 72 (FP32) or 144
+// (FP64) contiguous bytes and the column index is read once per 9 values.
+// LPR lanes share a node row; after the lane reduction lane j < 18 owns
+// output (row j / 6, column j % 6).
+template <int LPR, typename VT, typename MT>
+__device__ __forceinline__ void nb_row6(int bo, int nbk, int lane, const int32_t *__restrict__ bcol,
+                                        const MT *__restrict__ val, const VT *__restrict__ x, VT acc[3][NRHS])
+{
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int k = 0; k < NRHS; k++) acc[r][k] = VT(0);
+    const MT *v0 = val + 9 * (size_t)bo;
+    for (int k = lane; k < nbk; k += LPR) {
+        const VT *xb = x + (size_t)bcol[bo + k] * (3 * NRHS);
+        VT xv[3][NRHS];
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+#pragma unroll
+            for (int j = 0; j < NRHS; j += 2) {
+                if constexpr (sizeof(VT) == 4) {
+                    const float2 t = *reinterpret_cast<const float2 *>(xb + c * NRHS + j);
+                    xv[c][j] = t.x;
+                    xv[c][j + 1] = t.y;
+                } else {
+                    const double2 t = *reinterpret_cast<const double2 *>(xb + c * NRHS + j);
+                    xv[c][j] = t.x;
+                    xv[c][j + 1] = t.y;
+                }
+            }
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            const MT *vr = v0 + 3 * (size_t)r * nbk + 3 * k;
+            const VT a0 = (VT)vr[0], a1 = (VT)vr[1], a2 = (VT)vr[2];
+#pragma unroll
+            for (int j = 0; j < NRHS; j++) acc[r][j] = fma(a2, xv[2][j], fma(a1, xv[1][j], fma(a0, xv[0][j], acc[r][j])));
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int j = 0; j < NRHS; j++) {
+            VT w = acc[r][j];
+#pragma unroll
+            for (int off = LPR / 2; off > 0; off >>= 1) w += __shfl_xor(w, off, 64);
+            acc[r][j] = w;
+        }
+}
+
+// value of output j (0..17) of a reduced node row, selected without
+// dynamic register indexing
+template <typename VT>
+__device__ __forceinline__ VT pick18(const VT acc[3][NRHS], int j)
+{
+    VT v = VT(0);
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int c = 0; c < NRHS; c++) v = (j == r * NRHS + c) ? acc[r][c] : v;
+    return v;
+}
+
+// q = A p (FP32) in the node-block form; partials p.q
+template <int LPR>
+__global__ __launch_bounds__(BULK_TILE) void k_bulk_spmv_nb(BulkArgs A)
+{
+    static_assert(LPR >= 4, "18 outputs per node row need at least 4 lanes (stores spread over the lanes)");
+    __shared__ double sh[NRHS * (BULK_TILE / 64)];
+    const int4 t = A.tiles[blockIdx.x];
+    if (A.st[t.z].done) return;
+    double v[NRHS];
+#pragma unroll
+    for (int k = 0; k < NRHS; k++) v[k] = 0.0;
+    const int lane = threadIdx.x % LPR;
+    for (int I = t.x / 3 + threadIdx.x / LPR; I < t.y / 3; I += BULK_TILE / LPR) {
+        const int bo = A.bptr[I], nbk = A.bptr[I + 1] - bo;
+        float acc[3][NRHS];
+        nb_row6<LPR>(bo, nbk, lane, A.bcol, A.val32, A.p, acc);
+        for (int j = lane; j < 3 * NRHS; j += LPR) {
+            const size_t o = (size_t)I * (3 * NRHS) + j;
+            const float qj = pick18(acc, j);
+            A.q[o] = qj;
+            const double pq = (double)A.p[o] * (double)qj;
+#pragma unroll
+            for (int k = 0; k < NRHS; k++) v[k] += (j % NRHS == k) ? pq : 0.0;
+        }
+    }
+    block_sums<BULK_TILE, NRHS>(v, sh, A.part + (size_t)blockIdx.x * NPART);
+}
+
+// reliable update in the node-block form: r = b - A x (FP64), z = D^-1 r
+template <int LPR>
+__global__ __launch_bounds__(BULK_TILE) void k_bulk_refresh_nb(BulkArgs A)
+{
+    __shared__ double sh[NPART * (BULK_TILE / 64)];
+    const int4 t = A.tiles[blockIdx.x];
+    const BulkBlock &S = A.st[t.z];
+    if (S.done || !S.refresh) return;
+    double v[NPART];
+#pragma unroll
+    for (int k = 0; k < NPART; k++) v[k] = 0.0;
+    const int lane = threadIdx.x % LPR;
+    for (int I = t.x / 3 + threadIdx.x / LPR; I < t.y / 3; I += BULK_TILE / LPR) {
+        const int bo = A.bptr[I], nbk = A.bptr[I + 1] - bo;
+        double acc[3][NRHS];
+        nb_row6<LPR>(bo, nbk, lane, A.bcol, A.val64, A.x, acc);
+        for (int j = lane; j < 3 * NRHS; j += LPR) {
+            const size_t o = (size_t)I * (3 * NRHS) + j;
+            const double rt = A.b[o] - pick18(acc, j);
+            const float ri = (float)rt;
+            A.r[o] = ri;
+            const float zi = A.dinv[3 * I + j / NRHS] * ri;
+            A.z[o] = zi;
+            const double rz = (double)ri * (double)zi, rr = rt * rt;
+#pragma unroll
+            for (int k = 0; k < NRHS; k++) {
+                v[k] += (j % NRHS == k) ? rz : 0.0;
+                v[NRHS + k] += (j % NRHS == k) ? rr : 0.0;
+            }
+        }
+    }
+    block_sums<BULK_TILE, NPART>(v, sh, A.part + (size_t)blockIdx.x * NPART);
+}
+
 // p = z + beta p
 __global__ __launch_bounds__(BULK_TILE) void k_bulk_pupdate(BulkArgs A)
 {
@@ -570,12 +702,12 @@ __global__ __launch_bounds__(256) void k_bulk_reduce(BulkArgs A)
 typedef void (*bulk_kernel_t)(BulkArgs);
 struct BulkShape {
     int lpr;
-    bulk_kernel_t spmv, refresh;
+    bulk_kernel_t spmv, refresh, spmv_nb, refresh_nb;
 };
 static const BulkShape kBulkShapes[] = {
-    {4, k_bulk_spmv<4>, k_bulk_refresh<4>},
-    {8, k_bulk_spmv<8>, k_bulk_refresh<8>},
-    {16, k_bulk_spmv<16>, k_bulk_refresh<16>},
+    {4, k_bulk_spmv<4>, k_bulk_refresh<4>, k_bulk_spmv_nb<4>, k_bulk_refresh_nb<4>},
+    {8, k_bulk_spmv<8>, k_bulk_refresh<8>, k_bulk_spmv_nb<8>, k_bulk_refresh_nb<8>},
+    {16, k_bulk_spmv<16>, k_bulk_refresh<16>, k_bulk_spmv_nb<16>, k_bulk_refresh_nb<16>},
 };
 constexpr int NBULK = sizeof(kBulkShapes) / sizeof(kBulkShapes[0]);
 
@@ -637,6 +769,8 @@ struct hfem_batch {
     int nsmall = 0;
     // bulk phase
     int bulk_shape = 1;         // index into kBulkShapes
+    bool nodeblock_ok = false;  // node-block kernels apply to this matrix
+    bool nodeblock = true;      // use them when they apply
     int tiles_cap = 0;
     BulkBlock *d_bst = nullptr;
     int4 *d_tiles = nullptr;
@@ -685,8 +819,12 @@ extern "C" int hfem_batch_create(hipcsr_mtx *A, int nblocks, const int32_t *rows
     std::stable_sort(bt->h_sorted.begin(), bt->h_sorted.end(),
                      [&](int a, int c) { return bt->h_nnz[a] > bt->h_nnz[c]; });
     const size_t nv = (size_t)std::max(1, bt->n) * NRHS;
-    // tiles of BULK_TILE rows never straddle blocks: at most n / BULK_TILE + nblocks
-    bt->tiles_cap = bt->n / BULK_TILE + nblocks + 1;
+    // tiles of TILE_ROWS rows never straddle blocks: at most n / TILE_ROWS + nblocks
+    bt->tiles_cap = bt->n / TILE_ROWS + nblocks + 1;
+    // node-block form: the pattern has a node-block view and every block
+    // starts on a node boundary
+    bt->nodeblock_ok = A->nodeblocked;
+    for (int b = 0; b <= nblocks && bt->nodeblock_ok; b++) bt->nodeblock_ok = rows[b] % 3 == 0;
     auto fail = [&](int rc) { batch_free(bt); delete bt; return rc; };
     if (hipMalloc(&bt->d_rows, sizeof(int32_t) * (nblocks + 1)) != hipSuccess ||
         hipMalloc(&bt->d_order, sizeof(int) * nblocks) != hipSuccess ||
@@ -772,6 +910,18 @@ extern "C" int hfem_batch_set_bulk_shape(hfem_batch *bt, int shape)
     return 0;
 }
 
+extern "C" int hfem_batch_set_bulk_form(hfem_batch *bt, int nodeblock)
+{
+    HIPFEM_REQUIRE(bt, HIPFEM_EINVAL, "bulk form: null handle");
+    bt->nodeblock = nodeblock != 0;
+    return 0;
+}
+
+extern "C" int hfem_batch_bulk_form(const hfem_batch *bt)
+{
+    return bt && bt->nodeblock && bt->nodeblock_ok && bt->A->nodeblocked ? 1 : 0;
+}
+
 extern "C" int hfem_batch_set_split(hfem_batch *bt, long long split)
 {
     HIPFEM_REQUIRE(bt && split >= -1, HIPFEM_EINVAL, "batch split: bad value");
@@ -855,8 +1005,8 @@ static int bulk_tiles(hfem_batch *bt, const std::vector<int> &act)
     for (int a : act) {
         const int blk = bt->h_bulk[a];
         bt->h_act_t0.push_back((int)bt->h_tiles.size());
-        for (int r = bt->h_rows[blk]; r < bt->h_rows[blk + 1]; r += BULK_TILE)
-            bt->h_tiles.push_back(make_int4(r, std::min(r + BULK_TILE, bt->h_rows[blk + 1]), a, 0));
+        for (int r = bt->h_rows[blk]; r < bt->h_rows[blk + 1]; r += TILE_ROWS)
+            bt->h_tiles.push_back(make_int4(r, std::min(r + TILE_ROWS, bt->h_rows[blk + 1]), a, 0));
     }
     bt->h_act_t0.push_back((int)bt->h_tiles.size());
     HIPFEM_REQUIRE((int)bt->h_tiles.size() <= bt->tiles_cap, HIPFEM_EINVAL, "bulk tiles exceed their capacity");
@@ -890,6 +1040,8 @@ static int bulk_solve(hfem_batch *bt, const double *db, double *dx, int maxit)
     A.st = bt->d_bst;
     A.rowptr = M->d_rowptr;
     A.col = M->d_colind;
+    A.bptr = M->d_bptr;
+    A.bcol = M->d_bcol;
     A.val64 = M->d_val;
     A.val32 = M->d_val32;
     A.dinv = bt->d_dinv;
@@ -907,6 +1059,8 @@ static int bulk_solve(hfem_batch *bt, const double *db, double *dx, int maxit)
     A.updates = bt->d_upd;
     A.resid = bt->d_res;
     const BulkShape &sh = kBulkShapes[bt->bulk_shape];
+    const bool nb = bt->nodeblock && bt->nodeblock_ok && M->nodeblocked;
+    const bulk_kernel_t spmv = nb ? sh.spmv_nb : sh.spmv, refresh = nb ? sh.refresh_nb : sh.refresh;
     int ntiles = (int)bt->h_tiles.size(), nact = nbulk;
     k_bulk_init<<<ntiles, BULK_TILE, 0, st>>>(A);
     k_bulk_reduce<BP_INIT><<<nact, 256, 0, st>>>(A);
@@ -931,11 +1085,11 @@ static int bulk_solve(hfem_batch *bt, const double *db, double *dx, int maxit)
                 nact = (int)act.size();
             }
         }
-        hipLaunchKernelGGL(sh.spmv, dim3(ntiles), dim3(BULK_TILE), 0, st, A);
+        hipLaunchKernelGGL(spmv, dim3(ntiles), dim3(BULK_TILE), 0, st, A);
         k_bulk_reduce<BP_ALPHA><<<nact, 256, 0, st>>>(A);
         k_bulk_update<<<ntiles, BULK_TILE, 0, st>>>(A);
         k_bulk_reduce<BP_BETA><<<nact, 256, 0, st>>>(A);
-        hipLaunchKernelGGL(sh.refresh, dim3(ntiles), dim3(BULK_TILE), 0, st, A);
+        hipLaunchKernelGGL(refresh, dim3(ntiles), dim3(BULK_TILE), 0, st, A);
         k_bulk_reduce<BP_REFRESH><<<nact, 256, 0, st>>>(A);
         k_bulk_pupdate<<<ntiles, BULK_TILE, 0, st>>>(A);
         HIPFEM_CHECK(hipGetLastError());

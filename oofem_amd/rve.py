@@ -150,12 +150,19 @@ class RVEBatch:
                     cur_nnz += est
         self.t_setup = time.perf_counter() - t
 
-    def set_options(self, tol=1e-8, delta=0.1, maxit=20000, split=-1):
+    def set_options(self, tol=1e-8, delta=0.1, maxit=20000, split=-1, nodeblock=True):
         """split: blocks with more stored entries than this are solved by the
-        bulk-synchronous phase (-1: automatic, 0: every block)."""
+        bulk-synchronous phase (-1: automatic, 0: every block); nodeblock:
+        3x3 node-block products in that phase where the pattern allows."""
         for c in self.chunks:
             _check(lib().hfem_batch_set_options(c.h, C.c_double(tol), C.c_double(delta), C.c_int(maxit)), "options")
             _check(lib().hfem_batch_set_split(c.h, C.c_longlong(split)), "split")
+            _check(lib().hfem_batch_set_bulk_form(c.h, int(nodeblock)), "bulk form")
+
+    def bulk_forms(self):
+        """1 (node-block) or 0 (CSR rows) per chunk."""
+        lib().hfem_batch_bulk_form.argtypes = [C.c_void_p]
+        return [lib().hfem_batch_bulk_form(c.h) for c in self.chunks]
 
     def partition(self):
         """(bulk blocks, split used) per chunk."""
@@ -188,7 +195,9 @@ class RVEBatch:
             lib().hfem_batch_bulk_shape_info(bestb.value, C.byref(blpr))
             nb, thr = C.c_int(0), C.c_longlong(0)
             _check(lib().hfem_batch_partition_info(c.h, C.byref(nb), C.byref(thr)), "partition")
+            lib().hfem_batch_bulk_form.argtypes = [C.c_void_p]
             out.append(dict(etype=int(c.deck.etype), blocks=len(c.ids), bulk_blocks=nb.value, split=thr.value,
+                            bulk_nodeblock=lib().hfem_batch_bulk_form(c.h),
                             best=best.value, block=bs.value, lanes_per_row=lpr.value,
                             ms=[round(float(v), 3) for v in ms], bulk_lanes_per_row=blpr.value,
                             bulk_ms=[round(float(v), 3) for v in msb]))

@@ -146,18 +146,19 @@ def run_gpu(args, deck, precision):
     return out
 
 
-def run_newton(args, deck, precision, warm=1, steps=None, total=None, surrogate=None, keep_state=False):
+def run_newton(args, deck, precision, warm=1, steps=None, total=None, surrogate=None, keep_state=False, forcing=0.0):
     """StaticStructural + NRSolver load steps (full Newton, consistent tangent,
     rtolf 1e-8, tangent initial guess) with the prescribed end displacement
     ramped over `total` steps: `warm` untimed steps, then `steps` timed ones.
-    surrogate: (params, band, tol) of a trained MisesMat network, or None."""
+    surrogate: (params, band, tol) of a trained MisesMat network, or None;
+    forcing > 0: inexact Newton with forcing terms up to that value."""
     import numpy as np
 
     from oofem_amd.problem import StructuralProblem
     steps = args.nl_steps if steps is None else steps
     total = args.nl_total if total is None else total
     P = StructuralProblem(deck, lstol=args.nl_lstol, lsiter=args.maxit, lsprecond=1, mixed=precision,
-                          irtol=args.inner_tol, rtolf=1e-8, maxiter=60, manrmsteps=1)
+                          irtol=args.inner_tol, rtolf=1e-8, maxiter=60, manrmsteps=1, forcing=forcing)
     P.set_load(np.zeros(deck.neq), np.zeros(deck.npeq))
     net = None
     if surrogate is not None:
@@ -180,7 +181,8 @@ def run_newton(args, deck, precision, warm=1, steps=None, total=None, surrogate=
             raise RuntimeError(f"Newton step {k} failed: {rep}")
         P.update()
         reports.append(rep)
-        log(f"[bench] precision {precision}{' +surrogate' if net else ''} newton step {k}: {rep}")
+        log(f"[bench] precision {precision}{' +surrogate' if net else ''}{' +forcing' if forcing else ''} "
+            f"newton step {k}: {rep}")
     dt = time.perf_counter() - t0
     p = P.profile()
     out = dict(dt=dt, steps=steps, newton=[r["newton"] for r in reports[warm:]],
@@ -321,13 +323,18 @@ def main_newton(args):
     log(f"[bench] deck {deck.name}: nelem={deck.nelem} neq={deck.neq} ({time.perf_counter() - t:.1f} s)")
     kw = dict(warm=args.c3_warm, steps=args.c3_steps, total=args.c3_total)
     runs = {"fp64": run_newton(args, deck, 0, keep_state=True, **kw)}
+    if args.forcing > 0:
+        runs["fp64_inexact"] = run_newton(args, deck, 0, keep_state=True, forcing=args.forcing, **kw)
     if args.precision in ("both", "mixed", "all"):
         runs["mixed"] = run_newton(args, deck, 2, **kw)
+        if args.forcing > 0:
+            runs["mixed_inexact"] = run_newton(args, deck, 2, keep_state=True, forcing=args.forcing, **kw)
     sur_info = None
     if args.precision in ("both", "all") and args.sur_steps > 0:
         params, sur_info = train_surrogate(args, deck)
-        runs["mixed_surrogate"] = run_newton(args, deck, 2, surrogate=(params, args.sur_band, args.sur_tol),
-                                             keep_state=True, **kw)
+        name = "mixed_surrogate_inexact" if args.forcing > 0 else "mixed_surrogate"
+        runs[name] = run_newton(args, deck, 2, surrogate=(params, args.sur_band, args.sur_tol), keep_state=True,
+                                forcing=args.forcing, **kw)
     base = runs["fp64"]
     n = deck.neq
     variants = {}
@@ -460,6 +467,8 @@ def main():
     ap.add_argument("--workload", choices=["cfg1", "cfg0", "cfg3", "cfg4"], default="cfg3")
     ap.add_argument("--rve-count", type=int, default=512)
     ap.add_argument("--rve-tol", type=float, default=1e-8)
+    ap.add_argument("--forcing", type=float, default=0.1,
+                    help="inexact-Newton variants with forcing terms up to this value (0: none)")
     ap.add_argument("--rve-tune-sample", type=int, default=16)
     ap.add_argument("--rve-split", type=int, default=-1,
                     help="stored entries above which an RVE goes to the bulk phase (-1 automatic, 0 all)")

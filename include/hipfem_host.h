@@ -36,6 +36,12 @@ int hfem_problem_set_options(hfem_problem *p, int smtype, int lstype, double lst
 /* StaticStructural initial guess (_IFT_EngngModel_initialGuess,
  * staticstructural.C:130,247-297): 0 = IG_None, 1 = IG_Tangent (default) */
 int hfem_problem_set_initial_guess(hfem_problem *p, int type);
+/* Inexact Newton (no counterpart in the reference: NRSolver::solve,
+ * nrsolver.C:205-404, solves every correction to lstol).  on = 1: each
+ * Newton correction is solved to a forcing term between lstol and eta_max
+ * (Eisenstat-Walker choice 2 from the force-error history, never tighter
+ * than 0.1 rtolf / error); the convergence check is unchanged. */
+int hfem_problem_set_forcing(hfem_problem *p, int on, double eta_max);
 /* prescribed values (npeq) and total external load (neq, npeq) of the next step */
 int hfem_problem_set_prescribed(hfem_problem *p, const double *up);
 int hfem_problem_set_load(hfem_problem *p, const double *f_free, const double *f_presc);

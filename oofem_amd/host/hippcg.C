@@ -24,6 +24,13 @@ void HipPCGSolver::applyOptions()
     hipfemCheck(hippcg_set_reliable(s, reldelta), "hippcg_set_reliable");
 }
 
+void HipPCGSolver::setTolerance(double t)
+{
+    if (!(t > 0.0)) OOFEM_ERROR("tolerance must be positive, got %g", t);
+    tol = t;
+    hipfemCheck(hippcg_set_options(s, tol, maxite, precond), "hippcg_set_options");
+}
+
 // IMLSolver::initializeFrom imlsolver.C:67-101 keywords and defaults
 void HipPCGSolver::initializeFrom(InputRecord &ir)
 {

@@ -36,6 +36,9 @@ public:
     SparseMtrxType giveRecommendedMatrix(bool symmetric) const override { return SMT_HipCSR; }
     const char *giveClassName() const override { return "HipPCGSolver"; }
 
+    // relative residual tolerance of the following solves (lstol)
+    void setTolerance(double t);
+    double giveTolerance() const { return tol; }
     int giveLastIterations() const { return lastIterations; }
     double giveLastResidual() const { return lastResidual; }
     hippcg_solver *handle() { return s; }

@@ -61,6 +61,16 @@ int hfem_problem_set_options(hfem_problem *p, int smtype, int lstype, double lst
     });
 }
 
+int hfem_problem_set_forcing(hfem_problem *p, int on, double eta_max)
+{
+    return guarded([&] {
+        if (on && !(eta_max > 0.0 && eta_max < 1.0)) OOFEM_ERROR("forcing term bound must be in (0, 1), got %g", eta_max);
+        auto *P = reinterpret_cast<StructuralProblem *>(p);
+        P->nr.forcing = on != 0;
+        if (on) P->nr.forcingMax = eta_max;
+    });
+}
+
 int hfem_problem_set_initial_guess(hfem_problem *p, int type)
 {
     return guarded([&] {

@@ -195,6 +195,9 @@ NRStepReport StructuralProblem::solveNonlinearStep()
     // the matrix of the initial guess is that same tangent (no Gauss-point
     // state changed since), so it is reused
     if (!tangentCurrent) assemble(*K, 1);
+    auto *pcg = dynamic_cast<HipPCGSolver *>(linSolver.get());
+    const double lstol = pcg ? pcg->giveTolerance() : 0.0;
+    double prevErr = -1.0, prevEta = -1.0;
     for (int nite = 0;; ++nite) {
         double ebe[3], sums[6];
         internalForces(X, true, ebe);               // updateComponent(InternalRhs)
@@ -219,11 +222,26 @@ NRStepReport StructuralProblem::solveNonlinearStep()
         if (nite > 0 || !nr.calcStiffBeforeRes) {
             if (nr.manrmsteps > 0 && nite % nr.manrmsteps == 0) assemble(*K, 1);
         }
+        if (nr.forcing && pcg) {
+            double eta = nr.forcingMax;
+            if (prevErr > 0.0) {
+                eta = 0.9 * (maxErr / prevErr) * (maxErr / prevErr);
+                const double keep = 0.9 * prevEta * prevEta;
+                if (keep > 0.1) eta = std::max(eta, keep);
+                eta = std::min(eta, nr.forcingMax);
+            }
+            if (maxErr > 0.0) eta = std::max(eta, 0.1 * nr.rtolf / maxErr);
+            eta = std::min(std::max(eta, lstol), std::max(nr.forcingMax, lstol));
+            pcg->setTolerance(eta);
+            prevEta = eta;
+        }
+        prevErr = maxErr;
         ConvergedReason ls = linearSolve(rhs, ddX);  // ddX carries the previous correction
         rep.linearIterations += lastLinearIterations;
         if (ls != CR_CONVERGED) { rep.status = CR_FAILED; break; }
         X.axpby(1.0, &ddX, 1.0);
     }
+    if (nr.forcing && pcg) pcg->setTolerance(lstol);
     return rep;
 }
 

@@ -51,6 +51,15 @@ struct NRSolverOptions {
     int minIterations = 0;
     int manrmsteps = 0;        // 0: modified NR (default), k: tangent every k iterations
     bool calcStiffBeforeRes = true;
+    // inexact Newton (no counterpart in NRSolver, which solves every
+    // correction to lstol): the correction of iteration k is solved to the
+    // forcing term eta_k = min(forcingMax, max(0.9 (e_k / e_{k-1})^2,
+    // 0.9 eta_{k-1}^2 when that exceeds 0.1)), at least
+    // 0.1 rtolf / e_k (no tighter than the step's criterion can use) and
+    // at least lstol; e_k is the force error of the convergence check.
+    // The convergence check itself is unchanged.
+    bool forcing = false;
+    double forcingMax = 0.1;
 };
 
 // StaticStructural initial guess (_IFT_EngngModel_initialGuess): IG_None = 0,

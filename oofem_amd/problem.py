@@ -40,6 +40,7 @@ def hostlib():
         L.hfem_problem_destroy.argtypes = [C.c_void_p]
         L.hfem_problem_set_options.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int,
                                                C.c_double, C.c_double, C.c_int, C.c_int]
+        L.hfem_problem_set_forcing.argtypes = [C.c_void_p, C.c_int, C.c_double]
         _hl = L
     return _hl
 
@@ -55,7 +56,10 @@ def _p(a):
 
 class StructuralProblem:
     def __init__(self, deck, smtype=SMT_HipCSR, lstype=ST_HipPCG, lstol=1e-10, lsiter=100000, lsprecond=1,
-                 mixed=0, irtol=1e-4, rtolf=1e-8, maxiter=50, manrmsteps=1, initial_guess=1):
+                 mixed=0, irtol=1e-4, rtolf=1e-8, maxiter=50, manrmsteps=1, initial_guess=1, forcing=0.0):
+        """forcing > 0: inexact Newton, corrections solved to forcing terms
+        of at most this value (hfem_problem_set_forcing); 0: every
+        correction to lstol, as the reference's NRSolver."""
         self.deck = d = deck
         self._keep = [np.ascontiguousarray(x) for x in (d.coords, d.conn, d.emat, d.mat, d.eqn)]
         c, cn, em, mt, eq = self._keep
@@ -65,6 +69,8 @@ class StructuralProblem:
         _check(hostlib().hfem_problem_set_options(self.h, smtype, lstype, lstol, lsiter, lsprecond, mixed, irtol,
                                                   rtolf, maxiter, manrmsteps), "set_options")
         _check(hostlib().hfem_problem_set_initial_guess(self.h, int(initial_guess)), "set_initial_guess")
+        _check(hostlib().hfem_problem_set_forcing(self.h, int(forcing > 0), float(forcing) if forcing > 0 else 0.1),
+               "set_forcing")
 
     def close(self):
         if self.h:

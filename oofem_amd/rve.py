@@ -72,6 +72,12 @@ class _Chunk:
         self.b = DeviceVector(ctx, max(1, 6 * self.n))
         self.x = DeviceVector(ctx, max(1, 6 * self.n))
         self.up = [meshgen.affine_values(d, e) for e in np.eye(6)]
+        # prescribed dofs: reaction index, RVE, component and the node's
+        # position relative to the cell centre
+        node, comp = np.nonzero(d.eqn < 0)
+        self.p_idx = -d.eqn[node, comp] - 1
+        self.p_bin = (np.searchsorted(self.node_off, node, side="right") - 1) * 3 + comp
+        self.p_xc = d.coords[node] - 0.5
 
     def rhs(self):
         """b_k = -f_int(free 0, prescribed affine_k) for the six unit strains, interleaved."""
@@ -91,20 +97,17 @@ class _Chunk:
         return it, upd, res
 
     def homogenized(self):
-        """6x6 stiffness per RVE from the reactions of the six solves."""
+        """6x6 stiffness per RVE from the reactions of the six solves: per RVE
+        the sum over its prescribed dofs of R (x) (x - x_c), symmetrised (the
+        unit cell has volume 1)."""
         X = self.x.download().reshape(self.n, 6)
-        d = self.deck
-        m = d.eqn < 0
-        node, comp = np.nonzero(m)
-        Xc = d.coords - 0.5
-        out = np.zeros((len(self.ids), 6, 6))
+        nrve = len(self.ids)
+        out = np.zeros((nrve, 6, 6))
         for k in range(6):
-            _, fp, _ = self.ev.internal_forces(X[:, k], self.up[k])
-            R = np.zeros_like(d.coords)
-            R[node, comp] = fp[-d.eqn[m] - 1]
-            # per-RVE sum over its node range of R (x) (x - x_c)
-            P = R[:, :, None] * Xc[:, None, :]
-            S = np.add.reduceat(P, self.node_off[:-1], axis=0)
+            _, fp, _ = self.ev.internal_forces(np.ascontiguousarray(X[:, k]), self.up[k])
+            r = fp[self.p_idx]
+            S = np.stack([np.bincount(self.p_bin, weights=r * self.p_xc[:, j], minlength=3 * nrve)
+                          for j in range(3)], axis=1).reshape(nrve, 3, 3)
             S = 0.5 * (S + np.transpose(S, (0, 2, 1)))
             out[:, :, k] = np.stack([S[:, i, j] for (i, j) in meshgen.VOIGT_PAIRS], axis=1)
         return out

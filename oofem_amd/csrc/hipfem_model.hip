@@ -729,12 +729,14 @@ __global__ void k_fill_d(int n, double *__restrict__ a, const PcgState *__restri
 // elements in FP64 (neighbouring element forces of a smooth field nearly
 // cancel at a node, by a factor of about h/L, so an FP32 sum would lose that
 // many digits).
-// 8 threads per element (one per Gauss point), 32 elements per block.  Node
-// data are staged in LDS relative to node 1 (coordinates differenced in FP64,
-// displacements in FP32: a rigid translation adds nothing to the strain), the
-// Gauss-point forces B^T sigma dV are summed over the points in ascending
-// order and each of the 8 threads handles 3 of the 24 components: either
-// y += f over one color (elems = the color's list), or, with ebuf, the
+// 8 threads per element (one per Gauss point, consecutive lanes of one
+// wave), 32 elements per block.  Node data are staged in LDS relative to node
+// 1 (coordinates differenced in FP64, displacements in FP32: a rigid
+// translation adds nothing to the strain).  The 24 Gauss-point forces
+// B^T sigma dV of each thread are summed over the 8 points in registers by a
+// reduce-scatter butterfly (xor 4, 2, 1: 12 + 6 + 3 exchanges, a fixed
+// order), after which thread g holds the forces of the element's node g:
+// either y += f over one color (elems = the color's list), or, with ebuf, the
 // element's 24 forces are stored (elems = NULL: every element, one launch)
 // for k_mf_gather to sum per node.
 constexpr int MF_EPB = 32;
@@ -748,13 +750,12 @@ __global__ __launch_bounds__(256) void k_mf_hex(const int *__restrict__ elems, i
 {
     __shared__ float sX[MF_EPB][8][3];
     __shared__ float sU[MF_EPB][8][3];
-    __shared__ int sQ[MF_EPB][8][3];
-    __shared__ float sF[MF_EPB][8][25];
     if (st && halted(st)) return;
     const int t = threadIdx.x, le = t >> 3, g = t & 7;
     const int ei = blockIdx.x * MF_EPB + le;
     const bool active = ei < nel;
     int e = 0;
+    int myq[3] = {0, 0, 0};   // equations of this thread's node g
     if (active) {
         e = elems ? elems[ei] : ei;
         const int node = conn[(size_t)e * 8 + g], node0 = conn[(size_t)e * 8];
@@ -762,11 +763,17 @@ __global__ __launch_bounds__(256) void k_mf_hex(const int *__restrict__ elems, i
         for (int d = 0; d < 3; d++) {
             sX[le][g][d] = (float)(coords[3 * (size_t)node + d] - coords[3 * (size_t)node0 + d]);
             const int q = eqn[3 * (size_t)node + d];
-            sQ[le][g][d] = q;
+            myq[d] = q;
             sU[le][g][d] = q > 0 ? x[q - 1] : 0.0f;
         }
     }
     __syncthreads();
+    // the 8 threads of an element are consecutive lanes of one wave and all
+    // take the same branch (active depends on the element only), so the
+    // butterfly below exchanges within complete 8-lane groups
+    float fr[24];
+#pragma unroll
+    for (int k = 0; k < 24; k++) fr[k] = 0.f;
     if (active) {
         const float a = 0.577350269189626f;
         const float xi0 = (g & 4) ? a : -a, xi1 = (g & 2) ? a : -a, xi2 = (g & 1) ? a : -a;
@@ -865,21 +872,37 @@ __global__ __launch_bounds__(256) void k_mf_hex(const int *__restrict__ elems, i
         for (int n = 0; n < 8; n++)
 #pragma unroll
             for (int i = 0; i < 3; i++)
-                sF[le][g][3 * n + i] = T[3 * i] * dN[n][0] + T[3 * i + 1] * dN[n][1] + T[3 * i + 2] * dN[n][2];
+                fr[3 * n + i] = T[3 * i] * dN[n][0] + T[3 * i + 1] * dN[n][1] + T[3 * i + 2] * dN[n][2];
     }
-    __syncthreads();
+    // reduce-scatter over the 8 Gauss points: at the step with mask m the
+    // thread keeps the half of its current components selected by (g & m)
+    // and adds the partner's copy of that half; after m = 4, 2, 1 thread g
+    // holds components 12 b2 + 6 b1 + 3 b0 + (0..2) = 3 g + (0..2)
+    {
+        const bool h4 = g & 4, h2 = g & 2, h1 = g & 1;
+#pragma unroll
+        for (int j = 0; j < 12; j++) {
+            const float keep = h4 ? fr[12 + j] : fr[j], send = h4 ? fr[j] : fr[12 + j];
+            fr[j] = keep + __shfl_xor(send, 4, 64);
+        }
+#pragma unroll
+        for (int j = 0; j < 6; j++) {
+            const float keep = h2 ? fr[6 + j] : fr[j], send = h2 ? fr[j] : fr[6 + j];
+            fr[j] = keep + __shfl_xor(send, 2, 64);
+        }
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const float keep = h1 ? fr[3 + j] : fr[j], send = h1 ? fr[j] : fr[3 + j];
+            fr[j] = keep + __shfl_xor(send, 1, 64);
+        }
+    }
     if (active) {
 #pragma unroll
-        for (int k = 0; k < 3; k++) {
-            const int cidx = g + 8 * k;   // component 3 n + i
-            const int q = sQ[le][cidx / 3][cidx % 3];
-            float f = 0.f;
-#pragma unroll
-            for (int gg = 0; gg < 8; gg++) f += sF[le][gg][cidx];
+        for (int i = 0; i < 3; i++) {
             if (ebuf)
-                ebuf[(size_t)e * 24 + cidx] = f;
-            else if (q > 0)
-                y[q - 1] += (double)f;
+                ebuf[(size_t)e * 24 + 3 * g + i] = fr[i];
+            else if (myq[i] > 0)
+                y[myq[i] - 1] += (double)fr[i];
         }
     }
 }

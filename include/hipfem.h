@@ -81,6 +81,8 @@ int hipfem_dev_axpby(hipfem_ctx *ctx, int64_t n, double a, const double *x, doub
 int hipfem_dev_copy(hipfem_ctx *ctx, int64_t n, const double *x, double *y, int method);
 /* dot product x.y and 2-norm, FP64, fixed summation order */
 int hipfem_dev_dot(hipfem_ctx *ctx, int64_t n, const double *x, const double *y, double *out);
+/* max |x_i| (NRSolver's increment cap, nrsolver.C:340-353) */
+int hipfem_dev_amax(hipfem_ctx *ctx, int64_t n, const double *x, double *out);
 
 /* ------------------------------------------------------------------- hipcsr */
 /* SparseMtrx constructor / clone()  (sparsemtrx.h:95-99) */
@@ -250,6 +252,10 @@ int hfem_model_commit(hfem_model *md);
  * color adding into the FP64 result (default: measured faster on MI355X,
  * 0.27 vs 0.34 ms for 1M elements) */
 int hfem_model_set_element_operator(hfem_model *md, int form);
+/* Element::initForNewStep for every element (element.C:878-887, called by
+ * EngngModel::initStepIncrements before a restarted step): each Gauss
+ * point's temp values back to its committed ones */
+int hfem_model_restart(hfem_model *md);
 /* reset every Gauss point to the virgin state */
 int hfem_model_reset_state(hfem_model *md);
 /* Gauss point state, ngp_total x 38 doubles in the oracle's layout */

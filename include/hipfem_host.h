@@ -42,6 +42,31 @@ int hfem_problem_set_initial_guess(hfem_problem *p, int type);
  * (Eisenstat-Walker choice 2 from the force-error history, never tighter
  * than 0.1 rtolf / error); the convergence check is unchanged. */
 int hfem_problem_set_forcing(hfem_problem *p, int on, double eta_max);
+/* NRSolver keywords `linesearch`, `lsearchtol`, `lsearchamp`,
+ * `lsearchmaxeta` (nrsolver.h:63, linesearch.h:45-47; defaults 0.8, 2.5,
+ * 4.0) and `maxinc` (nrsolver.h:70; 1e20 = off).  The line search ends at
+ * its own point X + eta dX; the reference's NRSolver::solve adds the scaled
+ * correction once more afterwards (nrsolver.C:356). */
+int hfem_problem_set_nr_extras(hfem_problem *p, int linesearch, double ls_tol, double ls_amp, double ls_maxeta,
+                               double maxinc);
+/* StaticStructural time stepping from the current time to final_time
+ * (EngngModel::solveYourself engngm.C:573-662): prescribed values and
+ * external load at time t are up_ref / f_free / f_presc times the
+ * piecewise-linear load-time function through (ltf_t, ltf_v) (nltf = 0:
+ * t / final_time; f_free NULL: no external load).  reduction = 0 is
+ * NoReductionStrategy; 1 is SimpleReductionStrategy
+ * (timestepreductionstrategy.C:70-110: dtmin, dtmax, ncrf, nmaxrestarts,
+ * minrequirediter, maxrequirediter): a step whose Newton loop fails is
+ * restarted from the last converged state (temp Gauss-point values reset)
+ * with dt * ncrf, and dt adapts to each converged step's Newton iterations.
+ * Per accepted step (up to max_records): rec_time, rec_dt (the step's
+ * increment) and rec_int[6] = {newton, pcg, status, restarts,
+ * line searches, 0}; *nrec = steps taken. */
+int hfem_problem_run_steps(hfem_problem *p, const double *up_ref, const double *f_free, const double *f_presc,
+                           int nltf, const double *ltf_t, const double *ltf_v, double final_time, double dt,
+                           int reduction, double dtmin, double dtmax, double ncrf, int nmaxrestarts,
+                           int min_required_iter, int max_required_iter, int max_records, double *rec_time,
+                           double *rec_dt, int *rec_int, int *nrec);
 /* prescribed values (npeq) and total external load (neq, npeq) of the next step */
 int hfem_problem_set_prescribed(hfem_problem *p, const double *up);
 int hfem_problem_set_load(hfem_problem *p, const double *f_free, const double *f_presc);

@@ -522,6 +522,31 @@ __global__ __launch_bounds__(256) void k_dot_part(int64_t n, const double *__res
     if (threadIdx.x == 0) part[blockIdx.x] = sh[0] + sh[1] + sh[2] + sh[3];
 }
 
+// max |x_i|: per-block maxima, then one block over them
+__global__ void k_amax_part(int64_t n, const double *__restrict__ x, double *__restrict__ part)
+{
+    __shared__ double sh[4];
+    double m = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) m = fmax(m, fabs(x[i]));
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+    if (threadIdx.x % 64 == 0) sh[threadIdx.x / 64] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+}
+
+__global__ void k_amax_final(int nb, const double *__restrict__ part, double *__restrict__ out)
+{
+    __shared__ double sh[4];
+    double m = 0.0;
+    for (int i = threadIdx.x; i < nb; i += 256) m = fmax(m, part[i]);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+    if (threadIdx.x % 64 == 0) sh[threadIdx.x / 64] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) out[0] = fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
+}
+
 __global__ void k_dot_final(int nb, const double *__restrict__ part, double *__restrict__ out)
 {
     __shared__ double sh[4];
@@ -627,6 +652,25 @@ extern "C" int hipfem_dev_dot(hipfem_ctx *c, int64_t n, const double *x, const d
     if (n) {
         k_dot_part<<<nb, 256, 0, c->stream>>>(n, x, y, c->d_partials);
         k_dot_final<<<1, 256, 0, c->stream>>>(nb, c->d_partials, c->d_partials + nb);
+        HIPFEM_CHECK(hipGetLastError());
+        HIPFEM_CHECK(hipMemcpyAsync(c->h_pinned, c->d_partials + nb, sizeof(double), hipMemcpyDeviceToHost, c->stream));
+        HIPFEM_CHECK(hipStreamSynchronize(c->stream));
+        *out = c->h_pinned[0];
+    } else {
+        *out = 0.0;
+    }
+    return 0;
+}
+extern "C" int hipfem_dev_amax(hipfem_ctx *c, int64_t n, const double *x, double *out)
+{
+    HIPFEM_REQUIRE(c && out && (n == 0 || x), HIPFEM_EINVAL, "dev_amax: bad arguments");
+    DeviceGuard g(c->device);
+    const int nb = grid_for(n, 256, 1024);
+    int rc = ensure_partials(c, (size_t)nb + 1);
+    if (rc) return rc;
+    if (n) {
+        k_amax_part<<<nb, 256, 0, c->stream>>>(n, x, c->d_partials);
+        k_amax_final<<<1, 256, 0, c->stream>>>(nb, c->d_partials, c->d_partials + nb);
         HIPFEM_CHECK(hipGetLastError());
         HIPFEM_CHECK(hipMemcpyAsync(c->h_pinned, c->d_partials + nb, sizeof(double), hipMemcpyDeviceToHost, c->stream));
         HIPFEM_CHECK(hipStreamSynchronize(c->stream));

@@ -1017,6 +1017,24 @@ __global__ void k_commit(int ngp_total, double *__restrict__ state)
     state[6 * n + g] = state[13 * n + g];
 }
 
+// MisesMatStatus::initTempStatus (misesmat.C:707-715 with
+// StructuralMaterialStatus::initTempStatus, structuralms.C:121-141): temp
+// plastic strain, kappa and stress back to the committed values, trial
+// deviator cleared.  The temp strain has no committed copy here; the next
+// stress evaluation rewrites it before anything reads it.
+__global__ void k_restart(int ngp_total, double *__restrict__ state)
+{
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ngp_total) return;
+    const size_t n = ngp_total;
+    for (int i = 0; i < 6; i++) {
+        state[(7 + i) * n + g] = state[i * n + g];
+        state[(14 + i) * n + g] = 0.0;
+        state[(20 + i) * n + g] = state[(32 + i) * n + g];
+    }
+    state[13 * n + g] = state[6 * n + g];
+}
+
 }  // namespace hipfem
 
 using namespace hipfem;
@@ -1646,6 +1664,19 @@ extern "C" int hfem_model_commit(hfem_model *md)
         k_commit<<<(md->ngp_total + 255) / 256, 256, 0, md->ctx->stream>>>(md->ngp_total, md->d_state);
         HIPFEM_CHECK(hipGetLastError());
         // the committed state's tangent is MisesMat's own (temp = committed)
+        if (md->d_sflag) HIPFEM_CHECK(hipMemsetAsync(md->d_sflag, 0, (size_t)md->ngp_total, md->ctx->stream));
+        if (md->d_srej) HIPFEM_CHECK(hipMemsetAsync(md->d_srej, 0, (size_t)md->ngp_total, md->ctx->stream));
+    }
+    return 0;
+}
+
+extern "C" int hfem_model_restart(hfem_model *md)
+{
+    HIPFEM_REQUIRE(md && md->d_state, HIPFEM_EINVAL, "restart: no mesh");
+    DeviceGuard g(md->ctx->device);
+    if (md->ngp_total) {
+        k_restart<<<(md->ngp_total + 255) / 256, 256, 0, md->ctx->stream>>>(md->ngp_total, md->d_state);
+        HIPFEM_CHECK(hipGetLastError());
         if (md->d_sflag) HIPFEM_CHECK(hipMemsetAsync(md->d_sflag, 0, (size_t)md->ngp_total, md->ctx->stream));
         if (md->d_srej) HIPFEM_CHECK(hipMemsetAsync(md->d_srej, 0, (size_t)md->ngp_total, md->ctx->stream));
     }

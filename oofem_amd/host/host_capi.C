@@ -106,6 +106,64 @@ int hfem_problem_nonlinear_step(hfem_problem *p, int *out3, double *force_error)
     });
 }
 
+int hfem_problem_set_nr_extras(hfem_problem *p, int linesearch, double ls_tol, double ls_amp, double ls_maxeta,
+                               double maxinc)
+{
+    return guarded([&] {
+        if (!(maxinc > 0.0)) OOFEM_ERROR("maxinc must be positive, got %g", maxinc);
+        auto &nr = reinterpret_cast<StructuralProblem *>(p)->nr;
+        nr.lineSearch = linesearch != 0;
+        nr.lsTol = ls_tol;
+        nr.lsAmp = ls_amp;
+        nr.lsMaxEta = ls_maxeta;
+        nr.maxInc = maxinc;
+    });
+}
+
+int hfem_problem_run_steps(hfem_problem *p, const double *up_ref, const double *f_free, const double *f_presc,
+                           int nltf, const double *ltf_t, const double *ltf_v, double final_time, double dt,
+                           int reduction, double dtmin, double dtmax, double ncrf, int nmaxrestarts,
+                           int min_required_iter, int max_required_iter, int max_records, double *rec_time,
+                           double *rec_dt, int *rec_int, int *nrec)
+{
+    return guarded([&] {
+        auto *P = reinterpret_cast<StructuralProblem *>(p);
+        if (nltf < 0 || (nltf > 0 && (!ltf_t || !ltf_v))) OOFEM_ERROR("load time function: bad arguments");
+        if (P->giveNpeq() > 0 && !up_ref) OOFEM_ERROR("reference prescribed values missing");
+        ReductionStrategy rs;
+        rs.on = reduction != 0;
+        if (rs.on) {
+            if (!(dtmin >= 0.0) || !(dtmax >= dtmin) || !(ncrf > 0.0 && ncrf < 1.0) || nmaxrestarts < 0)
+                OOFEM_ERROR("reduction strategy: bad values (dtmin %g, dtmax %g, ncrf %g, nmaxrestarts %d)", dtmin,
+                            dtmax, ncrf, nmaxrestarts);
+            rs.dtmin = dtmin;
+            rs.dtmax = dtmax;
+            rs.ncrf = ncrf;
+            rs.nmaxrestarts = nmaxrestarts;
+            rs.minRequiredIter = min_required_iter;
+            rs.maxRequiredIter = max_required_iter;
+        }
+        std::vector<double> tt(ltf_t, ltf_t + nltf), vv(ltf_v, ltf_v + nltf);
+        std::vector<double> zero((size_t)std::max(1, P->giveNpeq()), 0.0);
+        auto recs = P->solveTimeSteps(up_ref ? up_ref : zero.data(), f_free, f_presc, tt, vv, final_time, dt, rs);
+        const int n = (int)std::min<size_t>(recs.size(), (size_t)std::max(0, max_records));
+        for (int i = 0; i < n; i++) {
+            if (rec_time) rec_time[i] = recs[i].time;
+            if (rec_dt) rec_dt[i] = recs[i].dt;
+            if (rec_int) {
+                int *r = rec_int + 6 * i;
+                r[0] = recs[i].newton;
+                r[1] = recs[i].pcg;
+                r[2] = recs[i].status;
+                r[3] = recs[i].restarts;
+                r[4] = recs[i].lineSearches;
+                r[5] = 0;
+            }
+        }
+        *nrec = (int)recs.size();
+    });
+}
+
 int hfem_problem_update(hfem_problem *p)
 {
     return guarded([&] { reinterpret_cast<StructuralProblem *>(p)->updateYourself(); });

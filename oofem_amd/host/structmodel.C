@@ -38,6 +38,13 @@ double DeviceVector::dot(const DeviceVector &o) const
     return r;
 }
 
+double DeviceVector::amax() const
+{
+    double r = 0.0;
+    hipfemCheck(hipfem_dev_amax(ctx, n, p, &r), "dev_amax");
+    return r;
+}
+
 // ------------------------------------------------------- StructuralProblem
 StructuralProblem::StructuralProblem(int nnode_, const double *coords, int etype_, int nelem_, const int *conn_,
                                      const int *emat, int nmat_, const double *mat, const int *eqn_, int neq_,
@@ -50,7 +57,7 @@ StructuralProblem::StructuralProblem(int nnode_, const double *coords, int etype
     hipfemCheck(hfem_model_create(ctx, &ev), "hfem_model_create");
     hipfemCheck(hfem_model_set_mesh(ev, nnode, coords, etype, nelem, conn_, emat, nmat, mat, eqn_, neq, npeq),
                 "set_mesh");
-    for (DeviceVector *v : {&X, &RT, &F, &rhs, &ddX}) v->allocate(ctx, neq);
+    for (DeviceVector *v : {&X, &RT, &F, &rhs, &ddX, &rb, &gls, &Xc}) v->allocate(ctx, neq);
     for (DeviceVector *v : {&up, &upPrev, &dup, &RTp, &Fp}) v->allocate(ctx, npeq);
     // IMLSolver defaults unless the caller overrides them
     solverRecord.set("lsprecond", 1);
@@ -239,10 +246,180 @@ NRStepReport StructuralProblem::solveNonlinearStep()
         ConvergedReason ls = linearSolve(rhs, ddX);  // ddX carries the previous correction
         rep.linearIterations += lastLinearIterations;
         if (ls != CR_CONVERGED) { rep.status = CR_FAILED; break; }
-        X.axpby(1.0, &ddX, 1.0);
+        // increment cap (nrsolver.C:340-353), applied before the line search
+        // so that the search runs along the capped direction
+        if (nr.maxInc < 1.0e20) {
+            const double mx = ddX.amax();
+            if (mx > nr.maxInc) ddX.axpby(0.0, nullptr, nr.maxInc / mx);
+        }
+        if (nr.lineSearch && nite > 0) {
+            // X moves to the line search's point X_start + eta ddX (the
+            // reference's NRSolver then adds the scaled ddX once more,
+            // nrsolver.C:356; this driver keeps the line search's point)
+            lineSearch(rep.lineSearches);
+        } else {
+            X.axpby(1.0, &ddX, 1.0);
+        }
     }
     if (nr.forcing && pcg) pcg->setTolerance(lstol);
     return rep;
+}
+
+// LineSearchNM::search linesearch.C:163-245 (1-based arrays as there)
+static void lsSearch(int istep, double *prod, double *eta, double amp, double maxeta, double mineta, int &ico)
+{
+    int ineg = 0;
+    double etaneg = 1.0, etamax = 0.0;
+    for (int i = 1; i <= istep; i++) {
+        etamax = std::max(etamax, eta[i]);
+        if (prod[i] >= 0.0 || eta[i] >= etaneg) continue;
+        etaneg = eta[i];
+        ineg = i;
+    }
+    if (ineg) {
+        int ipos = 1;
+        for (int i = 1; i <= istep; i++) {
+            if (prod[i] <= 0.0 || eta[i] > eta[ineg] || eta[i] < eta[ipos]) continue;
+            ipos = i;
+        }
+        double etaint = (prod[ineg] * eta[ipos] - prod[ipos] * eta[ineg]) / (prod[ineg] - prod[ipos]);
+        etaint = std::max(etaint, eta[ipos] + 0.2 * (eta[ineg] - eta[ipos]));
+        if (etaint < mineta) {
+            etaint = mineta;
+            ico = ico == 1 ? 2 : 1;
+        }
+        eta[istep + 1] = etaint;
+        return;
+    }
+    const double etamaxstep = amp * etamax;
+    const double etaextrap = (prod[istep] * eta[istep - 1] - prod[istep - 1] * eta[istep]) / (prod[istep] - prod[istep - 1]);
+    eta[istep + 1] = (etaextrap <= 0.0 || etaextrap > etamaxstep) ? etamaxstep : etaextrap;
+    if (eta[istep + 1] > maxeta && ico == 1) {
+        ico = 2;
+        return;
+    }
+    if (eta[istep + 1] > maxeta) {
+        ico = 1;
+        eta[istep + 1] = maxeta;
+    }
+}
+
+// LineSearchNM::solve linesearch.C:54-160 along ddX from the current X (whose
+// residual is in rhs): X ends at X_start + eta ddX with the Gauss-point state
+// of that point; ddX is scaled by eta.  Returns eta.
+double StructuralProblem::lineSearch(int &lineSearches)
+{
+    const double tol = std::min(std::max(nr.lsTol, 0.6), 0.95);
+    const double amp = std::min(std::max(nr.lsAmp, 1.0), 10.0);
+    const double maxeta = std::min(std::max(nr.lsMaxEta, 1.5), 15.0);
+    const double s0 = -rhs.dot(ddX);
+    double etav = 1.0;
+    if (s0 < 0.0) {
+        rb.axpby(1.0, &X, 0.0);
+        constexpr int maxIter = 10;
+        double prod[maxIter + 2], eta[maxIter + 2];
+        prod[1] = 1.0;
+        eta[1] = 0.0;
+        eta[2] = 1.0;
+        int ico = 0;
+        bool ok = false;
+        for (int ils = 2; ils <= maxIter; ils++) {
+            X.axpby(1.0, &rb, 0.0);
+            X.axpby(eta[ils], &ddX, 1.0);
+            internalForces(X, true, nullptr);
+            double sums[6];
+            hipfemCheck(hfem_model_residual(ev, RT.ptr(), F.ptr(), gls.ptr(), sums), "residual");
+            const double si = -gls.dot(ddX) / s0;
+            prod[ils] = si;
+            if (std::fabs(si) < tol) {
+                etav = eta[ils];
+                ok = true;
+                break;
+            }
+            lsSearch(ils, prod, eta, amp, maxeta, 0.2, ico);
+            if (ico == 2) break;
+        }
+        if (!ok) etav = 1.0;
+        X.axpby(1.0, &rb, 0.0);
+    }
+    X.axpby(etav, &ddX, 1.0);
+    if (etav != 1.0) {
+        ddX.axpby(0.0, nullptr, etav);
+        lineSearches++;
+    }
+    return etav;
+}
+
+std::vector<StepRecord> StructuralProblem::solveTimeSteps(const double *up_ref, const double *f_free,
+                                                          const double *f_presc, const std::vector<double> &ltf_t,
+                                                          const std::vector<double> &ltf_v, double finalTime,
+                                                          double dt, const ReductionStrategy &rs)
+{
+    if (!(finalTime > 0.0) || !(dt > 0.0)) OOFEM_ERROR("time stepping needs a positive final time and increment");
+    if (ltf_t.size() != ltf_v.size()) OOFEM_ERROR("load time function: %d times, %d values", (int)ltf_t.size(),
+                                                  (int)ltf_v.size());
+    auto ltf = [&](double t) {
+        if (ltf_t.empty()) return t / finalTime;
+        if (t <= ltf_t.front()) return ltf_v.front();
+        for (size_t i = 1; i < ltf_t.size(); i++)
+            if (t <= ltf_t[i])
+                return ltf_v[i - 1] + (ltf_v[i] - ltf_v[i - 1]) * (t - ltf_t[i - 1]) / (ltf_t[i] - ltf_t[i - 1]);
+        return ltf_v.back();
+    };
+    std::vector<double> upv(npeq), ff(f_free ? neq : 0), fp(f_free && f_presc ? npeq : 0);
+    std::vector<StepRecord> out;
+    double t = currentTime;
+    while (t < finalTime * (1.0 - 1.0e-12)) {
+        Xc.axpby(1.0, &X, 0.0);
+        int restarts = 0;
+        double target = 0.0;
+        NRStepReport rep;
+        for (;;) {
+            target = t + dt;
+            const double f = ltf(target);
+            for (int i = 0; i < npeq; i++) upv[i] = up_ref[i] * f;
+            setPrescribedValues(upv.data());
+            if (f_free) {
+                for (int i = 0; i < neq; i++) ff[i] = f_free[i] * f;
+                for (size_t i = 0; i < fp.size(); i++) fp[i] = f_presc[i] * f;
+                setExternalLoad(ff.data(), fp.empty() ? nullptr : fp.data());
+            }
+            rep = solveNonlinearStep();
+            if (rep.status == CR_CONVERGED || !rs.on) break;
+            // ConvergenceException: restart with a reduced increment
+            // (engngm.C:620-633, MetaStep::reduceTimeStep metastep.C:138-146)
+            const double dt2 = dt * rs.ncrf;
+            if (dt2 < rs.dtmin) OOFEM_ERROR("Minimum time increment reached");
+            if (++restarts > rs.nmaxrestarts) OOFEM_ERROR("Maximum number of time step reductions has been reached.");
+            dt = dt2;
+            X.axpby(1.0, &Xc, 0.0);                       // StaticStructural::restartYourself
+            hipfemCheck(hfem_model_restart(ev), "restart");  // EngngModel::initStepIncrements
+        }
+        StepRecord r;
+        r.time = target;
+        r.dt = dt;
+        r.newton = rep.iterations;
+        r.pcg = rep.linearIterations;
+        r.status = (int)rep.status;
+        r.restarts = restarts;
+        r.lineSearches = rep.lineSearches;
+        out.push_back(r);
+        if (rs.on) {
+            // MetaStep::adaptTimeStep metastep.C:149-183 with
+            // SimpleReductionStrategy::giveReqIterTimeIncrementAdaptationFactor
+            const int n = rep.iterations;
+            double fac = 1.0;
+            if (n != 0 && n < rs.minRequiredIter) fac = 2.0;
+            else if (n != 0 && n > rs.maxRequiredIter) fac = 0.5;
+            dt *= fac < 1.0 ? std::min(1.0, fac) : std::max(1.0, fac);
+            dt = std::min(std::max(dt, rs.dtmin), rs.dtmax);
+        }
+        if (target + dt > finalTime) dt = finalTime - target;
+        updateYourself();
+        t = target;
+        currentTime = t;
+    }
+    return out;
 }
 
 void StructuralProblem::updateYourself()

@@ -43,6 +43,7 @@ public:
     void axpby(double a, const DeviceVector *x, double b);   // this = a x + b this
     void zero() { axpby(0.0, nullptr, 0.0); }
     double dot(const DeviceVector &o) const;
+    double amax() const;   // max |x_i|
 };
 
 struct NRSolverOptions {
@@ -60,6 +61,32 @@ struct NRSolverOptions {
     // The convergence check itself is unchanged.
     bool forcing = false;
     double forcingMax = 0.1;
+    // line search after every correction but the first (NRSolver
+    // `linesearch`, nrsolver.C:183-188, 318-324; LineSearchNM keywords
+    // lsearchtol / lsearchamp / lsearchmaxeta and defaults, linesearch.C:44-51,
+    // clamped as in LineSearchNM::initializeFrom, linesearch.C:247-283)
+    bool lineSearch = false;
+    double lsTol = 0.8, lsAmp = 2.5, lsMaxEta = 4.0;
+    // cap on the largest component of a correction (NRSolver `maxinc`,
+    // nrsolver.C:340-353)
+    double maxInc = 1.0e20;
+};
+
+// SimpleReductionStrategy (timestepreductionstrategy.C:40-110) of the
+// metastep: on == false is NoReductionStrategy (a step that does not
+// converge is kept, the increment never changes)
+struct ReductionStrategy {
+    bool on = false;
+    double dtmin = 0.0, dtmax = 1.0e300;   // dtmin / dtmax (required with on)
+    double ncrf = 0.5;                     // noConvergenceReductionFactor
+    int nmaxrestarts = 5;                  // numberOfMaxTimeStepReductions
+    int minRequiredIter = 3, maxRequiredIter = 8;
+};
+
+// one accepted time step of solveTimeSteps
+struct StepRecord {
+    double time = 0.0, dt = 0.0;
+    int newton = 0, pcg = 0, status = 0, restarts = 0, lineSearches = 0;
 };
 
 // StaticStructural initial guess (_IFT_EngngModel_initialGuess): IG_None = 0,
@@ -70,6 +97,7 @@ struct NRStepReport {
     int iterations = 0;        // Newton iterations
     int linearIterations = 0;  // PCG iterations summed over the step
     double forceError = 0.0;   // final max force error over dof groups
+    int lineSearches = 0;      // line searches that ended with eta != 1
     ConvergedReason status = CR_UNKNOWN;
 };
 
@@ -85,6 +113,8 @@ class StructuralProblem : public EngngModel {
     std::unique_ptr<SparseLinearSystemNM> linSolver;
     // state on the device
     DeviceVector X, up, upPrev, dup, RT, RTp, F, Fp, rhs, ddX;
+    DeviceVector rb, gls, Xc;   // line-search start and residual, last converged solution
+    double currentTime = 0.0;   // target time of the last accepted step of solveTimeSteps
     bool structureBuilt = false;
     int lastLinearIterations = 0;
     bool profiling = false;
@@ -94,6 +124,7 @@ class StructuralProblem : public EngngModel {
     void ensureStructure();
     void internalForces(const DeviceVector &x, bool updateState, double *ebe3);
     ConvergedReason linearSolve(const DeviceVector &b, DeviceVector &x);
+    double lineSearch(int &lineSearches);
 
 public:
     NRSolverOptions nr;
@@ -120,6 +151,16 @@ public:
 
     ConvergedReason solveLinearStatic(int &iterations, double &residual);
     NRStepReport solveNonlinearStep();
+    // StaticStructural time stepping (EngngModel::solveYourself engngm.C:573-662
+    // with the metastep's reduction strategy): from the current time to
+    // finalTime with initial increment dt; prescribed values and external
+    // load at time t are the reference ones times ltf(t), the piecewise-linear
+    // function through (ltf_t, ltf_v) (empty: t / finalTime).  f_free /
+    // f_presc may be null (no external load).  Every accepted step is
+    // committed (updateYourself).
+    std::vector<StepRecord> solveTimeSteps(const double *up_ref, const double *f_free, const double *f_presc,
+                                           const std::vector<double> &ltf_t, const std::vector<double> &ltf_v,
+                                           double finalTime, double dt, const ReductionStrategy &rs);
     void updateYourself();   // commit Gauss-point state (EngngModel::updateYourself)
 
     void giveSolution(double *u) const { X.download(u); }

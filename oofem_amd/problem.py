@@ -41,6 +41,11 @@ def hostlib():
         L.hfem_problem_set_options.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int,
                                                C.c_double, C.c_double, C.c_int, C.c_int]
         L.hfem_problem_set_forcing.argtypes = [C.c_void_p, C.c_int, C.c_double]
+        L.hfem_problem_set_nr_extras.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]
+        L.hfem_problem_run_steps.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
+                                             C.c_void_p, C.c_double, C.c_double, C.c_int, C.c_double, C.c_double,
+                                             C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
+                                             C.c_void_p, C.c_void_p]
         _hl = L
     return _hl
 
@@ -71,6 +76,46 @@ class StructuralProblem:
         _check(hostlib().hfem_problem_set_initial_guess(self.h, int(initial_guess)), "set_initial_guess")
         _check(hostlib().hfem_problem_set_forcing(self.h, int(forcing > 0), float(forcing) if forcing > 0 else 0.1),
                "set_forcing")
+
+    def set_nr_extras(self, linesearch=False, ls_tol=0.8, ls_amp=2.5, ls_maxeta=4.0, maxinc=1e20):
+        """NRSolver line search (linesearch / lsearchtol / lsearchamp /
+        lsearchmaxeta) and increment cap (maxinc)."""
+        _check(hostlib().hfem_problem_set_nr_extras(self.h, int(bool(linesearch)), ls_tol, ls_amp, ls_maxeta, maxinc),
+               "set_nr_extras")
+
+    def run_steps(self, up_ref, final_time, dt, reduction=None, f_ref=None, ltf=None, max_records=10000):
+        """StaticStructural time stepping to final_time (hfem_problem_run_steps).
+        reduction: None or dict(dtmin, dtmax, ncrf=0.5, nmaxrestarts=5,
+        minrequirediter=3, maxrequirediter=8); f_ref: (f_free, f_presc) or
+        None; ltf: (times, values) or None (t / final_time).  Returns a list
+        of dict(time, dt, newton, pcg, status, restarts, line_searches)."""
+        d = self.deck
+        up = np.ascontiguousarray(up_ref, dtype=np.float64) if d.npeq else None
+        ff = fp = None
+        if f_ref is not None:
+            ff = np.ascontiguousarray(f_ref[0], dtype=np.float64)
+            fp = np.ascontiguousarray(f_ref[1], dtype=np.float64) if f_ref[1] is not None and d.npeq else None
+        tt = vv = None
+        nl = 0
+        if ltf is not None:
+            tt = np.ascontiguousarray(ltf[0], dtype=np.float64)
+            vv = np.ascontiguousarray(ltf[1], dtype=np.float64)
+            nl = len(tt)
+        red = dict(dtmin=0.0, dtmax=1e300, ncrf=0.5, nmaxrestarts=5, minrequirediter=3, maxrequirediter=8)
+        if reduction is not None:
+            red.update(reduction)
+        rt = np.zeros(max_records)
+        rd = np.zeros(max_records)
+        ri = np.zeros((max_records, 6), dtype=np.int32)
+        n = C.c_int(0)
+        _check(hostlib().hfem_problem_run_steps(self.h, _p(up), _p(ff), _p(fp), nl, _p(tt), _p(vv), float(final_time),
+                                                float(dt), int(reduction is not None), float(red["dtmin"]),
+                                                float(red["dtmax"]), float(red["ncrf"]), int(red["nmaxrestarts"]),
+                                                int(red["minrequirediter"]), int(red["maxrequirediter"]), max_records,
+                                                _p(rt), _p(rd), _p(ri), C.byref(n)), "run_steps")
+        return [dict(time=float(rt[i]), dt=float(rd[i]), newton=int(ri[i, 0]), pcg=int(ri[i, 1]),
+                     status=int(ri[i, 2]), restarts=int(ri[i, 3]), line_searches=int(ri[i, 4]))
+                for i in range(min(n.value, max_records))]
 
     def close(self):
         if self.h:

@@ -716,6 +716,24 @@ void orc_state_commit(const orc_mesh *m, double *state)
     }
 }
 
+/* MisesMatStatus::initTempStatus misesmat.C:707-715 (+ structuralms.C:121-141)
+ * for every Gauss point: temp plastic strain, kappa and stress := committed,
+ * trial deviator cleared (the temp strain has no committed copy; the next
+ * stress evaluation rewrites it) */
+void orc_state_restart(const orc_mesh *m, double *state)
+{
+    const size_t n = (size_t)m->nelem * orc_ngp(m->etype);
+    for (size_t g = 0; g < n; g++) {
+        double *st = state + ORC_GPSTATE * g;
+        for (int i = 0; i < 6; i++) {
+            st[7 + i] = st[i];
+            st[14 + i] = 0.0;
+            st[20 + i] = st[32 + i];
+        }
+        st[13] = st[6];
+    }
+}
+
 int orc_internal_forces(const orc_mesh *m, const double *u_free, const double *u_presc,
                         double *state, int update_state,
                         double *f_free, double *f_presc, double *ebe_norm3)
@@ -913,9 +931,81 @@ int orc_extrapolated_forces(const orc_mesh *m, const double *state, const double
  * out[1] CG iterations (initial guess included), out[2] status (1
  * converged, -1 maxiter, -2 out of range).  Returns the final max force
  * error in *err.  u_presc_prev may be NULL (all zero, the first step). */
+/* LineSearchNM::search linesearch.C:163-245: next trial step length from
+ * the ratios prod[1..istep] at eta[1..istep] (1-based, as the reference) */
+static void orc_ls_search(int istep, double *prod, double *eta, double amp, double maxeta, double mineta, int *ico)
+{
+    int ineg = 0;
+    double etaneg = 1.0, etamax = 0.0;
+    for (int i = 1; i <= istep; i++) {
+        if (eta[i] > etamax) etamax = eta[i];
+        if (prod[i] >= 0.0) continue;
+        if (eta[i] >= etaneg) continue;
+        etaneg = eta[i];
+        ineg = i;
+    }
+    if (ineg) {
+        int ipos = 1;
+        for (int i = 1; i <= istep; i++) {
+            if (prod[i] <= 0.0) continue;
+            if (eta[i] > eta[ineg]) continue;
+            if (eta[i] < eta[ipos]) continue;
+            ipos = i;
+        }
+        double etaint = (prod[ineg] * eta[ipos] - prod[ipos] * eta[ineg]) / (prod[ineg] - prod[ipos]);
+        const double etaalt = eta[ipos] + 0.2 * (eta[ineg] - eta[ipos]);
+        if (etaalt > etaint) etaint = etaalt;
+        if (etaint < mineta) {
+            etaint = mineta;
+            *ico = (*ico == 1) ? 2 : 1;
+        }
+        eta[istep + 1] = etaint;
+        return;
+    }
+    const double etamaxstep = amp * etamax;
+    const double etaextrap = (prod[istep] * eta[istep - 1] - prod[istep - 1] * eta[istep]) / (prod[istep] - prod[istep - 1]);
+    eta[istep + 1] = etaextrap;
+    if (etaextrap <= 0.0 || etaextrap > etamaxstep) eta[istep + 1] = etamaxstep;
+    if (eta[istep + 1] > maxeta && *ico == 1) {
+        *ico = 2;
+        return;
+    }
+    if (eta[istep + 1] > maxeta) {
+        *ico = 1;
+        eta[istep + 1] = maxeta;
+    }
+}
+
 int orc_nonlinear_step(const orc_mesh *m, double *state, double *X, const double *u_presc,
                        const double *u_presc_prev, int initial_guess, const double *RT,
                        double rtolf, int maxiter, int manrmsteps, double lstol, int *out, double *err)
+{
+    int o4[4];
+    const int rc = orc_nonlinear_step_ex(m, state, X, u_presc, u_presc_prev, initial_guess, RT, rtolf, maxiter,
+                                         manrmsteps, lstol, 0, 0.8, 2.5, 4.0, 1.0e20, o4, err);
+    out[0] = o4[0];
+    out[1] = o4[1];
+    out[2] = o4[2];
+    return rc;
+}
+
+/* out-of-balance product -(RT - F(X + eta dr)) . dr with the state updated
+ * (LineSearchNM::solve linesearch.C:94-118) */
+static int orc_ls_product(const orc_mesh *m, double *state, const double *rb, const double *dr, double eta,
+                          const double *u_presc, const double *RT, double *X, double *F, double *prod)
+{
+    for (int i = 0; i < m->neq; i++) X[i] = rb[i] + eta * dr[i];
+    if (orc_internal_forces(m, X, u_presc, state, 1, F, NULL, NULL)) return -11;
+    double s = 0.0;
+    for (int i = 0; i < m->neq; i++) s -= ((RT ? RT[i] : 0.0) - F[i]) * dr[i];
+    *prod = s;
+    return 0;
+}
+
+int orc_nonlinear_step_ex(const orc_mesh *m, double *state, double *X, const double *u_presc,
+                          const double *u_presc_prev, int initial_guess, const double *RT, double rtolf, int maxiter,
+                          int manrmsteps, double lstol, int linesearch, double ls_tol, double ls_amp, double ls_maxeta,
+                          double maxinc, int *out, double *err)
 {
     const long long nnz = orc_csr_count(m);
     long long *rowptr = (long long *)malloc(sizeof(long long) * ((size_t)m->neq + 1));
@@ -928,8 +1018,16 @@ int orc_nonlinear_step(const orc_mesh *m, double *state, double *X, const double
     int inuse[3] = {0, 0, 0}, rc = 0;
     for (long long i = 0; i < 3LL * m->nnode; i++)
         if (m->eqn[i] > 0) { dofid[m->eqn[i] - 1] = (int)(i % 3); inuse[i % 3]++; }
+    double *rb = (double *)malloc(sizeof(double) * ((size_t)m->neq + 1));
     orc_csr_fill(m, rowptr, colind);
-    out[0] = out[1] = out[2] = 0;
+    out[0] = out[1] = out[2] = out[3] = 0;
+    /* LineSearchNM::initializeFrom clamps (linesearch.C:247-283) */
+    if (ls_tol < 0.6) ls_tol = 0.6;
+    if (ls_tol > 0.95) ls_tol = 0.95;
+    if (ls_amp < 1.0) ls_amp = 1.0;
+    if (ls_amp > 10.0) ls_amp = 10.0;
+    if (ls_maxeta < 1.5) ls_maxeta = 1.5;
+    if (ls_maxeta > 15.0) ls_maxeta = 15.0;
     if (initial_guess && m->npeq > 0) {
         double *du = (double *)malloc(sizeof(double) * (size_t)m->npeq);
         for (int i = 0; i < m->npeq; i++) du[i] = u_presc[i] - (u_presc_prev ? u_presc_prev[i] : 0.0);
@@ -980,10 +1078,49 @@ int orc_nonlinear_step(const orc_mesh *m, double *state, double *X, const double
         double res = 0.0;
         orc_pcg(m->neq, rowptr, colind, val, rhs, ddX, lstol, 1000000, 1, &it, &res);
         out[1] += it;
-        for (int i = 0; i < m->neq; i++) X[i] += ddX[i];
+        /* maxIncAllowed (nrsolver.C:340-353); the reference caps after the
+         * line search, this restatement caps the correction before it, so
+         * the search runs along the capped direction */
+        double mxi = 0.0;
+        for (int i = 0; i < m->neq; i++) if (fabs(ddX[i]) > mxi) mxi = fabs(ddX[i]);
+        if (mxi > maxinc) for (int i = 0; i < m->neq; i++) ddX[i] *= maxinc / mxi;
+        if (linesearch && nite > 0) {
+            /* LineSearchNM::solve linesearch.C:54-160.  The step ends at
+             * X = X_start + eta dr with the state of that point; the
+             * reference then also adds the scaled dr to X in NRSolver::solve
+             * (nrsolver.C:356), which counts the step twice -- this
+             * restatement (and the GPU driver) take the line search's point. */
+            double prod[12], eta[12];
+            for (int i = 0; i < m->neq; i++) rb[i] = X[i];
+            double s0 = 0.0;
+            for (int i = 0; i < m->neq; i++) s0 -= rhs[i] * ddX[i];
+            double etav = 1.0;
+            if (s0 < 0.0) {
+                const int max_iter = 10;
+                int ico = 0, ok = 0, ils;
+                prod[1] = 1.0;
+                eta[1] = 0.0;
+                eta[2] = 1.0;
+                for (ils = 2; ils <= max_iter; ils++) {
+                    double si = 0.0;
+                    if (orc_ls_product(m, state, rb, ddX, eta[ils], u_presc, RT, X, F, &si)) { rc = -11; goto out; }
+                    si /= s0;
+                    prod[ils] = si;
+                    if (fabs(si) < ls_tol) { etav = eta[ils]; ok = 1; break; }
+                    orc_ls_search(ils, prod, eta, ls_amp, ls_maxeta, 0.2, &ico);
+                    if (ico == 2) break;
+                }
+                if (!ok) etav = 1.0;
+            }
+            if (etav != 1.0) out[3]++;
+            for (int i = 0; i < m->neq; i++) X[i] = rb[i] + etav * ddX[i];
+            for (int i = 0; i < m->neq; i++) ddX[i] *= etav;
+        } else {
+            for (int i = 0; i < m->neq; i++) X[i] += ddX[i];
+        }
     }
 out:
-    free(rowptr); free(colind); free(val); free(F); free(rhs); free(ddX); free(dofid);
+    free(rowptr); free(colind); free(val); free(F); free(rhs); free(ddX); free(dofid); free(rb);
     return rc;
 }
 

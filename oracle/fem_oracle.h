@@ -94,6 +94,7 @@ int orc_internal_forces(const orc_mesh *m, const double *u_free, const double *u
                         double *f_free, double *f_presc, double *ebe_norm3);
 void orc_state_init(const orc_mesh *m, double *state);
 void orc_state_commit(const orc_mesh *m, double *state);
+void orc_state_restart(const orc_mesh *m, double *state);
 
 /* surface traction (ConstantSurfaceLoad, global c.s.) on (element, surface) pairs */
 int orc_surface_load(const orc_mesh *m, int nface, const int *elem, const int *surf,
@@ -107,6 +108,17 @@ int orc_extrapolated_forces(const orc_mesh *m, const double *state, const double
 int orc_nonlinear_step(const orc_mesh *m, double *state, double *X, const double *u_presc,
                        const double *u_presc_prev, int initial_guess, const double *RT,
                        double rtolf, int maxiter, int manrmsteps, double lstol, int *out, double *err);
+
+/* NRSolver extras (nrsolver.C:106-200, 316-357; linesearch.C:42-245):
+ * linesearch != 0 runs LineSearchNM after every correction but the first
+ * (ls_tol, ls_amp, ls_maxeta as lsearchtol / lsearchamp / lsearchmaxeta,
+ * clamped as in LineSearchNM::initializeFrom); maxinc caps the largest
+ * component of a correction (maxIncAllowed).  out[3] receives the number
+ * of line searches that ended with eta != 1. */
+int orc_nonlinear_step_ex(const orc_mesh *m, double *state, double *X, const double *u_presc,
+                          const double *u_presc_prev, int initial_guess, const double *RT, double rtolf, int maxiter,
+                          int manrmsteps, double lstol, int linesearch, double ls_tol, double ls_amp, double ls_maxeta,
+                          double maxinc, int *out, double *err);
 
 /* Skyline U^T D U factorization + solve of a full symmetric CSR (direct
  * solver of the reference; CPU baseline only).  *nwk_out: profile size. */

@@ -175,6 +175,12 @@ def state_commit(deck, state):
     lib().orc_state_commit(m.ref, state.ctypes.data_as(C.c_void_p))
 
 
+def state_restart(deck, state):
+    """Element::initForNewStep on every element (temp := committed)."""
+    m = Mesh(deck)
+    lib().orc_state_restart(m.ref, state.ctypes.data_as(C.c_void_p))
+
+
 def internal_forces(deck, u_free, u_presc=None, state=None, update_state=False):
     m = Mesh(deck)
     f = np.zeros(deck.neq)
@@ -224,6 +230,88 @@ def nonlinear_step(deck, state, X, u_presc, RT=None, rtolf=1e-8, maxiter=50, man
     if rc:
         raise RuntimeError(f"orc_nonlinear_step rc={rc}")
     return out[0], out[1], out[2], err.value
+
+
+def nonlinear_step_ex(deck, state, X, u_presc, u_presc_prev=None, initial_guess=True, RT=None, rtolf=1e-8,
+                      maxiter=50, manrmsteps=1, lstol=1e-12, linesearch=False, ls_tol=0.8, ls_amp=2.5, ls_maxeta=4.0,
+                      maxinc=1e20):
+    """nonlinear_step with the NRSolver extras: line search (LineSearchNM,
+    linesearch.C:54-245) and the increment cap maxinc (nrsolver.C:340-353).
+    Returns dict(newton, pcg, status, force_error, line_searches)."""
+    m = Mesh(deck)
+    out = (C.c_int * 4)()
+    err = C.c_double(0)
+    rt = _d(RT) if RT is not None else None
+    prev = _d(u_presc_prev) if u_presc_prev is not None else None
+    rc = lib().orc_nonlinear_step_ex(m.ref, state.ctypes.data_as(C.c_void_p), X.ctypes.data_as(C.c_void_p),
+                                     _d(u_presc).ctypes.data_as(C.c_void_p),
+                                     prev.ctypes.data_as(C.c_void_p) if prev is not None else None,
+                                     int(bool(initial_guess)),
+                                     rt.ctypes.data_as(C.c_void_p) if rt is not None else None, C.c_double(rtolf),
+                                     maxiter, manrmsteps, C.c_double(lstol), int(bool(linesearch)), C.c_double(ls_tol),
+                                     C.c_double(ls_amp), C.c_double(ls_maxeta), C.c_double(maxinc), out,
+                                     C.byref(err))
+    if rc:
+        raise RuntimeError(f"orc_nonlinear_step_ex rc={rc}")
+    return dict(newton=out[0], pcg=out[1], status=out[2], force_error=err.value, line_searches=out[3])
+
+
+def static_structural(deck, up_ref, final_time, dt, ltf=None, reduction=None, **nr):
+    """StaticStructural time stepping with NRSolver steps and the time-step
+    reduction of EngngModel::solveYourself (engngm.C:573-662): a step that
+    fails (maximum iterations or divergence, a ConvergenceException) is
+    restarted from the last converged state with dt * ncrf
+    (MetaStep::reduceTimeStep, metastep.C:138-146; error below dtmin or
+    after nmaxrestarts restarts), and after every converged step dt is
+    adapted to its Newton iterations (MetaStep::adaptTimeStep,
+    metastep.C:149-183 with SimpleReductionStrategy's factors,
+    timestepreductionstrategy.C:70-90: x2 below minrequirediter, x0.5 above
+    maxrequirediter, clamped to [dtmin, dtmax] and to the final time).
+    reduction: None (NoReductionStrategy: a failed step is kept as it is and
+    dt never changes) or dict(dtmin, dtmax, ncrf=0.5, nmaxrestarts=5,
+    minrequirediter=3, maxrequirediter=8).  Prescribed values at time t are
+    up_ref * ltf(t) (ltf default t / final_time).  Returns (X, state, steps)
+    with steps a list of dict(time, dt, newton, pcg, status, restarts)."""
+    ltf = ltf or (lambda t: t / final_time)
+    st = state_new(deck)
+    X = np.zeros(deck.neq)
+    up_prev = np.zeros(deck.npeq)
+    t = 0.0
+    steps = []
+    red = None if reduction is None else dict(dict(ncrf=0.5, nmaxrestarts=5, minrequirediter=3, maxrequirediter=8),
+                                             **reduction)
+    while t < final_time - 1e-12 * final_time:
+        restarts = 0
+        while True:
+            target = t + dt
+            up = up_ref * ltf(target)
+            Xs = X.copy()
+            rep = nonlinear_step_ex(deck, st, X, up, u_presc_prev=up_prev, **nr)
+            if rep["status"] == 1 or red is None:
+                break
+            dt2 = dt * red["ncrf"]
+            if dt2 < red["dtmin"]:
+                raise RuntimeError("Minimum time increment reached")
+            restarts += 1
+            if restarts > red["nmaxrestarts"]:
+                raise RuntimeError("Maximum number of time step reductions has been reached.")
+            dt = dt2
+            X[:] = Xs
+            state_restart(deck, st)           # EngngModel::initStepIncrements
+        steps.append(dict(time=target, dt=dt, newton=rep["newton"], pcg=rep["pcg"], status=rep["status"],
+                          restarts=restarts))
+        if red is not None:
+            n = rep["newton"]
+            f = 1.0 if (n == 0 or red["minrequirediter"] <= n <= red["maxrequirediter"]) else (
+                2.0 if n < red["minrequirediter"] else 0.5)
+            dt = dt * (min(1.0, f) if f < 1 else max(1.0, f))
+            dt = min(max(dt, red["dtmin"]), red["dtmax"])
+        if target + dt > final_time:
+            dt = final_time - target
+        state_commit(deck, st)
+        up_prev = up
+        t = target
+    return X, st, steps
 
 
 def extrapolated_forces(deck, state, du_presc):

@@ -157,3 +157,52 @@ def test_tangent_product_matches_oracle(ctx, tangent):
     y, yp = ev.tangent_product(None, du, tangent=True)
     ref = oracle.extrapolated_forces(d, st, du)
     assert np.abs(y - ref).max() <= 1e-10 * np.abs(ref).max()
+
+
+def test_line_search_matches_oracle():
+    """NRSolver with LineSearchNM on the GPU driver: a single step to 3/8 of
+    the block's end displacement without the tangent predictor converges
+    with the oracle's Newton iteration and line-search counts, at its
+    displacements (1e-8) and plastic strains."""
+    from oofem_amd.problem import StructuralProblem
+    d = meshgen.config3(8, 8, 32)
+    up = d.prescribed_values(3 / 8)
+    st = oracle.state_new(d)
+    X = np.zeros(d.neq)
+    ref = oracle.nonlinear_step_ex(d, st, X, up, initial_guess=False, maxiter=40, lstol=1e-12, linesearch=True)
+    assert ref["status"] == 1 and ref["line_searches"] >= 1
+    P = StructuralProblem(d, lstol=1e-12, rtolf=1e-8, maxiter=40, manrmsteps=1, initial_guess=0)
+    P.set_nr_extras(linesearch=True)
+    P.set_load(np.zeros(d.neq), np.zeros(d.npeq))
+    P.set_prescribed(up)
+    rep = P.nonlinear_step()
+    assert rep["status"] == 1 and rep["newton"] == ref["newton"], (rep, ref)
+    u = P.solution()
+    assert np.linalg.norm(u - X) <= 1e-8 * np.linalg.norm(X)
+    kap = P.state()[:, 13]                    # temp kappa (the step is not committed)
+    kref = st.reshape(-1, 38)[:, 13]
+    assert np.abs(kap - kref).max() <= 1e-8 * kref.max()
+    P.close()
+
+
+def test_time_step_reduction_matches_oracle():
+    """SimpleReductionStrategy time stepping on the GPU driver against the
+    oracle's driver: the same accepted steps (time, Newton iterations,
+    restarts) and the same final displacements and plastic strains."""
+    from oofem_amd.problem import StructuralProblem
+    d = meshgen.config3(8, 8, 32)
+    up_ref = d.prescribed_values(1.0)
+    red = dict(dtmin=0.01, dtmax=1.0)
+    X, st, ref = oracle.static_structural(d, up_ref, 1.0, 1.0, reduction=red, maxiter=8, lstol=1e-12)
+    P = StructuralProblem(d, lstol=1e-12, rtolf=1e-8, maxiter=8, manrmsteps=1)
+    P.set_load(np.zeros(d.neq), np.zeros(d.npeq))
+    got = P.run_steps(up_ref, 1.0, 1.0, reduction=red)
+    key = lambda s: (round(s["time"], 12), s["newton"], s["restarts"], s["status"])
+    assert [key(s) for s in got] == [key(s) for s in ref], (got, ref)
+    assert any(s["restarts"] for s in got)
+    u = P.solution()
+    assert np.linalg.norm(u - X) <= 1e-8 * np.linalg.norm(X)
+    kap = P.state()[:, 6]
+    kref = st.reshape(-1, 38)[:, 6]
+    assert np.abs(kap - kref).max() <= 1e-8 * kref.max()
+    P.close()

@@ -425,3 +425,76 @@ def test_rve_batch_block_rows_and_config4_specs():
     q = np.mean([s[1] == meshgen.QSPACE for s in specs])
     assert 0.25 <= q <= 0.35
     assert np.allclose(np.linalg.norm(macro, axis=1), 1e-3)
+
+
+def test_line_search_converges_where_plain_newton_fails():
+    """A single step from the virgin state to 3/8 of the block's end
+    displacement without the tangent predictor: plain Newton does not
+    converge in 40 iterations, Newton with LineSearchNM does (with at least
+    one eta != 1), and it reaches the root that Newton from the tangent
+    predictor reaches (the step's discrete equations do not depend on the
+    iteration path)."""
+    d = meshgen.config3(8, 8, 32)
+    up = d.prescribed_values(3 / 8)
+    res = {}
+    for name, kw in (("plain", dict(initial_guess=False)), ("ls", dict(initial_guess=False, linesearch=True)),
+                     ("ig", dict(initial_guess=True))):
+        st = oracle.state_new(d)
+        X = np.zeros(d.neq)
+        r = oracle.nonlinear_step_ex(d, st, X, up, maxiter=40, lstol=1e-12, **kw)
+        res[name] = (r, X)
+    assert res["plain"][0]["status"] == -1
+    assert res["ls"][0]["status"] == 1 and res["ls"][0]["line_searches"] >= 1
+    assert res["ig"][0]["status"] == 1
+    assert np.linalg.norm(res["ls"][1] - res["ig"][1]) <= 1e-8 * np.linalg.norm(res["ig"][1])
+
+
+def test_maxinc_caps_every_correction():
+    """maxinc bounds the first correction's largest component; a cap above
+    every correction changes nothing."""
+    d = meshgen.config3(4, 4, 16)
+    up = d.prescribed_values(1 / 8)
+    X0 = np.zeros(d.neq)
+    r0 = oracle.nonlinear_step_ex(d, oracle.state_new(d), X0, up, initial_guess=False, maxiter=1, lstol=1e-12)
+    big = np.abs(X0).max()
+    X1 = np.zeros(d.neq)
+    oracle.nonlinear_step_ex(d, oracle.state_new(d), X1, up, initial_guess=False, maxiter=1, lstol=1e-12,
+                             maxinc=0.25 * big)
+    assert abs(np.abs(X1).max() - 0.25 * big) <= 1e-12 * big
+    assert np.allclose(X1, 0.25 * X0, rtol=0, atol=1e-12 * big)
+    X2 = np.zeros(d.neq)
+    oracle.nonlinear_step_ex(d, oracle.state_new(d), X2, up, initial_guess=False, maxiter=1, lstol=1e-12,
+                             maxinc=2 * big)
+    assert np.array_equal(X2, X0) and r0["status"] == -1
+
+
+def test_time_step_reduction_restarts_from_the_converged_state():
+    """SimpleReductionStrategy driver: one increment of 1.0 fails within 8
+    Newton iterations, the restart with 0.5 converges, 8 iterations keep the
+    increment, and the second step ends the run.  The result is bit-for-bit
+    the run of two plain steps of 0.5 (the restart resets the temp
+    Gauss-point values, so the failed attempt leaves no trace)."""
+    d = meshgen.config3(8, 8, 32)
+    up_ref = d.prescribed_values(1.0)
+    X, st, steps = oracle.static_structural(d, up_ref, 1.0, 1.0, reduction=dict(dtmin=0.01, dtmax=1.0), maxiter=8,
+                                            lstol=1e-10)
+    assert [(round(s["time"], 12), s["restarts"], s["status"]) for s in steps] == [(0.5, 1, 1), (1.0, 0, 1)]
+    st2 = oracle.state_new(d)
+    X2 = np.zeros(d.neq)
+    prev = np.zeros(d.npeq)
+    for t in (0.5, 1.0):
+        r = oracle.nonlinear_step_ex(d, st2, X2, up_ref * t, u_presc_prev=prev, maxiter=8, lstol=1e-10)
+        assert r["status"] == 1
+        oracle.state_commit(d, st2)
+        prev = up_ref * t
+    assert np.array_equal(X, X2)
+    assert np.array_equal(st.reshape(-1, 38)[:, :14], st2.reshape(-1, 38)[:, :14])
+
+
+def test_no_reduction_keeps_a_failed_step():
+    """NoReductionStrategy: the failed step is accepted as it is and the
+    increment never changes."""
+    d = meshgen.config3(4, 4, 16)
+    X, st, steps = oracle.static_structural(d, d.prescribed_values(1.0), 1.0, 0.5, maxiter=2, lstol=1e-10)
+    assert [round(s["time"], 12) for s in steps] == [0.5, 1.0]
+    assert any(s["status"] != 1 for s in steps)

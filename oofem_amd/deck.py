@@ -326,7 +326,8 @@ def read_deck(path) -> DeckProblem:
             nodes_s = setd[r.scalar("set", cast=int)].get("nodes", [])
             for dof in dofs:
                 bc_mask[nodes_s, dof - 1] = True
-            blist.append(dict(nodes=np.array(nodes_s, dtype=np.int64), dofs=dofs, values=vals, ltf=f))
+            blist.append(dict(nodes=np.array(nodes_s, dtype=np.int64), dofs=dofs, values=vals, ltf=f,
+                              number=r.number))
         elif r.keyword == "constantsurfaceload":
             dofs = r.array("dofs", int)
             comps = r.array("components")
@@ -340,8 +341,11 @@ def read_deck(path) -> DeckProblem:
     mat = np.array(mrows)
     deck = Deck(str(path), etype, coords, conn, emat, mat, bc_mask, np.zeros((nd, 3)))
     tol, checks = _parse_checks(checks_txt)
-    return DeckProblem(deck, engng.keyword, engng.scalar("nsteps", 1, int), engng, blist, llist, ltf,
+    prob = DeckProblem(deck, engng.keyword, engng.scalar("nsteps", 1, int), engng, blist, llist, ltf,
                        node_numbers, elem_numbers, checks, tol)
+    prob.output_file = lines[0].strip()
+    prob.description = lines[1].strip()
+    return prob
 
 
 def evaluate_checks(prob: DeckProblem, results: dict):

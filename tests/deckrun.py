@@ -17,7 +17,7 @@ def _collect(prob, u, up, reactions_presc, state):
         reac[(n, k + 1)] = reactions_presc[-d.eqn[n, k] - 1]
     ngp = meshgen.NGP[d.etype]
     st = state.reshape(d.nelem, ngp, -1)
-    return dict(u_node=u_node, reactions=reac, gp_sig=st[:, :, 20:26], gp_eps=st[:, :, 26:32], u=u)
+    return dict(u_node=u_node, reactions=reac, gp_sig=st[:, :, 20:26], gp_eps=st[:, :, 26:32], u=u, state=st)
 
 
 def run_deck_oracle(prob, tol=1e-14):

@@ -498,3 +498,42 @@ def test_no_reduction_keeps_a_failed_step():
     X, st, steps = oracle.static_structural(d, d.prescribed_values(1.0), 1.0, 0.5, maxiter=2, lstol=1e-10)
     assert [round(s["time"], 12) for s in steps] == [0.5, 1.0]
     assert any(s["status"] != 1 for s in steps)
+
+
+@pytest.mark.parametrize("name", ["patch301.in", "cantilever_Qspace.in"])
+def test_out_file_round_trip_and_checks(name):
+    """The .out writer on the oracle's solution of a reference deck: the file
+    reads back to the computed values within its print precision, every
+    #NODE check holds on the file, and on patch301 (tolerance 1e-5) every
+    check holds on the file, as the reference's extractor-based regression
+    tests would find."""
+    import io
+
+    from oofem_amd.outfile import check_out, read_out, write_header, write_step
+    prob = read_deck(os.path.join(GOLDEN, name))
+    res = run_deck_oracle(prob)
+    buf = io.StringIO()
+    write_header(buf, prob)
+    for k in sorted(res):
+        write_step(buf, prob, k, prob.time_of_step(k), res[k])
+    steps = read_out(buf.getvalue())
+    assert len(steps) == len(res)
+    d = prob.deck
+    for k, s in zip(sorted(res), steps):
+        r = res[k]
+        for i in range(d.nnode):
+            for c in range(3):
+                v = r["u_node"][i, c]
+                assert abs(s["u"][(int(prob.node_numbers[i]), c + 1)] - v) <= 5e-9 * max(abs(v), 1e-300)
+        for (n, c), v in r["reactions"].items():
+            assert abs(s["reactions"][(int(prob.node_numbers[n]), c)] - v) <= 5e-5 * max(abs(v), 1e-300)
+        for e in range(d.nelem):
+            for g in range(r["gp_sig"].shape[1]):
+                key = (int(prob.elem_numbers[e]), g + 1)
+                assert np.allclose(s["stresses"][key], r["gp_sig"][e, g], rtol=5e-5, atol=1e-300)
+                assert np.allclose(s["strains"][key], r["gp_eps"][e, g], rtol=5e-5, atol=1e-300)
+    checks = check_out(prob, steps)
+    assert len(checks) == len(prob.checks)
+    assert all(ok for c, v, ok in checks if c.kind == "node"), [(c.line, v) for c, v, ok in checks if not ok]
+    if name == "patch301.in":
+        assert all(ok for _, _, ok in checks)

@@ -413,7 +413,10 @@ def main_rve(args):
     log(f"[bench] autotune ({t_tune:.1f} s): {tuned}")
     for i in range(args.warmup):
         B.run(log=log)
+    B.set_profiling(True)
     runs = [B.run(log=log) for _ in range(args.steps)]
+    prof = B.profile()
+    forms = B.bulk_forms()
     t_solve = sum(r["t_solve"] for r in runs) / len(runs)
     t_rhs = sum(r["t_rhs"] for r in runs) / len(runs)
     t_hom = sum(r["t_homogenize"] for r in runs) / len(runs)
@@ -453,10 +456,60 @@ def main_rve(args):
                    "macro_stress_norm_mean": float(np.linalg.norm(sig, axis=1).mean())},
         "roofline": None, "cpu_baseline": None,
     }
+    if prof["launches"]:
+        avg_ms = prof["ms"] / prof["launches"]
+        ach = prof["bytes"] / (prof["ms"] * 1e-3) / 1e9
+        kname = "k_bulk_spmv_nb<%d>" if all(forms) else "k_bulk_spmv<%d>"
+        lpr = tuned[0]["bulk_lanes_per_row"] if tuned else 0
+        res["roofline"] = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                               frac=round(ach / HBM_PEAK_GBS, 4), traffic=None, kernel=kname % lpr,
+                               avg_launch_ms=round(avg_ms, 4), launches=prof["launches"],
+                               algorithmic_bytes=int(prof["bytes"] / prof["launches"]))
     B.close()
     ctx.close()
+    if not args.no_cpu_baseline:
+        res["cpu_baseline"] = cpu_baseline_rve(args, specs, t_solve)
     print(json.dumps(res), flush=True)
     return 0
+
+
+def cpu_baseline_rve(args, specs, t_gpu):
+    """Oracle (C restatement, one thread) on a seeded sample of the batch's
+    RVEs: six FP64 Jacobi-PCG solves each to the batch tolerance, timed, and
+    scaled to the whole batch by the sum over RVEs of stored entries times
+    cube edge (CG iterations grow with the edge length), the sample's
+    measured cost per unit of that product."""
+    import numpy as np
+
+    from oofem_amd import meshgen
+    from oracle import oracle
+    from oofem_amd.rve import NNZ_ROW
+    rng = np.random.default_rng(7)
+    # stored entries (the batch's own estimate) times the cube edge
+    w = np.array([3.0 * (s[0] + 1) ** 3 * (4 if s[1] == meshgen.QSPACE else 1) * NNZ_ROW[s[1]] * s[0] for s in specs])
+    cut = np.percentile(w, 25)
+    order = rng.permutation(len(specs))
+    t_cpu, w_done, done = 0.0, 0.0, []
+    for i in order:
+        if t_cpu > args.rve_cpu_seconds:
+            break
+        if w[i] > cut:
+            continue
+        d = meshgen.rve(*specs[i])
+        t = time.perf_counter()
+        for k in range(6):
+            e = np.zeros(6)
+            e[k] = 1.0
+            oracle.linear_static(d, tol=args.rve_tol, u_presc=meshgen.affine_values(d, e))
+        t_cpu += time.perf_counter() - t
+        w_done += w[i]
+        done.append(f"{'Q' if specs[i][1] == meshgen.QSPACE else 'L'}{specs[i][0]}")
+    t_batch = t_cpu * w.sum() / w_done
+    return dict(value=float(6 * len(specs) / t_batch), unit="solves/s", cores=1, kind="port",
+                sample=(f"oracle FP64 Jacobi-PCG (1 thread), six solves each on {len(done)} RVEs of the batch "
+                        f"({', '.join(done)}) in {t_cpu:.1f} s, scaled by stored entries x edge length "
+                        f"(sum over the batch / sum over the sample = {w.sum() / w_done:.1f}); estimated "
+                        f"batch {t_batch:.0f} s"))
 
 
 def main():
@@ -470,6 +523,8 @@ def main():
     ap.add_argument("--forcing", type=float, default=0.1,
                     help="inexact-Newton variants with forcing terms up to this value (0: none)")
     ap.add_argument("--rve-tune-sample", type=int, default=16)
+    ap.add_argument("--rve-cpu-seconds", type=float, default=15.0,
+                    help="CPU time budget of the oracle sample for the RVE cpu_baseline")
     ap.add_argument("--rve-split", type=int, default=-1,
                     help="stored entries above which an RVE goes to the bulk phase (-1 automatic, 0 all)")
     ap.add_argument("--rve-check", type=int, default=8)

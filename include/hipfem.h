@@ -302,6 +302,11 @@ int  hfem_batch_bulk_form(const hfem_batch *bt);
  * default) = the batch's entries divided by the workgroups the persistent
  * kernel keeps resident; 0 = every block */
 int  hfem_batch_set_split(hfem_batch *bt, long long split);
+/* HIP-event timing of the bulk-phase products on the batch's stream (on
+ * resets the counters): number of timed products, their summed duration
+ * (ms) and their summed algorithmic bytes */
+int  hfem_batch_set_profiling(hfem_batch *bt, int on);
+int  hfem_batch_profile(const hfem_batch *bt, long long *launches, double *ms, double *bytes);
 int  hfem_batch_partition_info(hfem_batch *bt, int *nbulk, long long *split_used);
 /* time every bulk shape on the blocks above the split (40 iterations
  * each), keep the fastest */

@@ -779,6 +779,13 @@ struct hfem_batch {
     std::vector<BulkBlock> h_bst;
     std::vector<int4> h_tiles;
     std::vector<int> h_act, h_act_t0;
+    double tile_bytes = 0.0;    // algorithmic bytes of one bulk product over the current tile list
+    // HIP-event timing of the bulk products (hfem_batch_set_profiling)
+    bool profiling = false;
+    long long prof_n = 0;
+    double prof_ms = 0.0, prof_bytes = 0.0;
+    std::vector<hipEvent_t> ev;  // 2 * BULK_CHECK events, reused
+    int ev_used = 0;
 };
 
 static int batch_free(hfem_batch *bt)
@@ -788,6 +795,8 @@ static int batch_free(hfem_batch *bt)
                   bt->d_act_t0, bt->d_part};
     for (void *p : ps)
         if (p) HIPFEM_CHECK(hipFree(p));
+    for (hipEvent_t e : bt->ev) HIPFEM_CHECK(hipEventDestroy(e));
+    bt->ev.clear();
     return 0;
 }
 
@@ -922,6 +931,33 @@ extern "C" int hfem_batch_bulk_form(const hfem_batch *bt)
     return bt && bt->nodeblock && bt->nodeblock_ok && bt->A->nodeblocked ? 1 : 0;
 }
 
+// HIP-event timing of the bulk products: on resets the counters; the
+// profile is the number of timed products, their summed duration and their
+// summed algorithmic bytes
+extern "C" int hfem_batch_set_profiling(hfem_batch *bt, int on)
+{
+    HIPFEM_REQUIRE(bt, HIPFEM_EINVAL, "batch profiling: null handle");
+    DeviceGuard g(bt->ctx->device);
+    if (on && bt->ev.empty()) {
+        bt->ev.resize(2 * BULK_CHECK);
+        for (auto &e : bt->ev) HIPFEM_CHECK(hipEventCreate(&e));
+    }
+    bt->profiling = on != 0;
+    bt->prof_n = 0;
+    bt->prof_ms = bt->prof_bytes = 0.0;
+    bt->ev_used = 0;
+    return 0;
+}
+
+extern "C" int hfem_batch_profile(const hfem_batch *bt, long long *launches, double *ms, double *bytes)
+{
+    HIPFEM_REQUIRE(bt && launches && ms && bytes, HIPFEM_EINVAL, "batch profile: null argument");
+    *launches = bt->prof_n;
+    *ms = bt->prof_ms;
+    *bytes = bt->prof_bytes;
+    return 0;
+}
+
 extern "C" int hfem_batch_set_split(hfem_batch *bt, long long split)
 {
     HIPFEM_REQUIRE(bt && split >= -1, HIPFEM_EINVAL, "batch split: bad value");
@@ -1002,6 +1038,18 @@ static int bulk_tiles(hfem_batch *bt, const std::vector<int> &act)
     bt->h_tiles.clear();
     bt->h_act = act;
     bt->h_act_t0.clear();
+    // algorithmic bytes of one product over these blocks: values, block
+    // columns and node-row pointers (node-block form) or columns and row
+    // pointers (CSR rows), p read once and q written once (six FP32
+    // columns per row)
+    const bool nb = bt->nodeblock && bt->nodeblock_ok && bt->A->nodeblocked;
+    bt->tile_bytes = 0.0;
+    for (int a : act) {
+        const int blk = bt->h_bulk[a];
+        const double nnz = (double)bt->h_nnz[blk], rows = (double)(bt->h_rows[blk + 1] - bt->h_rows[blk]);
+        bt->tile_bytes += nb ? 4.0 * nnz + 4.0 * nnz / 9.0 + 4.0 * rows / 3.0 + 48.0 * rows
+                             : 8.0 * nnz + 4.0 * rows + 48.0 * rows;
+    }
     for (int a : act) {
         const int blk = bt->h_bulk[a];
         bt->h_act_t0.push_back((int)bt->h_tiles.size());
@@ -1071,6 +1119,15 @@ static int bulk_solve(hfem_batch *bt, const double *db, double *dx, int maxit)
             HIPFEM_CHECK(hipMemcpyAsync(bt->h_bst.data(), bt->d_bst, sizeof(BulkBlock) * nbulk,
                                         hipMemcpyDeviceToHost, st));
             HIPFEM_CHECK(hipStreamSynchronize(st));
+            // the products since the last look ran over the current tile list
+            for (int k = 0; k < bt->ev_used; k++) {
+                float ms = 0.f;
+                HIPFEM_CHECK(hipEventElapsedTime(&ms, bt->ev[2 * k], bt->ev[2 * k + 1]));
+                bt->prof_ms += ms;
+                bt->prof_bytes += bt->tile_bytes;
+                bt->prof_n++;
+            }
+            bt->ev_used = 0;
             std::vector<int> still;
             for (int a : act)
                 if (!bt->h_bst[a].done) still.push_back(a);
@@ -1085,7 +1142,10 @@ static int bulk_solve(hfem_batch *bt, const double *db, double *dx, int maxit)
                 nact = (int)act.size();
             }
         }
+        const bool timed = bt->profiling && bt->ev_used < BULK_CHECK;
+        if (timed) HIPFEM_CHECK(hipEventRecord(bt->ev[2 * bt->ev_used], st));
         hipLaunchKernelGGL(spmv, dim3(ntiles), dim3(BULK_TILE), 0, st, A);
+        if (timed) HIPFEM_CHECK(hipEventRecord(bt->ev[2 * bt->ev_used++ + 1], st));
         k_bulk_reduce<BP_ALPHA><<<nact, 256, 0, st>>>(A);
         k_bulk_update<<<ntiles, BULK_TILE, 0, st>>>(A);
         k_bulk_reduce<BP_BETA><<<nact, 256, 0, st>>>(A);

@@ -162,6 +162,20 @@ class RVEBatch:
             _check(lib().hfem_batch_set_split(c.h, C.c_longlong(split)), "split")
             _check(lib().hfem_batch_set_bulk_form(c.h, int(nodeblock)), "bulk form")
 
+    def set_profiling(self, on=True):
+        for c in self.chunks:
+            _check(lib().hfem_batch_set_profiling(c.h, int(bool(on))), "profiling")
+
+    def profile(self):
+        """HIP-event timing of the bulk products summed over the chunks:
+        dict(launches, ms, bytes)."""
+        n, ms, by = 0, 0.0, 0.0
+        for c in self.chunks:
+            a, b, d = C.c_longlong(0), C.c_double(0), C.c_double(0)
+            _check(lib().hfem_batch_profile(c.h, C.byref(a), C.byref(b), C.byref(d)), "profile")
+            n, ms, by = n + a.value, ms + b.value, by + d.value
+        return dict(launches=n, ms=ms, bytes=by)
+
     def bulk_forms(self):
         """1 (node-block) or 0 (CSR rows) per chunk."""
         lib().hfem_batch_bulk_form.argtypes = [C.c_void_p]

@@ -175,7 +175,7 @@ __global__ __launch_bounds__(256) void k_spmv_nb(int nbr, const int32_t *__restr
     }
 }
 
-// row-group SpMV: one 32-lane half wave per row group (see grp_row)
+// row-group SpMV: GRP_LANES lanes per row group (see grp_row)
 template <typename MT>
 __global__ __launch_bounds__(256) void k_spmv_grp(int ngrp, const int32_t *__restrict__ grow,
                                                   const int32_t *__restrict__ gch, const uint32_t *__restrict__ ccol,
@@ -183,8 +183,8 @@ __global__ __launch_bounds__(256) void k_spmv_grp(int ngrp, const int32_t *__res
                                                   const MT *__restrict__ val, const double *__restrict__ x,
                                                   double *__restrict__ y)
 {
-    const int lane = threadIdx.x % 32;
-    for (int g = blockIdx.x * 8 + threadIdx.x / 32; g < ngrp; g += gridDim.x * 8) {
+    const int lane = threadIdx.x % GRP_LANES;
+    for (int g = blockIdx.x * GRP_PER_BLOCK + threadIdx.x / GRP_LANES; g < ngrp; g += gridDim.x * GRP_PER_BLOCK) {
         const int r0 = grow[g], nr = grow[g + 1] - r0, c0 = gch[g], nch = gch[g + 1] - c0;
         double acc[3];
         grp_row<double>(r0, nr, c0, nch, lane, rowptr, ccol, coff, val, [&](int c) { return x[c]; }, acc);
@@ -203,7 +203,7 @@ static int launch_spmv(const hipcsr_mtx *m, const SpmvShape &sh, hipStream_t st,
         k_spmv_nb<VT><<<grid, 256, 0, st>>>(m->nbr, m->d_bptr, m->d_bcol, v, x, y);
         ok = true;
     } else if (sh.nodeblock && m->grouped) {
-        const int grid = std::max(1, std::min((m->ngrp + 7) / 8, 1 << 20));
+        const int grid = std::max(1, std::min((m->ngrp + GRP_PER_BLOCK - 1) / GRP_PER_BLOCK, 1 << 20));
         k_spmv_grp<VT><<<grid, 256, 0, st>>>(m->ngrp, m->d_grow, m->d_gch, m->d_ccol, m->d_coff, rp, v, x, y);
         ok = true;
     } else if (sh.stream) {

@@ -197,9 +197,9 @@ __global__ __launch_bounds__(256) void k_pcg_spmv_p_grp(int ngrp, const int32_t 
 {
     __shared__ double sh[4];
     if (halted(st)) return;
-    const int lane = threadIdx.x % 32;
+    const int lane = threadIdx.x % GRP_LANES;
     double acc = 0.0;
-    for (int g = blockIdx.x * 8 + threadIdx.x / 32; g < ngrp; g += gridDim.x * 8) {
+    for (int g = blockIdx.x * GRP_PER_BLOCK + threadIdx.x / GRP_LANES; g < ngrp; g += gridDim.x * GRP_PER_BLOCK) {
         const int r0 = grow[g], nr = grow[g + 1] - r0, c0 = gch[g], nch = gch[g + 1] - c0;
         VT a[3];
         grp_row<VT>(r0, nr, c0, nch, lane, rowptr, ccol, coff, val, [&](int c) { return p[c]; }, a);
@@ -692,7 +692,7 @@ static int pcg_run(hippcg_solver *s, hipcsr_mtx *A, const MT *val, const VT *b, 
     const bool nodeblock = sh.nodeblock && A->nodeblocked;
     const bool grouped = sh.nodeblock && !A->nodeblocked && A->grouped;
     const int gs = nodeblock    ? std::max(1, std::min((A->nbr + 7) / 8, MAX_GRID))
-                   : grouped   ? std::max(1, std::min((A->ngrp + 7) / 8, MAX_GRID))
+                   : grouped   ? std::max(1, std::min((A->ngrp + GRP_PER_BLOCK - 1) / GRP_PER_BLOCK, MAX_GRID))
                    : sh.stream ? std::max(1, std::min((n + sh.rpb - 1) / sh.rpb, MAX_GRID))
                                : spmv_grid(n, sh.lanes);
     const bool fused = s->fused && !sh.stream && !nodeblock && !grouped;  // these forms gather a ready p
@@ -799,7 +799,7 @@ static int pcg_reliable(hippcg_solver *s, hipcsr_mtx *A, const double *db, doubl
     const bool nodeblock = sh.nodeblock && A->nodeblocked;
     const bool grouped = sh.nodeblock && !A->nodeblocked && A->grouped;
     const int gs = nodeblock    ? std::max(1, std::min((A->nbr + 7) / 8, MAX_GRID))
-                   : grouped   ? std::max(1, std::min((A->ngrp + 7) / 8, MAX_GRID))
+                   : grouped   ? std::max(1, std::min((A->ngrp + GRP_PER_BLOCK - 1) / GRP_PER_BLOCK, MAX_GRID))
                    : sh.stream ? std::max(1, std::min((n + sh.rpb - 1) / sh.rpb, MAX_GRID))
                                : spmv_grid(n, sh.lanes);
     rc = ensure_partials(s->ctx, 3 * (size_t)std::max(gv, gs));

@@ -247,10 +247,13 @@ int hfem_model_residual(hfem_model *md, const double *dRT, const double *dF, dou
 /* MisesMatStatus::updateYourself (misesmat.C:720-728) for every Gauss point */
 int hfem_model_commit(hfem_model *md);
 /* form of the FP32 element-by-element operator used by hippcg precision 2
- * (no counterpart in the reference): 1 = every element in one launch, its
- * forces stored and summed per node in FP64; 0 = one launch per element
- * color adding into the FP64 result (default: measured faster on MI355X,
- * 0.27 vs 0.34 ms for 1M elements) */
+ * (no counterpart in the reference): 0 = one launch per element color
+ * adding into the FP64 result; 1 = every element in one launch, its forces
+ * stored and summed per node in FP64; 2 (default) = every element in one
+ * launch, each force stored in its color's FP32 slot vector and summed per
+ * equation in color order (the result of form 0 bit for bit; measured on
+ * MI355X for 1M hexahedra: 0.248 ms, against 0.255 for form 0 and 0.34 for
+ * form 1) */
 int hfem_model_set_element_operator(hfem_model *md, int form);
 /* Element::initForNewStep for every element (element.C:878-887, called by
  * EngngModel::initStepIncrements before a restarted step): each Gauss

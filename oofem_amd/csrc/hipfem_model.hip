@@ -738,15 +738,20 @@ __global__ void k_fill_d(int n, double *__restrict__ a, const PcgState *__restri
 // order), after which thread g holds the forces of the element's node g:
 // either y += f over one color (elems = the color's list), or, with ebuf, the
 // element's 24 forces are stored (elems = NULL: every element, one launch)
-// for k_mf_gather to sum per node.
+// for k_mf_gather to sum per node, or, with yslot, each force is stored in
+// the element color's slot vector (yslot[color * neq + equation]) for
+// k_mf_slot_sum.  Node coordinates come as per-element FP32 differences
+// (erel, from k_mf_erel).
 constexpr int MF_EPB = 32;
 __global__ __launch_bounds__(256) void k_mf_hex(const int *__restrict__ elems, int nel,
-                                                const double *__restrict__ coords, const int *__restrict__ conn,
+                                                const float *__restrict__ erel, const int *__restrict__ conn,
                                                 const int *__restrict__ eqn, const int *__restrict__ emat,
                                                 const double *__restrict__ mat, const float *__restrict__ desc,
                                                 const float *__restrict__ dgen, int ngp_total,
                                                 const float *__restrict__ x, double *__restrict__ y,
-                                                float *__restrict__ ebuf, const PcgState *__restrict__ st)
+                                                float *__restrict__ ebuf, float *__restrict__ yslot,
+                                                const uint8_t *__restrict__ ecolor, int neq,
+                                                const PcgState *__restrict__ st)
 {
     __shared__ float sX[MF_EPB][8][3];
     __shared__ float sU[MF_EPB][8][3];
@@ -758,10 +763,10 @@ __global__ __launch_bounds__(256) void k_mf_hex(const int *__restrict__ elems, i
     int myq[3] = {0, 0, 0};   // equations of this thread's node g
     if (active) {
         e = elems ? elems[ei] : ei;
-        const int node = conn[(size_t)e * 8 + g], node0 = conn[(size_t)e * 8];
+        const int node = conn[(size_t)e * 8 + g];
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-            sX[le][g][d] = (float)(coords[3 * (size_t)node + d] - coords[3 * (size_t)node0 + d]);
+            sX[le][g][d] = g ? erel[(size_t)e * 21 + 3 * (g - 1) + d] : 0.0f;
             const int q = eqn[3 * (size_t)node + d];
             myq[d] = q;
             sU[le][g][d] = q > 0 ? x[q - 1] : 0.0f;
@@ -897,13 +902,48 @@ __global__ __launch_bounds__(256) void k_mf_hex(const int *__restrict__ elems, i
         }
     }
     if (active) {
+        const size_t slot = yslot ? (size_t)ecolor[e] * neq : 0;
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             if (ebuf)
                 ebuf[(size_t)e * 24 + 3 * g + i] = fr[i];
-            else if (myq[i] > 0)
-                y[myq[i] - 1] += (double)fr[i];
+            else if (myq[i] > 0) {
+                if (yslot)
+                    yslot[slot + myq[i] - 1] = fr[i];
+                else
+                    y[myq[i] - 1] += (double)fr[i];
+            }
         }
+    }
+}
+
+// per-element coordinates of nodes 2..8 relative to node 1, differenced in
+// FP64 and rounded to FP32 (what k_mf_hex uses; built once per mesh)
+__global__ void k_mf_erel(int nel, const double *__restrict__ coords, const int *__restrict__ conn,
+                          float *__restrict__ erel)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nel) return;
+    const int n0 = conn[(size_t)e * 8];
+    for (int a = 1; a < 8; a++) {
+        const int n = conn[(size_t)e * 8 + a];
+        for (int d = 0; d < 3; d++)
+            erel[(size_t)e * 21 + 3 * (a - 1) + d] = (float)(coords[3 * (size_t)n + d] - coords[3 * (size_t)n0 + d]);
+    }
+}
+
+// y[i] = sum over the colors touching equation i, ascending, of the FP32
+// slot values, in FP64
+__global__ void k_mf_slot_sum(int neq, int ncolors, const uint16_t *__restrict__ eqmask,
+                              const float *__restrict__ yslot, double *__restrict__ y, const PcgState *__restrict__ st)
+{
+    if (st && halted(st)) return;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < neq; i += gridDim.x * blockDim.x) {
+        const unsigned m = eqmask[i];
+        double s = 0.0;
+        for (int c = 0; c < ncolors; c++)
+            if (m >> c & 1u) s += (double)yslot[(size_t)c * neq + i];
+        y[i] = s;
     }
 }
 
@@ -1061,7 +1101,17 @@ struct hfem_model {
     float *d_ebuf = nullptr;                     // element forces of the gather form (24 per element)
     bool tdesc_used = false;     // false: every point elastic, D from the material table
     bool has_mises = false;
-    bool use_gather = false;     // element operator: gather form, else colored scatter (measured faster)
+    // element operator form: 0 one launch per color adding into the FP64
+    // result, 1 gather (element forces stored, summed per node), 2 slots
+    // (every element in one launch, each color's forces stored in its own
+    // FP32 slot vector, summed per equation in color order)
+    int mf_form = 2;
+    std::vector<uint8_t> h_ecolor;               // color of every element
+    std::vector<uint64_t> h_nodemask;            // colors of the elements at every node
+    uint8_t *d_ecolor = nullptr;
+    uint16_t *d_eqmask = nullptr;                // colors touching every free equation (slot form)
+    float *d_yslot = nullptr;                    // ncolors x neq FP32 slots (slot form)
+    float *d_erel = nullptr;                     // per element: nodes 2..8 relative to node 1, FP32 (21)
     std::vector<hipcsr_mtx *> linked;
     // MisesMat surrogate (hfem_model_set_surrogate)
     hfem_mlp *sur = nullptr;
@@ -1091,12 +1141,17 @@ static int model_free(hfem_model *md)
     }
     void *ps[] = {md->d_coords, md->d_mat, md->d_state, md->d_conn, md->d_emat, md->d_eqn, md->d_bad,
                   md->d_eoff,   md->d_colors, md->d_uf, md->d_up, md->d_ff, md->d_fp, md->d_out3, md->d_dofid,
-                  md->d_tdesc, md->d_nptr, md->d_nadj, md->d_ebuf};
+                  md->d_tdesc, md->d_nptr, md->d_nadj, md->d_ebuf, md->d_ecolor, md->d_eqmask, md->d_yslot,
+                  md->d_erel};
     for (void *p : ps)
         if (p) HIPFEM_CHECK(hipFree(p));
     md->d_tdesc = nullptr;
     md->d_nptr = md->d_nadj = nullptr;
     md->d_ebuf = nullptr;
+    md->d_ecolor = nullptr;
+    md->d_eqmask = nullptr;
+    md->d_yslot = nullptr;
+    md->d_erel = nullptr;
     md->d_coords = md->d_mat = md->d_state = md->d_uf = md->d_up = md->d_ff = md->d_fp = md->d_out3 = nullptr;
     md->d_conn = md->d_emat = md->d_eqn = md->d_bad = md->d_colors = nullptr;
     md->d_dofid = nullptr;
@@ -1344,6 +1399,8 @@ extern "C" int hfem_model_build_structure(hfem_model *md, hipcsr_mtx *K)
     HIPFEM_CHECK(hipStreamSynchronize(st));
     md->color_ptr = cptr;
     md->ncolors = ncol;
+    md->h_ecolor.assign(color.begin(), color.end());
+    md->h_nodemask = nodemask;
     md->pattern_id = K->pattern_id;
     return 0;
 }
@@ -1405,6 +1462,27 @@ extern "C" int hfem_model_assemble_stiffness(hfem_model *md, hipcsr_mtx *K, int 
                                                                           md->sur ? md->d_sflag : nullptr);
             HIPFEM_CHECK(hipGetLastError());
         }
+        if (!md->d_erel && md->nelem) {
+            // per-element FP32 relative coordinates (the differences k_mf_hex
+            // would otherwise form from the FP64 node coordinates each time)
+            HIPFEM_CHECK(hipMalloc(&md->d_erel, sizeof(float) * 21 * (size_t)md->nelem));
+            k_mf_erel<<<(md->nelem + 255) / 256, 256, 0, st>>>(md->nelem, md->d_coords, md->d_conn, md->d_erel);
+            HIPFEM_CHECK(hipGetLastError());
+        }
+        if (!md->d_ecolor && md->ncolors <= 16 && md->nelem && md->neq) {
+            // slot form: element colors and the colors touching each free equation
+            std::vector<uint16_t> eqmask((size_t)md->neq, 0);
+            for (int n = 0; n < md->nnode; n++)
+                for (int d = 0; d < 3; d++) {
+                    const int q = md->h_eqn[3 * (size_t)n + d];
+                    if (q > 0) eqmask[q - 1] = (uint16_t)md->h_nodemask[n];
+                }
+            HIPFEM_CHECK(hipMalloc(&md->d_ecolor, (size_t)md->nelem));
+            HIPFEM_CHECK(hipMalloc(&md->d_eqmask, sizeof(uint16_t) * (size_t)md->neq));
+            HIPFEM_CHECK(hipMalloc(&md->d_yslot, sizeof(float) * (size_t)md->ncolors * md->neq));
+            HIPFEM_CHECK(hipMemcpy(md->d_ecolor, md->h_ecolor.data(), (size_t)md->nelem, hipMemcpyHostToDevice));
+            HIPFEM_CHECK(hipMemcpy(md->d_eqmask, eqmask.data(), sizeof(uint16_t) * eqmask.size(), hipMemcpyHostToDevice));
+        }
         if (!md->d_nptr) {
             // node -> element adjacency of the gather form (built once per mesh)
             std::vector<int> cnt((size_t)md->nnode + 1, 0), adj((size_t)md->nelem * 8);
@@ -1448,12 +1526,28 @@ void model_unlink(hfem_model *md, hipcsr_mtx *m)
 int model_mf_apply_f32(hfem_model *md, const float *x, double *y, const PcgState *st)
 {
     hipStream_t s = md->ctx->stream;
-    if (md->d_nptr && md->use_gather) {
+    if (md->mf_form == 2 && md->d_yslot && md->d_erel) {
+        // slot form: every element in one launch, each writing its forces
+        // into its color's FP32 slot (no element of a color shares a node),
+        // then one FP64 sum per equation over the colors in ascending order
+        // (the order of the per-color launches: the same result, bit for bit)
+        if (md->nelem)
+            k_mf_hex<<<(md->nelem + MF_EPB - 1) / MF_EPB, 256, 0, s>>>(
+                nullptr, md->nelem, md->d_erel, md->d_conn, md->d_eqn, md->d_emat, md->d_mat,
+                md->tdesc_used ? md->d_tdesc : nullptr, md->d_dsur, md->ngp_total, x, y, nullptr, md->d_yslot,
+                md->d_ecolor, md->neq, st);
+        k_mf_slot_sum<<<std::min(8192, (md->neq + 255) / 256), 256, 0, s>>>(md->neq, md->ncolors, md->d_eqmask,
+                                                                            md->d_yslot, y, st);
+        HIPFEM_CHECK(hipGetLastError());
+        return 0;
+    }
+    if (md->d_nptr && md->mf_form == 1) {
         // gather form: every element in one launch, then one FP64 sum per node
         if (md->nelem)
             k_mf_hex<<<(md->nelem + MF_EPB - 1) / MF_EPB, 256, 0, s>>>(
-                nullptr, md->nelem, md->d_coords, md->d_conn, md->d_eqn, md->d_emat, md->d_mat,
-                md->tdesc_used ? md->d_tdesc : nullptr, md->d_dsur, md->ngp_total, x, y, md->d_ebuf, st);
+                nullptr, md->nelem, md->d_erel, md->d_conn, md->d_eqn, md->d_emat, md->d_mat,
+                md->tdesc_used ? md->d_tdesc : nullptr, md->d_dsur, md->ngp_total, x, y, md->d_ebuf, nullptr, nullptr,
+                0, st);
         k_mf_gather<<<std::min(8192, (md->nnode + 255) / 256), 256, 0, s>>>(md->nnode, md->d_nptr, md->d_nadj,
                                                                               md->d_eqn, md->d_ebuf, y, st);
         HIPFEM_CHECK(hipGetLastError());
@@ -1463,10 +1557,10 @@ int model_mf_apply_f32(hfem_model *md, const float *x, double *y, const PcgState
     for (int c = 0; c < md->ncolors; c++) {
         const int n = md->color_ptr[c + 1] - md->color_ptr[c];
         if (!n) continue;
-        k_mf_hex<<<(n + MF_EPB - 1) / MF_EPB, 256, 0, s>>>(md->d_colors + md->color_ptr[c], n, md->d_coords,
+        k_mf_hex<<<(n + MF_EPB - 1) / MF_EPB, 256, 0, s>>>(md->d_colors + md->color_ptr[c], n, md->d_erel,
                                                            md->d_conn, md->d_eqn, md->d_emat, md->d_mat,
                                                            md->tdesc_used ? md->d_tdesc : nullptr, md->d_dsur,
-                                                           md->ngp_total, x, y, nullptr, st);
+                                                           md->ngp_total, x, y, nullptr, nullptr, nullptr, 0, st);
     }
     HIPFEM_CHECK(hipGetLastError());
     return 0;
@@ -1623,8 +1717,8 @@ extern "C" int hfem_model_set_surrogate(hfem_model *md, hfem_mlp *net, double ba
 
 extern "C" int hfem_model_set_element_operator(hfem_model *md, int form)
 {
-    HIPFEM_REQUIRE(md && (form == 0 || form == 1), HIPFEM_EINVAL, "element operator form must be 0 or 1");
-    md->use_gather = form == 1;
+    HIPFEM_REQUIRE(md && form >= 0 && form <= 2, HIPFEM_EINVAL, "element operator form must be 0, 1 or 2");
+    md->mf_form = form;
     return 0;
 }
 

@@ -457,8 +457,10 @@ class BatchedEvaluator:
                                                     C.c_void_p(yp.addr if yp else 0)), "tangent_product")
         return yf.download()[: d.neq], (yp.download()[: d.npeq] if yp else np.zeros(0))
 
-    def set_element_operator(self, gather=True):
-        _check(lib().hfem_model_set_element_operator(self.h, int(bool(gather))), "set_element_operator")
+    def set_element_operator(self, form=2):
+        """FP32 element operator form: 0 one launch per color, 1 gather,
+        2 per-color slots (default)."""
+        _check(lib().hfem_model_set_element_operator(self.h, int(form)), "set_element_operator")
 
     def set_surrogate(self, net, band=0.05, tol=1e-4):
         _check(lib().hfem_model_set_surrogate(self.h, net.h if net is not None else None, C.c_double(band),

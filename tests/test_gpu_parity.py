@@ -314,9 +314,9 @@ def test_mixed_precision_reliable_updates(ctx, operator):
     assert got["iters"] <= (2.0 if operator == "element" else 2.5) * it64, (got["iters"], it64)
 
 
-@pytest.mark.parametrize("gather", [True, False])
+@pytest.mark.parametrize("form", [0, 1, 2])
 @pytest.mark.parametrize("plastic", [False, True])
-def test_element_operator_matches_assembled_matrix(ctx, plastic, gather):
+def test_element_operator_matches_assembled_matrix(ctx, plastic, form):
     """The FP32 element-by-element operator equals the assembled FP64 matrix
     to FP32 accuracy: elastic (IsoLE, log-normal E) and the MisesMat
     consistent tangent of a plastic state (descriptor form c I_dev + K 1x1 -
@@ -339,7 +339,7 @@ def test_element_operator_matches_assembled_matrix(ctx, plastic, gather):
         s = ev.get_state()
         assert (s[:, 13] > s[:, 6]).mean() > 0.3
     ev.assemble_stiffness(K, tangent=plastic)
-    ev.set_element_operator(gather)
+    ev.set_element_operator(form)
     assert K.is_matrix_free()
     rng = np.random.default_rng(1)
     for _ in range(3):
@@ -347,5 +347,11 @@ def test_element_operator_matches_assembled_matrix(ctx, plastic, gather):
         y64 = K.times(x)
         y32 = K.times_matrix_free(x)
         assert np.linalg.norm(y32 - y64) <= 2e-5 * np.linalg.norm(y64)
+        if form == 2:
+            # the slots are summed in color order: the per-color launches' result, bit for bit
+            ev.set_element_operator(0)
+            y0 = K.times_matrix_free(x)
+            ev.set_element_operator(2)
+            assert np.array_equal(y0, y32)
     K.zero()                                          # any change of the values drops the view
     assert not K.is_matrix_free()

@@ -188,6 +188,15 @@ int hippcg_solve(hippcg_solver *s, hipcsr_mtx *A, const double *b, double *x,
                  int *reason, int *iters, double *achieved);
 int hippcg_solve_dev(hippcg_solver *s, hipcsr_mtx *A, const double *db, double *dx,
                      int *reason, int *iters, double *achieved);
+/* solve(A, B, X) for nrhs right-hand sides (sparselinsystemnm.C:47-70; B
+ * and X n x nrhs, column-major, host; X holds the initial guesses): column
+ * by column, stopping at the first that does not converge.  With
+ * precision 2, the diagonal preconditioner and zero initial guesses the
+ * columns go six at a time through the batched block solver (hfem_batch,
+ * one block), which reads each matrix entry once per iteration for six
+ * columns.  *iters and *achieved are the largest over the columns. */
+int hippcg_solve_multi(hippcg_solver *s, hipcsr_mtx *A, int nrhs, const double *B, double *X,
+                       int *reason, int *iters, double *achieved);
 /* outer iterations (precision 1) or reliable updates (precision 2) of the
  * last mixed-precision solve */
 int hippcg_last_outer(hippcg_solver *s);

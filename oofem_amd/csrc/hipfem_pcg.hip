@@ -1072,6 +1072,90 @@ extern "C" int hippcg_solve_dev(hippcg_solver *s, hipcsr_mtx *A, const double *d
     return solve_dev(s, A, db, dx, reason, iters, achieved);
 }
 
+// SparseLinearSystemNM::solve(A, B, X) (sparselinsystemnm.C:47-70): the
+// columns of B (n x nrhs, column-major, host) in turn, X holding the initial
+// guesses and receiving the solutions, stopping at the first column that
+// does not converge.  With precision 2, the diagonal preconditioner and zero
+// initial guesses the columns go six at a time through the batched solver
+// (one block: its bulk phase over the whole matrix reads every matrix entry
+// once per iteration for six columns; FP32 PCG with FP64 reliable updates,
+// the same method as precision 2).  *iters and *achieved are the largest
+// over the columns.
+extern "C" int hippcg_solve_multi(hippcg_solver *s, hipcsr_mtx *A, int nrhs, const double *B, double *X,
+                                  int *reason, int *iters, double *achieved)
+{
+    HIPFEM_REQUIRE(s && A && reason && iters && achieved && nrhs >= 0 && (nrhs == 0 || (B && X)), HIPFEM_EINVAL,
+                   "solve_multi: bad arguments");
+    HIPFEM_REQUIRE(A->has_structure, HIPFEM_EINVAL, "solve_multi: matrix has no structure");
+    const int n = A->n;
+    *reason = HIPFEM_CR_CONVERGED;
+    *iters = 0;
+    *achieved = 0.0;
+    bool zero = true;
+    for (size_t i = 0; i < (size_t)n * nrhs && zero; i++) zero = X[i] == 0.0;
+    if (s->precision != 2 || s->precond != 1 || !zero || n == 0) {
+        for (int j = 0; j < nrhs; j++) {
+            int it = 0;
+            double ach = 0.0;
+            int rc = hippcg_solve(s, A, B + (size_t)j * n, X + (size_t)j * n, reason, &it, &ach);
+            if (rc) return rc;
+            *iters = std::max(*iters, it);
+            *achieved = std::max(*achieved, ach);
+            if (*reason != HIPFEM_CR_CONVERGED) return 0;
+        }
+        return 0;
+    }
+    DeviceGuard g(s->ctx->device);
+    const int32_t rows[2] = {0, n};
+    hfem_batch *bt = nullptr;
+    int rc = hfem_batch_create(A, 1, rows, &bt);
+    if (rc) return rc;
+    double *db = nullptr, *dx = nullptr;
+    if (hipMalloc(&db, sizeof(double) * 6 * (size_t)n) != hipSuccess ||
+        hipMalloc(&dx, sizeof(double) * 6 * (size_t)n) != hipSuccess) {
+        if (db) (void)hipFree(db);
+        hfem_batch_destroy(bt);
+        set_error("solve_multi: out of device memory");
+        return HIPFEM_ENOMEM;
+    }
+    rc = hfem_batch_set_options(bt, s->tol, s->delta, s->maxit);
+    if (!rc) rc = hfem_batch_set_split(bt, 0);
+    std::vector<double> hb(6 * (size_t)n), hx(6 * (size_t)n);
+    hipStream_t st = s->ctx->stream;
+    for (int j0 = 0; j0 < nrhs && !rc; j0 += 6) {
+        const int m = std::min(6, nrhs - j0);
+        for (int i = 0; i < n; i++)
+            for (int k = 0; k < 6; k++) hb[(size_t)i * 6 + k] = k < m ? B[(size_t)(j0 + k) * n + i] : 0.0;
+        int it = 0, upd = 0;
+        double res = 0.0;
+        if (hipMemcpyAsync(db, hb.data(), sizeof(double) * hb.size(), hipMemcpyHostToDevice, st) != hipSuccess) {
+            set_error("solve_multi: upload failed");
+            rc = HIPFEM_EHIP;
+            break;
+        }
+        rc = hfem_batch_solve_dev(bt, db, dx, &it, &upd, &res);
+        if (rc) break;
+        if (hipMemcpyAsync(hx.data(), dx, sizeof(double) * hx.size(), hipMemcpyDeviceToHost, st) != hipSuccess ||
+            hipStreamSynchronize(st) != hipSuccess) {
+            set_error("solve_multi: download failed");
+            rc = HIPFEM_EHIP;
+            break;
+        }
+        for (int k = 0; k < m; k++)
+            for (int i = 0; i < n; i++) X[(size_t)(j0 + k) * n + i] = hx[(size_t)i * 6 + k];
+        *iters = std::max(*iters, it);
+        *achieved = std::max(*achieved, std::fabs(res));
+        if (res < 0.0) {
+            *reason = HIPFEM_CR_DIVERGED_ITS;
+            break;
+        }
+    }
+    (void)hipFree(db);
+    (void)hipFree(dx);
+    hfem_batch_destroy(bt);
+    return rc;
+}
+
 extern "C" int hippcg_solve(hippcg_solver *s, hipcsr_mtx *A, const double *b, double *x, int *reason, int *iters,
                             double *achieved)
 {

@@ -362,6 +362,19 @@ class HipPCGSolver:
         _check(lib().hippcg_solve(self.h, A.h, _p(b), _p(x), C.byref(reason), C.byref(it), C.byref(ach)), "solve")
         return reason.value, it.value, ach.value
 
+    def solve_multi(self, A: HipCSRSparseMtrx, B, X):
+        """Several right-hand sides (columns of B, n x nrhs); X (n x nrhs,
+        float64, Fortran order, modified in place) holds the initial guesses.
+        Returns (ConvergedReason, largest iterations, largest residual)."""
+        B = np.asfortranarray(B, dtype=np.float64)
+        if not (isinstance(X, np.ndarray) and X.dtype == np.float64 and X.flags.f_contiguous and X.shape == B.shape):
+            raise TypeError("X must be a Fortran-ordered float64 array shaped like B (updated in place)")
+        reason, it, ach = C.c_int(0), C.c_int(0), C.c_double(0)
+        _check(lib().hippcg_solve_multi(self.h, A.h, int(B.shape[1]), C.c_void_p(B.ctypes.data),
+                                        C.c_void_p(X.ctypes.data), C.byref(reason), C.byref(it), C.byref(ach)),
+               "solve_multi")
+        return reason.value, it.value, ach.value
+
     def solve_dev(self, A: HipCSRSparseMtrx, db_ptr, dx_ptr):
         reason, it, ach = C.c_int(0), C.c_int(0), C.c_double(0)
         _check(lib().hippcg_solve_dev(self.h, A.h, C.c_void_p(db_ptr), C.c_void_p(dx_ptr), C.byref(reason),

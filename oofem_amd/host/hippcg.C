@@ -57,6 +57,49 @@ ConvergedReason HipPCGSolver::solve(SparseMtrx &A, FloatArray &b, FloatArray &x)
     return reason == HIPFEM_CR_CONVERGED ? CR_CONVERGED : CR_DIVERGED_ITS;
 }
 
+// SparseLinearSystemNM::solve(A, B, X) sparselinsystemnm.C:47-70
+ConvergedReason SparseLinearSystemNM::solve(SparseMtrx &A, FloatMatrix &B, FloatMatrix &X)
+{
+    const int neq = A.giveNumberOfRows(), nrhs = B.giveNumberOfColumns();
+    if (A.giveNumberOfRows() != B.giveNumberOfRows()) OOFEM_ERROR("A and B matrix mismatch");
+    if (X.giveNumberOfRows() != neq || X.giveNumberOfColumns() != nrhs) X.resize(neq, nrhs);
+    ConvergedReason status = CR_UNKNOWN;
+    FloatArray bi(neq), xi(neq);
+    for (int j = 1; j <= nrhs; ++j) {
+        for (int i = 1; i <= neq; ++i) {
+            bi.at(i) = B.at(i, j);
+            xi.at(i) = X.at(i, j);
+        }
+        status = this->solve(A, bi, xi);
+        if (status != CR_CONVERGED) return status;
+        for (int i = 1; i <= neq; ++i) X.at(i, j) = xi.at(i);
+    }
+    return status;
+}
+
+// several right-hand sides at once on the device (hippcg_solve_multi: six
+// columns per batched launch sequence with precision 2, else column by column)
+ConvergedReason HipPCGSolver::solve(SparseMtrx &A, FloatMatrix &B, FloatMatrix &X)
+{
+    auto *H = dynamic_cast<HipCSRSparseMtrx *>(&A);
+    if (!H) OOFEM_ERROR("hippcg needs a HipCSRSparseMtrx (smtype %d), got %s", (int)SMT_HipCSR, A.giveClassName());
+    const int neq = A.giveNumberOfRows(), nrhs = B.giveNumberOfColumns();
+    if (B.giveNumberOfRows() != neq) OOFEM_ERROR("A and B matrix mismatch");
+    if (X.giveNumberOfRows() != neq || X.giveNumberOfColumns() != nrhs) X.resize(neq, nrhs);
+    std::vector<double> b((size_t)neq * nrhs), x((size_t)neq * nrhs);
+    for (int j = 1; j <= nrhs; ++j)
+        for (int i = 1; i <= neq; ++i) {
+            b[(size_t)(j - 1) * neq + i - 1] = B.at(i, j);
+            x[(size_t)(j - 1) * neq + i - 1] = X.at(i, j);
+        }
+    int reason = 0;
+    hipfemCheck(hippcg_solve_multi(s, H->handle(), nrhs, b.data(), x.data(), &reason, &lastIterations, &lastResidual),
+                "hippcg_solve_multi");
+    for (int j = 1; j <= nrhs; ++j)
+        for (int i = 1; i <= neq; ++i) X.at(i, j) = x[(size_t)(j - 1) * neq + i - 1];
+    return reason == HIPFEM_CR_CONVERGED ? CR_CONVERGED : CR_DIVERGED_ITS;
+}
+
 ConvergedReason HipPCGSolver::solveDevice(HipCSRSparseMtrx &A, const double *db, double *dx)
 {
     int reason = 0;

@@ -30,6 +30,7 @@ public:
     ~HipPCGSolver() override;
     void initializeFrom(InputRecord &ir) override;
     ConvergedReason solve(SparseMtrx &A, FloatArray &b, FloatArray &x) override;
+    ConvergedReason solve(SparseMtrx &A, FloatMatrix &B, FloatMatrix &X) override;
     // device-resident form: b and x are device vectors of the matrix's context
     ConvergedReason solveDevice(HipCSRSparseMtrx &A, const double *db, double *dx);
     LinSystSolverType giveLinSystSolverType() const override { return ST_HipPCG; }

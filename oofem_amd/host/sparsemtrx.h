@@ -76,6 +76,9 @@ public:
     virtual ~SparseLinearSystemNM() {}
     virtual void initializeFrom(InputRecord &ir) {}
     virtual ConvergedReason solve(SparseMtrx &A, FloatArray &b, FloatArray &x) = 0;
+    // several right-hand sides (columns of B), one at a time; X holds the
+    // initial guesses (sparselinsystemnm.C:47-70)
+    virtual ConvergedReason solve(SparseMtrx &A, FloatMatrix &B, FloatMatrix &X);
     virtual LinSystSolverType giveLinSystSolverType() const = 0;
     virtual SparseMtrxType giveRecommendedMatrix(bool symmetric) const { return SMT_Skyline; }
     virtual const char *giveClassName() const = 0;

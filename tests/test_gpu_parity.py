@@ -355,3 +355,35 @@ def test_element_operator_matches_assembled_matrix(ctx, plastic, form):
             assert np.array_equal(y0, y32)
     K.zero()                                          # any change of the values drops the view
     assert not K.is_matrix_free()
+
+
+@pytest.mark.parametrize("precision", [0, 2])
+def test_pcg_multiple_right_hand_sides(ctx, precision):
+    """solve(A, B, X) with eight columns, one of them zero, on config 0: every
+    column meets the tolerance on its true residual (checked in numpy on the
+    oracle's CSR) and the zero column stays zero.  Precision 2 with zero
+    initial guesses takes the batched path (six columns per batched solve),
+    precision 0 the column-by-column loop of the reference's default."""
+    import scipy.sparse as sp
+
+    from oofem_amd.hipfem import BatchedEvaluator, HipCSRSparseMtrx, HipPCGSolver
+    d = meshgen.config0()
+    rp, ci, val = oracle.csr(d)
+    A = sp.csr_matrix((val, ci, rp), shape=(d.neq, d.neq))
+    K = HipCSRSparseMtrx(ctx)
+    ev = BatchedEvaluator(ctx, d)
+    ev.build_structure(K)
+    ev.assemble_stiffness(K)
+    rng = np.random.default_rng(3)
+    B = rng.standard_normal((d.neq, 8))
+    B[:, 7] = 0.0
+    X = np.zeros_like(B, order="F")
+    tol = 1e-9
+    s = HipPCGSolver(ctx, lstol=tol, lsiter=100000, lsprecond=1, precision=precision)
+    reason, it, ach = s.solve_multi(K, B, X)
+    assert reason == 1 and it > 0
+    for j in range(7):
+        r = np.linalg.norm(B[:, j] - A @ X[:, j]) / np.linalg.norm(B[:, j])
+        assert r <= 5 * tol, (j, r)
+    assert not X[:, 7].any()
+    s.close()

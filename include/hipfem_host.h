@@ -49,6 +49,10 @@ int hfem_problem_set_forcing(hfem_problem *p, int on, double eta_max);
  * correction once more afterwards (nrsolver.C:356). */
 int hfem_problem_set_nr_extras(hfem_problem *p, int linesearch, double ls_tol, double ls_amp, double ls_maxeta,
                                double maxinc);
+/* NonLinearStatic `stiffmode` (nlinearstatic.h:62-64): 0 = the tangent
+ * stiffness for every Newton matrix, 2 = the initial elastic stiffness,
+ * assembled once and kept; 1 (secant) is refused */
+int hfem_problem_set_stiffness_mode(hfem_problem *p, int mode);
 /* StaticStructural time stepping from the current time to final_time
  * (EngngModel::solveYourself engngm.C:573-662): prescribed values and
  * external load at time t are up_ref / f_free / f_presc times the

@@ -225,6 +225,12 @@ def read_deck(path) -> DeckProblem:
         raise DeckError("deck too short")
     pos = 2                                  # output file, description
     engng = Record(lines[pos]); pos += 1
+    # metastep records follow the engineering-model record (EngngModel::
+    # instanciateMetaSteps reads nmsteps of them); they carry the solver
+    # keywords of their steps
+    nmsteps = engng.scalar("nmsteps", 0, int)
+    metasteps = [Record("metastep 0 " + lines[pos + i]) for i in range(nmsteps)]
+    pos += nmsteps
     nmodules = engng.scalar("nmodules", 0, int)
     pos += nmodules                          # export module records
     if not lines[pos].lower().startswith("domain"):
@@ -344,6 +350,15 @@ def read_deck(path) -> DeckProblem:
     prob = DeckProblem(deck, engng.keyword, engng.scalar("nsteps", 1, int), engng, blist, llist, ltf,
                        node_numbers, elem_numbers, checks, tol)
     prob.output_file = lines[0].strip()
+    prob.metasteps = metasteps
+    if metasteps:
+        # one metastep is what the drivers here run: its step count and
+        # increment replace the engineering model's
+        if len(metasteps) > 1:
+            raise DeckError("decks with more than one metastep are not supported")
+        prob.nsteps = metasteps[0].scalar("nsteps", prob.nsteps, int)
+        if metasteps[0].has("deltat"):
+            prob.engng = Record(engng.line + f" deltat {metasteps[0].scalar('deltat')}")
     prob.description = lines[1].strip()
     return prob
 

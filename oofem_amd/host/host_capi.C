@@ -120,6 +120,15 @@ int hfem_problem_set_nr_extras(hfem_problem *p, int linesearch, double ls_tol, d
     });
 }
 
+int hfem_problem_set_stiffness_mode(hfem_problem *p, int mode)
+{
+    return guarded([&] {
+        if (mode == 1) OOFEM_ERROR("secant stiffness (stiffmode 1) is not part of this build");
+        if (mode != 0 && mode != 2) OOFEM_ERROR("unknown stiffness mode %d", mode);
+        reinterpret_cast<StructuralProblem *>(p)->nr.stiffMode = mode;
+    });
+}
+
 int hfem_problem_run_steps(hfem_problem *p, const double *up_ref, const double *f_free, const double *f_presc,
                            int nltf, const double *ltf_t, const double *ltf_v, double final_time, double dt,
                            int reduction, double dtmin, double dtmax, double ncrf, int nmaxrestarts,

@@ -109,6 +109,7 @@ void StructuralProblem::assemble(SparseMtrx &answer, int mode)
     auto *H = dynamic_cast<HipCSRSparseMtrx *>(&answer);
     if (!H) OOFEM_ERROR("the batched evaluator assembles into a HipCSRSparseMtrx only (got %s)", answer.giveClassName());
     hipfemCheck(hfem_model_assemble_stiffness(ev, H->handle(), mode), "assemble");
+    if (&answer == K.get()) kMode = mode;
 }
 
 void StructuralProblem::internalForces(const DeviceVector &x, bool updateState, double *ebe3)
@@ -165,6 +166,17 @@ ConvergedReason StructuralProblem::solveLinearStatic(int &iterations, double &re
     return s;
 }
 
+// the Newton matrix: the tangent, or the elastic stiffness assembled once
+// (NonLinearStatic::updateComponent NonLinearLhs, nlinearstatic.C:616-645)
+void StructuralProblem::assembleNewtonMatrix()
+{
+    if (nr.stiffMode == 2) {
+        if (kMode != 0) assemble(*K, 0);
+    } else {
+        assemble(*K, 1);
+    }
+}
+
 // StaticStructural::solveYourselfAt staticstructural.C:224-358 and
 // NRSolver::solve nrsolver.C:205-404.
 NRStepReport StructuralProblem::solveNonlinearStep()
@@ -185,7 +197,7 @@ NRStepReport StructuralProblem::solveNonlinearStep()
         hipfemCheck(hfem_model_tangent_product_dev(ev, 1, nullptr, dup.ptr(), rhs.ptr(), nullptr), "extrapolate");
         rhs.axpby(0.0, nullptr, -1.0);
         if (rhs.dot(rhs) > 0.0) {
-            assemble(*K, 1);                        // updateMatrix
+            assembleNewtonMatrix();                 // updateMatrix
             tangentCurrent = true;
             ddX.zero();
             ConvergedReason ls = linearSolve(rhs, ddX);
@@ -201,7 +213,7 @@ NRStepReport StructuralProblem::solveNonlinearStep()
     // stiffness before the first residual (mCalcStiffBeforeRes, nrsolver.C:245-256);
     // the matrix of the initial guess is that same tangent (no Gauss-point
     // state changed since), so it is reused
-    if (!tangentCurrent) assemble(*K, 1);
+    if (!tangentCurrent) assembleNewtonMatrix();
     auto *pcg = dynamic_cast<HipPCGSolver *>(linSolver.get());
     const double lstol = pcg ? pcg->giveTolerance() : 0.0;
     double prevErr = -1.0, prevEta = -1.0;
@@ -227,7 +239,7 @@ NRStepReport StructuralProblem::solveNonlinearStep()
         if (converged && nite >= nr.minIterations) { rep.status = CR_CONVERGED; break; }
         if (nite >= nr.maxiter) { rep.status = CR_DIVERGED_ITS; break; }
         if (nite > 0 || !nr.calcStiffBeforeRes) {
-            if (nr.manrmsteps > 0 && nite % nr.manrmsteps == 0) assemble(*K, 1);
+            if (nr.manrmsteps > 0 && nite % nr.manrmsteps == 0) assembleNewtonMatrix();
         }
         if (nr.forcing && pcg) {
             double eta = nr.forcingMax;

@@ -70,6 +70,10 @@ struct NRSolverOptions {
     // cap on the largest component of a correction (NRSolver `maxinc`,
     // nrsolver.C:340-353)
     double maxInc = 1.0e20;
+    // stiffness of the Newton matrix (NonLinearStatic `stiffmode`,
+    // nlinearstatic.h:62-64): 0 the tangent, 2 the initial elastic
+    // stiffness, assembled once and kept (nlinearstatic.C:635-644)
+    int stiffMode = 0;
 };
 
 // SimpleReductionStrategy (timestepreductionstrategy.C:40-110) of the
@@ -115,6 +119,8 @@ class StructuralProblem : public EngngModel {
     DeviceVector X, up, upPrev, dup, RT, RTp, F, Fp, rhs, ddX;
     DeviceVector rb, gls, Xc;   // line-search start and residual, last converged solution
     double currentTime = 0.0;   // target time of the last accepted step of solveTimeSteps
+    int kMode = -1;             // stiffness the values of K hold: 0 elastic, 1 tangent, -1 none
+    void assembleNewtonMatrix();
     bool structureBuilt = false;
     int lastLinearIterations = 0;
     bool profiling = false;

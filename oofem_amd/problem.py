@@ -41,6 +41,7 @@ def hostlib():
         L.hfem_problem_set_options.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, C.c_int,
                                                C.c_double, C.c_double, C.c_int, C.c_int]
         L.hfem_problem_set_forcing.argtypes = [C.c_void_p, C.c_int, C.c_double]
+        L.hfem_problem_set_stiffness_mode.argtypes = [C.c_void_p, C.c_int]
         L.hfem_problem_set_nr_extras.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]
         L.hfem_problem_run_steps.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
                                              C.c_void_p, C.c_double, C.c_double, C.c_int, C.c_double, C.c_double,
@@ -61,10 +62,12 @@ def _p(a):
 
 class StructuralProblem:
     def __init__(self, deck, smtype=SMT_HipCSR, lstype=ST_HipPCG, lstol=1e-10, lsiter=100000, lsprecond=1,
-                 mixed=0, irtol=1e-4, rtolf=1e-8, maxiter=50, manrmsteps=1, initial_guess=1, forcing=0.0):
+                 mixed=0, irtol=1e-4, rtolf=1e-8, maxiter=50, manrmsteps=1, initial_guess=1, forcing=0.0,
+                 stiff_mode=0):
         """forcing > 0: inexact Newton, corrections solved to forcing terms
         of at most this value (hfem_problem_set_forcing); 0: every
-        correction to lstol, as the reference's NRSolver."""
+        correction to lstol, as the reference's NRSolver.  stiff_mode:
+        NonLinearStatic's stiffmode (0 tangent, 2 initial elastic)."""
         self.deck = d = deck
         self._keep = [np.ascontiguousarray(x) for x in (d.coords, d.conn, d.emat, d.mat, d.eqn)]
         c, cn, em, mt, eq = self._keep
@@ -76,6 +79,7 @@ class StructuralProblem:
         _check(hostlib().hfem_problem_set_initial_guess(self.h, int(initial_guess)), "set_initial_guess")
         _check(hostlib().hfem_problem_set_forcing(self.h, int(forcing > 0), float(forcing) if forcing > 0 else 0.1),
                "set_forcing")
+        _check(hostlib().hfem_problem_set_stiffness_mode(self.h, int(stiff_mode)), "set_stiffness_mode")
 
     def set_nr_extras(self, linesearch=False, ls_tol=0.8, ls_amp=2.5, ls_maxeta=4.0, maxinc=1e20):
         """NRSolver line search (linesearch / lsearchtol / lsearchamp /

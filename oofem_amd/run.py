@@ -6,7 +6,15 @@ The engineering-model record chooses the driver:
 - ``LinearStatic``: one linear solve per step with that step's prescribed
   values and loads (LinearStatic::solveYourselfAt, linearstatic.C:177-252);
 - ``StaticStructural``: one NRSolver step per time step
-  (StaticStructural::solveYourselfAt, staticstructural.C:224-358).
+  (StaticStructural::solveYourselfAt, staticstructural.C:224-358);
+- ``NonLinearStatic`` with direct control (``controlmode 1``): one NRSolver
+  step per time step on the total loads of that time
+  (NonLinearStatic::proceedStep, nlinearstatic.C:453-567; refloadmode 0),
+  no initial guess unless ``initialguess 1``, and the Newton matrix from
+  ``stiffmode`` (0 tangent, 2 initial elastic).  Arc-length control
+  (controlmode 0, CylindricalALM), incremental reference loads
+  (refloadmode 1) and the secant stiffness are refused.  Solver keywords may
+  sit on the first metastep record.
 
 Its solver keywords are read as the reference reads them: smtype, lstype,
 lstol, lsiter, lsprecond, rtolf, maxiter, manrmsteps, initialguess.
@@ -51,12 +59,17 @@ def run_deck(path, out_path=None, keywords=None, log=None):
     prob = read_deck(path)
     e = prob.engng
     model = prob.model
-    if model not in ("linearstatic", "staticstructural"):
+    if model not in ("linearstatic", "staticstructural", "nonlinearstatic"):
         raise ValueError(f"engineering model '{model}' is not part of this build")
     over = {k.lower(): v for k, v in (keywords or {}).items()}
+    ms = prob.metasteps[0] if getattr(prob, "metasteps", None) else None
 
     def kwd(key, default, cast=float):
-        return cast(over[key]) if key in over else e.scalar(key, default, cast)
+        if key in over:
+            return cast(over[key])
+        if ms is not None and ms.has(key):
+            return ms.scalar(key, default, cast)
+        return e.scalar(key, default, cast)
 
     # IMLSolver / NRSolver / EngngModel defaults (imlsolver.C:67-101,
     # nrsolver.C:106-200, staticstructural.C:130)
@@ -65,6 +78,15 @@ def run_deck(path, out_path=None, keywords=None, log=None):
     if model == "staticstructural":
         kw.update(rtolf=kwd("rtolf", 1e-3), maxiter=kwd("maxiter", 100000000, int),
                   manrmsteps=kwd("manrmsteps", 0, int), initial_guess=kwd("initialguess", 1, int))
+    elif model == "nonlinearstatic":
+        if kwd("controlmode", 0, int) != 1:
+            raise ValueError("NonLinearStatic arc-length control (controlmode 0, CylindricalALM) is not part of this "
+                             "build; use controlmode 1")
+        if kwd("refloadmode", 0, int) != 0:
+            raise ValueError("NonLinearStatic incremental reference loads (refloadmode 1) are not part of this build")
+        kw.update(rtolf=kwd("rtolf", 1e-3), maxiter=kwd("maxiter", 100000000, int),
+                  manrmsteps=kwd("manrmsteps", 0, int), initial_guess=kwd("initialguess", 0, int),
+                  stiff_mode=kwd("stiffmode", 0, int))
     d = prob.deck
     P = StructuralProblem(d, **kw)
     results, iters = {}, {}
@@ -86,7 +108,7 @@ def run_deck(path, out_path=None, keywords=None, log=None):
                 if rep["status"] != 1:
                     raise RuntimeError(f"step {k}: Newton loop did not converge: {rep}")
             results[k] = _collect(prob, P, P.solution(), up)
-            if model == "staticstructural":
+            if model != "linearstatic":
                 P.update()
             if f:
                 write_step(f, prob, k, prob.time_of_step(k), results[k], time.perf_counter() - t0)

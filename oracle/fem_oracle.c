@@ -982,7 +982,7 @@ int orc_nonlinear_step(const orc_mesh *m, double *state, double *X, const double
 {
     int o4[4];
     const int rc = orc_nonlinear_step_ex(m, state, X, u_presc, u_presc_prev, initial_guess, RT, rtolf, maxiter,
-                                         manrmsteps, lstol, 0, 0.8, 2.5, 4.0, 1.0e20, o4, err);
+                                         manrmsteps, lstol, 0, 0.8, 2.5, 4.0, 1.0e20, 0, o4, err);
     out[0] = o4[0];
     out[1] = o4[1];
     out[2] = o4[2];
@@ -1005,8 +1005,9 @@ static int orc_ls_product(const orc_mesh *m, double *state, const double *rb, co
 int orc_nonlinear_step_ex(const orc_mesh *m, double *state, double *X, const double *u_presc,
                           const double *u_presc_prev, int initial_guess, const double *RT, double rtolf, int maxiter,
                           int manrmsteps, double lstol, int linesearch, double ls_tol, double ls_amp, double ls_maxeta,
-                          double maxinc, int *out, double *err)
+                          double maxinc, int stiffmode, int *out, double *err)
 {
+    const int tangent = stiffmode == 2 ? 0 : 1;
     const long long nnz = orc_csr_count(m);
     long long *rowptr = (long long *)malloc(sizeof(long long) * ((size_t)m->neq + 1));
     int *colind = (int *)malloc(sizeof(int) * (size_t)nnz);
@@ -1037,7 +1038,7 @@ int orc_nonlinear_step_ex(const orc_mesh *m, double *state, double *X, const dou
         double nrm = 0.0;
         for (int i = 0; i < m->neq; i++) { rhs[i] = -rhs[i]; nrm += rhs[i] * rhs[i]; }
         if (nrm > 0.0) {
-            if (orc_assemble_stiffness(m, rowptr, colind, val, state, 1)) { rc = -10; goto out; }
+            if (orc_assemble_stiffness(m, rowptr, colind, val, state, tangent)) { rc = -10; goto out; }
             int it = 0;
             double res = 0.0;
             orc_pcg(m->neq, rowptr, colind, val, rhs, ddX, lstol, 1000000, 1, &it, &res);
@@ -1045,7 +1046,7 @@ int orc_nonlinear_step_ex(const orc_mesh *m, double *state, double *X, const dou
             for (int i = 0; i < m->neq; i++) { X[i] += ddX[i]; ddX[i] = 0.0; }
         }
     }
-    if (orc_assemble_stiffness(m, rowptr, colind, val, state, 1)) { rc = -10; goto out; }
+    if (orc_assemble_stiffness(m, rowptr, colind, val, state, tangent)) { rc = -10; goto out; }
     for (int nite = 0;; nite++) {
         double ebe[3];
         if (orc_internal_forces(m, X, u_presc, state, 1, F, NULL, ebe)) { rc = -11; goto out; }
@@ -1073,7 +1074,7 @@ int orc_nonlinear_step_ex(const orc_mesh *m, double *state, double *X, const dou
         if (converged) { out[2] = 1; break; }
         if (nite >= maxiter) { out[2] = -1; break; }
         if (nite > 0 && manrmsteps > 0 && nite % manrmsteps == 0)
-            if (orc_assemble_stiffness(m, rowptr, colind, val, state, 1)) { rc = -10; goto out; }
+            if (orc_assemble_stiffness(m, rowptr, colind, val, state, tangent)) { rc = -10; goto out; }
         int it = 0;
         double res = 0.0;
         orc_pcg(m->neq, rowptr, colind, val, rhs, ddX, lstol, 1000000, 1, &it, &res);

@@ -114,11 +114,13 @@ int orc_nonlinear_step(const orc_mesh *m, double *state, double *X, const double
  * (ls_tol, ls_amp, ls_maxeta as lsearchtol / lsearchamp / lsearchmaxeta,
  * clamped as in LineSearchNM::initializeFrom); maxinc caps the largest
  * component of a correction (maxIncAllowed).  out[3] receives the number
- * of line searches that ended with eta != 1. */
+ * of line searches that ended with eta != 1.  stiffmode: NonLinearStatic's
+ * stiffness (nlinearstatic.h:62-64): 0 the tangent, 2 the elastic stiffness
+ * for every Newton matrix (the initial guess keeps the tangent product). */
 int orc_nonlinear_step_ex(const orc_mesh *m, double *state, double *X, const double *u_presc,
                           const double *u_presc_prev, int initial_guess, const double *RT, double rtolf, int maxiter,
                           int manrmsteps, double lstol, int linesearch, double ls_tol, double ls_amp, double ls_maxeta,
-                          double maxinc, int *out, double *err);
+                          double maxinc, int stiffmode, int *out, double *err);
 
 /* Skyline U^T D U factorization + solve of a full symmetric CSR (direct
  * solver of the reference; CPU baseline only).  *nwk_out: profile size. */

@@ -234,9 +234,10 @@ def nonlinear_step(deck, state, X, u_presc, RT=None, rtolf=1e-8, maxiter=50, man
 
 def nonlinear_step_ex(deck, state, X, u_presc, u_presc_prev=None, initial_guess=True, RT=None, rtolf=1e-8,
                       maxiter=50, manrmsteps=1, lstol=1e-12, linesearch=False, ls_tol=0.8, ls_amp=2.5, ls_maxeta=4.0,
-                      maxinc=1e20):
+                      maxinc=1e20, stiffmode=0):
     """nonlinear_step with the NRSolver extras: line search (LineSearchNM,
-    linesearch.C:54-245) and the increment cap maxinc (nrsolver.C:340-353).
+    linesearch.C:54-245), the increment cap maxinc (nrsolver.C:340-353) and
+    NonLinearStatic's stiffness mode (0 tangent, 2 elastic).
     Returns dict(newton, pcg, status, force_error, line_searches)."""
     m = Mesh(deck)
     out = (C.c_int * 4)()
@@ -249,8 +250,8 @@ def nonlinear_step_ex(deck, state, X, u_presc, u_presc_prev=None, initial_guess=
                                      int(bool(initial_guess)),
                                      rt.ctypes.data_as(C.c_void_p) if rt is not None else None, C.c_double(rtolf),
                                      maxiter, manrmsteps, C.c_double(lstol), int(bool(linesearch)), C.c_double(ls_tol),
-                                     C.c_double(ls_amp), C.c_double(ls_maxeta), C.c_double(maxinc), out,
-                                     C.byref(err))
+                                     C.c_double(ls_amp), C.c_double(ls_maxeta), C.c_double(maxinc), int(stiffmode),
+                                     out, C.byref(err))
     if rc:
         raise RuntimeError(f"orc_nonlinear_step_ex rc={rc}")
     return dict(newton=out[0], pcg=out[1], status=out[2], force_error=err.value, line_searches=out[3])

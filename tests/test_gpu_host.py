@@ -243,3 +243,46 @@ def test_run_deck_default_solver_is_rejected():
     from tests.conftest import GOLDEN
     with pytest.raises(ProblemError):
         run_deck(os.path.join(GOLDEN, "patch301.in"))
+
+
+def test_elastic_stiffness_newton_matches_oracle():
+    """NonLinearStatic stiffmode 2 on the GPU driver (the elastic matrix is
+    assembled once and kept): the oracle's Newton iteration count and root."""
+    from oofem_amd.problem import StructuralProblem
+    d = meshgen.config3(4, 4, 16)
+    d.mat = d.mat.copy()
+    d.mat[0, 4] = 5e4
+    up = d.prescribed_values(3 / 8)
+    st = oracle.state_new(d)
+    X = np.zeros(d.neq)
+    ref = oracle.nonlinear_step_ex(d, st, X, up, initial_guess=False, maxiter=400, lstol=1e-12, stiffmode=2)
+    assert ref["status"] == 1
+    P = StructuralProblem(d, lstol=1e-12, rtolf=1e-8, maxiter=400, manrmsteps=1, initial_guess=0, stiff_mode=2)
+    P.set_load(np.zeros(d.neq), np.zeros(d.npeq))
+    P.set_prescribed(up)
+    rep = P.nonlinear_step()
+    assert rep["status"] == 1 and abs(rep["newton"] - ref["newton"]) <= 1, (rep, ref)
+    assert np.linalg.norm(P.solution() - X) <= 1e-8 * np.linalg.norm(X)
+    P.close()
+
+
+def test_run_nonlinear_static_deck(tmp_path):
+    """patch301 rewritten as a NonLinearStatic deck (direct control, elastic
+    stiffness, one metastep of one step): every errorcheck record holds."""
+    import os
+
+    from oofem_amd.run import run_deck
+    from tests.conftest import GOLDEN
+    src = open(os.path.join(GOLDEN, "patch301.in")).read().splitlines()
+    i = next(k for k, l in enumerate(src) if l.lower().startswith("staticstructural"))
+    src[i] = "NonLinearStatic nsteps 1 nmsteps 1 nmodules 1"
+    src.insert(i + 1, "nsteps 1 controlmode 1 stiffmode 2 rtolf 1e-10")
+    p = tmp_path / "nls.in"
+    p.write_text("\n".join(src) + "\n")
+    r = run_deck(str(p), keywords=dict(smtype=11, lstype=9, lstol=1e-12))
+    bad = [(c.line, v) for c, v, ok in r["checks"] if not ok]
+    assert r["checks"] and not bad, bad
+    with pytest.raises(ValueError):
+        src[i + 1] = "nsteps 1 controlmode 0"
+        p.write_text("\n".join(src) + "\n")
+        run_deck(str(p), keywords=dict(smtype=11, lstype=9))

@@ -49,6 +49,15 @@ int hfem_problem_set_forcing(hfem_problem *p, int on, double eta_max);
  * correction once more afterwards (nrsolver.C:356). */
 int hfem_problem_set_nr_extras(hfem_problem *p, int linesearch, double ls_tol, double ls_amp, double ls_maxeta,
                                double maxinc);
+/* NRSolver direct displacement control (`ddm` / `ddv`, nrsolver.C:165-176,
+ * 458-568): the free equations eqs (1-based; the deck's node / dof pairs
+ * mapped to equations) are driven by a 1e6 penalty on the diagonal of every
+ * Newton matrix, their residual in the first iteration of a step being
+ * K_ii * value * factor; set_ddm_factor sets that step's factor (the
+ * increment of the `ddltf` function over the step, its value at the first
+ * step) */
+int hfem_problem_set_ddm(hfem_problem *p, int n, const int *eqs, const double *values);
+int hfem_problem_set_ddm_factor(hfem_problem *p, double factor);
 /* NonLinearStatic `stiffmode` (nlinearstatic.h:62-64): 0 = the tangent
  * stiffness for every Newton matrix, 2 = the initial elastic stiffness,
  * assembled once and kept; 1 (secant) is refused */

@@ -120,6 +120,23 @@ int hfem_problem_set_nr_extras(hfem_problem *p, int linesearch, double ls_tol, d
     });
 }
 
+int hfem_problem_set_ddm(hfem_problem *p, int n, const int *eqs, const double *values)
+{
+    return guarded([&] {
+        auto *P = reinterpret_cast<StructuralProblem *>(p);
+        if (n < 0 || (n > 0 && (!eqs || !values))) OOFEM_ERROR("ddm: bad arguments");
+        for (int k = 0; k < n; k++)
+            if (eqs[k] < 1 || eqs[k] > P->giveNeq()) OOFEM_ERROR("ddm: equation %d is not a free equation", eqs[k]);
+        P->nr.ddmEq.assign(eqs, eqs + n);
+        P->nr.ddmVal.assign(values, values + n);
+    });
+}
+
+int hfem_problem_set_ddm_factor(hfem_problem *p, double factor)
+{
+    return guarded([&] { reinterpret_cast<StructuralProblem *>(p)->nr.ddmFactor = factor; });
+}
+
 int hfem_problem_set_stiffness_mode(hfem_problem *p, int mode)
 {
     return guarded([&] {

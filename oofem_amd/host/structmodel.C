@@ -58,6 +58,9 @@ StructuralProblem::StructuralProblem(int nnode_, const double *coords, int etype
     hipfemCheck(hfem_model_set_mesh(ev, nnode, coords, etype, nelem, conn_, emat, nmat, mat, eqn_, neq, npeq),
                 "set_mesh");
     for (DeviceVector *v : {&X, &RT, &F, &rhs, &ddX, &rb, &gls, &Xc}) v->allocate(ctx, neq);
+    eqDofId.assign((size_t)std::max(neq, 0), 0);
+    for (size_t i = 0; i < eqn.size(); i++)
+        if (eqn[i] > 0) eqDofId[eqn[i] - 1] = (int8_t)(i % 3);
     for (DeviceVector *v : {&up, &upPrev, &dup, &RTp, &Fp}) v->allocate(ctx, npeq);
     // IMLSolver defaults unless the caller overrides them
     solverRecord.set("lsprecond", 1);
@@ -177,6 +180,41 @@ void StructuralProblem::assembleNewtonMatrix()
     }
 }
 
+// NRSolver::applyConstraintsToStiffness nrsolver.C:458-500: K_ii *= 1e6 at
+// the directly controlled equations, once per matrix version
+void StructuralProblem::applyConstraintsToStiffness()
+{
+    if (nr.ddmEq.empty()) return;
+    auto *H = dynamic_cast<HipCSRSparseMtrx *>(K.get());
+    if (!H) OOFEM_ERROR("direct displacement control needs the HipCSR matrix");
+    if (hipcsr_version(H->handle()) == smConstraintVersion) return;
+    for (int q : nr.ddmEq) {
+        double v = 0.0;
+        hipfemCheck(hipcsr_get(H->handle(), q, q, &v), "ddm diagonal");
+        hipfemCheck(hipcsr_set(H->handle(), q, q, v * 1.e6), "ddm diagonal");
+    }
+    smConstraintVersion = hipcsr_version(H->handle());
+}
+
+// NRSolver::applyConstraintsToLoadIncrement nrsolver.C:502-568: the residual
+// at a directly controlled equation is K_ii * value * factor in the first
+// iteration and 0 afterwards; the per-dof-id squared sums follow
+void StructuralProblem::applyConstraintsToLoadIncrement(int nite, double sums6[6])
+{
+    if (nr.ddmEq.empty()) return;
+    auto *H = dynamic_cast<HipCSRSparseMtrx *>(K.get());
+    for (size_t k = 0; k < nr.ddmEq.size(); k++) {
+        const int q = nr.ddmEq[k];
+        double old = 0.0, kii = 0.0;
+        hipfemCheck(hipfem_dev_download(ctx, &old, rhs.ptr() + q - 1, 1), "ddm residual");
+        if (nite == 0) hipfemCheck(hipcsr_get(H->handle(), q, q, &kii), "ddm diagonal");
+        const double v = nite == 0 ? kii * nr.ddmVal[k] * nr.ddmFactor : 0.0;
+        hipfemCheck(hipfem_dev_upload(ctx, rhs.ptr() + q - 1, &v, 1), "ddm residual");
+        double &sr = sums6[eqDofId[q - 1]];
+        sr = std::max(0.0, sr + v * v - old * old);
+    }
+}
+
 // StaticStructural::solveYourselfAt staticstructural.C:224-358 and
 // NRSolver::solve nrsolver.C:205-404.
 NRStepReport StructuralProblem::solveNonlinearStep()
@@ -214,6 +252,7 @@ NRStepReport StructuralProblem::solveNonlinearStep()
     // the matrix of the initial guess is that same tangent (no Gauss-point
     // state changed since), so it is reused
     if (!tangentCurrent) assembleNewtonMatrix();
+    applyConstraintsToStiffness();
     auto *pcg = dynamic_cast<HipPCGSolver *>(linSolver.get());
     const double lstol = pcg ? pcg->giveTolerance() : 0.0;
     double prevErr = -1.0, prevEta = -1.0;
@@ -221,6 +260,7 @@ NRStepReport StructuralProblem::solveNonlinearStep()
         double ebe[3], sums[6];
         internalForces(X, true, ebe);               // updateComponent(InternalRhs)
         hipfemCheck(hfem_model_residual(ev, RT.ptr(), F.ptr(), rhs.ptr(), sums), "residual");
+        applyConstraintsToLoadIncrement(nite, sums);
         // checkConvergence nrsolver.C:594-885, grouped by dof id; rtold is 0 (off)
         bool converged = true, outOfRange = false;
         double maxErr = 0.0;
@@ -239,7 +279,10 @@ NRStepReport StructuralProblem::solveNonlinearStep()
         if (converged && nite >= nr.minIterations) { rep.status = CR_CONVERGED; break; }
         if (nite >= nr.maxiter) { rep.status = CR_DIVERGED_ITS; break; }
         if (nite > 0 || !nr.calcStiffBeforeRes) {
-            if (nr.manrmsteps > 0 && nite % nr.manrmsteps == 0) assembleNewtonMatrix();
+            if (nr.manrmsteps > 0 && nite % nr.manrmsteps == 0) {
+                assembleNewtonMatrix();
+                applyConstraintsToStiffness();
+            }
         }
         if (nr.forcing && pcg) {
             double eta = nr.forcingMax;

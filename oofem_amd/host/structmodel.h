@@ -74,6 +74,13 @@ struct NRSolverOptions {
     // nlinearstatic.h:62-64): 0 the tangent, 2 the initial elastic
     // stiffness, assembled once and kept (nlinearstatic.C:635-644)
     int stiffMode = 0;
+    // direct displacement control (NRSolver `ddm` / `ddv` / `ddltf`,
+    // nrsolver.C:165-176, 458-568): free equations (1-based) held by a 1e6
+    // diagonal penalty, their values, and the step's increment factor of
+    // the ddltf function
+    std::vector<int> ddmEq;
+    std::vector<double> ddmVal;
+    double ddmFactor = 0.0;
 };
 
 // SimpleReductionStrategy (timestepreductionstrategy.C:40-110) of the
@@ -109,6 +116,7 @@ class StructuralProblem : public EngngModel {
     hipfem_ctx *ctx;
     int nnode, etype, nelem, nmat, neq, npeq;
     std::vector<int> conn, eqn;
+    std::vector<int8_t> eqDofId;   // dof id (0 u, 1 v, 2 w) of every free equation
     hfem_model *ev = nullptr;
     SparseMtrxType smtype = SMT_HipCSR;
     LinSystSolverType lstype = ST_HipPCG;
@@ -120,7 +128,10 @@ class StructuralProblem : public EngngModel {
     DeviceVector rb, gls, Xc;   // line-search start and residual, last converged solution
     double currentTime = 0.0;   // target time of the last accepted step of solveTimeSteps
     int kMode = -1;             // stiffness the values of K hold: 0 elastic, 1 tangent, -1 none
+    long smConstraintVersion = -1;   // K version after the last ddm penalty
     void assembleNewtonMatrix();
+    void applyConstraintsToStiffness();
+    void applyConstraintsToLoadIncrement(int nite, double sums6[6]);
     bool structureBuilt = false;
     int lastLinearIterations = 0;
     bool profiling = false;

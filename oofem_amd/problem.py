@@ -42,6 +42,8 @@ def hostlib():
                                                C.c_double, C.c_double, C.c_int, C.c_int]
         L.hfem_problem_set_forcing.argtypes = [C.c_void_p, C.c_int, C.c_double]
         L.hfem_problem_set_stiffness_mode.argtypes = [C.c_void_p, C.c_int]
+        L.hfem_problem_set_ddm.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
+        L.hfem_problem_set_ddm_factor.argtypes = [C.c_void_p, C.c_double]
         L.hfem_problem_set_nr_extras.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]
         L.hfem_problem_run_steps.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
                                              C.c_void_p, C.c_double, C.c_double, C.c_int, C.c_double, C.c_double,
@@ -80,6 +82,16 @@ class StructuralProblem:
         _check(hostlib().hfem_problem_set_forcing(self.h, int(forcing > 0), float(forcing) if forcing > 0 else 0.1),
                "set_forcing")
         _check(hostlib().hfem_problem_set_stiffness_mode(self.h, int(stiff_mode)), "set_stiffness_mode")
+
+    def set_ddm(self, eqs, values):
+        """NRSolver direct displacement control on free equations (1-based)."""
+        e = np.ascontiguousarray(eqs, dtype=np.int32)
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        _check(hostlib().hfem_problem_set_ddm(self.h, len(e), _p(e) if len(e) else None, _p(v) if len(v) else None),
+               "set_ddm")
+
+    def set_ddm_factor(self, factor):
+        _check(hostlib().hfem_problem_set_ddm_factor(self.h, float(factor)), "set_ddm_factor")
 
     def set_nr_extras(self, linesearch=False, ls_tol=0.8, ls_amp=2.5, ls_maxeta=4.0, maxinc=1e20):
         """NRSolver line search (linesearch / lsearchtol / lsearchamp /

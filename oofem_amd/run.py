@@ -15,6 +15,8 @@ The engineering-model record chooses the driver:
   (controlmode 0, CylindricalALM), incremental reference loads
   (refloadmode 1) and the secant stiffness are refused.  Solver keywords may
   sit on the first metastep record.
+NRSolver's direct displacement control (``ddm`` node/dof pairs, ``ddv``
+values, ``ddltf`` function) is applied in both nonlinear drivers.
 
 Its solver keywords are read as the reference reads them: smtype, lstype,
 lstol, lsiter, lsprecond, rtolf, maxiter, manrmsteps, initialguess.
@@ -89,6 +91,23 @@ def run_deck(path, out_path=None, keywords=None, log=None):
                   stiff_mode=kwd("stiffmode", 0, int))
     d = prob.deck
     P = StructuralProblem(d, **kw)
+    # NRSolver direct displacement control: ddm = (node label, dof id) pairs,
+    # ddv = values, ddltf = load-time function of the values
+    ddm = None
+    if model != "linearstatic" and (ms is not None and ms.has("ddm") or e.has("ddm")):
+        rec = ms if ms is not None and ms.has("ddm") else e
+        pairs = rec.array("ddm", int)
+        vals = rec.array("ddv")
+        if len(pairs) != 2 * len(vals):
+            P.close()
+            raise ValueError("direct displacement mask size mismatch")
+        nid = {int(n): i for i, n in enumerate(prob.node_numbers)}
+        eqs = [int(d.eqn[nid[pairs[2 * k]], pairs[2 * k + 1] - 1]) for k in range(len(vals))]
+        if min(eqs) <= 0:
+            P.close()
+            raise ValueError("ddm names a dof that is prescribed by a boundary condition")
+        P.set_ddm(eqs, vals)
+        ddm = prob.ltf[rec.scalar("ddltf", cast=int)]
     results, iters = {}, {}
     f = open(out_path, "w") if out_path else None
     try:
@@ -99,6 +118,11 @@ def run_deck(path, out_path=None, keywords=None, log=None):
             up, (ff, fp) = prob.step_loading(k)
             P.set_prescribed(up)
             P.set_load(ff, fp)
+            if ddm is not None:
+                # NRSolver::applyConstraintsToLoadIncrement: the increment of
+                # the function over the step (its value at the first step)
+                t = prob.time_of_step(k)
+                P.set_ddm_factor(ddm(t) - (ddm(prob.time_of_step(k - 1)) if k > 1 else 0.0))
             if model == "linearstatic":
                 it, _ = P.linear_static()
                 iters[k] = (0, it)

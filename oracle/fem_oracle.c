@@ -976,13 +976,33 @@ static void orc_ls_search(int istep, double *prod, double *eta, double amp, doub
     }
 }
 
+/* diagonal entry of row q of a CSR matrix (sorted columns) */
+static double orc_csr_diag(const long long *rowptr, const int *colind, const double *val, int q)
+{
+    for (long long k = rowptr[q]; k < rowptr[q + 1]; k++)
+        if (colind[k] == q) return val[k];
+    return 0.0;
+}
+
+/* NRSolver::applyConstraintsToStiffness nrsolver.C:458-500: K_ii *= 1e6 at
+ * the directly controlled equations */
+static void orc_ddm_stiffness(const long long *rowptr, const int *colind, double *val, int nddm, const int *ddm_eq)
+{
+    for (int k = 0; k < nddm; k++) {
+        const int q = ddm_eq[k] - 1;
+        for (long long j = rowptr[q]; j < rowptr[q + 1]; j++)
+            if (colind[j] == q) val[j] *= 1.e6;
+    }
+}
+
 int orc_nonlinear_step(const orc_mesh *m, double *state, double *X, const double *u_presc,
                        const double *u_presc_prev, int initial_guess, const double *RT,
                        double rtolf, int maxiter, int manrmsteps, double lstol, int *out, double *err)
 {
     int o4[4];
     const int rc = orc_nonlinear_step_ex(m, state, X, u_presc, u_presc_prev, initial_guess, RT, rtolf, maxiter,
-                                         manrmsteps, lstol, 0, 0.8, 2.5, 4.0, 1.0e20, 0, o4, err);
+                                         manrmsteps, lstol, 0, 0.8, 2.5, 4.0, 1.0e20, 0, 0, NULL, NULL, 0.0, o4,
+                                         err);
     out[0] = o4[0];
     out[1] = o4[1];
     out[2] = o4[2];
@@ -1005,7 +1025,8 @@ static int orc_ls_product(const orc_mesh *m, double *state, const double *rb, co
 int orc_nonlinear_step_ex(const orc_mesh *m, double *state, double *X, const double *u_presc,
                           const double *u_presc_prev, int initial_guess, const double *RT, double rtolf, int maxiter,
                           int manrmsteps, double lstol, int linesearch, double ls_tol, double ls_amp, double ls_maxeta,
-                          double maxinc, int stiffmode, int *out, double *err)
+                          double maxinc, int stiffmode, int nddm, const int *ddm_eq, const double *ddm_val,
+                          double ddm_factor, int *out, double *err)
 {
     const int tangent = stiffmode == 2 ? 0 : 1;
     const long long nnz = orc_csr_count(m);
@@ -1047,13 +1068,19 @@ int orc_nonlinear_step_ex(const orc_mesh *m, double *state, double *X, const dou
         }
     }
     if (orc_assemble_stiffness(m, rowptr, colind, val, state, tangent)) { rc = -10; goto out; }
+    orc_ddm_stiffness(rowptr, colind, val, nddm, ddm_eq);
     for (int nite = 0;; nite++) {
         double ebe[3];
         if (orc_internal_forces(m, X, u_presc, state, 1, F, NULL, ebe)) { rc = -11; goto out; }
         double s_r[3] = {0, 0, 0}, s_t[3] = {0, 0, 0};
+        for (int i = 0; i < m->neq; i++) rhs[i] = (RT ? RT[i] : 0.0) - F[i];
+        /* NRSolver::applyConstraintsToLoadIncrement nrsolver.C:502-568 */
+        for (int k = 0; k < nddm; k++) {
+            const int q = ddm_eq[k] - 1;
+            rhs[q] = nite == 0 ? orc_csr_diag(rowptr, colind, val, q) * ddm_val[k] * ddm_factor : 0.0;
+        }
         for (int i = 0; i < m->neq; i++) {
             const double rt = RT ? RT[i] : 0.0;
-            rhs[i] = rt - F[i];
             s_r[dofid[i]] += rhs[i] * rhs[i];
             s_t[dofid[i]] += rt * rt;
         }
@@ -1073,8 +1100,10 @@ int orc_nonlinear_step_ex(const orc_mesh *m, double *state, double *X, const dou
         if (outofrange) { out[2] = -2; break; }
         if (converged) { out[2] = 1; break; }
         if (nite >= maxiter) { out[2] = -1; break; }
-        if (nite > 0 && manrmsteps > 0 && nite % manrmsteps == 0)
+        if (nite > 0 && manrmsteps > 0 && nite % manrmsteps == 0) {
             if (orc_assemble_stiffness(m, rowptr, colind, val, state, tangent)) { rc = -10; goto out; }
+            orc_ddm_stiffness(rowptr, colind, val, nddm, ddm_eq);
+        }
         int it = 0;
         double res = 0.0;
         orc_pcg(m->neq, rowptr, colind, val, rhs, ddX, lstol, 1000000, 1, &it, &res);

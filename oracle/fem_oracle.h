@@ -116,11 +116,17 @@ int orc_nonlinear_step(const orc_mesh *m, double *state, double *X, const double
  * component of a correction (maxIncAllowed).  out[3] receives the number
  * of line searches that ended with eta != 1.  stiffmode: NonLinearStatic's
  * stiffness (nlinearstatic.h:62-64): 0 the tangent, 2 the elastic stiffness
- * for every Newton matrix (the initial guess keeps the tangent product). */
+ * for every Newton matrix (the initial guess keeps the tangent product).
+ * nddm > 0: NRSolver's direct displacement control (ddm / ddv,
+ * nrsolver.C:458-568): the diagonal of every Newton matrix at the free
+ * equations ddm_eq (1-based) is multiplied by 1e6, and the residual there is
+ * K_ii * ddm_val * ddm_factor in the first iteration, 0 afterwards
+ * (ddm_factor: the step's increment of the ddltf function). */
 int orc_nonlinear_step_ex(const orc_mesh *m, double *state, double *X, const double *u_presc,
                           const double *u_presc_prev, int initial_guess, const double *RT, double rtolf, int maxiter,
                           int manrmsteps, double lstol, int linesearch, double ls_tol, double ls_amp, double ls_maxeta,
-                          double maxinc, int stiffmode, int *out, double *err);
+                          double maxinc, int stiffmode, int nddm, const int *ddm_eq, const double *ddm_val,
+                          double ddm_factor, int *out, double *err);
 
 /* Skyline U^T D U factorization + solve of a full symmetric CSR (direct
  * solver of the reference; CPU baseline only).  *nwk_out: profile size. */

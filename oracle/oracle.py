@@ -234,16 +234,24 @@ def nonlinear_step(deck, state, X, u_presc, RT=None, rtolf=1e-8, maxiter=50, man
 
 def nonlinear_step_ex(deck, state, X, u_presc, u_presc_prev=None, initial_guess=True, RT=None, rtolf=1e-8,
                       maxiter=50, manrmsteps=1, lstol=1e-12, linesearch=False, ls_tol=0.8, ls_amp=2.5, ls_maxeta=4.0,
-                      maxinc=1e20, stiffmode=0):
+                      maxinc=1e20, stiffmode=0, ddm=None, ddm_factor=0.0):
     """nonlinear_step with the NRSolver extras: line search (LineSearchNM,
     linesearch.C:54-245), the increment cap maxinc (nrsolver.C:340-353) and
-    NonLinearStatic's stiffness mode (0 tangent, 2 elastic).
+    NonLinearStatic's stiffness mode (0 tangent, 2 elastic); ddm: (free
+    equations 1-based, values) of NRSolver's direct displacement control with
+    the step's increment factor ddm_factor.
     Returns dict(newton, pcg, status, force_error, line_searches)."""
     m = Mesh(deck)
     out = (C.c_int * 4)()
     err = C.c_double(0)
     rt = _d(RT) if RT is not None else None
     prev = _d(u_presc_prev) if u_presc_prev is not None else None
+    nd = 0
+    de = dv = None
+    if ddm is not None:
+        de = np.ascontiguousarray(ddm[0], dtype=np.int32)
+        dv = _d(ddm[1])
+        nd = len(de)
     rc = lib().orc_nonlinear_step_ex(m.ref, state.ctypes.data_as(C.c_void_p), X.ctypes.data_as(C.c_void_p),
                                      _d(u_presc).ctypes.data_as(C.c_void_p),
                                      prev.ctypes.data_as(C.c_void_p) if prev is not None else None,
@@ -251,7 +259,9 @@ def nonlinear_step_ex(deck, state, X, u_presc, u_presc_prev=None, initial_guess=
                                      rt.ctypes.data_as(C.c_void_p) if rt is not None else None, C.c_double(rtolf),
                                      maxiter, manrmsteps, C.c_double(lstol), int(bool(linesearch)), C.c_double(ls_tol),
                                      C.c_double(ls_amp), C.c_double(ls_maxeta), C.c_double(maxinc), int(stiffmode),
-                                     out, C.byref(err))
+                                     nd, de.ctypes.data_as(C.c_void_p) if nd else None,
+                                     dv.ctypes.data_as(C.c_void_p) if nd else None, C.c_double(ddm_factor), out,
+                                     C.byref(err))
     if rc:
         raise RuntimeError(f"orc_nonlinear_step_ex rc={rc}")
     return dict(newton=out[0], pcg=out[1], status=out[2], force_error=err.value, line_searches=out[3])

@@ -286,3 +286,37 @@ def test_run_nonlinear_static_deck(tmp_path):
         src[i + 1] = "nsteps 1 controlmode 0"
         p.write_text("\n".join(src) + "\n")
         run_deck(str(p), keywords=dict(smtype=11, lstype=9))
+
+
+def test_direct_displacement_control_matches_oracle():
+    """NRSolver ddm/ddv on the GPU driver: the uniaxial block past yield with
+    its top face's u_z free and driven by the 1e6 diagonal penalty reaches
+    the oracle's Newton iteration count, displacements (1e-8) and plastic
+    strains."""
+    from oofem_amd.problem import StructuralProblem
+    from tests import nlcases
+    d, top = nlcases.uniaxial_block(meshgen.LSPACE, n=3)
+    bc2 = d.bc_mask.copy()
+    bc2[top, 2] = False
+    d2 = meshgen.Deck("uniaxial_ddm", d.etype, d.coords, d.conn, d.emat, d.mat, bc2, np.zeros_like(d.coords))
+    eqs = d2.eqn[top, 2]
+    eps = 2.0 * nlcases.SIG0 / nlcases.E
+    vals = np.full(len(eqs), eps)
+    st = oracle.state_new(d2)
+    X = np.zeros(d2.neq)
+    ref = oracle.nonlinear_step_ex(d2, st, X, np.zeros(d2.npeq), initial_guess=False, maxiter=40, lstol=1e-13,
+                                   ddm=(eqs, vals), ddm_factor=1.0)
+    assert ref["status"] == 1
+    P = StructuralProblem(d2, lstol=1e-13, rtolf=1e-8, maxiter=40, manrmsteps=1, initial_guess=0)
+    P.set_load(np.zeros(d2.neq), np.zeros(d2.npeq))
+    P.set_prescribed(np.zeros(d2.npeq))
+    P.set_ddm(eqs, vals)
+    P.set_ddm_factor(1.0)
+    rep = P.nonlinear_step()
+    assert rep["status"] == 1 and rep["newton"] == ref["newton"], (rep, ref)
+    assert np.linalg.norm(P.solution() - X) <= 1e-8 * np.linalg.norm(X)
+    assert np.abs(P.solution()[eqs - 1] - eps).max() <= 1e-5 * eps
+    kap = P.state()[:, 13]
+    kref = st.reshape(-1, 38)[:, 13]
+    assert np.abs(kap - kref).max() <= 1e-8 * kref.max()
+    P.close()

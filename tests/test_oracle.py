@@ -572,3 +572,40 @@ def test_nonlinear_static_deck_keywords(tmp_path):
     prob = rd(str(p))
     assert prob.model == "nonlinearstatic" and prob.nsteps == 2 and prob.time_of_step(2) == 1.0
     assert prob.metasteps[0].scalar("stiffmode", 0, int) == 2
+
+
+def test_direct_displacement_control_matches_dirichlet():
+    """NRSolver ddm/ddv on the uniaxial block past yield: with the top face's
+    u_z free and driven by the 1e6 diagonal penalty, the Newton loop reaches
+    the displacements of the same block with u_z prescribed as a boundary
+    condition, to the penalty's accuracy (1e-5)."""
+    from tests import nlcases
+    d, top = nlcases.uniaxial_block(meshgen.LSPACE, n=3)
+    eps = 2.0 * nlcases.SIG0 / nlcases.E
+    up = d.prescribed_values(eps)
+    st = oracle.state_new(d)
+    X = np.zeros(d.neq)
+    r = oracle.nonlinear_step_ex(d, st, X, up, initial_guess=False, maxiter=40, lstol=1e-13)
+    assert r["status"] == 1
+    bc2 = d.bc_mask.copy()
+    bc2[top, 2] = False
+    d2 = meshgen.Deck("uniaxial_ddm", d.etype, d.coords, d.conn, d.emat, d.mat, bc2, np.zeros_like(d.coords))
+    eqs = d2.eqn[top, 2]
+    assert (eqs > 0).all()
+    st2 = oracle.state_new(d2)
+    X2 = np.zeros(d2.neq)
+    r2 = oracle.nonlinear_step_ex(d2, st2, X2, np.zeros(d2.npeq), initial_guess=False, maxiter=40, lstol=1e-13,
+                                  ddm=(eqs, np.full(len(eqs), eps)), ddm_factor=1.0)
+    assert r2["status"] == 1, r2
+    u1 = np.zeros_like(d.coords)
+    m = d.eqn > 0
+    u1[m] = X[d.eqn[m] - 1]
+    u1[top, 2] = eps
+    u2 = np.zeros_like(d.coords)
+    m2 = d2.eqn > 0
+    u2[m2] = X2[d2.eqn[m2] - 1]
+    assert np.abs(u2 - u1).max() <= 1e-5 * eps, np.abs(u2 - u1).max() / eps
+    # the plastic state matches too
+    k1 = st.reshape(-1, 38)[:, 13]
+    k2 = st2.reshape(-1, 38)[:, 13]
+    assert k1.min() > 0 and np.abs(k2 - k1).max() <= 1e-4 * k1.max()

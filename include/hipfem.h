@@ -81,6 +81,10 @@ int hipfem_dev_axpby(hipfem_ctx *ctx, int64_t n, double a, const double *x, doub
 int hipfem_dev_copy(hipfem_ctx *ctx, int64_t n, const double *x, double *y, int method);
 /* dot product x.y and 2-norm, FP64, fixed summation order */
 int hipfem_dev_dot(hipfem_ctx *ctx, int64_t n, const double *x, const double *y, double *out);
+/* y[i incy] = a x[i incx] + b y[i incy], i < n (x may be NULL: y = b y);
+ * gathers and scatters one column of interleaved multi-column vectors */
+int hipfem_dev_axpby_strided(hipfem_ctx *ctx, int64_t n, double a, const double *x, int64_t incx, double b,
+                             double *y, int64_t incy);
 /* max |x_i| (NRSolver's increment cap, nrsolver.C:340-353) */
 int hipfem_dev_amax(hipfem_ctx *ctx, int64_t n, const double *x, double *out);
 

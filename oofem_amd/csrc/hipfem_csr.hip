@@ -503,6 +503,16 @@ extern "C" void *hipfem_ctx_stream(hipfem_ctx *c) { return c ? (void *)c->stream
 
 // ------------------------------------------------------------ device vectors
 namespace hipfem {
+// y[i * incy] = a x[i * incx] + b y[i * incy] (x may be NULL: a x = 0)
+__global__ void k_axpby_strided(int64_t n, double a, const double *__restrict__ x, int64_t incx, double b,
+                                double *__restrict__ y, int64_t incy)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const double xv = x ? a * x[i * incx] : 0.0;
+        y[i * incy] = b == 0.0 ? xv : xv + b * y[i * incy];
+    }
+}
+
 __global__ void k_axpby(int64_t n, double a, const double *__restrict__ x, double b, double *__restrict__ y)
 {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
@@ -661,6 +671,18 @@ extern "C" int hipfem_dev_dot(hipfem_ctx *c, int64_t n, const double *x, const d
     }
     return 0;
 }
+extern "C" int hipfem_dev_axpby_strided(hipfem_ctx *c, int64_t n, double a, const double *x, int64_t incx, double b,
+                                        double *y, int64_t incy)
+{
+    HIPFEM_REQUIRE(c && n >= 0 && incx > 0 && incy > 0 && (n == 0 || y), HIPFEM_EINVAL, "dev_axpby_strided: bad arguments");
+    DeviceGuard g(c->device);
+    if (n) {
+        k_axpby_strided<<<grid_for(n), 256, 0, c->stream>>>(n, a, x, incx, b, y, incy);
+        HIPFEM_CHECK(hipGetLastError());
+    }
+    return 0;
+}
+
 extern "C" int hipfem_dev_amax(hipfem_ctx *c, int64_t n, const double *x, double *out)
 {
     HIPFEM_REQUIRE(c && out && (n == 0 || x), HIPFEM_EINVAL, "dev_amax: bad arguments");

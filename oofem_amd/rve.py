@@ -28,6 +28,12 @@ from .hipfem import BatchedEvaluator, DeviceVector, HipCSRSparseMtrx, _check, _p
 NNZ_ROW = {meshgen.LSPACE: 81, meshgen.QSPACE: 200}
 
 
+def _strided(ctx, n, a, x, incx, b, y, incy):
+    """y[i incy] = a x[i incx] + b y[i incy] on device addresses."""
+    _check(lib().hipfem_dev_axpby_strided(ctx.h, C.c_int64(n), C.c_double(a), C.c_void_p(x), C.c_int64(incx),
+                                          C.c_double(b), C.c_void_p(y), C.c_int64(incy)), "dev_axpby_strided")
+
+
 def _concat(decks):
     """One deck holding the RVEs side by side (node and element offsets; the
     shared two-phase material table)."""
@@ -72,6 +78,11 @@ class _Chunk:
         self.b = DeviceVector(ctx, max(1, 6 * self.n))
         self.x = DeviceVector(ctx, max(1, 6 * self.n))
         self.up = [meshgen.affine_values(d, e) for e in np.eye(6)]
+        # device-resident work vectors of the right-hand sides and reactions
+        self.d_up = [DeviceVector(ctx, max(1, d.npeq), u) for u in self.up]
+        self.d_u = DeviceVector(ctx, max(1, self.n))
+        self.d_f = DeviceVector(ctx, max(1, self.n))
+        self.d_fp = DeviceVector(ctx, max(1, d.npeq))
         # prescribed dofs: reaction index, RVE, component and the node's
         # position relative to the cell centre
         node, comp = np.nonzero(d.eqn < 0)
@@ -80,12 +91,12 @@ class _Chunk:
         self.p_xc = d.coords[node] - 0.5
 
     def rhs(self):
-        """b_k = -f_int(free 0, prescribed affine_k) for the six unit strains, interleaved."""
-        B = np.zeros((self.n, 6))
+        """b_k = -f_int(free 0, prescribed affine_k) for the six unit strains,
+        interleaved, on the device."""
+        self.d_u.fill(0.0)
         for k in range(6):
-            f, _, _ = self.ev.internal_forces(np.zeros(self.n), self.up[k])
-            B[:, k] = -f
-        self.b.upload(B.ravel())
+            self.ev.internal_forces_dev(self.d_u.addr, self.d_up[k].addr, self.d_f.addr, self.d_fp.addr)
+            _strided(self.ctx, self.n, -1.0, self.d_f.addr, 1, 0.0, self.b.addr + 8 * k, 6)
 
     def solve(self):
         nb = len(self.ids)
@@ -100,11 +111,12 @@ class _Chunk:
         """6x6 stiffness per RVE from the reactions of the six solves: per RVE
         the sum over its prescribed dofs of R (x) (x - x_c), symmetrised (the
         unit cell has volume 1)."""
-        X = self.x.download().reshape(self.n, 6)
         nrve = len(self.ids)
         out = np.zeros((nrve, 6, 6))
         for k in range(6):
-            _, fp, _ = self.ev.internal_forces(np.ascontiguousarray(X[:, k]), self.up[k])
+            _strided(self.ctx, self.n, 1.0, self.x.addr + 8 * k, 6, 0.0, self.d_u.addr, 1)
+            self.ev.internal_forces_dev(self.d_u.addr, self.d_up[k].addr, self.d_f.addr, self.d_fp.addr)
+            fp = self.d_fp.download()
             r = fp[self.p_idx]
             S = np.stack([np.bincount(self.p_bin, weights=r * self.p_xc[:, j], minlength=3 * nrve)
                           for j in range(3)], axis=1).reshape(nrve, 3, 3)
@@ -118,7 +130,7 @@ class _Chunk:
             lib().hfem_batch_destroy.restype = None
             lib().hfem_batch_destroy(self.h)
             self.h = C.c_void_p()
-        for o in (self.b, self.x, self.K, self.ev):
+        for o in (self.b, self.x, self.d_u, self.d_f, self.d_fp, *self.d_up, self.K, self.ev):
             o.close()
 
 

@@ -406,7 +406,7 @@ def main_rve(args):
     ctx = Context(0)
     B = RVEBatch(ctx, specs, log=log)
     log(f"[bench] {len(specs)} RVEs in {len(B.chunks)} chunks, setup {B.t_setup:.1f} s")
-    B.set_options(tol=args.rve_tol, split=args.rve_split)
+    B.set_options(tol=args.rve_tol, split=args.rve_split, bulk_form=args.rve_bulk_form)
     t = time.perf_counter()
     tuned = B.autotune(sample=args.rve_tune_sample, log=log)
     t_tune = time.perf_counter() - t
@@ -459,7 +459,8 @@ def main_rve(args):
     if prof["launches"]:
         avg_ms = prof["ms"] / prof["launches"]
         ach = prof["bytes"] / (prof["ms"] * 1e-3) / 1e9
-        kname = "k_bulk_spmv_nb<%d>" if all(forms) else "k_bulk_spmv<%d>"
+        kname = ("k_bulk_spmv_nbw<%d>" if all(f == 2 for f in forms) else
+                 "k_bulk_spmv_nb<%d>" if all(forms) else "k_bulk_spmv<%d>")
         lpr = tuned[0]["bulk_lanes_per_row"] if tuned else 0
         res["roofline"] = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                                frac=round(ach / HBM_PEAK_GBS, 4), traffic=None, kernel=kname % lpr,
@@ -525,6 +526,8 @@ def main():
     ap.add_argument("--rve-tune-sample", type=int, default=16)
     ap.add_argument("--rve-cpu-seconds", type=float, default=15.0,
                     help="CPU time budget of the oracle sample for the RVE cpu_baseline")
+    ap.add_argument("--rve-bulk-form", type=int, default=1, choices=[0, 1, 2],
+                    help="bulk products: 0 CSR rows, 1 node-block, 2 node-block from a padded block-major copy")
     ap.add_argument("--rve-split", type=int, default=-1,
                     help="stored entries above which an RVE goes to the bulk phase (-1 automatic, 0 all)")
     ap.add_argument("--rve-check", type=int, default=8)

@@ -309,10 +309,12 @@ int  hfem_batch_autotune(hfem_batch *bt, const double *db, double *dx, int sampl
 int  hfem_batch_nbulk_shapes(void);
 int  hfem_batch_bulk_shape_info(int shape, int *lanes_per_row);
 int  hfem_batch_set_bulk_shape(hfem_batch *bt, int shape);
-/* bulk products in the 3x3 node-block form (1, the default) when the
- * pattern is node-blocked and every block starts on a node, else CSR rows;
- * hfem_batch_bulk_form reports the form in use (1 node-block, 0 CSR) */
-int  hfem_batch_set_bulk_form(hfem_batch *bt, int nodeblock);
+/* bulk products: 1 (the default) the 3x3 node-block form over the CSR
+ * values, 2 the same over a block-major copy of the FP32 values padded to
+ * 12 per block (three 16-byte loads per block), 0 CSR rows; the node-block
+ * forms apply when the pattern is node-blocked and every block starts on a
+ * node.  hfem_batch_bulk_form reports the form in use (0, 1 or 2) */
+int  hfem_batch_set_bulk_form(hfem_batch *bt, int form);
 int  hfem_batch_bulk_form(const hfem_batch *bt);
 /* split: blocks with more stored entries go to the bulk phase; -1 (the
  * default) = the batch's entries divided by the workgroups the persistent

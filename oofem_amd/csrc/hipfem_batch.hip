@@ -301,6 +301,7 @@ struct BulkArgs {
     BulkBlock *st;
     const int32_t *rowptr, *col;
     const int32_t *bptr, *bcol;  // node-block view (node rows, block columns) when the pattern has one
+    const float4 *vwide;         // wide form: FP32 values block-major, 12 floats per block
     const double *val64;
     const float *val32, *dinv;
     const double *b;
@@ -524,6 +525,49 @@ __device__ __forceinline__ void nb_row6(int bo, int nbk, int lane, const int32_t
         }
 }
 
+// node-block product with the FP32 values of every 3x3 block stored
+// together, padded to 12 floats (three 16-byte loads per block instead of
+// nine 4-byte ones; vw from k_bulk_widen)
+template <int LPR>
+__device__ __forceinline__ void nb_row6_wide(int bo, int nbk, int lane, const int32_t *__restrict__ bcol,
+                                             const float4 *__restrict__ vw, const float *__restrict__ x,
+                                             float acc[3][NRHS])
+{
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int k = 0; k < NRHS; k++) acc[r][k] = 0.f;
+    for (int k = lane; k < nbk; k += LPR) {
+        const float *xb = x + (size_t)bcol[bo + k] * (3 * NRHS);
+        const float4 *v = vw + 3 * ((size_t)bo + k);
+        const float4 v0 = v[0], v1 = v[1], v2 = v[2];
+        const float a[3][3] = {{v0.x, v0.y, v0.z}, {v0.w, v1.x, v1.y}, {v1.z, v1.w, v2.x}};
+        float xv[3][NRHS];
+#pragma unroll
+        for (int c = 0; c < 3; c++)
+#pragma unroll
+            for (int j = 0; j < NRHS; j += 2) {
+                const float2 t = *reinterpret_cast<const float2 *>(xb + c * NRHS + j);
+                xv[c][j] = t.x;
+                xv[c][j + 1] = t.y;
+            }
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+#pragma unroll
+            for (int j = 0; j < NRHS; j++)
+                acc[r][j] = fmaf(a[r][2], xv[2][j], fmaf(a[r][1], xv[1][j], fmaf(a[r][0], xv[0][j], acc[r][j])));
+    }
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+#pragma unroll
+        for (int j = 0; j < NRHS; j++) {
+            float w = acc[r][j];
+#pragma unroll
+            for (int off = LPR / 2; off > 0; off >>= 1) w += __shfl_xor(w, off, 64);
+            acc[r][j] = w;
+        }
+}
+
 // value of output j (0..17) of a reduced node row, selected without
 // dynamic register indexing
 template <typename VT>
@@ -563,6 +607,50 @@ __global__ __launch_bounds__(BULK_TILE) void k_bulk_spmv_nb(BulkArgs A)
         }
     }
     block_sums<BULK_TILE, NRHS>(v, sh, A.part + (size_t)blockIdx.x * NPART);
+}
+
+// q = A p (FP32) in the wide node-block form; partials p.q
+template <int LPR>
+__global__ __launch_bounds__(BULK_TILE) void k_bulk_spmv_nbw(BulkArgs A)
+{
+    __shared__ double sh[NRHS * (BULK_TILE / 64)];
+    const int4 t = A.tiles[blockIdx.x];
+    if (A.st[t.z].done) return;
+    double v[NRHS];
+#pragma unroll
+    for (int k = 0; k < NRHS; k++) v[k] = 0.0;
+    const int lane = threadIdx.x % LPR;
+    for (int I = t.x / 3 + threadIdx.x / LPR; I < t.y / 3; I += BULK_TILE / LPR) {
+        const int bo = A.bptr[I], nbk = A.bptr[I + 1] - bo;
+        float acc[3][NRHS];
+        nb_row6_wide<LPR>(bo, nbk, lane, A.bcol, A.vwide, A.p, acc);
+        for (int j = lane; j < 3 * NRHS; j += LPR) {
+            const size_t o = (size_t)I * (3 * NRHS) + j;
+            const float qj = pick18(acc, j);
+            A.q[o] = qj;
+            const double pq = (double)A.p[o] * (double)qj;
+#pragma unroll
+            for (int k = 0; k < NRHS; k++) v[k] += (j % NRHS == k) ? pq : 0.0;
+        }
+    }
+    block_sums<BULK_TILE, NRHS>(v, sh, A.part + (size_t)blockIdx.x * NPART);
+}
+
+// block-major padded copy of the FP32 values of a node-blocked CSR: block
+// bo + k of node row I gets row r, column c at vw[12 (bo + k) + 3 r + c]
+__global__ void k_bulk_widen(int nbr, const int32_t *__restrict__ bptr, const float *__restrict__ val32,
+                             float *__restrict__ vw)
+{
+    for (int I = blockIdx.x * blockDim.x + threadIdx.x; I < nbr; I += gridDim.x * blockDim.x) {
+        const int bo = bptr[I], nbk = bptr[I + 1] - bo;
+        for (int k = 0; k < nbk; k++)
+            for (int r = 0; r < 3; r++) {
+                for (int c = 0; c < 3; c++)
+                    vw[12 * ((size_t)bo + k) + 3 * r + c] = val32[9 * (size_t)bo + 3 * (size_t)r * nbk + 3 * k + c];
+                if (r == 2)
+                    for (int c = 9; c < 12; c++) vw[12 * ((size_t)bo + k) + c] = 0.f;
+            }
+    }
 }
 
 // reliable update in the node-block form: r = b - A x (FP64), z = D^-1 r
@@ -702,12 +790,12 @@ __global__ __launch_bounds__(256) void k_bulk_reduce(BulkArgs A)
 typedef void (*bulk_kernel_t)(BulkArgs);
 struct BulkShape {
     int lpr;
-    bulk_kernel_t spmv, refresh, spmv_nb, refresh_nb;
+    bulk_kernel_t spmv, refresh, spmv_nb, refresh_nb, spmv_nbw;
 };
 static const BulkShape kBulkShapes[] = {
-    {4, k_bulk_spmv<4>, k_bulk_refresh<4>, k_bulk_spmv_nb<4>, k_bulk_refresh_nb<4>},
-    {8, k_bulk_spmv<8>, k_bulk_refresh<8>, k_bulk_spmv_nb<8>, k_bulk_refresh_nb<8>},
-    {16, k_bulk_spmv<16>, k_bulk_refresh<16>, k_bulk_spmv_nb<16>, k_bulk_refresh_nb<16>},
+    {4, k_bulk_spmv<4>, k_bulk_refresh<4>, k_bulk_spmv_nb<4>, k_bulk_refresh_nb<4>, k_bulk_spmv_nbw<4>},
+    {8, k_bulk_spmv<8>, k_bulk_refresh<8>, k_bulk_spmv_nb<8>, k_bulk_refresh_nb<8>, k_bulk_spmv_nbw<8>},
+    {16, k_bulk_spmv<16>, k_bulk_refresh<16>, k_bulk_spmv_nb<16>, k_bulk_refresh_nb<16>, k_bulk_spmv_nbw<16>},
 };
 constexpr int NBULK = sizeof(kBulkShapes) / sizeof(kBulkShapes[0]);
 
@@ -771,6 +859,9 @@ struct hfem_batch {
     int bulk_shape = 1;         // index into kBulkShapes
     bool nodeblock_ok = false;  // node-block kernels apply to this matrix
     bool nodeblock = true;      // use them when they apply
+    bool wide = false;          // node-block products from the block-major padded FP32 copy
+    float *d_vwide = nullptr;   // that copy (12 floats per block), built on first use
+    long vwide_version = -1;
     int tiles_cap = 0;
     BulkBlock *d_bst = nullptr;
     int4 *d_tiles = nullptr;
@@ -792,7 +883,7 @@ static int batch_free(hfem_batch *bt)
 {
     void *ps[] = {bt->d_rows, bt->d_order, bt->d_dinv, bt->d_x, bt->d_r64, bt->d_r, bt->d_z, bt->d_p, bt->d_q,
                   bt->d_ctl, bt->d_iters, bt->d_upd, bt->d_zero, bt->d_res, bt->d_bst, bt->d_tiles, bt->d_act,
-                  bt->d_act_t0, bt->d_part};
+                  bt->d_act_t0, bt->d_part, bt->d_vwide};
     for (void *p : ps)
         if (p) HIPFEM_CHECK(hipFree(p));
     for (hipEvent_t e : bt->ev) HIPFEM_CHECK(hipEventDestroy(e));
@@ -919,16 +1010,18 @@ extern "C" int hfem_batch_set_bulk_shape(hfem_batch *bt, int shape)
     return 0;
 }
 
-extern "C" int hfem_batch_set_bulk_form(hfem_batch *bt, int nodeblock)
+extern "C" int hfem_batch_set_bulk_form(hfem_batch *bt, int form)
 {
-    HIPFEM_REQUIRE(bt, HIPFEM_EINVAL, "bulk form: null handle");
-    bt->nodeblock = nodeblock != 0;
+    HIPFEM_REQUIRE(bt && form >= 0 && form <= 2, HIPFEM_EINVAL, "bulk form must be 0, 1 or 2");
+    bt->nodeblock = form != 0;
+    bt->wide = form == 2;
     return 0;
 }
 
 extern "C" int hfem_batch_bulk_form(const hfem_batch *bt)
 {
-    return bt && bt->nodeblock && bt->nodeblock_ok && bt->A->nodeblocked ? 1 : 0;
+    if (!(bt && bt->nodeblock && bt->nodeblock_ok && bt->A->nodeblocked)) return 0;
+    return bt->wide ? 2 : 1;
 }
 
 // HIP-event timing of the bulk products: on resets the counters; the
@@ -1047,8 +1140,8 @@ static int bulk_tiles(hfem_batch *bt, const std::vector<int> &act)
     for (int a : act) {
         const int blk = bt->h_bulk[a];
         const double nnz = (double)bt->h_nnz[blk], rows = (double)(bt->h_rows[blk + 1] - bt->h_rows[blk]);
-        bt->tile_bytes += nb ? 4.0 * nnz + 4.0 * nnz / 9.0 + 4.0 * rows / 3.0 + 48.0 * rows
-                             : 8.0 * nnz + 4.0 * rows + 48.0 * rows;
+        bt->tile_bytes += !nb ? 8.0 * nnz + 4.0 * rows + 48.0 * rows
+                          : (bt->wide ? 16.0 / 3.0 : 4.0) * nnz + 4.0 * nnz / 9.0 + 4.0 * rows / 3.0 + 48.0 * rows;
     }
     for (int a : act) {
         const int blk = bt->h_bulk[a];
@@ -1108,7 +1201,17 @@ static int bulk_solve(hfem_batch *bt, const double *db, double *dx, int maxit)
     A.resid = bt->d_res;
     const BulkShape &sh = kBulkShapes[bt->bulk_shape];
     const bool nb = bt->nodeblock && bt->nodeblock_ok && M->nodeblocked;
-    const bulk_kernel_t spmv = nb ? sh.spmv_nb : sh.spmv, refresh = nb ? sh.refresh_nb : sh.refresh;
+    const bool wide = nb && bt->wide;
+    if (wide && (!bt->d_vwide || bt->vwide_version != M->val32_version)) {
+        if (!bt->d_vwide) HIPFEM_CHECK(hipMalloc(&bt->d_vwide, sizeof(float) * 12 * (size_t)std::max<int64_t>(1, M->nnz / 9)));
+        k_bulk_widen<<<std::max(1, std::min((M->nbr + 255) / 256, 8192)), 256, 0, st>>>(M->nbr, M->d_bptr, M->d_val32,
+                                                                                        bt->d_vwide);
+        HIPFEM_CHECK(hipGetLastError());
+        bt->vwide_version = M->val32_version;
+    }
+    A.vwide = reinterpret_cast<const float4 *>(bt->d_vwide);
+    const bulk_kernel_t spmv = wide ? sh.spmv_nbw : nb ? sh.spmv_nb : sh.spmv;
+    const bulk_kernel_t refresh = nb ? sh.refresh_nb : sh.refresh;
     int ntiles = (int)bt->h_tiles.size(), nact = nbulk;
     k_bulk_init<<<ntiles, BULK_TILE, 0, st>>>(A);
     k_bulk_reduce<BP_INIT><<<nact, 256, 0, st>>>(A);

@@ -165,14 +165,16 @@ class RVEBatch:
                     cur_nnz += est
         self.t_setup = time.perf_counter() - t
 
-    def set_options(self, tol=1e-8, delta=0.1, maxit=20000, split=-1, nodeblock=True):
+    def set_options(self, tol=1e-8, delta=0.1, maxit=20000, split=-1, nodeblock=True, bulk_form=None):
         """split: blocks with more stored entries than this are solved by the
-        bulk-synchronous phase (-1: automatic, 0: every block); nodeblock:
-        3x3 node-block products in that phase where the pattern allows."""
+        bulk-synchronous phase (-1: automatic, 0: every block); bulk_form:
+        0 CSR rows, 1 3x3 node-block products from the CSR values, 2 the
+        same from a block-major padded FP32 copy (nodeblock=False is 0)."""
+        form = bulk_form if bulk_form is not None else int(bool(nodeblock))
         for c in self.chunks:
             _check(lib().hfem_batch_set_options(c.h, C.c_double(tol), C.c_double(delta), C.c_int(maxit)), "options")
             _check(lib().hfem_batch_set_split(c.h, C.c_longlong(split)), "split")
-            _check(lib().hfem_batch_set_bulk_form(c.h, int(nodeblock)), "bulk form")
+            _check(lib().hfem_batch_set_bulk_form(c.h, int(form)), "bulk form")
 
     def set_profiling(self, on=True):
         for c in self.chunks:

@@ -27,7 +27,7 @@ def _oracle_C(spec):
 # stored entries per block range from about 400 (n = 2, 3) to 60,000 (QSpace
 # n = 4): 0 sends every block to the bulk phase, 5000 splits the batch,
 # 10**12 keeps every block in the persistent kernel
-SPLITS = {"bulk": 0, "bulk_csr": 0, "mixed": 5000, "persistent": 10**12}
+SPLITS = {"bulk": 0, "bulk_csr": 0, "bulk_wide": 0, "mixed": 5000, "persistent": 10**12}
 
 
 @pytest.mark.parametrize("mode", sorted(SPLITS))
@@ -35,12 +35,13 @@ def test_batch_homogenized_stiffness_matches_oracle(ctx, mode):
     from oofem_amd.rve import RVEBatch
     B = RVEBatch(ctx, SPECS, nnz_budget=2e5)        # small budget: several chunks per element type
     assert len(B.chunks) >= 3
-    nodeblock = mode != "bulk_csr"
-    B.set_options(tol=1e-10, split=SPLITS[mode], nodeblock=nodeblock)
+    form = {"bulk_csr": 0, "bulk_wide": 2}.get(mode, 1)
+    B.set_options(tol=1e-10, split=SPLITS[mode], bulk_form=form)
     # every RVE pattern is node-blocked (boundary nodes are prescribed whole)
-    assert B.bulk_forms() == [int(nodeblock)] * len(B.chunks)
+    assert B.bulk_forms() == [form] * len(B.chunks)
     nbulk = sum(nb for nb, _ in B.partition())
     expect = {"bulk": lambda v: v == len(SPECS), "bulk_csr": lambda v: v == len(SPECS),
+              "bulk_wide": lambda v: v == len(SPECS),
               "mixed": lambda v: 0 < v < len(SPECS),
               "persistent": lambda v: v == 0}[mode]
     assert expect(nbulk), (mode, nbulk, B.partition())

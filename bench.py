@@ -48,6 +48,8 @@ def make_deck(args):
         return meshgen.config0()
     if args.workload == "cfg3":
         return meshgen.config3(nx=args.nx, ny=args.ny, nz=args.nz)
+    if args.workload == "cfg2":
+        return meshgen.config2()
     return meshgen.config1(nx=args.nx, ny=args.ny, nz=args.nz)
 
 
@@ -146,7 +148,8 @@ def run_gpu(args, deck, precision):
     return out
 
 
-def run_newton(args, deck, precision, warm=1, steps=None, total=None, surrogate=None, keep_state=False, forcing=0.0):
+def run_newton(args, deck, precision, warm=1, steps=None, total=None, surrogate=None, keep_state=False, forcing=0.0,
+               initial_guess=1):
     """StaticStructural + NRSolver load steps (full Newton, consistent tangent,
     rtolf 1e-8, tangent initial guess) with the prescribed end displacement
     ramped over `total` steps: `warm` untimed steps, then `steps` timed ones.
@@ -158,7 +161,8 @@ def run_newton(args, deck, precision, warm=1, steps=None, total=None, surrogate=
     steps = args.nl_steps if steps is None else steps
     total = args.nl_total if total is None else total
     P = StructuralProblem(deck, lstol=args.nl_lstol, lsiter=args.maxit, lsprecond=1, mixed=precision,
-                          irtol=args.inner_tol, rtolf=1e-8, maxiter=60, manrmsteps=1, forcing=forcing)
+                          irtol=args.inner_tol, rtolf=1e-8, maxiter=60, manrmsteps=1, forcing=forcing,
+                          initial_guess=initial_guess)
     P.set_load(np.zeros(deck.neq), np.zeros(deck.npeq))
     net = None
     if surrogate is not None:
@@ -185,7 +189,8 @@ def run_newton(args, deck, precision, warm=1, steps=None, total=None, surrogate=
             f"newton step {k}: {rep}")
     dt = time.perf_counter() - t0
     p = P.profile()
-    out = dict(dt=dt, steps=steps, newton=[r["newton"] for r in reports[warm:]],
+    out = dict(dt=dt, steps=steps, newton=[r["newton"] for r in reports[warm:]], force_error=[r["force_error"] for r in reports],
+               reactions=P.reactions(),
                pcg=[r["pcg"] for r in reports[warm:]], n_spmv=p["n64"], ms_spmv=p["ms64"], n_spmv32=p["n32"],
                ms_spmv32=p["ms32"], nnz=P.matrix_nnz(), u=P.solution(), info=P.solver_info(),
                layout=P.matrix_layout())
@@ -311,6 +316,32 @@ def cpu_baseline_newton(args, deck, rn):
                         f"{pcg:.0f} PCG iterations per step (estimated step {t_step:.1f} s)"))
 
 
+def spmv_roofline(args, deck, r):
+    """Roofline of the FP64 PCG SpMV of a Newton run: algorithmic bytes of the
+    matrix's layout over the mean HIP-event launch time."""
+    n = deck.neq
+    roof = None
+    if r["n_spmv"]:
+        avg_ms = r["ms_spmv"] / r["n_spmv"]
+        lay = r["layout"]
+        if lay["form"] == 2:
+            # row-group SpMV: values, one int32 column and one int32 offset per
+            # chunk, group bounds (2 int32), row pointers, gather p, write q
+            bytes_per = 8.0 * r["nnz"] + 8.0 * lay["blocks"] + 8.0 * lay["groups"] + 4.0 * n + 16.0 * n
+            kname = "k_pcg_spmv_p_grp<double, double>"
+        elif lay["form"] == 1:
+            bytes_per = 8.0 * r["nnz"] + 4.0 * r["nnz"] / 9.0 + 4.0 * n / 3.0 + 16.0 * n
+            kname = "k_pcg_spmv_p_nb<double, double>"
+        else:
+            bytes_per = 12.0 * r["nnz"] + 20.0 * n
+            kname = "k_pcg_spmv_p<double, double> (lane form)"
+        ach = bytes_per / (avg_ms * 1e-3) / 1e9
+        roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 4),
+                    traffic=measured_traffic(kname, deck.name, args), kernel=kname, avg_launch_ms=round(avg_ms, 4),
+                    launches=r["n_spmv"], algorithmic_bytes=int(bytes_per))
+    return roof
+
+
 def main_newton(args):
     """Newton-step throughput on the LSpace plasticity block (config 3):
     plain FP64 (hipcsr + hippcg FP64 + exact MisesMat) against the mixed
@@ -357,25 +388,7 @@ def main_newton(args):
         variants[name] = v
         log(f"[bench] {name}: {v}")
     r = base
-    roof = None
-    if r["n_spmv"]:
-        avg_ms = r["ms_spmv"] / r["n_spmv"]
-        lay = r["layout"]
-        if lay["form"] == 2:
-            # row-group SpMV: values, one int32 column and one int32 offset per
-            # chunk, group bounds (2 int32), row pointers, gather p, write q
-            bytes_per = 8.0 * r["nnz"] + 8.0 * lay["blocks"] + 8.0 * lay["groups"] + 4.0 * n + 16.0 * n
-            kname = "k_pcg_spmv_p_grp<double, double>"
-        elif lay["form"] == 1:
-            bytes_per = 8.0 * r["nnz"] + 4.0 * r["nnz"] / 9.0 + 4.0 * n / 3.0 + 16.0 * n
-            kname = "k_pcg_spmv_p_nb<double, double>"
-        else:
-            bytes_per = 12.0 * r["nnz"] + 20.0 * n
-            kname = "k_pcg_spmv_p<double, double> (lane form)"
-        ach = bytes_per / (avg_ms * 1e-3) / 1e9
-        roof = dict(bound="hbm", achieved=round(ach, 1), peak=HBM_PEAK_GBS, unit="GB/s", frac=round(ach / HBM_PEAK_GBS, 4),
-                    traffic=measured_traffic(kname, deck.name, args), kernel=kname, avg_launch_ms=round(avg_ms, 4),
-                    launches=r["n_spmv"], algorithmic_bytes=int(bytes_per))
+    roof = spmv_roofline(args, deck, r)
     cpu = None if args.no_cpu_baseline else cpu_baseline_newton(args, deck, r)
     out = {
         "metric": METRIC, "value": n * r["steps"] / r["dt"], "unit": "DOFs/s", "n_gpus": 1, "steps": r["steps"],
@@ -388,6 +401,52 @@ def main_newton(args):
                             f"{args.nl_lstol:g}"),
                    "precision": "fp64", "variants": variants},
         "roofline": roof, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+def main_tet(args):
+    """NonLinearStatic J2 plasticity on the 2.4M-element Kuhn-tet block
+    (config 2): direct control with the tangent predictor (initialguess 1:
+    without it the first step's boundary jump does not converge in 60
+    Newton iterations), consistent tangent, rtolf 1e-8, FP64; --c2-warm
+    untimed and --c2-steps timed load steps of the 20.  Size-independent
+    checks: every step's force error, and the balance of the reactions (no
+    external load: they sum to zero)."""
+    import numpy as np
+
+    from oofem_amd import meshgen
+    t = time.perf_counter()
+    deck = meshgen.config2()
+    log(f"[bench] deck {deck.name}: nelem={deck.nelem} neq={deck.neq} ({time.perf_counter() - t:.1f} s)")
+    r = run_newton(args, deck, 0, warm=args.c2_warm, steps=args.c2_steps, total=20, initial_guess=1)
+    n = deck.neq
+    R = r["reactions"]
+    m = deck.eqn < 0
+    node, comp = np.nonzero(m)
+    Rz = np.zeros(3)
+    top = np.zeros(3)
+    for c in range(3):
+        sel = comp == c
+        Rz[c] = R[-deck.eqn[node[sel], c] - 1].sum()
+    ztop = np.isclose(deck.coords[node, 2], deck.coords[:, 2].max()) & (comp == 2)
+    top_force = float(R[-deck.eqn[node[ztop], 2] - 1].sum())
+    roof = spmv_roofline(args, deck, r)
+    out = {
+        "metric": METRIC, "value": n * r["steps"] / r["dt"], "unit": "DOFs/s", "n_gpus": 1, "steps": r["steps"],
+        "warmup": args.c2_warm, "ms_per_step": r["dt"] / r["steps"] * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": deck.name, "elements": deck.nelem, "free_dofs": n, "nnz": r["nnz"],
+                   "step": (f"NonLinearStatic load steps {args.c2_warm + 1}..{args.c2_warm + args.c2_steps} of 20 "
+                            "(prescribed end displacement, 5x yield strain in total): direct control, tangent "
+                            "initial guess (initialguess 1), full Newton with consistent tangent, rtolf 1e-8, "
+                            f"Jacobi-PCG to {args.nl_lstol:g}"),
+                   "newton_iters": r["newton"], "pcg_iters": r["pcg"],
+                   "checks": {"max_force_error": float(max(r["force_error"])),
+                              "reaction_sum_xyz_over_end_force": [float(v / abs(top_force)) for v in Rz],
+                              "end_force": top_force}},
+        "roofline": roof, "cpu_baseline": None,
     }
     print(json.dumps(out), flush=True)
     return 0
@@ -518,7 +577,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", choices=["cfg1", "cfg0", "cfg3", "cfg4"], default="cfg3")
+    ap.add_argument("--workload", choices=["cfg1", "cfg0", "cfg2", "cfg3", "cfg4"], default="cfg3")
+    ap.add_argument("--c2-warm", type=int, default=1, help="untimed load steps of config 2")
+    ap.add_argument("--c2-steps", type=int, default=2, help="timed load steps of config 2")
     ap.add_argument("--rve-count", type=int, default=512)
     ap.add_argument("--rve-tol", type=float, default=1e-8)
     ap.add_argument("--forcing", type=float, default=0.1,
@@ -571,6 +632,8 @@ def main():
         return main_newton(args)
     if args.workload == "cfg4":
         return main_rve(args)
+    if args.workload == "cfg2" and args.mode == "step":
+        return main_tet(args)
     t = time.perf_counter()
     deck = make_deck(args)
     log(f"[bench] deck {deck.name}: nnode={deck.nnode} nelem={deck.nelem} neq={deck.neq} "

@@ -10,7 +10,8 @@ then dof managers, elements, cross sections, materials, boundary conditions,
 initial conditions, load-time functions and sets.
 
 Supported records: node; LSpace / LTRSpace / QSpace; SimpleCS; IsoLE and
-MisesMat; BoundaryCondition; ConstantSurfaceLoad; ConstantFunction and
+MisesMat (with density ``d``); BoundaryCondition; ConstantSurfaceLoad;
+DeadWeight; ConstantFunction and
 PiecewiseLinFunction; Set with nodes / elements / elementranges /
 elementBoundaries / noderanges.
 
@@ -25,7 +26,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .meshgen import LSPACE, LTRSPACE, QSPACE, MAT_ISOLE, MAT_MISES, MATP, Deck
+from .meshgen import LSPACE, LTRSPACE, QSPACE, MAT_ISOLE, MAT_MISES, MATP, Deck, body_load_vector
 
 _ETYPES = {"lspace": LSPACE, "ltrspace": LTRSPACE, "qspace": QSPACE}
 
@@ -138,14 +139,18 @@ class DeckProblem:
     elem_numbers: np.ndarray
     checks: list
     tolerance: float
+    bodyloads: list = field(default_factory=list)
+    density: np.ndarray = None     # per material index (record ``d``)
 
     def time_of_step(self, k):
         """StaticStructural / LinearStatic: step k (1-based) at t = k * deltaT."""
         dt = self.engng.scalar("deltat", 1.0) if self.engng.has("deltat") else 1.0
         return k * dt
 
-    def step_loading(self, k):
-        """(prescribed values per prescribed equation, (f_free, f_presc)) of step k."""
+    def step_loading(self, k, body_load=None):
+        """(prescribed values per prescribed equation, (f_free, f_presc)) of
+        step k. ``body_load(deck, elems, dens, b, factor)`` assembles each
+        DeadWeight (default: meshgen.body_load_vector)."""
         t = self.time_of_step(k)
         d = self.deck
         val = np.zeros_like(d.bc_value)
@@ -163,6 +168,13 @@ class DeckProblem:
             d.faces = ld["faces"]
             d.traction = np.tile(ld["traction"], (len(ld["faces"]), 1))
             ff, fp = d.external_load(fac)
+            f_free += ff
+            f_presc += fp
+        bl_fn = body_load or body_load_vector
+        for bl in self.bodyloads:
+            fac = self.ltf[bl["ltf"]](t)
+            el = bl["elements"]
+            ff, fp = bl_fn(d, el, self.density[d.emat[el]], bl["b"], fac)
             f_free += ff
             f_presc += fp
         return up, (f_free, f_presc)
@@ -284,8 +296,10 @@ def read_deck(path) -> DeckProblem:
     # materials
     mrows = []
     mnum = {}
+    dens = []
     for i, r in enumerate(mats):
         row = np.zeros(MATP)
+        dens.append(r.scalar("d", 0.0))
         if r.keyword == "isole":
             row[0] = MAT_ISOLE
         elif r.keyword == "misesmat":
@@ -323,7 +337,7 @@ def read_deck(path) -> DeckProblem:
             raise DeckError(f"unsupported load-time function {r.keyword}")
     # boundary conditions and loads
     bc_mask = np.zeros((nd, 3), dtype=bool)
-    blist, llist = [], []
+    blist, llist, bodies = [], [], []
     for r in bcs:
         f = r.scalar("loadtimefunction", cast=int)
         if r.keyword == "boundarycondition":
@@ -342,13 +356,21 @@ def read_deck(path) -> DeckProblem:
                 tr[dof - 1] = c
             faces = setd[r.scalar("set", cast=int)]["faces"]
             llist.append(dict(faces=faces, traction=tr, ltf=f))
+        elif r.keyword == "deadweight":
+            # deadweight.C: Components are the force per unit mass in the
+            # global c.s.; the element set carries it
+            comps = np.asarray(r.array("components"), dtype=np.float64)
+            if comps.size != 3:
+                raise DeckError("DeadWeight needs three components in 3d")
+            el = np.array(setd[r.scalar("set", cast=int)].get("elements", []), dtype=np.int64)
+            bodies.append(dict(elements=el, b=comps, ltf=f))
         else:
             raise DeckError(f"unsupported boundary condition / load {r.keyword}")
     mat = np.array(mrows)
     deck = Deck(str(path), etype, coords, conn, emat, mat, bc_mask, np.zeros((nd, 3)))
     tol, checks = _parse_checks(checks_txt)
     prob = DeckProblem(deck, engng.keyword, engng.scalar("nsteps", 1, int), engng, blist, llist, ltf,
-                       node_numbers, elem_numbers, checks, tol)
+                       node_numbers, elem_numbers, checks, tol, bodies, np.array(dens))
     prob.output_file = lines[0].strip()
     prob.metasteps = metasteps
     if metasteps:

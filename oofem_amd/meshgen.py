@@ -287,6 +287,80 @@ def surface_load_vector(deck: Deck, factor=1.0):
     return f_free, f_presc
 
 
+def _default_rule(etype):
+    """Default volume Gauss rule: 2x2x2 (LSpace), 3x3x3 (QSpace), the
+    one-point tetrahedron rule (LTRSpace; lspace.C:68, qspace.C:56,
+    ltrspace.C:69). Points (ngp, 3) and weights (ngp,)."""
+    if etype == LTRSPACE:
+        return np.full((1, 3), 0.25), np.array([1.0 / 6.0])
+    if etype == LSPACE:
+        x, w = np.array([-1.0, 1.0]) / np.sqrt(3.0), np.ones(2)
+    else:
+        x, w = np.array([-np.sqrt(0.6), 0.0, np.sqrt(0.6)]), np.array([5.0, 8.0, 5.0]) / 9.0
+    I, J, K = np.meshgrid(np.arange(x.size), np.arange(x.size), np.arange(x.size), indexing="ij")
+    I, J, K = I.ravel(), J.ravel(), K.ravel()
+    return np.stack([x[I], x[J], x[K]], axis=1), w[I] * w[J] * w[K]
+
+
+def _shape(etype, xi):
+    """N (nen,) and dN/dxi (nen, 3) at one local point, in the node orders of
+    fei3dhexalin.C:46-62, fei3dhexaquad.C:46-73 and fei3dtetlin.C:46-54."""
+    if etype == LTRSPACE:
+        u, v, w = xi
+        return (np.array([u, v, w, 1.0 - u - v - w]),
+                np.array([[1.0, 0, 0], [0, 1.0, 0], [0, 0, 1.0], [-1.0, -1.0, -1.0]]))
+    nen = NEN[etype]
+    N, dN = np.empty(nen), np.empty((nen, 3))
+    for a in range(nen):
+        p = _QS_POS[a].astype(np.float64)
+        g = 1.0 + p * xi                       # per-direction linear factors
+        if etype == LSPACE or a < 8:
+            lin = 0.125 * g.prod()
+            dlin = np.array([0.125 * p[j] * np.prod(np.delete(g, j)) for j in range(3)])
+            if etype == LSPACE:
+                N[a], dN[a] = lin, dlin
+            else:                              # serendipity corner
+                s = float(p @ xi) - 2.0
+                N[a], dN[a] = lin * s, dlin * s + lin * p
+        else:                                  # mid-edge node, p[k] == 0
+            k = int(np.nonzero(p == 0)[0][0])
+            o = [j for j in range(3) if j != k]
+            bub = 1.0 - xi[k] ** 2
+            N[a] = 0.25 * bub * g[o[0]] * g[o[1]]
+            dN[a, k] = -0.5 * xi[k] * g[o[0]] * g[o[1]]
+            dN[a, o[0]] = 0.25 * bub * p[o[0]] * g[o[1]]
+            dN[a, o[1]] = 0.25 * bub * p[o[1]] * g[o[0]]
+    return N, dN
+
+
+def body_load_vector(deck: Deck, elems, dens, b, factor=1.0):
+    """DeadWeight in the global c.s. (structuralelement.C:262-296): per
+    element, sum over the default Gauss rule of N^T b |det J| w rho.
+    ``elems`` 0-based element indices, ``dens`` their densities, ``b`` the
+    force per unit mass (3,). Returns (f_free, f_presc)."""
+    f_free = np.zeros(deck.neq)
+    f_presc = np.zeros(deck.npeq)
+    elems = np.asarray(elems, dtype=np.int64)
+    if elems.size == 0:
+        return f_free, f_presc
+    bf = np.asarray(b, dtype=np.float64) * factor
+    nodes = deck.conn[elems]                                           # (ne, nen)
+    X = deck.coords[nodes]                                             # (ne, nen, 3)
+    fe = np.zeros(X.shape)
+    pts, wts = _default_rule(deck.etype)
+    for xi, w in zip(pts, wts):
+        N, dN = _shape(deck.etype, xi)
+        J = np.einsum("ai,eaj->eij", dN, X)
+        dV = np.abs(np.linalg.det(J)) * w * np.asarray(dens, dtype=np.float64)
+        fe += N[None, :, None] * bf[None, None, :] * dV[:, None, None]
+    q = deck.eqn[nodes]                                                # (ne, nen, 3)
+    m = q > 0
+    np.add.at(f_free, q[m] - 1, fe[m])
+    m = q < 0
+    np.add.at(f_presc, -q[m] - 1, fe[m])
+    return f_free, f_presc
+
+
 # ------------------------------------------------------------------- RVEs
 # QSpace local node positions in (xi, eta, zeta) (fei3dhexaquad.C node
 # order: 8 corners as LSpace, 4 mid-edge nodes of the zeta = +1 face, 4 of

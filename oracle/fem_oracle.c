@@ -811,6 +811,67 @@ static const int HEX_SURF[6][4] = {{1, 4, 3, 2}, {5, 6, 7, 8}, {1, 2, 6, 5},
                                    {2, 3, 7, 6}, {3, 4, 8, 7}, {4, 1, 5, 8}};
 static const int TET_SURF[4][3] = {{1, 3, 2}, {1, 2, 4}, {2, 3, 4}, {1, 4, 3}};
 
+/* shape functions: FEI3dHexaLin::evalN fei3dhexalin.C:66-80, FEI3dTetLin::evalN
+ * fei3dtetlin.C:62-67, FEI3dHexaQuad::evalN fei3dhexaquad.C:128-189 (node
+ * order as in orc_dndxi) */
+static void orc_n(int etype, const double *xi, double *N)
+{
+    const double u = xi[0], v = xi[1], w = xi[2];
+    static const double sx[8] = {-1, -1, 1, 1, -1, -1, 1, 1};
+    static const double sy[8] = {-1, 1, 1, -1, -1, 1, 1, -1};
+    static const double sz[8] = {1, 1, 1, 1, -1, -1, -1, -1};
+    if (etype == ORC_LTRSPACE) {
+        N[0] = u; N[1] = v; N[2] = w; N[3] = 1.0 - u - v - w;
+        return;
+    }
+    if (etype == ORC_LSPACE) {
+        for (int a = 0; a < 8; a++) N[a] = 0.125 * (1. + sx[a] * u) * (1. + sy[a] * v) * (1. + sz[a] * w);
+        return;
+    }
+    for (int a = 0; a < 8; a++)
+        N[a] = 0.125 * (1. + sx[a] * u) * (1. + sy[a] * v) * (1. + sz[a] * w) * (sx[a] * u + sy[a] * v + sz[a] * w - 2.);
+    for (int layer = 0; layer < 2; layer++) {
+        const double pw = 1.0 + (layer == 0 ? 1.0 : -1.0) * w;
+        const int b = 8 + 4 * layer;
+        N[b + 0] = 0.25 * (1.0 - v * v) * (1.0 - u) * pw;
+        N[b + 1] = 0.25 * (1.0 - u * u) * (1.0 + v) * pw;
+        N[b + 2] = 0.25 * (1.0 - v * v) * (1.0 + u) * pw;
+        N[b + 3] = 0.25 * (1.0 - u * u) * (1.0 - v) * pw;
+    }
+    static const double cu[4] = {-1, -1, 1, 1};
+    static const double cv[4] = {-1, 1, 1, -1};
+    for (int k = 0; k < 4; k++) N[16 + k] = 0.25 * (1.0 + cu[k] * u) * (1.0 + cv[k] * v) * (1.0 - w * w);
+}
+
+/* StructuralElement::computeBodyLoadVectorAt structuralelement.C:262-296
+ * (DeadWeight): f_e = sum over the default Gauss rule of N^T b dV rho, with
+ * rho the element's density and b the global force per unit mass */
+int orc_body_load(const orc_mesh *m, int nel, const int *elem, const double *dens, const double *b,
+                  double *f_free, double *f_presc)
+{
+    const int nen = orc_nen(m->etype);
+    double xi[81], w[27], X[60], N[20];
+    const int ngp = orc_gauss(m->etype, xi, w);
+    for (int k = 0; k < nel; k++) {
+        const int e = elem[k];
+        gather_coords(m, e, X);
+        double fe[60] = {0};
+        for (int g = 0; g < ngp; g++) {
+            orc_n(m->etype, xi + 3 * g, N);
+            const double dV = fabs(orc_detj(m->etype, X, xi + 3 * g)) * w[g];
+            for (int a = 0; a < nen; a++)
+                for (int d = 0; d < 3; d++) fe[3 * a + d] += N[a] * b[d] * dV * dens[k];
+        }
+        for (int a = 0; a < nen; a++)
+            for (int d = 0; d < 3; d++) {
+                const int q = m->eqn[3 * (size_t)m->conn[(size_t)e * nen + a] + d];
+                if (q > 0 && f_free) f_free[q - 1] += fe[3 * a + d];
+                if (q < 0 && f_presc) f_presc[-q - 1] += fe[3 * a + d];
+            }
+    }
+    return 0;
+}
+
 int orc_surface_load(const orc_mesh *m, int nface, const int *elem, const int *surf,
                      const double *traction, double *f_free, double *f_presc)
 {

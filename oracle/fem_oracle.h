@@ -100,6 +100,11 @@ void orc_state_restart(const orc_mesh *m, double *state);
 int orc_surface_load(const orc_mesh *m, int nface, const int *elem, const int *surf,
                      const double *traction, double *f_free, double *f_presc);
 
+/* DeadWeight body load on the elements elem[0..nel) with densities dens and
+ * force per unit mass b (3), added into f_free / f_presc */
+int orc_body_load(const orc_mesh *m, int nel, const int *elem, const double *dens, const double *b,
+                  double *f_free, double *f_presc);
+
 /* f = sum_e K_e(tangent) du_e over prescribed increments (free rows) */
 int orc_extrapolated_forces(const orc_mesh *m, const double *state, const double *du_presc, double *f);
 

@@ -210,6 +210,23 @@ def surface_load(deck, factor=1.0):
     return f, fp
 
 
+def body_load(deck, elems, dens, b, factor=1.0):
+    """orc_body_load: DeadWeight of the elements ``elems`` (0-based) with
+    densities ``dens`` and force per unit mass ``b`` times ``factor``."""
+    m = Mesh(deck)
+    f = np.zeros(deck.neq)
+    fp = np.zeros(deck.npeq)
+    el = np.ascontiguousarray(elems, dtype=np.int32)
+    de = _d(np.asarray(dens, dtype=np.float64))
+    bb = _d(np.asarray(b, dtype=np.float64) * factor)
+    rc = lib().orc_body_load(m.ref, len(el), el.ctypes.data_as(C.c_void_p), de.ctypes.data_as(C.c_void_p),
+                             bb.ctypes.data_as(C.c_void_p), f.ctypes.data_as(C.c_void_p),
+                             fp.ctypes.data_as(C.c_void_p))
+    if rc:
+        raise RuntimeError(f"orc_body_load rc={rc}")
+    return f, fp
+
+
 def nonlinear_step(deck, state, X, u_presc, RT=None, rtolf=1e-8, maxiter=50, manrmsteps=1, lstol=1e-12,
                    u_presc_prev=None, initial_guess=True):
     """One StaticStructural step with NRSolver; X and state updated in place.

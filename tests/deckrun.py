@@ -26,7 +26,7 @@ def run_deck_oracle(prob, tol=1e-14):
     rp, ci, val = oracle.csr(d)
     out = {}
     for k in range(1, prob.nsteps + 1):
-        up, (f, fp) = prob.step_loading(k)
+        up, (f, fp) = prob.step_loading(k, body_load=oracle.body_load)
         fint, _, _ = oracle.internal_forces(d, np.zeros(d.neq), up)
         u, it, res, rc = oracle.pcg(rp, ci, val, f - fint, tol=tol, maxit=100000)
         state = oracle.state_new(d)

@@ -729,13 +729,21 @@ __global__ void k_fill_d(int n, double *__restrict__ a, const PcgState *__restri
 // elements in FP64 (neighbouring element forces of a smooth field nearly
 // cancel at a node, by a factor of about h/L, so an FP32 sum would lose that
 // many digits).
-// 8 threads per element (one per Gauss point, consecutive lanes of one
-// wave), 32 elements per block.  Node data are staged in LDS relative to node
-// 1 (coordinates differenced in FP64, displacements in FP32: a rigid
-// translation adds nothing to the strain).  The 24 Gauss-point forces
-// B^T sigma dV of each thread are summed over the 8 points in registers by a
-// reduce-scatter butterfly (xor 4, 2, 1: 12 + 6 + 3 exchanges, a fixed
-// order), after which thread g holds the forces of the element's node g:
+// 8 threads per element (consecutive lanes of one wave), 32 elements per
+// block.  Lane g stands for the Gauss point and for the node whose local
+// (xi, eta, zeta) signs are the bits (4, 2, 1) of g (LSpace nodes 5 1 6 2 8
+// 4 7 3 for g = 0..7).  The trilinear field v = sum_a N_a v_a equals
+// sum_S c_S prod_{j in S} xi_j over the subsets S of {xi, eta, zeta}, with
+// 8 c_S = sum_a v_a prod_{j in S} s_aj: a Walsh-Hadamard transform of the 8
+// nodal values, done across the 8 lanes by three xor butterflies (lane S
+// ends with c_S).  The lanes share the coefficients of x (coordinates
+// relative to node 1, differenced in FP64) and u (relative to the lane-0
+// node, in FP32: a rigid translation adds nothing to the strain) through
+// LDS; J and du/dxi at a point are then 3 FMAs per entry.  The forces take
+// the transposed route: each point's T = dV sigma J^-T / 8 gives 8 subset
+// terms F_S, whose sum over the points is a reduce-scatter butterfly (xor 4,
+// 2, 1: 12 + 6 + 3 exchanges, a fixed order) leaving F_g on lane g, and the
+// inverse transform (three more butterflies) gives lane g its node's force:
 // either y += f over one color (elems = the color's list), or, with ebuf, the
 // element's 24 forces are stored (elems = NULL: every element, one launch)
 // for k_mf_gather to sum per node, or, with yslot, each force is stored in
@@ -743,6 +751,22 @@ __global__ void k_fill_d(int n, double *__restrict__ a, const PcgState *__restri
 // k_mf_slot_sum.  Node coordinates come as per-element FP32 differences
 // (erel, from k_mf_erel).
 constexpr int MF_EPB = 32;
+
+// exchanges inside groups of 8 lanes with no address arithmetic: xor 1 and 2
+// by DPP quad permutes, xor 4 and the group's first lane by ds_swizzle in
+// bit-mask mode (source lane ((lane & and) | or) ^ xor within 32 lanes)
+__device__ __forceinline__ float lane_xor(float v, int m)
+{
+    const int i = __float_as_int(v);
+    if (m == 1) return __int_as_float(__builtin_amdgcn_mov_dpp(i, 0xB1, 0xF, 0xF, false));
+    if (m == 2) return __int_as_float(__builtin_amdgcn_mov_dpp(i, 0x4E, 0xF, 0xF, false));
+    return __int_as_float(__builtin_amdgcn_ds_swizzle(i, 0x1F | (4 << 10)));
+}
+
+__device__ __forceinline__ float lane_group_first(float v)
+{
+    return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x18));
+}
 __global__ __launch_bounds__(256) void k_mf_hex(const int *__restrict__ elems, int nel,
                                                 const float *__restrict__ erel, const int *__restrict__ conn,
                                                 const int *__restrict__ eqn, const int *__restrict__ emat,
@@ -753,56 +777,62 @@ __global__ __launch_bounds__(256) void k_mf_hex(const int *__restrict__ elems, i
                                                 const uint8_t *__restrict__ ecolor, int neq,
                                                 const PcgState *__restrict__ st)
 {
-    __shared__ float sX[MF_EPB][8][3];
-    __shared__ float sU[MF_EPB][8][3];
+    __shared__ __align__(16) float sC[MF_EPB][8][8];   // per element: c_S of x (0..2) and u (3..5)
     if (st && halted(st)) return;
     const int t = threadIdx.x, le = t >> 3, g = t & 7;
     const int ei = blockIdx.x * MF_EPB + le;
     const bool active = ei < nel;
+    const int nl = (0x26371504 >> (4 * g)) & 15;   // this lane's LSpace node (0-based)
     int e = 0;
-    int myq[3] = {0, 0, 0};   // equations of this thread's node g
+    int myq[3] = {0, 0, 0};   // equations of node nl
+    float v[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     if (active) {
         e = elems ? elems[ei] : ei;
-        const int node = conn[(size_t)e * 8 + g];
+        const int node = conn[(size_t)e * 8 + nl];
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-            sX[le][g][d] = g ? erel[(size_t)e * 21 + 3 * (g - 1) + d] : 0.0f;
+            v[d] = nl ? erel[(size_t)e * 21 + 3 * (nl - 1) + d] : 0.0f;
             const int q = eqn[3 * (size_t)node + d];
             myq[d] = q;
-            sU[le][g][d] = q > 0 ? x[q - 1] : 0.0f;
+            v[3 + d] = q > 0 ? x[q - 1] : 0.0f;
         }
     }
+    // the 8 lanes of an element are consecutive lanes of one wave and every
+    // lane takes part in the exchanges (inactive ones with zeros)
+#pragma unroll
+    for (int d = 3; d < 6; d++) v[d] -= lane_group_first(v[d]);
+#pragma unroll
+    for (int m = 4; m >= 1; m >>= 1) {
+        const float sg = (g & m) ? -1.f : 1.f;
+#pragma unroll
+        for (int k = 0; k < 6; k++) v[k] = fmaf(sg, lane_xor(v[k], m), v[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < 6; k++) sC[le][g][k] = 0.125f * v[k];
     __syncthreads();
-    // the 8 threads of an element are consecutive lanes of one wave and all
-    // take the same branch (active depends on the element only), so the
-    // butterfly below exchanges within complete 8-lane groups
     float fr[24];
 #pragma unroll
     for (int k = 0; k < 24; k++) fr[k] = 0.f;
     if (active) {
         const float a = 0.577350269189626f;
-        const float xi0 = (g & 4) ? a : -a, xi1 = (g & 2) ? a : -a, xi2 = (g & 1) ? a : -a;
-        float dN[8][3];
-        float J[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        float gu[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};   // du_i / dxi_j
-        const float u0[3] = {sU[le][0][0], sU[le][0][1], sU[le][0][2]};
+        const float xi = (g & 4) ? a : -a, eta = (g & 2) ? a : -a, ze = (g & 1) ? a : -a;
+        const float ez = eta * ze, xz = xi * ze, xe = xi * eta;
+        float C[8][6];
 #pragma unroll
-        for (int n = 0; n < 8; n++) {
-            const float sx = (n == 2 || n == 3 || n == 6 || n == 7) ? 1.f : -1.f;
-            const float sy = (n == 1 || n == 2 || n == 5 || n == 6) ? 1.f : -1.f;
-            const float sz = (n < 4) ? 1.f : -1.f;
-            dN[n][0] = sx * 0.125f * (1.f + sy * xi1) * (1.f + sz * xi2);
-            dN[n][1] = sy * 0.125f * (1.f + sx * xi0) * (1.f + sz * xi2);
-            dN[n][2] = sz * 0.125f * (1.f + sx * xi0) * (1.f + sy * xi1);
+        for (int S = 1; S < 8; S++) {
+            const float4 lo = *reinterpret_cast<const float4 *>(&sC[le][S][0]);
+            const float2 hi = *reinterpret_cast<const float2 *>(&sC[le][S][4]);
+            C[S][0] = lo.x; C[S][1] = lo.y; C[S][2] = lo.z; C[S][3] = lo.w; C[S][4] = hi.x; C[S][5] = hi.y;
+        }
+        float J[9], gu[9];   // J[3i+k] = dx_i/dxi_k, gu[3i+k] = du_i/dxi_k
 #pragma unroll
-            for (int i = 0; i < 3; i++) {
-                const float xv = sX[le][n][i], uv = sU[le][n][i] - u0[i];
-#pragma unroll
-                for (int j = 0; j < 3; j++) {
-                    J[3 * i + j] = fmaf(xv, dN[n][j], J[3 * i + j]);
-                    gu[3 * i + j] = fmaf(uv, dN[n][j], gu[3 * i + j]);
-                }
-            }
+        for (int i = 0; i < 3; i++) {
+            J[3 * i + 0] = fmaf(C[7][i], ez, fmaf(C[5][i], ze, fmaf(C[6][i], eta, C[4][i])));
+            J[3 * i + 1] = fmaf(C[7][i], xz, fmaf(C[3][i], ze, fmaf(C[6][i], xi, C[2][i])));
+            J[3 * i + 2] = fmaf(C[7][i], xe, fmaf(C[3][i], eta, fmaf(C[5][i], xi, C[1][i])));
+            gu[3 * i + 0] = fmaf(C[7][3 + i], ez, fmaf(C[5][3 + i], ze, fmaf(C[6][3 + i], eta, C[4][3 + i])));
+            gu[3 * i + 1] = fmaf(C[7][3 + i], xz, fmaf(C[3][3 + i], ze, fmaf(C[6][3 + i], xi, C[2][3 + i])));
+            gu[3 * i + 2] = fmaf(C[7][3 + i], xe, fmaf(C[3][3 + i], eta, fmaf(C[5][3 + i], xi, C[1][3 + i])));
         }
         const float det = J[0] * (J[4] * J[8] - J[5] * J[7]) - J[1] * (J[3] * J[8] - J[5] * J[6]) +
                           J[2] * (J[3] * J[7] - J[4] * J[6]);
@@ -864,41 +894,52 @@ __global__ __launch_bounds__(256) void k_mf_hex(const int *__restrict__ elems, i
             sig[4] = D[4] * eps[0] + D[9] * eps[1] + D[13] * eps[2] + D[16] * eps[3] + D[18] * eps[4] + D[19] * eps[5];
             sig[5] = D[5] * eps[0] + D[10] * eps[1] + D[14] * eps[2] + D[17] * eps[3] + D[19] * eps[4] + D[20] * eps[5];
         }
-        // T = dV sigma J^-T: f_n = T dN_n (dN/dx = J^-T dN/dxi in column form)
-        const float dV = fabsf(det);
+        // T = dV sigma J^-T / 8 (T[3i+k]); f_a = sum_k T[., k] dN_a/dxi_k
+        const float dV = 0.125f * fabsf(det);
         const float S[9] = {sig[0], sig[5], sig[4], sig[5], sig[1], sig[3], sig[4], sig[3], sig[2]};
-        float T[9];
 #pragma unroll
-        for (int i = 0; i < 3; i++)
+        for (int i = 0; i < 3; i++) {
+            float T[3];
 #pragma unroll
             for (int k = 0; k < 3; k++)
-                T[3 * i + k] = dV * (S[3 * i] * I[3 * k] + S[3 * i + 1] * I[3 * k + 1] + S[3 * i + 2] * I[3 * k + 2]);
-#pragma unroll
-        for (int n = 0; n < 8; n++)
-#pragma unroll
-            for (int i = 0; i < 3; i++)
-                fr[3 * n + i] = T[3 * i] * dN[n][0] + T[3 * i + 1] * dN[n][1] + T[3 * i + 2] * dN[n][2];
+                T[k] = dV * (S[3 * i] * I[3 * k] + S[3 * i + 1] * I[3 * k + 1] + S[3 * i + 2] * I[3 * k + 2]);
+            // subset terms F_S (S as lane bits xi 4, eta 2, zeta 1; F_0 = 0)
+            fr[3 + i] = T[2];
+            fr[6 + i] = T[1];
+            fr[9 + i] = fmaf(T[1], ze, T[2] * eta);
+            fr[12 + i] = T[0];
+            fr[15 + i] = fmaf(T[0], ze, T[2] * xi);
+            fr[18 + i] = fmaf(T[0], eta, T[1] * xi);
+            fr[21 + i] = fmaf(T[0], ez, fmaf(T[1], xz, T[2] * xe));
+        }
     }
     // reduce-scatter over the 8 Gauss points: at the step with mask m the
     // thread keeps the half of its current components selected by (g & m)
     // and adds the partner's copy of that half; after m = 4, 2, 1 thread g
-    // holds components 12 b2 + 6 b1 + 3 b0 + (0..2) = 3 g + (0..2)
+    // holds components 12 b2 + 6 b1 + 3 b0 + (0..2) = 3 g + (0..2), i.e. F_g
     {
         const bool h4 = g & 4, h2 = g & 2, h1 = g & 1;
 #pragma unroll
         for (int j = 0; j < 12; j++) {
             const float keep = h4 ? fr[12 + j] : fr[j], send = h4 ? fr[j] : fr[12 + j];
-            fr[j] = keep + __shfl_xor(send, 4, 64);
+            fr[j] = keep + lane_xor(send, 4);
         }
 #pragma unroll
         for (int j = 0; j < 6; j++) {
             const float keep = h2 ? fr[6 + j] : fr[j], send = h2 ? fr[j] : fr[6 + j];
-            fr[j] = keep + __shfl_xor(send, 2, 64);
+            fr[j] = keep + lane_xor(send, 2);
         }
 #pragma unroll
         for (int j = 0; j < 3; j++) {
             const float keep = h1 ? fr[3 + j] : fr[j], send = h1 ? fr[j] : fr[3 + j];
-            fr[j] = keep + __shfl_xor(send, 1, 64);
+            fr[j] = keep + lane_xor(send, 1);
+        }
+        // inverse transform: f_a = sum_S prod_{j in S} s_aj F_S
+#pragma unroll
+        for (int m = 4; m >= 1; m >>= 1) {
+            const float sg = (g & m) ? 1.f : -1.f;
+#pragma unroll
+            for (int i = 0; i < 3; i++) fr[i] = fmaf(sg, lane_xor(fr[i], m), fr[i]);
         }
     }
     if (active) {
@@ -906,7 +947,7 @@ __global__ __launch_bounds__(256) void k_mf_hex(const int *__restrict__ elems, i
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             if (ebuf)
-                ebuf[(size_t)e * 24 + 3 * g + i] = fr[i];
+                ebuf[(size_t)e * 24 + 3 * nl + i] = fr[i];
             else if (myq[i] > 0) {
                 if (yslot)
                     yslot[slot + myq[i] - 1] = fr[i];

@@ -89,6 +89,26 @@ int hfem_problem_linear_static(hfem_problem *p, int *iterations, double *residua
 /* StaticStructural::solveYourselfAt + NRSolver::solve (staticstructural.C:224-358,
  * nrsolver.C:205-404): out3 = {Newton iterations, PCG iterations, ConvergedReason} */
 int hfem_problem_nonlinear_step(hfem_problem *p, int *out3, double *force_error);
+/* NonLinearStatic with indirect control (controlmode 0): CylindricalALM
+ * (calmls.C:60-101, 681-897) with its keywords psi, maxiter, maxrestarts,
+ * minsteplength, steplength, initialsteplength (<= 0: steplength),
+ * reqiterations, miniter, manrmsteps, rtolf, rtold, rootselectiontype (0
+ * cosine, 1 dot); hyperplane control, the CALM line search and dof-group
+ * norms are not part of this build.  Called again, it keeps the arc length
+ * reached unless that exceeds the new steplength. */
+int hfem_problem_set_arc_length(hfem_problem *p, double psi, int maxiter, int maxrestarts, double minsteplength,
+                                double steplength, double initialsteplength, int reqiterations, int miniterations,
+                                int manrmsteps, double rtolf, double rtold, int rootselectiontype);
+/* the reference load R (NonLinearStatic::assembleIncrementalReferenceLoadVectors,
+ * refloadmode 0: the total loads of the first step), free and prescribed parts */
+int hfem_problem_set_reference_load(hfem_problem *p, const double *f_free, const double *f_presc);
+/* one step of CylindricalALM::solve (calmls.C:108-481): out4 = {iterations,
+ * PCG iterations, restarts, ConvergedReason}, out4d = {load level reached,
+ * arc length of the next step, force error, displacement error}.  After
+ * maxrestarts restarts the step is accepted as it stands (as the reference
+ * does, with a warning) and the reason is CR_DIVERGED_ITS.  The reactions
+ * then use the reached load level times R. */
+int hfem_problem_arc_length_step(hfem_problem *p, int *out4, double *out4d);
 /* EngngModel::updateYourself: commit the Gauss-point state */
 int hfem_problem_update(hfem_problem *p);
 

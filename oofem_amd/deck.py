@@ -16,7 +16,7 @@ PiecewiseLinFunction; Set with nodes / elements / elementranges /
 elementBoundaries / noderanges.
 
 The errorcheck block (errorcheckingexportmodule.C:78-735) is a list of
-``#NODE``, ``#REACTION`` and ``#ELEMENT`` lines, each an absolute window
+``#NODE``, ``#REACTION``, ``#ELEMENT`` and ``#LOADLEVEL`` lines, each an absolute window
 ``|value - expected| <= tolerance`` (errorcheckingexportmodule.C:66).
 """
 from __future__ import annotations
@@ -114,7 +114,7 @@ class LoadTimeFunction:
 
 @dataclass
 class Check:
-    kind: str          # "node", "reaction", "element"
+    kind: str          # "node", "reaction", "element", "loadlevel"
     tstep: int
     number: int
     dof: int = 0
@@ -224,6 +224,9 @@ def _parse_checks(txt):
         elif kind == "reaction":
             out.append(Check("reaction", int(kv["tstep"]), int(kv["number"]), dof=int(kv["dof"]),
                              value=float(kv["value"]), line=s))
+        elif kind == "loadlevel":
+            # errorcheckingexportmodule.C:660-703: the engineering model's load level
+            out.append(Check("loadlevel", int(kv["tstep"]), 0, value=float(kv["value"]), line=s))
         elif kind == "element":
             out.append(Check("element", int(kv["tstep"]), int(kv["number"]), gp=int(kv["gp"]),
                              keyword=int(kv["keyword"]), component=int(kv["component"]),
@@ -400,6 +403,10 @@ def evaluate_checks(prob: DeckProblem, results: dict):
             v = r["u_node"][nid[c.number], c.dof - 1]
         elif c.kind == "reaction":
             v = r["reactions"].get((nid[c.number], c.dof), 0.0)
+        elif c.kind == "loadlevel":
+            if "load_level" not in r:
+                raise DeckError("a #LOADLEVEL check needs a load-level (arc-length) analysis")
+            v = r["load_level"]
         else:
             arr = r["gp_sig"] if c.keyword == 1 else r["gp_eps"] if c.keyword == 4 else None
             if arr is None:

@@ -106,6 +106,52 @@ int hfem_problem_nonlinear_step(hfem_problem *p, int *out3, double *force_error)
     });
 }
 
+int hfem_problem_set_arc_length(hfem_problem *p, double psi, int maxiter, int maxrestarts, double minsteplength,
+                                double steplength, double initialsteplength, int reqiterations, int miniterations,
+                                int manrmsteps, double rtolf, double rtold, int rootselectiontype)
+{
+    return guarded([&] {
+        CALMOptions o;
+        o.psi = psi;
+        o.maxiter = maxiter;
+        o.maxRestarts = maxrestarts;
+        o.minStepLength = minsteplength;
+        o.maxStepLength = steplength;
+        o.initialStepLength = initialsteplength;
+        o.reqIterations = reqiterations;
+        o.minIterations = miniterations;
+        o.manrmsteps = manrmsteps;
+        o.rtolf = rtolf;
+        o.rtold = rtold;
+        o.rootSelection = rootselectiontype;
+        reinterpret_cast<StructuralProblem *>(p)->setArcLength(o);
+    });
+}
+
+int hfem_problem_set_reference_load(hfem_problem *p, const double *f_free, const double *f_presc)
+{
+    return guarded([&] {
+        auto *P = reinterpret_cast<StructuralProblem *>(p);
+        if (!f_free || (P->giveNpeq() > 0 && !f_presc)) OOFEM_ERROR("reference load: missing vector");
+        P->setReferenceLoad(f_free, f_presc);
+    });
+}
+
+int hfem_problem_arc_length_step(hfem_problem *p, int *out4, double *out4d)
+{
+    return guarded([&] {
+        CALMStepReport r = reinterpret_cast<StructuralProblem *>(p)->solveArcLengthStep();
+        out4[0] = r.iterations;
+        out4[1] = r.linearIterations;
+        out4[2] = r.restarts;
+        out4[3] = (int)r.status;
+        out4d[0] = r.lambda;
+        out4d[1] = r.stepLength;
+        out4d[2] = r.forceError;
+        out4d[3] = r.dispError;
+    });
+}
+
 int hfem_problem_set_nr_extras(hfem_problem *p, int linesearch, double ls_tol, double ls_amp, double ls_maxeta,
                                double maxinc)
 {

@@ -215,6 +215,31 @@ void StructuralProblem::applyConstraintsToLoadIncrement(int nite, double sums6[6
     }
 }
 
+// staticstructural.C:247-297 / nlinearstatic.C:523-539 (IG_Tangent):
+// extrapolated forces -sum_e K_e du_e of the prescribed increments (free
+// entries of du_e are zero at step start), then K inc = f from zero and
+// X += inc.  tangentCurrent tells whether K now holds the tangent of the
+// current state.  Returns false when the linear solve fails.
+bool StructuralProblem::applyInitialGuess(int &linearIterations, bool &tangentCurrent)
+{
+    tangentCurrent = false;
+    if (initialGuessType != IG_Tangent || npeq == 0) return true;
+    dup.axpby(1.0, &up, 0.0);
+    dup.axpby(-1.0, &upPrev, 1.0);
+    hipfemCheck(hfem_model_tangent_product_dev(ev, 1, nullptr, dup.ptr(), rhs.ptr(), nullptr), "extrapolate");
+    rhs.axpby(0.0, nullptr, -1.0);
+    if (rhs.dot(rhs) > 0.0) {
+        assembleNewtonMatrix();                 // updateMatrix
+        tangentCurrent = true;
+        ddX.zero();
+        ConvergedReason ls = linearSolve(rhs, ddX);
+        linearIterations += lastLinearIterations;
+        if (ls != CR_CONVERGED) return false;
+        X.axpby(1.0, &ddX, 1.0);
+    }
+    return true;
+}
+
 // StaticStructural::solveYourselfAt staticstructural.C:224-358 and
 // NRSolver::solve nrsolver.C:205-404.
 NRStepReport StructuralProblem::solveNonlinearStep()
@@ -226,26 +251,9 @@ NRStepReport StructuralProblem::solveNonlinearStep()
     for (size_t i = 0; i < eqn.size(); i++)
         if (eqn[i] > 0) idsInUse[i % 3]++;
     bool tangentCurrent = false;
-    if (initialGuessType == IG_Tangent && npeq > 0) {
-        // staticstructural.C:247-297: extrapolated forces -sum_e K_e du_e of
-        // the prescribed increments (free entries of du_e are zero at step
-        // start), then K inc = f from zero and X += inc
-        dup.axpby(1.0, &up, 0.0);
-        dup.axpby(-1.0, &upPrev, 1.0);
-        hipfemCheck(hfem_model_tangent_product_dev(ev, 1, nullptr, dup.ptr(), rhs.ptr(), nullptr), "extrapolate");
-        rhs.axpby(0.0, nullptr, -1.0);
-        if (rhs.dot(rhs) > 0.0) {
-            assembleNewtonMatrix();                 // updateMatrix
-            tangentCurrent = true;
-            ddX.zero();
-            ConvergedReason ls = linearSolve(rhs, ddX);
-            rep.linearIterations += lastLinearIterations;
-            if (ls != CR_CONVERGED) {
-                rep.status = CR_FAILED;
-                return rep;
-            }
-            X.axpby(1.0, &ddX, 1.0);
-        }
+    if (!applyInitialGuess(rep.linearIterations, tangentCurrent)) {
+        rep.status = CR_FAILED;
+        return rep;
     }
     ddX.zero();
     // stiffness before the first residual (mCalcStiffBeforeRes, nrsolver.C:245-256);
@@ -475,6 +483,190 @@ std::vector<StepRecord> StructuralProblem::solveTimeSteps(const double *up_ref, 
         currentTime = t;
     }
     return out;
+}
+
+void StructuralProblem::setArcLength(const CALMOptions &o)
+{
+    // CylindricalALM::initializeFrom calmls.C:681-897
+    if (!(o.maxStepLength > 0.0)) OOFEM_ERROR("calm: steplength must be positive, got %g", o.maxStepLength);
+    if (o.manrmsteps < 0 || o.maxRestarts < 0 || o.minIterations < 0) OOFEM_ERROR("calm: negative count");
+    if (o.rootSelection != 0 && o.rootSelection != 1) OOFEM_ERROR("calm: rootselectiontype must be 0 or 1");
+    calm = o;
+    if (calm.psi < 0.0) calm.psi = 1.0;
+    calm.maxiter = std::max(calm.maxiter, 30);
+    if (calm.initialStepLength <= 0.0) calm.initialStepLength = calm.maxStepLength;
+    calm.reqIterations = std::min(std::max(calm.reqIterations, 3), 1000);
+    if (calmDeltaL <= 0.0 || calmDeltaL > calm.maxStepLength) calmDeltaL = calm.initialStepLength;
+    calmMode = calm.manrmsteps > 0 ? 1 : 0;
+    calmOldMode = calmMode;
+    if (!calmSet) {
+        for (DeviceVector *v : {&Rref, &deltaXt, &deltaXb, &dXs, &dXm1, &XInit, &oldDX}) v->allocate(ctx, neq);
+        Rpref.allocate(ctx, npeq);
+    }
+    calmSet = true;
+}
+
+void StructuralProblem::setReferenceLoad(const double *f_free, const double *f_presc)
+{
+    if (!calmSet) OOFEM_ERROR("arc-length options must be set before the reference load");
+    Rref.upload(f_free);
+    if (npeq) Rpref.upload(f_presc);
+}
+
+// CylindricalALM::solve calmls.C:108-481 with computeDeltaLambda 995-1146
+// and checkConvergence 484-677 (hyperplane control off, no line search,
+// norms over all equations, no initial load vector R0).  rhs, ddX, X and the
+// Gauss-point state are the driver's; the solver's own vectors persist in
+// members as the reference's do.
+CALMStepReport StructuralProblem::solveArcLengthStep()
+{
+    if (!calmSet) OOFEM_ERROR("arc-length control needs its options (steplength) first");
+    ensureStructure();
+    CALMStepReport rep;
+    if (calmTick == 0) calmMode = calmOldMode;
+    if (calmTick > 0) calmTick--;
+    // nlinearstatic.C:523-539: the initial guess precedes the numerical method
+    bool tangentCurrent = false;
+    if (!applyInitialGuess(rep.linearIterations, tangentCurrent)) {
+        rep.status = CR_FAILED;
+        return rep;
+    }
+    auto solve = [&](const DeviceVector &b, DeviceVector &x) {
+        const ConvergedReason r = linearSolve(b, x);   // x is the initial guess, as in IMLSolver
+        rep.linearIterations += lastLinearIterations;
+        return r == CR_CONVERGED;
+    };
+    XInit.axpby(1.0, &X, 0.0);
+    deltaXt.zero();
+    deltaXb.zero();
+    const double RR = Rref.dot(Rref);
+    const double psi2 = calm.psi * calm.psi;
+    double Lambda = reachedLambda, DeltaLambda = 0.0, deltaLambda = 0.0;
+    int irest = 0, nite = 0;
+    double ebe[3];
+    for (;;) {
+        // A: predictor from the tangent of the step's start
+        dXs.zero();
+        assembleNewtonMatrix();
+        if (!solve(Rref, deltaXt)) { rep.status = CR_FAILED; return rep; }
+        const double p = std::sqrt(deltaXt.dot(deltaXt) + psi2 * RR);
+        Lambda = reachedLambda;
+        const double XR = (calm.rootSelection == 0 || !haveOldDX) ? deltaXt.dot(Rref) : deltaXt.dot(oldDX);
+        DeltaLambda = deltaLambda = (XR < 0.0 ? -1.0 : 1.0) * calmDeltaL / p;
+        Lambda += DeltaLambda;
+        rhs.axpby(DeltaLambda, &Rref, 0.0);
+        if (!solve(rhs, dXs)) { rep.status = CR_FAILED; return rep; }
+        X.axpby(1.0, &dXs, 1.0);
+        nite = 0;
+        internalForces(X, true, ebe);
+        bool restart = false, converged = false;
+        do {
+            // B: corrector iterations
+            nite++;
+            dXm1.axpby(1.0, &dXs, 0.0);
+            const double DeltaLambdam1 = DeltaLambda;
+            if (calmMode == 2 || (calmMode == 1 && nite % calm.manrmsteps == 0)) {
+                assembleNewtonMatrix();
+                if (!solve(Rref, deltaXt)) { rep.status = CR_FAILED; return rep; }
+            }
+            rhs.axpby(Lambda, &Rref, 0.0);
+            rhs.axpby(-1.0, &F, 1.0);
+            if (!solve(rhs, deltaXb)) { rep.status = CR_FAILED; return rep; }
+            // B.3: the cylindrical constraint |dX + ddX|^2 + psi^2 RR (DeltaLambda + dl)^2 = deltaL^2
+            {
+                const double XX = deltaXt.dot(deltaXt), XXt = dXs.dot(deltaXt), X_Xt = deltaXb.dot(deltaXt);
+                const double dXdX = dXs.dot(dXs), dXX_ = dXs.dot(deltaXb), X_X_ = deltaXb.dot(deltaXb);
+                const double a1 = XX + psi2 * RR;
+                const double a2 = RR * psi2 * DeltaLambda * 2.0 + 2.0 * XXt + 2.0 * X_Xt;
+                const double a3 = dXdX + 2.0 * dXX_ + X_X_ + DeltaLambda * DeltaLambda * RR * psi2 - calmDeltaL * calmDeltaL;
+                double discr = a2 * a2 - 4.0 * a1 * a3;
+                if (discr < 0.0) {
+                    // calmls.C:1060-1064: a warning, and no load change in this iteration
+                    deltaLambda = 0.0;
+                } else {
+                    discr = std::sqrt(discr);
+                    const double lam1 = (-a2 + discr) / 2. / a1, lam2 = (-a2 - discr) / 2. / a1;
+                    if (calm.rootSelection == 0) {
+                        const double a4 = dXX_ + dXdX, a5 = XXt;
+                        const double cos1 = (a4 + a5 * lam1) / calmDeltaL / calmDeltaL;
+                        const double cos2 = (a4 + a5 * lam2) / calmDeltaL / calmDeltaL;
+                        deltaLambda = cos1 > cos2 ? lam1 : lam2;
+                    } else {
+                        // RST_Dot uses the previous deltaLambda, as calmls.C:1108-1118 does
+                        const double s = DeltaLambda + deltaLambda;
+                        const double c1 = (dXdX + dXX_ + lam1 * XXt) + (psi2 * (s * s + s * lam1));
+                        const double c2 = (dXdX + dXX_ + lam2 * XXt) + (psi2 * (s * s + s * lam2));
+                        deltaLambda = c1 > c2 ? lam1 : lam2;
+                    }
+                }
+            }
+            // B.4-B.6
+            ddX.axpby(deltaLambda, &deltaXt, 0.0);
+            ddX.axpby(1.0, &deltaXb, 1.0);
+            dXs.axpby(1.0, &dXm1, 0.0);
+            dXs.axpby(1.0, &ddX, 1.0);
+            X.axpby(1.0, &XInit, 0.0);
+            X.axpby(1.0, &dXs, 1.0);
+            const double drProduct = ddX.dot(ddX);
+            DeltaLambda = DeltaLambdam1 + deltaLambda;
+            Lambda = reachedLambda + DeltaLambda;
+            internalForces(X, true, ebe);
+            // B.7: convergence check
+            rhs.axpby(Lambda, &Rref, 0.0);
+            rhs.axpby(-1.0, &F, 1.0);
+            double forceErr = rhs.dot(rhs);
+            const double dXX = X.dot(X), eNorm = ebe[0] + ebe[1] + ebe[2];
+            if (RR * Lambda * Lambda > 1.e-6)
+                forceErr = std::sqrt(forceErr / (RR * Lambda * Lambda));
+            else if (eNorm > 1.e-6)
+                forceErr = std::sqrt(forceErr / eNorm);
+            else
+                forceErr = std::sqrt(forceErr);
+            const double dispErr = dXX < 1.e-6 ? drProduct : std::sqrt(drProduct / dXX);
+            const bool outOfRange = std::fabs(forceErr) > calm.rtolf * 1.e10 || std::fabs(dispErr) > calm.rtold * 1.e10 ||
+                                    std::isnan(forceErr) || std::isnan(dispErr);
+            converged = !(std::fabs(forceErr) > calm.rtolf || std::fabs(dispErr) > calm.rtold);
+            rep.forceError = forceErr;
+            rep.dispError = dispErr;
+            if (nite >= calm.maxiter || outOfRange) {
+                irest++;
+                if (irest <= calm.maxRestarts) {
+                    // restart the step with a quarter of the arc length and full Newton for two steps
+                    calmDeltaL = std::max(calmDeltaL * 0.25, calm.minStepLength);
+                    X.axpby(1.0, &XInit, 0.0);
+                    hipfemCheck(hfem_model_restart(ev), "restart");   // EngngModel::initStepIncrements
+                    calmOldMode = calmMode;
+                    calmMode = 2;
+                    calmTick = 2;
+                    restart = true;
+                    break;
+                }
+                // calmls.C:418-424: a warning; the step is then accepted as it stands
+                rep.status = CR_DIVERGED_ITS;
+                break;
+            }
+        } while (!converged || nite < calm.minIterations);
+        if (!restart) break;
+    }
+    // arc length for the next step (calmls.C:450-470)
+    if (nite > calm.reqIterations)
+        calmDeltaL = calmDeltaL * calm.reqIterations / nite;
+    else
+        calmDeltaL = calmDeltaL * std::sqrt(std::sqrt((double)calm.reqIterations / (double)nite));
+    if (calmDeltaL > calm.maxStepLength) calmDeltaL = calm.maxStepLength;
+    if (calmDeltaL < calm.minStepLength) calmDeltaL = calm.minStepLength;
+    if (rep.status == CR_UNKNOWN) rep.status = CR_CONVERGED;
+    reachedLambda = Lambda;
+    oldDX.axpby(1.0, &dXs, 0.0);
+    haveOldDX = true;
+    // external loads of the reached level (NonLinearStatic::computeExternalLoadReactionContribution)
+    RT.axpby(Lambda, &Rref, 0.0);
+    if (npeq) RTp.axpby(Lambda, &Rpref, 0.0);
+    rep.iterations = nite;
+    rep.restarts = irest;
+    rep.lambda = Lambda;
+    rep.stepLength = calmDeltaL;
+    return rep;
 }
 
 void StructuralProblem::updateYourself()

@@ -104,6 +104,30 @@ struct StepRecord {
 // IG_Tangent = 1 (the default, staticstructural.C:130)
 enum InitialGuess { IG_None = 0, IG_Tangent = 1 };
 
+// CylindricalALM input (calmls.C:60-101 defaults, initializeFrom 681-897):
+// hyperplane control off, no CALM line search, norms over all equations
+struct CALMOptions {
+    double psi = 1.0;
+    int maxiter = 30;              // nsmax, at least 30
+    int maxRestarts = 4;
+    double minStepLength = 0.0;
+    double maxStepLength = 0.0;    // `steplength` (required)
+    double initialStepLength = 0.0;   // default: steplength
+    int reqIterations = 3;         // clamped to [3, 1000]
+    int minIterations = 0;
+    int manrmsteps = 0;            // > 0: tangent every manrmsteps iterations (accelerated NRM)
+    double rtolf = 1.e-3, rtold = 1.e-3;
+    int rootSelection = 0;         // 0 cosine of the increments (RST_Cos), 1 dot product (RST_Dot)
+};
+
+struct CALMStepReport {
+    int iterations = 0, linearIterations = 0, restarts = 0;
+    double lambda = 0.0;           // load level reached
+    double stepLength = 0.0;       // arc length for the next step
+    double forceError = 0.0, dispError = 0.0;
+    ConvergedReason status = CR_UNKNOWN;
+};
+
 struct NRStepReport {
     int iterations = 0;        // Newton iterations
     int linearIterations = 0;  // PCG iterations summed over the step
@@ -126,6 +150,13 @@ class StructuralProblem : public EngngModel {
     // state on the device
     DeviceVector X, up, upPrev, dup, RT, RTp, F, Fp, rhs, ddX;
     DeviceVector rb, gls, Xc;   // line-search start and residual, last converged solution
+    // arc-length state kept between steps (CylindricalALM members)
+    DeviceVector Rref, Rpref, deltaXt, deltaXb, dXs, dXm1, XInit, oldDX;
+    CALMOptions calm;
+    bool calmSet = false, haveOldDX = false;
+    double calmDeltaL = -1.0, reachedLambda = 0.0;
+    int calmMode = 0, calmOldMode = 0, calmTick = -1;   // 0 modified, 1 accelerated, 2 full NRM
+    bool applyInitialGuess(int &linearIterations, bool &tangentCurrent);
     double currentTime = 0.0;   // target time of the last accepted step of solveTimeSteps
     int kMode = -1;             // stiffness the values of K hold: 0 elastic, 1 tangent, -1 none
     long smConstraintVersion = -1;   // K version after the last ddm penalty
@@ -178,6 +209,13 @@ public:
     std::vector<StepRecord> solveTimeSteps(const double *up_ref, const double *f_free, const double *f_presc,
                                            const std::vector<double> &ltf_t, const std::vector<double> &ltf_v,
                                            double finalTime, double dt, const ReductionStrategy &rs);
+    // NonLinearStatic with indirect control (controlmode 0): one step of
+    // CylindricalALM::solve, calmls.C:108-481, on the reference load set by
+    // setReferenceLoad; the load level persists between steps
+    void setArcLength(const CALMOptions &o);
+    void setReferenceLoad(const double *f_free, const double *f_presc);
+    CALMStepReport solveArcLengthStep();
+    double giveLoadLevel() const { return reachedLambda; }
     void updateYourself();   // commit Gauss-point state (EngngModel::updateYourself)
 
     void giveSolution(double *u) const { X.download(u); }

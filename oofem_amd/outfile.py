@@ -57,6 +57,9 @@ def write_step(f, prob, step, t, res, seconds=0.0):
     f.write(f"\nOutput for time {t:.8e} ")
     f.write(f"\n{_RULE}\n")
     f.write(f"Output for domain {1:3d}\n")
+    if "load_level" in res:
+        # NonLinearStatic::printOutputAt nlinearstatic.C:674-688
+        f.write(f"Reached load level : {res['load_level']:20.6f} in {int(res.get('iterations', 0))} iterations\n")
     f.write("\n\nDofManager output:\n------------------\n")
     u = res["u_node"]
     for i in range(d.nnode):
@@ -104,6 +107,9 @@ def read_out(text):
             cur = dict(time=float(line.split()[3]), u={}, reactions={}, strains={}, stresses={})
             steps.append(cur)
             continue
+        if line.startswith("Reached load level") and cur is not None:
+            cur["load_level"] = float(line.split(":")[1].split()[0])
+            continue
         if cur is None:
             continue
         m = _NODE.match(line)
@@ -149,6 +155,8 @@ def check_out(prob, steps):
             v = s["u"][(c.number, c.dof)]
         elif c.kind == "reaction":
             v = s["reactions"].get((c.number, c.dof), 0.0)
+        elif c.kind == "loadlevel":
+            v = s.get("load_level", float("nan"))
         else:
             arr = s["stresses"] if c.keyword == 1 else s["strains"]
             v = arr[(c.number, c.gp)][c.component - 1]

@@ -45,6 +45,11 @@ def hostlib():
         L.hfem_problem_set_ddm.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
         L.hfem_problem_set_ddm_factor.argtypes = [C.c_void_p, C.c_double]
         L.hfem_problem_set_nr_extras.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]
+        L.hfem_problem_set_arc_length.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_double, C.c_double,
+                                                  C.c_double, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                                  C.c_int]
+        L.hfem_problem_set_reference_load.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.hfem_problem_arc_length_step.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.hfem_problem_run_steps.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
                                              C.c_void_p, C.c_double, C.c_double, C.c_int, C.c_double, C.c_double,
                                              C.c_double, C.c_int, C.c_int, C.c_int, C.c_int, C.c_void_p, C.c_void_p,
@@ -163,6 +168,32 @@ class StructuralProblem:
         err = C.c_double(0)
         _check(hostlib().hfem_problem_nonlinear_step(self.h, out, C.byref(err)), "nonlinear_step")
         return dict(newton=out[0], pcg=out[1], status=out[2], force_error=err.value)
+
+    def set_arc_length(self, steplength, psi=1.0, maxiter=30, maxrestarts=4, minsteplength=0.0,
+                       initialsteplength=0.0, reqiterations=3, miniter=0, manrmsteps=0, rtolf=1e-3, rtold=1e-3,
+                       rootselectiontype=0):
+        """NonLinearStatic indirect control (controlmode 0): CylindricalALM
+        options under the deck's keyword names (calmls.h:51-75)."""
+        _check(hostlib().hfem_problem_set_arc_length(self.h, float(psi), int(maxiter), int(maxrestarts),
+                                                     float(minsteplength), float(steplength),
+                                                     float(initialsteplength), int(reqiterations), int(miniter),
+                                                     int(manrmsteps), float(rtolf), float(rtold),
+                                                     int(rootselectiontype)), "set_arc_length")
+
+    def set_reference_load(self, f_free, f_presc=None):
+        f = np.ascontiguousarray(f_free, dtype=np.float64)
+        fp = np.ascontiguousarray(f_presc if f_presc is not None else np.zeros(self.deck.npeq), dtype=np.float64)
+        _check(hostlib().hfem_problem_set_reference_load(self.h, _p(f), _p(fp) if fp.size else None),
+               "set_reference_load")
+
+    def arc_length_step(self):
+        """One CylindricalALM step: iterations, PCG iterations, restarts,
+        status, the load level reached and the next arc length."""
+        out = (C.c_int * 4)()
+        outd = (C.c_double * 4)()
+        _check(hostlib().hfem_problem_arc_length_step(self.h, out, outd), "arc_length_step")
+        return dict(iterations=out[0], pcg=out[1], restarts=out[2], status=out[3], load_level=outd[0],
+                    step_length=outd[1], force_error=outd[2], disp_error=outd[3])
 
     def update(self):
         _check(hostlib().hfem_problem_update(self.h), "update")

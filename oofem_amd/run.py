@@ -11,10 +11,16 @@ The engineering-model record chooses the driver:
   step per time step on the total loads of that time
   (NonLinearStatic::proceedStep, nlinearstatic.C:453-567; refloadmode 0),
   no initial guess unless ``initialguess 1``, and the Newton matrix from
-  ``stiffmode`` (0 tangent, 2 initial elastic).  Arc-length control
-  (controlmode 0, CylindricalALM), incremental reference loads
-  (refloadmode 1) and the secant stiffness are refused.  Solver keywords may
-  sit on the first metastep record.
+  ``stiffmode`` (0 tangent, 2 initial elastic);
+- ``NonLinearStatic`` with indirect control (``controlmode 0``, the
+  reference's default): one CylindricalALM step per time step on the
+  reference load of the first step (calmls.C:108-481), keywords psi,
+  steplength, initialsteplength, minsteplength, maxiter, maxrestarts,
+  reqiterations, miniter, manrmsteps, rtolv / rtolf / rtold,
+  rootselectiontype; hyperplane control, the CALM line search and dof-group
+  norms are refused.  Incremental reference loads (refloadmode 1) and the
+  secant stiffness are refused.  Solver keywords may sit on the first
+  metastep record.
 NRSolver's direct displacement control (``ddm`` node/dof pairs, ``ddv``
 values, ``ddltf`` function) is applied in both nonlinear drivers.
 
@@ -80,21 +86,46 @@ def run_deck(path, out_path=None, keywords=None, log=None):
     if model == "staticstructural":
         kw.update(rtolf=kwd("rtolf", 1e-3), maxiter=kwd("maxiter", 100000000, int),
                   manrmsteps=kwd("manrmsteps", 0, int), initial_guess=kwd("initialguess", 1, int))
-    elif model == "nonlinearstatic":
-        if kwd("controlmode", 0, int) != 1:
-            raise ValueError("NonLinearStatic arc-length control (controlmode 0, CylindricalALM) is not part of this "
-                             "build; use controlmode 1")
+    calm = None
+    if model == "nonlinearstatic":
+        # "controllmode" is the reference's older spelling (nlinearstatic.C:106)
+        cmode = kwd("controlmode", kwd("controllmode", 0, int), int)
+        if cmode not in (0, 1):
+            raise ValueError(f"unsupported NonLinearStatic controlmode {cmode}")
         if kwd("refloadmode", 0, int) != 0:
             raise ValueError("NonLinearStatic incremental reference loads (refloadmode 1) are not part of this build")
-        kw.update(rtolf=kwd("rtolf", 1e-3), maxiter=kwd("maxiter", 100000000, int),
-                  manrmsteps=kwd("manrmsteps", 0, int), initial_guess=kwd("initialguess", 0, int),
-                  stiff_mode=kwd("stiffmode", 0, int))
+        kw.update(initial_guess=kwd("initialguess", 0, int), stiff_mode=kwd("stiffmode", 0, int))
+        if cmode == 1:
+            kw.update(rtolf=kwd("rtolf", 1e-3), maxiter=kwd("maxiter", 100000000, int),
+                      manrmsteps=kwd("manrmsteps", 0, int))
+        else:
+            # CylindricalALM::initializeFrom calmls.C:681-897
+            for key in ("hpcmode", "hpc", "hpcw", "nccdg"):
+                if key in over or (ms is not None and ms.has(key)) or e.has(key):
+                    raise ValueError(f"CylindricalALM '{key}' (hyperplane control / dof-group norms) is not part "
+                                     "of this build")
+            if kwd("linesearch", 0, int):
+                raise ValueError("the CylindricalALM line search is not part of this build")
+            if not (("steplength" in over) or (ms is not None and ms.has("steplength")) or e.has("steplength")):
+                raise ValueError("CylindricalALM needs steplength")
+            rtol = kwd("rtolv", 1e-3)
+            calm = dict(steplength=kwd("steplength", 0.0), psi=kwd("psi", 1.0), maxiter=kwd("maxiter", 30, int),
+                        maxrestarts=kwd("maxrestarts", 4, int), minsteplength=kwd("minsteplength", 0.0),
+                        initialsteplength=kwd("initialsteplength", 0.0), reqiterations=kwd("reqiterations", 3, int),
+                        miniter=kwd("miniter", 0, int), manrmsteps=kwd("manrmsteps", 0, int),
+                        rtolf=kwd("rtolf", rtol), rtold=kwd("rtold", rtol),
+                        rootselectiontype=kwd("rootselectiontype", 0, int))
     d = prob.deck
     P = StructuralProblem(d, **kw)
+    if calm is not None:
+        P.set_arc_length(**calm)
     # NRSolver direct displacement control: ddm = (node label, dof id) pairs,
     # ddv = values, ddltf = load-time function of the values
     ddm = None
     if model != "linearstatic" and (ms is not None and ms.has("ddm") or e.has("ddm")):
+        if calm is not None:
+            P.close()
+            raise ValueError("direct displacement control (ddm) belongs to NRSolver, not to CylindricalALM")
         rec = ms if ms is not None and ms.has("ddm") else e
         pairs = rec.array("ddm", int)
         vals = rec.array("ddv")
@@ -117,21 +148,37 @@ def run_deck(path, out_path=None, keywords=None, log=None):
             t0 = time.perf_counter()
             up, (ff, fp) = prob.step_loading(k)
             P.set_prescribed(up)
-            P.set_load(ff, fp)
+            if calm is None:
+                P.set_load(ff, fp)
+            elif k == 1:
+                # indirect control assembles the reference load once
+                # (nlinearstatic.C:496-507, loadInitFlag): the total loads of the first step
+                P.set_reference_load(ff, fp)
             if ddm is not None:
                 # NRSolver::applyConstraintsToLoadIncrement: the increment of
                 # the function over the step (its value at the first step)
                 t = prob.time_of_step(k)
                 P.set_ddm_factor(ddm(t) - (ddm(prob.time_of_step(k - 1)) if k > 1 else 0.0))
+            level = None
             if model == "linearstatic":
                 it, _ = P.linear_static()
                 iters[k] = (0, it)
+            elif calm is not None:
+                rep = P.arc_length_step()
+                iters[k] = (rep["iterations"], rep["pcg"])
+                level = rep["load_level"]
+                if rep["status"] != 1 and log:
+                    # the reference warns and accepts the step (calmls.C:418-424)
+                    log(f"[deck] step {k}: CylindricalALM did not converge after {rep['restarts']} restarts")
             else:
                 rep = P.nonlinear_step()
                 iters[k] = (rep["newton"], rep["pcg"])
                 if rep["status"] != 1:
                     raise RuntimeError(f"step {k}: Newton loop did not converge: {rep}")
             results[k] = _collect(prob, P, P.solution(), up)
+            if level is not None:
+                results[k]["load_level"] = level
+                results[k]["iterations"] = iters[k][0]
             if model != "linearstatic":
                 P.update()
             if f:

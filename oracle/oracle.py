@@ -365,3 +365,155 @@ def linear_static(deck, tol=1e-10, maxit=100000, u_presc=None):
                                  up.ctypes.data_as(C.c_void_p), C.c_double(tol), maxit,
                                  u.ctypes.data_as(C.c_void_p), R.ctypes.data_as(C.c_void_p), C.byref(it), C.byref(res))
     return dict(u=u, reactions=R, iters=it.value, resid=res.value, rc=rc)
+
+
+class ArcLength:
+    """CylindricalALM (calmls.C) restated over the oracle's assembly, internal
+    forces and CG: hyperplane control off, no CALM line search, norms over
+    all equations, no initial load vector.  Options as the deck's keywords
+    (initializeFrom, calmls.C:681-897, defaults of the constructor 60-101).
+    The arc length, the reached load level, the Newton mode and the previous
+    increment persist between steps as the reference's members do."""
+
+    def __init__(self, steplength, psi=1.0, maxiter=30, maxrestarts=4, minsteplength=0.0, initialsteplength=0.0,
+                 reqiterations=3, miniter=0, manrmsteps=0, rtolf=1e-3, rtold=1e-3, rootselectiontype=0,
+                 lstol=1e-12, stiffmode=0):
+        if not steplength > 0.0:
+            raise ValueError("steplength must be positive")
+        self.psi = 1.0 if psi < 0.0 else psi
+        self.nsmax = max(int(maxiter), 30)
+        self.maxrestarts = int(maxrestarts)
+        self.minsl, self.maxsl = float(minsteplength), float(steplength)
+        self.deltaL = initialsteplength if initialsteplength > 0.0 else steplength
+        self.reqit = min(max(int(reqiterations), 3), 1000)
+        self.miniter = int(miniter)
+        self.manrm = int(manrmsteps)
+        self.rtolf, self.rtold = float(rtolf), float(rtold)
+        self.rst = int(rootselectiontype)
+        self.lstol = lstol
+        self.stiffmode = stiffmode
+        self.mode = self.old_mode = 1 if self.manrm > 0 else 0   # 0 modified, 1 accelerated, 2 full NRM
+        self.tick = -1
+        self.reached = 0.0
+        self.old_dX = None
+
+    def step(self, deck, state, X, up, R):
+        """One step of CylindricalALM::solve (calmls.C:108-481); X (free
+        total displacements) and the temp Gauss-point state are updated in
+        place.  Returns dict(iterations, restarts, load_level, step_length,
+        force_error, disp_error, converged, pcg)."""
+        if self.tick == 0:
+            self.mode = self.old_mode
+        if self.tick > 0:
+            self.tick -= 1
+        n = deck.neq
+        XInit = X.copy()
+        deltaXt = np.zeros(n)
+        deltaXb = np.zeros(n)
+        RR = float(R @ R)
+        psi2 = self.psi * self.psi
+        pcg_total = [0]
+
+        def matrix():
+            if self.stiffmode == 2:
+                return csr(deck)
+            return csr(deck, state, tangent=True)
+
+        def solve(K, b, x0):
+            x, it, res, rc = pcg(*K, b, x0=x0, tol=self.lstol, maxit=100000)
+            if rc:
+                raise RuntimeError(f"orc_pcg rc={rc}")
+            pcg_total[0] += it
+            return x
+
+        irest = 0
+        converged = True
+        while True:
+            dX = np.zeros(n)
+            K = matrix()
+            deltaXt = solve(K, R, deltaXt)
+            p = np.sqrt(deltaXt @ deltaXt + psi2 * RR)
+            XR = deltaXt @ R if (self.rst == 0 or self.old_dX is None) else deltaXt @ self.old_dX
+            DL = dl = (-1.0 if XR < 0.0 else 1.0) * self.deltaL / p
+            Lam = self.reached + DL
+            dX = solve(K, DL * R, dX)
+            X[:] = XInit + dX
+            nite = 0
+            F, _, ebe = internal_forces(deck, X, up, state=state, update_state=True)
+            restart = False
+            while True:
+                nite += 1
+                dXm1 = dX.copy()
+                DLm1 = DL
+                if self.mode == 2 or (self.mode == 1 and nite % self.manrm == 0):
+                    K = matrix()
+                    deltaXt = solve(K, R, deltaXt)
+                deltaXb = solve(K, Lam * R - F, deltaXb)
+                # computeDeltaLambda calmls.C:995-1146 (eta = 1)
+                XX, XXt, X_Xt = deltaXt @ deltaXt, dX @ deltaXt, deltaXb @ deltaXt
+                dXdX, dXX_, X_X_ = dX @ dX, dX @ deltaXb, deltaXb @ deltaXb
+                a1 = XX + psi2 * RR
+                a2 = RR * psi2 * DL * 2.0 + 2.0 * XXt + 2.0 * X_Xt
+                a3 = dXdX + 2.0 * dXX_ + X_X_ + DL * DL * RR * psi2 - self.deltaL * self.deltaL
+                discr = a2 * a2 - 4.0 * a1 * a3
+                if discr < 0.0:
+                    dl = 0.0
+                else:
+                    sq = np.sqrt(discr)
+                    lam1, lam2 = (-a2 + sq) / 2.0 / a1, (-a2 - sq) / 2.0 / a1
+                    if self.rst == 0:
+                        a4, a5 = dXX_ + dXdX, XXt
+                        cos1 = (a4 + a5 * lam1) / self.deltaL / self.deltaL
+                        cos2 = (a4 + a5 * lam2) / self.deltaL / self.deltaL
+                        dl = lam1 if cos1 > cos2 else lam2
+                    else:
+                        s = DL + dl
+                        c1 = (dXdX + dXX_ + lam1 * XXt) + psi2 * (s * s + s * lam1)
+                        c2 = (dXdX + dXX_ + lam2 * XXt) + psi2 * (s * s + s * lam2)
+                        dl = lam1 if c1 > c2 else lam2
+                ddX = dl * deltaXt + deltaXb
+                dX = dXm1 + ddX
+                X[:] = XInit + dX
+                dr = ddX @ ddX
+                DL = DLm1 + dl
+                Lam = self.reached + DL
+                F, _, ebe = internal_forces(deck, X, up, state=state, update_state=True)
+                # checkConvergence calmls.C:484-677 (all equations)
+                r = Lam * R - F
+                fe = r @ r
+                dXX = X @ X
+                en = ebe.sum()
+                if RR * Lam * Lam > 1e-6:
+                    fe = np.sqrt(fe / (RR * Lam * Lam))
+                elif en > 1e-6:
+                    fe = np.sqrt(fe / en)
+                else:
+                    fe = np.sqrt(fe)
+                de = dr if dXX < 1e-6 else np.sqrt(dr / dXX)
+                out = abs(fe) > self.rtolf * 1e10 or abs(de) > self.rtold * 1e10 or np.isnan(fe) or np.isnan(de)
+                conv = not (abs(fe) > self.rtolf or abs(de) > self.rtold)
+                if nite >= self.nsmax or out:
+                    irest += 1
+                    if irest <= self.maxrestarts:
+                        self.deltaL = max(self.deltaL * 0.25, self.minsl)
+                        X[:] = XInit
+                        state_restart(deck, state)
+                        self.old_mode, self.mode, self.tick = self.mode, 2, 2
+                        restart = True
+                        break
+                    converged = False
+                    break
+                if conv and nite >= self.miniter:
+                    break
+            if not restart:
+                break
+        if nite > self.reqit:
+            self.deltaL = self.deltaL * self.reqit / nite
+        else:
+            self.deltaL = self.deltaL * np.sqrt(np.sqrt(self.reqit / nite))
+        self.deltaL = min(self.deltaL, self.maxsl)
+        self.deltaL = max(self.deltaL, self.minsl)
+        self.reached = Lam
+        self.old_dX = dX.copy()
+        return dict(iterations=nite, restarts=irest, load_level=Lam, step_length=self.deltaL, force_error=fe,
+                    disp_error=de, converged=converged, pcg=pcg_total[0])

@@ -320,3 +320,119 @@ def test_direct_displacement_control_matches_oracle():
     kref = st.reshape(-1, 38)[:, 13]
     assert np.abs(kap - kref).max() <= 1e-8 * kref.max()
     P.close()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["full_newton", "modified_restart", "psi0"])
+def test_arc_length_steps_match_oracle(case):
+    """NonLinearStatic indirect control (CylindricalALM) through the C++
+    driver on the GPU against the oracle's restatement, step by step, on the
+    traction-loaded MisesMat block through yield: the same corrector
+    iteration counts and restarts, load levels to 1e-10 and displacements to
+    1e-9; the reactions balance the reached load level times the reference
+    load."""
+    from oracle import oracle
+    from oofem_amd.problem import StructuralProblem
+    from tests import nlcases
+    d, top, R, Rp = nlcases.traction_block()
+    opts = dict(full_newton=dict(steplength=20.0, manrmsteps=1, rtolf=1e-10, rtold=1e-10),
+                modified_restart=dict(steplength=20.0, rtolf=1e-8, rtold=1e-8),
+                psi0=dict(steplength=2e-3, psi=0.0, manrmsteps=1, rtolf=1e-10, rtold=1e-10))[case]
+    al = oracle.ArcLength(lstol=1e-13, **opts)
+    st = oracle.state_new(d)
+    Xo = np.zeros(d.neq)
+    P = StructuralProblem(d, lstol=1e-13, initial_guess=0)
+    try:
+        P.set_arc_length(**opts)
+        P.set_reference_load(R, Rp)
+        P.set_prescribed(np.zeros(d.npeq))
+        restarts = 0
+        for k in range(7):
+            ro = al.step(d, st, Xo, np.zeros(d.npeq), R)
+            oracle.state_commit(d, st)
+            rg = P.arc_length_step()
+            P.update()
+            assert rg["status"] == 1 and ro["converged"]
+            assert (rg["iterations"], rg["restarts"]) == (ro["iterations"], ro["restarts"]), (k, rg, ro)
+            assert abs(rg["load_level"] - ro["load_level"]) <= 1e-10 * abs(ro["load_level"])
+            assert abs(rg["step_length"] - ro["step_length"]) <= 1e-10 * ro["step_length"]
+            u = P.solution()
+            assert np.abs(u - Xo).max() <= 1e-9 * np.abs(Xo).max(), k
+            reac = P.reactions()
+            # bottom reactions carry the load: sum of z reactions = -lambda (unit traction on a unit face)
+            bz = d.eqn[:, 2] < 0
+            assert abs(reac[-d.eqn[bz, 2] - 1].sum() + rg["load_level"]) <= 1e-8 * rg["load_level"]
+            restarts += rg["restarts"]
+        if case == "modified_restart":
+            assert restarts >= 1
+    finally:
+        P.close()
+
+
+@pytest.mark.gpu
+def test_arc_length_deck_end_to_end(tmp_path):
+    """patch301 (linear LTRSpace, surface load) as a NonLinearStatic deck
+    with indirect control (controlmode 0, CylindricalALM keywords on the
+    metastep): every #LOADLEVEL and #REACTION check holds, their values being
+    the closed form of a linear arc-length run (lambda_k = sum of deltaL_j / p,
+    deltaL growing by 3^(1/4) per one-iteration step up to steplength, the
+    reactions lambda_k times the exact 1.5), and the .out file carries the
+    reached load level of every step."""
+    import os
+
+    from oofem_amd.deck import read_deck
+    from oofem_amd.outfile import read_out
+    from oofem_amd.run import run_deck
+    from tests.conftest import GOLDEN
+    src = open(os.path.join(GOLDEN, "patch301.in")).read().splitlines()
+    i = next(k for k, l in enumerate(src) if l.lower().startswith("staticstructural"))
+    src[i] = "NonLinearStatic nsteps 1 nmsteps 1 nmodules 1"
+    src.insert(i + 1, "nsteps 4 controlmode 0 psi 1.0 steplength 0.5 initialsteplength 0.2 rtolv 1e-10")
+    j = next(k for k, l in enumerate(src) if l.startswith("#%BEGIN_CHECK%"))
+    base = tmp_path / "base.in"
+    base.write_text("\n".join(src[:j]) + "\n")
+    prob = read_deck(str(base))
+    d = prob.deck
+    _, (f, _) = prob.step_loading(1)
+    rp, ci, val = oracle.csr(d)
+    KR, _, _, _ = oracle.pcg(rp, ci, val, f, tol=1e-14)
+    p = np.sqrt(KR @ KR + f @ f)
+    lam, dl, checks = 0.0, 0.2, ["#%BEGIN_CHECK% tolerance 1.e-8"]
+    levels = []
+    for k in range(1, 5):
+        lam += dl / p
+        dl = min(dl * 3.0 ** 0.25, 0.5)
+        levels.append(lam)
+        checks.append(f"#LOADLEVEL tStep {k} value {lam:.16e}")
+        checks.append(f"#REACTION tStep {k} number 1 dof 3 value {1.5 * lam:.16e}")
+    checks.append("#%END_CHECK%")
+    path = tmp_path / "calm.in"
+    path.write_text("\n".join(src[:j] + checks) + "\n")
+    out = tmp_path / "calm.out"
+    r = run_deck(str(path), out_path=str(out), keywords=dict(smtype=11, lstype=9, lstol=1e-13))
+    assert len(r["checks"]) == 8
+    bad = [(c.line, v) for c, v, ok in r["checks"] if not ok]
+    assert not bad, bad
+    assert all(r["iterations"][k][0] == 1 for k in range(1, 5))
+    steps = read_out(out.read_text())
+    assert [round(s["load_level"], 6) for s in steps] == [round(v, 6) for v in levels]
+
+
+@pytest.mark.gpu
+def test_arc_length_deck_refuses_what_is_not_built(tmp_path):
+    """Hyperplane control, the CALM line search, ddm with indirect control
+    and a missing steplength are refused before any solve."""
+    import os
+
+    from oofem_amd.run import run_deck
+    from tests.conftest import GOLDEN
+    src = open(os.path.join(GOLDEN, "patch301.in")).read().splitlines()
+    i = next(k for k, l in enumerate(src) if l.lower().startswith("staticstructural"))
+    for extra, msg in (("steplength 0.5 hpcmode 1 hpc 2 4 3", "hpcmode"), ("steplength 0.5 linesearch 1", "line search"),
+                       ("psi 1.0", "steplength")):
+        s = list(src)
+        s[i] = f"NonLinearStatic nsteps 2 controlmode 0 {extra} nmodules 1"
+        p = tmp_path / "bad.in"
+        p.write_text("\n".join(s) + "\n")
+        with pytest.raises(ValueError, match=msg):
+            run_deck(str(p), keywords=dict(smtype=11, lstype=9))

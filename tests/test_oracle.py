@@ -563,7 +563,7 @@ def test_elastic_stiffness_newton_reaches_the_tangent_root():
 
 def test_nonlinear_static_deck_keywords(tmp_path):
     """A NonLinearStatic deck with one metastep record: the solver keywords
-    and step count come from the metastep; arc-length control is refused."""
+    and step count come from the metastep."""
     from oofem_amd.deck import read_deck as rd
     src = open(os.path.join(GOLDEN, "patch301.in")).read().splitlines()
     i = next(k for k, l in enumerate(src) if l.lower().startswith("staticstructural"))
@@ -697,3 +697,112 @@ def test_body_load_resultant_and_moment(etype):
     h, hp = meshgen.body_load_vector(dj, sel, dens[sel], b, factor=0.7)
     scale = np.abs(np.r_[g, gp]).max()
     assert np.abs(g - h).max() < 1e-13 * scale and np.abs(gp - hp).max() < 1e-13 * scale
+
+
+# ------------------------------------------------------------ arc length
+def _arc_block(material=None):
+    from tests import nlcases
+    return nlcases.traction_block(material=material)
+
+
+def test_arc_length_elastic_load_levels_closed_form():
+    """CylindricalALM on a linear problem: every step converges in one
+    corrector iteration with no load change, so the load level grows by
+    deltaL_k / p with p = sqrt(|K^-1 R|^2 + psi^2 |R|^2), and the arc length
+    of the next step is deltaL_k (reqiter / 1)^(1/4), capped by steplength
+    (calmls.C:221-243, 450-470).  The displacement is lambda K^-1 R, the
+    uniaxial closed form lambda / E on the loaded face."""
+    from tests import nlcases
+    iso = np.array([[meshgen.MAT_ISOLE, nlcases.E, nlcases.NU, 0, 0, 0, 0, 0]])
+    d, top, R, Rp = _arc_block(iso)
+    rp, ci, val = oracle.csr(d)
+    KR, _, _, _ = oracle.pcg(rp, ci, val, R, tol=1e-14)
+    for psi in (1.0, 0.5):
+        p = np.sqrt(KR @ KR + psi * psi * (R @ R))
+        al = oracle.ArcLength(steplength=20.0, initialsteplength=5.0, psi=psi, rtolf=1e-10, rtold=1e-10,
+                              lstol=1e-14)
+        st = oracle.state_new(d)
+        X = np.zeros(d.neq)
+        dl, lam = 5.0, 0.0
+        for k in range(8):
+            r = al.step(d, st, X, np.zeros(d.npeq), R)
+            oracle.state_commit(d, st)
+            lam += dl / p
+            dl = min(dl * 3.0 ** 0.25, 20.0)
+            assert r["iterations"] == 1 and r["restarts"] == 0
+            assert abs(r["load_level"] - lam) <= 1e-12 * lam
+            assert abs(r["step_length"] - dl) <= 1e-12 * dl
+            uz = X[d.eqn[top, 2] - 1]
+            assert np.abs(uz - lam / nlcases.E).max() <= 1e-9 * lam / nlcases.E
+
+
+def test_arc_length_plastic_path_follows_the_uniaxial_curve():
+    """Full Newton inside the arc-length steps (manrmsteps 1) through yield:
+    every converged point lies on the closed-form curve
+    eps = sigma / E + (sigma - sig0) / H at sigma = lambda (unit traction),
+    and the step's increments satisfy the cylindrical constraint
+    |dX|^2 + psi^2 dLambda^2 |R|^2 = deltaL^2 (calmls.C:1012-1034)."""
+    from tests import nlcases
+    d, top, R, Rp = _arc_block()
+    st = oracle.state_new(d)
+    X = np.zeros(d.neq)
+    al = oracle.ArcLength(steplength=20.0, manrmsteps=1, rtolf=1e-10, rtold=1e-10, lstol=1e-14)
+    RR = R @ R
+    lam_prev = 0.0
+    for k in range(8):
+        X0 = X.copy()
+        dl = al.deltaL
+        r = al.step(d, st, X, np.zeros(d.npeq), R)
+        oracle.state_commit(d, st)
+        lam = r["load_level"]
+        uz = X[d.eqn[top, 2] - 1]
+        eps = nlcases.uniaxial_strain(lam)
+        assert np.abs(uz - eps).max() <= 1e-8 * eps, (k, uz, eps)
+        dX = X - X0
+        # the corrector that converged left a residual of the order of rtol
+        assert abs(dX @ dX + (lam - lam_prev) ** 2 * RR - dl * dl) <= 1e-8 * dl * dl
+        lam_prev = lam
+    assert lam_prev > 1.6 * nlcases.SIG0
+
+
+def test_arc_length_restart_quarters_the_arc_and_switches_to_full_newton():
+    """Modified Newton (the default) cannot converge a step that enters the
+    plastic range within maxiter (30) iterations: the step restarts from its
+    start with a quarter of the arc length and full Newton for this and the
+    next two steps (calmls.C:384-412, CALM_RESET_STEP_REDUCE 0.25,
+    CALM_DEFAULT_NRM_TICKS 2); the converged point still satisfies the
+    constraint with the reduced arc and lies on the closed-form curve.  With
+    psi = 0 the control is on |dX| alone."""
+    from tests import nlcases
+    d, top, R, Rp = _arc_block()
+    st = oracle.state_new(d)
+    X = np.zeros(d.neq)
+    al = oracle.ArcLength(steplength=20.0, rtolf=1e-8, rtold=1e-8, lstol=1e-14)
+    RR = R @ R
+    restarted = None
+    lam_prev = 0.0
+    for k in range(7):
+        X0 = X.copy()
+        dl = al.deltaL
+        r = al.step(d, st, X, np.zeros(d.npeq), R)
+        oracle.state_commit(d, st)
+        lam = r["load_level"]
+        if r["restarts"]:
+            restarted = k
+            dX = X - X0
+            assert abs(dX @ dX + (lam - lam_prev) ** 2 * RR - (0.25 * dl) ** 2) <= 1e-6 * (0.25 * dl) ** 2
+            assert al.mode == 2 and al.tick == 2
+        uz = X[d.eqn[top, 2] - 1]
+        assert np.abs(uz - nlcases.uniaxial_strain(lam)).max() <= 1e-6 * nlcases.uniaxial_strain(lam)
+        lam_prev = lam
+    assert restarted == 4
+    assert al.mode == 2 and al.tick == 0      # the full-Newton ticks ran out; the next step returns to modified
+    # displacement control: psi = 0
+    st = oracle.state_new(d)
+    X = np.zeros(d.neq)
+    al = oracle.ArcLength(steplength=2e-3, psi=0.0, manrmsteps=1, rtolf=1e-10, rtold=1e-10, lstol=1e-14)
+    for k in range(5):
+        X0 = X.copy()
+        al.step(d, st, X, np.zeros(d.npeq), R)
+        oracle.state_commit(d, st)
+        assert abs(np.linalg.norm(X - X0) - 2e-3) <= 1e-9 * 2e-3

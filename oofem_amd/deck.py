@@ -161,6 +161,12 @@ class DeckProblem:
         up = np.zeros(d.npeq)
         m = d.eqn < 0
         up[-d.eqn[m] - 1] = val[m]
+        return up, self.loads_at(t, body_load=body_load)
+
+    def loads_at(self, t, body_load=None):
+        """(f_free, f_presc): every load of the deck with its load-time
+        function evaluated at time t."""
+        d = self.deck
         f_free = np.zeros(d.neq)
         f_presc = np.zeros(d.npeq)
         for ld in self.loads:
@@ -177,7 +183,7 @@ class DeckProblem:
             ff, fp = bl_fn(d, el, self.density[d.emat[el]], bl["b"], fac)
             f_free += ff
             f_presc += fp
-        return up, (f_free, f_presc)
+        return f_free, f_presc
 
 
 def _records(path):

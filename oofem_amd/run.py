@@ -11,16 +11,19 @@ The engineering-model record chooses the driver:
   step per time step on the total loads of that time
   (NonLinearStatic::proceedStep, nlinearstatic.C:453-567; refloadmode 0),
   no initial guess unless ``initialguess 1``, and the Newton matrix from
-  ``stiffmode`` (0 tangent, 2 initial elastic);
+  ``stiffmode`` (0 tangent, 2 initial elastic); the step's load is its
+  reference load (total loads at its time, refloadmode 0, or their
+  increment over the step, refloadmode 1) plus the sum of the earlier
+  steps' reference loads, as NonLinearStatic::updateLoadVectors keeps it;
 - ``NonLinearStatic`` with indirect control (``controlmode 0``, the
   reference's default): one CylindricalALM step per time step on the
   reference load of the first step (calmls.C:108-481), keywords psi,
   steplength, initialsteplength, minsteplength, maxiter, maxrestarts,
   reqiterations, miniter, manrmsteps, rtolv / rtolf / rtold,
   rootselectiontype; hyperplane control, the CALM line search and dof-group
-  norms are refused.  Incremental reference loads (refloadmode 1) and the
-  secant stiffness are refused.  Solver keywords may sit on the first
-  metastep record.
+  norms are refused, and so are incremental reference loads with it and
+  the secant stiffness.  Solver keywords may sit on the first metastep
+  record.
 NRSolver's direct displacement control (``ddm`` node/dof pairs, ``ddv``
 values, ``ddltf`` function) is applied in both nonlinear drivers.
 
@@ -92,8 +95,12 @@ def run_deck(path, out_path=None, keywords=None, log=None):
         cmode = kwd("controlmode", kwd("controllmode", 0, int), int)
         if cmode not in (0, 1):
             raise ValueError(f"unsupported NonLinearStatic controlmode {cmode}")
-        if kwd("refloadmode", 0, int) != 0:
-            raise ValueError("NonLinearStatic incremental reference loads (refloadmode 1) are not part of this build")
+        refload = kwd("refloadmode", 0, int)
+        if refload not in (0, 1):
+            raise ValueError(f"unknown NonLinearStatic refloadmode {refload}")
+        if refload == 1 and cmode == 0:
+            raise ValueError("incremental reference loads (refloadmode 1) with arc-length control are not part "
+                             "of this build")
         kw.update(initial_guess=kwd("initialguess", 0, int), stiff_mode=kwd("stiffmode", 0, int))
         if cmode == 1:
             kw.update(rtolf=kwd("rtolf", 1e-3), maxiter=kwd("maxiter", 100000000, int),
@@ -139,6 +146,7 @@ def run_deck(path, out_path=None, keywords=None, log=None):
             raise ValueError("ddm names a dof that is prescribed by a boundary condition")
         P.set_ddm(eqs, vals)
         ddm = prob.ltf[rec.scalar("ddltf", cast=int)]
+    R0 = (np.zeros(d.neq), np.zeros(d.npeq))   # NonLinearStatic's initialLoadVector (free, prescribed)
     results, iters = {}, {}
     f = open(out_path, "w") if out_path else None
     try:
@@ -148,17 +156,34 @@ def run_deck(path, out_path=None, keywords=None, log=None):
             t0 = time.perf_counter()
             up, (ff, fp) = prob.step_loading(k)
             P.set_prescribed(up)
-            if calm is None:
+            if model == "nonlinearstatic" and calm is None:
+                # direct control (nlinearstatic.C:396-430, 496-507, 552-556):
+                # the reference load R of the step is the total loads at its
+                # time (refloadmode 0) or their increment over the step
+                # (refloadmode 1, Function::evaluate VM_Incremental), and
+                # the solver's total load is R + R0, where R0 gathers
+                # loadLevel (1 under NRSolver) times every earlier R
+                if refload == 1:
+                    f0, p0 = prob.loads_at(prob.time_of_step(k - 1))   # t - dt
+                    ff, fp = ff - f0, fp - p0
+                P.set_load(ff + R0[0], fp + R0[1])
+                R0 = (R0[0] + ff, R0[1] + fp)
+            elif calm is None:
                 P.set_load(ff, fp)
             elif k == 1:
                 # indirect control assembles the reference load once
                 # (nlinearstatic.C:496-507, loadInitFlag): the total loads of the first step
                 P.set_reference_load(ff, fp)
             if ddm is not None:
-                # NRSolver::applyConstraintsToLoadIncrement: the increment of
-                # the function over the step (its value at the first step)
+                # NRSolver::applyConstraintsToLoadIncrement nrsolver.C:512-520:
+                # with total reference loads the increment of the function
+                # over the step (its value at the first step), with
+                # incremental ones (NonLinearStatic refloadmode 1) its value
                 t = prob.time_of_step(k)
-                P.set_ddm_factor(ddm(t) - (ddm(prob.time_of_step(k - 1)) if k > 1 else 0.0))
+                if model == "nonlinearstatic" and refload == 1:
+                    P.set_ddm_factor(ddm(t))
+                else:
+                    P.set_ddm_factor(ddm(t) - (ddm(prob.time_of_step(k - 1)) if k > 1 else 0.0))
             level = None
             if model == "linearstatic":
                 it, _ = P.linear_static()

@@ -282,7 +282,8 @@ def test_run_nonlinear_static_deck(tmp_path):
     r = run_deck(str(p), keywords=dict(smtype=11, lstype=9, lstol=1e-12))
     bad = [(c.line, v) for c, v, ok in r["checks"] if not ok]
     assert r["checks"] and not bad, bad
-    with pytest.raises(ValueError):
+    with pytest.raises(ValueError, match="steplength"):
+        # indirect control needs its arc length
         src[i + 1] = "nsteps 1 controlmode 0"
         p.write_text("\n".join(src) + "\n")
         run_deck(str(p), keywords=dict(smtype=11, lstype=9))
@@ -436,3 +437,36 @@ def test_arc_length_deck_refuses_what_is_not_built(tmp_path):
         p.write_text("\n".join(s) + "\n")
         with pytest.raises(ValueError, match=msg):
             run_deck(str(p), keywords=dict(smtype=11, lstype=9))
+
+
+@pytest.mark.gpu
+def test_nonlinear_static_direct_control_accumulates_reference_loads(tmp_path):
+    """NonLinearStatic direct control keeps every step's reference load in
+    its initial load vector (updateLoadVectors, nlinearstatic.C:419-430,
+    loadLevel 1 under NRSolver): patch301 (linear, constant surface load)
+    over three steps carries 1, 2 and 3 times the load with refloadmode 0,
+    so the displacements and reactions are 1, 2 and 3 times the exact
+    one-step solution; with refloadmode 1 the constant function has no
+    increment (Function::evaluate VM_Incremental) and nothing moves."""
+    import os
+
+    from oofem_amd.run import run_deck
+    from tests.conftest import GOLDEN
+    src = open(os.path.join(GOLDEN, "patch301.in")).read().splitlines()
+    i = next(k for k, l in enumerate(src) if l.lower().startswith("staticstructural"))
+    j = next(k for k, l in enumerate(src) if l.startswith("#%BEGIN_CHECK%"))
+    out = {}
+    for mode in (0, 1):
+        s = list(src[:j])
+        s[i] = f"NonLinearStatic nsteps 3 controlmode 1 refloadmode {mode} rtolf 1e-10 nmodules 1"
+        p = tmp_path / f"nls{mode}.in"
+        p.write_text("\n".join(s) + "\n")
+        out[mode] = run_deck(str(p), keywords=dict(smtype=11, lstype=9, lstol=1e-13))
+    r0 = out[0]["results"]
+    u1 = r0[1]["u_node"]
+    assert abs(r0[1]["reactions"][(0, 3)] - 1.5) <= 1e-9
+    for k in (2, 3):
+        assert np.abs(r0[k]["u_node"] - k * u1).max() <= 1e-9 * np.abs(u1).max()
+        assert abs(r0[k]["reactions"][(0, 3)] - 1.5 * k) <= 1e-9 * k
+    for k in (1, 2, 3):
+        assert np.abs(out[1]["results"][k]["u_node"]).max() == 0.0

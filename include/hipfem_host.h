@@ -58,9 +58,11 @@ int hfem_problem_set_nr_extras(hfem_problem *p, int linesearch, double ls_tol, d
  * step) */
 int hfem_problem_set_ddm(hfem_problem *p, int n, const int *eqs, const double *values);
 int hfem_problem_set_ddm_factor(hfem_problem *p, double factor);
-/* NonLinearStatic `stiffmode` (nlinearstatic.h:62-64): 0 = the tangent
- * stiffness for every Newton matrix, 2 = the initial elastic stiffness,
- * assembled once and kept; 1 (secant) is refused */
+/* NonLinearStatic `stiffmode` (nlinearstatic.h:61-65): 0 = the tangent
+ * stiffness for every Newton matrix, 1 = the secant stiffness for every
+ * one, 2 = the initial elastic stiffness, assembled once and kept, 3 = the
+ * secant stiffness assembled once.  IsoLE and MisesMat answer the secant
+ * request with the elastic matrix (misesmat.C:496-504). */
 int hfem_problem_set_stiffness_mode(hfem_problem *p, int mode);
 /* StaticStructural time stepping from the current time to final_time
  * (EngngModel::solveYourself engngm.C:573-662): prescribed values and

@@ -186,8 +186,7 @@ int hfem_problem_set_ddm_factor(hfem_problem *p, double factor)
 int hfem_problem_set_stiffness_mode(hfem_problem *p, int mode)
 {
     return guarded([&] {
-        if (mode == 1) OOFEM_ERROR("secant stiffness (stiffmode 1) is not part of this build");
-        if (mode != 0 && mode != 2) OOFEM_ERROR("unknown stiffness mode %d", mode);
+        if (mode < 0 || mode > 3) OOFEM_ERROR("unknown stiffness mode %d", mode);
         reinterpret_cast<StructuralProblem *>(p)->nr.stiffMode = mode;
     });
 }

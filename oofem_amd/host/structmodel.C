@@ -171,10 +171,16 @@ ConvergedReason StructuralProblem::solveLinearStatic(int &iterations, double &re
 
 // the Newton matrix: the tangent, or the elastic stiffness assembled once
 // (NonLinearStatic::updateComponent NonLinearLhs, nlinearstatic.C:616-645)
+// NonLinearStatic::updateComponent(NonLinearLhs) nlinearstatic.C:615-650:
+// 0 tangent every time; 1 secant every time; 2 elastic once; 3 secant
+// once.  IsoLE and MisesMat answer SecantStiffness with the elastic matrix
+// (misesmat.C:496-504), so 1 reassembles the elastic matrix and 3 is 2.
 void StructuralProblem::assembleNewtonMatrix()
 {
-    if (nr.stiffMode == 2) {
+    if (nr.stiffMode == 2 || nr.stiffMode == 3) {
         if (kMode != 0) assemble(*K, 0);
+    } else if (nr.stiffMode == 1) {
+        assemble(*K, 0);
     } else {
         assemble(*K, 1);
     }

@@ -74,7 +74,8 @@ class StructuralProblem:
         """forcing > 0: inexact Newton, corrections solved to forcing terms
         of at most this value (hfem_problem_set_forcing); 0: every
         correction to lstol, as the reference's NRSolver.  stiff_mode:
-        NonLinearStatic's stiffmode (0 tangent, 2 initial elastic)."""
+        NonLinearStatic's stiffmode (0 tangent, 1 secant, 2 initial elastic,
+        3 initial secant)."""
         self.deck = d = deck
         self._keep = [np.ascontiguousarray(x) for x in (d.coords, d.conn, d.emat, d.mat, d.eqn)]
         c, cn, em, mt, eq = self._keep

@@ -11,7 +11,8 @@ The engineering-model record chooses the driver:
   step per time step on the total loads of that time
   (NonLinearStatic::proceedStep, nlinearstatic.C:453-567; refloadmode 0),
   no initial guess unless ``initialguess 1``, and the Newton matrix from
-  ``stiffmode`` (0 tangent, 2 initial elastic); the step's load is its
+  ``stiffmode`` (0 tangent, 1 secant, 2 initial elastic, 3 initial
+  secant); the step's load is its
   reference load (total loads at its time, refloadmode 0, or their
   increment over the step, refloadmode 1) plus the sum of the earlier
   steps' reference loads, as NonLinearStatic::updateLoadVectors keeps it;
@@ -21,9 +22,8 @@ The engineering-model record chooses the driver:
   steplength, initialsteplength, minsteplength, maxiter, maxrestarts,
   reqiterations, miniter, manrmsteps, rtolv / rtolf / rtold,
   rootselectiontype; hyperplane control, the CALM line search and dof-group
-  norms are refused, and so are incremental reference loads with it and
-  the secant stiffness.  Solver keywords may sit on the first metastep
-  record.
+  norms are refused, and so are incremental reference loads with it.
+  Solver keywords may sit on the first metastep record.
 NRSolver's direct displacement control (``ddm`` node/dof pairs, ``ddv``
 values, ``ddltf`` function) is applied in both nonlinear drivers.
 

@@ -245,9 +245,13 @@ def test_run_deck_default_solver_is_rejected():
         run_deck(os.path.join(GOLDEN, "patch301.in"))
 
 
-def test_elastic_stiffness_newton_matches_oracle():
+@pytest.mark.parametrize("mode", [2, 1, 3])
+def test_elastic_stiffness_newton_matches_oracle(mode):
     """NonLinearStatic stiffmode 2 on the GPU driver (the elastic matrix is
-    assembled once and kept): the oracle's Newton iteration count and root."""
+    assembled once and kept): the oracle's Newton iteration count and root.
+    The secant modes 1 and 3 reach the same iterates, because IsoLE and
+    MisesMat answer SecantStiffness with the elastic matrix
+    (misesmat.C:496-504)."""
     from oofem_amd.problem import StructuralProblem
     d = meshgen.config3(4, 4, 16)
     d.mat = d.mat.copy()
@@ -257,7 +261,7 @@ def test_elastic_stiffness_newton_matches_oracle():
     X = np.zeros(d.neq)
     ref = oracle.nonlinear_step_ex(d, st, X, up, initial_guess=False, maxiter=400, lstol=1e-12, stiffmode=2)
     assert ref["status"] == 1
-    P = StructuralProblem(d, lstol=1e-12, rtolf=1e-8, maxiter=400, manrmsteps=1, initial_guess=0, stiff_mode=2)
+    P = StructuralProblem(d, lstol=1e-12, rtolf=1e-8, maxiter=400, manrmsteps=1, initial_guess=0, stiff_mode=mode)
     P.set_load(np.zeros(d.neq), np.zeros(d.npeq))
     P.set_prescribed(up)
     rep = P.nonlinear_step()

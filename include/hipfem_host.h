@@ -95,12 +95,20 @@ int hfem_problem_nonlinear_step(hfem_problem *p, int *out3, double *force_error)
  * (calmls.C:60-101, 681-897) with its keywords psi, maxiter, maxrestarts,
  * minsteplength, steplength, initialsteplength (<= 0: steplength),
  * reqiterations, miniter, manrmsteps, rtolf, rtold, rootselectiontype (0
- * cosine, 1 dot); hyperplane control, the CALM line search and dof-group
+ * cosine, 1 dot); hyperplane control through
+ * hfem_problem_set_arc_length_hpc; the CALM line search and dof-group
  * norms are not part of this build.  Called again, it keeps the arc length
  * reached unless that exceeds the new steplength. */
 int hfem_problem_set_arc_length(hfem_problem *p, double psi, int maxiter, int maxrestarts, double minsteplength,
                                 double steplength, double initialsteplength, int reqiterations, int miniterations,
                                 int manrmsteps, double rtolf, double rtold, int rootselectiontype);
+/* CylindricalALM hyperplane control (hpcmode, hpc, hpcw; calmls.C:754-797,
+ * 900-957): 1 = the constraint over the controlled equations only, 2 = the
+ * linearized control sum_i w_i dX_i = steplength; eqs are the controlled
+ * dofs' free equations (1-based) in the domain's dof-manager order, 0 for a
+ * prescribed dof (skipped, as there); 0 switches the control off.  Call it
+ * after hfem_problem_set_arc_length. */
+int hfem_problem_set_arc_length_hpc(hfem_problem *p, int hpcmode, int n, const int *eqs, const double *weights);
 /* the reference load R (NonLinearStatic::assembleIncrementalReferenceLoadVectors,
  * refloadmode 0: the total loads of the first step), free and prescribed parts */
 int hfem_problem_set_reference_load(hfem_problem *p, const double *f_free, const double *f_presc);

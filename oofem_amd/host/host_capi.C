@@ -128,6 +128,17 @@ int hfem_problem_set_arc_length(hfem_problem *p, double psi, int maxiter, int ma
     });
 }
 
+int hfem_problem_set_arc_length_hpc(hfem_problem *p, int hpcmode, int n, const int *eqs, const double *weights)
+{
+    return guarded([&] {
+        if (n < 0 || (n > 0 && !eqs) || (hpcmode == 2 && n > 0 && !weights)) OOFEM_ERROR("hpc: bad arguments");
+        std::vector<int> e(eqs, eqs + n);
+        std::vector<double> w;
+        if (hpcmode == 2) w.assign(weights, weights + n);
+        reinterpret_cast<StructuralProblem *>(p)->setArcLengthHpc(hpcmode, e, w);
+    });
+}
+
 int hfem_problem_set_reference_load(hfem_problem *p, const double *f_free, const double *f_presc)
 {
     return guarded([&] {

@@ -497,6 +497,11 @@ void StructuralProblem::setArcLength(const CALMOptions &o)
     if (!(o.maxStepLength > 0.0)) OOFEM_ERROR("calm: steplength must be positive, got %g", o.maxStepLength);
     if (o.manrmsteps < 0 || o.maxRestarts < 0 || o.minIterations < 0) OOFEM_ERROR("calm: negative count");
     if (o.rootSelection != 0 && o.rootSelection != 1) OOFEM_ERROR("calm: rootselectiontype must be 0 or 1");
+    if (o.hpcMode < 0 || o.hpcMode > 2) OOFEM_ERROR("calm: hpcmode must be 0, 1 or 2");
+    if (o.hpcMode && o.hpcEq.empty()) OOFEM_ERROR("HPC Map must be specified");
+    if (o.hpcMode == 2 && o.hpcW.size() != o.hpcEq.size()) OOFEM_ERROR("HPC map size and weight array size mismatch");
+    for (int q : o.hpcEq)
+        if (q < 0 || q > neq) OOFEM_ERROR("calm: hpc equation %d out of range", q);
     calm = o;
     if (calm.psi < 0.0) calm.psi = 1.0;
     calm.maxiter = std::max(calm.maxiter, 30);
@@ -510,6 +515,22 @@ void StructuralProblem::setArcLength(const CALMOptions &o)
         Rpref.allocate(ctx, npeq);
     }
     calmSet = true;
+}
+
+void StructuralProblem::setArcLengthHpc(int mode, const std::vector<int> &eqs, const std::vector<double> &w)
+{
+    if (!calmSet) OOFEM_ERROR("arc-length options must be set before the hyperplane control");
+    CALMOptions o = calm;
+    o.hpcMode = mode;
+    o.hpcEq = eqs;
+    o.hpcW = w;
+    if (mode == 0) {
+        o.hpcEq.clear();
+        o.hpcW.clear();
+    }
+    const double keep = calmDeltaL;
+    setArcLength(o);
+    calmDeltaL = keep;
 }
 
 void StructuralProblem::setReferenceLoad(const double *f_free, const double *f_presc)
@@ -547,6 +568,21 @@ CALMStepReport StructuralProblem::solveArcLengthStep()
     deltaXb.zero();
     const double RR = Rref.dot(Rref);
     const double psi2 = calm.psi * calm.psi;
+    // hyperplane control: the controlled free equations (prescribed ones,
+    // equation 0, are skipped as calmls.C does)
+    std::vector<int> hpcOn;
+    std::vector<double> hpcWOn, hX, hD, hB, hR;
+    for (size_t i = 0; i < calm.hpcEq.size(); i++)
+        if (calm.hpcEq[i] > 0) {
+            hpcOn.push_back(calm.hpcEq[i]);
+            hpcWOn.push_back(calm.hpcMode == 2 ? calm.hpcW[i] : 1.0);
+        }
+    auto gatherHpc = [&](const DeviceVector &v, std::vector<double> &out) {
+        out.resize(hpcOn.size());
+        for (size_t i = 0; i < hpcOn.size(); i++)
+            hipfemCheck(hipfem_dev_download(ctx, &out[i], v.ptr() + hpcOn[i] - 1, 1), "hpc entry");
+    };
+    if (calm.hpcMode) gatherHpc(Rref, hR);
     double Lambda = reachedLambda, DeltaLambda = 0.0, deltaLambda = 0.0;
     int irest = 0, nite = 0;
     double ebe[3];
@@ -555,7 +591,22 @@ CALMStepReport StructuralProblem::solveArcLengthStep()
         dXs.zero();
         assembleNewtonMatrix();
         if (!solve(Rref, deltaXt)) { rep.status = CR_FAILED; return rep; }
-        const double p = std::sqrt(deltaXt.dot(deltaXt) + psi2 * RR);
+        double p;
+        if (calm.hpcMode == 0) {
+            p = std::sqrt(deltaXt.dot(deltaXt) + psi2 * RR);
+        } else if (calm.hpcMode == 1) {
+            gatherHpc(deltaXt, hX);
+            double xx = 0.0, rr = 0.0;
+            for (size_t i = 0; i < hpcOn.size(); i++) {
+                xx += hX[i] * hX[i];
+                rr += hR[i] * hR[i];
+            }
+            p = std::sqrt(xx + psi2 * rr);
+        } else {
+            gatherHpc(deltaXt, hX);
+            p = 0.0;
+            for (size_t i = 0; i < hpcOn.size(); i++) p += hX[i] * hpcWOn[i];
+        }
         Lambda = reachedLambda;
         const double XR = (calm.rootSelection == 0 || !haveOldDX) ? deltaXt.dot(Rref) : deltaXt.dot(oldDX);
         DeltaLambda = deltaLambda = (XR < 0.0 ? -1.0 : 1.0) * calmDeltaL / p;
@@ -579,7 +630,56 @@ CALMStepReport StructuralProblem::solveArcLengthStep()
             rhs.axpby(-1.0, &F, 1.0);
             if (!solve(rhs, deltaXb)) { rep.status = CR_FAILED; return rep; }
             // B.3: the cylindrical constraint |dX + ddX|^2 + psi^2 RR (DeltaLambda + dl)^2 = deltaL^2
-            {
+            if (calm.hpcMode == 2) {
+                // linearized control over the weighted hpc equations (calmls.C:1123-1143)
+                gatherHpc(dXs, hD);
+                gatherHpc(deltaXb, hB);
+                gatherHpc(deltaXt, hX);
+                double nom = 0.0, denom = 0.0;
+                for (size_t i = 0; i < hpcOn.size(); i++) {
+                    nom += (hD[i] + hB[i]) * hpcWOn[i];
+                    denom += hX[i] * hpcWOn[i];
+                }
+                if (std::fabs(denom) < 1.e-20) OOFEM_ERROR("zero denominator in linearized control");
+                deltaLambda = (calmDeltaL - nom) / denom;
+            } else if (calm.hpcMode == 1) {
+                // the constraint over the hpc equations only (calmls.C:1036-1058, 1082-1099)
+                gatherHpc(dXs, hD);
+                gatherHpc(deltaXb, hB);
+                gatherHpc(deltaXt, hX);
+                double rr = 0.0, RRh = 0.0, b2 = 0.0, b3 = 0.0, a4 = 0.0, a5 = 0.0;
+                for (size_t i = 0; i < hpcOn.size(); i++) {
+                    const double pr = hD[i] + hB[i];
+                    rr += hX[i] * hX[i];
+                    RRh += hR[i] * hR[i];
+                    b2 += hX[i] * pr;
+                    b3 += pr * pr;
+                    a4 += hD[i] * hB[i] + hD[i] * hD[i];
+                    a5 += hD[i] * hX[i];
+                }
+                const double a1 = rr + psi2 * RRh;
+                const double a2 = RRh * psi2 * DeltaLambda * 2.0 + 2.0 * b2;
+                const double a3 = b3 - calmDeltaL * calmDeltaL + DeltaLambda * DeltaLambda * RRh * psi2;
+                double discr = a2 * a2 - 4.0 * a1 * a3;
+                if (discr < 0.0) {
+                    deltaLambda = 0.0;
+                } else {
+                    discr = std::sqrt(discr);
+                    const double lam1 = (-a2 + discr) / 2. / a1, lam2 = (-a2 - discr) / 2. / a1;
+                    if (calm.rootSelection == 0) {
+                        const double cos1 = (a4 + a5 * lam1) / calmDeltaL / calmDeltaL;
+                        const double cos2 = (a4 + a5 * lam2) / calmDeltaL / calmDeltaL;
+                        deltaLambda = cos1 > cos2 ? lam1 : lam2;
+                    } else {
+                        // RST_Dot takes the full vectors, as calmls.C:1104-1118 does
+                        const double XXt = dXs.dot(deltaXt), dXdX = dXs.dot(dXs), dXX_ = dXs.dot(deltaXb);
+                        const double s = DeltaLambda + deltaLambda;
+                        const double c1 = (dXdX + dXX_ + lam1 * XXt) + (psi2 * (s * s + s * lam1));
+                        const double c2 = (dXdX + dXX_ + lam2 * XXt) + (psi2 * (s * s + s * lam2));
+                        deltaLambda = c1 > c2 ? lam1 : lam2;
+                    }
+                }
+            } else {
                 const double XX = deltaXt.dot(deltaXt), XXt = dXs.dot(deltaXt), X_Xt = deltaXb.dot(deltaXt);
                 const double dXdX = dXs.dot(dXs), dXX_ = dXs.dot(deltaXb), X_X_ = deltaXb.dot(deltaXb);
                 const double a1 = XX + psi2 * RR;

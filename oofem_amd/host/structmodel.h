@@ -118,6 +118,9 @@ struct CALMOptions {
     int manrmsteps = 0;            // > 0: tangent every manrmsteps iterations (accelerated NRM)
     double rtolf = 1.e-3, rtold = 1.e-3;
     int rootSelection = 0;         // 0 cosine of the increments (RST_Cos), 1 dot product (RST_Dot)
+    int hpcMode = 0;               // 0 off, 1 constraint on the hpc equations only, 2 linearized (weighted)
+    std::vector<int> hpcEq;        // controlled free equations (1-based; 0 = prescribed, skipped)
+    std::vector<double> hpcW;      // weights of the linearized control
 };
 
 struct CALMStepReport {
@@ -214,6 +217,9 @@ public:
     // setReferenceLoad; the load level persists between steps
     void setArcLength(const CALMOptions &o);
     void setReferenceLoad(const double *f_free, const double *f_presc);
+    // hyperplane control (hpcmode 1 or 2; 0 switches it off): controlled
+    // free equations, 0 for a prescribed dof, and the weights of mode 2
+    void setArcLengthHpc(int mode, const std::vector<int> &eqs, const std::vector<double> &w);
     CALMStepReport solveArcLengthStep();
     double giveLoadLevel() const { return reachedLambda; }
     void updateYourself();   // commit Gauss-point state (EngngModel::updateYourself)

@@ -49,6 +49,7 @@ def hostlib():
                                                   C.c_double, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
                                                   C.c_int]
         L.hfem_problem_set_reference_load.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.hfem_problem_set_arc_length_hpc.argtypes = [C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_void_p]
         L.hfem_problem_arc_length_step.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
         L.hfem_problem_run_steps.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_int, C.c_void_p,
                                              C.c_void_p, C.c_double, C.c_double, C.c_int, C.c_double, C.c_double,
@@ -180,6 +181,15 @@ class StructuralProblem:
                                                      float(initialsteplength), int(reqiterations), int(miniter),
                                                      int(manrmsteps), float(rtolf), float(rtold),
                                                      int(rootselectiontype)), "set_arc_length")
+
+    def set_arc_length_hpc(self, hpcmode, eqs, weights=None):
+        """CylindricalALM hyperplane control: hpcmode 1 (constraint over the
+        controlled free equations eqs, 1-based, 0 = prescribed) or 2
+        (linearized, sum of weights times the increments = steplength)."""
+        e = np.ascontiguousarray(eqs, dtype=np.int32)
+        w = np.ascontiguousarray(weights if weights is not None else np.zeros(len(e)), dtype=np.float64)
+        _check(hostlib().hfem_problem_set_arc_length_hpc(self.h, int(hpcmode), len(e), _p(e) if len(e) else None,
+                                                         _p(w) if len(w) else None), "set_arc_length_hpc")
 
     def set_reference_load(self, f_free, f_presc=None):
         f = np.ascontiguousarray(f_free, dtype=np.float64)

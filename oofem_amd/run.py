@@ -21,8 +21,9 @@ The engineering-model record chooses the driver:
   reference load of the first step (calmls.C:108-481), keywords psi,
   steplength, initialsteplength, minsteplength, maxiter, maxrestarts,
   reqiterations, miniter, manrmsteps, rtolv / rtolf / rtold,
-  rootselectiontype; hyperplane control, the CALM line search and dof-group
-  norms are refused, and so are incremental reference loads with it.
+  rootselectiontype, and the hyperplane control hpcmode / hpc / hpcw; the
+  CALM line search and dof-group norms are refused, and so are incremental
+  reference loads with it.
   Solver keywords may sit on the first metastep record.
 NRSolver's direct displacement control (``ddm`` node/dof pairs, ``ddv``
 values, ``ddltf`` function) is applied in both nonlinear drivers.
@@ -107,10 +108,8 @@ def run_deck(path, out_path=None, keywords=None, log=None):
                       manrmsteps=kwd("manrmsteps", 0, int))
         else:
             # CylindricalALM::initializeFrom calmls.C:681-897
-            for key in ("hpcmode", "hpc", "hpcw", "nccdg"):
-                if key in over or (ms is not None and ms.has(key)) or e.has(key):
-                    raise ValueError(f"CylindricalALM '{key}' (hyperplane control / dof-group norms) is not part "
-                                     "of this build")
+            if "nccdg" in over or (ms is not None and ms.has("nccdg")) or e.has("nccdg"):
+                raise ValueError("CylindricalALM dof-group norms (nccdg) are not part of this build")
             if kwd("linesearch", 0, int):
                 raise ValueError("the CylindricalALM line search is not part of this build")
             if not (("steplength" in over) or (ms is not None and ms.has("steplength")) or e.has("steplength")):
@@ -126,6 +125,32 @@ def run_deck(path, out_path=None, keywords=None, log=None):
     P = StructuralProblem(d, **kw)
     if calm is not None:
         P.set_arc_length(**calm)
+        # hyperplane control (calmls.C:754-797): the hpc array holds (node
+        # label, dof id) pairs; given without hpcmode it means mode 1
+        rec = ms if ms is not None and ms.has("hpc") else e
+        hmode = kwd("hpcmode", 0, int)
+        if rec.has("hpc"):
+            pairs = rec.array("hpc", int)
+            if len(pairs) % 2:
+                P.close()
+                raise ValueError("HPC Map size must be even number, it contains pairs <node, nodeDof>")
+            hmode = 2 if hmode == 2 else 1
+            wsrc = (ms if ms is not None and ms.has("hpcw") else e).array("hpcw", float, default=[])
+            if hmode == 2 and len(wsrc) != len(pairs) // 2:
+                P.close()
+                raise ValueError("HPC map size and weight array size mismatch")
+            # convertHPCMap calmls.C:900-957: the dof-manager order of the domain
+            eqs, ws = [], []
+            for n, label in enumerate(prob.node_numbers):
+                for k in range(len(pairs) // 2):
+                    if pairs[2 * k] == int(label):
+                        q = int(d.eqn[n, pairs[2 * k + 1] - 1])
+                        eqs.append(q if q > 0 else 0)
+                        ws.append(wsrc[k] if hmode == 2 else 1.0)
+            P.set_arc_length_hpc(hmode, eqs, ws if hmode == 2 else None)
+        elif hmode:
+            P.close()
+            raise ValueError("HPC Map must be specified")
     # NRSolver direct displacement control: ddm = (node label, dof id) pairs,
     # ddv = values, ddltf = load-time function of the values
     ddm = None

@@ -377,7 +377,7 @@ class ArcLength:
 
     def __init__(self, steplength, psi=1.0, maxiter=30, maxrestarts=4, minsteplength=0.0, initialsteplength=0.0,
                  reqiterations=3, miniter=0, manrmsteps=0, rtolf=1e-3, rtold=1e-3, rootselectiontype=0,
-                 lstol=1e-12, stiffmode=0):
+                 lstol=1e-12, stiffmode=0, hpcmode=0, hpc_eqs=(), hpc_weights=()):
         if not steplength > 0.0:
             raise ValueError("steplength must be positive")
         self.psi = 1.0 if psi < 0.0 else psi
@@ -396,6 +396,12 @@ class ArcLength:
         self.tick = -1
         self.reached = 0.0
         self.old_dX = None
+        # hyperplane control (calmls.C:754-797, 900-957): controlled free
+        # equations (1-based; 0 = prescribed, skipped), weights of mode 2
+        self.hpc = int(hpcmode)
+        keep = [i for i, q in enumerate(hpc_eqs) if q > 0]
+        self.hpc_idx = np.array([hpc_eqs[i] - 1 for i in keep], dtype=np.int64)
+        self.hpc_w = np.array([hpc_weights[i] for i in keep]) if self.hpc == 2 else np.ones(len(keep))
 
     def step(self, deck, state, X, up, R):
         """One step of CylindricalALM::solve (calmls.C:108-481); X (free
@@ -432,7 +438,13 @@ class ArcLength:
             dX = np.zeros(n)
             K = matrix()
             deltaXt = solve(K, R, deltaXt)
-            p = np.sqrt(deltaXt @ deltaXt + psi2 * RR)
+            m = self.hpc_idx
+            if self.hpc == 1:
+                p = np.sqrt(deltaXt[m] @ deltaXt[m] + psi2 * (R[m] @ R[m]))
+            elif self.hpc == 2:
+                p = deltaXt[m] @ self.hpc_w
+            else:
+                p = np.sqrt(deltaXt @ deltaXt + psi2 * RR)
             XR = deltaXt @ R if (self.rst == 0 or self.old_dX is None) else deltaXt @ self.old_dX
             DL = dl = (-1.0 if XR < 0.0 else 1.0) * self.deltaL / p
             Lam = self.reached + DL
@@ -455,14 +467,27 @@ class ArcLength:
                 a1 = XX + psi2 * RR
                 a2 = RR * psi2 * DL * 2.0 + 2.0 * XXt + 2.0 * X_Xt
                 a3 = dXdX + 2.0 * dXX_ + X_X_ + DL * DL * RR * psi2 - self.deltaL * self.deltaL
+                a4, a5 = dXX_ + dXdX, XXt
+                if self.hpc == 1:
+                    pr = dX[m] + deltaXb[m]
+                    Rm = R[m] @ R[m]
+                    a1 = deltaXt[m] @ deltaXt[m] + psi2 * Rm
+                    a2 = Rm * psi2 * DL * 2.0 + 2.0 * (deltaXt[m] @ pr)
+                    a3 = pr @ pr - self.deltaL * self.deltaL + DL * DL * Rm * psi2
+                    a4 = dX[m] @ deltaXb[m] + dX[m] @ dX[m]
+                    a5 = dX[m] @ deltaXt[m]
                 discr = a2 * a2 - 4.0 * a1 * a3
-                if discr < 0.0:
+                if self.hpc == 2:
+                    denom = deltaXt[m] @ self.hpc_w
+                    if abs(denom) < 1e-20:
+                        raise RuntimeError("zero denominator in linearized control")
+                    dl = (self.deltaL - (dX[m] + deltaXb[m]) @ self.hpc_w) / denom
+                elif discr < 0.0:
                     dl = 0.0
                 else:
                     sq = np.sqrt(discr)
                     lam1, lam2 = (-a2 + sq) / 2.0 / a1, (-a2 - sq) / 2.0 / a1
                     if self.rst == 0:
-                        a4, a5 = dXX_ + dXdX, XXt
                         cos1 = (a4 + a5 * lam1) / self.deltaL / self.deltaL
                         cos2 = (a4 + a5 * lam2) / self.deltaL / self.deltaL
                         dl = lam1 if cos1 > cos2 else lam2

@@ -328,7 +328,7 @@ def test_direct_displacement_control_matches_oracle():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("case", ["full_newton", "modified_restart", "psi0"])
+@pytest.mark.parametrize("case", ["full_newton", "modified_restart", "psi0", "hpc1", "hpc2"])
 def test_arc_length_steps_match_oracle(case):
     """NonLinearStatic indirect control (CylindricalALM) through the C++
     driver on the GPU against the oracle's restatement, step by step, on the
@@ -342,13 +342,20 @@ def test_arc_length_steps_match_oracle(case):
     d, top, R, Rp = nlcases.traction_block()
     opts = dict(full_newton=dict(steplength=20.0, manrmsteps=1, rtolf=1e-10, rtold=1e-10),
                 modified_restart=dict(steplength=20.0, rtolf=1e-8, rtold=1e-8),
-                psi0=dict(steplength=2e-3, psi=0.0, manrmsteps=1, rtolf=1e-10, rtold=1e-10))[case]
-    al = oracle.ArcLength(lstol=1e-13, **opts)
+                psi0=dict(steplength=2e-3, psi=0.0, manrmsteps=1, rtolf=1e-10, rtold=1e-10),
+                hpc1=dict(steplength=4e-3, psi=0.0, manrmsteps=1, rtolf=1e-10, rtold=1e-10),
+                hpc2=dict(steplength=4e-3, manrmsteps=1, rtolf=1e-10, rtold=1e-10))[case]
+    qz = [int(q) for q in d.eqn[top, 2]]
+    hpc = dict(hpc1=(1, qz, None), hpc2=(2, qz[:1] + [0], [1.0, 0.5])).get(case)
+    okw = dict(hpcmode=hpc[0], hpc_eqs=hpc[1], hpc_weights=hpc[2] or ()) if hpc else {}
+    al = oracle.ArcLength(lstol=1e-13, **opts, **okw)
     st = oracle.state_new(d)
     Xo = np.zeros(d.neq)
     P = StructuralProblem(d, lstol=1e-13, initial_guess=0)
     try:
         P.set_arc_length(**opts)
+        if hpc:
+            P.set_arc_length_hpc(*hpc)
         P.set_reference_load(R, Rp)
         P.set_prescribed(np.zeros(d.npeq))
         restarts = 0
@@ -425,16 +432,18 @@ def test_arc_length_deck_end_to_end(tmp_path):
 
 @pytest.mark.gpu
 def test_arc_length_deck_refuses_what_is_not_built(tmp_path):
-    """Hyperplane control, the CALM line search, ddm with indirect control
-    and a missing steplength are refused before any solve."""
+    """Dof-group norms, the CALM line search, a missing steplength, hpcmode
+    without its map and the linearized control without weights are refused
+    before any solve."""
     import os
 
     from oofem_amd.run import run_deck
     from tests.conftest import GOLDEN
     src = open(os.path.join(GOLDEN, "patch301.in")).read().splitlines()
     i = next(k for k, l in enumerate(src) if l.lower().startswith("staticstructural"))
-    for extra, msg in (("steplength 0.5 hpcmode 1 hpc 2 4 3", "hpcmode"), ("steplength 0.5 linesearch 1", "line search"),
-                       ("psi 1.0", "steplength")):
+    for extra, msg in (("steplength 0.5 nccdg 1", "nccdg"), ("steplength 0.5 linesearch 1", "line search"),
+                       ("psi 1.0", "steplength"), ("steplength 0.5 hpcmode 2", "HPC Map"),
+                       ("steplength 0.5 hpcmode 2 hpc 2 4 3", "weight")):
         s = list(src)
         s[i] = f"NonLinearStatic nsteps 2 controlmode 0 {extra} nmodules 1"
         p = tmp_path / "bad.in"
@@ -474,3 +483,37 @@ def test_nonlinear_static_direct_control_accumulates_reference_loads(tmp_path):
         assert abs(r0[k]["reactions"][(0, 3)] - 1.5 * k) <= 1e-9 * k
     for k in (1, 2, 3):
         assert np.abs(out[1]["results"][k]["u_node"]).max() == 0.0
+
+
+@pytest.mark.gpu
+def test_arc_length_deck_with_linearized_hyperplane_control(tmp_path):
+    """patch301 under indirect control with hpcmode 2 on node 4's z dof
+    (hpc 2 4 3, hpcw 1 1.0): every step moves that dof by +steplength (the
+    weight's direction), so on this linear problem the load level after k
+    steps is k steplength over that dof's displacement u4 under the
+    reference load; u4 < 0 here, so the load runs backwards."""
+    import os
+
+    from oofem_amd.deck import read_deck
+    from oofem_amd.run import run_deck
+    from tests.conftest import GOLDEN
+    src = open(os.path.join(GOLDEN, "patch301.in")).read().splitlines()
+    i = next(k for k, l in enumerate(src) if l.lower().startswith("staticstructural"))
+    j = next(k for k, l in enumerate(src) if l.startswith("#%BEGIN_CHECK%"))
+    s = list(src[:j])
+    s[i] = ("NonLinearStatic nsteps 3 controlmode 0 steplength 0.4 hpcmode 2 hpc 2 4 3 hpcw 1 1.0 "
+            "rtolv 1e-10 nmodules 1")
+    p = tmp_path / "hpc.in"
+    p.write_text("\n".join(s) + "\n")
+    prob = read_deck(str(p))
+    d = prob.deck
+    _, (f, _) = prob.step_loading(1)
+    rp, ci, val = oracle.csr(d)
+    KR, _, _, _ = oracle.pcg(rp, ci, val, f, tol=1e-14)
+    n4 = list(prob.node_numbers).index(4)
+    u4 = KR[d.eqn[n4, 2] - 1]
+    r = run_deck(str(p), keywords=dict(smtype=11, lstype=9, lstol=1e-13))
+    assert u4 < 0.0
+    for k in (1, 2, 3):
+        assert abs(r["results"][k]["u_node"][n4, 2] - 0.4 * k) <= 1e-9
+        assert abs(r["results"][k]["load_level"] - 0.4 * k / u4) <= 1e-9 * 0.4 * k / abs(u4)

@@ -806,3 +806,34 @@ def test_arc_length_restart_quarters_the_arc_and_switches_to_full_newton():
         al.step(d, st, X, np.zeros(d.npeq), R)
         oracle.state_commit(d, st)
         assert abs(np.linalg.norm(X - X0) - 2e-3) <= 1e-9 * 2e-3
+
+
+def test_arc_length_hyperplane_control():
+    """Hyperplane control (calmls.C:1036-1058, 1123-1143): hpcmode 2 with a
+    unit weight on one loaded-face dof makes every step move that dof by
+    exactly the arc length (the linearized constraint is solved exactly in
+    each corrector), hpcmode 1 with psi = 0 keeps the norm of the controlled
+    increments equal to the arc length; both stay on the closed-form
+    uniaxial curve through yield."""
+    from tests import nlcases
+    d, top, R, Rp = _arc_block()
+    qz = d.eqn[top, 2]
+    for mode, kw in ((2, dict(hpc_eqs=[int(qz[0])], hpc_weights=[1.0])),
+                     (1, dict(hpc_eqs=[int(q) for q in qz], psi=0.0))):
+        st = oracle.state_new(d)
+        X = np.zeros(d.neq)
+        al = oracle.ArcLength(steplength=4e-3, initialsteplength=4e-3, manrmsteps=1, rtolf=1e-10, rtold=1e-10,
+                              lstol=1e-14, hpcmode=mode, **kw)
+        for k in range(5):
+            X0 = X.copy()
+            dl = al.deltaL
+            r = al.step(d, st, X, np.zeros(d.npeq), R)
+            oracle.state_commit(d, st)
+            inc = (X - X0)[qz - 1]
+            if mode == 2:
+                assert abs(inc[0] - dl) <= 1e-12 * dl
+            else:
+                assert abs(np.linalg.norm(inc) - dl) <= 1e-9 * dl
+            eps = nlcases.uniaxial_strain(r["load_level"])
+            assert np.abs(X[qz - 1] - eps).max() <= 1e-8 * eps
+        assert r["load_level"] > nlcases.SIG0

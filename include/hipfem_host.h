@@ -58,6 +58,12 @@ int hfem_problem_set_nr_extras(hfem_problem *p, int linesearch, double ls_tol, d
  * step) */
 int hfem_problem_set_ddm(hfem_problem *p, int n, const int *eqs, const double *values);
 int hfem_problem_set_ddm_factor(hfem_problem *p, double factor);
+/* NRSolver `constrainednrminiter` (nrsolver.C:183-185, 320-326): non-zero
+ * halves a Newton correction, from iteration miniter + 1 on, whenever the
+ * squared norm of the per-dof-group force errors grew since the previous
+ * iteration (the reference's constrainedNRalpha 0.5, which no keyword sets);
+ * not applied to corrections that go through the line search */
+int hfem_problem_set_constrained_nr(hfem_problem *p, int miniter);
 /* NonLinearStatic `stiffmode` (nlinearstatic.h:61-65): 0 = the tangent
  * stiffness for every Newton matrix, 1 = the secant stiffness for every
  * one, 2 = the initial elastic stiffness, assembled once and kept, 3 = the

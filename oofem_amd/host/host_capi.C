@@ -189,6 +189,11 @@ int hfem_problem_set_ddm(hfem_problem *p, int n, const int *eqs, const double *v
     });
 }
 
+int hfem_problem_set_constrained_nr(hfem_problem *p, int miniter)
+{
+    return guarded([&] { reinterpret_cast<StructuralProblem *>(p)->nr.constrainedMinIter = miniter; });
+}
+
 int hfem_problem_set_ddm_factor(hfem_problem *p, double factor)
 {
     return guarded([&] { reinterpret_cast<StructuralProblem *>(p)->nr.ddmFactor = factor; });

@@ -270,6 +270,9 @@ NRStepReport StructuralProblem::solveNonlinearStep()
     auto *pcg = dynamic_cast<HipPCGSolver *>(linSolver.get());
     const double lstol = pcg ? pcg->giveTolerance() : 0.0;
     double prevErr = -1.0, prevEta = -1.0;
+    // per-dof-group force errors of this and the previous check (NRSolver
+    // forceErrVec / forceErrVecOld, nrsolver.C:612-623) for constrained Newton
+    double feNew[3] = {0.0, 0.0, 0.0}, feOld[3] = {0.0, 0.0, 0.0};
     for (int nite = 0;; ++nite) {
         double ebe[3], sums[6];
         internalForces(X, true, ebe);               // updateComponent(InternalRhs)
@@ -278,10 +281,16 @@ NRStepReport StructuralProblem::solveNonlinearStep()
         // checkConvergence nrsolver.C:594-885, grouped by dof id; rtold is 0 (off)
         bool converged = true, outOfRange = false;
         double maxErr = 0.0;
+        if (nr.constrainedMinIter)
+            for (int dg = 0; dg < 3; dg++) {
+                feOld[dg] = feNew[dg];
+                feNew[dg] = 0.0;
+            }
         for (int dg = 0; dg < 3; dg++) {
             if (!idsInUse[dg]) continue;
             const double denom = sums[3 + dg] + ebe[dg];
             double forceErr = denom >= 1.e-6 ? std::sqrt(sums[dg] / denom) : std::sqrt(sums[dg]);
+            feNew[dg] = forceErr;
             if (std::isnan(forceErr) || forceErr > nr.rtolf * 1.e20) outOfRange = true;
             if (forceErr > nr.rtolf) converged = false;
             if (forceErr > 0.0 && nite == 0) converged = false;
@@ -315,6 +324,14 @@ NRStepReport StructuralProblem::solveNonlinearStep()
         ConvergedReason ls = linearSolve(rhs, ddX);  // ddX carries the previous correction
         rep.linearIterations += lastLinearIterations;
         if (ls != CR_CONVERGED) { rep.status = CR_FAILED; break; }
+        // constrained Newton (nrsolver.C:320-326): from iteration
+        // constrainednrminiter + 1 on, halve the correction when the squared
+        // norm of the force errors grew; not with the line search
+        if (nr.constrainedMinIter && !(nr.lineSearch && nite > 0) && nite > nr.constrainedMinIter) {
+            const double a = feNew[0] * feNew[0] + feNew[1] * feNew[1] + feNew[2] * feNew[2];
+            const double b = feOld[0] * feOld[0] + feOld[1] * feOld[1] + feOld[2] * feOld[2];
+            if (a > b) ddX.axpby(0.0, nullptr, 0.5);
+        }
         // increment cap (nrsolver.C:340-353), applied before the line search
         // so that the search runs along the capped direction
         if (nr.maxInc < 1.0e20) {

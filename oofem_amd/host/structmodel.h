@@ -81,6 +81,7 @@ struct NRSolverOptions {
     std::vector<int> ddmEq;
     std::vector<double> ddmVal;
     double ddmFactor = 0.0;
+    int constrainedMinIter = 0;   // constrainednrminiter (0 = off): halve a correction when the force errors grew
 };
 
 // SimpleReductionStrategy (timestepreductionstrategy.C:40-110) of the

@@ -44,6 +44,7 @@ def hostlib():
         L.hfem_problem_set_stiffness_mode.argtypes = [C.c_void_p, C.c_int]
         L.hfem_problem_set_ddm.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
         L.hfem_problem_set_ddm_factor.argtypes = [C.c_void_p, C.c_double]
+        L.hfem_problem_set_constrained_nr.argtypes = [C.c_void_p, C.c_int]
         L.hfem_problem_set_nr_extras.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]
         L.hfem_problem_set_arc_length.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_double, C.c_double,
                                                   C.c_double, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
@@ -96,6 +97,10 @@ class StructuralProblem:
         v = np.ascontiguousarray(values, dtype=np.float64)
         _check(hostlib().hfem_problem_set_ddm(self.h, len(e), _p(e) if len(e) else None, _p(v) if len(v) else None),
                "set_ddm")
+
+    def set_constrained_nr(self, miniter):
+        """NRSolver constrainednrminiter (0 = off)."""
+        _check(hostlib().hfem_problem_set_constrained_nr(self.h, int(miniter)), "set_constrained_nr")
 
     def set_ddm_factor(self, factor):
         _check(hostlib().hfem_problem_set_ddm_factor(self.h, float(factor)), "set_ddm_factor")

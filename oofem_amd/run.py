@@ -29,7 +29,9 @@ NRSolver's direct displacement control (``ddm`` node/dof pairs, ``ddv``
 values, ``ddltf`` function) is applied in both nonlinear drivers.
 
 Its solver keywords are read as the reference reads them: smtype, lstype,
-lstol, lsiter, lsprecond, rtolf, maxiter, manrmsteps, initialguess.
+lstol, lsiter, lsprecond, rtolf, maxiter, manrmsteps, initialguess, and
+NRSolver's linesearch, lsearchtol, lsearchamp, lsearchmaxeta, maxinc and
+constrainednrminiter.
 smtype and lstype select the plug-ins exactly as the class factory does, so
 a deck runs on this path when it says ``smtype 11 lstype 9``; any other type
 is rejected by the factory with an error.  ``keywords`` adds or replaces
@@ -123,6 +125,14 @@ def run_deck(path, out_path=None, keywords=None, log=None):
                         rootselectiontype=kwd("rootselectiontype", 0, int))
     d = prob.deck
     P = StructuralProblem(d, **kw)
+    if model != "linearstatic" and calm is None:
+        # NRSolver::initializeFrom nrsolver.C:106-200 and LineSearchNM's
+        # defaults (linesearch.C:44-52; its initializeFrom clamps them)
+        P.set_nr_extras(linesearch=bool(kwd("linesearch", 0, int)), ls_tol=kwd("lsearchtol", 0.8),
+                        ls_amp=kwd("lsearchamp", 2.5), ls_maxeta=kwd("lsearchmaxeta", 4.0),
+                        maxinc=kwd("maxinc", 1.0e20))
+        if kwd("constrainednrminiter", 0, int):
+            P.set_constrained_nr(kwd("constrainednrminiter", 0, int))
     if calm is not None:
         P.set_arc_length(**calm)
         # hyperplane control (calmls.C:754-797): the hpc array holds (node

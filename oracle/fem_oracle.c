@@ -1062,8 +1062,8 @@ int orc_nonlinear_step(const orc_mesh *m, double *state, double *X, const double
 {
     int o4[4];
     const int rc = orc_nonlinear_step_ex(m, state, X, u_presc, u_presc_prev, initial_guess, RT, rtolf, maxiter,
-                                         manrmsteps, lstol, 0, 0.8, 2.5, 4.0, 1.0e20, 0, 0, NULL, NULL, 0.0, o4,
-                                         err);
+                                         manrmsteps, lstol, 0, 0.8, 2.5, 4.0, 1.0e20, 0, 0, NULL, NULL, 0.0, 0,
+                                         o4, err);
     out[0] = o4[0];
     out[1] = o4[1];
     out[2] = o4[2];
@@ -1087,9 +1087,13 @@ int orc_nonlinear_step_ex(const orc_mesh *m, double *state, double *X, const dou
                           const double *u_presc_prev, int initial_guess, const double *RT, double rtolf, int maxiter,
                           int manrmsteps, double lstol, int linesearch, double ls_tol, double ls_amp, double ls_maxeta,
                           double maxinc, int stiffmode, int nddm, const int *ddm_eq, const double *ddm_val,
-                          double ddm_factor, int *out, double *err)
+                          double ddm_factor, int cnr_miniter, int *out, double *err)
 {
     const int tangent = stiffmode == 2 ? 0 : 1;
+    /* NRSolver's constrained Newton (nrsolver.C:183-185, 320-326, 612-623):
+     * from iteration cnr_miniter + 1 on, a correction is halved when the
+     * squared norm of the per-dof-group force errors grew */
+    double fe_new[3] = {0, 0, 0}, fe_old[3] = {0, 0, 0};
     const long long nnz = orc_csr_count(m);
     long long *rowptr = (long long *)malloc(sizeof(long long) * ((size_t)m->neq + 1));
     int *colind = (int *)malloc(sizeof(int) * (size_t)nnz);
@@ -1147,10 +1151,14 @@ int orc_nonlinear_step_ex(const orc_mesh *m, double *state, double *X, const dou
         }
         int converged = 1, outofrange = 0;
         double mx = 0.0;
+        if (cnr_miniter) {
+            for (int d = 0; d < 3; d++) { fe_old[d] = fe_new[d]; fe_new[d] = 0.0; }
+        }
         for (int d = 0; d < 3; d++) {
             if (!inuse[d]) continue;
             const double den = s_t[d] + ebe[d];
             const double fe = den >= 1.e-6 ? sqrt(s_r[d] / den) : sqrt(s_r[d]);
+            fe_new[d] = fe;
             if (isnan(fe) || fe > rtolf * 1.e20) outofrange = 1;
             if (fe > rtolf) converged = 0;
             if (fe > 0.0 && nite == 0) converged = 0;
@@ -1172,6 +1180,11 @@ int orc_nonlinear_step_ex(const orc_mesh *m, double *state, double *X, const dou
         /* maxIncAllowed (nrsolver.C:340-353); the reference caps after the
          * line search, this restatement caps the correction before it, so
          * the search runs along the capped direction */
+        if (cnr_miniter && !(linesearch && nite > 0) && nite > cnr_miniter) {
+            const double a = fe_new[0] * fe_new[0] + fe_new[1] * fe_new[1] + fe_new[2] * fe_new[2];
+            const double b = fe_old[0] * fe_old[0] + fe_old[1] * fe_old[1] + fe_old[2] * fe_old[2];
+            if (a > b) for (int i = 0; i < m->neq; i++) ddX[i] *= 0.5;
+        }
         double mxi = 0.0;
         for (int i = 0; i < m->neq; i++) if (fabs(ddX[i]) > mxi) mxi = fabs(ddX[i]);
         if (mxi > maxinc) for (int i = 0; i < m->neq; i++) ddX[i] *= maxinc / mxi;

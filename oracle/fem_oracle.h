@@ -131,7 +131,7 @@ int orc_nonlinear_step_ex(const orc_mesh *m, double *state, double *X, const dou
                           const double *u_presc_prev, int initial_guess, const double *RT, double rtolf, int maxiter,
                           int manrmsteps, double lstol, int linesearch, double ls_tol, double ls_amp, double ls_maxeta,
                           double maxinc, int stiffmode, int nddm, const int *ddm_eq, const double *ddm_val,
-                          double ddm_factor, int *out, double *err);
+                          double ddm_factor, int cnr_miniter, int *out, double *err);
 
 /* Skyline U^T D U factorization + solve of a full symmetric CSR (direct
  * solver of the reference; CPU baseline only).  *nwk_out: profile size. */

@@ -251,7 +251,7 @@ def nonlinear_step(deck, state, X, u_presc, RT=None, rtolf=1e-8, maxiter=50, man
 
 def nonlinear_step_ex(deck, state, X, u_presc, u_presc_prev=None, initial_guess=True, RT=None, rtolf=1e-8,
                       maxiter=50, manrmsteps=1, lstol=1e-12, linesearch=False, ls_tol=0.8, ls_amp=2.5, ls_maxeta=4.0,
-                      maxinc=1e20, stiffmode=0, ddm=None, ddm_factor=0.0):
+                      maxinc=1e20, stiffmode=0, ddm=None, ddm_factor=0.0, constrainednrminiter=0):
     """nonlinear_step with the NRSolver extras: line search (LineSearchNM,
     linesearch.C:54-245), the increment cap maxinc (nrsolver.C:340-353) and
     NonLinearStatic's stiffness mode (0 tangent, 2 elastic); ddm: (free
@@ -277,7 +277,8 @@ def nonlinear_step_ex(deck, state, X, u_presc, u_presc_prev=None, initial_guess=
                                      maxiter, manrmsteps, C.c_double(lstol), int(bool(linesearch)), C.c_double(ls_tol),
                                      C.c_double(ls_amp), C.c_double(ls_maxeta), C.c_double(maxinc), int(stiffmode),
                                      nd, de.ctypes.data_as(C.c_void_p) if nd else None,
-                                     dv.ctypes.data_as(C.c_void_p) if nd else None, C.c_double(ddm_factor), out,
+                                     dv.ctypes.data_as(C.c_void_p) if nd else None, C.c_double(ddm_factor),
+                                     int(constrainednrminiter), out,
                                      C.byref(err))
     if rc:
         raise RuntimeError(f"orc_nonlinear_step_ex rc={rc}")

@@ -517,3 +517,28 @@ def test_arc_length_deck_with_linearized_hyperplane_control(tmp_path):
     for k in (1, 2, 3):
         assert abs(r["results"][k]["u_node"][n4, 2] - 0.4 * k) <= 1e-9
         assert abs(r["results"][k]["load_level"] - 0.4 * k / u4) <= 1e-9 * 0.4 * k / abs(u4)
+
+
+@pytest.mark.gpu
+def test_constrained_newton_matches_oracle():
+    """NRSolver constrainednrminiter 1 on the GPU driver: the oracle's
+    Newton iteration count and root on a step whose corrections get
+    halved."""
+    from oofem_amd.problem import StructuralProblem
+    d = meshgen.config3(4, 4, 16)
+    up = d.prescribed_values(3 / 8)
+    st = oracle.state_new(d)
+    X = np.zeros(d.neq)
+    ref = oracle.nonlinear_step_ex(d, st, X, up, maxiter=60, lstol=1e-12, initial_guess=False, rtolf=1e-8,
+                                   constrainednrminiter=1)
+    assert ref["status"] == 1
+    P = StructuralProblem(d, lstol=1e-12, rtolf=1e-8, maxiter=60, manrmsteps=1, initial_guess=0)
+    try:
+        P.set_constrained_nr(1)
+        P.set_load(np.zeros(d.neq), np.zeros(d.npeq))
+        P.set_prescribed(up)
+        rep = P.nonlinear_step()
+        assert rep["status"] == 1 and rep["newton"] == ref["newton"], (rep, ref)
+        assert np.linalg.norm(P.solution() - X) <= 1e-8 * np.linalg.norm(X)
+    finally:
+        P.close()

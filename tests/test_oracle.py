@@ -837,3 +837,23 @@ def test_arc_length_hyperplane_control():
             eps = nlcases.uniaxial_strain(r["load_level"])
             assert np.abs(X[qz - 1] - eps).max() <= 1e-8 * eps
         assert r["load_level"] > nlcases.SIG0
+
+
+def test_constrained_newton_halves_growing_corrections():
+    """NRSolver constrainednrminiter (nrsolver.C:320-326): corrections after
+    iteration miniter are halved when the force errors grew, which changes
+    the iteration path but not the root of the step's equations; with a
+    miniter beyond the iterations taken nothing changes, bit for bit."""
+    d = meshgen.config3(4, 4, 16)
+    up = d.prescribed_values(3 / 8)
+    res = {}
+    for cnr in (0, 1, 100):
+        st = oracle.state_new(d)
+        X = np.zeros(d.neq)
+        r = oracle.nonlinear_step_ex(d, st, X, up, maxiter=60, lstol=1e-12, initial_guess=False, rtolf=1e-8,
+                                     constrainednrminiter=cnr)
+        assert r["status"] == 1
+        res[cnr] = (r, X)
+    assert res[100][0] == res[0][0] and np.array_equal(res[100][1], res[0][1])
+    assert res[1][0]["pcg"] != res[0][0]["pcg"]
+    assert np.linalg.norm(res[1][1] - res[0][1]) <= 1e-7 * np.linalg.norm(res[0][1])

@@ -125,6 +125,12 @@ int hfem_problem_set_reference_load(hfem_problem *p, const double *f_free, const
  * does, with a warning) and the reason is CR_DIVERGED_ITS.  The reactions
  * then use the reached load level times R. */
 int hfem_problem_arc_length_step(hfem_problem *p, int *out4, double *out4d);
+/* steps driven by the caller with a time-step reduction: save_converged
+ * keeps the current solution; restart_step returns to it and resets the
+ * temp Gauss-point values (StaticStructural::restartYourself and
+ * EngngModel::initStepIncrements, engngm.C:620-633) */
+int hfem_problem_save_converged(hfem_problem *p);
+int hfem_problem_restart_step(hfem_problem *p);
 /* EngngModel::updateYourself: commit the Gauss-point state */
 int hfem_problem_update(hfem_problem *p);
 

@@ -151,7 +151,10 @@ class DeckProblem:
         """(prescribed values per prescribed equation, (f_free, f_presc)) of
         step k. ``body_load(deck, elems, dens, b, factor)`` assembles each
         DeadWeight (default: meshgen.body_load_vector)."""
-        t = self.time_of_step(k)
+        return self.loading_at(self.time_of_step(k), body_load=body_load)
+
+    def loading_at(self, t, body_load=None):
+        """(prescribed values, (f_free, f_presc)) at time t."""
         d = self.deck
         val = np.zeros_like(d.bc_value)
         for bc in self.bcs:

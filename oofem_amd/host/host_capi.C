@@ -189,6 +189,16 @@ int hfem_problem_set_ddm(hfem_problem *p, int n, const int *eqs, const double *v
     });
 }
 
+int hfem_problem_save_converged(hfem_problem *p)
+{
+    return guarded([&] { reinterpret_cast<StructuralProblem *>(p)->saveConverged(); });
+}
+
+int hfem_problem_restart_step(hfem_problem *p)
+{
+    return guarded([&] { reinterpret_cast<StructuralProblem *>(p)->restartStep(); });
+}
+
 int hfem_problem_set_constrained_nr(hfem_problem *p, int miniter)
 {
     return guarded([&] { reinterpret_cast<StructuralProblem *>(p)->nr.constrainedMinIter = miniter; });

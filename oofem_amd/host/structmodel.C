@@ -792,6 +792,12 @@ CALMStepReport StructuralProblem::solveArcLengthStep()
     return rep;
 }
 
+void StructuralProblem::restartStep()
+{
+    X.axpby(1.0, &Xc, 0.0);
+    hipfemCheck(hfem_model_restart(ev), "restart");
+}
+
 void StructuralProblem::updateYourself()
 {
     hipfemCheck(hfem_model_commit(ev), "commit");

@@ -224,6 +224,11 @@ public:
     CALMStepReport solveArcLengthStep();
     double giveLoadLevel() const { return reachedLambda; }
     void updateYourself();   // commit Gauss-point state (EngngModel::updateYourself)
+    // a step driven from outside: keep the converged solution, and go back
+    // to it with the temp Gauss-point values reset
+    // (StaticStructural::restartYourself, EngngModel::initStepIncrements)
+    void saveConverged() { Xc.axpby(1.0, &X, 0.0); }
+    void restartStep();
 
     void giveSolution(double *u) const { X.download(u); }
     void giveReactions(double *r);

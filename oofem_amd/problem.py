@@ -45,6 +45,8 @@ def hostlib():
         L.hfem_problem_set_ddm.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p]
         L.hfem_problem_set_ddm_factor.argtypes = [C.c_void_p, C.c_double]
         L.hfem_problem_set_constrained_nr.argtypes = [C.c_void_p, C.c_int]
+        L.hfem_problem_save_converged.argtypes = [C.c_void_p]
+        L.hfem_problem_restart_step.argtypes = [C.c_void_p]
         L.hfem_problem_set_nr_extras.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]
         L.hfem_problem_set_arc_length.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_double, C.c_double,
                                                   C.c_double, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
@@ -210,6 +212,13 @@ class StructuralProblem:
         _check(hostlib().hfem_problem_arc_length_step(self.h, out, outd), "arc_length_step")
         return dict(iterations=out[0], pcg=out[1], restarts=out[2], status=out[3], load_level=outd[0],
                     step_length=outd[1], force_error=outd[2], disp_error=outd[3])
+
+    def save_converged(self):
+        _check(hostlib().hfem_problem_save_converged(self.h), "save_converged")
+
+    def restart_step(self):
+        """Back to the saved solution with the temp Gauss-point values reset."""
+        _check(hostlib().hfem_problem_restart_step(self.h), "restart_step")
 
     def update(self):
         _check(hostlib().hfem_problem_update(self.h), "update")

@@ -25,6 +25,10 @@ The engineering-model record chooses the driver:
   CALM line search and dof-group norms are refused, and so are incremental
   reference loads with it.
   Solver keywords may sit on the first metastep record.
+StaticStructural with ``treductiontype simplereduction`` (dtmin, dtmax,
+ncrf, nmaxrestarts, minrequirediter, maxrequirediter) steps adaptively:
+a step whose Newton loop fails restarts from the converged state with a
+reduced increment, and errorcheck step numbers count accepted steps.
 NRSolver's direct displacement control (``ddm`` node/dof pairs, ``ddv``
 values, ``ddltf`` function) is applied in both nonlinear drivers.
 
@@ -181,13 +185,71 @@ def run_deck(path, out_path=None, keywords=None, log=None):
             raise ValueError("ddm names a dof that is prescribed by a boundary condition")
         P.set_ddm(eqs, vals)
         ddm = prob.ltf[rec.scalar("ddltf", cast=int)]
+    # MetaStep treductiontype (metastep.C:56-61): SimpleReductionStrategy
+    # (timestepreductionstrategy.C:95-110) drives StaticStructural's steps
+    red = None
+    if kwd("treductiontype", "noreduction", str).lower() == "simplereduction":
+        if model != "staticstructural" or ddm is not None:
+            P.close()
+            raise ValueError("time-step reduction is built for StaticStructural without ddm")
+        try:
+            red = dict(dtmin=kwd("dtmin", None), dtmax=kwd("dtmax", None))
+        except Exception:
+            red = None
+        if red is None or red["dtmin"] is None or red["dtmax"] is None:
+            P.close()
+            raise ValueError("SimpleReductionStrategy needs dtmin and dtmax")
+        red.update(ncrf=kwd("ncrf", 0.5), nmaxrestarts=kwd("nmaxrestarts", 5, int),
+                   minreq=kwd("minrequirediter", 3, int), maxreq=kwd("maxrequirediter", 8, int))
     R0 = (np.zeros(d.neq), np.zeros(d.npeq))   # NonLinearStatic's initialLoadVector (free, prescribed)
     results, iters = {}, {}
     f = open(out_path, "w") if out_path else None
     try:
         if f:
             write_header(f, prob)
-        for k in range(1, prob.nsteps + 1):
+        if red is not None:
+            # EngngModel::solveYourself engngm.C:573-662: a failed Newton loop
+            # restarts the step from the converged state with dt * ncrf; the
+            # next increment follows the Newton iterations (metastep.C:138-183)
+            dt = prob.time_of_step(1)
+            final = prob.time_of_step(prob.nsteps)
+            tcur, k = 0.0, 0
+            while tcur < final * (1.0 - 1.0e-12):
+                t0 = time.perf_counter()
+                P.save_converged()
+                restarts = 0
+                while True:
+                    target = tcur + dt
+                    up, (ff, fp) = prob.loading_at(target)
+                    P.set_prescribed(up)
+                    P.set_load(ff, fp)
+                    rep = P.nonlinear_step()
+                    if rep["status"] == 1:
+                        break
+                    dt2 = dt * red["ncrf"]
+                    if dt2 < red["dtmin"]:
+                        raise RuntimeError("Minimum time increment reached")
+                    restarts += 1
+                    if restarts > red["nmaxrestarts"]:
+                        raise RuntimeError("Maximum number of time step reductions has been reached.")
+                    dt = dt2
+                    P.restart_step()
+                k += 1
+                iters[k] = (rep["newton"], rep["pcg"])
+                results[k] = _collect(prob, P, P.solution(), up)
+                results[k].update(time=target, restarts=restarts)
+                n = rep["newton"]
+                fac = 2.0 if (n != 0 and n < red["minreq"]) else 0.5 if (n != 0 and n > red["maxreq"]) else 1.0
+                dt = min(max(dt * fac, red["dtmin"]), red["dtmax"])
+                if target + dt > final:
+                    dt = final - target
+                P.update()
+                if f:
+                    write_step(f, prob, k, target, results[k], time.perf_counter() - t0)
+                if log:
+                    log(f"[deck] step {k} (t = {target:g}, {restarts} reductions): {iters[k]}")
+                tcur = target
+        for k in range(1, prob.nsteps + 1 if red is None else 1):
             t0 = time.perf_counter()
             up, (ff, fp) = prob.step_loading(k)
             P.set_prescribed(up)

@@ -542,3 +542,29 @@ def test_constrained_newton_matches_oracle():
         assert np.linalg.norm(P.solution() - X) <= 1e-8 * np.linalg.norm(X)
     finally:
         P.close()
+
+
+@pytest.mark.gpu
+def test_run_deck_time_step_reduction_matches_oracle(tmp_path):
+    """A StaticStructural deck with treductiontype simplereduction (the
+    8x8x32 MisesMat block pulled to 1/1 of its end displacement in one
+    step of dt 1, maxiter 8): run_deck's adaptive steps are the oracle
+    driver's accepted steps (time, Newton iterations, restarts), and the
+    final displacements agree."""
+    from oofem_amd.run import run_deck
+    from tests.deckwriter import write_block_deck
+    d = meshgen.config3(8, 8, 32)
+    path = tmp_path / "red.in"
+    write_block_deck(d, path, "StaticStructural nsteps 1 nmsteps 1 nmodules 1",
+                     "nsteps 1 deltat 1.0 treductiontype simplereduction dtmin 0.01 dtmax 1.0 maxiter 8 "
+                     "rtolf 1e-8 manrmsteps 1")
+    X, st, ref = oracle.static_structural(d, d.prescribed_values(1.0), 1.0, 1.0, reduction=dict(dtmin=0.01, dtmax=1.0),
+                                          maxiter=8, lstol=1e-12)
+    # lsiter: the IMLSolver default of 200 iterations is far too few for this mesh
+    r = run_deck(str(path), keywords=dict(smtype=11, lstype=9, lstol=1e-12, lsiter=100000))
+    res = r["results"]
+    got = [(round(res[k]["time"], 12), r["iterations"][k][0], res[k]["restarts"]) for k in sorted(res)]
+    assert got == [(round(s["time"], 12), s["newton"], s["restarts"]) for s in ref], (got, ref)
+    assert any(s["restarts"] for s in ref)
+    u = res[max(res)]["u"]
+    assert np.linalg.norm(u - X) <= 1e-8 * np.linalg.norm(X)

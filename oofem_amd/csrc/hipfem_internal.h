@@ -192,6 +192,12 @@ long next_pattern_id();
 // at once inside kernels when the PCG state is halted
 bool model_mf_available(const hipcsr_mtx *m);
 int model_mf_apply_f32(hfem_model *md, const float *x, double *y, const PcgState *st);
+// the slot form's element pass alone: every element's FP32 forces into its
+// colour's slot vector; yslot / eqmask / ncolors describe the slots for a
+// caller that sums them itself.  Sets *yslot = nullptr (and launches
+// nothing) when the model does not use the slot form.
+int model_mf_slots_f32(hfem_model *md, const float *x, const PcgState *st, const float **yslot,
+                       const uint16_t **eqmask, int *ncolors);
 void model_unlink(hfem_model *md, hipcsr_mtx *m);
 }  // namespace hipfem
 

@@ -1564,6 +1564,26 @@ void model_unlink(hfem_model *md, hipcsr_mtx *m)
     m->mf_version = -1;
 }
 
+int model_mf_slots_f32(hfem_model *md, const float *x, const PcgState *st, const float **yslot,
+                       const uint16_t **eqmask, int *ncolors)
+{
+    *yslot = nullptr;
+    *eqmask = nullptr;
+    *ncolors = 0;
+    if (!(md->mf_form == 2 && md->d_yslot && md->d_erel)) return 0;
+    hipStream_t s = md->ctx->stream;
+    if (md->nelem)
+        k_mf_hex<<<(md->nelem + MF_EPB - 1) / MF_EPB, 256, 0, s>>>(
+            nullptr, md->nelem, md->d_erel, md->d_conn, md->d_eqn, md->d_emat, md->d_mat,
+            md->tdesc_used ? md->d_tdesc : nullptr, md->d_dsur, md->ngp_total, x, nullptr, nullptr, md->d_yslot,
+            md->d_ecolor, md->neq, st);
+    HIPFEM_CHECK(hipGetLastError());
+    *yslot = md->d_yslot;
+    *eqmask = md->d_eqmask;
+    *ncolors = md->ncolors;
+    return 0;
+}
+
 int model_mf_apply_f32(hfem_model *md, const float *x, double *y, const PcgState *st)
 {
     hipStream_t s = md->ctx->stream;

@@ -476,6 +476,30 @@ __global__ __launch_bounds__(256) void k_dot_pq(int n, const float *__restrict__
     if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
+// the element operator's slot sum fused with k_dot_pq: q32 = (float) of the
+// FP64 sum over the colours touching each equation (ascending) of the FP32
+// slot values, and the partials of p.q in that FP64 sum; the same grid and
+// order as k_mf_slot_sum followed by k_dot_pq, so the same bits
+__global__ __launch_bounds__(256) void k_slot_sum_dot(int n, int ncolors, const uint16_t *__restrict__ eqmask,
+                                                      const float *__restrict__ yslot, const float *__restrict__ p,
+                                                      float *__restrict__ q, const PcgState *__restrict__ st,
+                                                      double *__restrict__ partials)
+{
+    __shared__ double sh[4];
+    if (halted(st)) return;
+    double acc = 0.0;
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
+        const unsigned m = eqmask[i];
+        double y = 0.0;
+        for (int c = 0; c < ncolors; c++)
+            if (m >> c & 1u) y += (double)yslot[(size_t)c * n + i];
+        q[i] = (float)y;
+        acc += (double)p[i] * y;
+    }
+    const double t = block_sum<256>(acc, sh);
+    if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
 // after a refresh: beta from the true residual (the direction is kept), rmax
 __global__ __launch_bounds__(RED_THREADS) void k_reduce_refresh(PcgState *st, const double *__restrict__ partials,
                                                                 int nb, int first)
@@ -838,9 +862,18 @@ static int pcg_reliable(hippcg_solver *s, hipcsr_mtx *A, const double *db, doubl
             first = false;
             if (prof) HIPFEM_CHECK(hipEventRecord(s->ev[2 * (k - 1)], st));
             if (mf) {
-                rc = model_mf_apply_f32(A->mf_model, s->fp0, s->q, s->st);
+                const float *ys = nullptr;
+                const uint16_t *em = nullptr;
+                int nc = 0;
+                rc = model_mf_slots_f32(A->mf_model, s->fp0, s->st, &ys, &em, &nc);
                 if (rc) return rc;
-                k_dot_pq<<<gv, 256, 0, st>>>(n, s->fp0, s->q, s->fq, s->st, part);
+                if (ys) {
+                    k_slot_sum_dot<<<gv, 256, 0, st>>>(n, nc, em, ys, s->fp0, s->fq, s->st, part);
+                } else {
+                    rc = model_mf_apply_f32(A->mf_model, s->fp0, s->q, s->st);
+                    if (rc) return rc;
+                    k_dot_pq<<<gv, 256, 0, st>>>(n, s->fp0, s->q, s->fq, s->st, part);
+                }
             } else if (nodeblock) {
                 k_pcg_spmv_p_nb<float, float><<<gs, 256, 0, st>>>(A->nbr, A->d_bptr, A->d_bcol, A->d_val32, s->fp0,
                                                                   s->fq, s->st, part);

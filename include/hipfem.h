@@ -215,6 +215,15 @@ int hippcg_set_profiling(hippcg_solver *s, int on);
  * place in its own pass and gathers p once per nonzero. */
 int hippcg_set_fused(hippcg_solver *s, int fused);
 int hippcg_profile(hippcg_solver *s, int *n_spmv64, double *ms_spmv64, int *n_spmv32, double *ms_spmv32);
+/* Residual history of the plain PCG (precision 0, and the inner solves of
+ * precision 1): with a capacity cap > 0 every solve records ||r_k|| / ||b||
+ * for k = 0, 1, ... (the quantity cg.h compares with lstol, iml/cg.h:34-60,
+ * as IMLSolver::solve runs it, imlsolver.C:126-129); hippcg_get_history
+ * copies the last solve's min(cap, iterations + 1) values and gives their
+ * count in *len.  The reference logs only the final value; the history is
+ * the observable its CG template is compared on. */
+int hippcg_set_history(hippcg_solver *s, int cap);
+int hippcg_get_history(hippcg_solver *s, double *out, int cap, int *len);
 
 /* --------------------------------------------------- batched evaluator */
 int  hfem_model_create(hipfem_ctx *ctx, hfem_model **out);
@@ -257,6 +266,10 @@ int hfem_model_tangent_product_dev(hfem_model *md, int mode, const double *dx_fr
  * sums6 = {sum rhs^2 over u, v, w equations, sum RT^2 over u, v, w}, the
  * per-dof-id quantities of NRSolver::checkConvergence (nrsolver.C:648-660). */
 int hfem_model_residual(hfem_model *md, const double *dRT, const double *dF, double *drhs, double *sums6);
+/* per dof id squared sums of two device vectors over the free equations:
+ * sums6 = {sum x^2 (u, v, w), sum y^2 (u, v, w)} -- the dg_dispErr /
+ * dg_totalDisp sums of NRSolver::checkConvergence (nrsolver.C:648-660) */
+int hfem_model_group_sums(hfem_model *md, const double *dx, const double *dy, double *sums6);
 /* MisesMatStatus::updateYourself (misesmat.C:720-728) for every Gauss point */
 int hfem_model_commit(hfem_model *md);
 /* form of the FP32 element-by-element operator used by hippcg precision 2

@@ -44,11 +44,32 @@ int hfem_problem_set_initial_guess(hfem_problem *p, int type);
 int hfem_problem_set_forcing(hfem_problem *p, int on, double eta_max);
 /* NRSolver keywords `linesearch`, `lsearchtol`, `lsearchamp`,
  * `lsearchmaxeta` (nrsolver.h:63, linesearch.h:45-47; defaults 0.8, 2.5,
- * 4.0) and `maxinc` (nrsolver.h:70; 1e20 = off).  The line search ends at
- * its own point X + eta dX; the reference's NRSolver::solve adds the scaled
- * correction once more afterwards (nrsolver.C:356). */
+ * 4.0) and `maxinc` (nrsolver.h:70; 1e20 = off).  After the search, the
+ * correction is capped and added to X as in NRSolver::solve
+ * (nrsolver.C:318-350), unless hfem_problem_set_nr_criteria chose
+ * ls_keep_point. */
 int hfem_problem_set_nr_extras(hfem_problem *p, int linesearch, double ls_tol, double ls_amp, double ls_maxeta,
                                double maxinc);
+/* NRSolver `rtold` (displacement criterion, nrsolver.C:138-148, 786-806; 0
+ * = off, the default) and `miniter` (nrsolver.C:113); ls_keep_point = 1
+ * (no counterpart in the reference) caps a correction before the line
+ * search and keeps the search's point, where the reference's NRSolver
+ * caps after it and adds the searched step to X once more
+ * (nrsolver.C:318-350), the default here too. */
+int hfem_problem_set_nr_criteria(hfem_problem *p, double rtold, int miniter, int ls_keep_point);
+/* convergence checks of the last nonlinear step (NRSolver::checkConvergence,
+ * nrsolver.C:594-885): 6 values per check -- the force errors of the u, v,
+ * w dof groups and their displacement errors, -1 for a group without free
+ * equations or a criterion that is off; *n = checks (up to cap copied) */
+int hfem_problem_nr_errors(hfem_problem *p, double *out, int cap, int *n);
+/* StructuralEngngModel::updateInternalRHS (structengngmodel.C:215-233) called
+ * through the EngngModel base: internal forces of the current solution
+ * (neq) and the per-dof-id EBE norms (3) */
+int hfem_problem_update_internal_rhs(hfem_problem *p, double *f, double *enorm3);
+/* EngngModel::assemble (engngm.h:957-958) called through the EngngModel base
+ * with TangentAssembler(response_mode) into the problem's stiffness matrix:
+ * 0 TangentStiffness, 1 SecantStiffness, 2 ElasticStiffness */
+int hfem_problem_assemble(hfem_problem *p, int response_mode);
 /* NRSolver direct displacement control (`ddm` / `ddv`, nrsolver.C:165-176,
  * 458-568): the free equations eqs (1-based; the deck's node / dof pairs
  * mapped to equations) are driven by a 1e6 penalty on the diagonal of every

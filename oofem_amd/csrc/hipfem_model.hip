@@ -1065,6 +1065,31 @@ __global__ __launch_bounds__(256) void k_residual_groups(int neq, const int8_t *
         sh[threadIdx.x][0] + sh[threadIdx.x][1] + sh[threadIdx.x][2] + sh[threadIdx.x][3];
 }
 
+// squared sums of x and y per dof id (u, v, w) over the free equations: the
+// displacement-criterion sums of NRSolver::checkConvergence (nrsolver.C:648-660)
+__global__ __launch_bounds__(256) void k_group_sums(int neq, const int8_t *__restrict__ dofid,
+                                                    const double *__restrict__ x, const double *__restrict__ y,
+                                                    double *__restrict__ partials, int np)
+{
+    __shared__ double sh[6][4];
+    double s[6] = {0, 0, 0, 0, 0, 0};
+    for (int i = blockIdx.x * 256 + threadIdx.x; i < neq; i += gridDim.x * 256) {
+        const int d = dofid[i];
+        const double xi = x[i], yi = y[i];
+        s[d] += xi * xi;
+        s[3 + d] += yi * yi;
+    }
+    for (int k = 0; k < 6; k++) {
+        double v = s[k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+        if (threadIdx.x % 64 == 0) sh[k][threadIdx.x / 64] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 6) partials[(size_t)threadIdx.x * np + blockIdx.x] =
+        sh[threadIdx.x][0] + sh[threadIdx.x][1] + sh[threadIdx.x][2] + sh[threadIdx.x][3];
+}
+
 __global__ __launch_bounds__(1024) void k_reduce6(const double *__restrict__ partials, int np, double *out)
 {
     __shared__ double sh[16];
@@ -1310,6 +1335,27 @@ extern "C" int hfem_model_residual(hfem_model *md, const double *dRT, const doub
     if (rc) return rc;
     if (md->neq) {
         k_residual_groups<<<np, 256, 0, st>>>(md->neq, md->d_dofid, dRT, dF, drhs, md->ctx->d_partials, np);
+        k_reduce6<<<1, 1024, 0, st>>>(md->ctx->d_partials, np, md->d_out3);
+        HIPFEM_CHECK(hipGetLastError());
+        HIPFEM_CHECK(hipMemcpyAsync(md->ctx->h_pinned, md->d_out3, 6 * sizeof(double), hipMemcpyDeviceToHost, st));
+        HIPFEM_CHECK(hipStreamSynchronize(st));
+        for (int k = 0; k < 6; k++) sums6[k] = md->ctx->h_pinned[k];
+    } else {
+        for (int k = 0; k < 6; k++) sums6[k] = 0.0;
+    }
+    return 0;
+}
+
+extern "C" int hfem_model_group_sums(hfem_model *md, const double *dx, const double *dy, double *sums6)
+{
+    HIPFEM_REQUIRE(md && md->d_dofid && dx && dy && sums6, HIPFEM_EINVAL, "group_sums: bad arguments");
+    DeviceGuard g(md->ctx->device);
+    hipStream_t st = md->ctx->stream;
+    const int np = std::max(1, std::min((md->neq + 255) / 256, 2048));
+    int rc = ensure_partials(md->ctx, 6 * (size_t)np);
+    if (rc) return rc;
+    if (md->neq) {
+        k_group_sums<<<np, 256, 0, st>>>(md->neq, md->d_dofid, dx, dy, md->ctx->d_partials, np);
         k_reduce6<<<1, 1024, 0, st>>>(md->ctx->d_partials, np, md->d_out3);
         HIPFEM_CHECK(hipGetLastError());
         HIPFEM_CHECK(hipMemcpyAsync(md->ctx->h_pinned, md->d_out3, 6 * sizeof(double), hipMemcpyDeviceToHost, st));

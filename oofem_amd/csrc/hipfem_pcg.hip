@@ -288,7 +288,7 @@ __device__ double reduce_partials(const double *__restrict__ p, int nb, double *
 }
 
 __global__ __launch_bounds__(RED_THREADS) void k_reduce_init(PcgState *st, const double *__restrict__ partials, int nb,
-                                                             double tol)
+                                                             double tol, double *__restrict__ hist, int hist_cap)
 {
     __shared__ double sh[RED_THREADS / 64];
     const double bb = reduce_partials(partials, nb, sh);
@@ -307,6 +307,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_reduce_init(PcgState *st, const
         st->bad = 0;
         st->pause = 0;
         st->done = (st->res <= tol) ? 1 : 0;
+        if (hist_cap > 0) hist[0] = st->res;
     }
 }
 
@@ -319,7 +320,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_reduce_alpha(PcgState *st, cons
 }
 
 __global__ __launch_bounds__(RED_THREADS) void k_reduce_beta(PcgState *st, const double *__restrict__ partials, int nb,
-                                                             int iter)
+                                                             int iter, double *__restrict__ hist, int hist_cap)
 {
     __shared__ double sh[RED_THREADS / 64];
     if (halted(st)) return;
@@ -329,6 +330,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_reduce_beta(PcgState *st, const
         const double res = sqrt(rr) / st->normb;
         st->iter = iter;
         st->res = res;
+        if (iter < hist_cap) hist[iter] = res;
         if (res <= st->tol) {
             st->done = 1;
         } else if (!(res == res)) {
@@ -630,6 +632,10 @@ struct hippcg_solver {
     std::vector<hipEvent_t> ev;           // 2 events per iteration slot
     int prof_n64 = 0, prof_n32 = 0;
     double prof_ms64 = 0.0, prof_ms32 = 0.0;
+    // optional residual history of the last plain solve: ||r_k|| / ||b||
+    double *d_hist = nullptr;
+    int hist_cap = 0;
+    int hist_len = 0;
 };
 
 static int ensure_events(hippcg_solver *s, size_t slots)
@@ -734,7 +740,7 @@ static int pcg_run(hippcg_solver *s, hipcsr_mtx *A, const MT *val, const VT *b, 
         }
     }
     k_pcg_init<VT><<<gv, 256, 0, st>>>(n, b, q, x_zero, r, z, dinv, part);
-    k_reduce_init<<<1, RED_THREADS, 0, st>>>(s->st, part, gv, tol);
+    k_reduce_init<<<1, RED_THREADS, 0, st>>>(s->st, part, gv, tol, s->d_hist, s->hist_cap);
     HIPFEM_CHECK(hipGetLastError());
     PcgState h;
     rc = read_state(s, &h);
@@ -743,6 +749,7 @@ static int pcg_run(hippcg_solver *s, hipcsr_mtx *A, const MT *val, const VT *b, 
         *iters = 0;
         *res = h.res;
         *converged = 1;
+        s->hist_len = std::min(1, s->hist_cap);
         return 0;
     }
     int it = 1;
@@ -772,7 +779,7 @@ static int pcg_run(hippcg_solver *s, hipcsr_mtx *A, const MT *val, const VT *b, 
             if (prof) HIPFEM_CHECK(hipEventRecord(s->ev[2 * (it - 1) + 1], st));
             k_reduce_alpha<<<1, RED_THREADS, 0, st>>>(s->st, part, gs);
             k_pcg_update<XT, VT><<<gv, 256, 0, st>>>(n, x, r, z, fused ? pnew : p0, q, dinv, s->st, part);
-            k_reduce_beta<<<1, RED_THREADS, 0, st>>>(s->st, part, gv, it);
+            k_reduce_beta<<<1, RED_THREADS, 0, st>>>(s->st, part, gv, it, s->d_hist, s->hist_cap);
             if (fused) std::swap(pold, pnew);
         }
         HIPFEM_CHECK(hipGetLastError());
@@ -784,6 +791,7 @@ static int pcg_run(hippcg_solver *s, hipcsr_mtx *A, const MT *val, const VT *b, 
     *iters = h.iter;
     *res = h.res;
     *converged = (h.done && !h.bad && h.res <= tol) ? 1 : 0;
+    s->hist_len = std::min(h.iter + 1, s->hist_cap);
     if (prof) {
         // launches after convergence returned at once: time iterations 1..h.iter only
         HIPFEM_CHECK(hipStreamSynchronize(st));
@@ -1033,6 +1041,7 @@ extern "C" void hippcg_destroy(hippcg_solver *s)
         DeviceGuard g(c->device);
         (void)hipStreamSynchronize(c->stream);
         solver_free(s);
+        if (s->d_hist) (void)hipFree(s->d_hist);
         for (hipEvent_t e : s->ev) (void)hipEventDestroy(e);
         delete s;
     }
@@ -1076,6 +1085,36 @@ extern "C" int hippcg_set_fused(hippcg_solver *s, int fused)
 {
     HIPFEM_REQUIRE(s, HIPFEM_EINVAL, "null solver");
     s->fused = fused ? 1 : 0;
+    return 0;
+}
+
+extern "C" int hippcg_set_history(hippcg_solver *s, int cap)
+{
+    HIPFEM_REQUIRE(s, HIPFEM_EINVAL, "null solver");
+    HIPFEM_REQUIRE(cap >= 0, HIPFEM_EINVAL, "history capacity %d < 0", cap);
+    DeviceGuard g(s->ctx->device);
+    HIPFEM_CHECK(hipStreamSynchronize(s->ctx->stream));
+    if (s->d_hist) HIPFEM_CHECK(hipFree(s->d_hist));
+    s->d_hist = nullptr;
+    s->hist_cap = 0;
+    s->hist_len = 0;
+    if (cap > 0) {
+        HIPFEM_CHECK(hipMalloc(&s->d_hist, sizeof(double) * (size_t)cap));
+        s->hist_cap = cap;
+    }
+    return 0;
+}
+
+extern "C" int hippcg_get_history(hippcg_solver *s, double *out, int cap, int *len)
+{
+    HIPFEM_REQUIRE(s && len, HIPFEM_EINVAL, "null argument");
+    *len = s->hist_len;
+    const int n = std::min(cap, s->hist_len);
+    if (n > 0 && out) {
+        DeviceGuard g(s->ctx->device);
+        HIPFEM_CHECK(hipMemcpyAsync(out, s->d_hist, sizeof(double) * (size_t)n, hipMemcpyDeviceToHost, s->ctx->stream));
+        HIPFEM_CHECK(hipStreamSynchronize(s->ctx->stream));
+    }
     return 0;
 }
 

@@ -400,6 +400,18 @@ class HipPCGSolver:
         _check(lib().hippcg_profile(self.h, C.byref(n64), C.byref(ms64), C.byref(n32), C.byref(ms32)), "profile")
         return dict(n64=n64.value, ms64=ms64.value, n32=n32.value, ms32=ms32.value)
 
+    def set_history(self, cap):
+        """Record ||r_k||/||b|| of every plain solve, up to cap values."""
+        _check(lib().hippcg_set_history(self.h, int(cap)), "set_history")
+
+    def history(self, cap=1 << 20):
+        """The last solve's residual history (k = 0..iterations)."""
+        n = C.c_int(0)
+        _check(lib().hippcg_get_history(self.h, None, 0, C.byref(n)), "get_history")
+        out = np.zeros(max(1, min(cap, n.value)))
+        _check(lib().hippcg_get_history(self.h, _p(out), int(out.shape[0]), C.byref(n)), "get_history")
+        return out[:min(cap, n.value)]
+
     def giveClassName(self):
         return "HipPCGSolver"
 

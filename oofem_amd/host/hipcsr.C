@@ -1,5 +1,6 @@
 #include "hipcsr.h"
 
+#include <algorithm>
 #include <cstdlib>
 #include <mutex>
 
@@ -131,13 +132,26 @@ int HipCSRSparseMtrx::buildInternalStructure(EngngModel *eModel, int di, const U
     const int neq = eModel->giveNumberOfDomainEquations(di, s);
     mirror.clear();
     mirrorValid = mirrorDirty = false;
-    if (hfem_model *ev = eModel->giveBatchedEvaluator(di)) {
+    auto *bp = dynamic_cast<BatchedEvaluatorProvider *>(eModel);
+    if (hfem_model *ev = bp ? bp->giveBatchedEvaluator(di) : nullptr) {
         hipfemCheck(hfem_model_build_structure(ev, m), "buildInternalStructure");
     } else {
-        std::vector<int> loc;
-        int nelem = 0, ndofe = 0;
-        eModel->giveElementLocationArrays(di, loc, nelem, ndofe);
-        hipfemCheck(hipcsr_build_structure(m, neq, nelem, ndofe, loc.data()), "buildInternalStructure");
+        // union of loc x loc over the elements (compcol.C:174-197); location
+        // arrays of different lengths are padded with 0 (no equation)
+        IntArray loc;
+        Domain *domain = eModel->giveDomain(di);
+        std::vector<IntArray> locs;
+        locs.reserve((size_t)domain->giveNumberOfElements());
+        int ndofe = 0;
+        for (auto &elem : domain->giveElements()) {
+            elem->giveLocationArray(loc, s);
+            ndofe = std::max(ndofe, loc.giveSize());
+            locs.push_back(loc);
+        }
+        std::vector<int> all(locs.size() * (size_t)ndofe, 0);
+        for (size_t e = 0; e < locs.size(); e++)
+            for (int i = 0; i < locs[e].giveSize(); i++) all[e * (size_t)ndofe + i] = locs[e][i];
+        hipfemCheck(hipcsr_build_structure(m, neq, (int)locs.size(), ndofe, all.data()), "buildInternalStructure");
     }
     nRows = nColumns = neq;
     syncVersion();

@@ -1,9 +1,10 @@
 // HipCSRSparseMtrx: the SparseMtrx of type SMT_HipCSR.  Storage and every
 // product live on the GPU (libhipfem, include/hipfem.h); this class only maps
 // the reference's SparseMtrx contract (sparsemtrx.h:60-281) onto the C ABI:
-//   buildInternalStructure  pattern from the engineering model's element
-//                           location arrays (compcol.C:174-272 semantics), or
-//                           from the batched evaluator when the model has one
+//   buildInternalStructure  pattern from the location arrays of the domain's
+//                           elements (compcol.C:174-197: giveDomain,
+//                           giveElements, giveLocationArray), or from the
+//                           batched evaluator when the model provides one
 //   assemble(loc, mat)      staged on the host, sent at assembleEnd()
 //   times / timesT          device SpMV
 //   at(i,j) const           single-entry read; at(i,j) non-const works on a
@@ -14,6 +15,7 @@
 #include <vector>
 
 #include "../../include/hipfem.h"
+#include "engngm.h"
 #include "sparsemtrx.h"
 
 namespace oofem {
@@ -23,14 +25,13 @@ namespace oofem {
 hipfem_ctx *hipfemDefaultContext();
 void hipfemCheck(int rc, const char *what);
 
-class EngngModel {
+// Engineering models that evaluate their elements in batches implement this
+// interface (no counterpart in the reference): the matrix then takes its
+// pattern from the evaluator instead of one location array per element.
+class BatchedEvaluatorProvider {
 public:
-    virtual ~EngngModel() {}
-    virtual int giveNumberOfDomainEquations(int di, const UnknownNumberingScheme &s) = 0;
-    // location arrays of every element (nelem x ndofe, 1-based, 0 = prescribed)
-    virtual void giveElementLocationArrays(int di, std::vector<int> &loc, int &nelem, int &ndofe) = 0;
-    // batched element/material evaluator of the domain, if it has one
-    virtual hfem_model *giveBatchedEvaluator(int di) { return nullptr; }
+    virtual ~BatchedEvaluatorProvider() {}
+    virtual hfem_model *giveBatchedEvaluator(int di) = 0;
 };
 
 class HipCSRSparseMtrx : public SparseMtrx {

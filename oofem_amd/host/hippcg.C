@@ -31,18 +31,31 @@ void HipPCGSolver::setTolerance(double t)
     hipfemCheck(hippcg_set_options(s, tol, maxite, precond), "hippcg_set_options");
 }
 
-// IMLSolver::initializeFrom imlsolver.C:67-101 keywords and defaults
+// IMLSolver::initializeFrom imlsolver.C:67-97: the same keywords, defaults
+// and reading macros.  IMLSolver's stype 1 (GMRES) and its ILU / IC
+// preconditioners (lsprecond 2-4) are not part of hippcg and are refused.
 void HipPCGSolver::initializeFrom(InputRecord &ir)
 {
-    if (ir.has("stype") && (int)ir.get("stype", 0) != 0)
-        OOFEM_ERROR("hippcg implements the CG solver type (stype 0) only");
-    tol = ir.get("lstol", 1.e-5);
-    maxite = (int)ir.get("lsiter", 200);
-    precond = (int)ir.get("lsprecond", 0);
-    if (precond != 0 && precond != 1) OOFEM_ERROR("unknown preconditioner type %d", precond);
-    mixed = (int)ir.get("mixed", 0);
-    irtol = ir.get("irtol", 1e-4);
-    reldelta = ir.get("reldelta", 0.1);
+    int val = 0;
+    IR_GIVE_OPTIONAL_FIELD(ir, val, _IFT_IMLSolver_stype);
+    if (val != 0) throw ValueInputException(ir, _IFT_IMLSolver_stype, "hippcg implements the CG solver (stype 0) only");
+
+    tol = 1.e-5;
+    IR_GIVE_OPTIONAL_FIELD(ir, tol, _IFT_IMLSolver_lstol);
+    maxite = 200;
+    IR_GIVE_OPTIONAL_FIELD(ir, maxite, _IFT_IMLSolver_lsiter);
+    val = 0;
+    IR_GIVE_OPTIONAL_FIELD(ir, val, _IFT_IMLSolver_lsprecond);
+    if (val != 0 && val != 1) throw ValueInputException(ir, _IFT_IMLSolver_lsprecond, "unknown preconditioner type");
+    precond = val;
+
+    mixed = 0;
+    IR_GIVE_OPTIONAL_FIELD(ir, mixed, _IFT_HipPCGSolver_mixed);
+    if (mixed < 0 || mixed > 2) throw ValueInputException(ir, _IFT_HipPCGSolver_mixed, "mixed is 0, 1 or 2");
+    irtol = 1e-4;
+    IR_GIVE_OPTIONAL_FIELD(ir, irtol, _IFT_HipPCGSolver_irtol);
+    reldelta = 0.1;
+    IR_GIVE_OPTIONAL_FIELD(ir, reldelta, _IFT_HipPCGSolver_reldelta);
     applyOptions();
 }
 

@@ -8,6 +8,12 @@
 #pragma once
 
 #include "hipcsr.h"
+#include "imlsolver.h"
+
+// hippcg's own keywords (no counterpart in IMLSolver)
+#define _IFT_HipPCGSolver_mixed "mixed"
+#define _IFT_HipPCGSolver_irtol "irtol"
+#define _IFT_HipPCGSolver_reldelta "reldelta"
 
 namespace oofem {
 

@@ -1,6 +1,7 @@
 // C entry points of the host layer (libhipfem_host.so), used by the Python
 // driver and tests: a StructuralProblem created from flat arrays, solved
 // through the class factory (smtype / lstype) and the mirrored drivers.
+#include <algorithm>
 #include <cstring>
 #include <string>
 
@@ -49,12 +50,12 @@ int hfem_problem_set_options(hfem_problem *p, int smtype, int lstype, double lst
         auto *P = reinterpret_cast<StructuralProblem *>(p);
         P->setSparseMtrxType(smtype);
         P->setLinSystSolverType(lstype);
-        InputRecord &ir = P->giveSolverRecord();
-        ir.set("lstol", lstol);
-        ir.set("lsiter", lsiter);
-        ir.set("lsprecond", lsprecond);
-        ir.set("mixed", mixed);
-        ir.set("irtol", irtol);
+        DynamicInputRecord &ir = P->giveSolverRecord();
+        ir.setField(lstol, _IFT_IMLSolver_lstol);
+        ir.setField(lsiter, _IFT_IMLSolver_lsiter);
+        ir.setField(lsprecond, _IFT_IMLSolver_lsprecond);
+        ir.setField(mixed, _IFT_HipPCGSolver_mixed);
+        ir.setField(irtol, _IFT_HipPCGSolver_irtol);
         P->nr.rtolf = rtolf;
         if (maxiter > 0) P->nr.maxiter = maxiter;
         P->nr.manrmsteps = manrmsteps;
@@ -174,6 +175,51 @@ int hfem_problem_set_nr_extras(hfem_problem *p, int linesearch, double ls_tol, d
         nr.lsAmp = ls_amp;
         nr.lsMaxEta = ls_maxeta;
         nr.maxInc = maxinc;
+    });
+}
+
+int hfem_problem_set_nr_criteria(hfem_problem *p, double rtold, int miniter, int ls_keep_point)
+{
+    return guarded([&] {
+        if (miniter < 0) OOFEM_ERROR("miniter must be >= 0, got %d", miniter);
+        auto &nr = reinterpret_cast<StructuralProblem *>(p)->nr;
+        nr.rtold = rtold;
+        nr.minIterations = miniter;
+        nr.lsKeepPoint = ls_keep_point != 0;
+    });
+}
+
+int hfem_problem_nr_errors(hfem_problem *p, double *out, int cap, int *n)
+{
+    return guarded([&] {
+        const std::vector<double> &e = reinterpret_cast<StructuralProblem *>(p)->giveLastNRErrors();
+        *n = (int)(e.size() / 6);
+        for (int i = 0; i < std::min(cap, *n) * 6; i++) out[i] = e[(size_t)i];
+    });
+}
+
+int hfem_problem_update_internal_rhs(hfem_problem *p, double *f, double *enorm3)
+{
+    return guarded([&] {
+        auto *P = reinterpret_cast<StructuralProblem *>(p);
+        EngngModel *em = P;   // through the reference's virtual hook
+        FloatArray answer, eNorm;
+        em->updateInternalRHS(answer, P->giveCurrentStep(), em->giveDomain(1), &eNorm);
+        std::memcpy(f, answer.givePointer(), sizeof(double) * (size_t)answer.giveSize());
+        for (int k = 0; k < 3; k++) enorm3[k] = eNorm.at(k + 1);
+    });
+}
+
+int hfem_problem_assemble(hfem_problem *p, int response_mode)
+{
+    return guarded([&] {
+        auto *P = reinterpret_cast<StructuralProblem *>(p);
+        if (response_mode < 0 || response_mode > 2) OOFEM_ERROR("MatResponseMode %d is not a stiffness", response_mode);
+        SparseMtrx *K = P->giveStiffnessMatrix();
+        if (!K) OOFEM_ERROR("the problem has no stiffness matrix yet");
+        EngngModel *em = P;
+        em->assemble(*K, P->giveCurrentStep(), TangentAssembler((MatResponseMode)response_mode),
+                     EModelDefaultEquationNumbering(), em->giveDomain(1));
     });
 }
 

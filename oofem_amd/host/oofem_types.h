@@ -122,6 +122,12 @@ public:
     int operator[](int i) const { return v[(size_t)i]; }
     int *givePointer() { return v.data(); }
     const int *givePointer() const { return v.data(); }
+    void clear() { v.clear(); }
+    void followedBy(int b) { v.push_back(b); }
+    std::vector<int>::iterator begin() { return v.begin(); }
+    std::vector<int>::iterator end() { return v.end(); }
+    std::vector<int>::const_iterator begin() const { return v.begin(); }
+    std::vector<int>::const_iterator end() const { return v.end(); }
 };
 
 // column-major like the reference's FloatMatrix
@@ -154,29 +160,5 @@ public:
     virtual bool isDefault() const { return true; }
 };
 class EModelDefaultEquationNumbering : public UnknownNumberingScheme {};
-
-// src/core/inputrecord.h: keyword/value records (only what the plug-ins read)
-class InputRecord {
-public:
-    std::vector<std::pair<std::string, double>> fields;
-    bool has(const std::string &k) const
-    {
-        for (auto &f : fields)
-            if (f.first == k) return true;
-        return false;
-    }
-    double get(const std::string &k, double def) const
-    {
-        for (auto &f : fields)
-            if (f.first == k) return f.second;
-        return def;
-    }
-    void set(const std::string &k, double v)
-    {
-        for (auto &f : fields)
-            if (f.first == k) { f.second = v; return; }
-        fields.push_back({k, v});
-    }
-};
 
 }  // namespace oofem

@@ -7,6 +7,7 @@
 
 #include <memory>
 
+#include "inputrecord.h"
 #include "oofem_types.h"
 
 namespace oofem {

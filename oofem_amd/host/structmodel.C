@@ -62,8 +62,20 @@ StructuralProblem::StructuralProblem(int nnode_, const double *coords, int etype
     for (size_t i = 0; i < eqn.size(); i++)
         if (eqn[i] > 0) eqDofId[eqn[i] - 1] = (int8_t)(i % 3);
     for (DeviceVector *v : {&up, &upPrev, &dup, &RTp, &Fp}) v->allocate(ctx, npeq);
+    // one domain whose elements answer their location arrays from conn / eqn
+    domainList.push_back(std::make_unique<Domain>(1));
+    domainList[0]->setElements(nelem, [this, nen](int e, IntArray &loc, IntArray *dofIds) {
+        loc.resize(3 * nen);
+        if (dofIds) dofIds->resize(3 * nen);
+        for (int a = 0; a < nen; a++)
+            for (int d = 0; d < 3; d++) {
+                const int q = eqn[3 * (size_t)conn[(size_t)(e - 1) * nen + a] + d];
+                loc.at(3 * a + d + 1) = q > 0 ? q : 0;
+                if (dofIds) dofIds->at(3 * a + d + 1) = d + 1;
+            }
+    });
     // IMLSolver defaults unless the caller overrides them
-    solverRecord.set("lsprecond", 1);
+    solverRecord.setField(1, _IFT_IMLSolver_lsprecond);
 }
 
 StructuralProblem::~StructuralProblem()
@@ -71,20 +83,6 @@ StructuralProblem::~StructuralProblem()
     linSolver.reset();
     K.reset();
     if (ev) hfem_model_destroy(ev);
-}
-
-void StructuralProblem::giveElementLocationArrays(int di, std::vector<int> &loc, int &ne, int &ndofe)
-{
-    const int nen = (int)(conn.size() / std::max(1, nelem));
-    ne = nelem;
-    ndofe = 3 * nen;
-    loc.assign((size_t)nelem * ndofe, 0);
-    for (int e = 0; e < nelem; e++)
-        for (int a = 0; a < nen; a++)
-            for (int d = 0; d < 3; d++) {
-                const int q = eqn[3 * (size_t)conn[(size_t)e * nen + a] + d];
-                loc[(size_t)e * ndofe + 3 * a + d] = q > 0 ? q : 0;
-            }
 }
 
 void StructuralProblem::setPrescribedValues(const double *h) { up.upload(h); }
@@ -107,12 +105,44 @@ void StructuralProblem::ensureStructure()
     structureBuilt = true;
 }
 
-void StructuralProblem::assemble(SparseMtrx &answer, int mode)
+void StructuralProblem::assembleStiffness(SparseMtrx &answer, int mode)
 {
     auto *H = dynamic_cast<HipCSRSparseMtrx *>(&answer);
     if (!H) OOFEM_ERROR("the batched evaluator assembles into a HipCSRSparseMtrx only (got %s)", answer.giveClassName());
     hipfemCheck(hfem_model_assemble_stiffness(ev, H->handle(), mode), "assemble");
     if (&answer == K.get()) kMode = mode;
+}
+
+namespace {
+// reads TangentAssembler's protected response mode through a member pointer
+// named in a derived class (no change to the assembler's declaration)
+struct TangentModeOf : TangentAssembler {
+    static MatResponseMode get(const TangentAssembler &a) { return a.*(&TangentModeOf::rmode); }
+};
+}  // namespace
+
+void StructuralProblem::assemble(SparseMtrx &answer, TimeStep *tStep, const MatrixAssembler &ma,
+                                 const UnknownNumberingScheme &s, Domain *domain)
+{
+    auto *ta = dynamic_cast<const TangentAssembler *>(&ma);
+    if (!ta) OOFEM_ERROR("the batched evaluator assembles stiffness matrices (TangentAssembler) only");
+    if (domain != giveDomain(1)) OOFEM_ERROR("unknown domain");
+    const MatResponseMode rm = TangentModeOf::get(*ta);
+    assembleStiffness(answer, rm == TangentStiffness ? 1 : 0);
+}
+
+void StructuralProblem::updateInternalRHS(FloatArray &answer, TimeStep *tStep, Domain *d, FloatArray *eNorm)
+{
+    if (d != giveDomain(1)) OOFEM_ERROR("unknown domain");
+    if (tStep) tStep->incrementStateCounter();
+    double ebe[3];
+    internalForces(X, true, ebe);
+    answer.resize(neq);
+    F.download(answer.givePointer());
+    if (eNorm) {
+        eNorm->resize(3);
+        for (int k = 0; k < 3; k++) eNorm->at(k + 1) = ebe[k];
+    }
 }
 
 void StructuralProblem::internalForces(const DeviceVector &x, bool updateState, double *ebe3)
@@ -155,7 +185,9 @@ ConvergedReason StructuralProblem::linearSolve(const DeviceVector &b, DeviceVect
 ConvergedReason StructuralProblem::solveLinearStatic(int &iterations, double &residual)
 {
     ensureStructure();
-    assemble(*K, 0);
+    // LinearStatic assembles the stiffness with TangentAssembler(TangentStiffness)
+    // (linearstatic.C:199); for IsoLE that is the elastic matrix
+    assembleStiffness(*K, 0);
     X.zero();
     internalForces(X, false, nullptr);            // f_int of the prescribed values
     double sums[6];
@@ -177,12 +209,15 @@ ConvergedReason StructuralProblem::solveLinearStatic(int &iterations, double &re
 // (misesmat.C:496-504), so 1 reassembles the elastic matrix and 3 is 2.
 void StructuralProblem::assembleNewtonMatrix()
 {
+    // this->assemble(K, tStep, TangentAssembler(mode), numbering, domain) as
+    // NonLinearStatic::updateComponent does (nlinearstatic.C:625-641)
+    const EModelDefaultEquationNumbering dn;
     if (nr.stiffMode == 2 || nr.stiffMode == 3) {
-        if (kMode != 0) assemble(*K, 0);
+        if (kMode != 0) this->assemble(*K, &currentStep, TangentAssembler(ElasticStiffness), dn, giveDomain(1));
     } else if (nr.stiffMode == 1) {
-        assemble(*K, 0);
+        this->assemble(*K, &currentStep, TangentAssembler(SecantStiffness), dn, giveDomain(1));
     } else {
-        assemble(*K, 1);
+        this->assemble(*K, &currentStep, TangentAssembler(TangentStiffness), dn, giveDomain(1));
     }
 }
 
@@ -274,11 +309,12 @@ NRStepReport StructuralProblem::solveNonlinearStep()
     // forceErrVec / forceErrVecOld, nrsolver.C:612-623) for constrained Newton
     double feNew[3] = {0.0, 0.0, 0.0}, feOld[3] = {0.0, 0.0, 0.0};
     for (int nite = 0;; ++nite) {
-        double ebe[3], sums[6];
+        double ebe[3], sums[6], dsums[6] = {0, 0, 0, 0, 0, 0};
         internalForces(X, true, ebe);               // updateComponent(InternalRhs)
         hipfemCheck(hfem_model_residual(ev, RT.ptr(), F.ptr(), rhs.ptr(), sums), "residual");
         applyConstraintsToLoadIncrement(nite, sums);
-        // checkConvergence nrsolver.C:594-885, grouped by dof id; rtold is 0 (off)
+        if (nr.rtold > 0.0) hipfemCheck(hfem_model_group_sums(ev, ddX.ptr(), X.ptr(), dsums), "group sums");
+        // checkConvergence nrsolver.C:594-885, grouped by dof id
         bool converged = true, outOfRange = false;
         double maxErr = 0.0;
         if (nr.constrainedMinIter)
@@ -286,15 +322,28 @@ NRStepReport StructuralProblem::solveNonlinearStep()
                 feOld[dg] = feNew[dg];
                 feNew[dg] = 0.0;
             }
+        const size_t e0 = rep.errors.size();
+        rep.errors.resize(e0 + 6, -1.0);
         for (int dg = 0; dg < 3; dg++) {
             if (!idsInUse[dg]) continue;
-            const double denom = sums[3 + dg] + ebe[dg];
-            double forceErr = denom >= 1.e-6 ? std::sqrt(sums[dg] / denom) : std::sqrt(sums[dg]);
-            feNew[dg] = forceErr;
-            if (std::isnan(forceErr) || forceErr > nr.rtolf * 1.e20) outOfRange = true;
-            if (forceErr > nr.rtolf) converged = false;
-            if (forceErr > 0.0 && nite == 0) converged = false;
-            maxErr = std::max(maxErr, forceErr);
+            if (nr.rtolf > 0.0) {
+                const double denom = sums[3 + dg] + ebe[dg];
+                double forceErr = denom >= 1.e-6 ? std::sqrt(sums[dg] / denom) : std::sqrt(sums[dg]);
+                feNew[dg] = forceErr;
+                if (std::isnan(forceErr) || forceErr > nr.rtolf * 1.e20) outOfRange = true;
+                if (forceErr > nr.rtolf) converged = false;
+                if (forceErr > 0.0 && nite == 0) converged = false;
+                maxErr = std::max(maxErr, forceErr);
+                rep.errors[e0 + dg] = forceErr;
+            }
+            if (nr.rtold > 0.0) {
+                // displacement error (nrsolver.C:786-806): last correction over the total
+                const double dispErr = dsums[3 + dg] > 1.e-6 ? std::sqrt(dsums[dg] / dsums[3 + dg])
+                                                             : std::sqrt(dsums[dg]);
+                if (std::isnan(dispErr) || dispErr > nr.rtold * 1.e20) outOfRange = true;
+                if (dispErr > nr.rtold) converged = false;
+                rep.errors[e0 + 3 + dg] = dispErr;
+            }
         }
         rep.forceError = maxErr;
         rep.iterations = nite;
@@ -324,30 +373,41 @@ NRStepReport StructuralProblem::solveNonlinearStep()
         ConvergedReason ls = linearSolve(rhs, ddX);  // ddX carries the previous correction
         rep.linearIterations += lastLinearIterations;
         if (ls != CR_CONVERGED) { rep.status = CR_FAILED; break; }
+        const bool doLineSearch = nr.lineSearch && nite > 0;
         // constrained Newton (nrsolver.C:320-326): from iteration
         // constrainednrminiter + 1 on, halve the correction when the squared
         // norm of the force errors grew; not with the line search
-        if (nr.constrainedMinIter && !(nr.lineSearch && nite > 0) && nite > nr.constrainedMinIter) {
+        if (nr.constrainedMinIter && !doLineSearch && nite > nr.constrainedMinIter) {
             const double a = feNew[0] * feNew[0] + feNew[1] * feNew[1] + feNew[2] * feNew[2];
             const double b = feOld[0] * feOld[0] + feOld[1] * feOld[1] + feOld[2] * feOld[2];
             if (a > b) ddX.axpby(0.0, nullptr, 0.5);
         }
-        // increment cap (nrsolver.C:340-353), applied before the line search
-        // so that the search runs along the capped direction
-        if (nr.maxInc < 1.0e20) {
-            const double mx = ddX.amax();
-            if (mx > nr.maxInc) ddX.axpby(0.0, nullptr, nr.maxInc / mx);
-        }
-        if (nr.lineSearch && nite > 0) {
-            // X moves to the line search's point X_start + eta ddX (the
-            // reference's NRSolver then adds the scaled ddX once more,
-            // nrsolver.C:356; this driver keeps the line search's point)
-            lineSearch(rep.lineSearches);
+        // increment cap (nrsolver.C:332-345): after the line search in the
+        // reference's order, before it when the search keeps its point
+        auto capIncrement = [&]() {
+            if (nr.maxInc < 1.0e20) {
+                const double mx = ddX.amax();
+                if (mx > nr.maxInc) ddX.axpby(0.0, nullptr, nr.maxInc / mx);
+            }
+        };
+        if (!doLineSearch || nr.lsKeepPoint) capIncrement();
+        if (doLineSearch) {
+            // LineSearchNM::solve moves X to X_start + eta ddX (ddX scaled by
+            // eta) when it meets its tolerance, else to X_start + ddX
+            bool accepted = false;
+            lineSearch(rep.lineSearches, accepted);
+            if (!nr.lsKeepPoint) {
+                // NRSolver::solve then caps the correction and adds it to X
+                // (nrsolver.C:332-350): the search's step is passed twice
+                capIncrement();
+                X.axpby(1.0, &ddX, 1.0);
+            }
         } else {
             X.axpby(1.0, &ddX, 1.0);
         }
     }
     if (nr.forcing && pcg) pcg->setTolerance(lstol);
+    lastNRErrors = rep.errors;
     return rep;
 }
 
@@ -391,15 +451,18 @@ static void lsSearch(int istep, double *prod, double *eta, double amp, double ma
 }
 
 // LineSearchNM::solve linesearch.C:54-160 along ddX from the current X (whose
-// residual is in rhs): X ends at X_start + eta ddX with the Gauss-point state
-// of that point; ddX is scaled by eta.  Returns eta.
-double StructuralProblem::lineSearch(int &lineSearches)
+// residual is in rhs), with the directly controlled equations masked out of
+// the products (eqnmask): X ends at X_start + eta ddX and ddX is scaled by
+// eta when the search meets its tolerance (accepted), else X ends at
+// X_start + ddX.  Returns eta.
+double StructuralProblem::lineSearch(int &lineSearches, bool &accepted)
 {
     const double tol = std::min(std::max(nr.lsTol, 0.6), 0.95);
     const double amp = std::min(std::max(nr.lsAmp, 1.0), 10.0);
     const double maxeta = std::min(std::max(nr.lsMaxEta, 1.5), 15.0);
     const double s0 = -rhs.dot(ddX);
     double etav = 1.0;
+    bool ok = false;
     if (s0 < 0.0) {
         rb.axpby(1.0, &X, 0.0);
         constexpr int maxIter = 10;
@@ -408,13 +471,15 @@ double StructuralProblem::lineSearch(int &lineSearches)
         eta[1] = 0.0;
         eta[2] = 1.0;
         int ico = 0;
-        bool ok = false;
         for (int ils = 2; ils <= maxIter; ils++) {
             X.axpby(1.0, &rb, 0.0);
             X.axpby(eta[ils], &ddX, 1.0);
             internalForces(X, true, nullptr);
             double sums[6];
             hipfemCheck(hfem_model_residual(ev, RT.ptr(), F.ptr(), gls.ptr(), sums), "residual");
+            // the eqnmask equations (the directly controlled ones) leave the product
+            const double zero = 0.0;
+            for (int q : nr.ddmEq) hipfemCheck(hipfem_dev_upload(ctx, gls.ptr() + q - 1, &zero, 1), "ls mask");
             const double si = -gls.dot(ddX) / s0;
             prod[ils] = si;
             if (std::fabs(si) < tol) {
@@ -433,6 +498,7 @@ double StructuralProblem::lineSearch(int &lineSearches)
         ddX.axpby(0.0, nullptr, etav);
         lineSearches++;
     }
+    accepted = ok;
     return etav;
 }
 

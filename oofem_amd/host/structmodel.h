@@ -18,7 +18,9 @@
 #include <memory>
 #include <vector>
 
+#include "dynamicinputrecord.h"
 #include "hippcg.h"
+#include "structengngmodel.h"
 
 namespace oofem {
 
@@ -48,6 +50,7 @@ public:
 
 struct NRSolverOptions {
     double rtolf = 1.e-3;      // nrsolver.C:137 default
+    double rtold = 0.0;        // displacement criterion, off by default (nrsolver.C:138-141)
     int maxiter = 100000000;   // nsmax default 1e8 (nrsolver.C:106)
     int minIterations = 0;
     int manrmsteps = 0;        // 0: modified NR (default), k: tangent every k iterations
@@ -67,6 +70,11 @@ struct NRSolverOptions {
     // clamped as in LineSearchNM::initializeFrom, linesearch.C:247-283)
     bool lineSearch = false;
     double lsTol = 0.8, lsAmp = 2.5, lsMaxEta = 4.0;
+    // false: the reference's order after the search (nrsolver.C:318-350):
+    // the cap, then X += ddX, so the search's step is passed twice; true
+    // (no counterpart in the reference): the cap before the search, which
+    // then runs along the capped direction, and X stays at the search's point
+    bool lsKeepPoint = false;
     // cap on the largest component of a correction (NRSolver `maxinc`,
     // nrsolver.C:340-353)
     double maxInc = 1.0e20;
@@ -137,10 +145,13 @@ struct NRStepReport {
     int linearIterations = 0;  // PCG iterations summed over the step
     double forceError = 0.0;   // final max force error over dof groups
     int lineSearches = 0;      // line searches that ended with eta != 1
+    // every convergence check: force errors of the u, v, w groups, then their
+    // displacement errors (-1: group without free equations, or criterion off)
+    std::vector<double> errors;
     ConvergedReason status = CR_UNKNOWN;
 };
 
-class StructuralProblem : public EngngModel {
+class StructuralProblem : public StructuralEngngModel, public BatchedEvaluatorProvider {
     hipfem_ctx *ctx;
     int nnode, etype, nelem, nmat, neq, npeq;
     std::vector<int> conn, eqn;
@@ -148,7 +159,7 @@ class StructuralProblem : public EngngModel {
     hfem_model *ev = nullptr;
     SparseMtrxType smtype = SMT_HipCSR;
     LinSystSolverType lstype = ST_HipPCG;
-    InputRecord solverRecord;
+    DynamicInputRecord solverRecord{"hippcg"};
     std::unique_ptr<SparseMtrx> K;
     std::unique_ptr<SparseLinearSystemNM> linSolver;
     // state on the device
@@ -164,11 +175,14 @@ class StructuralProblem : public EngngModel {
     double currentTime = 0.0;   // target time of the last accepted step of solveTimeSteps
     int kMode = -1;             // stiffness the values of K hold: 0 elastic, 1 tangent, -1 none
     long smConstraintVersion = -1;   // K version after the last ddm penalty
+    TimeStep currentStep{1, 0.0};
+    void assembleStiffness(SparseMtrx &answer, int mode);
     void assembleNewtonMatrix();
     void applyConstraintsToStiffness();
     void applyConstraintsToLoadIncrement(int nite, double sums6[6]);
     bool structureBuilt = false;
     int lastLinearIterations = 0;
+    std::vector<double> lastNRErrors;   // NRStepReport::errors of the last step
     bool profiling = false;
     int profN64 = 0, profN32 = 0;
     double profMs64 = 0.0, profMs32 = 0.0;
@@ -176,7 +190,7 @@ class StructuralProblem : public EngngModel {
     void ensureStructure();
     void internalForces(const DeviceVector &x, bool updateState, double *ebe3);
     ConvergedReason linearSolve(const DeviceVector &b, DeviceVector &x);
-    double lineSearch(int &lineSearches);
+    double lineSearch(int &lineSearches, bool &accepted);
 
 public:
     NRSolverOptions nr;
@@ -187,19 +201,29 @@ public:
     ~StructuralProblem() override;
 
     // EngngModel hooks used by the plug-ins
-    int giveNumberOfDomainEquations(int di, const UnknownNumberingScheme &s) override { return neq; }
-    void giveElementLocationArrays(int di, std::vector<int> &loc, int &ne, int &ndofe) override;
+    int giveNumberOfDomainEquations(int di, const UnknownNumberingScheme &num) override { return neq; }
     hfem_model *giveBatchedEvaluator(int di) override { return ev; }
+    // EngngModel::assemble (engngm.h:957-958): one batched launch into a
+    // HipCSR matrix instead of the element loop; the TangentAssembler's
+    // response mode picks the consistent tangent (TangentStiffness) or the
+    // elastic matrix (ElasticStiffness; SecantStiffness answers with it too,
+    // misesmat.C:496-504)
+    void assemble(SparseMtrx &answer, TimeStep *tStep, const MatrixAssembler &ma, const UnknownNumberingScheme &s,
+                  Domain *domain) override;
+    // StructuralEngngModel::updateInternalRHS (structengngmodel.C:215-233):
+    // internal forces of the current solution (default numbering) through
+    // the batched evaluator, the Gauss-point temp state updated, and the
+    // per-dof-id EBE norms (u, v, w) in eNorm
+    void updateInternalRHS(FloatArray &answer, TimeStep *tStep, Domain *d, FloatArray *eNorm) override;
+    TimeStep *giveCurrentStep() { return &currentStep; }
 
     void setSparseMtrxType(int t) { smtype = (SparseMtrxType)t; }
     void setLinSystSolverType(int t) { lstype = (LinSystSolverType)t; }
-    InputRecord &giveSolverRecord() { return solverRecord; }
+    DynamicInputRecord &giveSolverRecord() { return solverRecord; }
 
     void setPrescribedValues(const double *up_host);               // npeq values for the step
     void setExternalLoad(const double *f_free, const double *f_presc);  // total external load
 
-    // EngngModel::assemble for the tangent (mode 1) or elastic (mode 0) stiffness
-    void assemble(SparseMtrx &answer, int mode);
 
     ConvergedReason solveLinearStatic(int &iterations, double &residual);
     NRStepReport solveNonlinearStep();
@@ -236,6 +260,7 @@ public:
     int giveNeq() const { return neq; }
     int giveNpeq() const { return npeq; }
     int giveLastLinearIterations() const { return lastLinearIterations; }
+    const std::vector<double> &giveLastNRErrors() const { return lastNRErrors; }
     // SpMV launch timing of the hippcg solver, summed over every solve since
     // setProfiling(true)
     void setProfiling(bool on);

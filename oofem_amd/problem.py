@@ -48,6 +48,10 @@ def hostlib():
         L.hfem_problem_save_converged.argtypes = [C.c_void_p]
         L.hfem_problem_restart_step.argtypes = [C.c_void_p]
         L.hfem_problem_set_nr_extras.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]
+        L.hfem_problem_set_nr_criteria.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int]
+        L.hfem_problem_nr_errors.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_void_p]
+        L.hfem_problem_update_internal_rhs.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+        L.hfem_problem_assemble.argtypes = [C.c_void_p, C.c_int]
         L.hfem_problem_set_arc_length.argtypes = [C.c_void_p, C.c_double, C.c_int, C.c_int, C.c_double, C.c_double,
                                                   C.c_double, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
                                                   C.c_int]
@@ -112,6 +116,34 @@ class StructuralProblem:
         lsearchmaxeta) and increment cap (maxinc)."""
         _check(hostlib().hfem_problem_set_nr_extras(self.h, int(bool(linesearch)), ls_tol, ls_amp, ls_maxeta, maxinc),
                "set_nr_extras")
+
+    def set_nr_criteria(self, rtold=0.0, miniter=0, ls_keep_point=False):
+        """NRSolver rtold and miniter; ls_keep_point: cap before the line
+        search and keep its point (the reference caps after it and adds the
+        searched step once more)."""
+        _check(hostlib().hfem_problem_set_nr_criteria(self.h, float(rtold), int(miniter), int(bool(ls_keep_point))),
+               "set_nr_criteria")
+
+    def nr_errors(self, cap=100000):
+        """Every convergence check of the last nonlinear step: rows of the
+        u, v, w force errors and displacement errors (-1: group unused or
+        criterion off)."""
+        n = C.c_int(0)
+        out = np.zeros((cap, 6))
+        _check(hostlib().hfem_problem_nr_errors(self.h, _p(out), cap, C.byref(n)), "nr_errors")
+        return out[:min(cap, n.value)].copy()
+
+    def update_internal_rhs(self):
+        """EngngModel::updateInternalRHS through the base-class hook: (f, eNorm)."""
+        f = np.zeros(self.deck.neq)
+        e = np.zeros(3)
+        _check(hostlib().hfem_problem_update_internal_rhs(self.h, _p(f), _p(e)), "update_internal_rhs")
+        return f, e
+
+    def assemble(self, response_mode=0):
+        """EngngModel::assemble through the base-class hook with
+        TangentAssembler(response_mode): 0 tangent, 1 secant, 2 elastic."""
+        _check(hostlib().hfem_problem_assemble(self.h, int(response_mode)), "assemble")
 
     def run_steps(self, up_ref, final_time, dt, reduction=None, f_ref=None, ltf=None, max_records=10000):
         """StaticStructural time stepping to final_time (hfem_problem_run_steps).

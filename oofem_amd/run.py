@@ -33,9 +33,11 @@ NRSolver's direct displacement control (``ddm`` node/dof pairs, ``ddv``
 values, ``ddltf`` function) is applied in both nonlinear drivers.
 
 Its solver keywords are read as the reference reads them: smtype, lstype,
-lstol, lsiter, lsprecond, rtolf, maxiter, manrmsteps, initialguess, and
-NRSolver's linesearch, lsearchtol, lsearchamp, lsearchmaxeta, maxinc and
-constrainednrminiter.
+lstol, lsiter, lsprecond, rtolv, rtolf, rtold, miniter, maxiter, manrmsteps,
+initialguess, and NRSolver's linesearch, lsearchtol, lsearchamp,
+lsearchmaxeta, maxinc and constrainednrminiter.  The line search runs in the
+reference's order (search, cap, then the correction added to X once more);
+``lskeeppoint 1``, a keyword of this build only, keeps the search's point.
 smtype and lstype select the plug-ins exactly as the class factory does, so
 a deck runs on this path when it says ``smtype 11 lstype 9``; any other type
 is rejected by the factory with an error.  ``keywords`` adds or replaces
@@ -89,12 +91,20 @@ def run_deck(path, out_path=None, keywords=None, log=None):
             return ms.scalar(key, default, cast)
         return e.scalar(key, default, cast)
 
+    def _has(key):
+        return key in over or (ms is not None and ms.has(key)) or e.has(key)
+
     # IMLSolver / NRSolver / EngngModel defaults (imlsolver.C:67-101,
     # nrsolver.C:106-200, staticstructural.C:130)
     kw = dict(smtype=kwd("smtype", 0, int), lstype=kwd("lstype", 0, int), lstol=kwd("lstol", 1e-5),
               lsiter=kwd("lsiter", 200, int), lsprecond=kwd("lsprecond", 0, int))
+    # NRSolver::initializeFrom nrsolver.C:133-148: rtolv sets both rtolf and
+    # rtold, and rtolf / rtold given on their own override it
+    rtolv = kwd("rtolv", None, float) if _has("rtolv") else None
+    nr_rtolf = kwd("rtolf", 1e-3 if rtolv is None else rtolv)
+    nr_rtold = kwd("rtold", 0.0 if rtolv is None else rtolv)
     if model == "staticstructural":
-        kw.update(rtolf=kwd("rtolf", 1e-3), maxiter=kwd("maxiter", 100000000, int),
+        kw.update(rtolf=nr_rtolf, maxiter=kwd("maxiter", 100000000, int),
                   manrmsteps=kwd("manrmsteps", 0, int), initial_guess=kwd("initialguess", 1, int))
     calm = None
     if model == "nonlinearstatic":
@@ -110,7 +120,7 @@ def run_deck(path, out_path=None, keywords=None, log=None):
                              "of this build")
         kw.update(initial_guess=kwd("initialguess", 0, int), stiff_mode=kwd("stiffmode", 0, int))
         if cmode == 1:
-            kw.update(rtolf=kwd("rtolf", 1e-3), maxiter=kwd("maxiter", 100000000, int),
+            kw.update(rtolf=nr_rtolf, maxiter=kwd("maxiter", 100000000, int),
                       manrmsteps=kwd("manrmsteps", 0, int))
         else:
             # CylindricalALM::initializeFrom calmls.C:681-897
@@ -137,6 +147,10 @@ def run_deck(path, out_path=None, keywords=None, log=None):
                         maxinc=kwd("maxinc", 1.0e20))
         if kwd("constrainednrminiter", 0, int):
             P.set_constrained_nr(kwd("constrainednrminiter", 0, int))
+        # rtold, miniter (nrsolver.C:113, 138-148); lskeeppoint is this
+        # build's keyword for the line search that keeps its own point
+        P.set_nr_criteria(rtold=nr_rtold, miniter=kwd("miniter", 0, int),
+                          ls_keep_point=bool(kwd("lskeeppoint", 0, int)))
     if calm is not None:
         P.set_arc_length(**calm)
         # hyperplane control (calmls.C:754-797): the hpc array holds (node

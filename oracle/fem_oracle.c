@@ -642,6 +642,15 @@ int orc_pcg(int n, const long long *rowptr, const int *colind, const double *val
             const double *b, double *x, double tol, int maxit, int precond,
             int *iters, double *resid)
 {
+    return orc_pcg_hist(n, rowptr, colind, val, b, x, tol, maxit, precond, iters, resid, NULL, 0);
+}
+
+/* orc_pcg that also stores ||r_k|| / ||b|| for k = 0, 1, ... (the values
+ * cg.h tests against tol) in hist[0..cap) */
+int orc_pcg_hist(int n, const long long *rowptr, const int *colind, const double *val,
+                 const double *b, double *x, double tol, int maxit, int precond,
+                 int *iters, double *resid, double *hist, int cap)
+{
     /* precond bit 0: DiagPreconditioner; bit 4: SpMV in SymCompCol order */
     void (*mv)(int, const long long *, const int *, const double *, const double *, double *) =
         (precond & 16) ? orc_spmv_symcompcol : orc_spmv;
@@ -665,6 +674,7 @@ int orc_pcg(int n, const long long *rowptr, const int *colind, const double *val
         for (int i = 0; i < n; i++) r[i] = b[i] - q[i];
         if (normb == 0.0) normb = 1;
         double res = sqrt(dotp(n, r, r)) / normb, rho_1 = 0.0;
+        if (hist && cap > 0) hist[0] = res;
         if (res <= tol) { *resid = res; *iters = 0; rc = 0; goto done; }
         for (int it = 1; it <= maxit; it++) {
             for (int i = 0; i < n; i++) z[i] = r[i] * dinv[i];
@@ -679,6 +689,7 @@ int orc_pcg(int n, const long long *rowptr, const int *colind, const double *val
             for (int i = 0; i < n; i++) x[i] += alpha * p[i];
             for (int i = 0; i < n; i++) r[i] -= alpha * q[i];
             res = sqrt(dotp(n, r, r)) / normb;
+            if (hist && it < cap) hist[it] = res;
             *resid = res;
             *iters = it;
             if (res <= tol) { rc = 0; goto done; }
@@ -1071,14 +1082,17 @@ int orc_nonlinear_step(const orc_mesh *m, double *state, double *X, const double
 }
 
 /* out-of-balance product -(RT - F(X + eta dr)) . dr with the state updated
- * (LineSearchNM::solve linesearch.C:94-118) */
+ * (LineSearchNM::solve linesearch.C:94-118; the eqnmask equations, the
+ * ddm ones, are left out of the product) */
 static int orc_ls_product(const orc_mesh *m, double *state, const double *rb, const double *dr, double eta,
-                          const double *u_presc, const double *RT, double *X, double *F, double *prod)
+                          const double *u_presc, const double *RT, const unsigned char *mask, double *X, double *F,
+                          double *prod)
 {
     for (int i = 0; i < m->neq; i++) X[i] = rb[i] + eta * dr[i];
     if (orc_internal_forces(m, X, u_presc, state, 1, F, NULL, NULL)) return -11;
     double s = 0.0;
-    for (int i = 0; i < m->neq; i++) s -= ((RT ? RT[i] : 0.0) - F[i]) * dr[i];
+    for (int i = 0; i < m->neq; i++)
+        if (!mask[i]) s -= ((RT ? RT[i] : 0.0) - F[i]) * dr[i];
     *prod = s;
     return 0;
 }
@@ -1089,7 +1103,50 @@ int orc_nonlinear_step_ex(const orc_mesh *m, double *state, double *X, const dou
                           double maxinc, int stiffmode, int nddm, const int *ddm_eq, const double *ddm_val,
                           double ddm_factor, int cnr_miniter, int *out, double *err)
 {
-    const int tangent = stiffmode == 2 ? 0 : 1;
+    orc_nr_opts o;
+    orc_nr_opts_default(&o);
+    o.initial_guess = initial_guess;
+    o.rtolf = rtolf;
+    o.maxiter = maxiter;
+    o.manrmsteps = manrmsteps;
+    o.lstol = lstol;
+    o.linesearch = linesearch;
+    o.ls_tol = ls_tol;
+    o.ls_amp = ls_amp;
+    o.ls_maxeta = ls_maxeta;
+    o.maxinc = maxinc;
+    o.stiffmode = stiffmode;
+    o.nddm = nddm;
+    o.ddm_eq = ddm_eq;
+    o.ddm_val = ddm_val;
+    o.ddm_factor = ddm_factor;
+    o.cnr_miniter = cnr_miniter;
+    return orc_nonlinear_step_opt(m, state, X, u_presc, u_presc_prev, RT, &o, out, err, NULL, 0);
+}
+
+void orc_nr_opts_default(orc_nr_opts *o)
+{
+    memset(o, 0, sizeof *o);
+    o->initial_guess = 1;
+    o->rtolf = 1.e-3;            /* nrsolver.C:137 */
+    o->rtold = 0.0;              /* off, nrsolver.C:138-141 */
+    o->maxiter = 100000000;      /* nsmax, nrsolver.C:106 */
+    o->lstol = 1.e-5;
+    o->ls_tol = 0.8;             /* linesearch.C:46-50 */
+    o->ls_amp = 2.5;
+    o->ls_maxeta = 4.0;
+    o->maxinc = 1.0e20;
+}
+
+/* NRSolver::solve nrsolver.C:205-404 and checkConvergence :594-885 (dof
+ * groups u, v, w), LineSearchNM::solve linesearch.C:54-160 */
+int orc_nonlinear_step_opt(const orc_mesh *m, double *state, double *X, const double *u_presc,
+                           const double *u_presc_prev, const double *RT, const orc_nr_opts *o, int *out, double *err,
+                           double *errhist, int errcap)
+{
+    const int tangent = (o->stiffmode == 2 || o->stiffmode == 1 || o->stiffmode == 3) ? 0 : 1;
+    const int nddm = o->nddm;
+    const int *ddm_eq = o->ddm_eq;
     /* NRSolver's constrained Newton (nrsolver.C:183-185, 320-326, 612-623):
      * from iteration cnr_miniter + 1 on, a correction is halved when the
      * squared norm of the per-dof-group force errors grew */
@@ -1102,20 +1159,23 @@ int orc_nonlinear_step_ex(const orc_mesh *m, double *state, double *X, const dou
     double *rhs = (double *)malloc(sizeof(double) * ((size_t)m->neq + 1));
     double *ddX = (double *)calloc((size_t)m->neq + 1, sizeof(double));
     int *dofid = (int *)malloc(sizeof(int) * ((size_t)m->neq + 1));
+    unsigned char *mask = (unsigned char *)calloc((size_t)m->neq + 1, 1);
     int inuse[3] = {0, 0, 0}, rc = 0;
     for (long long i = 0; i < 3LL * m->nnode; i++)
         if (m->eqn[i] > 0) { dofid[m->eqn[i] - 1] = (int)(i % 3); inuse[i % 3]++; }
+    for (int k = 0; k < nddm; k++) mask[ddm_eq[k] - 1] = 1;
     double *rb = (double *)malloc(sizeof(double) * ((size_t)m->neq + 1));
     orc_csr_fill(m, rowptr, colind);
     out[0] = out[1] = out[2] = out[3] = 0;
     /* LineSearchNM::initializeFrom clamps (linesearch.C:247-283) */
+    double ls_tol = o->ls_tol, ls_amp = o->ls_amp, ls_maxeta = o->ls_maxeta;
     if (ls_tol < 0.6) ls_tol = 0.6;
     if (ls_tol > 0.95) ls_tol = 0.95;
     if (ls_amp < 1.0) ls_amp = 1.0;
     if (ls_amp > 10.0) ls_amp = 10.0;
     if (ls_maxeta < 1.5) ls_maxeta = 1.5;
     if (ls_maxeta > 15.0) ls_maxeta = 15.0;
-    if (initial_guess && m->npeq > 0) {
+    if (o->initial_guess && m->npeq > 0) {
         double *du = (double *)malloc(sizeof(double) * (size_t)m->npeq);
         for (int i = 0; i < m->npeq; i++) du[i] = u_presc[i] - (u_presc_prev ? u_presc_prev[i] : 0.0);
         rc = orc_extrapolated_forces(m, state, du, rhs);
@@ -1127,7 +1187,7 @@ int orc_nonlinear_step_ex(const orc_mesh *m, double *state, double *X, const dou
             if (orc_assemble_stiffness(m, rowptr, colind, val, state, tangent)) { rc = -10; goto out; }
             int it = 0;
             double res = 0.0;
-            orc_pcg(m->neq, rowptr, colind, val, rhs, ddX, lstol, 1000000, 1, &it, &res);
+            orc_pcg(m->neq, rowptr, colind, val, rhs, ddX, o->lstol, 1000000, 1, &it, &res);
             out[1] += it;
             for (int i = 0; i < m->neq; i++) { X[i] += ddX[i]; ddX[i] = 0.0; }
         }
@@ -1137,94 +1197,124 @@ int orc_nonlinear_step_ex(const orc_mesh *m, double *state, double *X, const dou
     for (int nite = 0;; nite++) {
         double ebe[3];
         if (orc_internal_forces(m, X, u_presc, state, 1, F, NULL, ebe)) { rc = -11; goto out; }
-        double s_r[3] = {0, 0, 0}, s_t[3] = {0, 0, 0};
+        double s_r[3] = {0, 0, 0}, s_t[3] = {0, 0, 0}, s_dd[3] = {0, 0, 0}, s_x[3] = {0, 0, 0};
         for (int i = 0; i < m->neq; i++) rhs[i] = (RT ? RT[i] : 0.0) - F[i];
         /* NRSolver::applyConstraintsToLoadIncrement nrsolver.C:502-568 */
         for (int k = 0; k < nddm; k++) {
             const int q = ddm_eq[k] - 1;
-            rhs[q] = nite == 0 ? orc_csr_diag(rowptr, colind, val, q) * ddm_val[k] * ddm_factor : 0.0;
+            rhs[q] = nite == 0 ? orc_csr_diag(rowptr, colind, val, q) * o->ddm_val[k] * o->ddm_factor : 0.0;
         }
         for (int i = 0; i < m->neq; i++) {
             const double rt = RT ? RT[i] : 0.0;
             s_r[dofid[i]] += rhs[i] * rhs[i];
             s_t[dofid[i]] += rt * rt;
+            s_dd[dofid[i]] += ddX[i] * ddX[i];
+            s_x[dofid[i]] += X[i] * X[i];
         }
         int converged = 1, outofrange = 0;
         double mx = 0.0;
-        if (cnr_miniter) {
+        if (o->cnr_miniter) {
             for (int d = 0; d < 3; d++) { fe_old[d] = fe_new[d]; fe_new[d] = 0.0; }
         }
+        double *eh = (errhist && nite < errcap) ? errhist + 6 * (size_t)nite : NULL;
         for (int d = 0; d < 3; d++) {
+            if (eh) { eh[d] = -1.0; eh[3 + d] = -1.0; }
             if (!inuse[d]) continue;
-            const double den = s_t[d] + ebe[d];
-            const double fe = den >= 1.e-6 ? sqrt(s_r[d] / den) : sqrt(s_r[d]);
-            fe_new[d] = fe;
-            if (isnan(fe) || fe > rtolf * 1.e20) outofrange = 1;
-            if (fe > rtolf) converged = 0;
-            if (fe > 0.0 && nite == 0) converged = 0;
-            if (fe > mx) mx = fe;
+            if (o->rtolf > 0.0) {
+                /* force error of the group (nrsolver.C:754-782) */
+                const double den = s_t[d] + ebe[d];
+                const double fe = den >= 1.e-6 ? sqrt(s_r[d] / den) : sqrt(s_r[d]);
+                fe_new[d] = fe;
+                if (isnan(fe) || fe > o->rtolf * 1.e20) outofrange = 1;
+                if (fe > o->rtolf) converged = 0;
+                if (fe > 0.0 && nite == 0) converged = 0;
+                if (fe > mx) mx = fe;
+                if (eh) eh[d] = fe;
+            }
+            if (o->rtold > 0.0) {
+                /* displacement error of the group (nrsolver.C:786-806) */
+                const double de = s_x[d] > 1.e-6 ? sqrt(s_dd[d] / s_x[d]) : sqrt(s_dd[d]);
+                if (isnan(de) || de > o->rtold * 1.e20) outofrange = 1;
+                if (de > o->rtold) converged = 0;
+                if (eh) eh[3 + d] = de;
+            }
         }
         *err = mx;
         out[0] = nite;
         if (outofrange) { out[2] = -2; break; }
-        if (converged) { out[2] = 1; break; }
-        if (nite >= maxiter) { out[2] = -1; break; }
-        if (nite > 0 && manrmsteps > 0 && nite % manrmsteps == 0) {
+        if (converged && nite >= o->miniter) { out[2] = 1; break; }
+        if (nite >= o->maxiter) { out[2] = -1; break; }
+        if (nite > 0 && o->manrmsteps > 0 && nite % o->manrmsteps == 0) {
             if (orc_assemble_stiffness(m, rowptr, colind, val, state, tangent)) { rc = -10; goto out; }
             orc_ddm_stiffness(rowptr, colind, val, nddm, ddm_eq);
         }
         int it = 0;
         double res = 0.0;
-        orc_pcg(m->neq, rowptr, colind, val, rhs, ddX, lstol, 1000000, 1, &it, &res);
+        orc_pcg(m->neq, rowptr, colind, val, rhs, ddX, o->lstol, 1000000, 1, &it, &res);
         out[1] += it;
-        /* maxIncAllowed (nrsolver.C:340-353); the reference caps after the
-         * line search, this restatement caps the correction before it, so
-         * the search runs along the capped direction */
-        if (cnr_miniter && !(linesearch && nite > 0) && nite > cnr_miniter) {
+        const int do_ls = o->linesearch && nite > 0;
+        if (!do_ls && o->cnr_miniter && nite > o->cnr_miniter) {
             const double a = fe_new[0] * fe_new[0] + fe_new[1] * fe_new[1] + fe_new[2] * fe_new[2];
             const double b = fe_old[0] * fe_old[0] + fe_old[1] * fe_old[1] + fe_old[2] * fe_old[2];
             if (a > b) for (int i = 0; i < m->neq; i++) ddX[i] *= 0.5;
         }
         double mxi = 0.0;
-        for (int i = 0; i < m->neq; i++) if (fabs(ddX[i]) > mxi) mxi = fabs(ddX[i]);
-        if (mxi > maxinc) for (int i = 0; i < m->neq; i++) ddX[i] *= maxinc / mxi;
-        if (linesearch && nite > 0) {
-            /* LineSearchNM::solve linesearch.C:54-160.  The step ends at
-             * X = X_start + eta dr with the state of that point; the
-             * reference then also adds the scaled dr to X in NRSolver::solve
-             * (nrsolver.C:356), which counts the step twice -- this
-             * restatement (and the GPU driver) take the line search's point. */
+        if (o->ls_keep_point || !do_ls) {
+            /* maxIncAllowed (nrsolver.C:332-345).  The reference caps after
+             * the line search; ls_keep_point caps before it so that the
+             * search runs along the capped direction */
+            for (int i = 0; i < m->neq; i++) if (fabs(ddX[i]) > mxi) mxi = fabs(ddX[i]);
+            if (mxi > o->maxinc) for (int i = 0; i < m->neq; i++) ddX[i] *= o->maxinc / mxi;
+        }
+        if (do_ls) {
+            /* LineSearchNM::solve linesearch.C:54-160 with eqnmask = the ddm
+             * equations.  It moves X itself: to X_start + eta dr when the
+             * search meets its tolerance (dr scaled by eta), else to
+             * X_start + dr, with the state of that point. */
             double prod[12], eta[12];
             for (int i = 0; i < m->neq; i++) rb[i] = X[i];
             double s0 = 0.0;
-            for (int i = 0; i < m->neq; i++) s0 -= rhs[i] * ddX[i];
+            for (int i = 0; i < m->neq; i++)
+                if (!mask[i]) s0 -= ((RT ? RT[i] : 0.0) - F[i]) * ddX[i];
             double etav = 1.0;
+            int ok = 0;
             if (s0 < 0.0) {
                 const int max_iter = 10;
-                int ico = 0, ok = 0, ils;
+                int ico = 0, ils;
                 prod[1] = 1.0;
                 eta[1] = 0.0;
                 eta[2] = 1.0;
                 for (ils = 2; ils <= max_iter; ils++) {
                     double si = 0.0;
-                    if (orc_ls_product(m, state, rb, ddX, eta[ils], u_presc, RT, X, F, &si)) { rc = -11; goto out; }
+                    if (orc_ls_product(m, state, rb, ddX, eta[ils], u_presc, RT, mask, X, F, &si)) {
+                        rc = -11;
+                        goto out;
+                    }
                     si /= s0;
                     prod[ils] = si;
                     if (fabs(si) < ls_tol) { etav = eta[ils]; ok = 1; break; }
                     orc_ls_search(ils, prod, eta, ls_amp, ls_maxeta, 0.2, &ico);
                     if (ico == 2) break;
                 }
-                if (!ok) etav = 1.0;
             }
+            if (!ok) etav = 1.0;
             if (etav != 1.0) out[3]++;
             for (int i = 0; i < m->neq; i++) X[i] = rb[i] + etav * ddX[i];
-            for (int i = 0; i < m->neq; i++) ddX[i] *= etav;
+            if (ok) for (int i = 0; i < m->neq; i++) ddX[i] *= etav;
+            if (!o->ls_keep_point) {
+                /* reference order: the cap of the (scaled) correction after
+                 * the search (nrsolver.C:332-345), then X.add(ddX) (:349),
+                 * which passes the line search's step a second time */
+                for (int i = 0; i < m->neq; i++) if (fabs(ddX[i]) > mxi) mxi = fabs(ddX[i]);
+                if (mxi > o->maxinc) for (int i = 0; i < m->neq; i++) ddX[i] *= o->maxinc / mxi;
+                for (int i = 0; i < m->neq; i++) X[i] += ddX[i];
+            }
         } else {
             for (int i = 0; i < m->neq; i++) X[i] += ddX[i];
         }
     }
 out:
-    free(rowptr); free(colind); free(val); free(F); free(rhs); free(ddX); free(dofid); free(rb);
+    free(rowptr); free(colind); free(val); free(F); free(rhs); free(ddX); free(dofid); free(rb); free(mask);
     return rc;
 }
 

@@ -87,6 +87,10 @@ void orc_spmv_symcompcol(int n, const long long *rowptr, const int *colind, cons
 int orc_pcg(int n, const long long *rowptr, const int *colind, const double *val,
             const double *b, double *x, double tol, int maxit, int precond,
             int *iters, double *resid);
+/* the same, storing ||r_k||/||b|| for k = 0..min(iters, cap-1) in hist */
+int orc_pcg_hist(int n, const long long *rowptr, const int *colind, const double *val,
+                 const double *b, double *x, double tol, int maxit, int precond,
+                 int *iters, double *resid, double *hist, int cap);
 
 /* element loops over a displacement field */
 int orc_internal_forces(const orc_mesh *m, const double *u_free, const double *u_presc,
@@ -132,6 +136,39 @@ int orc_nonlinear_step_ex(const orc_mesh *m, double *state, double *X, const dou
                           int manrmsteps, double lstol, int linesearch, double ls_tol, double ls_amp, double ls_maxeta,
                           double maxinc, int stiffmode, int nddm, const int *ddm_eq, const double *ddm_val,
                           double ddm_factor, int cnr_miniter, int *out, double *err);
+
+/* NRSolver / StaticStructural step options (nrsolver.C:100-200 keywords and
+ * defaults; orc_nr_opts_default sets them) */
+typedef struct {
+    int initial_guess;     /* StaticStructural IG_Tangent over the prescribed increments */
+    double rtolf, rtold;   /* force / displacement criteria (<= 0: off) */
+    int maxiter, miniter, manrmsteps;
+    double lstol;          /* relative tolerance of every linear solve */
+    int linesearch;        /* LineSearchNM after every correction but the first */
+    double ls_tol, ls_amp, ls_maxeta;
+    int ls_keep_point;     /* 0: the reference's order (search, cap, X += ddX:
+                            * the search's step is passed twice); 1: cap
+                            * before the search and keep its point */
+    double maxinc;         /* maxIncAllowed */
+    int stiffmode;         /* 0 tangent, 1 secant, 2 elastic once, 3 secant once */
+    int nddm;
+    const int *ddm_eq;
+    const double *ddm_val;
+    double ddm_factor;
+    int cnr_miniter;       /* constrainednrminiter */
+} orc_nr_opts;
+
+void orc_nr_opts_default(orc_nr_opts *o);
+
+/* One step with options o.  out[4] = {Newton iterations, PCG iterations,
+ * status (1 converged, -1 maxiter, -2 out of range), line searches with
+ * eta != 1}; *err = the largest force error of the last check.  errhist
+ * (may be NULL) receives 6 values per iteration up to errcap iterations:
+ * the force errors of the u, v, w groups and their displacement errors, -1
+ * for a group with no free equation or a criterion that is off. */
+int orc_nonlinear_step_opt(const orc_mesh *m, double *state, double *X, const double *u_presc,
+                           const double *u_presc_prev, const double *RT, const orc_nr_opts *o, int *out, double *err,
+                           double *errhist, int errcap);
 
 /* Skyline U^T D U factorization + solve of a full symmetric CSR (direct
  * solver of the reference; CPU baseline only).  *nwk_out: profile size. */

@@ -152,6 +152,54 @@ def pcg(rowptr, colind, val, b, x0=None, tol=1e-10, maxit=100000, precond=1):
     return x, it.value, res.value, rc
 
 
+def pcg_hist(rowptr, colind, val, b, x0=None, tol=1e-10, maxit=100000, precond=1):
+    """orc_pcg with its residual history ||r_k||/||b||, k = 0..iters."""
+    n = rowptr.shape[0] - 1
+    x = np.zeros(n) if x0 is None else _d(x0).copy()
+    it = C.c_int(0)
+    res = C.c_double(0)
+    hist = np.zeros(min(maxit, 1 << 22) + 1)
+    rc = lib().orc_pcg_hist(n, rowptr.ctypes.data_as(C.c_void_p), colind.ctypes.data_as(C.c_void_p),
+                            val.ctypes.data_as(C.c_void_p), _d(b).ctypes.data_as(C.c_void_p),
+                            x.ctypes.data_as(C.c_void_p), C.c_double(tol), maxit, precond, C.byref(it),
+                            C.byref(res), hist.ctypes.data_as(C.c_void_p), hist.shape[0])
+    return x, it.value, res.value, rc, hist[:it.value + 1].copy()
+
+
+_REF_CG_PATH = os.path.join(_HERE, "_ref", "libref_iml_cg.so")
+
+
+def have_ref_cg():
+    """True when the reference's iml/cg.h has been compiled into oracle/_ref."""
+    return os.path.exists(_REF_CG_PATH)
+
+
+def ref_cg(rowptr, colind, val, b, x0=None, tol=1e-10, maxit=100000, storage=0, precond=1):
+    """The reference's own CG template (iml/cg.h, compiled unchanged into
+    oracle/_ref/libref_iml_cg.so by oracle/Makefile) on a symmetric CSR
+    matrix.  storage 0: SymCompCol product order (symcompcol.C:201-218),
+    1: CompCol order (compcol.C:128-142).  Returns (x, flag, iters, tol,
+    hist) with hist[k] = ||r_k||/||b|| as cg.h computes it."""
+    L = C.CDLL(_REF_CG_PATH)
+    n = rowptr.shape[0] - 1
+    x = np.zeros(n) if x0 is None else _d(x0).copy()
+    it = C.c_int(0)
+    t = C.c_double(0)
+    cap = min(maxit, 1 << 22) + 2
+    norms = np.zeros(cap)
+    nn = C.c_int(0)
+    flag = L.ref_iml_cg(n, np.ascontiguousarray(rowptr, np.int64).ctypes.data_as(C.c_void_p),
+                        np.ascontiguousarray(colind, np.int32).ctypes.data_as(C.c_void_p),
+                        _d(val).ctypes.data_as(C.c_void_p), _d(b).ctypes.data_as(C.c_void_p),
+                        x.ctypes.data_as(C.c_void_p), maxit, C.c_double(tol), storage, precond, C.byref(it),
+                        C.byref(t), norms.ctypes.data_as(C.c_void_p), cap, C.byref(nn))
+    if flag < 0:
+        raise RuntimeError(f"ref_iml_cg failed with {flag}")
+    rec = norms[:nn.value]
+    normb = rec[0] if rec[0] != 0.0 else 1.0
+    return x, flag, it.value, t.value, rec[1:] / normb
+
+
 def skyline_solve(rowptr, colind, val, b):
     """Profile U^T D U direct solve (Skyline, skyline.C:248-290,429-497).
     Returns (x, nwk)."""
@@ -249,15 +297,31 @@ def nonlinear_step(deck, state, X, u_presc, RT=None, rtolf=1e-8, maxiter=50, man
     return out[0], out[1], out[2], err.value
 
 
+class NROpts(C.Structure):
+    _fields_ = [("initial_guess", C.c_int), ("rtolf", C.c_double), ("rtold", C.c_double), ("maxiter", C.c_int),
+                ("miniter", C.c_int), ("manrmsteps", C.c_int), ("lstol", C.c_double), ("linesearch", C.c_int),
+                ("ls_tol", C.c_double), ("ls_amp", C.c_double), ("ls_maxeta", C.c_double),
+                ("ls_keep_point", C.c_int), ("maxinc", C.c_double), ("stiffmode", C.c_int), ("nddm", C.c_int),
+                ("ddm_eq", C.c_void_p), ("ddm_val", C.c_void_p), ("ddm_factor", C.c_double),
+                ("cnr_miniter", C.c_int)]
+
+
 def nonlinear_step_ex(deck, state, X, u_presc, u_presc_prev=None, initial_guess=True, RT=None, rtolf=1e-8,
                       maxiter=50, manrmsteps=1, lstol=1e-12, linesearch=False, ls_tol=0.8, ls_amp=2.5, ls_maxeta=4.0,
-                      maxinc=1e20, stiffmode=0, ddm=None, ddm_factor=0.0, constrainednrminiter=0):
+                      maxinc=1e20, stiffmode=0, ddm=None, ddm_factor=0.0, constrainednrminiter=0, rtold=0.0,
+                      miniter=0, ls_keep_point=False, history=False):
     """nonlinear_step with the NRSolver extras: line search (LineSearchNM,
-    linesearch.C:54-245), the increment cap maxinc (nrsolver.C:340-353) and
-    NonLinearStatic's stiffness mode (0 tangent, 2 elastic); ddm: (free
-    equations 1-based, values) of NRSolver's direct displacement control with
-    the step's increment factor ddm_factor.
-    Returns dict(newton, pcg, status, force_error, line_searches)."""
+    linesearch.C:54-245) in the reference's order (search, cap, X += ddX,
+    nrsolver.C:316-350) or, with ls_keep_point, capped before the search and
+    ending at the search's point; the increment cap maxinc; the displacement
+    criterion rtold and miniter; NonLinearStatic's stiffness mode (0
+    tangent, 1/3 secant = elastic, 2 elastic); ddm: (free equations
+    1-based, values) of NRSolver's direct displacement control with the
+    step's increment factor ddm_factor.
+    Returns dict(newton, pcg, status, force_error, line_searches[, errors]):
+    errors (with history) is an (iterations + 1) x 6 array of the u, v, w
+    force errors and displacement errors of every convergence check (-1:
+    group unused or criterion off)."""
     m = Mesh(deck)
     out = (C.c_int * 4)()
     err = C.c_double(0)
@@ -269,20 +333,25 @@ def nonlinear_step_ex(deck, state, X, u_presc, u_presc_prev=None, initial_guess=
         de = np.ascontiguousarray(ddm[0], dtype=np.int32)
         dv = _d(ddm[1])
         nd = len(de)
-    rc = lib().orc_nonlinear_step_ex(m.ref, state.ctypes.data_as(C.c_void_p), X.ctypes.data_as(C.c_void_p),
-                                     _d(u_presc).ctypes.data_as(C.c_void_p),
-                                     prev.ctypes.data_as(C.c_void_p) if prev is not None else None,
-                                     int(bool(initial_guess)),
-                                     rt.ctypes.data_as(C.c_void_p) if rt is not None else None, C.c_double(rtolf),
-                                     maxiter, manrmsteps, C.c_double(lstol), int(bool(linesearch)), C.c_double(ls_tol),
-                                     C.c_double(ls_amp), C.c_double(ls_maxeta), C.c_double(maxinc), int(stiffmode),
-                                     nd, de.ctypes.data_as(C.c_void_p) if nd else None,
-                                     dv.ctypes.data_as(C.c_void_p) if nd else None, C.c_double(ddm_factor),
-                                     int(constrainednrminiter), out,
-                                     C.byref(err))
+    o = NROpts(int(bool(initial_guess)), float(rtolf), float(rtold), int(maxiter), int(miniter), int(manrmsteps),
+               float(lstol), int(bool(linesearch)), float(ls_tol), float(ls_amp), float(ls_maxeta),
+               int(bool(ls_keep_point)), float(maxinc), int(stiffmode), nd,
+               de.ctypes.data if nd else None, dv.ctypes.data if nd else None, float(ddm_factor),
+               int(constrainednrminiter))
+    cap = min(int(maxiter), 10000) + 1
+    hist = np.zeros((cap, 6)) if history else None
+    rc = lib().orc_nonlinear_step_opt(m.ref, state.ctypes.data_as(C.c_void_p), X.ctypes.data_as(C.c_void_p),
+                                      _d(u_presc).ctypes.data_as(C.c_void_p),
+                                      prev.ctypes.data_as(C.c_void_p) if prev is not None else None,
+                                      rt.ctypes.data_as(C.c_void_p) if rt is not None else None, C.byref(o), out,
+                                      C.byref(err), hist.ctypes.data_as(C.c_void_p) if history else None,
+                                      cap if history else 0)
     if rc:
-        raise RuntimeError(f"orc_nonlinear_step_ex rc={rc}")
-    return dict(newton=out[0], pcg=out[1], status=out[2], force_error=err.value, line_searches=out[3])
+        raise RuntimeError(f"orc_nonlinear_step_opt rc={rc}")
+    r = dict(newton=out[0], pcg=out[1], status=out[2], force_error=err.value, line_searches=out[3])
+    if history:
+        r["errors"] = hist[:out[0] + 1].copy()
+    return r
 
 
 def static_structural(deck, up_ref, final_time, dt, ltf=None, reduction=None, **nr):

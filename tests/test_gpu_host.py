@@ -160,8 +160,9 @@ def test_tangent_product_matches_oracle(ctx, tangent):
 
 
 def test_line_search_matches_oracle():
-    """NRSolver with LineSearchNM on the GPU driver: a single step to 3/8 of
-    the block's end displacement without the tangent predictor converges
+    """NRSolver with LineSearchNM on the GPU driver (search's point kept): a
+    single step to 3/8 of the block's end displacement without the tangent
+    predictor converges
     with the oracle's Newton iteration and line-search counts, at its
     displacements (1e-8) and plastic strains."""
     from oofem_amd.problem import StructuralProblem
@@ -169,10 +170,12 @@ def test_line_search_matches_oracle():
     up = d.prescribed_values(3 / 8)
     st = oracle.state_new(d)
     X = np.zeros(d.neq)
-    ref = oracle.nonlinear_step_ex(d, st, X, up, initial_guess=False, maxiter=40, lstol=1e-12, linesearch=True)
+    ref = oracle.nonlinear_step_ex(d, st, X, up, initial_guess=False, maxiter=40, lstol=1e-12, linesearch=True,
+                                   ls_keep_point=True)
     assert ref["status"] == 1 and ref["line_searches"] >= 1
     P = StructuralProblem(d, lstol=1e-12, rtolf=1e-8, maxiter=40, manrmsteps=1, initial_guess=0)
     P.set_nr_extras(linesearch=True)
+    P.set_nr_criteria(ls_keep_point=True)
     P.set_load(np.zeros(d.neq), np.zeros(d.npeq))
     P.set_prescribed(up)
     rep = P.nonlinear_step()

@@ -14,6 +14,10 @@ from oofem_amd.deck import evaluate_checks, read_deck
 from oracle import oracle
 from tests.conftest import GOLDEN
 from tests.deckrun import run_deck_gpu, run_deck_oracle
+from tests.imlcases import CASES as IML_CASES
+from tests.imlcases import fixture as _iml_fixture
+
+IML = _iml_fixture()
 
 pytestmark = pytest.mark.gpu
 
@@ -125,6 +129,35 @@ def test_config0_linear_static_parity(ctx):
     assert abs(reaction_sums(d, got["reactions"])[1] - 1.0) < 1e-9
 
 
+@pytest.mark.parametrize("case", list(IML_CASES))
+def test_pcg_history_matches_reference_cg(ctx, case):
+    """hippcg against the reference's own CG template (iml/cg.h compiled from
+    the reference tree; its residual history is the committed fixture):
+    iterations within 1, the first 50 residuals ||r_k||/||b|| to 1e-10
+    relative, every residual within 10 % (round-off of the different
+    dot-product order grows late in the solve)."""
+    from oofem_amd.hipfem import BatchedEvaluator, HipCSRSparseMtrx, HipPCGSolver
+    d = IML_CASES[case]()
+    ref = IML[case]
+    K = HipCSRSparseMtrx(ctx)
+    ev = BatchedEvaluator(ctx, d)
+    ev.build_structure(K)
+    ev.assemble_stiffness(K)
+    f, _ = oracle.surface_load(d)
+    S = HipPCGSolver(ctx, lstol=1e-10, lsiter=100000, lsprecond=1)
+    S.set_history(100000)
+    u = np.zeros(d.neq)
+    reason, it, res = S.solve(K, f, u)
+    h = S.history()
+    assert reason == 1 and res <= 1e-10
+    assert len(h) == it + 1 and h[-1] == res
+    href = np.array(ref["hist"])
+    assert abs(it - ref["iters"]) <= 1, (it, ref["iters"])
+    np.testing.assert_allclose(h[:50], href[:50], rtol=1e-10)
+    m = min(len(h), len(href))
+    assert np.max(np.abs(h[:m] / href[:m] - 1.0)) < 0.1
+
+
 def test_config0_tight_solve_matches_direct(ctx):
     """Both sides solved to 1e-13: displacements agree to 1e-12."""
     from oofem_amd.drivers import linear_static
@@ -150,16 +183,22 @@ def test_reference_decks_on_gpu(ctx, name):
 def _mises_deck(etype):
     if etype == meshgen.LTRSPACE:
         coords, conn = meshgen.kuhn_tets(3, 3, 4, 1.0, 1.0, 1.3, jitter=0.1, seed=5)
+    elif etype == meshgen.QSPACE:
+        coords, conn = meshgen._rve_mesh(2, meshgen.QSPACE)
+        coords = coords * np.array([1.0, 1.0, 1.3])
+        rng = np.random.default_rng(5)
+        inner = np.all((coords > 1e-9) & (coords < np.array([1.0, 1.0, 1.3]) - 1e-9), axis=1)
+        coords[inner] += rng.uniform(-0.03, 0.03, (int(inner.sum()), 3))
     else:
         d0 = meshgen.cantilever_hex(3, 3, 4, lz=1.3, jitter=0.2, seed=5)
         coords, conn = d0.coords, d0.conn
     bc = np.zeros((coords.shape[0], 3), bool)
-    bc[:16] = True
+    bc[coords[:, 2] <= 1e-12] = True
     mat = np.array([[meshgen.MAT_MISES, 210e3, 0.3, 250.0, 2e3, 0, 0, 0]])
     return meshgen.Deck("mises", etype, coords, conn, np.zeros(len(conn), np.int32), mat, bc, np.zeros_like(coords))
 
 
-@pytest.mark.parametrize("etype", [meshgen.LSPACE, meshgen.LTRSPACE])
+@pytest.mark.parametrize("etype", [meshgen.LSPACE, meshgen.LTRSPACE, meshgen.QSPACE])
 def test_mises_internal_forces_state_and_tangent(ctx, etype):
     from oofem_amd.hipfem import BatchedEvaluator, HipCSRSparseMtrx
     d = _mises_deck(etype)

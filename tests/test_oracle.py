@@ -2,10 +2,14 @@
 mathematics.  No GPU needed.
 
 Pins, and the mistakes each one catches:
-- config 0 on the compiled reference: 197 IML-CG iterations to 1e-10, nnz
-  1,020,582, sum of y-reactions 1.0000000000018 -- a wrong K entry, load
-  vector, preconditioner or stopping rule moves the iteration count (the
-  residual at 197 is 0.02% under the tolerance) or the reaction sum;
+- the reference's own CG template, iml/cg.h compiled unchanged from the
+  reference tree (oracle/_ref): the oracle's CG reproduces its iteration
+  count and residual history bit for bit on config 0 (197 iterations to
+  1e-10) and on a jittered 8x8x24 block (350); the committed history
+  fixture carries that pin to machines without the reference -- a wrong
+  stopping rule, preconditioner, K entry or load vector moves the history
+  (a planted preconditioned-norm stop rule misses it by 30 %);
+- config 0: global equilibrium of the reactions with the applied traction;
 - patch301 (LTRSpace, exact solution, tol 1e-5) and cantilever_Qspace
   (QSpace, tol 1e-8) decks of the reference, unchanged;
 - element invariants: rigid-body null space (6 modes), symmetry, exact
@@ -27,6 +31,11 @@ from oofem_amd.deck import evaluate_checks, read_deck
 from oracle import oracle
 from tests.conftest import GOLDEN
 from tests.deckrun import run_deck_oracle
+from tests import nrpins
+from tests.imlcases import CASES as IML_CASES
+from tests.imlcases import fixture as _iml_fixture
+
+IML = _iml_fixture()
 
 
 @pytest.fixture(scope="module")
@@ -44,13 +53,95 @@ def test_config0_counts(cfg0):
 def test_config0_reference_iterations_and_reactions(cfg0):
     r = oracle.linear_static(cfg0, tol=1e-10)
     assert r["rc"] == 0
-    assert r["iters"] == 197
+    # the reference's own CG template reaches lstol 1e-10 in this many iterations
+    assert r["iters"] == IML["config0"]["iters"] == 197
     assert 9.9e-11 < r["resid"] <= 1e-10
     R = r["reactions"]
     m = cfg0.eqn < 0
     ydir = np.nonzero(m)[1] == 1
     sum_y = R[(-cfg0.eqn[m] - 1)[ydir]].sum()
-    assert abs(sum_y - 1.0000000000018) < 5e-13
+    # global equilibrium: the clamp carries the 1 MPa x 1 m^2 traction in -y,
+    # up to the CG residual (1e-10 relative)
+    assert abs(sum_y - 1.0) < 1e-9
+
+
+@pytest.mark.parametrize("case", list(IML_CASES))
+def test_reference_cg_template_equals_oracle(case):
+    """iml/cg.h compiled from the reference tree (oracle/_ref) and the oracle's
+    CG in the same product order: the same iterations and the same residual
+    history, bit for bit; the committed fixture is that history."""
+    if not oracle.have_ref_cg():
+        pytest.skip("oracle/_ref/libref_iml_cg.so is built only where the reference tree is present")
+    d = IML_CASES[case]()
+    rp, ci, val = oracle.csr(d)
+    f, _ = oracle.surface_load(d)
+    xr, flag, it, tol, h = oracle.ref_cg(rp, ci, val, f, tol=1e-10, storage=0, precond=1)
+    xo, ito, reso, rc, ho = oracle.pcg_hist(rp, ci, val, f, tol=1e-10, precond=1 | 16)
+    assert flag == 0 and rc == 0
+    assert it == ito
+    np.testing.assert_array_equal(h, ho)
+    np.testing.assert_array_equal(xr, xo)
+    assert tol == reso
+    assert it == IML[case]["iters"]
+    np.testing.assert_array_equal(h, np.array(IML[case]["hist"]))
+    # CompCol order: same iterations, history equal up to the product's rounding
+    _, flag1, it1, _, h1 = oracle.ref_cg(rp, ci, val, f, tol=1e-10, storage=1, precond=1)
+    assert flag1 == 0 and it1 == it
+    np.testing.assert_allclose(h1[:50], h[:50], rtol=1e-10)
+
+
+def _history_close(h, ref, iters_ref):
+    """The bar a CG with another product / dot order meets against the
+    reference history: iterations within 1, the first 50 residuals to
+    1e-10 relative (round-off has not grown yet), all within 10 %."""
+    m = min(len(h), len(ref))
+    assert abs((len(h) - 1) - iters_ref) <= 1
+    np.testing.assert_allclose(h[:50], ref[:50], rtol=1e-10)
+    assert np.max(np.abs(h[:m] / ref[:m] - 1.0)) < 0.1
+
+
+@pytest.mark.parametrize("case", list(IML_CASES))
+def test_oracle_cg_history_matches_reference_fixture(case):
+    d = IML_CASES[case]()
+    rp, ci, val = oracle.csr(d)
+    f, _ = oracle.surface_load(d)
+    ref = np.array(IML[case]["hist"])
+    _, it, res, rc, h = oracle.pcg_hist(rp, ci, val, f, tol=1e-10, precond=1 | 16)
+    assert rc == 0 and it == IML[case]["iters"]
+    np.testing.assert_array_equal(h, ref)
+    _, it, res, rc, h = oracle.pcg_hist(rp, ci, val, f, tol=1e-10, precond=1)   # row-order product
+    assert rc == 0 and res <= 1e-10
+    _history_close(h, ref, IML[case]["iters"])
+
+
+def test_reference_history_catches_a_wrong_stop_rule():
+    """Planted mistake, on the CPU: CG stopped on the preconditioned norm
+    sqrt(r.z)/sqrt(b.M^-1 b) instead of cg.h's ||r||/||b|| misses the
+    fixture's history by far more than the bar of _history_close."""
+    d = IML_CASES["config0"]()
+    rp, ci, val = oracle.csr(d)
+    f, _ = oracle.surface_load(d)
+    A = sp.csr_matrix((val, ci, rp))
+    dinv = 1.0 / A.diagonal()
+    r = f.copy()
+    z = dinv * r
+    p = z.copy()
+    rho = r @ z
+    nb = np.sqrt(f @ (dinv * f))
+    hist = [np.sqrt(rho) / nb]
+    for _ in range(400):
+        q = A @ p
+        a = rho / (p @ q)
+        r -= a * q
+        z = dinv * r
+        rn = r @ z
+        hist.append(np.sqrt(rn) / nb)
+        if hist[-1] <= 1e-10:
+            break
+        p = z + (rn / rho) * p
+        rho = rn
+    with pytest.raises(AssertionError):
+        _history_close(np.array(hist), np.array(IML["config0"]["hist"]), IML["config0"]["iters"])
 
 
 def test_config0_cg_vs_direct(cfg0):
@@ -432,23 +523,99 @@ def test_rve_batch_block_rows_and_config4_specs():
 def test_line_search_converges_where_plain_newton_fails():
     """A single step from the virgin state to 3/8 of the block's end
     displacement without the tangent predictor: plain Newton does not
-    converge in 40 iterations, Newton with LineSearchNM does (with at least
-    one eta != 1), and it reaches the root that Newton from the tangent
-    predictor reaches (the step's discrete equations do not depend on the
-    iteration path)."""
+    converge in 40 iterations; Newton with LineSearchNM in the reference's
+    order (the searched step added to X a second time) does not either;
+    with the search's point kept it does (with at least one eta != 1), and
+    it reaches the root that Newton from the tangent predictor reaches (the
+    step's discrete equations do not depend on the iteration path)."""
     d = meshgen.config3(8, 8, 32)
     up = d.prescribed_values(3 / 8)
     res = {}
-    for name, kw in (("plain", dict(initial_guess=False)), ("ls", dict(initial_guess=False, linesearch=True)),
+    for name, kw in (("plain", dict(initial_guess=False)), ("ls_ref", dict(initial_guess=False, linesearch=True)),
+                     ("ls", dict(initial_guess=False, linesearch=True, ls_keep_point=True)),
                      ("ig", dict(initial_guess=True))):
         st = oracle.state_new(d)
         X = np.zeros(d.neq)
         r = oracle.nonlinear_step_ex(d, st, X, up, maxiter=40, lstol=1e-12, **kw)
         res[name] = (r, X)
     assert res["plain"][0]["status"] == -1
+    assert res["ls_ref"][0]["status"] == -1
     assert res["ls"][0]["status"] == 1 and res["ls"][0]["line_searches"] >= 1
     assert res["ig"][0]["status"] == 1
     assert np.linalg.norm(res["ls"][1] - res["ig"][1]) <= 1e-8 * np.linalg.norm(res["ig"][1])
+
+
+def test_force_criterion_closed_form_traction_step():
+    """First check of a traction step from rest: the residual is the load,
+    the v group's error is exactly 1 (the load norm in the denominator) and
+    the unloaded groups fall back to the absolute norm 0."""
+    d = nrpins.traction_block()
+    f, _ = d.external_load()
+    r = oracle.nonlinear_step_ex(d, oracle.state_new(d), np.zeros(d.neq), d.prescribed_values(), RT=f,
+                                 initial_guess=False, rtolf=1e-8, maxiter=5, lstol=1e-13, history=True)
+    assert r["status"] == 1 and r["newton"] == 1
+    np.testing.assert_array_equal(r["errors"][0], [0.0, 1.0, 0.0, -1.0, -1.0, -1.0])
+
+
+@pytest.mark.parametrize("nx,ny", [(2, 3), (3, 2), (1, 1), (4, 1)])
+def test_force_criterion_closed_form_uniaxial_layer(nx, ny):
+    """First check of a prescribed-displacement step: the per-dof-group
+    force errors of the closed form (EBE norms over every element entry)."""
+    d = nrpins.uniaxial_layer(nx, ny)
+    r = oracle.nonlinear_step_ex(d, oracle.state_new(d), np.zeros(d.neq), d.prescribed_values(),
+                                 initial_guess=False, rtolf=1e-8, maxiter=5, lstol=1e-13, history=True)
+    assert r["status"] == 1
+    np.testing.assert_allclose(r["errors"][0, :3], nrpins.uniaxial_layer_errors(nx, ny), rtol=1e-13)
+
+
+def _linear_block_root():
+    d = nrpins.linear_block()
+    f, _ = d.external_load()
+    rp, ci, val = oracle.csr(d)
+    Xs, _, _, rc = oracle.pcg(rp, ci, val, f, tol=1e-14)
+    assert rc == 0
+    return d, f, Xs
+
+
+@pytest.mark.parametrize("mode,maxiter,factor,status,newton", [
+    ("reference", 2, 1.25, -1, 2), ("reference", 5, 0.75, -1, 5),
+    ("keep_point", 10, 1.0, 1, 4), ("no_search", 10, 1.0, 1, 4)])
+def test_line_search_and_maxinc_closed_form(mode, maxiter, factor, status, newton):
+    """Linear block, maxinc = max|X*|/4, corrections solved exactly.
+    Reference order (search, cap, X += ddX): X = X*/4, then the search
+    accepts eta = 1 at X*, the capped step X*/4 is added again -> 1.25 X*,
+    then 0.75 X*, 1.25 X*, ... (the searched step counted twice keeps the
+    error at X*/4).  Cap before the search, search's point kept: X*/4,
+    X*/2, 3X*/4, X*, converged at the fourth check; without a search the
+    same path."""
+    d, f, Xs = _linear_block_root()
+    m0 = np.abs(Xs).max()
+    X = np.zeros(d.neq)
+    kw = dict(linesearch=mode != "no_search", ls_keep_point=mode == "keep_point")
+    r = oracle.nonlinear_step_ex(d, oracle.state_new(d), X, d.prescribed_values(), RT=f, initial_guess=False,
+                                 rtolf=1e-8, maxiter=maxiter, lstol=1e-14, maxinc=0.25 * m0, **kw)
+    assert r["status"] == status and r["newton"] == newton
+    assert np.abs(X - factor * Xs).max() <= 1e-9 * m0
+
+
+def test_displacement_criterion_closed_form():
+    """rtold on a linear step: the first check has no correction yet (0),
+    the second sees ddX = X (error 1 in every group whose sum X^2 exceeds
+    1e-6), the third a round-off correction; rtolf alone stops after one."""
+    d, f, Xs = _linear_block_root()
+    r = oracle.nonlinear_step_ex(d, oracle.state_new(d), np.zeros(d.neq), d.prescribed_values(), RT=f,
+                                 initial_guess=False, rtolf=1e-8, rtold=1e-6, maxiter=10, lstol=1e-14, history=True)
+    e = r["errors"]
+    assert r["status"] == 1 and r["newton"] == 2
+    grp = np.nonzero(d.eqn > 0)[1][np.argsort(d.eqn[d.eqn > 0])]
+    sx = np.array([(Xs[grp == k] ** 2).sum() for k in range(3)])
+    assert (sx > 1e-6).all()
+    np.testing.assert_array_equal(e[0, 3:], [0.0, 0.0, 0.0])
+    np.testing.assert_allclose(e[1, 3:], [1.0, 1.0, 1.0], rtol=1e-12)
+    assert (e[2, 3:] < 1e-10).all()
+    r1 = oracle.nonlinear_step_ex(d, oracle.state_new(d), np.zeros(d.neq), d.prescribed_values(), RT=f,
+                                  initial_guess=False, rtolf=1e-8, maxiter=10, lstol=1e-14)
+    assert r1["newton"] == 1
 
 
 def test_maxinc_caps_every_correction():

@@ -127,6 +127,10 @@ int hipcsr_times_mat(hipcsr_mtx *m, int ncol, const double *B, double *Y);
 int hipcsr_get(hipcsr_mtx *m, int i, int j, double *v);
 int hipcsr_set(hipcsr_mtx *m, int i, int j, double v);
 int hipcsr_is_allocated(hipcsr_mtx *m, int i, int j);    /* isAllocatedAt :270 */
+/* position of entry (i, j) (1-based) in the value array of hipcsr_download,
+ * -1 when it is not in the pattern (the monotone search of CompCol::at,
+ * compcol.C:339) */
+int hipcsr_find(const hipcsr_mtx *m, int i, int j, int64_t *pos);
 
 int     hipcsr_nrows(const hipcsr_mtx *m);
 int64_t hipcsr_nnz(const hipcsr_mtx *m);

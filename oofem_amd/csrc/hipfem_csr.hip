@@ -1058,6 +1058,14 @@ extern "C" int hipcsr_get(hipcsr_mtx *m, int i, int j, double *v)
     return 0;
 }
 
+extern "C" int hipcsr_find(const hipcsr_mtx *m, int i, int j, int64_t *pos)
+{
+    HIPFEM_REQUIRE(m && m->has_structure && pos, HIPFEM_EINVAL, "find: bad arguments");
+    HIPFEM_REQUIRE(i >= 1 && j >= 1 && i <= m->n && j <= m->n, HIPFEM_ERANGE, "at(%d,%d) outside %dx%d", i, j, m->n, m->n);
+    *pos = find_pos(m, i - 1, j - 1);
+    return 0;
+}
+
 extern "C" int hipcsr_set(hipcsr_mtx *m, int i, int j, double v)
 {
     HIPFEM_REQUIRE(m && m->has_structure, HIPFEM_EINVAL, "at: no structure");

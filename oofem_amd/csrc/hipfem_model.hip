@@ -682,12 +682,17 @@ __global__ __launch_bounds__(64) void k_internal(const int *__restrict__ elems, 
 // m = 0.  Fields: 0 c, 1 K, 2-7 m.
 __global__ void k_tangent_desc(int ngp_total, int ngp, const int *__restrict__ emat, const double *__restrict__ mat,
                                const double *__restrict__ state, int mode, float *__restrict__ desc,
-                               const int8_t *__restrict__ sflag)
+                               const int8_t *__restrict__ sflag, const float *__restrict__ dsur,
+                               float *__restrict__ dsnap)
 {
     const int gp = blockIdx.x * blockDim.x + threadIdx.x;
     if (gp >= ngp_total) return;
     if (mode == 1 && sflag && sflag[gp]) {
-        desc[gp] = -1.0f;   // general D: the surrogate's tangent (read by the operator)
+        // general D: the surrogate's tangent of this assembly, kept for the
+        // operator (the live one changes with every state update)
+        desc[gp] = -1.0f;
+#pragma unroll
+        for (int k = 0; k < 21; k++) dsnap[(size_t)k * ngp_total + gp] = dsur[(size_t)k * ngp_total + gp];
         return;
     }
     const double *mp = mat + (size_t)HIPFEM_MATP * emat[gp / ngp];
@@ -1185,6 +1190,7 @@ struct hfem_model {
     float sur_scales[3] = {1, 1, 1};
     int *d_slot = nullptr, *d_qcount = nullptr;
     float *d_sx = nullptr, *d_sy = nullptr, *d_dsur = nullptr;
+    float *d_dsnap = nullptr;    // surrogate tangents of the last assembly (what K holds)
     int8_t *d_sflag = nullptr, *d_srej = nullptr;
     unsigned long long *d_scounts = nullptr;   // accepted, rejected
     unsigned long long queued_total = 0;
@@ -1195,12 +1201,12 @@ static int model_free(hfem_model *md)
     for (hipcsr_mtx *k : md->linked) k->mf_model = nullptr;
     md->linked.clear();
     {
-        void *sp[] = {md->d_slot, md->d_qcount, md->d_sx, md->d_sy, md->d_dsur, md->d_sflag, md->d_srej,
+        void *sp[] = {md->d_slot, md->d_qcount, md->d_sx, md->d_sy, md->d_dsur, md->d_dsnap, md->d_sflag, md->d_srej,
                       md->d_scounts};
         for (void *p : sp)
             if (p) HIPFEM_CHECK(hipFree(p));
         md->d_slot = md->d_qcount = nullptr;
-        md->d_sx = md->d_sy = md->d_dsur = nullptr;
+        md->d_sx = md->d_sy = md->d_dsur = md->d_dsnap = nullptr;
         md->d_sflag = md->d_srej = nullptr;
         md->d_scounts = nullptr;
         md->sur = nullptr;
@@ -1546,7 +1552,8 @@ extern "C" int hfem_model_assemble_stiffness(hfem_model *md, hipcsr_mtx *K, int 
             if (!md->d_tdesc) HIPFEM_CHECK(hipMalloc(&md->d_tdesc, sizeof(float) * 8 * (size_t)md->ngp_total));
             k_tangent_desc<<<(md->ngp_total + 255) / 256, 256, 0, st>>>(md->ngp_total, md->ngp, md->d_emat,
                                                                           md->d_mat, md->d_state, mode, md->d_tdesc,
-                                                                          md->sur ? md->d_sflag : nullptr);
+                                                                          md->sur ? md->d_sflag : nullptr,
+                                                                          md->d_dsur, md->d_dsnap);
             HIPFEM_CHECK(hipGetLastError());
         }
         if (!md->d_erel && md->nelem) {
@@ -1621,7 +1628,7 @@ int model_mf_slots_f32(hfem_model *md, const float *x, const PcgState *st, const
     if (md->nelem)
         k_mf_hex<<<(md->nelem + MF_EPB - 1) / MF_EPB, 256, 0, s>>>(
             nullptr, md->nelem, md->d_erel, md->d_conn, md->d_eqn, md->d_emat, md->d_mat,
-            md->tdesc_used ? md->d_tdesc : nullptr, md->d_dsur, md->ngp_total, x, nullptr, nullptr, md->d_yslot,
+            md->tdesc_used ? md->d_tdesc : nullptr, md->d_dsnap, md->ngp_total, x, nullptr, nullptr, md->d_yslot,
             md->d_ecolor, md->neq, st);
     HIPFEM_CHECK(hipGetLastError());
     *yslot = md->d_yslot;
@@ -1641,7 +1648,7 @@ int model_mf_apply_f32(hfem_model *md, const float *x, double *y, const PcgState
         if (md->nelem)
             k_mf_hex<<<(md->nelem + MF_EPB - 1) / MF_EPB, 256, 0, s>>>(
                 nullptr, md->nelem, md->d_erel, md->d_conn, md->d_eqn, md->d_emat, md->d_mat,
-                md->tdesc_used ? md->d_tdesc : nullptr, md->d_dsur, md->ngp_total, x, y, nullptr, md->d_yslot,
+                md->tdesc_used ? md->d_tdesc : nullptr, md->d_dsnap, md->ngp_total, x, y, nullptr, md->d_yslot,
                 md->d_ecolor, md->neq, st);
         k_mf_slot_sum<<<std::min(8192, (md->neq + 255) / 256), 256, 0, s>>>(md->neq, md->ncolors, md->d_eqmask,
                                                                             md->d_yslot, y, st);
@@ -1653,7 +1660,7 @@ int model_mf_apply_f32(hfem_model *md, const float *x, double *y, const PcgState
         if (md->nelem)
             k_mf_hex<<<(md->nelem + MF_EPB - 1) / MF_EPB, 256, 0, s>>>(
                 nullptr, md->nelem, md->d_erel, md->d_conn, md->d_eqn, md->d_emat, md->d_mat,
-                md->tdesc_used ? md->d_tdesc : nullptr, md->d_dsur, md->ngp_total, x, y, md->d_ebuf, nullptr, nullptr,
+                md->tdesc_used ? md->d_tdesc : nullptr, md->d_dsnap, md->ngp_total, x, y, md->d_ebuf, nullptr, nullptr,
                 0, st);
         k_mf_gather<<<std::min(8192, (md->nnode + 255) / 256), 256, 0, s>>>(md->nnode, md->d_nptr, md->d_nadj,
                                                                               md->d_eqn, md->d_ebuf, y, st);
@@ -1666,7 +1673,7 @@ int model_mf_apply_f32(hfem_model *md, const float *x, double *y, const PcgState
         if (!n) continue;
         k_mf_hex<<<(n + MF_EPB - 1) / MF_EPB, 256, 0, s>>>(md->d_colors + md->color_ptr[c], n, md->d_erel,
                                                            md->d_conn, md->d_eqn, md->d_emat, md->d_mat,
-                                                           md->tdesc_used ? md->d_tdesc : nullptr, md->d_dsur,
+                                                           md->tdesc_used ? md->d_tdesc : nullptr, md->d_dsnap,
                                                            md->ngp_total, x, y, nullptr, nullptr, nullptr, 0, st);
     }
     HIPFEM_CHECK(hipGetLastError());
@@ -1808,6 +1815,7 @@ extern "C" int hfem_model_set_surrogate(hfem_model *md, hfem_mlp *net, double ba
         HIPFEM_CHECK(hipMalloc(&md->d_sx, sizeof(float) * n * 13));
         HIPFEM_CHECK(hipMalloc(&md->d_sy, sizeof(float) * n * 27));
         HIPFEM_CHECK(hipMalloc(&md->d_dsur, sizeof(float) * n * 21));
+        HIPFEM_CHECK(hipMalloc(&md->d_dsnap, sizeof(float) * n * 21));
         HIPFEM_CHECK(hipMalloc(&md->d_sflag, n));
         HIPFEM_CHECK(hipMalloc(&md->d_srej, n));
         HIPFEM_CHECK(hipMalloc(&md->d_scounts, 2 * sizeof(unsigned long long)));

@@ -354,8 +354,9 @@ __global__ void k_dinv(int n, const int32_t *__restrict__ diag, const double *__
         if (d == 0.0) atomicMin(zero_row, i);
     }
     dinv[i] = 1. / d;
-    dinv32[i] = (float)(1. / d);
+    if (dinv32) dinv32[i] = (float)(1. / d);
 }
+
 
 __global__ void k_d2f(int n, const double *__restrict__ a, float *__restrict__ b)
 {
@@ -700,6 +701,15 @@ static int init_precond(hippcg_solver *s, hipcsr_mtx *A)
     return 0;
 }
 
+static int read_state_at(hipfem_ctx *ctx, const PcgState *dst, PcgState *out)
+{
+    hipStream_t st = ctx->stream;
+    HIPFEM_CHECK(hipMemcpyAsync(ctx->h_pinned, dst, sizeof(PcgState), hipMemcpyDeviceToHost, st));
+    HIPFEM_CHECK(hipStreamSynchronize(st));
+    memcpy(out, ctx->h_pinned, sizeof(PcgState));
+    return 0;
+}
+
 static int read_state(hippcg_solver *s, PcgState *out)
 {
     hipStream_t st = s->ctx->stream;
@@ -709,13 +719,38 @@ static int read_state(hippcg_solver *s, PcgState *out)
     return 0;
 }
 
+// Where a plain PCG run keeps its device state: the solver's own buffers
+// (state, the context's partial sums, optional history and launch timing),
+// or caller-provided memory (the allocation-free entry point: part != null,
+// no history, no timing).
+struct PcgRunBufs {
+    PcgState *st = nullptr;
+    double *part = nullptr;   // null: the context's partials, grown as needed
+    size_t part_cap = 0;      // doubles available at part
+    bool prof = false;
+    double *hist = nullptr;
+    int hist_cap = 0;
+};
+
+static PcgRunBufs solver_bufs(hippcg_solver *s)
+{
+    PcgRunBufs b;
+    b.st = s->st;
+    b.prof = s->profiling != 0;
+    b.hist = s->d_hist;
+    b.hist_cap = s->hist_cap;
+    return b;
+}
+
 // Generic PCG on device vectors.  VT: vector/arith type, MT: matrix values.
 // b, x: length n; x is the initial guess (x_zero: known to be 0).
 template <typename VT, typename XT, typename MT>
-static int pcg_run(hippcg_solver *s, hipcsr_mtx *A, const MT *val, const VT *b, XT *x, int x_zero, double tol, int maxit,
-                   VT *r, VT *z, VT *p0, VT *p1, VT *q, const VT *dinv, int *iters, double *res, int *converged)
+static int pcg_run(hippcg_solver *s, const PcgRunBufs &bufs, hipcsr_mtx *A, const MT *val, const VT *b, XT *x,
+                   int x_zero, double tol, int maxit, VT *r, VT *z, VT *p0, VT *p1, VT *q, const VT *dinv, int *iters,
+                   double *res, int *converged)
 {
     hipStream_t st = s->ctx->stream;
+    PcgState *const pst = bufs.st;
     const int n = A->n;
     const SpmvShape sh = A->shape;
     const int gv = vec_grid(n);
@@ -726,9 +761,17 @@ static int pcg_run(hippcg_solver *s, hipcsr_mtx *A, const MT *val, const VT *b, 
                    : sh.stream ? std::max(1, std::min((n + sh.rpb - 1) / sh.rpb, MAX_GRID))
                                : spmv_grid(n, sh.lanes);
     const bool fused = s->fused && !sh.stream && !nodeblock && !grouped;  // these forms gather a ready p
-    int rc = ensure_partials(s->ctx, 3 * (size_t)std::max(gv, gs));
-    if (rc) return rc;
-    double *part = s->ctx->d_partials;
+    const size_t need = 3 * (size_t)std::max(gv, gs);
+    double *part = bufs.part;
+    int rc = 0;
+    if (!part) {
+        rc = ensure_partials(s->ctx, need);
+        if (rc) return rc;
+        part = s->ctx->d_partials;
+    } else {
+        HIPFEM_REQUIRE(need <= bufs.part_cap, HIPFEM_ETOOBIG, "pcg: %zu partial sums exceed the %zu provided", need,
+                       bufs.part_cap);
+    }
     if (!x_zero) {
         // q = A x with x in XT (FP64 only for the plain path)
         if constexpr (std::is_same<VT, double>::value && std::is_same<XT, double>::value) {
@@ -740,22 +783,22 @@ static int pcg_run(hippcg_solver *s, hipcsr_mtx *A, const MT *val, const VT *b, 
         }
     }
     k_pcg_init<VT><<<gv, 256, 0, st>>>(n, b, q, x_zero, r, z, dinv, part);
-    k_reduce_init<<<1, RED_THREADS, 0, st>>>(s->st, part, gv, tol, s->d_hist, s->hist_cap);
+    k_reduce_init<<<1, RED_THREADS, 0, st>>>(pst, part, gv, tol, bufs.hist, bufs.hist_cap);
     HIPFEM_CHECK(hipGetLastError());
     PcgState h;
-    rc = read_state(s, &h);
+    rc = read_state_at(s->ctx, pst, &h);
     if (rc) return rc;
     if (h.done) {
         *iters = 0;
         *res = h.res;
         *converged = 1;
-        s->hist_len = std::min(1, s->hist_cap);
+        if (bufs.hist) s->hist_len = std::min(1, bufs.hist_cap);
         return 0;
     }
     int it = 1;
     VT *pold = p0, *pnew = p1;
     int chunk = s->check_every;
-    const bool prof = s->profiling != 0;
+    const bool prof = bufs.prof;
     while (it <= maxit) {
         const int stop = std::min(maxit, it + chunk - 1);
         if (prof) {
@@ -764,26 +807,26 @@ static int pcg_run(hippcg_solver *s, hipcsr_mtx *A, const MT *val, const VT *b, 
         }
         for (; it <= stop; it++) {
             const int first = (it == 1);
-            if (!fused) k_pcg_pupdate<VT><<<gv, 256, 0, st>>>(n, z, p0, s->st, first);
+            if (!fused) k_pcg_pupdate<VT><<<gv, 256, 0, st>>>(n, z, p0, pst, first);
             if (prof) HIPFEM_CHECK(hipEventRecord(s->ev[2 * (it - 1)], st));
             if (nodeblock)
-                k_pcg_spmv_p_nb<VT, MT><<<gs, 256, 0, st>>>(A->nbr, A->d_bptr, A->d_bcol, val, p0, q, s->st, part);
+                k_pcg_spmv_p_nb<VT, MT><<<gs, 256, 0, st>>>(A->nbr, A->d_bptr, A->d_bcol, val, p0, q, pst, part);
             else if (grouped)
                 k_pcg_spmv_p_grp<VT, MT><<<gs, 256, 0, st>>>(A->ngrp, A->d_grow, A->d_gch, A->d_ccol, A->d_coff,
-                                                             A->d_rowptr, val, p0, q, s->st, part);
+                                                             A->d_rowptr, val, p0, q, pst, part);
             else if (fused)
-                launch_pcg_spmv_fused<VT, MT>(sh, gs, st, n, A->d_rowptr, A->d_colind, val, z, pold, pnew, q, s->st,
+                launch_pcg_spmv_fused<VT, MT>(sh, gs, st, n, A->d_rowptr, A->d_colind, val, z, pold, pnew, q, pst,
                                               part, first);
             else
-                launch_pcg_spmv_p<VT, MT>(sh, gs, st, n, A->d_rowptr, A->d_colind, val, p0, q, s->st, part);
+                launch_pcg_spmv_p<VT, MT>(sh, gs, st, n, A->d_rowptr, A->d_colind, val, p0, q, pst, part);
             if (prof) HIPFEM_CHECK(hipEventRecord(s->ev[2 * (it - 1) + 1], st));
-            k_reduce_alpha<<<1, RED_THREADS, 0, st>>>(s->st, part, gs);
-            k_pcg_update<XT, VT><<<gv, 256, 0, st>>>(n, x, r, z, fused ? pnew : p0, q, dinv, s->st, part);
-            k_reduce_beta<<<1, RED_THREADS, 0, st>>>(s->st, part, gv, it, s->d_hist, s->hist_cap);
+            k_reduce_alpha<<<1, RED_THREADS, 0, st>>>(pst, part, gs);
+            k_pcg_update<XT, VT><<<gv, 256, 0, st>>>(n, x, r, z, fused ? pnew : p0, q, dinv, pst, part);
+            k_reduce_beta<<<1, RED_THREADS, 0, st>>>(pst, part, gv, it, bufs.hist, bufs.hist_cap);
             if (fused) std::swap(pold, pnew);
         }
         HIPFEM_CHECK(hipGetLastError());
-        rc = read_state(s, &h);
+        rc = read_state_at(s->ctx, pst, &h);
         if (rc) return rc;
         if (h.done) break;
         chunk = std::min(chunk * 2, 64);
@@ -791,7 +834,7 @@ static int pcg_run(hippcg_solver *s, hipcsr_mtx *A, const MT *val, const VT *b, 
     *iters = h.iter;
     *res = h.res;
     *converged = (h.done && !h.bad && h.res <= tol) ? 1 : 0;
-    s->hist_len = std::min(h.iter + 1, s->hist_cap);
+    if (bufs.hist) s->hist_len = std::min(h.iter + 1, bufs.hist_cap);
     if (prof) {
         // launches after convergence returned at once: time iterations 1..h.iter only
         HIPFEM_CHECK(hipStreamSynchronize(st));
@@ -967,8 +1010,8 @@ static int solve_dev(hippcg_solver *s, hipcsr_mtx *A, const double *db, double *
         return 0;
     }
     if (s->precision == 0) {
-        rc = pcg_run<double, double, double>(s, A, A->d_val, db, dx, 0, s->tol, s->maxit, s->r, s->z, s->p0, s->p1,
-                                              s->q, s->dinv, &it, &res, &conv);
+        rc = pcg_run<double, double, double>(s, solver_bufs(s), A, A->d_val, db, dx, 0, s->tol, s->maxit, s->r, s->z,
+                                              s->p0, s->p1, s->q, s->dinv, &it, &res, &conv);
         if (rc) return rc;
     } else if (s->precision == 2) {
         rc = pcg_reliable(s, A, db, dx, s->maxit, &it, &res, &conv);
@@ -1007,8 +1050,8 @@ static int solve_dev(hippcg_solver *s, hipcsr_mtx *A, const double *db, double *
             const double itol = std::max(s->inner_tol, 0.5 * s->tol / res);
             int iit = 0, iconv = 0;
             double ires = 0;
-            rc = pcg_run<float, float, float>(s, A, A->d_val32, s->fb, s->fx, 1, itol, s->maxit - total, s->fr, s->fz,
-                                              s->fp0, s->fp1, s->fq, s->dinv32, &iit, &ires, &iconv);
+            rc = pcg_run<float, float, float>(s, solver_bufs(s), A, A->d_val32, s->fb, s->fx, 1, itol, s->maxit - total,
+                                              s->fr, s->fz, s->fp0, s->fp1, s->fq, s->dinv32, &iit, &ires, &iconv);
             if (rc) return rc;
             total += iit;
             k_add_f2d<<<gv, 256, 0, st>>>(n, s->fx, dx);

@@ -195,24 +195,15 @@ void HipCSRSparseMtrx::zero()
 double &HipCSRSparseMtrx::at(int i, int j)
 {
     checkBounds(i, j);
-    if (!hipcsr_is_allocated(m, i, j)) OOFEM_ERROR("at(%d,%d): entry not in the sparse structure", i, j);
-    const int64_t nnz = hipcsr_nnz(m);
-    static thread_local std::vector<int64_t> rp;
-    static thread_local std::vector<int32_t> ci;
-    rp.resize((size_t)nRows + 1);
-    ci.resize((size_t)nnz);
+    int64_t pos = -1;
+    hipfemCheck(hipcsr_find(m, i, j, &pos), "at");
+    if (pos < 0) OOFEM_ERROR("at(%d,%d): entry not in the sparse structure", i, j);
     if (!mirrorValid) {
-        mirror.resize((size_t)nnz);
-        hipfemCheck(hipcsr_download(m, rp.data(), ci.data(), mirror.data()), "download");
+        // one download of the values per matrix version; later entries are
+        // found in the host pattern in O(log row length)
+        mirror.resize((size_t)hipcsr_nnz(m));
+        hipfemCheck(hipcsr_download(m, nullptr, nullptr, mirror.data()), "download");
         mirrorValid = true;
-    } else {
-        hipfemCheck(hipcsr_download(m, rp.data(), ci.data(), nullptr), "download");
-    }
-    int64_t lo = rp[i - 1], hi = rp[i] - 1, pos = -1;
-    while (lo <= hi) {
-        const int64_t mid = (lo + hi) / 2;
-        if (ci[mid] == j - 1) { pos = mid; break; }
-        if (ci[mid] < j - 1) lo = mid + 1; else hi = mid - 1;
     }
     mirrorDirty = true;
     this->version++;

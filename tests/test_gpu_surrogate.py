@@ -213,3 +213,28 @@ def test_newton_with_surrogate_converges_to_plain_solution(ctx, trained):
         P.close()
     assert res["stats"]["queued"] > 0
     assert np.linalg.norm(res[True] - res[False]) <= 5e-2 * np.linalg.norm(res[False])
+
+
+def test_element_operator_keeps_the_assembled_surrogate_tangent(ctx, trained):
+    """The FP32 element operator applies the tangent the FP64 matrix was
+    assembled with, also at surrogate points: after a later state update
+    (new surrogate tangents at those points) it still equals K to FP32
+    accuracy."""
+    d, ev, K, u = _plastic_block(ctx, 0.25)
+    up = np.zeros(d.npeq)
+    ev.set_surrogate(trained, band=0.05, tol=3e-2)
+    ev.surrogate_stats(reset=True)
+    ev.internal_forces(u, up, update_state=True)
+    assert ev.surrogate_stats()["accepted"] > 0
+    ev.assemble_stiffness(K, tangent=True)
+    ev.set_element_operator(2)
+    assert K.is_matrix_free()
+    ev.internal_forces(1.3 * u, up, update_state=True)      # the live surrogate tangents change
+    assert K.is_matrix_free()
+    rng = np.random.default_rng(7)
+    for _ in range(2):
+        x = rng.standard_normal(d.neq)
+        y64 = K.times(x)
+        y32 = K.times_matrix_free(x)
+        assert np.linalg.norm(y32 - y64) <= 2e-5 * np.linalg.norm(y64)
+    ev.set_surrogate(None)

@@ -100,9 +100,12 @@ def test_engngmodel_hooks_through_the_base_class():
     P.set_load(np.zeros(d.neq), np.zeros(d.npeq))
     P.set_prescribed(up)
     assert P.nonlinear_step()["status"] == 1
+    # out of balance: the converged displacements with the end moved further
+    up2 = d.prescribed_values(0.7)
+    P.set_prescribed(up2)
     st = oracle.state_new(d)
     X = P.solution()
-    fo, _, ebeo = oracle.internal_forces(d, X, up, state=st, update_state=True)
+    fo, _, ebeo = oracle.internal_forces(d, X, up2, state=st, update_state=True)
     f, e = P.update_internal_rhs()
     assert np.abs(f - fo).max() <= 1e-12 * np.abs(fo).max()
     np.testing.assert_allclose(e, ebeo, rtol=1e-12)

@@ -219,6 +219,20 @@ int hippcg_set_profiling(hippcg_solver *s, int on);
  * place in its own pass and gathers p once per nonzero. */
 int hippcg_set_fused(hippcg_solver *s, int fused);
 int hippcg_profile(hippcg_solver *s, int *n_spmv64, double *ms_spmv64, int *n_spmv32, double *ms_spmv32);
+/* Allocation-free plain PCG.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same contract and
+ * result as hippcg_solve_dev with precision 0 (the same kernels in the same
+ * order), but the diagonal inverse, the work vectors, the partial sums and
+ * the solver state live in one fixed device array per GPU (about 48 MiB,
+ * kept until the process ends) instead of the solver's own buffers, so the
+ * call allocates nothing.  Limits: precision 0 only, at most
+ * HIPPCG_NOALLOC_MAX_N equations (HIPFEM_ETOOBIG beyond), no residual
+ * history and no profiling.  Every call on one GPU shares that array:
+ * calls that overlap on the same GPU race. */
+#define HIPPCG_NOALLOC_MAX_N 1048576
+int hippcg_solve_dev_noalloc(hippcg_solver *s, hipcsr_mtx *A, const double *db, double *dx, int *reason, int *iters,
+                             double *achieved);
 /* Residual history of the plain PCG (precision 0, and the inner solves of
  * precision 1): with a capacity cap > 0 every solve records ||r_k|| / ||b||
  * for k = 0, 1, ... (the quantity cg.h compares with lstol, iml/cg.h:34-60,

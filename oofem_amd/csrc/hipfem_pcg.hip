@@ -32,6 +32,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <type_traits>
 
 #include "hipfem_internal.h"
@@ -357,6 +358,10 @@ __global__ void k_dinv(int n, const int32_t *__restrict__ diag, const double *__
     if (dinv32) dinv32[i] = (float)(1. / d);
 }
 
+__global__ void k_set_int(int *p, int v)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
 
 __global__ void k_d2f(int n, const double *__restrict__ a, float *__restrict__ b)
 {
@@ -1060,6 +1065,88 @@ static int solve_dev(hippcg_solver *s, hipcsr_mtx *A, const double *db, double *
         }
         it = total;
     }
+    *reason = conv ? HIPFEM_CR_CONVERGED : HIPFEM_CR_DIVERGED_ITS;
+    *iters = it;
+    *achieved = res;
+    return 0;
+}
+
+// ------------------------------------------------- allocation-free plain PCG
+// Device memory of hippcg_solve_dev_noalloc, one copy per GPU, held until the
+// process ends: the diagonal inverse and five work vectors of up to
+// HIPPCG_NOALLOC_MAX_N equations, the block partial sums, the solver state
+// and the zero-diagonal flag.  Host code never uses the array's name as an
+// address: it takes the device's address once from hipGetSymbolAddress.
+constexpr size_t NA_N = HIPPCG_NOALLOC_MAX_N;
+constexpr size_t NA_PART = 3 * (size_t)MAX_GRID;
+constexpr size_t NA_OFF_PART = 6 * NA_N * sizeof(double);
+constexpr size_t NA_OFF_STATE = NA_OFF_PART + NA_PART * sizeof(double);
+constexpr size_t NA_OFF_FLAG = NA_OFF_STATE + 256;
+constexpr size_t NA_BYTES = NA_OFF_FLAG + 256;
+static_assert(sizeof(PcgState) <= 256, "PcgState exceeds its scratch slot");
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+
+static std::mutex na_mutex;
+static unsigned char *na_ptr[64] = {};
+
+// address of this device's scratch (the current device must be `device`)
+static int noalloc_scratch(int device, unsigned char **out)
+{
+    HIPFEM_REQUIRE(device >= 0 && device < 64, HIPFEM_EINVAL, "device %d out of range", device);
+    std::lock_guard<std::mutex> lock(na_mutex);
+    if (!na_ptr[device]) {
+        void *ptr = nullptr;
+        HIPFEM_CHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        HIPFEM_REQUIRE(ptr != nullptr, HIPFEM_EHIP, "no device address for the solver scratch");
+        na_ptr[device] = static_cast<unsigned char *>(ptr);
+    }
+    *out = na_ptr[device];
+    return 0;
+}
+
+extern "C" int hippcg_solve_dev_noalloc(hippcg_solver *s, hipcsr_mtx *A, const double *db, double *dx, int *reason,
+                                        int *iters, double *achieved)
+{
+    HIPFEM_REQUIRE(s && A && db && dx && reason && iters && achieved, HIPFEM_EINVAL, "solve_noalloc: null argument");
+    HIPFEM_REQUIRE(A->ctx == s->ctx, HIPFEM_EINVAL, "solve_noalloc: matrix and solver on different contexts");
+    HIPFEM_REQUIRE(A->has_structure, HIPFEM_EINVAL, "solve_noalloc: matrix has no structure");
+    HIPFEM_REQUIRE(!A->staged, HIPFEM_EINVAL, "solve_noalloc: assemble() contributions not flushed (call assembleEnd)");
+    HIPFEM_REQUIRE(s->precision == 0, HIPFEM_EINVAL, "solve_noalloc: plain FP64 PCG (precision 0) only");
+    HIPFEM_REQUIRE((size_t)A->n <= NA_N, HIPFEM_ETOOBIG, "solve_noalloc: %d equations exceed the limit of %zu", A->n,
+                   NA_N);
+    DeviceGuard g(s->ctx->device);
+    if (A->n == 0) {
+        *reason = HIPFEM_CR_CONVERGED;
+        *iters = 0;
+        *achieved = 0.0;
+        return 0;
+    }
+    unsigned char *base = nullptr;
+    int rc = noalloc_scratch(s->ctx->device, &base);
+    if (rc) return rc;
+    double *v = reinterpret_cast<double *>(base);
+    double *dinv = v, *r = v + NA_N, *z = v + 2 * NA_N, *p0 = v + 3 * NA_N, *p1 = v + 4 * NA_N, *q = v + 5 * NA_N;
+    PcgRunBufs bufs;
+    bufs.part = reinterpret_cast<double *>(base + NA_OFF_PART);
+    bufs.part_cap = NA_PART;
+    bufs.st = reinterpret_cast<PcgState *>(base + NA_OFF_STATE);
+    int *zero_row = reinterpret_cast<int *>(base + NA_OFF_FLAG);
+    hipStream_t st = s->ctx->stream;
+    // DiagPreconditioner::init on every call: the scratch holds whatever the
+    // last call on this GPU left there
+    k_set_int<<<1, 64, 0, st>>>(zero_row, INT32_MAX);
+    k_dinv<<<(A->n + 255) / 256, 256, 0, st>>>(A->n, A->d_diag, A->d_val, s->precond == 1, dinv, nullptr, zero_row);
+    HIPFEM_CHECK(hipGetLastError());
+    HIPFEM_CHECK(hipMemcpyAsync(s->ctx->h_pinned, zero_row, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPFEM_CHECK(hipStreamSynchronize(st));
+    int zr = 0;
+    memcpy(&zr, s->ctx->h_pinned, sizeof(int));
+    HIPFEM_REQUIRE(zr == INT32_MAX, HIPFEM_ENUMERIC, "DiagPreconditioner: zero diagonal detected in equation %d", zr + 1);
+    int it = 0, conv = 0;
+    double res = 0.0;
+    rc = pcg_run<double, double, double>(s, bufs, A, A->d_val, db, dx, 0, s->tol, s->maxit, r, z, p0, p1, q, dinv,
+                                          &it, &res, &conv);
+    if (rc) return rc;
     *reason = conv ? HIPFEM_CR_CONVERGED : HIPFEM_CR_DIVERGED_ITS;
     *iters = it;
     *achieved = res;

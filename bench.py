@@ -4,26 +4,33 @@
 Default workload (config 3): the 64x64x256 LSpace block (1,048,576 elements,
 3,240,575 free equations, jitter +-0.2 h) with MisesMat (E 210e3, nu 0.3,
 sig0 250, H 2e3), clamped at z = 0, u_z prescribed on z = 4 and ramped to
-5x the yield strain over 8 load steps.  One step = one StaticStructural load
-step: tangent initial guess, full Newton with the consistent tangent to the
-1e-8 force criterion, Jacobi-PCG to 1e-6 per correction.  After --c3-warm
-untimed steps, --c3-steps are timed for three forms:
-  fp64             hipcsr + hippcg FP64 + plain return mapping (value)
-  mixed            hippcg mixed 2: FP32 PCG with FP64 reliable updates and
-                   the FP32 element-by-element operator
-  mixed_surrogate  the same with the trained MisesMat surrogate (BF16 MFMA
-                   inference, residual-checked, plain fallback)
-value = free DOFs x timed steps / wall time of the FP64 form; the others,
-their speed-ups and their displacement / stress errors against it are in
-config.variants.
+5x the yield strain over 20 load steps (more when --warmup + --steps
+exceed 20).  One step = one StaticStructural load step: tangent initial
+guess, full Newton with the consistent tangent to the 1e-8 force criterion,
+Jacobi-PCG to 1e-6 per correction.  --warmup untimed steps, then --steps
+timed ones, for every form of --variants:
+  fp64                     hipcsr + hippcg FP64 + plain return mapping (value)
+  mixed                    hippcg mixed 2: FP32 PCG with FP64 reliable updates
+                           and the FP32 element-by-element operator
+  mixed_surrogate          the same with the trained MisesMat surrogate (BF16
+                           MFMA inference, residual-checked, plain fallback)
+  fp64_inexact, mixed_surrogate_inexact
+                           the same two with Eisenstat-Walker inexact Newton
+value = free DOFs x timed steps / wall time of the FP64 form; every other
+form's speed-up against the FP64 form with the same Newton forcing, its
+errors against plain FP64, the errors of every form against the FP64 oracle
+on a slab, and the surrogate's acceptance against its tolerance are in
+config.
 
 roofline: the FP64 PCG SpMV (row-group form on this pattern: the top face
 constrains u_z only), algorithmic bytes per launch over its mean HIP-event
-duration.  cpu_baseline: the oracle (one thread) on a 64x64x16 slab of the
-same problem, scaled by elements, nnz and the GPU's iteration counts.
+duration.  cpu_baseline: the oracle on the full mesh with the reference's
+OpenMP element loops, timed on the host, with the GPU run's iteration
+counts.
 
 Other workloads: --workload cfg1 (LinearStatic cantilever, FP64 vs mixed),
-cfg0, cfg4 (512-RVE batch, solves/s); --mode kernels (counter runs).
+cfg0, cfg2 (tet plasticity), cfg4 (512-RVE batch, solves/s); --mode kernels
+(counter runs).
 """
 import argparse
 import json
@@ -96,7 +103,21 @@ def run_kernels(args, deck):
     K = HipCSRSparseMtrx(ctx)
     ev = BatchedEvaluator(ctx, deck)
     ev.build_structure(K)
-    ev.assemble_stiffness(K)
+    plastic = bool(deck.mat[:, 0].max() >= 1) and args.kernels_plastic
+    if plastic:
+        # MisesMat tangent of a plastic state: the uniaxial field of 3/8 of
+        # the end displacement (about 1.9x the yield strain) at every node
+        up = deck.prescribed_values(3.0 / 8.0)
+        zmax = deck.coords[:, 2].max()
+        uz = deck.coords[:, 2] / zmax * 3.0 / 8.0 * deck.meta["u_end"]
+        u = np.zeros(deck.neq)
+        fe = deck.eqn[:, 2] > 0
+        u[deck.eqn[fe, 2] - 1] = uz[fe]
+        ev.internal_forces(u, up, update_state=True)
+        s = ev.get_state()
+        log(f"[bench] kernels: plastic Gauss points {(s[:, 13] > s[:, 6]).mean():.3f}")
+        del s
+    ev.assemble_stiffness(K, tangent=plastic)
     apply_shape(K, args.shape)
     f, _ = deck.external_load()
     if not f.any():
@@ -112,7 +133,8 @@ def run_kernels(args, deck):
         p = s.profile()
         out[name] = (p["ms64"] / max(p["n64"], 1)) if prec == 0 else (p["ms32"] / max(p["n32"], 1))
         s.close()
-    print(json.dumps(dict(mode="kernels", neq=deck.neq, nnz=K.nnz(), shape=K.spmv_shape(), copy_doubles=n_copy,
+    print(json.dumps(dict(mode="kernels", workload=deck.name, plastic_tangent=plastic, neq=deck.neq, nnz=K.nnz(),
+                          shape=K.spmv_shape(), copy_doubles=n_copy,
                           copy_gbs=copy_gbs, spmv_avg_ms=out)), flush=True)
     return 0
 
@@ -149,7 +171,7 @@ def run_gpu(args, deck, precision):
 
 
 def run_newton(args, deck, precision, warm=1, steps=None, total=None, surrogate=None, keep_state=False, forcing=0.0,
-               initial_guess=1):
+               initial_guess=1, keep_last_start=False):
     """StaticStructural + NRSolver load steps (full Newton, consistent tangent,
     rtolf 1e-8, tangent initial guess) with the prescribed end displacement
     ramped over `total` steps: `warm` untimed steps, then `steps` timed ones.
@@ -173,23 +195,35 @@ def run_newton(args, deck, precision, warm=1, steps=None, total=None, surrogate=
         P.set_surrogate(net, band=surrogate[1], tol=surrogate[2])
     reports = []
     t0 = None
+    last_start = None
+    t_excluded = 0.0
     for k in range(1, warm + steps + 1):
+        if keep_last_start and k == warm + steps:
+            # the committed Gauss-point state the last step starts from; the
+            # device-to-host copy is taken out of the timed region
+            t_copy = time.perf_counter()
+            last_start = P.state()
+            if t0 is not None:
+                t_excluded = time.perf_counter() - t_copy
         P.set_prescribed(deck.prescribed_values(k / total))
         if k == warm + 1:
             P.set_profiling(True)
             if net is not None:
                 P.surrogate_stats(reset=True)
             t0 = time.perf_counter()
+        ts = time.perf_counter()
         rep = P.nonlinear_step()
         if rep["status"] != 1:
             raise RuntimeError(f"Newton step {k} failed: {rep}")
         P.update()
+        rep["wall_s"] = time.perf_counter() - ts
         reports.append(rep)
         log(f"[bench] precision {precision}{' +surrogate' if net else ''}{' +forcing' if forcing else ''} "
             f"newton step {k}: {rep}")
-    dt = time.perf_counter() - t0
+    dt = time.perf_counter() - t0 - t_excluded
     p = P.profile()
     out = dict(dt=dt, steps=steps, newton=[r["newton"] for r in reports[warm:]], force_error=[r["force_error"] for r in reports],
+               step_s=[round(r["wall_s"], 3) for r in reports[warm:]],
                reactions=P.reactions(),
                pcg=[r["pcg"] for r in reports[warm:]], n_spmv=p["n64"], ms_spmv=p["ms64"], n_spmv32=p["n32"],
                ms_spmv32=p["ms32"], nnz=P.matrix_nnz(), u=P.solution(), info=P.solver_info(),
@@ -199,6 +233,9 @@ def run_newton(args, deck, precision, warm=1, steps=None, total=None, surrogate=
         out["stress"] = gs[:, 20:26].copy()
         out["kappa"] = gs[:, 6].copy()
         del gs
+    if last_start is not None:
+        out["last_start"] = last_start
+        out["last_up"] = deck.prescribed_values((warm + steps) / total)
     if net is not None:
         out["surrogate"] = P.surrogate_stats()
         P.set_surrogate(None)
@@ -280,40 +317,178 @@ def cpu_baseline(args, deck_full, gpu_iters, nnz_full):
 
 
 def cpu_baseline_newton(args, deck, rn):
-    """Oracle (one thread) on a slab of the same plasticity problem: tangent
-    assembly, f_int with the return mapping and CG iterations timed there,
-    scaled to the full step by element count, nnz and the GPU run's Newton
-    and PCG iteration counts."""
+    """The CPU path on the full mesh of the plasticity block, timed on this
+    host: the oracle (a C restatement of the reference's element loops,
+    MisesMat, CompCol storage and IML CG) with its element loops in the
+    reference's OpenMP form (parallel elements, critical-section scatter,
+    engngm.C:902-923, 1375-1408; CG serial as IMLSolver runs it) on every
+    host thread.  Timed: f_int with the return mapping, the consistent
+    tangent assembled into the CSR pattern, and --cpu-cg-iters CG
+    iterations on that matrix.  Only the iteration counts come from the GPU
+    run (its Newton and PCG iterations per step, the same algorithm): step =
+    (newton + 1) f_int + (newton + 1) assembly (the initial-guess tangent
+    included) + pcg CG iterations.  The single-thread figure repeats the
+    element loops serially on a slab of the same problem (scaled by element
+    count, stated); skyline LDL^T on configuration 0 is the reference's
+    direct solver."""
     import numpy as np
 
     from oofem_amd import meshgen
     from oracle import oracle
-    s = meshgen.config3(nx=args.nx, ny=args.ny, nz=args.cpu_sample_nz)
-    st = oracle.state_new(s)
-    up = s.prescribed_values(3.0 / args.c3_total)
+    nthr = oracle.omp_threads()
     t = time.perf_counter()
-    _, _, _ = oracle.internal_forces(s, np.zeros(s.neq), up, state=st, update_state=True)
+    rp, ci = oracle.csr_structure(deck)
+    t_struct = time.perf_counter() - t
+    st = oracle.state_new(deck)
+    up = deck.prescribed_values(3.0 / 8.0)   # 1.9x the yield strain: a plastic state
+    zmax = deck.coords[:, 2].max()
+    u = np.zeros(deck.neq)
+    fe = deck.eqn[:, 2] > 0
+    u[deck.eqn[fe, 2] - 1] = deck.coords[fe, 2] / zmax * up.max()
+    t = time.perf_counter()
+    oracle.internal_forces(deck, u, up, state=st, update_state=True, omp=True)
     t_fint = time.perf_counter() - t
     t = time.perf_counter()
-    rp, ci, val = oracle.csr(s, st, tangent=True)
+    val = oracle.assemble(deck, rp, ci, st, tangent=True, omp=True)
     t_asm = time.perf_counter() - t
-    f = np.ones(s.neq)
-    nit = args.cpu_sample_iters
+    nit = args.cpu_cg_iters
     t = time.perf_counter()
-    oracle.pcg(rp, ci, val, f, tol=1e-30, maxit=nit)
+    oracle.pcg(rp, ci, val, np.ones(deck.neq), tol=1e-300, maxit=nit)
     t_cg = (time.perf_counter() - t) / nit
-    scale_e = deck.nelem / s.nelem
-    scale_nnz = rn["nnz"] / len(val)
+    del val, rp, ci
+    plastic = float((st.reshape(-1, 38)[:, 13] > st.reshape(-1, 38)[:, 6]).mean())
+    del st
     newton = sum(rn["newton"]) / rn["steps"]
     pcg = sum(rn["pcg"]) / rn["steps"]
-    # per Newton iteration: one tangent assembly and one f_int (+1 f_int per step)
-    t_step = ((newton + 1) * t_fint + newton * t_asm) * scale_e + pcg * t_cg * scale_nnz
-    return dict(value=deck.neq / t_step, unit="DOFs/s", cores=1, kind="port",
-                sample=(f"oracle (C restatement, 1 thread) on a {args.nx}x{args.ny}x{args.cpu_sample_nz} slab of the "
-                        f"same problem ({s.nelem} elements, {len(val)} nnz): f_int with return mapping {t_fint:.2f} s, "
-                        f"tangent assembly {t_asm:.2f} s, {nit} CG iterations at {t_cg * 1e3:.1f} ms; scaled by "
-                        f"elements x{scale_e:.1f}, nnz x{scale_nnz:.1f} and the GPU's {newton:.1f} Newton / "
-                        f"{pcg:.0f} PCG iterations per step (estimated step {t_step:.1f} s)"))
+    t_step = (newton + 1) * (t_fint + t_asm) + pcg * t_cg
+    # one thread: the element loops of a slab, scaled by element count
+    s = meshgen.config3(nx=args.nx, ny=args.ny, nz=args.cpu_sample_nz)
+    sst = oracle.state_new(s)
+    sup = s.prescribed_values(3.0 / 8.0)
+    su = np.zeros(s.neq)
+    sfe = s.eqn[:, 2] > 0
+    su[s.eqn[sfe, 2] - 1] = s.coords[sfe, 2] / s.coords[:, 2].max() * sup.max()
+    srp, sci = oracle.csr_structure(s)
+    t = time.perf_counter()
+    oracle.internal_forces(s, su, sup, state=sst, update_state=True)
+    t_fint1 = (time.perf_counter() - t) * deck.nelem / s.nelem
+    t = time.perf_counter()
+    oracle.assemble(s, srp, sci, sst, tangent=True)
+    t_asm1 = (time.perf_counter() - t) * deck.nelem / s.nelem
+    t_step1 = (newton + 1) * (t_fint1 + t_asm1) + pcg * t_cg
+    # configuration 0: LinearStatic with the reference's direct solver
+    c0 = meshgen.config0()
+    t = time.perf_counter()
+    r0, c0i, v0 = oracle.csr(c0)
+    f0, _ = oracle.surface_load(c0)
+    oracle.internal_forces(c0, np.zeros(c0.neq), c0.prescribed_values())
+    t0a = time.perf_counter() - t
+    t = time.perf_counter()
+    _, nwk = oracle.skyline_solve(r0, c0i, v0, f0)
+    t0s = time.perf_counter() - t
+    t = time.perf_counter()
+    _, it0, _, _ = oracle.pcg(r0, c0i, v0, f0, tol=1e-10)
+    t0c = time.perf_counter() - t
+    return dict(value=deck.neq / t_step, unit="DOFs/s", cores=nthr, kind="port",
+                sample=(f"oracle on the full {deck.name} mesh ({deck.nelem} elements, {deck.neq} free DOFs, "
+                        f"{rn['nnz']} nnz), {nthr} host threads for the element loops (OpenMP, critical-section "
+                        f"scatter) and one for CG: f_int with the return mapping {t_fint:.2f} s ({plastic:.2f} of "
+                        f"the points plastic), tangent assembly {t_asm:.2f} s, CG {t_cg * 1e3:.0f} ms per "
+                        f"iteration ({nit} timed); with the GPU run's {newton:.1f} Newton and {pcg:.0f} PCG "
+                        f"iterations per step: {t_step:.1f} s per step (pattern build {t_struct:.1f} s, once)"),
+                single_thread=dict(value=deck.neq / t_step1, cores=1,
+                                   sample=(f"element loops on one thread on a {args.nx}x{args.ny}x"
+                                           f"{args.cpu_sample_nz} slab ({s.nelem} elements) scaled by "
+                                           f"x{deck.nelem / s.nelem:.0f} elements: f_int {t_fint1:.1f} s, assembly "
+                                           f"{t_asm1:.1f} s per full mesh; CG as above; {t_step1:.1f} s per step")),
+                config0=dict(workload=c0.name, free_dofs=c0.neq, assembly_s=t0a,
+                             skyline_ldlt_s=t0s, skyline_profile_entries=int(nwk),
+                             skyline_dofs_per_s=c0.neq / (t0a + t0s),
+                             iml_cg_s=t0c, iml_cg_iters=int(it0), iml_cg_dofs_per_s=c0.neq / (t0a + t0c)))
+
+
+def oracle_check_newton(args, variants):
+    """Errors of every GPU form against the FP64 oracle on the largest slab
+    of the plasticity block the oracle solves within the budget: the same
+    load steps (ramp of --c3-total), the oracle's Newton with CG to 1e-12
+    per correction; displacement and Gauss-point stress errors per form."""
+    import numpy as np
+
+    from oofem_amd import meshgen
+    from oracle import oracle
+    d = meshgen.config3(nx=args.check_nx, ny=args.check_nx, nz=4 * args.check_nx)
+    nsteps = args.check_steps
+    t = time.perf_counter()
+    st = oracle.state_new(d)
+    X = np.zeros(d.neq)
+    prev = None
+    for k in range(1, nsteps + 1):
+        up = d.prescribed_values(k / args.c3_total)
+        r = oracle.nonlinear_step_ex(d, st, X, up, u_presc_prev=prev, rtolf=1e-8, maxiter=60, lstol=1e-12)
+        if r["status"] != 1:
+            raise RuntimeError(f"oracle step {k} failed: {r}")
+        oracle.state_commit(d, st)
+        prev = up
+    t_oracle = time.perf_counter() - t
+    so = st.reshape(-1, 38)
+    sy = 250.0 + 2e3 * so[:, 6]
+    out = dict(workload=d.name, free_dofs=d.neq, load_steps=nsteps, oracle_s=round(t_oracle, 1), forms={})
+    for name, (precision, forcing, sur) in variants.items():
+        sub = argparse.Namespace(**vars(args))
+        sub.nl_steps = nsteps
+        r = run_newton(sub, d, precision, warm=0, steps=nsteps, total=args.c3_total, surrogate=sur, keep_state=True,
+                       forcing=forcing)
+        out["forms"][name] = dict(
+            u_rel_err=float(np.linalg.norm(r["u"] - X) / np.linalg.norm(X)),
+            stress_err_max_over_sigma_y=float((np.abs(r["stress"] - so[:, 20:26]).max(axis=1) / sy).max()))
+    return out
+
+
+def surrogate_curve(args, deck, params, start_state, X, up, tols):
+    """Accepted share of the queued Gauss points against the acceptance
+    tolerance, at the last timed step's material update of the plain FP64
+    run (its committed start state, its converged displacements), with the
+    time of the update and the stress error of the accepted points against
+    the plain return mapping."""
+    import numpy as np
+
+    from oofem_amd.hipfem import BatchedEvaluator, Context, HipCSRSparseMtrx, MisesSurrogate
+    ctx = Context(0)
+    ev = BatchedEvaluator(ctx, deck)
+    K = HipCSRSparseMtrx(ctx)
+    ev.build_structure(K)
+    m = deck.mat[0]
+    net = MisesSurrogate(ctx, m[1], m[2], m[3], m[4])
+    net.set_params(params)
+
+    def update(use, tol):
+        ev.set_state(start_state)
+        ev.restart()
+        ev.set_surrogate(net if use else None, band=args.sur_band, tol=tol)
+        if use:
+            ev.surrogate_stats(reset=True)
+        ctx.synchronize()
+        t = time.perf_counter()
+        ev.internal_forces(X, up, update_state=True)
+        ctx.synchronize()
+        dt = time.perf_counter() - t
+        return dt, ev.get_state()[:, 20:26].copy(), (ev.surrogate_stats() if use else None)
+
+    t_plain, s_plain, _ = update(False, 0.0)
+    sy = 250.0 + 2e3 * start_state.reshape(-1, 38)[:, 6]
+    curve = []
+    for tol in tols:
+        dt, s, stt = update(True, tol)
+        err = np.abs(s - s_plain).max(axis=1) / sy
+        curve.append(dict(tol=tol, queued=int(stt["queued"]), accepted=int(stt["accepted"]),
+                          accepted_share=stt["accepted"] / max(stt["queued"], 1),
+                          max_stress_err_over_sigma_y=float(err.max()), update_s=round(dt, 4)))
+    ev.set_surrogate(None)
+    net.close()
+    K.close()
+    ev.close()
+    ctx.close()
+    return dict(plain_update_s=round(t_plain, 4), points=int(s_plain.shape[0]), curve=curve)
 
 
 def spmv_roofline(args, deck, r):
@@ -344,62 +519,87 @@ def spmv_roofline(args, deck, r):
 
 def main_newton(args):
     """Newton-step throughput on the LSpace plasticity block (config 3):
-    plain FP64 (hipcsr + hippcg FP64 + exact MisesMat) against the mixed
-    solver and the mixed solver with the trained surrogate."""
+    --warmup untimed and --steps timed load steps of a ramp of
+    max(--c3-total, warmup + steps) steps, for every form of --variants:
+      fp64                     plain FP64 (hipcsr + hippcg FP64 + MisesMat) -- value
+      mixed                    FP32 PCG with FP64 reliable updates, FP32 element operator
+      mixed_surrogate          the same with the trained MisesMat surrogate
+      fp64_inexact             plain FP64 with Eisenstat-Walker inexact Newton
+      mixed_surrogate_inexact  mixed + surrogate with inexact Newton
+    Every variant's speed-up is given against the FP64 form with the same
+    Newton forcing (speedup_same_newton), and against plain FP64
+    (speedup_vs_plain_fp64)."""
     import numpy as np
 
     from oofem_amd import meshgen
     t = time.perf_counter()
     deck = meshgen.config3(nx=args.nx, ny=args.ny, nz=args.nz)
     log(f"[bench] deck {deck.name}: nelem={deck.nelem} neq={deck.neq} ({time.perf_counter() - t:.1f} s)")
-    kw = dict(warm=args.c3_warm, steps=args.c3_steps, total=args.c3_total)
-    runs = {"fp64": run_newton(args, deck, 0, keep_state=True, **kw)}
-    if args.forcing > 0:
-        runs["fp64_inexact"] = run_newton(args, deck, 0, keep_state=True, forcing=args.forcing, **kw)
-    if args.precision in ("both", "mixed", "all"):
-        runs["mixed"] = run_newton(args, deck, 2, **kw)
-        if args.forcing > 0:
-            runs["mixed_inexact"] = run_newton(args, deck, 2, keep_state=True, forcing=args.forcing, **kw)
-    sur_info = None
-    if args.precision in ("both", "all") and args.sur_steps > 0:
+    total = max(args.c3_total, args.warmup + args.steps)
+    kw = dict(warm=args.warmup, steps=args.steps, total=total)
+    want = [v for v in args.variants.split(",") if v]
+    if "fp64" not in want:
+        want.insert(0, "fp64")
+    params, sur_info = None, None
+    if any("surrogate" in v for v in want):
         params, sur_info = train_surrogate(args, deck)
-        name = "mixed_surrogate_inexact" if args.forcing > 0 else "mixed_surrogate"
-        runs[name] = run_newton(args, deck, 2, surrogate=(params, args.sur_band, args.sur_tol), keep_state=True,
-                                forcing=args.forcing, **kw)
+    sur = (params, args.sur_band, args.sur_tol) if params is not None else None
+    forms = {"fp64": (0, 0.0, None), "mixed": (2, 0.0, None), "mixed_surrogate": (2, 0.0, sur),
+             "fp64_inexact": (0, args.forcing, None), "mixed_inexact": (2, args.forcing, None),
+             "mixed_surrogate_inexact": (2, args.forcing, sur)}
+    runs = {}
+    for name in want:
+        precision, forcing, su = forms[name]
+        runs[name] = run_newton(args, deck, precision, keep_state=True, forcing=forcing, surrogate=su,
+                                keep_last_start=(name == "fp64" and sur is not None), **kw)
     base = runs["fp64"]
     n = deck.neq
+    dofs = {name: n * r["steps"] / r["dt"] for name, r in runs.items()}
     variants = {}
     for name, r in runs.items():
-        v = dict(dofs_per_s=n * r["steps"] / r["dt"], ms_per_step=r["dt"] / r["steps"] * 1e3,
-                 newton_iters=r["newton"], pcg_iters=r["pcg"])
+        v = dict(dofs_per_s=dofs[name], ms_per_step=r["dt"] / r["steps"] * 1e3, newton_iters=r["newton"],
+                 pcg_iters=r["pcg"], step_s=r["step_s"])
         if name != "fp64":
-            v["speedup_vs_fp64"] = v["dofs_per_s"] / (n * base["steps"] / base["dt"])
-            v["u_rel_err_vs_fp64"] = float(np.linalg.norm(r["u"] - base["u"]) / np.linalg.norm(base["u"]))
+            same = "fp64_inexact" if name.endswith("inexact") else "fp64"
+            if same in dofs:
+                v["speedup_same_newton"] = dofs[name] / dofs[same]
+                v["same_newton_baseline"] = same
+            v["speedup_vs_plain_fp64"] = dofs[name] / dofs["fp64"]
+            v["u_rel_err_vs_fp64_gpu"] = float(np.linalg.norm(r["u"] - base["u"]) / np.linalg.norm(base["u"]))
             v["reliable_updates_last_solve"] = r["info"]["outer"]
             v["element_operator"] = r["info"]["matrix_free"]
             if r["n_spmv32"]:
                 v["fp32_operator_avg_ms"] = r["ms_spmv32"] / r["n_spmv32"]
-        if "surrogate" in r:
-            v["surrogate"] = dict(r["surrogate"], band=args.sur_band, tol=args.sur_tol, training=sur_info)
-        if "stress" in r and name != "fp64":
             sy = 250.0 + 2e3 * base["kappa"]
             e = np.abs(r["stress"] - base["stress"]).max(axis=1) / sy
-            v["stress_err_vs_fp64"] = dict(max_rel=float(e.max()), rms_rel=float(np.sqrt((e * e).mean())))
+            v["stress_err_vs_fp64_gpu"] = dict(max_rel=float(e.max()), rms_rel=float(np.sqrt((e * e).mean())))
+        if "surrogate" in r:
+            v["surrogate"] = dict(r["surrogate"], band=args.sur_band, tol=args.sur_tol, training=sur_info)
         variants[name] = v
         log(f"[bench] {name}: {v}")
+    extra = {}
+    if sur is not None and "last_start" in base:
+        extra["surrogate_acceptance"] = surrogate_curve(args, deck, params, base["last_start"], base["u"],
+                                                        base["last_up"], [1e-4, 1e-3, 1e-2, 3e-2])
+        log(f"[bench] surrogate acceptance: {extra['surrogate_acceptance']}")
+    if args.check_nx > 0:
+        extra["oracle_check"] = oracle_check_newton(args, {name: forms[name] for name in want})
+        log(f"[bench] oracle check: {extra['oracle_check']}")
     r = base
     roof = spmv_roofline(args, deck, r)
+    for k in ("last_start",):
+        base.pop(k, None)
     cpu = None if args.no_cpu_baseline else cpu_baseline_newton(args, deck, r)
     out = {
-        "metric": METRIC, "value": n * r["steps"] / r["dt"], "unit": "DOFs/s", "n_gpus": 1, "steps": r["steps"],
-        "warmup": args.c3_warm, "ms_per_step": r["dt"] / r["steps"] * 1e3, "higher_is_better": True,
+        "metric": METRIC, "value": dofs["fp64"], "unit": "DOFs/s", "n_gpus": 1, "steps": r["steps"],
+        "warmup": args.warmup, "ms_per_step": r["dt"] / r["steps"] * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": deck.name, "elements": deck.nelem, "free_dofs": n, "nnz": r["nnz"],
-                   "step": (f"StaticStructural load step {args.c3_warm + 1}..{args.c3_warm + args.c3_steps} of "
-                            f"{args.c3_total} (prescribed end displacement, 5x yield strain in total): tangent "
-                            "initial guess, full Newton with consistent tangent, rtolf 1e-8, Jacobi-PCG to "
-                            f"{args.nl_lstol:g}"),
-                   "precision": "fp64", "variants": variants},
+                   "step": (f"StaticStructural load step of a {total}-step ramp of the prescribed end displacement "
+                            f"(5x the yield strain in total); steps {args.warmup + 1}..{args.warmup + args.steps} "
+                            "timed: tangent initial guess, full Newton with consistent tangent, rtolf 1e-8, "
+                            f"Jacobi-PCG to {args.nl_lstol:g}"),
+                   "precision": "fp64", "variants": variants, **extra},
         "roofline": roof, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
@@ -575,8 +775,9 @@ def cpu_baseline_rve(args, specs, t_gpu):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=4,
+                    help="untimed load steps (cfg3: the first four of the 20-step ramp reach the yield strain)")
     ap.add_argument("--workload", choices=["cfg1", "cfg0", "cfg2", "cfg3", "cfg4"], default="cfg3")
     ap.add_argument("--c2-warm", type=int, default=1, help="untimed load steps of config 2")
     ap.add_argument("--c2-steps", type=int, default=2, help="timed load steps of config 2")
@@ -592,9 +793,14 @@ def main():
     ap.add_argument("--rve-split", type=int, default=-1,
                     help="stored entries above which an RVE goes to the bulk phase (-1 automatic, 0 all)")
     ap.add_argument("--rve-check", type=int, default=8)
-    ap.add_argument("--c3-warm", type=int, default=1, help="untimed load steps before the timed ones (cfg3)")
-    ap.add_argument("--c3-steps", type=int, default=1, help="timed load steps (cfg3)")
-    ap.add_argument("--c3-total", type=int, default=8, help="load steps of the full ramp (cfg3)")
+    ap.add_argument("--c3-total", type=int, default=20,
+                    help="load steps of the ramp (cfg3; at least warmup + steps)")
+    ap.add_argument("--variants", default="fp64,mixed,mixed_surrogate,fp64_inexact,mixed_surrogate_inexact",
+                    help="cfg3 forms to run (comma separated)")
+    ap.add_argument("--check-nx", type=int, default=10,
+                    help="cfg3 oracle check on an n x n x 4n slab (0: off)")
+    ap.add_argument("--check-steps", type=int, default=6, help="load steps of the oracle check")
+    ap.add_argument("--cpu-cg-iters", type=int, default=5, help="CG iterations timed on the full mesh")
     ap.add_argument("--sur-samples", type=int, default=20_000_000)
     ap.add_argument("--sur-steps", type=int, default=20000)
     ap.add_argument("--sur-batch", type=int, default=4096)
@@ -617,6 +823,8 @@ def main():
     ap.add_argument("--shape", default="", help="SpMV shape: lanes:L:U[:nt] or stream:R:U")
     ap.add_argument("--mode", choices=["step", "kernels"], default="step")
     ap.add_argument("--iters", type=int, default=50, help="PCG iterations in --mode kernels")
+    ap.add_argument("--kernels-plastic", type=int, default=1,
+                    help="--mode kernels on cfg3: assemble the MisesMat tangent of a plastic state (1) or K (0)")
     ap.add_argument("--newton-steps", type=int, default=2, help="timed Newton steps of the tet plasticity variant")
     ap.add_argument("--nl-total", type=int, default=20, help="load steps of the full ramp")
     ap.add_argument("--nl-lstol", type=float, default=1e-6, help="PCG tolerance inside Newton")

@@ -307,6 +307,10 @@ int hfem_model_restart(hfem_model *md);
 int hfem_model_reset_state(hfem_model *md);
 /* Gauss point state, ngp_total x 38 doubles in the oracle's layout */
 int hfem_model_get_state(hfem_model *md, double *state);
+/* the inverse of hfem_model_get_state: every Gauss point's 38 fields
+ * (a restart from a saved state, as the reference's status objects are
+ * restored from a context file) */
+int hfem_model_set_state(hfem_model *md, const double *state);
 int hfem_model_ngp_total(const hfem_model *md);
 int hfem_model_ncolors(const hfem_model *md);
 

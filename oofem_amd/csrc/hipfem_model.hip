@@ -1917,5 +1917,18 @@ extern "C" int hfem_model_get_state(hfem_model *md, double *state)
     return 0;
 }
 
+extern "C" int hfem_model_set_state(hfem_model *md, const double *state)
+{
+    HIPFEM_REQUIRE(md && md->d_state && state, HIPFEM_EINVAL, "set_state: bad arguments");
+    DeviceGuard g(md->ctx->device);
+    const size_t n = md->ngp_total;
+    std::vector<double> soa(NST * std::max<size_t>(1, n));
+    for (size_t g2 = 0; g2 < n; g2++)
+        for (int k = 0; k < NST; k++) soa[k * n + g2] = state[g2 * NST + k];
+    HIPFEM_CHECK(hipStreamSynchronize(md->ctx->stream));
+    HIPFEM_CHECK(hipMemcpy(md->d_state, soa.data(), sizeof(double) * NST * n, hipMemcpyHostToDevice));
+    return 0;
+}
+
 extern "C" int hfem_model_ngp_total(const hfem_model *md) { return md ? md->ngp_total : 0; }
 extern "C" int hfem_model_ncolors(const hfem_model *md) { return md ? md->ncolors : 0; }

@@ -496,11 +496,22 @@ class BatchedEvaluator:
         _check(lib().hfem_model_surrogate_stats(self.h, _p(c), int(bool(reset))), "surrogate_stats")
         return dict(queued=int(c[0]), accepted=int(c[1]), rejected=int(c[2]))
 
+    def restart(self):
+        """Temp Gauss-point values := committed (EngngModel::initStepIncrements)."""
+        _check(lib().hfem_model_restart(self.h), "restart")
+
     def commit(self):
         _check(lib().hfem_model_commit(self.h), "commit")
 
     def reset_state(self):
         _check(lib().hfem_model_reset_state(self.h), "reset_state")
+
+    def set_state(self, state):
+        """Gauss-point state (ngp x 38, as get_state returns it)."""
+        st = np.ascontiguousarray(state, dtype=np.float64)
+        if st.size != lib().hfem_model_ngp_total(self.h) * 38:
+            raise ValueError("state size mismatch")
+        _check(lib().hfem_model_set_state(self.h, _p(st)), "set_state")
 
     def get_state(self):
         n = lib().hfem_model_ngp_total(self.h)

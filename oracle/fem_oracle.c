@@ -11,6 +11,10 @@
  */
 #include "fem_oracle.h"
 
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -595,6 +599,43 @@ int orc_assemble_stiffness(const orc_mesh *m, const long long *rowptr, const int
     return rc;
 }
 
+int orc_assemble_stiffness_omp(const orc_mesh *m, const long long *rowptr, const int *colind, double *val,
+                               const double *state, int tangent)
+{
+    const int nen = orc_nen(m->etype), nc = 3 * nen, ngp = orc_ngp(m->etype);
+    memset(val, 0, sizeof(double) * (size_t)rowptr[m->neq]);
+    int rc = 0;
+#pragma omp parallel
+    {
+        double X[60], *Ke = (double *)malloc(sizeof(double) * nc * nc), *Dgp = (double *)malloc(sizeof(double) * 36 * ngp);
+        int loc[60];
+#pragma omp for schedule(static)
+        for (int e = 0; e < m->nelem; e++) {
+            int sym = 1;
+            gather_coords(m, e, X);
+            element_D(m, e, state, tangent, Dgp, &sym);
+            const int bad = orc_element_stiffness(m->etype, X, Dgp, 36, sym, Ke);
+            location(m, e, loc, NULL);
+#pragma omp critical
+            {
+                if (bad) rc = -1 - e;
+                for (int i = 0; i < nc && !bad; i++) {
+                    if (!loc[i]) continue;
+                    for (int j = 0; j < nc; j++) {
+                        if (!loc[j]) continue;
+                        const long long p = find_pos(rowptr, colind, loc[i] - 1, loc[j] - 1);
+                        if (p < 0) { rc = -2; break; }
+                        val[p] += Ke[i * nc + j];
+                    }
+                }
+            }
+        }
+        free(Ke);
+        free(Dgp);
+    }
+    return rc;
+}
+
 /* CompCol::times compcol.C:128 restated row-wise: y = A x */
 void orc_spmv(int n, const long long *rowptr, const int *colind, const double *val,
               const double *x, double *y)
@@ -745,63 +786,76 @@ void orc_state_restart(const orc_mesh *m, double *state)
     }
 }
 
+/* one element of orc_internal_forces: f_e = sum_gp B^T sigma dV with the
+ * Gauss-point state update; loc / ploc its free / prescribed equations */
+static int elem_internal_forces(const orc_mesh *m, int e, const double *u_free, const double *u_presc,
+                                double *state, int update_state, const double *xi, const double *w, double *fe,
+                                int *loc, int *ploc)
+{
+    const int nen = orc_nen(m->etype), nc = 3 * nen, ngp = orc_ngp(m->etype);
+    double X[60], dNdx[60], B[6 * 60], ue[60];
+    const double *mp = m->mat + (size_t)ORC_MATP * m->emat[e];
+    gather_coords(m, e, X);
+    location(m, e, loc, ploc);
+    for (int i = 0; i < nc; i++)
+        ue[i] = loc[i] ? u_free[loc[i] - 1] : (ploc[i] && u_presc ? u_presc[ploc[i] - 1] : 0.0);
+    memset(fe, 0, sizeof(double) * nc);
+    for (int g = 0; g < ngp; g++) {
+        int err = 0;
+        orc_dndx(m->etype, X, xi + 3 * g, dNdx, &err);
+        if (err) return -1 - e;
+        orc_bmatrix(nen, dNdx, B);
+        double eps[6], sig[6];
+        for (int i = 0; i < 6; i++) {
+            double s = 0.;
+            for (int j = 0; j < nc; j++) s += B[i * nc + j] * ue[j];
+            eps[i] = s;
+        }
+        double *st = state ? state + (size_t)ORC_GPSTATE * ((size_t)e * ngp + g) : NULL;
+        if ((int)mp[0] == ORC_MAT_MISES) {
+            double epsp_t[6], kappa_t, sdev[6];
+            orc_mises_return(mp, eps, st ? st : (const double[6]){0, 0, 0, 0, 0, 0},
+                             st ? st[6] : 0.0, sig, epsp_t, &kappa_t, sdev);
+            if (st && update_state) {
+                for (int i = 0; i < 6; i++) { st[7 + i] = epsp_t[i]; st[14 + i] = sdev[i]; }
+                st[13] = kappa_t;
+            }
+        } else {
+            /* LinearElasticMaterial::giveRealStressVector_3d linearelasticmaterial.C:102-124 */
+            double D[36];
+            orc_isole_D(mp[1], mp[2], D);
+            for (int i = 0; i < 6; i++) {
+                double s = 0.;
+                for (int k = 0; k < 6; k++) s += D[6 * i + k] * eps[k];
+                sig[i] = s;
+            }
+        }
+        if (st && update_state)
+            for (int i = 0; i < 6; i++) { st[20 + i] = sig[i]; st[26 + i] = eps[i]; }
+        const double dV = fabs(orc_detj(m->etype, X, xi + 3 * g)) * w[g];
+        for (int i = 0; i < nc; i++) {
+            double s = 0.;
+            for (int k = 0; k < 6; k++) s += B[k * nc + i] * sig[k];
+            fe[i] += s * dV;
+        }
+    }
+    return 0;
+}
+
 int orc_internal_forces(const orc_mesh *m, const double *u_free, const double *u_presc,
                         double *state, int update_state,
                         double *f_free, double *f_presc, double *ebe_norm3)
 {
-    const int nen = orc_nen(m->etype), nc = 3 * nen, ngp = orc_ngp(m->etype);
-    double xi[81], w[27], X[60], dNdx[60], B[6 * 60], ue[60], fe[60];
+    const int nc = 3 * orc_nen(m->etype);
+    double xi[81], w[27], fe[60];
     int loc[60], ploc[60];
     orc_gauss(m->etype, xi, w);
     if (f_free) memset(f_free, 0, sizeof(double) * (size_t)m->neq);
     if (f_presc) memset(f_presc, 0, sizeof(double) * (size_t)m->npeq);
     if (ebe_norm3) ebe_norm3[0] = ebe_norm3[1] = ebe_norm3[2] = 0.0;
     for (int e = 0; e < m->nelem; e++) {
-        const double *mp = m->mat + (size_t)ORC_MATP * m->emat[e];
-        gather_coords(m, e, X);
-        location(m, e, loc, ploc);
-        for (int i = 0; i < nc; i++)
-            ue[i] = loc[i] ? u_free[loc[i] - 1] : (ploc[i] && u_presc ? u_presc[ploc[i] - 1] : 0.0);
-        memset(fe, 0, sizeof fe);
-        for (int g = 0; g < ngp; g++) {
-            int err = 0;
-            orc_dndx(m->etype, X, xi + 3 * g, dNdx, &err);
-            if (err) return -1 - e;
-            orc_bmatrix(nen, dNdx, B);
-            double eps[6], sig[6];
-            for (int i = 0; i < 6; i++) {
-                double s = 0.;
-                for (int j = 0; j < nc; j++) s += B[i * nc + j] * ue[j];
-                eps[i] = s;
-            }
-            double *st = state ? state + (size_t)ORC_GPSTATE * ((size_t)e * ngp + g) : NULL;
-            if ((int)mp[0] == ORC_MAT_MISES) {
-                double epsp_t[6], kappa_t, sdev[6];
-                orc_mises_return(mp, eps, st ? st : (const double[6]){0, 0, 0, 0, 0, 0},
-                                 st ? st[6] : 0.0, sig, epsp_t, &kappa_t, sdev);
-                if (st && update_state) {
-                    for (int i = 0; i < 6; i++) { st[7 + i] = epsp_t[i]; st[14 + i] = sdev[i]; }
-                    st[13] = kappa_t;
-                }
-            } else {
-                /* LinearElasticMaterial::giveRealStressVector_3d linearelasticmaterial.C:102-124 */
-                double D[36];
-                orc_isole_D(mp[1], mp[2], D);
-                for (int i = 0; i < 6; i++) {
-                    double s = 0.;
-                    for (int k = 0; k < 6; k++) s += D[6 * i + k] * eps[k];
-                    sig[i] = s;
-                }
-            }
-            if (st && update_state)
-                for (int i = 0; i < 6; i++) { st[20 + i] = sig[i]; st[26 + i] = eps[i]; }
-            const double dV = fabs(orc_detj(m->etype, X, xi + 3 * g)) * w[g];
-            for (int i = 0; i < nc; i++) {
-                double s = 0.;
-                for (int k = 0; k < 6; k++) s += B[k * nc + i] * sig[k];
-                fe[i] += s * dV;
-            }
-        }
+        const int rc = elem_internal_forces(m, e, u_free, u_presc, state, update_state, xi, w, fe, loc, ploc);
+        if (rc) return rc;
         for (int i = 0; i < nc; i++) {
             if (loc[i] && f_free) f_free[loc[i] - 1] += fe[i];
             if (ploc[i] && f_presc) f_presc[ploc[i] - 1] += fe[i];
@@ -809,6 +863,50 @@ int orc_internal_forces(const orc_mesh *m, const double *u_free, const double *u
         }
     }
     return 0;
+}
+
+/* The reference's OpenMP build (USE_OPENMP_PARALLEL) of the two element
+ * loops: EngngModel::assembleVectorFromElements engngm.C:1375-1408 and
+ * EngngModel::assemble engngm.C:902-923 -- the elements shared out by
+ * "omp parallel for", each element's vector or matrix scattered inside one
+ * "omp critical" section.  The order of the additions then depends on the
+ * schedule, as it does there: these are timing baselines of the CPU path,
+ * not the restatement the parity tests compare with. */
+int orc_omp_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+int orc_internal_forces_omp(const orc_mesh *m, const double *u_free, const double *u_presc, double *state,
+                            int update_state, double *f_free, double *f_presc, double *ebe_norm3)
+{
+    const int nc = 3 * orc_nen(m->etype);
+    double xi[81], w[27];
+    orc_gauss(m->etype, xi, w);
+    if (f_free) memset(f_free, 0, sizeof(double) * (size_t)m->neq);
+    if (f_presc) memset(f_presc, 0, sizeof(double) * (size_t)m->npeq);
+    if (ebe_norm3) ebe_norm3[0] = ebe_norm3[1] = ebe_norm3[2] = 0.0;
+    int rc = 0;
+#pragma omp parallel for schedule(static)
+    for (int e = 0; e < m->nelem; e++) {
+        double fe[60];
+        int loc[60], ploc[60];
+        const int r = elem_internal_forces(m, e, u_free, u_presc, state, update_state, xi, w, fe, loc, ploc);
+#pragma omp critical
+        {
+            if (r) rc = r;
+            for (int i = 0; i < nc; i++) {
+                if (loc[i] && f_free) f_free[loc[i] - 1] += fe[i];
+                if (ploc[i] && f_presc) f_presc[ploc[i] - 1] += fe[i];
+                if (ebe_norm3) ebe_norm3[i % 3] += fe[i] * fe[i];
+            }
+        }
+    }
+    return rc;
 }
 
 /* ------------------------------------------------------------------ */

@@ -100,6 +100,14 @@ void orc_state_init(const orc_mesh *m, double *state);
 void orc_state_commit(const orc_mesh *m, double *state);
 void orc_state_restart(const orc_mesh *m, double *state);
 
+/* OpenMP element loops as the reference's USE_OPENMP_PARALLEL build runs
+ * them (parallel elements, critical-section scatter): CPU timing baselines */
+int orc_omp_threads(void);
+int orc_internal_forces_omp(const orc_mesh *m, const double *u_free, const double *u_presc, double *state,
+                            int update_state, double *f_free, double *f_presc, double *ebe_norm3);
+int orc_assemble_stiffness_omp(const orc_mesh *m, const long long *rowptr, const int *colind, double *val,
+                               const double *state, int tangent);
+
 /* surface traction (ConstantSurfaceLoad, global c.s.) on (element, surface) pairs */
 int orc_surface_load(const orc_mesh *m, int nface, const int *elem, const int *surf,
                      const double *traction, double *f_free, double *f_presc);

@@ -117,21 +117,41 @@ def mises_tangent(mp, kappa_n, kappa_t, sdev):
     return D.reshape(6, 6)
 
 
-def csr(deck, state=None, tangent=False):
-    """Assembled free-free stiffness (rowptr int64, colind int32, val) in the
-    reference's element order."""
+def csr_structure(deck):
+    """Free-free pattern (rowptr int64, colind int32) of the deck's elements."""
     m = Mesh(deck)
     nnz = lib().orc_csr_count(m.ref)
     rowptr = np.zeros(deck.neq + 1, dtype=np.int64)
     colind = np.zeros(nnz, dtype=np.int32)
-    val = np.zeros(nnz)
     lib().orc_csr_fill(m.ref, rowptr.ctypes.data_as(C.c_void_p), colind.ctypes.data_as(C.c_void_p))
+    return rowptr, colind
+
+
+def assemble(deck, rowptr, colind, state=None, tangent=False, omp=False, val=None):
+    """Stiffness values on a pattern of csr_structure: the reference's
+    element order, or (omp) its OpenMP element loop with critical-section
+    scatter (timing baseline)."""
+    m = Mesh(deck)
+    if val is None:
+        val = np.zeros(colind.shape[0])
     st = state.ctypes.data_as(C.c_void_p) if state is not None else None
-    rc = lib().orc_assemble_stiffness(m.ref, rowptr.ctypes.data_as(C.c_void_p), colind.ctypes.data_as(C.c_void_p),
-                                      val.ctypes.data_as(C.c_void_p), st, int(tangent))
+    fn = lib().orc_assemble_stiffness_omp if omp else lib().orc_assemble_stiffness
+    rc = fn(m.ref, rowptr.ctypes.data_as(C.c_void_p), colind.ctypes.data_as(C.c_void_p),
+            val.ctypes.data_as(C.c_void_p), st, int(tangent))
     if rc:
         raise RuntimeError(f"orc_assemble_stiffness rc={rc}")
-    return rowptr, colind, val
+    return val
+
+
+def omp_threads():
+    return int(lib().orc_omp_threads())
+
+
+def csr(deck, state=None, tangent=False):
+    """Assembled free-free stiffness (rowptr int64, colind int32, val) in the
+    reference's element order."""
+    rowptr, colind = csr_structure(deck)
+    return rowptr, colind, assemble(deck, rowptr, colind, state=state, tangent=tangent)
 
 
 def spmv(rowptr, colind, val, x):
@@ -229,16 +249,18 @@ def state_restart(deck, state):
     lib().orc_state_restart(m.ref, state.ctypes.data_as(C.c_void_p))
 
 
-def internal_forces(deck, u_free, u_presc=None, state=None, update_state=False):
+def internal_forces(deck, u_free, u_presc=None, state=None, update_state=False, omp=False):
+    """f_int (free, prescribed) and the per-dof-id EBE norms; omp: the
+    reference's OpenMP element loop (timing baseline)."""
     m = Mesh(deck)
     f = np.zeros(deck.neq)
     fp = np.zeros(deck.npeq)
     ebe = np.zeros(3)
     up = _d(u_presc) if u_presc is not None else np.zeros(deck.npeq)
     st = state.ctypes.data_as(C.c_void_p) if state is not None else None
-    rc = lib().orc_internal_forces(m.ref, _d(u_free).ctypes.data_as(C.c_void_p), up.ctypes.data_as(C.c_void_p),
-                                   st, int(update_state), f.ctypes.data_as(C.c_void_p),
-                                   fp.ctypes.data_as(C.c_void_p), ebe.ctypes.data_as(C.c_void_p))
+    fn = lib().orc_internal_forces_omp if omp else lib().orc_internal_forces
+    rc = fn(m.ref, _d(u_free).ctypes.data_as(C.c_void_p), up.ctypes.data_as(C.c_void_p), st, int(update_state),
+            f.ctypes.data_as(C.c_void_p), fp.ctypes.data_as(C.c_void_p), ebe.ctypes.data_as(C.c_void_p))
     if rc:
         raise RuntimeError(f"orc_internal_forces rc={rc}")
     return f, fp, ebe

@@ -1024,3 +1024,23 @@ def test_constrained_newton_halves_growing_corrections():
     assert res[100][0] == res[0][0] and np.array_equal(res[100][1], res[0][1])
     assert res[1][0]["pcg"] != res[0][0]["pcg"]
     assert np.linalg.norm(res[1][1] - res[0][1]) <= 1e-7 * np.linalg.norm(res[0][1])
+
+
+def test_openmp_element_loops_equal_the_serial_ones():
+    """The OpenMP element loops of the CPU baseline (the reference's
+    USE_OPENMP_PARALLEL form) compute what the serial restatement computes,
+    up to the order of the critical-section additions."""
+    d = meshgen.config3(6, 6, 24)
+    up = d.prescribed_values(3 / 8)
+    u = np.random.default_rng(3).standard_normal(d.neq) * 1e-3
+    s1, s2 = oracle.state_new(d), oracle.state_new(d)
+    f1, fp1, e1 = oracle.internal_forces(d, u, up, state=s1, update_state=True)
+    f2, fp2, e2 = oracle.internal_forces(d, u, up, state=s2, update_state=True, omp=True)
+    assert np.abs(f1 - f2).max() <= 1e-13 * np.abs(f1).max()
+    assert np.abs(fp1 - fp2).max() <= 1e-13 * np.abs(fp1).max()
+    np.testing.assert_allclose(e2, e1, rtol=1e-13)
+    np.testing.assert_array_equal(s1, s2)
+    rp, ci = oracle.csr_structure(d)
+    v1 = oracle.assemble(d, rp, ci, s1, tangent=True)
+    v2 = oracle.assemble(d, rp, ci, s1, tangent=True, omp=True)
+    assert np.abs(v1 - v2).max() <= 1e-13 * np.abs(v1).max()

@@ -7,8 +7,9 @@ sig0 250, H 2e3), clamped at z = 0, u_z prescribed on z = 4 and ramped to
 5x the yield strain over 20 load steps (more when --warmup + --steps
 exceed 20).  One step = one StaticStructural load step: tangent initial
 guess, full Newton with the consistent tangent to the 1e-8 force criterion,
-Jacobi-PCG to 1e-6 per correction.  --warmup untimed steps, then --steps
-timed ones, for every form of --variants:
+Jacobi-PCG to 1e-6 per correction.  The plain FP64 form runs --warmup
+untimed steps, then --steps timed ones; every other form of --variants runs
+the same warm-up and the first --compare-steps of those timed steps:
   fp64                     hipcsr + hippcg FP64 + plain return mapping (value)
   mixed                    hippcg mixed 2: FP32 PCG with FP64 reliable updates
                            and the FP32 element-by-element operator
@@ -17,10 +18,10 @@ timed ones, for every form of --variants:
   fp64_inexact, mixed_surrogate_inexact
                            the same two with Eisenstat-Walker inexact Newton
 value = free DOFs x timed steps / wall time of the FP64 form; every other
-form's speed-up against the FP64 form with the same Newton forcing, its
-errors against plain FP64, the errors of every form against the FP64 oracle
-on a slab, and the surrogate's acceptance against its tolerance are in
-config.
+form's speed-up against the FP64 form with the same Newton forcing over the
+same load steps, its errors against plain FP64, the errors of every form
+against the FP64 oracle on a slab, and the surrogate's acceptance against
+its tolerance are in config.
 
 roofline: the FP64 PCG SpMV (row-group form on this pattern: the top face
 constrains u_z only), algorithmic bytes per launch over its mean HIP-event
@@ -125,6 +126,7 @@ def run_kernels(args, deck):
     db = DeviceVector(ctx, deck.neq, f)
     dx = DeviceVector(ctx, deck.neq)
     out = {}
+    iter_ms = {}
     for prec, name in ((0, "fp64_nodeblock"), (1, "fp32_nodeblock"), (2, "fp32_element_operator")):
         s = HipPCGSolver(ctx, lstol=1e-30, lsiter=args.iters, lsprecond=1, precision=prec, inner_tol=1e-30)
         s.set_profiling(True)
@@ -133,9 +135,27 @@ def run_kernels(args, deck):
         p = s.profile()
         out[name] = (p["ms64"] / max(p["n64"], 1)) if prec == 0 else (p["ms32"] / max(p["n32"], 1))
         s.close()
+    # whole PCG iterations (wall clock over --iters iterations): the plain
+    # PCG with p updated in its own pass (fused 0) and formed inside the
+    # SpMV gather (fused 1), and the mixed form
+    for prec, fl in ((0, (0, 1)), (2, (0,))):
+        for flags in fl:
+            s = HipPCGSolver(ctx, lstol=1e-30, lsiter=args.iters, lsprecond=1, precision=prec, inner_tol=1e-30)
+            s.set_fused(flags)
+            dx.fill(0.0)
+            s.solve_dev(K, db.addr, dx.addr)
+            ctx.synchronize()
+            dx.fill(0.0)
+            ctx.synchronize()
+            t = time.perf_counter()
+            _, nit, _ = s.solve_dev(K, db.addr, dx.addr)
+            ctx.synchronize()
+            iter_ms[f"precision{prec}_flags{flags}"] = (time.perf_counter() - t) * 1e3 / max(nit, 1)
+            s.close()
+    log(f"[bench] kernels: ms per PCG iteration {iter_ms}")
     print(json.dumps(dict(mode="kernels", workload=deck.name, plastic_tangent=plastic, neq=deck.neq, nnz=K.nnz(),
                           shape=K.spmv_shape(), copy_doubles=n_copy,
-                          copy_gbs=copy_gbs, spmv_avg_ms=out)), flush=True)
+                          copy_gbs=copy_gbs, spmv_avg_ms=out, iteration_ms=iter_ms)), flush=True)
     return 0
 
 
@@ -171,12 +191,14 @@ def run_gpu(args, deck, precision):
 
 
 def run_newton(args, deck, precision, warm=1, steps=None, total=None, surrogate=None, keep_state=False, forcing=0.0,
-               initial_guess=1, keep_last_start=False):
+               initial_guess=1, keep_last_start=False, snap_step=None):
     """StaticStructural + NRSolver load steps (full Newton, consistent tangent,
     rtolf 1e-8, tangent initial guess) with the prescribed end displacement
     ramped over `total` steps: `warm` untimed steps, then `steps` timed ones.
     surrogate: (params, band, tol) of a trained MisesMat network, or None;
-    forcing > 0: inexact Newton with forcing terms up to that value."""
+    forcing > 0: inexact Newton with forcing terms up to that value.
+    snap_step: also keep the solution and Gauss-point stress at the end of
+    that load step (the copies are taken out of the timed region)."""
     import numpy as np
 
     from oofem_amd.problem import StructuralProblem
@@ -196,6 +218,7 @@ def run_newton(args, deck, precision, warm=1, steps=None, total=None, surrogate=
     reports = []
     t0 = None
     last_start = None
+    snap = None
     t_excluded = 0.0
     for k in range(1, warm + steps + 1):
         if keep_last_start and k == warm + steps:
@@ -204,7 +227,7 @@ def run_newton(args, deck, precision, warm=1, steps=None, total=None, surrogate=
             t_copy = time.perf_counter()
             last_start = P.state()
             if t0 is not None:
-                t_excluded = time.perf_counter() - t_copy
+                t_excluded += time.perf_counter() - t_copy
         P.set_prescribed(deck.prescribed_values(k / total))
         if k == warm + 1:
             P.set_profiling(True)
@@ -218,6 +241,12 @@ def run_newton(args, deck, precision, warm=1, steps=None, total=None, surrogate=
         P.update()
         rep["wall_s"] = time.perf_counter() - ts
         reports.append(rep)
+        if k == snap_step:
+            t_copy = time.perf_counter()
+            gs = P.state()
+            snap = dict(u=P.solution(), stress=gs[:, 20:26].copy(), kappa=gs[:, 6].copy())
+            del gs
+            t_excluded += time.perf_counter() - t_copy
         log(f"[bench] precision {precision}{' +surrogate' if net else ''}{' +forcing' if forcing else ''} "
             f"newton step {k}: {rep}")
     dt = time.perf_counter() - t0 - t_excluded
@@ -233,6 +262,8 @@ def run_newton(args, deck, precision, warm=1, steps=None, total=None, surrogate=
         out["stress"] = gs[:, 20:26].copy()
         out["kappa"] = gs[:, 6].copy()
         del gs
+    if snap is not None:
+        out["snap"] = snap
     if last_start is not None:
         out["last_start"] = last_start
         out["last_up"] = deck.prescribed_values((warm + steps) / total)
@@ -518,17 +549,23 @@ def spmv_roofline(args, deck, r):
 
 
 def main_newton(args):
-    """Newton-step throughput on the LSpace plasticity block (config 3):
-    --warmup untimed and --steps timed load steps of a ramp of
-    max(--c3-total, warmup + steps) steps, for every form of --variants:
-      fp64                     plain FP64 (hipcsr + hippcg FP64 + MisesMat) -- value
+    """Newton-step throughput on the LSpace plasticity block (config 3).
+    The plain FP64 form (hipcsr + hippcg FP64 + MisesMat: the reference's
+    algorithm, whose rate is `value`) runs --warmup untimed and --steps timed
+    load steps of a ramp of max(--c3-total, warmup + steps) steps.  Every
+    other form of --variants runs the same warm-up and the first
+    --compare-steps of those timed steps (a plastic load step of this block
+    costs about 20 s in FP64, so every form over every requested step would
+    hold the GPU for many minutes):
       mixed                    FP32 PCG with FP64 reliable updates, FP32 element operator
       mixed_surrogate          the same with the trained MisesMat surrogate
       fp64_inexact             plain FP64 with Eisenstat-Walker inexact Newton
       mixed_surrogate_inexact  mixed + surrogate with inexact Newton
-    Every variant's speed-up is given against the FP64 form with the same
-    Newton forcing (speedup_same_newton), and against plain FP64
-    (speedup_vs_plain_fp64)."""
+    A form's speed-up is its time over the compared load steps against the
+    time of the FP64 form with the same Newton forcing over the same load
+    steps from the same start (speedup_same_newton), and against plain FP64
+    (speedup_vs_plain_fp64); its displacement and stress errors are taken
+    against plain FP64 at the end of the compared steps."""
     import numpy as np
 
     from oofem_amd import meshgen
@@ -536,7 +573,7 @@ def main_newton(args):
     deck = meshgen.config3(nx=args.nx, ny=args.ny, nz=args.nz)
     log(f"[bench] deck {deck.name}: nelem={deck.nelem} neq={deck.neq} ({time.perf_counter() - t:.1f} s)")
     total = max(args.c3_total, args.warmup + args.steps)
-    kw = dict(warm=args.warmup, steps=args.steps, total=total)
+    ncmp = max(1, min(args.compare_steps, args.steps))
     want = [v for v in args.variants.split(",") if v]
     if "fp64" not in want:
         want.insert(0, "fp64")
@@ -550,34 +587,46 @@ def main_newton(args):
     runs = {}
     for name in want:
         precision, forcing, su = forms[name]
-        runs[name] = run_newton(args, deck, precision, keep_state=True, forcing=forcing, surrogate=su,
-                                keep_last_start=(name == "fp64" and sur is not None), **kw)
+        if name == "fp64":
+            runs[name] = run_newton(args, deck, precision, warm=args.warmup, steps=args.steps, total=total,
+                                    keep_state=True, forcing=forcing, surrogate=su,
+                                    keep_last_start=sur is not None, snap_step=args.warmup + ncmp)
+        else:
+            runs[name] = run_newton(args, deck, precision, warm=args.warmup, steps=ncmp, total=total,
+                                    keep_state=True, forcing=forcing, surrogate=su)
     base = runs["fp64"]
     n = deck.neq
-    dofs = {name: n * r["steps"] / r["dt"] for name, r in runs.items()}
+    # the plain FP64 reference over the compared steps, and its state there
+    ref_cmp = dict(dt=sum(base["step_s"][:ncmp]), u=base["snap"]["u"], stress=base["snap"]["stress"],
+                   kappa=base["snap"]["kappa"])
+    cmp_dt = {name: (ref_cmp["dt"] if name == "fp64" else sum(r["step_s"])) for name, r in runs.items()}
     variants = {}
     for name, r in runs.items():
-        v = dict(dofs_per_s=dofs[name], ms_per_step=r["dt"] / r["steps"] * 1e3, newton_iters=r["newton"],
-                 pcg_iters=r["pcg"], step_s=r["step_s"])
-        if name != "fp64":
+        if name == "fp64":
+            v = dict(dofs_per_s=n * r["steps"] / r["dt"], ms_per_step=r["dt"] / r["steps"] * 1e3,
+                     newton_iters=r["newton"], pcg_iters=r["pcg"], step_s=r["step_s"],
+                     compared_steps_s=ref_cmp["dt"])
+        else:
+            v = dict(dofs_per_s=n * r["steps"] / r["dt"], ms_per_step=r["dt"] / r["steps"] * 1e3,
+                     newton_iters=r["newton"], pcg_iters=r["pcg"], step_s=r["step_s"])
             same = "fp64_inexact" if name.endswith("inexact") else "fp64"
-            if same in dofs:
-                v["speedup_same_newton"] = dofs[name] / dofs[same]
+            if same in cmp_dt:
+                v["speedup_same_newton"] = cmp_dt[same] / cmp_dt[name]
                 v["same_newton_baseline"] = same
-            v["speedup_vs_plain_fp64"] = dofs[name] / dofs["fp64"]
-            v["u_rel_err_vs_fp64_gpu"] = float(np.linalg.norm(r["u"] - base["u"]) / np.linalg.norm(base["u"]))
+            v["speedup_vs_plain_fp64"] = cmp_dt["fp64"] / cmp_dt[name]
+            v["u_rel_err_vs_fp64_gpu"] = float(np.linalg.norm(r["u"] - ref_cmp["u"]) / np.linalg.norm(ref_cmp["u"]))
             v["reliable_updates_last_solve"] = r["info"]["outer"]
             v["element_operator"] = r["info"]["matrix_free"]
             if r["n_spmv32"]:
                 v["fp32_operator_avg_ms"] = r["ms_spmv32"] / r["n_spmv32"]
-            sy = 250.0 + 2e3 * base["kappa"]
-            e = np.abs(r["stress"] - base["stress"]).max(axis=1) / sy
+            sy = 250.0 + 2e3 * ref_cmp["kappa"]
+            e = np.abs(r["stress"] - ref_cmp["stress"]).max(axis=1) / sy
             v["stress_err_vs_fp64_gpu"] = dict(max_rel=float(e.max()), rms_rel=float(np.sqrt((e * e).mean())))
         if "surrogate" in r:
             v["surrogate"] = dict(r["surrogate"], band=args.sur_band, tol=args.sur_tol, training=sur_info)
         variants[name] = v
         log(f"[bench] {name}: {v}")
-    extra = {}
+    extra = {"compared_load_steps": [args.warmup + 1, args.warmup + ncmp]}
     if sur is not None and "last_start" in base:
         extra["surrogate_acceptance"] = surrogate_curve(args, deck, params, base["last_start"], base["u"],
                                                         base["last_up"], [1e-4, 1e-3, 1e-2, 3e-2])
@@ -587,18 +636,18 @@ def main_newton(args):
         log(f"[bench] oracle check: {extra['oracle_check']}")
     r = base
     roof = spmv_roofline(args, deck, r)
-    for k in ("last_start",):
+    for k in ("last_start", "snap"):
         base.pop(k, None)
     cpu = None if args.no_cpu_baseline else cpu_baseline_newton(args, deck, r)
     out = {
-        "metric": METRIC, "value": dofs["fp64"], "unit": "DOFs/s", "n_gpus": 1, "steps": r["steps"],
+        "metric": METRIC, "value": n * r["steps"] / r["dt"], "unit": "DOFs/s", "n_gpus": 1, "steps": r["steps"],
         "warmup": args.warmup, "ms_per_step": r["dt"] / r["steps"] * 1e3, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": deck.name, "elements": deck.nelem, "free_dofs": n, "nnz": r["nnz"],
                    "step": (f"StaticStructural load step of a {total}-step ramp of the prescribed end displacement "
-                            f"(5x the yield strain in total); steps {args.warmup + 1}..{args.warmup + args.steps} "
-                            "timed: tangent initial guess, full Newton with consistent tangent, rtolf 1e-8, "
-                            f"Jacobi-PCG to {args.nl_lstol:g}"),
+                            f"(5x the yield strain over 20 steps); steps {args.warmup + 1}.."
+                            f"{args.warmup + args.steps} timed: tangent initial guess, full Newton with consistent "
+                            f"tangent, rtolf 1e-8, Jacobi-PCG to {args.nl_lstol:g}"),
                    "precision": "fp64", "variants": variants, **extra},
         "roofline": roof, "cpu_baseline": cpu,
     }
@@ -795,6 +844,8 @@ def main():
     ap.add_argument("--rve-check", type=int, default=8)
     ap.add_argument("--c3-total", type=int, default=20,
                     help="load steps of the ramp (cfg3; at least warmup + steps)")
+    ap.add_argument("--compare-steps", type=int, default=2,
+                    help="cfg3: timed load steps of every form other than plain FP64 (the first of its timed steps)")
     ap.add_argument("--variants", default="fp64,mixed,mixed_surrogate,fp64_inexact,mixed_surrogate_inexact",
                     help="cfg3 forms to run (comma separated)")
     ap.add_argument("--check-nx", type=int, default=10,

@@ -215,8 +215,9 @@ int hippcg_last_outer(hippcg_solver *s);
  * the FP32 inner launches separately. */
 int hippcg_set_profiling(hippcg_solver *s, int on);
 /* Iteration layout (no counterpart in the reference; same arithmetic):
- * fused 1 forms p = z + beta p inside the SpMV gather, fused 0 updates p in
- * place in its own pass and gathers p once per nonzero. */
+ * fused 1 forms p = z + beta p inside the SpMV gather (lane, node-block and
+ * row-group forms of the plain PCG), fused 0 (default) updates p in place in
+ * its own pass and gathers p once per nonzero. */
 int hippcg_set_fused(hippcg_solver *s, int fused);
 int hippcg_profile(hippcg_solver *s, int *n_spmv64, double *ms_spmv64, int *n_spmv32, double *ms_spmv32);
 /* Allocation-free plain PCG.  No counterpart in the reference: it is for

@@ -675,11 +675,14 @@ __global__ __launch_bounds__(64) void k_internal(const int *__restrict__ elems, 
 }
 
 // ------------------------------------------------- element-by-element operator
-// Per-GP tangent descriptor (FP32, SoA, 8 fields): the TangentStiffness D of
-// misesmat.C:496-544 written as D = c I_dev + K 1x1 - m m^T with
-// c = 2G + factor2 and m = sqrt(-factor1) s_trial (factor1 <= 0), so that
+// Per-GP tangent descriptor (FP32, six per point, point-major: desc[6 gp + k]):
+// the TangentStiffness D of misesmat.C:496-544 written as
+// D = c I_dev + K 1x1 - m m^T with c = 2G + factor2 and
+// m = sqrt(-factor1) s_trial (factor1 <= 0), so that
 // D eps = c dev(eps) + K tr(eps) 1 - m (m . eps); elastic points have c = 2G,
-// m = 0.  Fields: 0 c, 1 K, 2-7 m.
+// m = 0.  Fields: c, m0, m1, m3, m4, m5.  The bulk modulus K comes from the
+// material table and m2 = -(m0 + m1), since s_trial is deviatoric; c = -1
+// marks a point whose general D (the surrogate's) sits in the snapshot.
 __global__ void k_tangent_desc(int ngp_total, int ngp, const int *__restrict__ emat, const double *__restrict__ mat,
                                const double *__restrict__ state, int mode, float *__restrict__ desc,
                                const int8_t *__restrict__ sflag, const float *__restrict__ dsur,
@@ -690,7 +693,7 @@ __global__ void k_tangent_desc(int ngp_total, int ngp, const int *__restrict__ e
     if (mode == 1 && sflag && sflag[gp]) {
         // general D: the surrogate's tangent of this assembly, kept for the
         // operator (the live one changes with every state update)
-        desc[gp] = -1.0f;
+        desc[6 * (size_t)gp] = -1.0f;
 #pragma unroll
         for (int k = 0; k < 21; k++) dsnap[(size_t)k * ngp_total + gp] = dsur[(size_t)k * ngp_total + gp];
         return;
@@ -698,7 +701,6 @@ __global__ void k_tangent_desc(int ngp_total, int ngp, const int *__restrict__ e
     const double *mp = mat + (size_t)HIPFEM_MATP * emat[gp / ngp];
     const double E = mp[1], nu = mp[2];
     const double G = E / (2.0 * (1. + nu));
-    const double K = E / (3.0 * (1. - 2. * nu));
     double c = 2.0 * G, m[6] = {0, 0, 0, 0, 0, 0};
     if (mode == 1 && (int)mp[0] == HIPFEM_MAT_MISES) {
         const double kn = state[(size_t)6 * ngp_total + gp], kt = state[(size_t)13 * ngp_total + gp];
@@ -718,10 +720,13 @@ __global__ void k_tangent_desc(int ngp_total, int ngp, const int *__restrict__ e
             for (int i = 0; i < 6; i++) m[i] = sc * s[i];
         }
     }
-    desc[gp] = (float)c;
-    desc[(size_t)ngp_total + gp] = (float)K;
-#pragma unroll
-    for (int i = 0; i < 6; i++) desc[(size_t)(2 + i) * ngp_total + gp] = (float)m[i];
+    float *d = desc + 6 * (size_t)gp;
+    d[0] = (float)c;
+    d[1] = (float)m[0];
+    d[2] = (float)m[1];
+    d[3] = (float)m[3];
+    d[4] = (float)m[4];
+    d[5] = (float)m[5];
 }
 
 __global__ void k_fill_d(int n, double *__restrict__ a, const PcgState *__restrict__ st)
@@ -763,8 +768,8 @@ constexpr int MF_EPB = 32;
 __device__ __forceinline__ float lane_xor(float v, int m)
 {
     const int i = __float_as_int(v);
-    if (m == 1) return __int_as_float(__builtin_amdgcn_mov_dpp(i, 0xB1, 0xF, 0xF, false));
-    if (m == 2) return __int_as_float(__builtin_amdgcn_mov_dpp(i, 0x4E, 0xF, 0xF, false));
+    if (m == 1) return __int_as_float(__builtin_amdgcn_mov_dpp(i, 0xB1, 0xF, 0xF, true));
+    if (m == 2) return __int_as_float(__builtin_amdgcn_mov_dpp(i, 0x4E, 0xF, 0xF, true));
     return __int_as_float(__builtin_amdgcn_ds_swizzle(i, 0x1F | (4 << 10)));
 }
 
@@ -841,7 +846,7 @@ __global__ __launch_bounds__(256) void k_mf_hex(const int *__restrict__ elems, i
         }
         const float det = J[0] * (J[4] * J[8] - J[5] * J[7]) - J[1] * (J[3] * J[8] - J[5] * J[6]) +
                           J[2] * (J[3] * J[7] - J[4] * J[6]);
-        const float id = 1.f / det;
+        const float id = __builtin_amdgcn_rcpf(det);   // v_rcp_f32 (1 ulp)
         float I[9];   // J^-1
         I[0] = (J[4] * J[8] - J[5] * J[7]) * id;
         I[1] = (J[2] * J[7] - J[1] * J[8]) * id;
@@ -861,17 +866,23 @@ __global__ __launch_bounds__(256) void k_mf_hex(const int *__restrict__ elems, i
                 H[3 * i + j] = gu[3 * i] * I[j] + gu[3 * i + 1] * I[3 + j] + gu[3 * i + 2] * I[6 + j];
         const float eps[6] = {H[0], H[4], H[8], H[5] + H[7], H[2] + H[6], H[1] + H[3]};
         const size_t gp = (size_t)e * 8 + g;
-        float c, Kb, m[6];
+        float c, m[6];
+        const double *mp = mat + (size_t)HIPFEM_MATP * emat[e];
+        const float E = (float)mp[1], nu = (float)mp[2];
+        const float Kb = E * __builtin_amdgcn_rcpf(3.f * (1.f - 2.f * nu));
         if (desc) {
-            c = desc[gp];
-            Kb = desc[(size_t)ngp_total + gp];
-#pragma unroll
-            for (int i = 0; i < 6; i++) m[i] = desc[(size_t)(2 + i) * ngp_total + gp];
+            const float2 d01 = *reinterpret_cast<const float2 *>(desc + 6 * gp);
+            const float2 d23 = *reinterpret_cast<const float2 *>(desc + 6 * gp + 2);
+            const float2 d45 = *reinterpret_cast<const float2 *>(desc + 6 * gp + 4);
+            c = d01.x;
+            m[0] = d01.y;
+            m[1] = d23.x;
+            m[2] = -(d01.y + d23.x);
+            m[3] = d23.y;
+            m[4] = d45.x;
+            m[5] = d45.y;
         } else {
-            const double *mp = mat + (size_t)HIPFEM_MATP * emat[e];
-            const float E = (float)mp[1], nu = (float)mp[2];
-            c = E / (1.f + nu);
-            Kb = E / (3.f * (1.f - 2.f * nu));
+            c = E * __builtin_amdgcn_rcpf(1.f + nu);
 #pragma unroll
             for (int i = 0; i < 6; i++) m[i] = 0.f;
         }
@@ -1549,7 +1560,7 @@ extern "C" int hfem_model_assemble_stiffness(hfem_model *md, hipcsr_mtx *K, int 
     if (md->etype == HIPFEM_LSPACE && md->ngp_total > 0) {
         md->tdesc_used = md->has_mises && mode == 1;
         if (md->tdesc_used) {
-            if (!md->d_tdesc) HIPFEM_CHECK(hipMalloc(&md->d_tdesc, sizeof(float) * 8 * (size_t)md->ngp_total));
+            if (!md->d_tdesc) HIPFEM_CHECK(hipMalloc(&md->d_tdesc, sizeof(float) * 6 * (size_t)md->ngp_total));
             k_tangent_desc<<<(md->ngp_total + 255) / 256, 256, 0, st>>>(md->ngp_total, md->ngp, md->d_emat,
                                                                           md->d_mat, md->d_state, mode, md->d_tdesc,
                                                                           md->sur ? md->d_sflag : nullptr,

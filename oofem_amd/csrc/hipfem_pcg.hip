@@ -9,12 +9,18 @@
 //       q = A p;  alpha = rho / p.q;  x += alpha p;  r -= alpha q
 //       stop if |r|/|b| <= tol
 //
-// One iteration is four launches on the context stream:
-//   k_pcg_spmv     p_new = z + beta p_old formed on the fly in the gather,
-//                  q = A p_new, per-block p.q partials
+// One iteration is five launches on the context stream:
+//   k_pcg_pupdate  p = z + beta p in place
+//   k_pcg_spmv     q = A p, per-block p.q partials
 //   k_reduce_alpha alpha = rho / sum(p.q)                 (one block)
 //   k_pcg_update   x += alpha p; r -= alpha q; z = D^-1 r; partials r.z, r.r
 //   k_reduce_beta  convergence test, beta, rho            (one block)
+// On request (hippcg_set_fused) p = z + beta p_old is formed inside the
+// SpMV gather instead and written by the row owner, saving the first launch;
+// on gfx950 that form is slower (the gather reads two vectors).  Summing the
+// partials in the last block of the producing kernel (an agent-scope
+// release fence per block and a counter) was measured slower still: each
+// block's fence writes its XCD's L2 back to memory.
 // Dot products are block partials summed in a fixed order, so a solve is
 // bitwise reproducible.  Kernels return at once when the device-side "done"
 // flag is set, which lets the host enqueue iterations in chunks and read
@@ -64,6 +70,61 @@ __device__ __forceinline__ double block_sum(double v, double *sh)
         for (int i = 0; i < NT / 64; i++) s += sh[i];
     __syncthreads();
     return s;
+}
+
+// The PCG scalar updates of the one-block reduction kernels.
+__device__ __forceinline__ void step_alpha(PcgState *st, double pq) { st->alpha = st->rho / pq; }
+
+// alpha of one FP32 iteration; a non-positive curvature p.q (the FP32
+// operator has lost the definiteness of A on this direction) pauses the
+// iteration for an update that restarts the direction
+__device__ __forceinline__ void step_alpha_rel(PcgState *st, double pq)
+{
+    if (pq > 0.0) {
+        st->alpha = st->rho / pq;
+    } else {
+        st->alpha = 0.0;
+        st->pause = 1;
+        st->restart = 1;
+    }
+}
+
+// convergence test, beta and rho of a plain iteration (cg.h:50-70)
+__device__ __forceinline__ void step_beta(PcgState *st, double rz, double rr, int iter, double *hist, int hist_cap)
+{
+    const double res = sqrt(rr) / st->normb;
+    st->iter = iter;
+    st->res = res;
+    if (iter < hist_cap) hist[iter] = res;
+    if (res <= st->tol) {
+        st->done = 1;
+    } else if (!(res == res)) {
+        st->done = 1;
+        st->bad = 1;
+    } else {
+        st->beta = rz / st->rho;
+        st->rho = rz;
+    }
+}
+
+// convergence / reliable-update test of one FP32 iteration
+__device__ __forceinline__ void step_beta_rel(PcgState *st, double rz, double rr, int iter)
+{
+    const double res = sqrt(rr) / st->normb;
+    st->iter = iter;
+    st->res = res;
+    if (!(res == res)) {
+        st->done = 1;
+        st->bad = 1;
+    } else if (res <= st->tol || res < st->delta * st->rmax) {
+        st->pause = 1;   // the host replaces r by the true residual
+    } else if (res > 100.0 * st->rmax) {
+        st->pause = 1;   // the recursion has drifted: update and restart the direction
+        st->restart = 1;
+    } else {
+        st->beta = rz / st->rho;
+        st->rho = rz;
+    }
 }
 
 // fused form: p_new = z + beta p_old is formed inside the gather (two
@@ -152,15 +213,26 @@ __global__ __launch_bounds__(256) void k_pcg_spmv_p_stream(int n, const int32_t 
     if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
-// node-block form of k_pcg_spmv_p (bcol per 3x3 block, one gather per block)
-template <typename VT, typename MT>
+// node-block form of k_pcg_spmv_p (bcol per 3x3 block, one gather per block).
+// FP: p_new = z + beta p_old is formed inside the gather (p = p_old, first:
+// p_new = z) and written by the row owner to pnew, as k_pcg_pupdate would.
+template <typename VT, typename MT, bool FP>
 __global__ __launch_bounds__(256) void k_pcg_spmv_p_nb(int nbr, const int32_t *__restrict__ bptr,
                                                        const int32_t *__restrict__ bcol, const MT *__restrict__ val,
-                                                       const VT *__restrict__ p, VT *__restrict__ q,
-                                                       const PcgState *__restrict__ st, double *__restrict__ partials)
+                                                       const VT *__restrict__ p, const VT *__restrict__ z,
+                                                       VT *__restrict__ pnew, VT *__restrict__ q,
+                                                       const PcgState *__restrict__ st, double *__restrict__ partials,
+                                                       int first)
 {
     __shared__ double sh[4];
     if (halted(st)) return;
+    const VT beta = (FP && !first) ? (VT)st->beta : VT(0);
+    auto pv = [&](int i) -> VT {
+        if constexpr (FP)
+            return first ? z[i] : fma(beta, p[i], z[i]);
+        else
+            return p[i];
+    };
     const int lane = threadIdx.x % 32;
     double acc = 0.0;
     for (int I = blockIdx.x * 8 + threadIdx.x / 32; I < nbr; I += gridDim.x * 8) {
@@ -168,46 +240,59 @@ __global__ __launch_bounds__(256) void k_pcg_spmv_p_nb(int nbr, const int32_t *_
         VT a[3];
         nb_row<VT>(bo, nbk, 9 * bo, lane, bcol, val,
                    [&](int bc, VT &x0, VT &x1, VT &x2) {
-                       const VT *pp = p + 3 * bc;
-                       x0 = pp[0];
-                       x1 = pp[1];
-                       x2 = pp[2];
+                       x0 = pv(3 * bc);
+                       x1 = pv(3 * bc + 1);
+                       x2 = pv(3 * bc + 2);
                    },
                    a);
         if (lane == 0) {
-            q[3 * I] = a[0];
-            q[3 * I + 1] = a[1];
-            q[3 * I + 2] = a[2];
-            acc += (double)p[3 * I] * (double)a[0] + (double)p[3 * I + 1] * (double)a[1] +
-                   (double)p[3 * I + 2] * (double)a[2];
+#pragma unroll
+            for (int r = 0; r < 3; r++) {
+                const VT pi = pv(3 * I + r);
+                if constexpr (FP) pnew[3 * I + r] = pi;
+                q[3 * I + r] = a[r];
+                acc += (double)pi * (double)a[r];
+            }
         }
     }
     const double t = block_sum<256>(acc, sh);
     if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
-// row-group form of k_pcg_spmv_p (partially constrained node patterns)
-template <typename VT, typename MT>
+// row-group form of k_pcg_spmv_p (partially constrained node patterns);
+// FP as in k_pcg_spmv_p_nb
+template <typename VT, typename MT, bool FP>
 __global__ __launch_bounds__(256) void k_pcg_spmv_p_grp(int ngrp, const int32_t *__restrict__ grow,
                                                         const int32_t *__restrict__ gch,
                                                         const uint32_t *__restrict__ ccol,
                                                         const int32_t *__restrict__ coff,
                                                         const int32_t *__restrict__ rowptr, const MT *__restrict__ val,
-                                                        const VT *__restrict__ p, VT *__restrict__ q,
-                                                        const PcgState *__restrict__ st, double *__restrict__ partials)
+                                                        const VT *__restrict__ p, const VT *__restrict__ z,
+                                                        VT *__restrict__ pnew, VT *__restrict__ q,
+                                                        const PcgState *__restrict__ st, double *__restrict__ partials,
+                                                        int first)
 {
     __shared__ double sh[4];
     if (halted(st)) return;
+    const VT beta = (FP && !first) ? (VT)st->beta : VT(0);
+    auto pv = [&](int i) -> VT {
+        if constexpr (FP)
+            return first ? z[i] : fma(beta, p[i], z[i]);
+        else
+            return p[i];
+    };
     const int lane = threadIdx.x % GRP_LANES;
     double acc = 0.0;
     for (int g = blockIdx.x * GRP_PER_BLOCK + threadIdx.x / GRP_LANES; g < ngrp; g += gridDim.x * GRP_PER_BLOCK) {
         const int r0 = grow[g], nr = grow[g + 1] - r0, c0 = gch[g], nch = gch[g + 1] - c0;
         VT a[3];
-        grp_row<VT>(r0, nr, c0, nch, lane, rowptr, ccol, coff, val, [&](int c) { return p[c]; }, a);
+        grp_row<VT>(r0, nr, c0, nch, lane, rowptr, ccol, coff, val, pv, a);
         if (lane == 0)
             for (int r = 0; r < nr; r++) {
+                const VT pi = pv(r0 + r);
+                if constexpr (FP) pnew[r0 + r] = pi;
                 q[r0 + r] = a[r];
-                acc += (double)p[r0 + r] * (double)a[r];
+                acc += (double)pi * (double)a[r];
             }
     }
     const double t = block_sum<256>(acc, sh);
@@ -307,6 +392,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_reduce_init(PcgState *st, const
         st->iter = 0;
         st->bad = 0;
         st->pause = 0;
+        st->restart = 0;
         st->done = (st->res <= tol) ? 1 : 0;
         if (hist_cap > 0) hist[0] = st->res;
     }
@@ -317,7 +403,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_reduce_alpha(PcgState *st, cons
     __shared__ double sh[RED_THREADS / 64];
     if (halted(st)) return;
     const double pq = reduce_partials(partials, nb, sh);
-    if (threadIdx.x == 0) st->alpha = st->rho / pq;
+    if (threadIdx.x == 0) step_alpha(st, pq);
 }
 
 __global__ __launch_bounds__(RED_THREADS) void k_reduce_beta(PcgState *st, const double *__restrict__ partials, int nb,
@@ -327,21 +413,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_reduce_beta(PcgState *st, const
     if (halted(st)) return;
     const double rz = reduce_partials(partials, nb, sh);
     const double rr = reduce_partials(partials + nb, nb, sh);
-    if (threadIdx.x == 0) {
-        const double res = sqrt(rr) / st->normb;
-        st->iter = iter;
-        st->res = res;
-        if (iter < hist_cap) hist[iter] = res;
-        if (res <= st->tol) {
-            st->done = 1;
-        } else if (!(res == res)) {
-            st->done = 1;
-            st->bad = 1;
-        } else {
-            st->beta = rz / st->rho;
-            st->rho = rz;
-        }
-    }
+    if (threadIdx.x == 0) step_beta(st, rz, rr, iter, hist, hist_cap);
 }
 
 __global__ void k_dinv(int n, const int32_t *__restrict__ diag, const double *__restrict__ val, int use_diag,
@@ -524,27 +596,16 @@ __global__ __launch_bounds__(RED_THREADS) void k_reduce_refresh(PcgState *st, co
     }
 }
 
-// alpha of one FP32 iteration; a non-positive curvature p.q (the FP32
-// operator has lost the definiteness of A on this direction) pauses the
-// iteration for an update that restarts the direction
+// one-block reduction forms of step_alpha_rel / step_beta_rel
 __global__ __launch_bounds__(RED_THREADS) void k_reduce_alpha_rel(PcgState *st, const double *__restrict__ partials,
                                                                   int nb)
 {
     __shared__ double sh[RED_THREADS / 64];
     if (halted(st)) return;
     const double pq = reduce_partials(partials, nb, sh);
-    if (threadIdx.x == 0) {
-        if (pq > 0.0) {
-            st->alpha = st->rho / pq;
-        } else {
-            st->alpha = 0.0;
-            st->pause = 1;
-            st->restart = 1;
-        }
-    }
+    if (threadIdx.x == 0) step_alpha_rel(st, pq);
 }
 
-// convergence / reliable-update test of one FP32 iteration
 __global__ __launch_bounds__(RED_THREADS) void k_reduce_beta_rel(PcgState *st, const double *__restrict__ partials,
                                                                  int nb, int iter)
 {
@@ -552,23 +613,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_reduce_beta_rel(PcgState *st, c
     if (halted(st)) return;
     const double rz = reduce_partials(partials, nb, sh);
     const double rr = reduce_partials(partials + nb, nb, sh);
-    if (threadIdx.x == 0) {
-        const double res = sqrt(rr) / st->normb;
-        st->iter = iter;
-        st->res = res;
-        if (!(res == res)) {
-            st->done = 1;
-            st->bad = 1;
-        } else if (res <= st->tol || res < st->delta * st->rmax) {
-            st->pause = 1;   // the host replaces r by the true residual
-        } else if (res > 100.0 * st->rmax) {
-            st->pause = 1;   // the recursion has drifted: update and restart the direction
-            st->restart = 1;
-        } else {
-            st->beta = rz / st->rho;
-            st->rho = rz;
-        }
-    }
+    if (threadIdx.x == 0) step_beta_rel(st, rz, rr, iter);
 }
 
 static int vec_grid(int n) { return std::max(1, std::min((n + 255) / 256, MAX_GRID)); }
@@ -765,7 +810,9 @@ static int pcg_run(hippcg_solver *s, const PcgRunBufs &bufs, hipcsr_mtx *A, cons
                    : grouped   ? std::max(1, std::min((A->ngrp + GRP_PER_BLOCK - 1) / GRP_PER_BLOCK, MAX_GRID))
                    : sh.stream ? std::max(1, std::min((n + sh.rpb - 1) / sh.rpb, MAX_GRID))
                                : spmv_grid(n, sh.lanes);
-    const bool fused = s->fused && !sh.stream && !nodeblock && !grouped;  // these forms gather a ready p
+    // p = z + beta p formed in the product's gather (the lane, node-block
+    // and row-group forms), or in a pass of its own
+    const bool fused = s->fused && !sh.stream;
     const size_t need = 3 * (size_t)std::max(gv, gs);
     double *part = bufs.part;
     int rc = 0;
@@ -814,16 +861,31 @@ static int pcg_run(hippcg_solver *s, const PcgRunBufs &bufs, hipcsr_mtx *A, cons
             const int first = (it == 1);
             if (!fused) k_pcg_pupdate<VT><<<gv, 256, 0, st>>>(n, z, p0, pst, first);
             if (prof) HIPFEM_CHECK(hipEventRecord(s->ev[2 * (it - 1)], st));
-            if (nodeblock)
-                k_pcg_spmv_p_nb<VT, MT><<<gs, 256, 0, st>>>(A->nbr, A->d_bptr, A->d_bcol, val, p0, q, pst, part);
-            else if (grouped)
-                k_pcg_spmv_p_grp<VT, MT><<<gs, 256, 0, st>>>(A->ngrp, A->d_grow, A->d_gch, A->d_ccol, A->d_coff,
-                                                             A->d_rowptr, val, p0, q, pst, part);
-            else if (fused)
+            // plain PCG never pauses, so the two direction buffers can swap
+            // on the host: after "done" every launch returns at once
+            VT *pin = fused ? pold : p0;
+            if (nodeblock) {
+                if (fused)
+                    k_pcg_spmv_p_nb<VT, MT, true><<<gs, 256, 0, st>>>(A->nbr, A->d_bptr, A->d_bcol, val, pin, z, pnew,
+                                                                      q, pst, part, first);
+                else
+                    k_pcg_spmv_p_nb<VT, MT, false><<<gs, 256, 0, st>>>(A->nbr, A->d_bptr, A->d_bcol, val, pin, z,
+                                                                       nullptr, q, pst, part, first);
+            } else if (grouped) {
+                if (fused)
+                    k_pcg_spmv_p_grp<VT, MT, true><<<gs, 256, 0, st>>>(A->ngrp, A->d_grow, A->d_gch, A->d_ccol,
+                                                                       A->d_coff, A->d_rowptr, val, pin, z, pnew, q,
+                                                                       pst, part, first);
+                else
+                    k_pcg_spmv_p_grp<VT, MT, false><<<gs, 256, 0, st>>>(A->ngrp, A->d_grow, A->d_gch, A->d_ccol,
+                                                                        A->d_coff, A->d_rowptr, val, pin, z, nullptr,
+                                                                        q, pst, part, first);
+            } else if (fused) {
                 launch_pcg_spmv_fused<VT, MT>(sh, gs, st, n, A->d_rowptr, A->d_colind, val, z, pold, pnew, q, pst,
                                               part, first);
-            else
+            } else {
                 launch_pcg_spmv_p<VT, MT>(sh, gs, st, n, A->d_rowptr, A->d_colind, val, p0, q, pst, part);
+            }
             if (prof) HIPFEM_CHECK(hipEventRecord(s->ev[2 * (it - 1) + 1], st));
             k_reduce_alpha<<<1, RED_THREADS, 0, st>>>(pst, part, gs);
             k_pcg_update<XT, VT><<<gv, 256, 0, st>>>(n, x, r, z, fused ? pnew : p0, q, dinv, pst, part);
@@ -931,12 +993,13 @@ static int pcg_reliable(hippcg_solver *s, hipcsr_mtx *A, const double *db, doubl
                     k_dot_pq<<<gv, 256, 0, st>>>(n, s->fp0, s->q, s->fq, s->st, part);
                 }
             } else if (nodeblock) {
-                k_pcg_spmv_p_nb<float, float><<<gs, 256, 0, st>>>(A->nbr, A->d_bptr, A->d_bcol, A->d_val32, s->fp0,
-                                                                  s->fq, s->st, part);
+                k_pcg_spmv_p_nb<float, float, false><<<gs, 256, 0, st>>>(A->nbr, A->d_bptr, A->d_bcol, A->d_val32,
+                                                                         s->fp0, nullptr, nullptr, s->fq, s->st, part,
+                                                                         0);
             } else if (grouped) {
-                k_pcg_spmv_p_grp<float, float><<<gs, 256, 0, st>>>(A->ngrp, A->d_grow, A->d_gch, A->d_ccol,
-                                                                   A->d_coff, A->d_rowptr, A->d_val32, s->fp0,
-                                                                   s->fq, s->st, part);
+                k_pcg_spmv_p_grp<float, float, false><<<gs, 256, 0, st>>>(A->ngrp, A->d_grow, A->d_gch, A->d_ccol,
+                                                                          A->d_coff, A->d_rowptr, A->d_val32, s->fp0,
+                                                                          nullptr, nullptr, s->fq, s->st, part, 0);
             } else {
                 launch_pcg_spmv_p<float, float>(sh, gs, st, n, A->d_rowptr, A->d_colind, A->d_val32, s->fp0, s->fq,
                                                 s->st, part);

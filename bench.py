@@ -370,6 +370,7 @@ def cpu_baseline_newton(args, deck, rn):
     t = time.perf_counter()
     rp, ci = oracle.csr_structure(deck)
     t_struct = time.perf_counter() - t
+    log(f"[bench] cpu baseline: pattern {t_struct:.1f} s ({nthr} threads)")
     st = oracle.state_new(deck)
     up = deck.prescribed_values(3.0 / 8.0)   # 1.9x the yield strain: a plastic state
     zmax = deck.coords[:, 2].max()
@@ -382,10 +383,12 @@ def cpu_baseline_newton(args, deck, rn):
     t = time.perf_counter()
     val = oracle.assemble(deck, rp, ci, st, tangent=True, omp=True)
     t_asm = time.perf_counter() - t
+    log(f"[bench] cpu baseline: f_int {t_fint:.1f} s, tangent assembly {t_asm:.1f} s")
     nit = args.cpu_cg_iters
     t = time.perf_counter()
     oracle.pcg(rp, ci, val, np.ones(deck.neq), tol=1e-300, maxit=nit)
     t_cg = (time.perf_counter() - t) / nit
+    log(f"[bench] cpu baseline: CG {t_cg * 1e3:.0f} ms per iteration")
     del val, rp, ci
     plastic = float((st.reshape(-1, 38)[:, 13] > st.reshape(-1, 38)[:, 6]).mean())
     del st
@@ -407,6 +410,7 @@ def cpu_baseline_newton(args, deck, rn):
     oracle.assemble(s, srp, sci, sst, tangent=True)
     t_asm1 = (time.perf_counter() - t) * deck.nelem / s.nelem
     t_step1 = (newton + 1) * (t_fint1 + t_asm1) + pcg * t_cg
+    log(f"[bench] cpu baseline: one thread f_int {t_fint1:.1f} s, assembly {t_asm1:.1f} s per full mesh")
     # configuration 0: LinearStatic with the reference's direct solver
     c0 = meshgen.config0()
     t = time.perf_counter()
@@ -460,6 +464,7 @@ def oracle_check_newton(args, variants):
             raise RuntimeError(f"oracle step {k} failed: {r}")
         oracle.state_commit(d, st)
         prev = up
+        log(f"[bench] oracle check: step {k} ({time.perf_counter() - t:.1f} s)")
     t_oracle = time.perf_counter() - t
     so = st.reshape(-1, 38)
     sy = 250.0 + 2e3 * so[:, 6]

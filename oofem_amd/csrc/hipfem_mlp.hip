@@ -298,20 +298,45 @@ __device__ __forceinline__ float bf2f(uint16_t b)
 // 8 (r/4) + 4 (l/32) + r%4, column l%32
 __device__ __forceinline__ int tile_row(int l, int r) { return 8 * (r >> 2) + 4 * (l >> 5) + (r & 3); }
 
-__global__ __launch_bounds__(256) void k_mlp_bf16(int n, const float *__restrict__ X, float *__restrict__ Y,
-                                                  const uint16_t *__restrict__ wpk, const float *__restrict__ params)
+// the activation tile of a wave is private to it: LDS operations of one
+// wave complete in order, so a wave-level barrier (no block barrier) orders
+// its stores before its own loads of the next layer
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// z silu(z) = z / (1 + e^-z) with the hardware exp2 and reciprocal (the
+// result is rounded to BF16, 8 bits, right after), then to BF16 by
+// v_cvt_pk_bf16_f32 (round to nearest even, as f2bf)
+__device__ __forceinline__ uint16_t silu_bf16(float z)
+{
+    const float h = z * __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(-1.4426950408889634f * z));
+    const __bf16 b = (__bf16)h;
+    uint16_t u;
+    memcpy(&u, &b, 2);
+    return u;
+}
+
+constexpr int BF_WAVES = 8;   // waves per workgroup sharing the LDS weights
+
+__global__ __launch_bounds__(64 * BF_WAVES) void k_mlp_bf16(int n, const float *__restrict__ X,
+                                                             float *__restrict__ Y, const uint16_t *__restrict__ wpk,
+                                                             const float *__restrict__ params)
 {
     extern __shared__ uint16_t lds[];
     uint16_t *sW = lds;                                   // BF_TOTAL
-    uint16_t *sA = lds + BF_TOTAL;                        // 4 waves x 32 x ACT_LD
-    for (int i = threadIdx.x; i < BF_TOTAL / 8; i += 256)
+    uint16_t *sA = lds + BF_TOTAL;                        // BF_WAVES waves x 32 x ACT_LD
+    for (int i = threadIdx.x; i < BF_TOTAL / 8; i += 64 * BF_WAVES)
         reinterpret_cast<uint4 *>(sW)[i] = reinterpret_cast<const uint4 *>(wpk)[i];
     const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
     uint16_t *act = sA + wv * 32 * ACT_LD;
     const uint16_t *W1 = sW, *W2 = sW + BF_L1, *W3 = W2 + BF_L2, *W4 = W3 + BF_L3;
     const int ntile = (n + 31) / 32;
     __syncthreads();
-    for (int base = blockIdx.x * 4; base < ntile; base += gridDim.x * 4) {
+    for (int base = blockIdx.x * BF_WAVES; base < ntile; base += gridDim.x * BF_WAVES) {
         const int tile = base + wv;
         const int row0 = tile * 32;
         // layer 1: A operand straight from X (13 inputs, zero padded to 16)
@@ -341,12 +366,9 @@ __global__ __launch_bounds__(256) void k_mlp_bf16(int n, const float *__restrict
                 const int j = nt * 32 + (l & 31);
                 const float bj = bias[j];
 #pragma unroll
-                for (int r = 0; r < 16; r++) {
-                    const float z = acc[nt][r] + bj;
-                    act[tile_row(l, r) * ACT_LD + j] = f2bf(z * sigmoidf_(z));
-                }
+                for (int r = 0; r < 16; r++) act[tile_row(l, r) * ACT_LD + j] = silu_bf16(acc[nt][r] + bj);
             }
-            __syncthreads();
+            wave_lds_sync();
             const uint16_t *Wn = layer == 0 ? W2 : layer == 1 ? W3 : W4;
 #pragma unroll
             for (int nt = 0; nt < 4; nt++) acc[nt] = (f32x16){0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
@@ -370,7 +392,7 @@ __global__ __launch_bounds__(256) void k_mlp_bf16(int n, const float *__restrict
                     acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, b, acc[0], 0, 0, 0);
                 }
             }
-            __syncthreads();
+            wave_lds_sync();
         }
         // output layer: 27 of 32 columns
         const int j = l & 31;
@@ -743,15 +765,16 @@ int mlp_infer_bf16_dev(hfem_mlp *m, int n, const float *dX, float *dY)
         m->packed_version = m->param_version;
     }
     if (n <= 0) return 0;
-    const size_t lds = sizeof(uint16_t) * (BF_TOTAL + 4 * 32 * ACT_LD);
+    const size_t lds = sizeof(uint16_t) * (BF_TOTAL + BF_WAVES * 32 * ACT_LD);
     int dev = 0, maxlds = 0;
     HIPFEM_CHECK(hipGetDevice(&dev));
     HIPFEM_CHECK(hipDeviceGetAttribute(&maxlds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     HIPFEM_REQUIRE((size_t)maxlds >= lds, HIPFEM_EINVAL,
                    "surrogate inference needs %zu bytes of LDS per workgroup, the device offers %d", lds, maxlds);
     const int ntile = (n + 31) / 32;
-    const int grid = std::max(1, std::min((ntile + 3) / 4, 256 * 4));
-    k_mlp_bf16<<<grid, 256, lds, s>>>(n, dX, dY, m->d_wpk, m->d_p);
+    // one workgroup per CU (the LDS holds one), each looping over its tiles
+    const int grid = std::max(1, std::min((ntile + BF_WAVES - 1) / BF_WAVES, 256));
+    k_mlp_bf16<<<grid, 64 * BF_WAVES, lds, s>>>(n, dX, dY, m->d_wpk, m->d_p);
     HIPFEM_CHECK(hipGetLastError());
     return 0;
 }

@@ -238,3 +238,80 @@ def test_element_operator_keeps_the_assembled_surrogate_tangent(ctx, trained):
         y32 = K.times_matrix_free(x)
         assert np.linalg.norm(y32 - y64) <= 2e-5 * np.linalg.norm(y64)
     ev.set_surrogate(None)
+
+
+def test_surrogate_against_plain_return_map_on_the_plasticity_block(ctx, trained):
+    """What the trained network gains over the plain return mapping on the
+    64x64x256 LSpace plasticity block (1,048,576 elements, 8,388,608 Gauss
+    points).  Committed state: the uniaxial field of load step 5 of the
+    20-step ramp of the block's end displacement (1.25x the yield strain);
+    update: the field of step 6, so every plastic point is queued.  Measured, on device vectors: the
+    material update (internal forces with state update) with the plain return
+    mapping, and with the surrogate at the stated tolerances (3e-2, and 1e-4,
+    the accuracy target): queued and accepted points and the update time.
+    Asserted: the accepted points meet the bound the residual check implies
+    (4 tol sigma_y), every other point is bitwise the plain update, and at
+    3e-2 the network serves a non-zero share of the queue.  The surrogate
+    does not make the update cheaper: the closed-form radial return costs
+    about 100 flops per point against the network's 75.8 kflop."""
+    import time
+
+    from oofem_amd import meshgen
+    from oofem_amd.hipfem import BatchedEvaluator, DeviceVector, HipCSRSparseMtrx
+    d = meshgen.config3()
+    ev = BatchedEvaluator(ctx, d)
+    K = HipCSRSparseMtrx(ctx)
+    ev.build_structure(K)
+    zmax = d.coords[:, 2].max()
+    fe = d.eqn[:, 2] > 0
+
+    def field(frac):
+        up = d.prescribed_values(frac)
+        u = np.zeros(d.neq)
+        u[d.eqn[fe, 2] - 1] = d.coords[fe, 2] / zmax * frac * d.meta["u_end"]
+        return u, up
+
+    u0, up0 = field(5.0 / 20.0)
+    ev.internal_forces(u0, up0, update_state=True)
+    ev.commit()
+    u1, up1 = field(6.0 / 20.0)
+    du = DeviceVector(ctx, d.neq, u1)
+    dup = DeviceVector(ctx, d.npeq, up1)
+    df = DeviceVector(ctx, d.neq)
+    dfp = DeviceVector(ctx, d.npeq)
+
+    def update(net, tol):
+        ev.set_surrogate(net, band=0.05, tol=tol)
+        best = None
+        for _ in range(3):
+            ev.restart()
+            if net is not None:
+                ev.surrogate_stats(reset=True)
+            ctx.synchronize()
+            t = time.perf_counter()
+            ev.internal_forces_dev(du.addr, dup.addr, df.addr, dfp.addr, update_state=True)
+            ctx.synchronize()
+            dt = time.perf_counter() - t
+            best = dt if best is None else min(best, dt)
+        st = ev.surrogate_stats() if net is not None else None
+        s = ev.get_state()
+        return best, st, s[:, 20:26].copy(), s[:, 6].copy()
+
+    t_plain, _, s_plain, kappa_n = update(None, 0.0)
+    sy = SIG0 + H * kappa_n
+    report = {"plain_update_ms": round(t_plain * 1e3, 2)}
+    for tol in (3e-2, 1e-4):
+        dt, st, s_sur, _ = update(trained, tol)
+        err = np.abs(s_sur - s_plain).max(axis=1) / sy
+        assert st["accepted"] + st["rejected"] == st["queued"]
+        assert st["queued"] > 0.5 * len(s_plain)
+        assert err.max() <= 4 * tol, (tol, err.max())
+        assert (err == 0).sum() >= len(s_plain) - st["accepted"]
+        report[f"tol {tol:g}"] = dict(queued=st["queued"], accepted=st["accepted"],
+                                      accepted_share=round(st["accepted"] / st["queued"], 4),
+                                      update_ms=round(dt * 1e3, 2))
+    ev.set_surrogate(None)
+    print(f"surrogate on the plasticity block: {report}", flush=True)
+    assert report["tol 0.03"]["accepted"] > 0
+    K.close()
+    ev.close()

@@ -110,6 +110,30 @@ def trained(ctx):
     n.close()
 
 
+def test_surrogate_accuracy_limits(trained):
+    """Why the 1e-4 acceptance target is out of this network's reach, each
+    cause measured on plastic samples of its own training set (targets from
+    the plain return mapping; the stress outputs are in units of sigma_0):
+    the FP32 forward's stress error against the targets (what training
+    leaves), and the change of the stress outputs by BF16 inference (BF16
+    keeps 8 significant bits, so rounding the last hidden layer's 128
+    activations alone moves an output by about 2^-9 of the norm of its
+    terms).  Each median exceeds 1e-4 sigma_0 on its own."""
+    X, Y = trained.samples(0, 200000)
+    plastic = np.abs(Y[:, :6]).max(axis=1) > 1e-6
+    assert plastic.sum() > 1000
+    X, Y = X[plastic], Y[plastic]
+    y32 = trained.forward(X)
+    y16 = trained.infer_bf16(X)
+    e_train = np.abs(y32[:, :6] - Y[:, :6]).max(axis=1)
+    e_bf16 = np.abs(y16[:, :6] - y32[:, :6]).max(axis=1)
+    rep = {name: dict(median=float(np.median(e)), p90=float(np.percentile(e, 90)))
+           for name, e in (("fp32_vs_return_map", e_train), ("bf16_vs_fp32", e_bf16))}
+    print(f"surrogate stress-output errors over sigma_0 on {len(X)} plastic samples: {rep}", flush=True)
+    assert rep["fp32_vs_return_map"]["median"] > 1e-4
+    assert rep["bf16_vs_fp32"]["median"] > 1e-4
+
+
 def _plastic_block(ctx, amp):
     from oofem_amd import meshgen
     from oofem_amd.hipfem import BatchedEvaluator, HipCSRSparseMtrx

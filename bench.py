@@ -137,11 +137,15 @@ def run_kernels(args, deck):
         s.close()
     # whole PCG iterations (wall clock over --iters iterations): the plain
     # PCG with p updated in its own pass (fused 0) and formed inside the
-    # SpMV gather (fused 1), and the mixed form
-    for prec, fl in ((0, (0, 1)), (2, (0,))):
+    # SpMV gather (fused 1), and the mixed form with the slot-form element
+    # operator (form 2)
+    for prec, fl in ((0, (0, 1)), (2, (2,))):
         for flags in fl:
             s = HipPCGSolver(ctx, lstol=1e-30, lsiter=args.iters, lsprecond=1, precision=prec, inner_tol=1e-30)
-            s.set_fused(flags)
+            if prec == 0:
+                s.set_fused(flags)
+            else:
+                ev.set_element_operator(flags)   # 2: the slot form
             dx.fill(0.0)
             s.solve_dev(K, db.addr, dx.addr)
             ctx.synchronize()
@@ -150,8 +154,10 @@ def run_kernels(args, deck):
             t = time.perf_counter()
             _, nit, _ = s.solve_dev(K, db.addr, dx.addr)
             ctx.synchronize()
-            iter_ms[f"precision{prec}_flags{flags}"] = (time.perf_counter() - t) * 1e3 / max(nit, 1)
+            iter_ms[f"precision{prec}_{'fused' if prec == 0 else 'form'}{flags}"] = \
+                (time.perf_counter() - t) * 1e3 / max(nit, 1)
             s.close()
+    ev.set_element_operator(2)
     log(f"[bench] kernels: ms per PCG iteration {iter_ms}")
     print(json.dumps(dict(mode="kernels", workload=deck.name, plastic_tangent=plastic, neq=deck.neq, nnz=K.nnz(),
                           shape=K.spmv_shape(), copy_doubles=n_copy,

@@ -777,50 +777,38 @@ __device__ __forceinline__ float lane_group_first(float v)
 {
     return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x18));
 }
-__global__ __launch_bounds__(256) void k_mf_hex(const int *__restrict__ elems, int nel,
-                                                const float *__restrict__ erel, const int *__restrict__ conn,
-                                                const int *__restrict__ eqn, const int *__restrict__ emat,
-                                                const double *__restrict__ mat, const float *__restrict__ desc,
-                                                const float *__restrict__ dgen, int ngp_total,
-                                                const float *__restrict__ x, double *__restrict__ y,
-                                                float *__restrict__ ebuf, float *__restrict__ yslot,
-                                                const uint8_t *__restrict__ ecolor, int neq,
-                                                const PcgState *__restrict__ st)
+// The element arithmetic of k_mf_hex for the 8 lanes of one element
+// (lane g: Gauss point g and node nl(g)).  v[0..2]: the lane node's
+// coordinates relative to node 1, v[3..5]: its operand values; sC: the
+// element's 8x8 LDS coefficient slots; c / m / Kb: the lane point's
+// tangent (c < 0: the general D of dgen at gp).  Every lane of the group
+// takes part (inactive ones with zeros).  Returns in fr[0..2] the force of
+// the lane's node.  WAVE: the LDS slots are private to the wave (wave-level
+// ordering), else the caller's block-wide barrier orders them.
+template <bool WAVE>
+__device__ __forceinline__ void mf_hex_forces(int g, bool active, float v[6], float (*sC)[8], float c, const float m[6],
+                                              float Kb, const float *__restrict__ dgen, int ngp_total, size_t gp,
+                                              float fr[24])
 {
-    __shared__ __align__(16) float sC[MF_EPB][8][8];   // per element: c_S of x (0..2) and u (3..5)
-    if (st && halted(st)) return;
-    const int t = threadIdx.x, le = t >> 3, g = t & 7;
-    const int ei = blockIdx.x * MF_EPB + le;
-    const bool active = ei < nel;
-    const int nl = (0x26371504 >> (4 * g)) & 15;   // this lane's LSpace node (0-based)
-    int e = 0;
-    int myq[3] = {0, 0, 0};   // equations of node nl
-    float v[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    if (active) {
-        e = elems ? elems[ei] : ei;
-        const int node = conn[(size_t)e * 8 + nl];
-#pragma unroll
-        for (int d = 0; d < 3; d++) {
-            v[d] = nl ? erel[(size_t)e * 21 + 3 * (nl - 1) + d] : 0.0f;
-            const int q = eqn[3 * (size_t)node + d];
-            myq[d] = q;
-            v[3 + d] = q > 0 ? x[q - 1] : 0.0f;
-        }
-    }
     // the 8 lanes of an element are consecutive lanes of one wave and every
     // lane takes part in the exchanges (inactive ones with zeros)
 #pragma unroll
     for (int d = 3; d < 6; d++) v[d] -= lane_group_first(v[d]);
 #pragma unroll
-    for (int m = 4; m >= 1; m >>= 1) {
-        const float sg = (g & m) ? -1.f : 1.f;
+    for (int mk = 4; mk >= 1; mk >>= 1) {
+        const float sg = (g & mk) ? -1.f : 1.f;
 #pragma unroll
-        for (int k = 0; k < 6; k++) v[k] = fmaf(sg, lane_xor(v[k], m), v[k]);
+        for (int k = 0; k < 6; k++) v[k] = fmaf(sg, lane_xor(v[k], mk), v[k]);
     }
 #pragma unroll
-    for (int k = 0; k < 6; k++) sC[le][g][k] = 0.125f * v[k];
-    __syncthreads();
-    float fr[24];
+    for (int k = 0; k < 6; k++) sC[g][k] = 0.125f * v[k];
+    if constexpr (WAVE) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    } else {
+        __syncthreads();
+    }
 #pragma unroll
     for (int k = 0; k < 24; k++) fr[k] = 0.f;
     if (active) {
@@ -830,8 +818,8 @@ __global__ __launch_bounds__(256) void k_mf_hex(const int *__restrict__ elems, i
         float C[8][6];
 #pragma unroll
         for (int S = 1; S < 8; S++) {
-            const float4 lo = *reinterpret_cast<const float4 *>(&sC[le][S][0]);
-            const float2 hi = *reinterpret_cast<const float2 *>(&sC[le][S][4]);
+            const float4 lo = *reinterpret_cast<const float4 *>(&sC[S][0]);
+            const float2 hi = *reinterpret_cast<const float2 *>(&sC[S][4]);
             C[S][0] = lo.x; C[S][1] = lo.y; C[S][2] = lo.z; C[S][3] = lo.w; C[S][4] = hi.x; C[S][5] = hi.y;
         }
         float J[9], gu[9];   // J[3i+k] = dx_i/dxi_k, gu[3i+k] = du_i/dxi_k
@@ -865,27 +853,6 @@ __global__ __launch_bounds__(256) void k_mf_hex(const int *__restrict__ elems, i
             for (int j = 0; j < 3; j++)
                 H[3 * i + j] = gu[3 * i] * I[j] + gu[3 * i + 1] * I[3 + j] + gu[3 * i + 2] * I[6 + j];
         const float eps[6] = {H[0], H[4], H[8], H[5] + H[7], H[2] + H[6], H[1] + H[3]};
-        const size_t gp = (size_t)e * 8 + g;
-        float c, m[6];
-        const double *mp = mat + (size_t)HIPFEM_MATP * emat[e];
-        const float E = (float)mp[1], nu = (float)mp[2];
-        const float Kb = E * __builtin_amdgcn_rcpf(3.f * (1.f - 2.f * nu));
-        if (desc) {
-            const float2 d01 = *reinterpret_cast<const float2 *>(desc + 6 * gp);
-            const float2 d23 = *reinterpret_cast<const float2 *>(desc + 6 * gp + 2);
-            const float2 d45 = *reinterpret_cast<const float2 *>(desc + 6 * gp + 4);
-            c = d01.x;
-            m[0] = d01.y;
-            m[1] = d23.x;
-            m[2] = -(d01.y + d23.x);
-            m[3] = d23.y;
-            m[4] = d45.x;
-            m[5] = d45.y;
-        } else {
-            c = E * __builtin_amdgcn_rcpf(1.f + nu);
-#pragma unroll
-            for (int i = 0; i < 6; i++) m[i] = 0.f;
-        }
         float sig[6];
         if (c >= 0.f) {
             const float tr = eps[0] + eps[1] + eps[2];
@@ -952,12 +919,77 @@ __global__ __launch_bounds__(256) void k_mf_hex(const int *__restrict__ elems, i
         }
         // inverse transform: f_a = sum_S prod_{j in S} s_aj F_S
 #pragma unroll
-        for (int m = 4; m >= 1; m >>= 1) {
-            const float sg = (g & m) ? 1.f : -1.f;
+        for (int mk = 4; mk >= 1; mk >>= 1) {
+            const float sg = (g & mk) ? 1.f : -1.f;
 #pragma unroll
-            for (int i = 0; i < 3; i++) fr[i] = fmaf(sg, lane_xor(fr[i], m), fr[i]);
+            for (int i = 0; i < 3; i++) fr[i] = fmaf(sg, lane_xor(fr[i], mk), fr[i]);
         }
     }
+}
+
+// the tangent of Gauss point gp: the descriptor (c, m0, m1, m3, m4, m5;
+// m2 = -(m0 + m1)) or, without one, the elastic D of the material table
+__device__ __forceinline__ void mf_point_tangent(const float *__restrict__ desc, size_t gp, float E, float nu,
+                                                 float &c, float m[6])
+{
+    if (desc) {
+        const float2 d01 = *reinterpret_cast<const float2 *>(desc + 6 * gp);
+        const float2 d23 = *reinterpret_cast<const float2 *>(desc + 6 * gp + 2);
+        const float2 d45 = *reinterpret_cast<const float2 *>(desc + 6 * gp + 4);
+        c = d01.x;
+        m[0] = d01.y;
+        m[1] = d23.x;
+        m[2] = -(d01.y + d23.x);
+        m[3] = d23.y;
+        m[4] = d45.x;
+        m[5] = d45.y;
+    } else {
+        c = E * __builtin_amdgcn_rcpf(1.f + nu);
+#pragma unroll
+        for (int i = 0; i < 6; i++) m[i] = 0.f;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_mf_hex(const int *__restrict__ elems, int nel,
+                                                const float *__restrict__ erel, const int *__restrict__ conn,
+                                                const int *__restrict__ eqn, const int *__restrict__ emat,
+                                                const double *__restrict__ mat, const float *__restrict__ desc,
+                                                const float *__restrict__ dgen, int ngp_total,
+                                                const float *__restrict__ x, double *__restrict__ y,
+                                                float *__restrict__ ebuf, float *__restrict__ yslot,
+                                                const uint8_t *__restrict__ ecolor, int neq,
+                                                const PcgState *__restrict__ st)
+{
+    __shared__ __align__(16) float sC[MF_EPB][8][8];   // per element: c_S of x (0..2) and u (3..5)
+    if (st && halted(st)) return;
+    const int t = threadIdx.x, le = t >> 3, g = t & 7;
+    const int ei = blockIdx.x * MF_EPB + le;
+    const bool active = ei < nel;
+    const int nl = (0x26371504 >> (4 * g)) & 15;   // this lane's LSpace node (0-based)
+    int e = 0;
+    int myq[3] = {0, 0, 0};   // equations of node nl
+    float v[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (active) {
+        e = elems ? elems[ei] : ei;
+        const int node = conn[(size_t)e * 8 + nl];
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            v[d] = nl ? erel[(size_t)e * 21 + 3 * (nl - 1) + d] : 0.0f;
+            const int q = eqn[3 * (size_t)node + d];
+            myq[d] = q;
+            v[3 + d] = q > 0 ? x[q - 1] : 0.0f;
+        }
+    }
+    const size_t gp = (size_t)e * 8 + g;
+    float c = 0.f, m[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, Kb = 0.f;
+    if (active) {
+        const double *mp = mat + (size_t)HIPFEM_MATP * emat[e];
+        const float E = (float)mp[1], nu = (float)mp[2];
+        Kb = E * __builtin_amdgcn_rcpf(3.f * (1.f - 2.f * nu));
+        mf_point_tangent(desc, gp, E, nu, c, m);
+    }
+    float fr[24];
+    mf_hex_forces<false>(g, active, v, sC[le], c, m, Kb, dgen, ngp_total, gp, fr);
     if (active) {
         const size_t slot = yslot ? (size_t)ecolor[e] * neq : 0;
 #pragma unroll
@@ -1628,6 +1660,20 @@ void model_unlink(hfem_model *md, hipcsr_mtx *m)
     m->mf_version = -1;
 }
 
+// the element pass of the slot form: one launch over every element
+static int mf_slot_pass(hfem_model *md, const float *x, const PcgState *st)
+{
+    hipStream_t s = md->ctx->stream;
+    if (!md->nelem) return 0;
+    const float *desc = md->tdesc_used ? md->d_tdesc : nullptr;
+    k_mf_hex<<<(md->nelem + MF_EPB - 1) / MF_EPB, 256, 0, s>>>(nullptr, md->nelem, md->d_erel, md->d_conn, md->d_eqn,
+                                                               md->d_emat, md->d_mat, desc, md->d_dsnap, md->ngp_total,
+                                                               x, nullptr, nullptr, md->d_yslot, md->d_ecolor, md->neq,
+                                                               st);
+    HIPFEM_CHECK(hipGetLastError());
+    return 0;
+}
+
 int model_mf_slots_f32(hfem_model *md, const float *x, const PcgState *st, const float **yslot,
                        const uint16_t **eqmask, int *ncolors)
 {
@@ -1635,13 +1681,8 @@ int model_mf_slots_f32(hfem_model *md, const float *x, const PcgState *st, const
     *eqmask = nullptr;
     *ncolors = 0;
     if (!(md->mf_form == 2 && md->d_yslot && md->d_erel)) return 0;
-    hipStream_t s = md->ctx->stream;
-    if (md->nelem)
-        k_mf_hex<<<(md->nelem + MF_EPB - 1) / MF_EPB, 256, 0, s>>>(
-            nullptr, md->nelem, md->d_erel, md->d_conn, md->d_eqn, md->d_emat, md->d_mat,
-            md->tdesc_used ? md->d_tdesc : nullptr, md->d_dsnap, md->ngp_total, x, nullptr, nullptr, md->d_yslot,
-            md->d_ecolor, md->neq, st);
-    HIPFEM_CHECK(hipGetLastError());
+    int rc = mf_slot_pass(md, x, st);
+    if (rc) return rc;
     *yslot = md->d_yslot;
     *eqmask = md->d_eqmask;
     *ncolors = md->ncolors;
@@ -1656,11 +1697,8 @@ int model_mf_apply_f32(hfem_model *md, const float *x, double *y, const PcgState
         // into its color's FP32 slot (no element of a color shares a node),
         // then one FP64 sum per equation over the colors in ascending order
         // (the order of the per-color launches: the same result, bit for bit)
-        if (md->nelem)
-            k_mf_hex<<<(md->nelem + MF_EPB - 1) / MF_EPB, 256, 0, s>>>(
-                nullptr, md->nelem, md->d_erel, md->d_conn, md->d_eqn, md->d_emat, md->d_mat,
-                md->tdesc_used ? md->d_tdesc : nullptr, md->d_dsnap, md->ngp_total, x, y, nullptr, md->d_yslot,
-                md->d_ecolor, md->neq, st);
+        int rc = mf_slot_pass(md, x, st);
+        if (rc) return rc;
         k_mf_slot_sum<<<std::min(8192, (md->neq + 255) / 256), 256, 0, s>>>(md->neq, md->ncolors, md->d_eqmask,
                                                                             md->d_yslot, y, st);
         HIPFEM_CHECK(hipGetLastError());

@@ -11,12 +11,13 @@ Jacobi-PCG to 1e-6 per correction.  The plain FP64 form runs --warmup
 untimed steps, then --steps timed ones; every other form of --variants runs
 the same warm-up and the first --compare-steps of those timed steps:
   fp64                     hipcsr + hippcg FP64 + plain return mapping (value)
-  mixed                    hippcg mixed 2: FP32 PCG with FP64 reliable updates
-                           and the FP32 element-by-element operator
-  mixed_surrogate          the same with the trained MisesMat surrogate (BF16
-                           MFMA inference, residual-checked, plain fallback)
+  mixed_surrogate          hippcg mixed 2 (FP32 PCG with FP64 reliable updates
+                           and the FP32 element-by-element operator) with the
+                           trained MisesMat surrogate (BF16 MFMA inference,
+                           residual-checked, plain fallback)
   fp64_inexact, mixed_surrogate_inexact
                            the same two with Eisenstat-Walker inexact Newton
+  mixed, mixed_inexact     (on request) the mixed form without the surrogate
 value = free DOFs x timed steps / wall time of the FP64 form; every other
 form's speed-up against the FP64 form with the same Newton forcing over the
 same load steps, its errors against plain FP64, the errors of every form
@@ -857,7 +858,7 @@ def main():
                     help="load steps of the ramp (cfg3; at least warmup + steps)")
     ap.add_argument("--compare-steps", type=int, default=2,
                     help="cfg3: timed load steps of every form other than plain FP64 (the first of its timed steps)")
-    ap.add_argument("--variants", default="fp64,mixed,mixed_surrogate,fp64_inexact,mixed_surrogate_inexact",
+    ap.add_argument("--variants", default="fp64,mixed_surrogate,fp64_inexact,mixed_surrogate_inexact",
                     help="cfg3 forms to run (comma separated)")
     ap.add_argument("--check-nx", type=int, default=10,
                     help="cfg3 oracle check on an n x n x 4n slab (0: off)")

@@ -9,8 +9,9 @@ model, export modules, ``domain``, ``OutputManager``, the component counts,
 then dof managers, elements, cross sections, materials, boundary conditions,
 initial conditions, load-time functions and sets.
 
-Supported records: node; LSpace / LTRSpace / QSpace; SimpleCS; IsoLE and
-MisesMat (with density ``d``); BoundaryCondition; ConstantSurfaceLoad;
+Supported records: node; LSpace / LTRSpace / QSpace; SimpleCS; IsoLE,
+MisesMat and TutorialMaterial (the same J2 return as MisesMat; with density
+``d``); BoundaryCondition; ConstantSurfaceLoad;
 DeadWeight; ConstantFunction and
 PiecewiseLinFunction; Set with nodes / elements / elementranges /
 elementBoundaries / noderanges.
@@ -320,6 +321,15 @@ def read_deck(path) -> DeckProblem:
                 raise DeckError("MisesMat: only htype 0 (linear hardening) is supported")
             row[3] = r.scalar("sig0")
             row[4] = r.scalar("h", 0.0)
+        elif r.keyword == "tutorialmaterial":
+            # TutorialMaterial (tutorialmaterial.C:80-163) is the same J2
+            # radial return with linear isotropic hardening and the same
+            # consistent tangent as MisesMat: yield sqrt(3 J2) - (sigy + H k),
+            # multiplier phi / (3G + H); its records map onto the MisesMat
+            # kernels (sigy -> sig0, H -> h)
+            row[0] = MAT_MISES
+            row[3] = r.scalar("sigy")
+            row[4] = r.scalar("h", 0.0)
         else:
             raise DeckError(f"unsupported material {r.keyword}")
         row[1] = r.scalar("e")
@@ -327,9 +337,14 @@ def read_deck(path) -> DeckProblem:
         mrows.append(row)
         mnum[r.number] = i
     emat = np.full(ne, -1, dtype=np.int32)
+    csmat = {cs.number: mnum[cs.scalar("material", cast=int)] for cs in css if cs.has("material")}
     for i, r in enumerate(elems):
         if r.has("mat"):
             emat[i] = mnum[r.scalar("mat", cast=int)]
+        elif r.has("crosssect") and r.scalar("crosssect", cast=int) in csmat:
+            # the element's cross section names the material (Element::giveMaterial
+            # through SimpleCrossSection, simplecrosssection.C)
+            emat[i] = csmat[r.scalar("crosssect", cast=int)]
     for cs in css:
         if cs.keyword != "simplecs":
             raise DeckError(f"unsupported cross section {cs.keyword}")

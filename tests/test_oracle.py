@@ -182,6 +182,48 @@ def test_reference_deck_checks_oracle(name):
     assert not bad, bad
 
 
+def run_strain_driven_deck_oracle(prob):
+    """A deck whose every DOF is prescribed (no free equations): each step
+    is one return map per Gauss point from the committed state, then a
+    commit, as the reference's StaticStructural step does when neq = 0."""
+    from tests.deckrun import _collect
+    d = prob.deck
+    assert d.neq == 0
+    st = oracle.state_new(d)
+    out = {}
+    for k in range(1, prob.nsteps + 1):
+        up, (_, fp) = prob.step_loading(k)
+        _, fpi, _ = oracle.internal_forces(d, np.zeros(0), up, state=st, update_state=True)
+        oracle.state_commit(d, st)
+        out[k] = _collect(prob, np.zeros(0), up, fpi - fp, st)
+    return out
+
+
+def test_tutorialmaterial_deck_pins_the_3d_return_map():
+    """tests/sm/tutorialmaterial.in: one LSpace brick under a prescribed
+    strain path loaded past yield and then unloaded, with TutorialMaterial,
+    which computes the same J2 radial return and isotropic hardening as
+    MisesMat (tutorialmaterial.C:80-163 against misesmat.C).  Its 48 #ELEMENT
+    checks are the reference's own stresses at every Gauss point and step."""
+    prob = read_deck(os.path.join(GOLDEN, "tutorialmaterial.in"))
+    assert prob.deck.mat[0, 0] == meshgen.MAT_MISES and prob.deck.neq == 0
+    res = run_strain_driven_deck_oracle(prob)
+    checks = evaluate_checks(prob, res)
+    assert len(checks) == len(prob.checks) == 48
+    bad = [(c.line, v) for c, v, ok in checks if not ok]
+    assert not bad, bad
+    # the path goes plastic: the equivalent plastic strain grows
+    kappa = [res[k]["state"][:, :, 6].max() for k in sorted(res)]
+    assert max(kappa) > 0
+    # planted mistakes, on the CPU: H 1 % off, sigma_y 0.1 % off, nu 1 % off
+    # each move half of the checks beyond the deck's 1e4 Pa tolerance
+    for col, f in ((4, 1.01), (3, 1.001), (2, 1.01)):
+        p2 = read_deck(os.path.join(GOLDEN, "tutorialmaterial.in"))
+        p2.deck.mat[0, col] *= f
+        c2 = evaluate_checks(p2, run_strain_driven_deck_oracle(p2))
+        assert sum(not ok for _, _, ok in c2) >= 12, (col, f)
+
+
 # ----------------------------------------------------------------- elements
 def _jittered(etype, seed=0):
     rng = np.random.default_rng(seed)

@@ -184,8 +184,9 @@ def test_tutorialmaterial_deck_on_gpu(ctx):
     """The reference's tutorialmaterial deck (one LSpace brick, every DOF
     prescribed, TutorialMaterial = the MisesMat J2 return) through the
     evaluator's MisesMat kernel: each step one return map per Gauss point
-    and a commit.  The 48 #ELEMENT stress checks hold, and stresses, plastic
-    strains and kappa agree with the oracle's to 1e-12 of their scale."""
+    and a commit.  The 48 #ELEMENT stress checks hold, and at every step the
+    stresses, plastic strains and kappa agree with the oracle's to 1e-12 of
+    their scale."""
     from oofem_amd.hipfem import BatchedEvaluator, HipCSRSparseMtrx
     from tests.deckrun import _collect
     from tests.test_oracle import run_strain_driven_deck_oracle

@@ -195,16 +195,18 @@ def run_strain_driven_deck_oracle(prob):
         up, (_, fp) = prob.step_loading(k)
         _, fpi, _ = oracle.internal_forces(d, np.zeros(0), up, state=st, update_state=True)
         oracle.state_commit(d, st)
-        out[k] = _collect(prob, np.zeros(0), up, fpi - fp, st)
+        out[k] = _collect(prob, np.zeros(0), up, fpi - fp, st.copy())
     return out
 
 
 def test_tutorialmaterial_deck_pins_the_3d_return_map():
-    """tests/sm/tutorialmaterial.in: one LSpace brick under a prescribed
-    strain path loaded past yield and then unloaded, with TutorialMaterial,
-    which computes the same J2 radial return and isotropic hardening as
-    MisesMat (tutorialmaterial.C:80-163 against misesmat.C).  Its 48 #ELEMENT
-    checks are the reference's own stresses at every Gauss point and step."""
+    """tests/sm/tutorialmaterial.in: one LSpace brick in uniaxial strain
+    (bottom face held, top face pulled 3 % in z with its lateral motion
+    held) over five steps, loaded to about ten times the yield stress, with
+    TutorialMaterial, which computes the same J2 radial return and linear
+    isotropic hardening as MisesMat (tutorialmaterial.C:80-163 against
+    misesmat.C).  Its 48 #ELEMENT checks are the reference's own stresses at
+    every Gauss point at the last step."""
     prob = read_deck(os.path.join(GOLDEN, "tutorialmaterial.in"))
     assert prob.deck.mat[0, 0] == meshgen.MAT_MISES and prob.deck.neq == 0
     res = run_strain_driven_deck_oracle(prob)

@@ -319,6 +319,10 @@ def read_deck(path) -> DeckProblem:
             row[0] = MAT_MISES
             if r.scalar("htype", 0, int) != 0:
                 raise DeckError("MisesMat: only htype 0 (linear hardening) is supported")
+            if r.scalar("omega_crit", 0.0) != 0.0:
+                # misesmat.C:121-133, 451-478, 535-546: stress (1 - omega) sigma_eff
+                # and an unsymmetric tangent term; not part of this build
+                raise DeckError("MisesMat: damage (omega_crit != 0) is not supported")
             row[3] = r.scalar("sig0")
             row[4] = r.scalar("h", 0.0)
         elif r.keyword == "tutorialmaterial":

@@ -199,6 +199,24 @@ def run_strain_driven_deck_oracle(prob):
     return out
 
 
+def test_deck_reader_refuses_mises_damage(tmp_path):
+    """MisesMat with damage (omega_crit, misesmat.C:451-478) scales the
+    stress by (1 - omega) and adds an unsymmetric tangent term, which this
+    path does not compute: the reader refuses such a deck instead of
+    dropping the keyword."""
+    from oofem_amd.deck import DeckError
+    text = open(os.path.join(GOLDEN, "tutorialmaterial.in")).read()
+    old = "tutorialmaterial 1 d 0. E 200.0e9 n 0.3 sigy 500.0e6 H 10.0e9 talpha -1.0"
+    assert old in text
+    ok = tmp_path / "mises.in"
+    ok.write_text(text.replace(old, "MisesMat 1 d 0. E 200.0e9 n 0.3 sig0 500.0e6 H 10.0e9 tAlpha 0."))
+    assert read_deck(str(ok)).deck.mat[0, 3] == 500e6
+    bad = tmp_path / "mises_damage.in"
+    bad.write_text(text.replace(old, "MisesMat 1 d 0. E 200.0e9 n 0.3 sig0 500.0e6 H 10.0e9 omega_crit 0.1 a 0.1"))
+    with pytest.raises(DeckError, match="damage"):
+        read_deck(str(bad))
+
+
 def test_tutorialmaterial_deck_pins_the_3d_return_map():
     """tests/sm/tutorialmaterial.in: one LSpace brick in uniaxial strain
     (bottom face held, top face pulled 3 % in z with its lateral motion

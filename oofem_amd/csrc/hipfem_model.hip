@@ -782,16 +782,14 @@ __device__ __forceinline__ float lane_group_first(float v)
 // coordinates relative to node 1, v[3..5]: its operand values; sC: the
 // element's 8x8 LDS coefficient slots; c / m / Kb: the lane point's
 // tangent (c < 0: the general D of dgen at gp).  Every lane of the group
-// takes part (inactive ones with zeros).  Returns in fr[0..2] the force of
-// the lane's node.  WAVE: the LDS slots are private to the wave (wave-level
-// ordering), else the caller's block-wide barrier orders them.
-template <bool WAVE>
-__device__ __forceinline__ void mf_hex_forces(int g, bool active, float v[6], float (*sC)[8], float c, const float m[6],
+// takes part, and the caller's block-wide barrier orders the LDS slots.
+// Returns in fr[0..2] the force of the lane's node.
+__device__ __forceinline__ void mf_hex_forces(int g, float v[6], float (*sC)[8], float c, const float m[6],
                                               float Kb, const float *__restrict__ dgen, int ngp_total, size_t gp,
                                               float fr[24])
 {
     // the 8 lanes of an element are consecutive lanes of one wave and every
-    // lane takes part in the exchanges (inactive ones with zeros)
+    // lane takes part in the exchanges
 #pragma unroll
     for (int d = 3; d < 6; d++) v[d] -= lane_group_first(v[d]);
 #pragma unroll
@@ -802,16 +800,9 @@ __device__ __forceinline__ void mf_hex_forces(int g, bool active, float v[6], fl
     }
 #pragma unroll
     for (int k = 0; k < 6; k++) sC[g][k] = 0.125f * v[k];
-    if constexpr (WAVE) {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    } else {
-        __syncthreads();
-    }
-#pragma unroll
-    for (int k = 0; k < 24; k++) fr[k] = 0.f;
-    if (active) {
+    __syncthreads();
+    fr[0] = fr[1] = fr[2] = 0.f;   // F_0 = 0
+    {
         const float a = 0.577350269189626f;
         const float xi = (g & 4) ? a : -a, eta = (g & 2) ? a : -a, ze = (g & 1) ? a : -a;
         const float ez = eta * ze, xz = xi * ze, xe = xi * eta;
@@ -835,24 +826,25 @@ __device__ __forceinline__ void mf_hex_forces(int g, bool active, float v[6], fl
         const float det = J[0] * (J[4] * J[8] - J[5] * J[7]) - J[1] * (J[3] * J[8] - J[5] * J[6]) +
                           J[2] * (J[3] * J[7] - J[4] * J[6]);
         const float id = __builtin_amdgcn_rcpf(det);   // v_rcp_f32 (1 ulp)
-        float I[9];   // J^-1
-        I[0] = (J[4] * J[8] - J[5] * J[7]) * id;
-        I[1] = (J[2] * J[7] - J[1] * J[8]) * id;
-        I[2] = (J[1] * J[5] - J[2] * J[4]) * id;
-        I[3] = (J[5] * J[6] - J[3] * J[8]) * id;
-        I[4] = (J[0] * J[8] - J[2] * J[6]) * id;
-        I[5] = (J[2] * J[3] - J[0] * J[5]) * id;
-        I[6] = (J[3] * J[7] - J[4] * J[6]) * id;
-        I[7] = (J[1] * J[6] - J[0] * J[7]) * id;
-        I[8] = (J[0] * J[4] - J[1] * J[3]) * id;
-        // H = du/dx = gu J^-1
+        float I[9];   // adj J = det J J^-1
+        I[0] = J[4] * J[8] - J[5] * J[7];
+        I[1] = J[2] * J[7] - J[1] * J[8];
+        I[2] = J[1] * J[5] - J[2] * J[4];
+        I[3] = J[5] * J[6] - J[3] * J[8];
+        I[4] = J[0] * J[8] - J[2] * J[6];
+        I[5] = J[2] * J[3] - J[0] * J[5];
+        I[6] = J[3] * J[7] - J[4] * J[6];
+        I[7] = J[1] * J[6] - J[0] * J[7];
+        I[8] = J[0] * J[4] - J[1] * J[3];
+        // H = du/dx det J = gu adj J; the strain takes the 1 / det J
         float H[9];
 #pragma unroll
         for (int i = 0; i < 3; i++)
 #pragma unroll
             for (int j = 0; j < 3; j++)
                 H[3 * i + j] = gu[3 * i] * I[j] + gu[3 * i + 1] * I[3 + j] + gu[3 * i + 2] * I[6 + j];
-        const float eps[6] = {H[0], H[4], H[8], H[5] + H[7], H[2] + H[6], H[1] + H[3]};
+        const float eps[6] = {H[0] * id, H[4] * id, H[8] * id, (H[5] + H[7]) * id, (H[2] + H[6]) * id,
+                              (H[1] + H[3]) * id};
         float sig[6];
         if (c >= 0.f) {
             const float tr = eps[0] + eps[1] + eps[2];
@@ -877,15 +869,18 @@ __device__ __forceinline__ void mf_hex_forces(int g, bool active, float v[6], fl
             sig[4] = D[4] * eps[0] + D[9] * eps[1] + D[13] * eps[2] + D[16] * eps[3] + D[18] * eps[4] + D[19] * eps[5];
             sig[5] = D[5] * eps[0] + D[10] * eps[1] + D[14] * eps[2] + D[17] * eps[3] + D[19] * eps[4] + D[20] * eps[5];
         }
-        // T = dV sigma J^-T / 8 (T[3i+k]); f_a = sum_k T[., k] dN_a/dxi_k
-        const float dV = 0.125f * fabsf(det);
-        const float S[9] = {sig[0], sig[5], sig[4], sig[5], sig[1], sig[3], sig[4], sig[3], sig[2]};
+        // T = dV sigma J^-T / 8 (T[3i+k]) = sign(det J) / 8 sigma adj J^T,
+        // with dV = |det J| (the unit Gauss weight); f_a = sum_k T[., k] dN_a/dxi_k
+        const float sc = det < 0.f ? -0.125f : 0.125f;
+        const float s0 = sc * sig[0], s1 = sc * sig[1], s2 = sc * sig[2], s3 = sc * sig[3], s4 = sc * sig[4],
+                    s5 = sc * sig[5];
+        const float S[9] = {s0, s5, s4, s5, s1, s3, s4, s3, s2};
 #pragma unroll
         for (int i = 0; i < 3; i++) {
             float T[3];
 #pragma unroll
             for (int k = 0; k < 3; k++)
-                T[k] = dV * (S[3 * i] * I[3 * k] + S[3 * i + 1] * I[3 * k + 1] + S[3 * i + 2] * I[3 * k + 2]);
+                T[k] = S[3 * i] * I[3 * k] + S[3 * i + 1] * I[3 * k + 1] + S[3 * i + 2] * I[3 * k + 2];
             // subset terms F_S (S as lane bits xi 4, eta 2, zeta 1; F_0 = 0)
             fr[3 + i] = T[2];
             fr[6 + i] = T[1];
@@ -963,14 +958,16 @@ __global__ __launch_bounds__(256) void k_mf_hex(const int *__restrict__ elems, i
     __shared__ __align__(16) float sC[MF_EPB][8][8];   // per element: c_S of x (0..2) and u (3..5)
     if (st && halted(st)) return;
     const int t = threadIdx.x, le = t >> 3, g = t & 7;
-    const int ei = blockIdx.x * MF_EPB + le;
-    const bool active = ei < nel;
+    // the lanes of a slot past the element count (last block only) repeat
+    // the last element and store nothing (nel >= 1)
+    const int ei0 = blockIdx.x * MF_EPB + le;
+    const bool active = ei0 < nel;
+    const int ei = active ? ei0 : nel - 1;
     const int nl = (0x26371504 >> (4 * g)) & 15;   // this lane's LSpace node (0-based)
-    int e = 0;
-    int myq[3] = {0, 0, 0};   // equations of node nl
-    float v[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    if (active) {
-        e = elems ? elems[ei] : ei;
+    const int e = elems ? elems[ei] : ei;
+    int myq[3];   // equations of node nl
+    float v[6];
+    {
         const int node = conn[(size_t)e * 8 + nl];
 #pragma unroll
         for (int d = 0; d < 3; d++) {
@@ -981,15 +978,16 @@ __global__ __launch_bounds__(256) void k_mf_hex(const int *__restrict__ elems, i
         }
     }
     const size_t gp = (size_t)e * 8 + g;
-    float c = 0.f, m[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f}, Kb = 0.f;
-    if (active) {
-        const double *mp = mat + (size_t)HIPFEM_MATP * emat[e];
-        const float E = (float)mp[1], nu = (float)mp[2];
-        Kb = E * __builtin_amdgcn_rcpf(3.f * (1.f - 2.f * nu));
-        mf_point_tangent(desc, gp, E, nu, c, m);
-    }
+    float c, m[6];
+    const double *mp = mat + (size_t)HIPFEM_MATP * emat[e];
+    const float E = (float)mp[1], nu = (float)mp[2];
+    const float Kb = E * __builtin_amdgcn_rcpf(3.f * (1.f - 2.f * nu));
+    mf_point_tangent(desc, gp, E, nu, c, m);
     float fr[24];
-    mf_hex_forces<false>(g, active, v, sC[le], c, m, Kb, dgen, ngp_total, gp, fr);
+    // every lane computes: the 8 lanes of an element are all active or all
+    // inactive, and the exchanges stay inside the 8, so what inactive lanes
+    // compute reaches no stored force
+    mf_hex_forces(g, v, sC[le], c, m, Kb, dgen, ngp_total, gp, fr);
     if (active) {
         const size_t slot = yslot ? (size_t)ecolor[e] * neq : 0;
 #pragma unroll

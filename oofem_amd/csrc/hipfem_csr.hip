@@ -949,8 +949,10 @@ extern "C" int hipcsr_add_diagonal(hipcsr_mtx *m, double a, const double *d)
     double *dd = nullptr;
     HIPFEM_CHECK(hipMalloc(&dd, sizeof(double) * std::max(1, m->n)));
     HIPFEM_CHECK(hipMemcpyAsync(dd, d, sizeof(double) * m->n, hipMemcpyHostToDevice, m->ctx->stream));
-    k_add_diag<<<(m->n + 255) / 256, 256, 0, m->ctx->stream>>>(m->n, a, dd, m->d_diag, m->d_val);
-    HIPFEM_CHECK(hipGetLastError());
+    if (m->n) {
+        k_add_diag<<<(m->n + 255) / 256, 256, 0, m->ctx->stream>>>(m->n, a, dd, m->d_diag, m->d_val);
+        HIPFEM_CHECK(hipGetLastError());
+    }
     HIPFEM_CHECK(hipStreamSynchronize(m->ctx->stream));
     HIPFEM_CHECK(hipFree(dd));
     m->version++;

@@ -738,8 +738,11 @@ static int init_precond(hippcg_solver *s, hipcsr_mtx *A)
     hipStream_t st = s->ctx->stream;
     const int big = INT32_MAX;
     HIPFEM_CHECK(hipMemcpyAsync(s->zero_row, &big, sizeof(int), hipMemcpyHostToDevice, st));
-    k_dinv<<<(A->n + 255) / 256, 256, 0, st>>>(A->n, A->d_diag, A->d_val, s->precond == 1, s->dinv, s->dinv32, s->zero_row);
-    HIPFEM_CHECK(hipGetLastError());
+    if (A->n) {
+        k_dinv<<<(A->n + 255) / 256, 256, 0, st>>>(A->n, A->d_diag, A->d_val, s->precond == 1, s->dinv, s->dinv32,
+                                                   s->zero_row);
+        HIPFEM_CHECK(hipGetLastError());
+    }
     int zr = big;
     HIPFEM_CHECK(hipMemcpyAsync(&zr, s->zero_row, sizeof(int), hipMemcpyDeviceToHost, st));
     HIPFEM_CHECK(hipStreamSynchronize(st));
@@ -1198,8 +1201,10 @@ extern "C" int hippcg_solve_dev_noalloc(hippcg_solver *s, hipcsr_mtx *A, const d
     // DiagPreconditioner::init on every call: the scratch holds whatever the
     // last call on this GPU left there
     k_set_int<<<1, 64, 0, st>>>(zero_row, INT32_MAX);
-    k_dinv<<<(A->n + 255) / 256, 256, 0, st>>>(A->n, A->d_diag, A->d_val, s->precond == 1, dinv, nullptr, zero_row);
-    HIPFEM_CHECK(hipGetLastError());
+    if (A->n) {
+        k_dinv<<<(A->n + 255) / 256, 256, 0, st>>>(A->n, A->d_diag, A->d_val, s->precond == 1, dinv, nullptr, zero_row);
+        HIPFEM_CHECK(hipGetLastError());
+    }
     HIPFEM_CHECK(hipMemcpyAsync(s->ctx->h_pinned, zero_row, sizeof(int), hipMemcpyDeviceToHost, st));
     HIPFEM_CHECK(hipStreamSynchronize(st));
     int zr = 0;

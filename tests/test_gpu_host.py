@@ -211,13 +211,15 @@ def test_time_step_reduction_matches_oracle():
     P.close()
 
 
-@pytest.mark.parametrize("name", ["patch301.in", "cantilever_Qspace.in", "deadweight02.in"])
+@pytest.mark.parametrize("name", ["patch301.in", "cantilever_Qspace.in", "deadweight02.in", "tutorialmaterial.in"])
 def test_run_reference_deck_end_to_end(tmp_path, name):
     """A reference deck through the deck reader, the C++ StaticStructural
     driver behind the hipcsr / hippcg plug-ins (the deck's own keywords plus
     smtype 11 lstype 9 lstol 1e-12), the .out writer and the errorcheck
     block: every check holds on the computed values, and every #NODE check
-    holds on the .out file read back."""
+    holds on the .out file read back.  tutorialmaterial prescribes every
+    DOF (no free equations): each step is one return map per Gauss point
+    and the Newton loop converges at its first check, as NRSolver does."""
     import os
 
     from oofem_amd.deck import read_deck

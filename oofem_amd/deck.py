@@ -142,6 +142,7 @@ class DeckProblem:
     tolerance: float
     bodyloads: list = field(default_factory=list)
     density: np.ndarray = None     # per material index (record ``d``)
+    nodalloads: list = field(default_factory=list)
 
     def time_of_step(self, k):
         """StaticStructural / LinearStatic: step k (1-based) at t = k * deltaT."""
@@ -180,6 +181,12 @@ class DeckProblem:
             ff, fp = d.external_load(fac)
             f_free += ff
             f_presc += fp
+        for nl in self.nodalloads:
+            fac = self.ltf[nl["ltf"]](t)
+            for dof, c in zip(nl["dofs"], nl["comps"]):
+                q = d.eqn[nl["nodes"], dof - 1]
+                np.add.at(f_free, q[q > 0] - 1, c * fac)
+                np.add.at(f_presc, -q[q < 0] - 1, c * fac)
         bl_fn = body_load or body_load_vector
         for bl in self.bodyloads:
             fac = self.ltf[bl["ltf"]](t)
@@ -288,6 +295,12 @@ def read_deck(path) -> DeckProblem:
     if len(etypes) != 1 or next(iter(etypes)) not in _ETYPES:
         raise DeckError(f"unsupported element types {etypes}")
     etype = _ETYPES[next(iter(etypes))]
+    for r in elems:
+        # NLStructuralElement nlGeometry 1 (Green-Lagrange strain,
+        # nlstructuralelement.C:151-241, 332-448) is outside this small-strain path
+        if r.scalar("nlgeo", 0, int) != 0:
+            raise DeckError(f"element {r.number}: nlgeo {r.scalar('nlgeo', 0, int)} (geometric nonlinearity) "
+                            "is not supported")
     elem_numbers = np.array([r.number for r in elems])
     eid = {n: i for i, n in enumerate(elem_numbers)}
     conn = np.array([[nid[v] for v in r.array("nodes", int)] for r in elems], dtype=np.int32)
@@ -368,7 +381,7 @@ def read_deck(path) -> DeckProblem:
             raise DeckError(f"unsupported load-time function {r.keyword}")
     # boundary conditions and loads
     bc_mask = np.zeros((nd, 3), dtype=bool)
-    blist, llist, bodies = [], [], []
+    blist, llist, bodies, nodal = [], [], [], []
     for r in bcs:
         f = r.scalar("loadtimefunction", cast=int)
         if r.keyword == "boundarycondition":
@@ -387,6 +400,16 @@ def read_deck(path) -> DeckProblem:
                 tr[dof - 1] = c
             faces = setd[r.scalar("set", cast=int)]["faces"]
             llist.append(dict(faces=faces, traction=tr, ltf=f))
+        elif r.keyword == "nodalload":
+            # NodalLoad (nodalload.C): Components on the listed dofs of every
+            # node of the set, times the load-time function; on a prescribed
+            # dof it enters the reactions as an external load
+            dofs = r.array("dofs", int)
+            comps = r.array("components")
+            if len(comps) != len(dofs):
+                raise DeckError("NodalLoad: one component per listed dof")
+            nodes_s = np.array(setd[r.scalar("set", cast=int)].get("nodes", []), dtype=np.int64)
+            nodal.append(dict(nodes=nodes_s, dofs=dofs, comps=comps, ltf=f))
         elif r.keyword == "deadweight":
             # deadweight.C: Components are the force per unit mass in the
             # global c.s.; the element set carries it
@@ -401,7 +424,7 @@ def read_deck(path) -> DeckProblem:
     deck = Deck(str(path), etype, coords, conn, emat, mat, bc_mask, np.zeros((nd, 3)))
     tol, checks = _parse_checks(checks_txt)
     prob = DeckProblem(deck, engng.keyword, engng.scalar("nsteps", 1, int), engng, blist, llist, ltf,
-                       node_numbers, elem_numbers, checks, tol, bodies, np.array(dens))
+                       node_numbers, elem_numbers, checks, tol, bodies, np.array(dens), nodal)
     prob.output_file = lines[0].strip()
     prob.metasteps = metasteps
     if metasteps:

@@ -211,7 +211,8 @@ def test_time_step_reduction_matches_oracle():
     P.close()
 
 
-@pytest.mark.parametrize("name", ["patch301.in", "cantilever_Qspace.in", "deadweight02.in", "tutorialmaterial.in"])
+@pytest.mark.parametrize("name", ["patch301.in", "patch300.in", "cantilever_Qspace.in", "deadweight02.in",
+                                  "tutorialmaterial.in"])
 def test_run_reference_deck_end_to_end(tmp_path, name):
     """A reference deck through the deck reader, the C++ StaticStructural
     driver behind the hipcsr / hippcg plug-ins (the deck's own keywords plus

@@ -167,7 +167,7 @@ def test_config0_tight_solve_matches_direct(ctx):
     assert np.linalg.norm(got["u"] - ref["u"]) / np.linalg.norm(ref["u"]) < 1e-12
 
 
-@pytest.mark.parametrize("name", ["patch301.in", "cantilever_Qspace.in", "deadweight02.in"])
+@pytest.mark.parametrize("name", ["patch301.in", "patch300.in", "cantilever_Qspace.in", "deadweight02.in"])
 def test_reference_decks_on_gpu(ctx, name):
     prob = read_deck(os.path.join(GOLDEN, name))
     got = run_deck_gpu(ctx, prob)

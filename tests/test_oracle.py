@@ -170,7 +170,7 @@ def test_traction_resultant(cfg0):
     assert abs(f.sum() + 1.0) < 1e-14      # 1 MPa on a 1 m^2 face, in -y
 
 
-@pytest.mark.parametrize("name", ["patch301.in", "cantilever_Qspace.in", "deadweight02.in"])
+@pytest.mark.parametrize("name", ["patch301.in", "patch300.in", "cantilever_Qspace.in", "deadweight02.in"])
 def test_reference_deck_checks_oracle(name):
     prob = read_deck(os.path.join(GOLDEN, name))
     res = run_deck_oracle(prob)
@@ -199,11 +199,12 @@ def run_strain_driven_deck_oracle(prob):
     return out
 
 
-def test_deck_reader_refuses_mises_damage(tmp_path):
+def test_deck_reader_refuses_what_the_path_does_not_compute(tmp_path):
     """MisesMat with damage (omega_crit, misesmat.C:451-478) scales the
-    stress by (1 - omega) and adds an unsymmetric tangent term, which this
-    path does not compute: the reader refuses such a deck instead of
-    dropping the keyword."""
+    stress by (1 - omega) and adds an unsymmetric tangent term, and nlgeo 1
+    elements use the Green-Lagrange strain; this small-strain path computes
+    neither, so the reader refuses such decks instead of dropping the
+    keyword."""
     from oofem_amd.deck import DeckError
     text = open(os.path.join(GOLDEN, "tutorialmaterial.in")).read()
     old = "tutorialmaterial 1 d 0. E 200.0e9 n 0.3 sigy 500.0e6 H 10.0e9 talpha -1.0"
@@ -215,6 +216,13 @@ def test_deck_reader_refuses_mises_damage(tmp_path):
     bad.write_text(text.replace(old, "MisesMat 1 d 0. E 200.0e9 n 0.3 sig0 500.0e6 H 10.0e9 omega_crit 0.1 a 0.1"))
     with pytest.raises(DeckError, match="damage"):
         read_deck(str(bad))
+    # geometric nonlinearity (the reference's brick_nlgeo decks) likewise
+    old_el = "lspace  1 nodes 8 1 2 3 4 5 6 7 8 crossSect 1"
+    assert old_el in text
+    nl = tmp_path / "nlgeo.in"
+    nl.write_text(text.replace(old_el, old_el + " nlgeo 1"))
+    with pytest.raises(DeckError, match="nlgeo"):
+        read_deck(str(nl))
 
 
 def test_tutorialmaterial_deck_pins_the_3d_return_map():
@@ -731,7 +739,7 @@ def test_no_reduction_keeps_a_failed_step():
     assert any(s["status"] != 1 for s in steps)
 
 
-@pytest.mark.parametrize("name", ["patch301.in", "cantilever_Qspace.in", "deadweight02.in"])
+@pytest.mark.parametrize("name", ["patch301.in", "patch300.in", "cantilever_Qspace.in", "deadweight02.in"])
 def test_out_file_round_trip_and_checks(name):
     """The .out writer on the oracle's solution of a reference deck: the file
     reads back to the computed values within its print precision, every

@@ -48,6 +48,7 @@ namespace hipfem {
 
 constexpr int RED_THREADS = 1024;
 constexpr int MAX_GRID = 4096;
+constexpr int GRP_MAX_GRID = 8192;   // row-group PCG product: more resident workgroups
 
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v)
@@ -810,7 +811,7 @@ static int pcg_run(hippcg_solver *s, const PcgRunBufs &bufs, hipcsr_mtx *A, cons
     const bool nodeblock = sh.nodeblock && A->nodeblocked;
     const bool grouped = sh.nodeblock && !A->nodeblocked && A->grouped;
     const int gs = nodeblock    ? std::max(1, std::min((A->nbr + 7) / 8, MAX_GRID))
-                   : grouped   ? std::max(1, std::min((A->ngrp + GRP_PER_BLOCK - 1) / GRP_PER_BLOCK, MAX_GRID))
+                   : grouped   ? std::max(1, std::min((A->ngrp + GRP_PER_BLOCK - 1) / GRP_PER_BLOCK, GRP_MAX_GRID))
                    : sh.stream ? std::max(1, std::min((n + sh.rpb - 1) / sh.rpb, MAX_GRID))
                                : spmv_grid(n, sh.lanes);
     // p = z + beta p formed in the product's gather (the lane, node-block
@@ -944,7 +945,7 @@ static int pcg_reliable(hippcg_solver *s, hipcsr_mtx *A, const double *db, doubl
     const bool nodeblock = sh.nodeblock && A->nodeblocked;
     const bool grouped = sh.nodeblock && !A->nodeblocked && A->grouped;
     const int gs = nodeblock    ? std::max(1, std::min((A->nbr + 7) / 8, MAX_GRID))
-                   : grouped   ? std::max(1, std::min((A->ngrp + GRP_PER_BLOCK - 1) / GRP_PER_BLOCK, MAX_GRID))
+                   : grouped   ? std::max(1, std::min((A->ngrp + GRP_PER_BLOCK - 1) / GRP_PER_BLOCK, GRP_MAX_GRID))
                    : sh.stream ? std::max(1, std::min((n + sh.rpb - 1) / sh.rpb, MAX_GRID))
                                : spmv_grid(n, sh.lanes);
     rc = ensure_partials(s->ctx, 3 * (size_t)std::max(gv, gs));
@@ -1144,7 +1145,7 @@ static int solve_dev(hippcg_solver *s, hipcsr_mtx *A, const double *db, double *
 // and the zero-diagonal flag.  Host code never uses the array's name as an
 // address: it takes the device's address once from hipGetSymbolAddress.
 constexpr size_t NA_N = HIPPCG_NOALLOC_MAX_N;
-constexpr size_t NA_PART = 3 * (size_t)MAX_GRID;
+constexpr size_t NA_PART = 3 * (size_t)std::max(MAX_GRID, GRP_MAX_GRID);
 constexpr size_t NA_OFF_PART = 6 * NA_N * sizeof(double);
 constexpr size_t NA_OFF_STATE = NA_OFF_PART + NA_PART * sizeof(double);
 constexpr size_t NA_OFF_FLAG = NA_OFF_STATE + 256;

@@ -48,7 +48,7 @@ namespace hipfem {
 
 constexpr int RED_THREADS = 1024;
 constexpr int MAX_GRID = 4096;
-constexpr int GRP_MAX_GRID = 8192;   // row-group PCG product: more resident workgroups
+constexpr int GRP_MAX_GRID = 16384;   // row-group PCG product: more resident workgroups
 
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v)

@@ -367,10 +367,23 @@ __global__ __launch_bounds__(256) void k_pcg_init(int n, const VT *__restrict__ 
     }
 }
 
+// thread t sums p[t], p[t + RED_THREADS], ... in that order; the loads of
+// eight terms are issued before their adds (the same sum, bit for bit, with
+// one memory latency per eight terms instead of one per term)
 __device__ double reduce_partials(const double *__restrict__ p, int nb, double *sh)
 {
     double s = 0.0;
-    for (int i = threadIdx.x; i < nb; i += RED_THREADS) s += p[i];
+    for (int i0 = threadIdx.x; i0 < nb; i0 += 8 * RED_THREADS) {
+        double v[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+            const int i = i0 + j * RED_THREADS;
+            v[j] = i < nb ? p[i] : 0.0;
+        }
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            if (i0 + j * RED_THREADS < nb) s += v[j];
+    }
     return block_sum<RED_THREADS>(s, sh);
 }
 

@@ -48,7 +48,7 @@ namespace hipfem {
 
 constexpr int RED_THREADS = 1024;
 constexpr int MAX_GRID = 4096;
-constexpr int GRP_MAX_GRID = 16384;   // row-group PCG product: more resident workgroups
+constexpr int GRP_MAX_GRID = 16384;   // row-group and node-block PCG products: more resident workgroups
 
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v)
@@ -823,7 +823,7 @@ static int pcg_run(hippcg_solver *s, const PcgRunBufs &bufs, hipcsr_mtx *A, cons
     const int gv = vec_grid(n);
     const bool nodeblock = sh.nodeblock && A->nodeblocked;
     const bool grouped = sh.nodeblock && !A->nodeblocked && A->grouped;
-    const int gs = nodeblock    ? std::max(1, std::min((A->nbr + 7) / 8, MAX_GRID))
+    const int gs = nodeblock    ? std::max(1, std::min((A->nbr + 7) / 8, GRP_MAX_GRID))
                    : grouped   ? std::max(1, std::min((A->ngrp + GRP_PER_BLOCK - 1) / GRP_PER_BLOCK, GRP_MAX_GRID))
                    : sh.stream ? std::max(1, std::min((n + sh.rpb - 1) / sh.rpb, MAX_GRID))
                                : spmv_grid(n, sh.lanes);
@@ -957,7 +957,7 @@ static int pcg_reliable(hippcg_solver *s, hipcsr_mtx *A, const double *db, doubl
     const int gv = vec_grid(n);
     const bool nodeblock = sh.nodeblock && A->nodeblocked;
     const bool grouped = sh.nodeblock && !A->nodeblocked && A->grouped;
-    const int gs = nodeblock    ? std::max(1, std::min((A->nbr + 7) / 8, MAX_GRID))
+    const int gs = nodeblock    ? std::max(1, std::min((A->nbr + 7) / 8, GRP_MAX_GRID))
                    : grouped   ? std::max(1, std::min((A->ngrp + GRP_PER_BLOCK - 1) / GRP_PER_BLOCK, GRP_MAX_GRID))
                    : sh.stream ? std::max(1, std::min((n + sh.rpb - 1) / sh.rpb, MAX_GRID))
                                : spmv_grid(n, sh.lanes);

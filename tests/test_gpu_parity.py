@@ -213,6 +213,53 @@ def test_tutorialmaterial_deck_on_gpu(ctx):
             assert np.abs(a - b).max() <= 1e-12 * max(np.abs(b).max(), 1e-300), (k, sl)
 
 
+def test_c_abi_refuses_bad_inputs(ctx):
+    """The boundary's error behaviour (OOFEM_ERROR in the reference): bad
+    indices, unknown materials, unflushed assembly and entries outside the
+    pattern are refused with an error before any kernel reads them, and the
+    objects stay usable afterwards."""
+    import copy
+
+    from oofem_amd.hipfem import BatchedEvaluator, HipCSRSparseMtrx, HipFemError, HipPCGSolver
+    d = meshgen.cantilever_hex(2, 2, 3, name="tiny")
+    ev = BatchedEvaluator(ctx)
+    for field, change, msg in (("conn", lambda a: a.__setitem__((0, 0), d.nnode), "conn"),
+                               ("emat", lambda a: a.__setitem__(0, len(d.mat)), "emat"),
+                               ("mat", lambda a: a.__setitem__((0, 0), 7), "unknown model")):
+        bad = copy.deepcopy(d)
+        arr = getattr(bad, field).copy()
+        change(arr)
+        setattr(bad, field, arr)
+        with pytest.raises(HipFemError, match=msg):
+            ev.set_mesh(bad)
+    ev.set_mesh(d)                                   # still usable
+    with pytest.raises(HipFemError, match="build_structure"):
+        ev.internal_forces(np.zeros(d.neq), np.zeros(d.npeq))   # no colouring yet
+    K0 = HipCSRSparseMtrx(ctx)
+    ev.build_structure(K0)
+    f, _ = ev.internal_forces(np.zeros(d.neq), np.zeros(d.npeq))[:2]
+    assert f.shape == (d.neq,) and not f.any()
+    ev.close()
+    K = HipCSRSparseMtrx(ctx)
+    K.buildInternalStructure(4, np.array([[1, 2, 3, 0], [3, 4, 0, 0]]))
+    with pytest.raises(HipFemError, match="not in the pattern"):
+        K.assemble(np.array([1, 4]), np.eye(2))      # (1,4): no element couples them
+    with pytest.raises(HipFemError, match="row 5"):
+        K.assemble(np.array([5]), np.eye(1))
+    K.zero()
+    K.assembleBegin()
+    K.assemble(np.array([1, 2, 3]), 4.0 * np.eye(3))
+    K.assemble(np.array([3, 4]), 2.0 * np.eye(2))
+    s = HipPCGSolver(ctx, lstol=1e-12, lsiter=50, lsprecond=1)
+    x = np.zeros(4)
+    with pytest.raises(HipFemError, match="assembleEnd"):
+        s.solve(K, np.ones(4), x)                    # contributions not flushed
+    K.assembleEnd()
+    reason, it, res = s.solve(K, np.ones(4), x)      # diagonal system 4, 4, 6, 2
+    assert reason == 1 and res <= 1e-12
+    np.testing.assert_allclose(x, [0.25, 0.25, 1.0 / 6.0, 0.5], rtol=1e-14)
+
+
 def _mises_deck(etype):
     if etype == meshgen.LTRSPACE:
         coords, conn = meshgen.kuhn_tets(3, 3, 4, 1.0, 1.0, 1.3, jitter=0.1, seed=5)

@@ -746,10 +746,10 @@ __global__ void k_fill_d(int n, double *__restrict__ a, const PcgState *__restri
 // sum_S c_S prod_{j in S} xi_j over the subsets S of {xi, eta, zeta}, with
 // 8 c_S = sum_a v_a prod_{j in S} s_aj: a Walsh-Hadamard transform of the 8
 // nodal values, done across the 8 lanes by three xor butterflies (lane S
-// ends with c_S).  The lanes share the coefficients of x (coordinates
-// relative to node 1, differenced in FP64) and u (relative to the lane-0
-// node, in FP32: a rigid translation adds nothing to the strain) through
-// LDS; J and du/dxi at a point are then 3 FMAs per entry.  The forces take
+// ends with c_S).  The lanes share the coefficients of x (formed per element
+// in FP64 and rounded once, by k_mf_erel) and of u (transformed here, in
+// FP32, relative to the lane-0 node: a rigid translation adds nothing to the
+// strain) through LDS; J and du/dxi at a point are then 3 FMAs per entry.  The forces take
 // the transposed route: each point's T = dV sigma J^-T / 8 gives 8 subset
 // terms F_S, whose sum over the points is a reduce-scatter butterfly (xor 4,
 // 2, 1: 12 + 6 + 3 exchanges, a fixed order) leaving F_g on lane g, and the
@@ -758,8 +758,7 @@ __global__ void k_fill_d(int n, double *__restrict__ a, const PcgState *__restri
 // element's 24 forces are stored (elems = NULL: every element, one launch)
 // for k_mf_gather to sum per node, or, with yslot, each force is stored in
 // the element color's slot vector (yslot[color * neq + equation]) for
-// k_mf_slot_sum.  Node coordinates come as per-element FP32 differences
-// (erel, from k_mf_erel).
+// k_mf_slot_sum.
 constexpr int MF_EPB = 32;
 
 // exchanges inside groups of 8 lanes with no address arithmetic: xor 1 and 2
@@ -778,8 +777,9 @@ __device__ __forceinline__ float lane_group_first(float v)
     return __int_as_float(__builtin_amdgcn_ds_swizzle(__float_as_int(v), 0x18));
 }
 // The element arithmetic of k_mf_hex for the 8 lanes of one element
-// (lane g: Gauss point g and node nl(g)).  v[0..2]: the lane node's
-// coordinates relative to node 1, v[3..5]: its operand values; sC: the
+// (lane g: Gauss point g and node nl(g)).  v[0..2]: the coefficient c_g of
+// the element's coordinate field (zero on lane 0), v[3..5]: the operand
+// values at the lane's node; sC: the
 // element's 8x8 LDS coefficient slots; c / m / Kb: the lane point's
 // tangent (c < 0: the general D of dgen at gp).  Every lane of the group
 // takes part, and the caller's block-wide barrier orders the LDS slots.
@@ -796,10 +796,12 @@ __device__ __forceinline__ void mf_hex_forces(int g, float v[6], float (*sC)[8],
     for (int mk = 4; mk >= 1; mk >>= 1) {
         const float sg = (g & mk) ? -1.f : 1.f;
 #pragma unroll
-        for (int k = 0; k < 6; k++) v[k] = fmaf(sg, lane_xor(v[k], mk), v[k]);
+        for (int k = 3; k < 6; k++) v[k] = fmaf(sg, lane_xor(v[k], mk), v[k]);
     }
 #pragma unroll
-    for (int k = 0; k < 6; k++) sC[g][k] = 0.125f * v[k];
+    for (int k = 0; k < 3; k++) sC[g][k] = v[k];   // the coordinate coefficients come transformed
+#pragma unroll
+    for (int k = 3; k < 6; k++) sC[g][k] = 0.125f * v[k];
     __syncthreads();
     fr[0] = fr[1] = fr[2] = 0.f;   // F_0 = 0
     {
@@ -945,7 +947,7 @@ __device__ __forceinline__ void mf_point_tangent(const float *__restrict__ desc,
     }
 }
 
-__global__ __launch_bounds__(256) void k_mf_hex(const int *__restrict__ elems, int nel,
+__global__ __launch_bounds__(256, 8) void k_mf_hex(const int *__restrict__ elems, int nel,
                                                 const float *__restrict__ erel, const int *__restrict__ conn,
                                                 const int *__restrict__ eqn, const int *__restrict__ emat,
                                                 const double *__restrict__ mat, const float *__restrict__ desc,
@@ -971,7 +973,7 @@ __global__ __launch_bounds__(256) void k_mf_hex(const int *__restrict__ elems, i
         const int node = conn[(size_t)e * 8 + nl];
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-            v[d] = nl ? erel[(size_t)e * 21 + 3 * (nl - 1) + d] : 0.0f;
+            v[d] = g ? erel[(size_t)e * 21 + 3 * (g - 1) + d] : 0.0f;   // lane g: c_g of x
             const int q = eqn[3 * (size_t)node + d];
             myq[d] = q;
             v[3 + d] = q > 0 ? x[q - 1] : 0.0f;
@@ -1004,19 +1006,28 @@ __global__ __launch_bounds__(256) void k_mf_hex(const int *__restrict__ elems, i
     }
 }
 
-// per-element coordinates of nodes 2..8 relative to node 1, differenced in
-// FP64 and rounded to FP32 (what k_mf_hex uses; built once per mesh)
+// per-element geometry of k_mf_hex, built once per mesh: the coefficients
+// c_S (S = 1..7) of the trilinear coordinate field, c_S = 1/8 sum_g x_g
+// prod_{j in S} s_j(g) over the lanes g (node nl(g), local signs s_j(g) =
+// +1 where bit j of g is set, bits (4, 2, 1) for xi, eta, zeta), formed in
+// FP64 from coordinates relative to node 1 and rounded to FP32 once
 __global__ void k_mf_erel(int nel, const double *__restrict__ coords, const int *__restrict__ conn,
                           float *__restrict__ erel)
 {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= nel) return;
     const int n0 = conn[(size_t)e * 8];
-    for (int a = 1; a < 8; a++) {
-        const int n = conn[(size_t)e * 8 + a];
-        for (int d = 0; d < 3; d++)
-            erel[(size_t)e * 21 + 3 * (a - 1) + d] = (float)(coords[3 * (size_t)n + d] - coords[3 * (size_t)n0 + d]);
+    double x[8][3];
+    for (int g = 0; g < 8; g++) {
+        const int n = conn[(size_t)e * 8 + ((0x26371504 >> (4 * g)) & 15)];
+        for (int d = 0; d < 3; d++) x[g][d] = coords[3 * (size_t)n + d] - coords[3 * (size_t)n0 + d];
     }
+    for (int S = 1; S < 8; S++)
+        for (int d = 0; d < 3; d++) {
+            double c = 0.0;
+            for (int g = 0; g < 8; g++) c += (__builtin_popcount(S & ~g) & 1) ? -x[g][d] : x[g][d];
+            erel[(size_t)e * 21 + 3 * (S - 1) + d] = (float)(0.125 * c);
+        }
 }
 
 // y[i] = sum over the colors touching equation i, ascending, of the FP32
@@ -1223,7 +1234,7 @@ struct hfem_model {
     uint8_t *d_ecolor = nullptr;
     uint16_t *d_eqmask = nullptr;                // colors touching every free equation (slot form)
     float *d_yslot = nullptr;                    // ncolors x neq FP32 slots (slot form)
-    float *d_erel = nullptr;                     // per element: nodes 2..8 relative to node 1, FP32 (21)
+    float *d_erel = nullptr;                     // per element: coordinate coefficients c_1..c_7 (21 FP32)
     std::vector<hipcsr_mtx *> linked;
     // MisesMat surrogate (hfem_model_set_surrogate)
     hfem_mlp *sur = nullptr;
@@ -1598,8 +1609,9 @@ extern "C" int hfem_model_assemble_stiffness(hfem_model *md, hipcsr_mtx *K, int 
             HIPFEM_CHECK(hipGetLastError());
         }
         if (!md->d_erel && md->nelem) {
-            // per-element FP32 relative coordinates (the differences k_mf_hex
-            // would otherwise form from the FP64 node coordinates each time)
+            // per-element coefficients of the trilinear coordinate field, in
+            // FP64 rounded once to FP32 (k_mf_hex would otherwise transform
+            // FP32 coordinates each time)
             HIPFEM_CHECK(hipMalloc(&md->d_erel, sizeof(float) * 21 * (size_t)md->nelem));
             k_mf_erel<<<(md->nelem + 255) / 256, 256, 0, st>>>(md->nelem, md->d_coords, md->d_conn, md->d_erel);
             HIPFEM_CHECK(hipGetLastError());

@@ -2,7 +2,8 @@
 
 Bars: integer data (patterns) bit-exact; FP64 values within a few ulps of
 the largest entry (tolerances written per test); solutions within 1e-10
-relative of the oracle's, the bar north_star sets for the plain FP64 path.
+relative of the oracle's, the plain FP64 path's bar against the reference
+solver.
 """
 import os
 

@@ -543,9 +543,9 @@ def spmv_roofline(args, deck, r):
         avg_ms = r["ms_spmv"] / r["n_spmv"]
         lay = r["layout"]
         if lay["form"] == 2:
-            # row-group SpMV: values, one int32 column and one int32 offset per
-            # chunk, group bounds (2 int32), row pointers, gather p, write q
-            bytes_per = 8.0 * r["nnz"] + 8.0 * lay["blocks"] + 8.0 * lay["groups"] + 4.0 * n + 16.0 * n
+            # row-group SpMV: values, one int32 column (with the chunk length)
+            # per chunk, group bounds (2 int32), row pointers, gather p, write q
+            bytes_per = 8.0 * r["nnz"] + 4.0 * lay["blocks"] + 8.0 * lay["groups"] + 4.0 * n + 16.0 * n
             kname = "k_pcg_spmv_p_grp<double, double>"
         elif lay["form"] == 1:
             bytes_per = 8.0 * r["nnz"] + 4.0 * r["nnz"] / 9.0 + 4.0 * n / 3.0 + 16.0 * n

@@ -179,7 +179,7 @@ __global__ __launch_bounds__(256) void k_spmv_nb(int nbr, const int32_t *__restr
 template <typename MT>
 __global__ __launch_bounds__(256) void k_spmv_grp(int ngrp, const int32_t *__restrict__ grow,
                                                   const int32_t *__restrict__ gch, const uint32_t *__restrict__ ccol,
-                                                  const int32_t *__restrict__ coff, const int32_t *__restrict__ rowptr,
+                                                  const int32_t *__restrict__ rowptr,
                                                   const MT *__restrict__ val, const double *__restrict__ x,
                                                   double *__restrict__ y)
 {
@@ -187,7 +187,7 @@ __global__ __launch_bounds__(256) void k_spmv_grp(int ngrp, const int32_t *__res
     for (int g = blockIdx.x * GRP_PER_BLOCK + threadIdx.x / GRP_LANES; g < ngrp; g += gridDim.x * GRP_PER_BLOCK) {
         const int r0 = grow[g], nr = grow[g + 1] - r0, c0 = gch[g], nch = gch[g + 1] - c0;
         double acc[3];
-        grp_row<double>(r0, nr, c0, nch, lane, rowptr, ccol, coff, val, [&](int c) { return x[c]; }, acc);
+        grp_row<double>(r0, nr, c0, nch, lane, rowptr, ccol, val, [&](int c) { return x[c]; }, acc);
         if (lane == 0)
             for (int r = 0; r < nr; r++) y[r0 + r] = acc[r];
     }
@@ -204,7 +204,7 @@ static int launch_spmv(const hipcsr_mtx *m, const SpmvShape &sh, hipStream_t st,
         ok = true;
     } else if (sh.nodeblock && m->grouped) {
         const int grid = std::max(1, std::min((m->ngrp + GRP_PER_BLOCK - 1) / GRP_PER_BLOCK, 1 << 20));
-        k_spmv_grp<VT><<<grid, 256, 0, st>>>(m->ngrp, m->d_grow, m->d_gch, m->d_ccol, m->d_coff, rp, v, x, y);
+        k_spmv_grp<VT><<<grid, 256, 0, st>>>(m->ngrp, m->d_grow, m->d_gch, m->d_ccol, rp, v, x, y);
         ok = true;
     } else if (sh.stream) {
         const int grid = (int)std::max<int64_t>(1, std::min<int64_t>((n + sh.rpb - 1) / sh.rpb, 1 << 20));
@@ -234,11 +234,11 @@ int csr_free(hipcsr_mtx *m)
     for (void *p : {(void *)m->d_rowptr, (void *)m->d_colind, (void *)m->d_diag, (void *)m->d_val,
                     (void *)m->d_val32, (void *)m->d_tptr, (void *)m->d_tcol, (void *)m->d_tmap,
                     (void *)m->d_bptr, (void *)m->d_bcol, (void *)m->d_grow, (void *)m->d_gch,
-                    (void *)m->d_ccol, (void *)m->d_coff})
+                    (void *)m->d_ccol})
         if (p) HIPFEM_CHECK(hipFree(p));
     m->d_rowptr = m->d_colind = m->d_diag = m->d_tptr = m->d_tcol = m->d_tmap = nullptr;
     m->d_bptr = m->d_bcol = nullptr;
-    m->d_grow = m->d_gch = m->d_coff = nullptr;
+    m->d_grow = m->d_gch = nullptr;
     m->d_ccol = nullptr;
     m->grouped = false;
     m->ngrp = 0;
@@ -281,14 +281,12 @@ static bool node_blocked(int n, const std::vector<int32_t> &rp, const std::vecto
 // consecutive columns.  Worth using when groups and chunks are mostly full
 // (FEM patterns with some partially constrained nodes).
 static bool row_groups(int n, const std::vector<int32_t> &rp, const std::vector<int32_t> &ci,
-                       std::vector<int32_t> &grow, std::vector<int32_t> &gch, std::vector<uint32_t> &ccol,
-                       std::vector<int32_t> &coff)
+                       std::vector<int32_t> &grow, std::vector<int32_t> &gch, std::vector<uint32_t> &ccol)
 {
     if (n == 0 || n >= (int)GRP_COL_MASK) return false;
     grow.assign(1, 0);
     gch.assign(1, 0);
     ccol.clear();
-    coff.clear();
     int r = 0;
     int64_t group_cols = 0;
     while (r < n) {
@@ -303,7 +301,6 @@ static bool row_groups(int n, const std::vector<int32_t> &rp, const std::vector<
             int L = 1;
             while (L < 3 && i + L < len && c[i + L] == c[i] + L) L++;
             ccol.push_back((uint32_t)c[i] | ((uint32_t)(L - 1) << 30));
-            coff.push_back(i);
             i += L;
         }
         r += nr;
@@ -334,9 +331,9 @@ int csr_alloc(hipcsr_mtx *m, int n, const std::vector<int32_t> &rowptr, const st
     std::vector<int32_t> bptr, bcol;
     m->nodeblocked = node_blocked(n, rowptr, colind, bptr, bcol);
     m->nbr = m->nodeblocked ? n / 3 : 0;
-    std::vector<int32_t> grow, gch, coff;
+    std::vector<int32_t> grow, gch;
     std::vector<uint32_t> ccol;
-    m->grouped = !m->nodeblocked && row_groups(n, rowptr, colind, grow, gch, ccol, coff);
+    m->grouped = !m->nodeblocked && row_groups(n, rowptr, colind, grow, gch, ccol);
     m->ngrp = m->grouped ? (int)grow.size() - 1 : 0;
     m->shape.nodeblock = (m->nodeblocked || m->grouped) ? 1 : 0;
     HIPFEM_CHECK(hipSetDevice(m->ctx->device));
@@ -344,13 +341,11 @@ int csr_alloc(hipcsr_mtx *m, int n, const std::vector<int32_t> &rowptr, const st
         HIPFEM_CHECK(hipMalloc(&m->d_grow, sizeof(int32_t) * grow.size()));
         HIPFEM_CHECK(hipMalloc(&m->d_gch, sizeof(int32_t) * gch.size()));
         HIPFEM_CHECK(hipMalloc(&m->d_ccol, sizeof(uint32_t) * std::max<size_t>(1, ccol.size())));
-        HIPFEM_CHECK(hipMalloc(&m->d_coff, sizeof(int32_t) * std::max<size_t>(1, coff.size())));
         hipStream_t s0 = m->ctx->stream;
         HIPFEM_CHECK(hipMemcpyAsync(m->d_grow, grow.data(), sizeof(int32_t) * grow.size(), hipMemcpyHostToDevice, s0));
         HIPFEM_CHECK(hipMemcpyAsync(m->d_gch, gch.data(), sizeof(int32_t) * gch.size(), hipMemcpyHostToDevice, s0));
         if (!ccol.empty()) {
             HIPFEM_CHECK(hipMemcpyAsync(m->d_ccol, ccol.data(), sizeof(uint32_t) * ccol.size(), hipMemcpyHostToDevice, s0));
-            HIPFEM_CHECK(hipMemcpyAsync(m->d_coff, coff.data(), sizeof(int32_t) * coff.size(), hipMemcpyHostToDevice, s0));
         }
         // the host vectors die with this scope: finish the copies first
         HIPFEM_CHECK(hipStreamSynchronize(s0));

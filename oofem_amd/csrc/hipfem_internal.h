@@ -166,7 +166,7 @@ struct hipcsr_mtx {
     // row-group view (patterns with partially constrained nodes), see grp_row
     bool grouped = false;
     int ngrp = 0;
-    int32_t *d_grow = nullptr, *d_gch = nullptr, *d_coff = nullptr;
+    int32_t *d_grow = nullptr, *d_gch = nullptr;
     uint32_t *d_ccol = nullptr;
     // element-by-element view: set when the values were assembled by a batched
     // evaluator and not changed since (mf_version == version); the FP32 PCG

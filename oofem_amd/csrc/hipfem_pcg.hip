@@ -266,7 +266,6 @@ template <typename VT, typename MT, bool FP>
 __global__ __launch_bounds__(256) void k_pcg_spmv_p_grp(int ngrp, const int32_t *__restrict__ grow,
                                                         const int32_t *__restrict__ gch,
                                                         const uint32_t *__restrict__ ccol,
-                                                        const int32_t *__restrict__ coff,
                                                         const int32_t *__restrict__ rowptr, const MT *__restrict__ val,
                                                         const VT *__restrict__ p, const VT *__restrict__ z,
                                                         VT *__restrict__ pnew, VT *__restrict__ q,
@@ -287,7 +286,7 @@ __global__ __launch_bounds__(256) void k_pcg_spmv_p_grp(int ngrp, const int32_t 
     for (int g = blockIdx.x * GRP_PER_BLOCK + threadIdx.x / GRP_LANES; g < ngrp; g += gridDim.x * GRP_PER_BLOCK) {
         const int r0 = grow[g], nr = grow[g + 1] - r0, c0 = gch[g], nch = gch[g + 1] - c0;
         VT a[3];
-        grp_row<VT>(r0, nr, c0, nch, lane, rowptr, ccol, coff, val, pv, a);
+        grp_row<VT>(r0, nr, c0, nch, lane, rowptr, ccol, val, pv, a);
         if (lane == 0)
             for (int r = 0; r < nr; r++) {
                 const VT pi = pv(r0 + r);
@@ -891,11 +890,11 @@ static int pcg_run(hippcg_solver *s, const PcgRunBufs &bufs, hipcsr_mtx *A, cons
             } else if (grouped) {
                 if (fused)
                     k_pcg_spmv_p_grp<VT, MT, true><<<gs, 256, 0, st>>>(A->ngrp, A->d_grow, A->d_gch, A->d_ccol,
-                                                                       A->d_coff, A->d_rowptr, val, pin, z, pnew, q,
+                                                                       A->d_rowptr, val, pin, z, pnew, q,
                                                                        pst, part, first);
                 else
                     k_pcg_spmv_p_grp<VT, MT, false><<<gs, 256, 0, st>>>(A->ngrp, A->d_grow, A->d_gch, A->d_ccol,
-                                                                        A->d_coff, A->d_rowptr, val, pin, z, nullptr,
+                                                                        A->d_rowptr, val, pin, z, nullptr,
                                                                         q, pst, part, first);
             } else if (fused) {
                 launch_pcg_spmv_fused<VT, MT>(sh, gs, st, n, A->d_rowptr, A->d_colind, val, z, pold, pnew, q, pst,
@@ -1015,7 +1014,7 @@ static int pcg_reliable(hippcg_solver *s, hipcsr_mtx *A, const double *db, doubl
                                                                          0);
             } else if (grouped) {
                 k_pcg_spmv_p_grp<float, float, false><<<gs, 256, 0, st>>>(A->ngrp, A->d_grow, A->d_gch, A->d_ccol,
-                                                                          A->d_coff, A->d_rowptr, A->d_val32, s->fp0,
+                                                                          A->d_rowptr, A->d_val32, s->fp0,
                                                                           nullptr, nullptr, s->fq, s->st, part, 0);
             } else {
                 launch_pcg_spmv_p<float, float>(sh, gs, st, n, A->d_rowptr, A->d_colind, A->d_val32, s->fp0, s->fq,

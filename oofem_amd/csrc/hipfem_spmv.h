@@ -100,28 +100,52 @@ __device__ __forceinline__ void nb_row(int bo, int nbk, int vbase, int lane, con
 // Row-group form, for patterns with partially constrained nodes: rows
 // [r0, r0 + nr) (nr <= 3) share one column list, which is cut into chunks of
 // up to 3 consecutive columns.  Chunk k of the group starts at column
-// ccol & GRP_COL_MASK, holds (ccol >> 30) + 1 columns and sits at offset coff
-// inside every row of the group.  Lane k of a GRP_LANES-lane group takes
-// chunks k, k + GRP_LANES, ...; for a fully free node row this is the
-// node-block access pattern with one more int32 per chunk.  16 lanes (four
-// groups per wave, up to two chunks per lane for a hexahedral node row of 27
-// chunks) measured 11 % faster on gfx950 than 32 lanes (one chunk per lane)
-// and than 8 lanes, for FP64 and FP32 values alike.
+// ccol & GRP_COL_MASK and holds (ccol >> 30) + 1 columns; its offset inside
+// every row of the group is the sum of the lengths of the chunks before it,
+// which the group's lanes form by a prefix sum over the 16 lanes of a DPP
+// row (four row shifts, then the row total from lane 15 as the carry into
+// the next 16 chunks), so no offset is stored.  Lane k of a GRP_LANES-lane
+// group takes chunks k, k + GRP_LANES, ...; for a fully free node row this
+// is the node-block access pattern.  16 lanes (four groups per wave, up to
+// two chunks per lane for a hexahedral node row of 27 chunks) measured 11 %
+// faster on gfx950 than 32 lanes (one chunk per lane) and than 8 lanes, for
+// FP64 and FP32 values alike.
 constexpr uint32_t GRP_COL_MASK = 0x3FFFFFFFu;
 constexpr int GRP_LANES = 16;
 constexpr int GRP_PER_BLOCK = 256 / GRP_LANES;   // groups per 256-thread block
 
+// inclusive prefix sum of v over the 16 lanes of a DPP row (lanes without a
+// source in the row add 0); every lane of the row must be active
+__device__ __forceinline__ int row16_inclusive_sum(int v)
+{
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xF, 0xF, false);   // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xF, 0xF, false);   // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xF, 0xF, false);   // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xF, 0xF, false);   // row_shr:8
+    return v;
+}
+
 template <typename AT, typename MT, typename G>
 __device__ __forceinline__ void grp_row(int r0, int nr, int c0, int nch, int lane, const int32_t *__restrict__ rowptr,
-                                        const uint32_t *__restrict__ ccol, const int32_t *__restrict__ coff,
-                                        const MT *__restrict__ val, G gather, AT acc[3])
+                                        const uint32_t *__restrict__ ccol, const MT *__restrict__ val, G gather,
+                                        AT acc[3])
 {
+    static_assert(GRP_LANES == 16, "the chunk offsets are summed over one 16-lane DPP row");
     acc[0] = acc[1] = acc[2] = AT(0);
     const int rp0 = rowptr[r0], rp1 = nr > 1 ? rowptr[r0 + 1] : 0, rp2 = nr > 2 ? rowptr[r0 + 2] : 0;
-    for (int k = lane; k < nch; k += GRP_LANES) {
-        const uint32_t cc = ccol[c0 + k];
-        const int col = (int)(cc & GRP_COL_MASK), L = (int)(cc >> 30) + 1;
-        const int off = coff[c0 + k];
+    int carry = 0;   // columns in the chunks of the earlier rounds
+    // the round count is the same for the group's 16 lanes, so all of them
+    // take part in the row sums
+    for (int base = 0; base < nch; base += GRP_LANES) {
+        const int k = base + lane;
+        const bool valid = k < nch;
+        const uint32_t cc = valid ? ccol[c0 + k] : 0u;
+        const int L = valid ? (int)(cc >> 30) + 1 : 0;
+        const int incl = row16_inclusive_sum(L);
+        const int off = carry + incl - L;
+        carry += __shfl(incl, GRP_LANES - 1, GRP_LANES);
+        if (!valid) continue;
+        const int col = (int)(cc & GRP_COL_MASK);
         if (nr == 3 && L == 3) {
             // full 3x3 block (the common case): nine independent loads
             const AT x0 = gather(col), x1 = gather(col + 1), x2 = gather(col + 2);

@@ -949,7 +949,8 @@ __device__ __forceinline__ void mf_point_tangent(const float *__restrict__ desc,
 
 __global__ __launch_bounds__(256, 8) void k_mf_hex(const int *__restrict__ elems, int nel,
                                                 const float *__restrict__ erel, const int *__restrict__ conn,
-                                                const int *__restrict__ eqn, const int *__restrict__ emat,
+                                                const int *__restrict__ eqn, const uint32_t *__restrict__ eloc,
+                                                const int *__restrict__ emat,
                                                 const double *__restrict__ mat, const float *__restrict__ desc,
                                                 const float *__restrict__ dgen, int ngp_total,
                                                 const float *__restrict__ x, double *__restrict__ y,
@@ -967,17 +968,25 @@ __global__ __launch_bounds__(256, 8) void k_mf_hex(const int *__restrict__ elems
     const int ei = active ? ei0 : nel - 1;
     const int nl = (0x26371504 >> (4 * g)) & 15;   // this lane's LSpace node (0-based)
     const int e = elems ? elems[ei] : ei;
-    int myq[3];   // equations of node nl
+    int myq[3];   // equations of node nl (> 0 free)
     float v[6];
-    {
+    if (eloc) {
+        // one word per lane: the node's first free equation and its free
+        // dofs, numbered consecutively (no dependent load of eqn)
+        const uint32_t w = eloc[(size_t)e * 8 + g];
+        const int base = (int)(w & 0x1FFFFFFFu) + 1;
+        const uint32_t fm = w >> 29;
+#pragma unroll
+        for (int d = 0; d < 3; d++) myq[d] = (fm >> d & 1u) ? base + __builtin_popcount(fm & ((1u << d) - 1u)) : 0;
+    } else {
         const int node = conn[(size_t)e * 8 + nl];
 #pragma unroll
-        for (int d = 0; d < 3; d++) {
-            v[d] = g ? erel[(size_t)e * 21 + 3 * (g - 1) + d] : 0.0f;   // lane g: c_g of x
-            const int q = eqn[3 * (size_t)node + d];
-            myq[d] = q;
-            v[3 + d] = q > 0 ? x[q - 1] : 0.0f;
-        }
+        for (int d = 0; d < 3; d++) myq[d] = eqn[3 * (size_t)node + d];
+    }
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        v[d] = g ? erel[(size_t)e * 21 + 3 * (g - 1) + d] : 0.0f;   // lane g: c_g of x
+        v[3 + d] = myq[d] > 0 ? x[myq[d] - 1] : 0.0f;
     }
     const size_t gp = (size_t)e * 8 + g;
     float c, m[6];
@@ -1235,6 +1244,11 @@ struct hfem_model {
     uint16_t *d_eqmask = nullptr;                // colors touching every free equation (slot form)
     float *d_yslot = nullptr;                    // ncolors x neq FP32 slots (slot form)
     float *d_erel = nullptr;                     // per element: coordinate coefficients c_1..c_7 (21 FP32)
+    // per element and k_mf_hex lane g: first free equation (0-based, bits
+    // 0..28) and free-dof mask (bits 29..31) of the lane's node; built once
+    // per mesh when every node numbers its free dofs consecutively
+    uint32_t *d_eloc = nullptr;
+    bool eloc_checked = false;
     std::vector<hipcsr_mtx *> linked;
     // MisesMat surrogate (hfem_model_set_surrogate)
     hfem_mlp *sur = nullptr;
@@ -1266,7 +1280,7 @@ static int model_free(hfem_model *md)
     void *ps[] = {md->d_coords, md->d_mat, md->d_state, md->d_conn, md->d_emat, md->d_eqn, md->d_bad,
                   md->d_eoff,   md->d_colors, md->d_uf, md->d_up, md->d_ff, md->d_fp, md->d_out3, md->d_dofid,
                   md->d_tdesc, md->d_nptr, md->d_nadj, md->d_ebuf, md->d_ecolor, md->d_eqmask, md->d_yslot,
-                  md->d_erel};
+                  md->d_erel, md->d_eloc};
     for (void *p : ps)
         if (p) HIPFEM_CHECK(hipFree(p));
     md->d_tdesc = nullptr;
@@ -1276,6 +1290,8 @@ static int model_free(hfem_model *md)
     md->d_eqmask = nullptr;
     md->d_yslot = nullptr;
     md->d_erel = nullptr;
+    md->d_eloc = nullptr;
+    md->eloc_checked = false;
     md->d_coords = md->d_mat = md->d_state = md->d_uf = md->d_up = md->d_ff = md->d_fp = md->d_out3 = nullptr;
     md->d_conn = md->d_emat = md->d_eqn = md->d_bad = md->d_colors = nullptr;
     md->d_dofid = nullptr;
@@ -1616,6 +1632,37 @@ extern "C" int hfem_model_assemble_stiffness(hfem_model *md, hipcsr_mtx *K, int 
             k_mf_erel<<<(md->nelem + 255) / 256, 256, 0, st>>>(md->nelem, md->d_coords, md->d_conn, md->d_erel);
             HIPFEM_CHECK(hipGetLastError());
         }
+        if (!md->eloc_checked && md->nelem) {
+            // k_mf_hex's equation words, where each node's free dofs carry
+            // consecutive equation numbers in dof order (the node-by-node
+            // numbering); otherwise the kernel reads conn and eqn
+            md->eloc_checked = true;
+            bool ok = md->neq < (1 << 29);
+            std::vector<uint32_t> nodew(ok ? (size_t)md->nnode : 0);
+            for (int n = 0; ok && n < md->nnode; n++) {
+                uint32_t fm = 0;
+                int first = 0, k = 0;
+                for (int d = 0; d < 3 && ok; d++) {
+                    const int q = md->h_eqn[3 * (size_t)n + d];
+                    if (q <= 0) continue;
+                    if (!fm)
+                        first = q - 1;
+                    else if (q - 1 != first + k)
+                        ok = false;
+                    fm |= 1u << d;
+                    k++;
+                }
+                nodew[n] = (uint32_t)first | fm << 29;
+            }
+            if (ok) {
+                std::vector<uint32_t> eloc((size_t)md->nelem * 8);
+                for (size_t e = 0; e < (size_t)md->nelem; e++)
+                    for (int g = 0; g < 8; g++)
+                        eloc[e * 8 + g] = nodew[md->h_conn[e * 8 + ((0x26371504 >> (4 * g)) & 15)]];
+                HIPFEM_CHECK(hipMalloc(&md->d_eloc, sizeof(uint32_t) * eloc.size()));
+                HIPFEM_CHECK(hipMemcpy(md->d_eloc, eloc.data(), sizeof(uint32_t) * eloc.size(), hipMemcpyHostToDevice));
+            }
+        }
         if (!md->d_ecolor && md->ncolors <= 16 && md->nelem && md->neq) {
             // slot form: element colors and the colors touching each free equation
             std::vector<uint16_t> eqmask((size_t)md->neq, 0);
@@ -1677,9 +1724,9 @@ static int mf_slot_pass(hfem_model *md, const float *x, const PcgState *st)
     if (!md->nelem) return 0;
     const float *desc = md->tdesc_used ? md->d_tdesc : nullptr;
     k_mf_hex<<<(md->nelem + MF_EPB - 1) / MF_EPB, 256, 0, s>>>(nullptr, md->nelem, md->d_erel, md->d_conn, md->d_eqn,
-                                                               md->d_emat, md->d_mat, desc, md->d_dsnap, md->ngp_total,
-                                                               x, nullptr, nullptr, md->d_yslot, md->d_ecolor, md->neq,
-                                                               st);
+                                                               md->d_eloc, md->d_emat, md->d_mat, desc, md->d_dsnap,
+                                                               md->ngp_total, x, nullptr, nullptr, md->d_yslot,
+                                                               md->d_ecolor, md->neq, st);
     HIPFEM_CHECK(hipGetLastError());
     return 0;
 }
@@ -1718,7 +1765,7 @@ int model_mf_apply_f32(hfem_model *md, const float *x, double *y, const PcgState
         // gather form: every element in one launch, then one FP64 sum per node
         if (md->nelem)
             k_mf_hex<<<(md->nelem + MF_EPB - 1) / MF_EPB, 256, 0, s>>>(
-                nullptr, md->nelem, md->d_erel, md->d_conn, md->d_eqn, md->d_emat, md->d_mat,
+                nullptr, md->nelem, md->d_erel, md->d_conn, md->d_eqn, md->d_eloc, md->d_emat, md->d_mat,
                 md->tdesc_used ? md->d_tdesc : nullptr, md->d_dsnap, md->ngp_total, x, y, md->d_ebuf, nullptr, nullptr,
                 0, st);
         k_mf_gather<<<std::min(8192, (md->nnode + 255) / 256), 256, 0, s>>>(md->nnode, md->d_nptr, md->d_nadj,
@@ -1731,8 +1778,8 @@ int model_mf_apply_f32(hfem_model *md, const float *x, double *y, const PcgState
         const int n = md->color_ptr[c + 1] - md->color_ptr[c];
         if (!n) continue;
         k_mf_hex<<<(n + MF_EPB - 1) / MF_EPB, 256, 0, s>>>(md->d_colors + md->color_ptr[c], n, md->d_erel,
-                                                           md->d_conn, md->d_eqn, md->d_emat, md->d_mat,
-                                                           md->tdesc_used ? md->d_tdesc : nullptr, md->d_dsnap,
+                                                           md->d_conn, md->d_eqn, md->d_eloc, md->d_emat,
+                                                           md->d_mat, md->tdesc_used ? md->d_tdesc : nullptr, md->d_dsnap,
                                                            md->ngp_total, x, y, nullptr, nullptr, nullptr, 0, st);
     }
     HIPFEM_CHECK(hipGetLastError());

@@ -477,6 +477,56 @@ def test_element_operator_matches_assembled_matrix(ctx, plastic, form):
     assert not K.is_matrix_free()
 
 
+@pytest.mark.parametrize("plastic", [False, True])
+def test_element_operator_equation_lookup_forms_agree(ctx, plastic):
+    """The element operator finds a lane's equations in one stored word when
+    every node numbers its free dofs consecutively (node-by-node numbering),
+    and through the connectivity and the equation table otherwise.  The same
+    mesh and state under both numberings (the second a random permutation of
+    the free equations) gives the same element forces, so the same products
+    bit for bit once permuted back, in all three operator forms."""
+    import copy
+
+    from oofem_amd.hipfem import BatchedEvaluator, HipCSRSparseMtrx
+    from tests import nlcases
+    d = meshgen.cantilever_hex(5, 4, 7, jitter=0.2, e_sigma=0.5, seed=9)
+    if plastic:
+        d.mat = nlcases.MISES.copy()
+        d.emat[:] = 0
+    rng = np.random.default_rng(2)
+    perm = rng.permutation(d.neq)          # old equation k -> new perm[k]
+    d2 = copy.deepcopy(d)
+    free = d2.eqn > 0
+    d2.eqn[free] = perm[d2.eqn[free] - 1] + 1
+    assert any((np.diff(r[r > 0]) != 1).any() for r in d2.eqn)   # the table lookup is taken
+    u = rng.standard_normal(d.neq) * (3e-3 if plastic else 1e-3)
+    xs = [rng.standard_normal(d.neq) for _ in range(2)]
+    out = []
+    for deck, to_new in ((d, np.arange(d.neq)), (d2, perm)):
+        ev = BatchedEvaluator(ctx, deck)
+        K = HipCSRSparseMtrx(ctx)
+        ev.build_structure(K)
+        un = np.empty(d.neq)
+        un[to_new] = u
+        ev.internal_forces(un, np.zeros(deck.npeq), update_state=True)
+        ev.assemble_stiffness(K, tangent=plastic)
+        ys = []
+        for form in (0, 1, 2):
+            ev.set_element_operator(form)
+            assert K.is_matrix_free()
+            for x in xs:
+                xn = np.empty(d.neq)
+                xn[to_new] = x
+                ys.append(K.times_matrix_free(xn)[to_new])
+        out.append(ys)
+        K.close()
+        ev.close()
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b)
+    # and the products are the operator's, not zeros
+    assert np.linalg.norm(out[0][0]) > 0
+
+
 @pytest.mark.parametrize("precision", [0, 2])
 def test_pcg_multiple_right_hand_sides(ctx, precision):
     """solve(A, B, X) with eight columns, one of them zero, on config 0: every

@@ -1084,6 +1084,11 @@ __global__ void k_fill(int64_t n, double v, double *__restrict__ a)
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) a[i] = v;
 }
 
+__global__ void k_zero_f32(int64_t n, float *__restrict__ a)
+{
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) a[i] = 0.f;
+}
+
 __global__ __launch_bounds__(1024) void k_reduce3(const double *__restrict__ partials, int np, double *out)
 {
     __shared__ double sh[16];
@@ -1674,6 +1679,11 @@ extern "C" int hfem_model_assemble_stiffness(hfem_model *md, hipcsr_mtx *K, int 
             HIPFEM_CHECK(hipMalloc(&md->d_ecolor, (size_t)md->nelem));
             HIPFEM_CHECK(hipMalloc(&md->d_eqmask, sizeof(uint16_t) * (size_t)md->neq));
             HIPFEM_CHECK(hipMalloc(&md->d_yslot, sizeof(float) * (size_t)md->ncolors * md->neq));
+            // the slot of a color that touches no element at an equation is
+            // never written and stays zero, so a sum may read every slot
+            const int64_t ns = (int64_t)md->ncolors * md->neq;
+            k_zero_f32<<<(int)std::min<int64_t>((ns + 255) / 256, 8192), 256, 0, st>>>(ns, md->d_yslot);
+            HIPFEM_CHECK(hipGetLastError());
             HIPFEM_CHECK(hipMemcpy(md->d_ecolor, md->h_ecolor.data(), (size_t)md->nelem, hipMemcpyHostToDevice));
             HIPFEM_CHECK(hipMemcpy(md->d_eqmask, eqmask.data(), sizeof(uint16_t) * eqmask.size(), hipMemcpyHostToDevice));
         }

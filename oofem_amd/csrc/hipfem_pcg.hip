@@ -572,7 +572,11 @@ __global__ __launch_bounds__(256) void k_dot_pq(int n, const float *__restrict__
 // the element operator's slot sum fused with k_dot_pq: q32 = (float) of the
 // FP64 sum over the colours touching each equation (ascending) of the FP32
 // slot values, and the partials of p.q in that FP64 sum; the same grid and
-// order as k_mf_slot_sum followed by k_dot_pq, so the same bits
+// order as k_mf_slot_sum followed by k_dot_pq, so the same bits.  NC > 0:
+// exactly NC colours, every slot read at once (the slots of colours that
+// touch no element at an equation hold zero, and adding +0 to a sum that
+// starts at +0 changes no bit)
+template <int NC>
 __global__ __launch_bounds__(256) void k_slot_sum_dot(int n, int ncolors, const uint16_t *__restrict__ eqmask,
                                                       const float *__restrict__ yslot, const float *__restrict__ p,
                                                       float *__restrict__ q, const PcgState *__restrict__ st,
@@ -582,10 +586,18 @@ __global__ __launch_bounds__(256) void k_slot_sum_dot(int n, int ncolors, const 
     if (halted(st)) return;
     double acc = 0.0;
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
-        const unsigned m = eqmask[i];
         double y = 0.0;
-        for (int c = 0; c < ncolors; c++)
-            if (m >> c & 1u) y += (double)yslot[(size_t)c * n + i];
+        if constexpr (NC > 0) {
+            float v[NC];
+#pragma unroll
+            for (int c = 0; c < NC; c++) v[c] = yslot[(size_t)c * n + i];
+#pragma unroll
+            for (int c = 0; c < NC; c++) y += (double)v[c];
+        } else {
+            const unsigned m = eqmask[i];
+            for (int c = 0; c < ncolors; c++)
+                if (m >> c & 1u) y += (double)yslot[(size_t)c * n + i];
+        }
         q[i] = (float)y;
         acc += (double)p[i] * y;
     }
@@ -1002,7 +1014,10 @@ static int pcg_reliable(hippcg_solver *s, hipcsr_mtx *A, const double *db, doubl
                 rc = model_mf_slots_f32(A->mf_model, s->fp0, s->st, &ys, &em, &nc);
                 if (rc) return rc;
                 if (ys) {
-                    k_slot_sum_dot<<<gv, 256, 0, st>>>(n, nc, em, ys, s->fp0, s->fq, s->st, part);
+                    if (nc == 8)
+                        k_slot_sum_dot<8><<<gv, 256, 0, st>>>(n, nc, em, ys, s->fp0, s->fq, s->st, part);
+                    else
+                        k_slot_sum_dot<0><<<gv, 256, 0, st>>>(n, nc, em, ys, s->fp0, s->fq, s->st, part);
                 } else {
                     rc = model_mf_apply_f32(A->mf_model, s->fp0, s->q, s->st);
                     if (rc) return rc;

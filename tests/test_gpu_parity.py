@@ -477,6 +477,28 @@ def test_element_operator_matches_assembled_matrix(ctx, plastic, form):
     assert not K.is_matrix_free()
 
 
+def test_mixed_precision_operator_forms_same_bits(ctx):
+    """Inside the mixed PCG the slot form sums each equation's colour slots
+    in one kernel together with the p.q partials (on an eight-colour mesh
+    every slot is read at once: the slots of colours with no element at an
+    equation hold zero); the per-colour form adds each colour's launch into
+    the FP64 product and then forms p.q.  They are the same sums in the same
+    order, so the solves take the same iterations to the same bits."""
+    from oofem_amd.drivers import LinearStaticProblem
+    d = meshgen.cantilever_hex(8, 8, 24, jitter=0.2, e_sigma=0.5, seed=3)
+    out = []
+    for form in (2, 0):
+        P = LinearStaticProblem(ctx, d, lstol=1e-10, precision=2)
+        P.K.set_matrix_free(True)
+        P.ev.set_element_operator(form)
+        assert P.ev.ncolors() == 8
+        got = P.solve()
+        assert P.solver.last_matrix_free()
+        out.append(got)
+    assert out[0]["iters"] == out[1]["iters"] and out[0]["outer"] == out[1]["outer"]
+    assert np.array_equal(out[0]["u"], out[1]["u"])
+
+
 @pytest.mark.parametrize("plastic", [False, True])
 def test_element_operator_equation_lookup_forms_agree(ctx, plastic):
     """The element operator finds a lane's equations in one stored word when

@@ -924,29 +924,15 @@ __device__ __forceinline__ void mf_hex_forces(int g, float v[6], float (*sC)[8],
     }
 }
 
-// the tangent of Gauss point gp: the descriptor (c, m0, m1, m3, m4, m5;
-// m2 = -(m0 + m1)) or, without one, the elastic D of the material table
-__device__ __forceinline__ void mf_point_tangent(const float *__restrict__ desc, size_t gp, float E, float nu,
-                                                 float &c, float m[6])
-{
-    if (desc) {
-        const float2 d01 = *reinterpret_cast<const float2 *>(desc + 6 * gp);
-        const float2 d23 = *reinterpret_cast<const float2 *>(desc + 6 * gp + 2);
-        const float2 d45 = *reinterpret_cast<const float2 *>(desc + 6 * gp + 4);
-        c = d01.x;
-        m[0] = d01.y;
-        m[1] = d23.x;
-        m[2] = -(d01.y + d23.x);
-        m[3] = d23.y;
-        m[4] = d45.x;
-        m[5] = d45.y;
-    } else {
-        c = E * __builtin_amdgcn_rcpf(1.f + nu);
-#pragma unroll
-        for (int i = 0; i < 6; i++) m[i] = 0.f;
-    }
-}
-
+// K_e x_e over the elements (or over elems[0..nel) of one color).  EL: the
+// lanes' equations come from the per-lane words of eloc (first free
+// equation and free-dof mask of the node), else from conn and eqn.  DESC:
+// per-point tangent descriptors (c, m0, m1, m3, m4, m5; m2 = -(m0 + m1)),
+// else the elastic D of the material table.  Every load is issued without a
+// lane-dependent branch, those that need only the element first, so one
+// wait covers them; x has at least one entry (the host launches nothing
+// when there are no free equations).
+template <bool EL, bool DESC>
 __global__ __launch_bounds__(256, 8) void k_mf_hex(const int *__restrict__ elems, int nel,
                                                 const float *__restrict__ erel, const int *__restrict__ conn,
                                                 const int *__restrict__ eqn, const uint32_t *__restrict__ eloc,
@@ -968,12 +954,23 @@ __global__ __launch_bounds__(256, 8) void k_mf_hex(const int *__restrict__ elems
     const int ei = active ? ei0 : nel - 1;
     const int nl = (0x26371504 >> (4 * g)) & 15;   // this lane's LSpace node (0-based)
     const int e = elems ? elems[ei] : ei;
+    const size_t gp = (size_t)e * 8 + g;
+    // loads that need only the element
+    const int me = emat[e];
+    float cx[3];   // lane g > 0: c_g of x (lane 0 reads c_1 and drops it)
+#pragma unroll
+    for (int d = 0; d < 3; d++) cx[d] = erel[(size_t)e * 21 + 3 * (g ? g - 1 : 0) + d];
+    float2 d01, d23, d45;
+    if constexpr (DESC) {
+        d01 = *reinterpret_cast<const float2 *>(desc + 6 * gp);
+        d23 = *reinterpret_cast<const float2 *>(desc + 6 * gp + 2);
+        d45 = *reinterpret_cast<const float2 *>(desc + 6 * gp + 4);
+    }
     int myq[3];   // equations of node nl (> 0 free)
-    float v[6];
-    if (eloc) {
+    if constexpr (EL) {
         // one word per lane: the node's first free equation and its free
         // dofs, numbered consecutively (no dependent load of eqn)
-        const uint32_t w = eloc[(size_t)e * 8 + g];
+        const uint32_t w = eloc[gp];
         const int base = (int)(w & 0x1FFFFFFFu) + 1;
         const uint32_t fm = w >> 29;
 #pragma unroll
@@ -983,17 +980,30 @@ __global__ __launch_bounds__(256, 8) void k_mf_hex(const int *__restrict__ elems
 #pragma unroll
         for (int d = 0; d < 3; d++) myq[d] = eqn[3 * (size_t)node + d];
     }
+    float v[6];
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-        v[d] = g ? erel[(size_t)e * 21 + 3 * (g - 1) + d] : 0.0f;   // lane g: c_g of x
-        v[3 + d] = myq[d] > 0 ? x[myq[d] - 1] : 0.0f;
+        const float xv = x[myq[d] > 0 ? myq[d] - 1 : 0];
+        v[d] = g ? cx[d] : 0.0f;
+        v[3 + d] = myq[d] > 0 ? xv : 0.0f;
     }
-    const size_t gp = (size_t)e * 8 + g;
-    float c, m[6];
-    const double *mp = mat + (size_t)HIPFEM_MATP * emat[e];
+    const double *mp = mat + (size_t)HIPFEM_MATP * me;
     const float E = (float)mp[1], nu = (float)mp[2];
     const float Kb = E * __builtin_amdgcn_rcpf(3.f * (1.f - 2.f * nu));
-    mf_point_tangent(desc, gp, E, nu, c, m);
+    float c, m[6];
+    if constexpr (DESC) {
+        c = d01.x;
+        m[0] = d01.y;
+        m[1] = d23.x;
+        m[2] = -(d01.y + d23.x);
+        m[3] = d23.y;
+        m[4] = d45.x;
+        m[5] = d45.y;
+    } else {
+        c = E * __builtin_amdgcn_rcpf(1.f + nu);
+#pragma unroll
+        for (int i = 0; i < 6; i++) m[i] = 0.f;
+    }
     float fr[24];
     // every lane computes: the 8 lanes of an element are all active or all
     // inactive, and the exchanges stay inside the 8, so what inactive lanes
@@ -1727,16 +1737,34 @@ void model_unlink(hfem_model *md, hipcsr_mtx *m)
     m->mf_version = -1;
 }
 
+// k_mf_hex over nel elements (elems: one color's list, or NULL for all),
+// in the lookup and tangent forms of this model and its last assembly
+static void launch_mf_hex(const hfem_model *md, hipStream_t s, const int *elems, int nel, const float *x, double *y,
+                          float *ebuf, float *yslot, const PcgState *st)
+{
+    const float *desc = md->tdesc_used ? md->d_tdesc : nullptr;
+    const int grid = (nel + MF_EPB - 1) / MF_EPB;
+    const uint8_t *ecolor = yslot ? md->d_ecolor : nullptr;
+    const int neq = yslot ? md->neq : 0;
+#define MF_ARGS                                                                                                    \
+    elems, nel, md->d_erel, md->d_conn, md->d_eqn, md->d_eloc, md->d_emat, md->d_mat, desc, md->d_dsnap, md->ngp_total, \
+        x, y, ebuf, yslot, ecolor, neq, st
+    if (md->d_eloc && desc)
+        k_mf_hex<true, true><<<grid, 256, 0, s>>>(MF_ARGS);
+    else if (md->d_eloc)
+        k_mf_hex<true, false><<<grid, 256, 0, s>>>(MF_ARGS);
+    else if (desc)
+        k_mf_hex<false, true><<<grid, 256, 0, s>>>(MF_ARGS);
+    else
+        k_mf_hex<false, false><<<grid, 256, 0, s>>>(MF_ARGS);
+#undef MF_ARGS
+}
+
 // the element pass of the slot form: one launch over every element
 static int mf_slot_pass(hfem_model *md, const float *x, const PcgState *st)
 {
-    hipStream_t s = md->ctx->stream;
-    if (!md->nelem) return 0;
-    const float *desc = md->tdesc_used ? md->d_tdesc : nullptr;
-    k_mf_hex<<<(md->nelem + MF_EPB - 1) / MF_EPB, 256, 0, s>>>(nullptr, md->nelem, md->d_erel, md->d_conn, md->d_eqn,
-                                                               md->d_eloc, md->d_emat, md->d_mat, desc, md->d_dsnap,
-                                                               md->ngp_total, x, nullptr, nullptr, md->d_yslot,
-                                                               md->d_ecolor, md->neq, st);
+    if (!md->nelem || !md->neq) return 0;
+    launch_mf_hex(md, md->ctx->stream, nullptr, md->nelem, x, nullptr, nullptr, md->d_yslot, st);
     HIPFEM_CHECK(hipGetLastError());
     return 0;
 }
@@ -1773,11 +1801,7 @@ int model_mf_apply_f32(hfem_model *md, const float *x, double *y, const PcgState
     }
     if (md->d_nptr && md->mf_form == 1) {
         // gather form: every element in one launch, then one FP64 sum per node
-        if (md->nelem)
-            k_mf_hex<<<(md->nelem + MF_EPB - 1) / MF_EPB, 256, 0, s>>>(
-                nullptr, md->nelem, md->d_erel, md->d_conn, md->d_eqn, md->d_eloc, md->d_emat, md->d_mat,
-                md->tdesc_used ? md->d_tdesc : nullptr, md->d_dsnap, md->ngp_total, x, y, md->d_ebuf, nullptr, nullptr,
-                0, st);
+        if (md->nelem && md->neq) launch_mf_hex(md, s, nullptr, md->nelem, x, y, md->d_ebuf, nullptr, st);
         k_mf_gather<<<std::min(8192, (md->nnode + 255) / 256), 256, 0, s>>>(md->nnode, md->d_nptr, md->d_nadj,
                                                                               md->d_eqn, md->d_ebuf, y, st);
         HIPFEM_CHECK(hipGetLastError());
@@ -1786,11 +1810,8 @@ int model_mf_apply_f32(hfem_model *md, const float *x, double *y, const PcgState
     if (md->neq) k_fill_d<<<std::min(4096, (md->neq + 255) / 256), 256, 0, s>>>(md->neq, y, st);
     for (int c = 0; c < md->ncolors; c++) {
         const int n = md->color_ptr[c + 1] - md->color_ptr[c];
-        if (!n) continue;
-        k_mf_hex<<<(n + MF_EPB - 1) / MF_EPB, 256, 0, s>>>(md->d_colors + md->color_ptr[c], n, md->d_erel,
-                                                           md->d_conn, md->d_eqn, md->d_eloc, md->d_emat,
-                                                           md->d_mat, md->tdesc_used ? md->d_tdesc : nullptr, md->d_dsnap,
-                                                           md->ngp_total, x, y, nullptr, nullptr, nullptr, 0, st);
+        if (!n || !md->neq) continue;
+        launch_mf_hex(md, s, md->d_colors + md->color_ptr[c], n, x, y, nullptr, nullptr, st);
     }
     HIPFEM_CHECK(hipGetLastError());
     return 0;

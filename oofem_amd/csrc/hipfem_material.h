@@ -90,6 +90,32 @@ __device__ __forceinline__ void mises_tangent(double E, double nu, double sig0, 
     for (int i = 0; i < 36; i++) D[i] += factor2 * idev[i];
 }
 
+// The same tangent in the form D = c I_dev + K 1x1 - m m^T (I_dev as in
+// mises_tangent; 1 the Voigt unit in the three normal rows): c = 2G +
+// factor2 and m = sqrt(-factor1) s (factor1 <= 0), and c = 2G, m = 0 on the
+// elastic branch.  Returns K.
+__device__ __forceinline__ double mises_tangent_parts(double E, double nu, double sig0, double H, double kappa_n,
+                                                      double kappa_t, const double s[6], double &c, double m[6])
+{
+    const double G = E / (2.0 * (1. + nu));
+    const double K = E / (3.0 * (1. - 2. * nu));
+    c = 2. * G;
+#pragma unroll
+    for (int i = 0; i < 6; i++) m[i] = 0.0;
+    const double dKappa = kappa_t - kappa_n;
+    if (dKappa <= 0.0) return K;
+    const double sigmaY = sig0 + H * kappa_n;
+    const double trialS = sqrt(s[0] * s[0] + s[1] * s[1] + s[2] * s[2] + 2. * s[3] * s[3] + 2. * s[4] * s[4] +
+                               2. * s[5] * s[5]);
+    const double factor = -2. * sqrt(6.) * G * G / trialS;
+    const double factor1 = factor * sigmaY / ((H + 3. * G) * trialS * trialS);
+    c += factor * dKappa;
+    const double sc = sqrt(-factor1);
+#pragma unroll
+    for (int i = 0; i < 6; i++) m[i] = sc * s[i];
+    return K;
+}
+
 // Elastic trial stress D_e (eps - epsp_n) (Voigt, engineering shear) and
 // the entries of the upper triangle of D_e (row-major, k = 0..20).
 __device__ __forceinline__ void elastic_trial(double E, double nu, const double el[6], double sig[6])

@@ -228,8 +228,13 @@ __global__ __launch_bounds__(64) void k_assemble(const int *__restrict__ elems, 
     __shared__ double sN[NGP][NEN * 3];
     __shared__ double sdV[NGP];
     __shared__ double sD[(ET == HIPFEM_LTRSPACE) ? 36 : NGP * 36];
+    // MisesMat tangent parts D = c I_dev + K 1x1 - m m^T per point: c/2 and
+    // K - c/3, m, and M g_a (M the symmetric tensor of m) per node
+    __shared__ double sCK[NGP][2];
+    __shared__ double sM[NGP][6];
+    __shared__ double sV[NGP][NEN * 3];
     __shared__ int sEq[NEN * 3];
-    __shared__ int sGeneral;
+    __shared__ int sGeneral, sSur;
 
     const int ei = blockIdx.x;
     if (ei >= nelem_color) return;
@@ -243,7 +248,10 @@ __global__ __launch_bounds__(64) void k_assemble(const int *__restrict__ elems, 
     const double *mp = mat + (size_t)HIPFEM_MATP * emat[e];
     const int model = (int)mp[0];
     const double E = mp[1], nu = mp[2];
-    if (t == 0) sGeneral = (model == HIPFEM_MAT_MISES && tangent) ? 1 : 0;
+    if (t == 0) {
+        sGeneral = (model == HIPFEM_MAT_MISES && tangent) ? 1 : 0;
+        sSur = 0;   // set when a point of the element has a surrogate tangent
+    }
     __syncthreads();
     for (int g = t; g < NGP; g += 64) {
         double xi[3], w, detJ;
@@ -261,16 +269,35 @@ __global__ __launch_bounds__(64) void k_assemble(const int *__restrict__ elems, 
                         D[6 * r + c] = D[6 * c + r] = (double)dsur[(size_t)k * ngp_total + gp];
                         k++;
                     }
+                sSur = 1;
             } else {
                 double s[6];
 #pragma unroll
                 for (int i = 0; i < 6; i++) s[i] = state[(size_t)(14 + i) * ngp_total + gp];
-                mises_tangent(E, nu, mp[3], mp[4], state[(size_t)6 * ngp_total + gp],
-                              state[(size_t)13 * ngp_total + gp], s, sD + 36 * g);
+                const double kn = state[(size_t)6 * ngp_total + gp], kt = state[(size_t)13 * ngp_total + gp];
+                mises_tangent(E, nu, mp[3], mp[4], kn, kt, s, sD + 36 * g);
+                double c, m[6];
+                const double Kb = mises_tangent_parts(E, nu, mp[3], mp[4], kn, kt, s, c, m);
+                sCK[g][0] = 0.5 * c;
+                sCK[g][1] = Kb - c / 3.0;
+#pragma unroll
+                for (int i = 0; i < 6; i++) sM[g][i] = m[i];
             }
         }
     }
     __syncthreads();
+    const bool parts = sGeneral && !sSur;   // every point's tangent in the parts form
+    if (parts) {
+        for (int k = t; k < NGP * NEN; k += 64) {
+            const int g = k / NEN, a = k % NEN;
+            const double n0 = sN[g][3 * a], n1 = sN[g][3 * a + 1], n2 = sN[g][3 * a + 2];
+            const double *m = sM[g];   // M = [[m0 m5 m4] [m5 m1 m3] [m4 m3 m2]]
+            sV[g][3 * a] = m[0] * n0 + m[5] * n1 + m[4] * n2;
+            sV[g][3 * a + 1] = m[5] * n0 + m[1] * n1 + m[3] * n2;
+            sV[g][3 * a + 2] = m[4] * n0 + m[3] * n1 + m[2] * n2;
+        }
+        __syncthreads();
+    }
     const double G = E / (2.0 * (1. + nu));
     const double Kb = E / (3.0 * (1. - 2. * nu));
     const double lam = Kb - 2.0 * G / 3.0;
@@ -289,6 +316,23 @@ __global__ __launch_bounds__(64) void k_assemble(const int *__restrict__ elems, 
 #pragma unroll
                     for (int j = 0; j < 3; j++)
                         Kab[3 * i + j] += dv * (lam * n[i] * m[j] + G * (n[j] * m[i] + (i == j ? nm : 0.0)));
+            }
+        } else if (parts) {
+            // B_a^T D B_b = c/2 ((g_a . g_b) I + g_b g_a^T) + (K - c/3) g_a g_b^T
+            //               - (M g_a) (M g_b)^T
+            for (int g = 0; g < NGP; g++) {
+                const double n[3] = {sN[g][3 * a], sN[g][3 * a + 1], sN[g][3 * a + 2]};
+                const double m[3] = {sN[g][3 * b], sN[g][3 * b + 1], sN[g][3 * b + 2]};
+                const double va[3] = {sV[g][3 * a], sV[g][3 * a + 1], sV[g][3 * a + 2]};
+                const double vb[3] = {sV[g][3 * b], sV[g][3 * b + 1], sV[g][3 * b + 2]};
+                const double dv = sdV[g], ch = sCK[g][0], kl = sCK[g][1];
+                const double nm = n[0] * m[0] + n[1] * m[1] + n[2] * m[2];
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+#pragma unroll
+                    for (int j = 0; j < 3; j++)
+                        Kab[3 * i + j] +=
+                            dv * (ch * (m[i] * n[j] + (i == j ? nm : 0.0)) + kl * n[i] * m[j] - va[i] * vb[j]);
             }
         } else {
             for (int g = 0; g < NGP; g++) {

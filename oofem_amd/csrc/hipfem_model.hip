@@ -27,11 +27,13 @@
 //                    owns node pairs (a,b) and accumulates the 3x3 block
 //                    B_a^T D B_b dV over the Gauss points in registers before
 //                    scattering it.  IsoLE uses the closed form
-//                    lambda n_i m_j + G (n_j m_i + delta_ij n.m).
+//                    lambda n_i m_j + G (n_j m_i + delta_ij n.m); MisesMat
+//                    the parts c I_dev + K 1x1 - m m^T of its tangent.
 //   k_internal<ET>   one thread per Gauss point: geometry, strain B u_e, the
 //                    material update (radial return for MisesMat), B^T sigma dV
 //                    into LDS; then per element the Gauss-point sums in
-//                    ascending order are scattered to f.
+//                    ascending order are stored, and k_gather_nodal sums them
+//                    per node in ascending element order.
 
 #include <algorithm>
 #include <cmath>
@@ -552,9 +554,13 @@ __global__ __launch_bounds__(64) void k_internal(const int *__restrict__ elems, 
                                                  const double *__restrict__ mat, const double *__restrict__ u_free,
                                                  const double *__restrict__ u_presc, double *__restrict__ state,
                                                  int ngp_total, int update_state, double *__restrict__ f_free,
-                                                 double *__restrict__ f_presc, double *__restrict__ partials,
-                                                 int npart, int *__restrict__ bad, int op, SurArgs sur)
+                                                 double *__restrict__ f_presc, double *__restrict__ fbuf,
+                                                 double *__restrict__ partials, int npart, int *__restrict__ bad,
+                                                 int op, SurArgs sur)
 {
+    // elems == nullptr: every element, ei = e.  fbuf: the element force
+    // vectors are stored (NC per element) for k_gather_nodal, else added to
+    // f_free / f_presc (one color per launch)
     constexpr int NEN = ElemT<ET>::NEN, NGP = ElemT<ET>::NGP, EPB = IntCfg<ET>::EPB, NC = 3 * NEN;
     __shared__ double sF[EPB][NGP][NC];
     __shared__ double sh[3];
@@ -566,7 +572,7 @@ __global__ __launch_bounds__(64) void k_internal(const int *__restrict__ elems, 
     const int ei = blockIdx.x * EPB + el;
     const bool active = (el < EPB) && (ei < nelem_color);
     if (active) {
-        const int e = elems[ei];
+        const int e = elems ? elems[ei] : ei;
         double X[NC], ue[NC];
         for (int a = 0; a < NEN; a++) {
             const int node = conn[(size_t)e * NEN + a];
@@ -696,13 +702,17 @@ __global__ __launch_bounds__(64) void k_internal(const int *__restrict__ elems, 
         const int l = idx / NC, c = idx % NC;
         const int ej = blockIdx.x * EPB + l;
         if (ej >= nelem_color) continue;
-        const int e = elems[ej];
+        const int e = elems ? elems[ej] : ej;
         double f = 0.0;
         for (int gg = 0; gg < NGP; gg++) f += sF[l][gg][c];
-        const int node = conn[(size_t)e * NEN + c / 3];
-        const int q = eqn[3 * (size_t)node + c % 3];
-        if (q > 0 && f_free) f_free[q - 1] += f;
-        if (q < 0 && f_presc) f_presc[-q - 1] += f;
+        if (fbuf) {
+            fbuf[(size_t)e * NC + c] = f;
+        } else {
+            const int node = conn[(size_t)e * NEN + c / 3];
+            const int q = eqn[3 * (size_t)node + c % 3];
+            if (q > 0 && f_free) f_free[q - 1] += f;
+            if (q < 0 && f_presc) f_presc[-q - 1] += f;
+        }
         n3[c % 3] += f * f;
     }
     // block partial of the EBE norms (fixed order)
@@ -715,6 +725,52 @@ __global__ __launch_bounds__(64) void k_internal(const int *__restrict__ elems, 
         }
         if (t == 0)
             for (int k = 0; k < 3; k++) partials[(size_t)k * npart + blockIdx.x] = sh[k];
+    }
+}
+
+// f per dof = the sum of the stored element force vectors over the elements
+// at the node, FP64, in the fixed order of the node's list (nptr / nadj:
+// element * nen + local node, ascending element); one thread per node
+__global__ void k_gather_nodal(int nnode, int nen, const int *__restrict__ nptr, const int *__restrict__ nadj,
+                               const int *__restrict__ eqn, const double *__restrict__ fbuf,
+                               double *__restrict__ f_free, double *__restrict__ f_presc)
+{
+    for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < nnode; n += gridDim.x * blockDim.x) {
+        double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+        const int j1 = nptr[n + 1];
+        int j = nptr[n];
+        // four list entries' loads in flight before their sums (same order)
+        for (; j + 4 <= j1; j += 4) {
+            double v[4][3];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const int ea = nadj[j + k];
+                const double *f = fbuf + (size_t)(ea / nen) * 3 * nen + 3 * (ea % nen);
+                v[k][0] = f[0];
+                v[k][1] = f[1];
+                v[k][2] = f[2];
+            }
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                s0 += v[k][0];
+                s1 += v[k][1];
+                s2 += v[k][2];
+            }
+        }
+        for (; j < j1; j++) {
+            const int ea = nadj[j];
+            const double *f = fbuf + (size_t)(ea / nen) * 3 * nen + 3 * (ea % nen);
+            s0 += f[0];
+            s1 += f[1];
+            s2 += f[2];
+        }
+        const double sv[3] = {s0, s1, s2};
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const int q = eqn[3 * (size_t)n + d];
+            if (q > 0 && f_free) f_free[q - 1] = sv[d];
+            if (q < 0 && f_presc) f_presc[-q - 1] = sv[d];
+        }
     }
 }
 
@@ -1303,6 +1359,10 @@ struct hfem_model {
     uint16_t *d_eqmask = nullptr;                // colors touching every free equation (slot form)
     float *d_yslot = nullptr;                    // ncolors x neq FP32 slots (slot form)
     float *d_erel = nullptr;                     // per element: coordinate coefficients c_1..c_7 (21 FP32)
+    // internal forces: node -> (element * nen + local node), ascending
+    // element, and the element force vectors of one evaluation
+    int *d_fnptr = nullptr, *d_fnadj = nullptr;
+    double *d_fbuf = nullptr;
     // per element and k_mf_hex lane g: first free equation (0-based, bits
     // 0..28) and free-dof mask (bits 29..31) of the lane's node; built once
     // per mesh when every node numbers its free dofs consecutively
@@ -1339,7 +1399,7 @@ static int model_free(hfem_model *md)
     void *ps[] = {md->d_coords, md->d_mat, md->d_state, md->d_conn, md->d_emat, md->d_eqn, md->d_bad,
                   md->d_eoff,   md->d_colors, md->d_uf, md->d_up, md->d_ff, md->d_fp, md->d_out3, md->d_dofid,
                   md->d_tdesc, md->d_nptr, md->d_nadj, md->d_ebuf, md->d_ecolor, md->d_eqmask, md->d_yslot,
-                  md->d_erel, md->d_eloc};
+                  md->d_erel, md->d_eloc, md->d_fnptr, md->d_fnadj, md->d_fbuf};
     for (void *p : ps)
         if (p) HIPFEM_CHECK(hipFree(p));
     md->d_tdesc = nullptr;
@@ -1351,6 +1411,8 @@ static int model_free(hfem_model *md)
     md->d_erel = nullptr;
     md->d_eloc = nullptr;
     md->eloc_checked = false;
+    md->d_fnptr = md->d_fnadj = nullptr;
+    md->d_fbuf = nullptr;
     md->d_coords = md->d_mat = md->d_state = md->d_uf = md->d_up = md->d_ff = md->d_fp = md->d_out3 = nullptr;
     md->d_conn = md->d_emat = md->d_eqn = md->d_bad = md->d_colors = nullptr;
     md->d_dofid = nullptr;
@@ -1870,10 +1932,24 @@ static int internal_dev(hfem_model *md, const double *uf, const double *up, doub
                    "internal forces: call hfem_model_build_structure first (it builds the element coloring)");
     int epb = md->etype == HIPFEM_LSPACE ? IntCfg<HIPFEM_LSPACE>::EPB
               : md->etype == HIPFEM_LTRSPACE ? IntCfg<HIPFEM_LTRSPACE>::EPB : IntCfg<HIPFEM_QSPACE>::EPB;
-    int npart = 0;
-    for (int c = 0; c < md->ncolors; c++) npart += (md->color_ptr[c + 1] - md->color_ptr[c] + epb - 1) / epb;
+    const int npart = (md->nelem + epb - 1) / epb;
     int rc = ensure_partials(md->ctx, 3 * (size_t)std::max(1, npart));
     if (rc) return rc;
+    if (!md->d_fnptr && md->nelem) {
+        // node -> element adjacency of the force gather (built once per mesh)
+        const int nen = md->nen;
+        std::vector<int> cnt((size_t)md->nnode + 1, 0), adj((size_t)md->nelem * nen);
+        for (size_t i = 0; i < (size_t)md->nelem * nen; i++) cnt[md->h_conn[i] + 1]++;
+        for (int n = 0; n < md->nnode; n++) cnt[n + 1] += cnt[n];
+        std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+        for (int e = 0; e < md->nelem; e++)
+            for (int a = 0; a < nen; a++) adj[fill[md->h_conn[(size_t)e * nen + a]]++] = e * nen + a;
+        HIPFEM_CHECK(hipMalloc(&md->d_fnptr, sizeof(int) * cnt.size()));
+        HIPFEM_CHECK(hipMalloc(&md->d_fnadj, sizeof(int) * adj.size()));
+        HIPFEM_CHECK(hipMalloc(&md->d_fbuf, sizeof(double) * 3 * nen * (size_t)md->nelem));
+        HIPFEM_CHECK(hipMemcpy(md->d_fnptr, cnt.data(), sizeof(int) * cnt.size(), hipMemcpyHostToDevice));
+        HIPFEM_CHECK(hipMemcpy(md->d_fnadj, adj.data(), sizeof(int) * adj.size(), hipMemcpyHostToDevice));
+    }
     rc = reset_bad(md);
     if (rc) return rc;
     if (ff && md->neq) k_fill<<<std::min(8192, (md->neq + 255) / 256), 256, 0, st>>>(md->neq, 0.0, ff);
@@ -1911,23 +1987,24 @@ static int internal_dev(hfem_model *md, const double *uf, const double *up, doub
         sur.tan_scale = md->sur_scales[2];
         sur.tol = md->sur_tol;
     }
-    int poff = 0;
-    for (int c = 0; c < md->ncolors; c++) {
-        const int n = md->color_ptr[c + 1] - md->color_ptr[c];
-        if (!n) continue;
-        const int nb = (n + epb - 1) / epb;
-        const int *el = md->d_colors + md->color_ptr[c];
-        // partial slots of this color: [poff, poff + nb) in each of the 3 planes
-#define LAUNCH_INT(ET)                                                                                          \
-    k_internal<ET><<<nb, 64, 0, st>>>(el, n, md->d_coords, md->d_conn, md->d_eqn, md->d_emat, md->d_mat, uf, up, \
-                                      md->d_state, md->ngp_total, update, ff, fp, part + poff, npart, md->d_bad, op, \
-                                      sur)
+    if (md->nelem) {
+        // every element in one launch (the Gauss-point state is read and
+        // written in whole lines), each storing its force vector; then one
+        // FP64 sum per node in the fixed order of its element list
+#define LAUNCH_INT(ET)                                                                                              \
+    k_internal<ET><<<npart, 64, 0, st>>>(nullptr, md->nelem, md->d_coords, md->d_conn, md->d_eqn, md->d_emat,        \
+                                         md->d_mat, uf, up, md->d_state, md->ngp_total, update, nullptr, nullptr,    \
+                                         md->d_fbuf, part, npart, md->d_bad, op, sur)
         if (md->etype == HIPFEM_LSPACE) LAUNCH_INT(HIPFEM_LSPACE);
         else if (md->etype == HIPFEM_LTRSPACE) LAUNCH_INT(HIPFEM_LTRSPACE);
         else LAUNCH_INT(HIPFEM_QSPACE);
 #undef LAUNCH_INT
         HIPFEM_CHECK(hipGetLastError());
-        poff += nb;
+        if (ff || fp) {
+            k_gather_nodal<<<std::min(8192, (md->nnode + 255) / 256), 256, 0, st>>>(
+                md->nnode, md->nen, md->d_fnptr, md->d_fnadj, md->d_eqn, md->d_fbuf, ff, fp);
+            HIPFEM_CHECK(hipGetLastError());
+        }
     }
     if (ebe3) {
         k_reduce3<<<1, 1024, 0, st>>>(part, npart, md->d_out3);

@@ -223,8 +223,11 @@ __global__ __launch_bounds__(64) void k_assemble(const int *__restrict__ elems, 
                                                  const double *__restrict__ state, int ngp_total, int tangent,
                                                  const int32_t *__restrict__ rowptr, double *__restrict__ val,
                                                  int *__restrict__ bad, const int8_t *__restrict__ sflag,
-                                                 const float *__restrict__ dsur)
+                                                 const float *__restrict__ dsur, double *__restrict__ kbuf)
 {
+    // elems == nullptr: every element (block = element).  kbuf: the element's
+    // 3x3 blocks are stored (9 per node pair (a, b), row-major) for
+    // k_assemble_gather, else added into val (one color per launch)
     constexpr int NEN = ElemT<ET>::NEN, NGP = ElemT<ET>::NGP;
     __shared__ double sX[NEN * 3];
     __shared__ double sN[NGP][NEN * 3];
@@ -240,7 +243,7 @@ __global__ __launch_bounds__(64) void k_assemble(const int *__restrict__ elems, 
 
     const int ei = blockIdx.x;
     if (ei >= nelem_color) return;
-    const int e = elems[ei];
+    const int e = elems ? elems[ei] : ei;
     const int t = threadIdx.x;
     for (int k = t; k < NEN * 3; k += 64) {
         const int node = conn[(size_t)e * NEN + k / 3];
@@ -303,8 +306,20 @@ __global__ __launch_bounds__(64) void k_assemble(const int *__restrict__ elems, 
     const double G = E / (2.0 * (1. + nu));
     const double Kb = E / (3.0 * (1. - 2. * nu));
     const double lam = Kb - 2.0 * G / 3.0;
-    for (int blk = t; blk < NEN * NEN; blk += 64) {
-        const int a = blk / NEN, b = blk % NEN;
+    // stored blocks (kbuf): the pairs a <= b, each also stored transposed as
+    // (b, a), so every element matrix is exactly symmetric
+    const int npairs = kbuf ? NEN * (NEN + 1) / 2 : NEN * NEN;
+    for (int blk = t; blk < npairs; blk += 64) {
+        int a = blk / NEN, b = blk % NEN;
+        if (kbuf) {
+            int rem = blk;
+            a = 0;
+            while (rem >= NEN - a) {
+                rem -= NEN - a;
+                a++;
+            }
+            b = a + rem;
+        }
         double Kab[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
         if (!sGeneral) {
             for (int g = 0; g < NGP; g++) {
@@ -358,6 +373,19 @@ __global__ __launch_bounds__(64) void k_assemble(const int *__restrict__ elems, 
                 }
             }
         }
+        if (kbuf) {
+            double *o = kbuf + ((size_t)e * NEN * NEN + a * NEN + b) * 9;
+#pragma unroll
+            for (int k = 0; k < 9; k++) o[k] = Kab[k];
+            if (a != b) {
+                double *ot = kbuf + ((size_t)e * NEN * NEN + b * NEN + a) * 9;
+#pragma unroll
+                for (int i = 0; i < 3; i++)
+#pragma unroll
+                    for (int j = 0; j < 3; j++) ot[3 * j + i] = Kab[3 * i + j];
+            }
+            continue;
+        }
         const int off = eoff[((size_t)e * NEN + a) * NEN + b];
         if (off == 0xFFFF) continue;
 #pragma unroll
@@ -373,6 +401,76 @@ __global__ __launch_bounds__(64) void k_assemble(const int *__restrict__ elems, 
                 rnk++;
             }
         }
+    }
+}
+
+// The rows of node n from the stored element blocks.  The node's element
+// list (element * NEN + local node a, ascending element) is taken 64 / NEN
+// elements at a time; lane (slot, b) loads the block (a, b) of its element
+// with its row offset (eoff) and column dofs, so that all loads of a pass are
+// in flight together.  The slots then add into the LDS rows one after the
+// other, in ascending element order, and each of the node's rows is written
+// once.  One wave per node; the node's rows share one column list of length
+// <= ASM_LMAX.
+constexpr int ASM_LMAX = 256;
+template <int NEN>
+__global__ __launch_bounds__(64) void k_assemble_gather(int nnode, const int *__restrict__ nptr,
+                                                        const int *__restrict__ nadj, const int *__restrict__ conn,
+                                                        const int *__restrict__ eqn, const uint16_t *__restrict__ eoff,
+                                                        const double *__restrict__ kbuf,
+                                                        const int32_t *__restrict__ rowptr, double *__restrict__ val)
+{
+    static_assert(64 % NEN == 0, "element slots per pass");
+    constexpr int SLOTS = 64 / NEN;
+    __shared__ double sR[3][ASM_LMAX];
+    const int n = blockIdx.x;
+    if (n >= nnode) return;
+    const int t = threadIdx.x, k = t / NEN, b = t % NEN;
+    int q[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) q[i] = eqn[3 * (size_t)n + i];
+    const int r0 = q[0] > 0 ? q[0] : q[1] > 0 ? q[1] : q[2];
+    if (r0 <= 0) return;   // no free dof: no rows
+    const int L = rowptr[r0] - rowptr[r0 - 1];
+    for (int p = t; p < L; p += 64) sR[0][p] = sR[1][p] = sR[2][p] = 0.0;
+    const int j0 = nptr[n], cnt = nptr[n + 1] - j0;
+    for (int base = 0; base < cnt; base += SLOTS) {
+        const int slot = base + k;
+        double v[9];
+        int pos[3] = {-1, -1, -1};
+        if (slot < cnt) {
+            const int ea = nadj[j0 + slot], e = ea / NEN, a = ea % NEN;
+            const int off = eoff[((size_t)e * NEN + a) * NEN + b];
+            const int nb = conn[(size_t)e * NEN + b];
+            const double *src = kbuf + (((size_t)e * NEN + a) * NEN + b) * 9;
+#pragma unroll
+            for (int x = 0; x < 9; x++) v[x] = src[x];
+            if (off != 0xFFFF) {
+                int r = 0;
+#pragma unroll
+                for (int j = 0; j < 3; j++)
+                    if (eqn[3 * (size_t)nb + j] > 0) pos[j] = off + r++;
+            }
+        }
+        __syncthreads();
+        for (int s = 0; s < SLOTS; s++) {
+            if (k == s) {
+#pragma unroll
+                for (int i = 0; i < 3; i++) {
+                    if (q[i] <= 0) continue;
+#pragma unroll
+                    for (int j = 0; j < 3; j++)
+                        if (pos[j] >= 0) sR[i][pos[j]] += v[3 * i + j];
+                }
+            }
+            __syncthreads();
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+        if (q[i] <= 0) continue;
+        double *row = val + rowptr[q[i] - 1];
+        for (int p = t; p < L; p += 64) row[p] = sR[i][p];
     }
 }
 
@@ -1363,6 +1461,11 @@ struct hfem_model {
     // element, and the element force vectors of one evaluation
     int *d_fnptr = nullptr, *d_fnadj = nullptr;
     double *d_fbuf = nullptr;
+    // stiffness assembly by rows (LSpace): the element blocks of one
+    // assembly (64 x 9 doubles per element) and the longest row
+    double *d_kbuf = nullptr;
+    bool kbuf_failed = false;
+    int max_rowlen = 0;
     // per element and k_mf_hex lane g: first free equation (0-based, bits
     // 0..28) and free-dof mask (bits 29..31) of the lane's node; built once
     // per mesh when every node numbers its free dofs consecutively
@@ -1399,7 +1502,7 @@ static int model_free(hfem_model *md)
     void *ps[] = {md->d_coords, md->d_mat, md->d_state, md->d_conn, md->d_emat, md->d_eqn, md->d_bad,
                   md->d_eoff,   md->d_colors, md->d_uf, md->d_up, md->d_ff, md->d_fp, md->d_out3, md->d_dofid,
                   md->d_tdesc, md->d_nptr, md->d_nadj, md->d_ebuf, md->d_ecolor, md->d_eqmask, md->d_yslot,
-                  md->d_erel, md->d_eloc, md->d_fnptr, md->d_fnadj, md->d_fbuf};
+                  md->d_erel, md->d_eloc, md->d_fnptr, md->d_fnadj, md->d_fbuf, md->d_kbuf};
     for (void *p : ps)
         if (p) HIPFEM_CHECK(hipFree(p));
     md->d_tdesc = nullptr;
@@ -1413,6 +1516,9 @@ static int model_free(hfem_model *md)
     md->eloc_checked = false;
     md->d_fnptr = md->d_fnadj = nullptr;
     md->d_fbuf = nullptr;
+    md->d_kbuf = nullptr;
+    md->kbuf_failed = false;
+    md->max_rowlen = 0;
     md->d_coords = md->d_mat = md->d_state = md->d_uf = md->d_up = md->d_ff = md->d_fp = md->d_out3 = nullptr;
     md->d_conn = md->d_emat = md->d_eqn = md->d_bad = md->d_colors = nullptr;
     md->d_dofid = nullptr;
@@ -1612,7 +1718,9 @@ extern "C" int hfem_model_build_structure(hfem_model *md, hipcsr_mtx *K)
     }
     std::vector<int32_t> rowptr((size_t)neq + 1, 0);
     int64_t total = 0;
+    int maxlen = 0;
     for (int r = 0; r < neq; r++) {
+        maxlen = std::max(maxlen, (int)rowlen[r]);
         total += rowlen[r];
         HIPFEM_REQUIRE(total < INT32_MAX, HIPFEM_ETOOBIG, "pattern exceeds int32 indices");
         rowptr[r + 1] = (int32_t)total;
@@ -1681,6 +1789,7 @@ extern "C" int hfem_model_build_structure(hfem_model *md, hipcsr_mtx *K)
     HIPFEM_CHECK(hipStreamSynchronize(st));
     md->color_ptr = cptr;
     md->ncolors = ncol;
+    md->max_rowlen = maxlen;
     md->h_ecolor.assign(color.begin(), color.end());
     md->h_nodemask = nodemask;
     md->pattern_id = K->pattern_id;
@@ -1704,6 +1813,8 @@ static int reset_bad(hfem_model *md)
     return 0;
 }
 
+static int node_adjacency(hfem_model *md);
+
 extern "C" int hfem_model_assemble_stiffness(hfem_model *md, hipcsr_mtx *K, int mode)
 {
     HIPFEM_REQUIRE(md && K && K->has_structure, HIPFEM_EINVAL, "assemble: no structure");
@@ -1711,19 +1822,47 @@ extern "C" int hfem_model_assemble_stiffness(hfem_model *md, hipcsr_mtx *K, int 
                    "assemble: matrix pattern was not built by hfem_model_build_structure for this model");
     HIPFEM_REQUIRE(mode == 0 || mode == 1, HIPFEM_EINVAL, "assemble: mode must be 0 or 1");
     DeviceGuard g(md->ctx->device);
-    int rc = hipcsr_zero(K);
-    if (rc) return rc;
-    rc = reset_bad(md);
-    if (rc) return rc;
     hipStream_t st = md->ctx->stream;
-    for (int c = 0; c < md->ncolors; c++) {
+    int rc = reset_bad(md);
+    if (rc) return rc;
+    // LSpace by rows: every element's blocks stored in one launch, then each
+    // node's rows summed in ascending element order and written once (no
+    // read-modify-write of partial lines); else one launch per color adding
+    // into the zeroed values
+    bool by_rows = md->etype == HIPFEM_LSPACE && md->nelem > 0 && md->max_rowlen <= ASM_LMAX && !md->kbuf_failed;
+    if (by_rows) {
+        rc = node_adjacency(md);
+        if (rc) return rc;
+        if (!md->d_kbuf && hipMalloc(&md->d_kbuf, sizeof(double) * 64 * 9 * (size_t)md->nelem) != hipSuccess) {
+            (void)hipGetLastError();   // the colored form needs no buffer
+            md->d_kbuf = nullptr;
+            md->kbuf_failed = true;
+            by_rows = false;
+        }
+    }
+    if (by_rows) {
+        k_assemble<HIPFEM_LSPACE><<<md->nelem, 64, 0, st>>>(nullptr, md->nelem, md->d_coords, md->d_conn, md->d_eqn,
+                                                            md->d_emat, md->d_mat, md->d_eoff, md->d_state,
+                                                            md->ngp_total, mode, K->d_rowptr, K->d_val, md->d_bad,
+                                                            md->sur ? md->d_sflag : nullptr, md->d_dsur, md->d_kbuf);
+        HIPFEM_CHECK(hipGetLastError());
+        k_assemble_gather<8><<<md->nnode, 64, 0, st>>>(md->nnode, md->d_fnptr, md->d_fnadj, md->d_conn, md->d_eqn,
+                                                       md->d_eoff, md->d_kbuf, K->d_rowptr, K->d_val);
+        HIPFEM_CHECK(hipGetLastError());
+        K->h_stage.clear();   // every value was written: what hipcsr_zero resets
+        K->staged = false;
+    } else {
+        rc = hipcsr_zero(K);
+        if (rc) return rc;
+    }
+    for (int c = 0; c < md->ncolors && !by_rows; c++) {
         const int n = md->color_ptr[c + 1] - md->color_ptr[c];
         if (!n) continue;
         const int *el = md->d_colors + md->color_ptr[c];
 #define LAUNCH_ASM(ET)                                                                                              \
     k_assemble<ET><<<n, 64, 0, st>>>(el, n, md->d_coords, md->d_conn, md->d_eqn, md->d_emat, md->d_mat, md->d_eoff, \
                                      md->d_state, md->ngp_total, mode, K->d_rowptr, K->d_val, md->d_bad,     \
-                                     md->sur ? md->d_sflag : nullptr, md->d_dsur)
+                                     md->sur ? md->d_sflag : nullptr, md->d_dsur, nullptr)
         if (md->etype == HIPFEM_LSPACE) LAUNCH_ASM(HIPFEM_LSPACE);
         else if (md->etype == HIPFEM_LTRSPACE) LAUNCH_ASM(HIPFEM_LTRSPACE);
         else LAUNCH_ASM(HIPFEM_QSPACE);
@@ -1924,6 +2063,25 @@ int model_mf_apply_f32(hfem_model *md, const float *x, double *y, const PcgState
 }
 }  // namespace hipfem
 
+// node -> (element * nen + local node) lists in ascending element order,
+// built once per mesh (internal-force gather, stiffness assembly by rows)
+static int node_adjacency(hfem_model *md)
+{
+    if (md->d_fnptr || !md->nelem) return 0;
+    const int nen = md->nen;
+    std::vector<int> cnt((size_t)md->nnode + 1, 0), adj((size_t)md->nelem * nen);
+    for (size_t i = 0; i < (size_t)md->nelem * nen; i++) cnt[md->h_conn[i] + 1]++;
+    for (int n = 0; n < md->nnode; n++) cnt[n + 1] += cnt[n];
+    std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+    for (int e = 0; e < md->nelem; e++)
+        for (int a = 0; a < nen; a++) adj[fill[md->h_conn[(size_t)e * nen + a]]++] = e * nen + a;
+    HIPFEM_CHECK(hipMalloc(&md->d_fnptr, sizeof(int) * cnt.size()));
+    HIPFEM_CHECK(hipMalloc(&md->d_fnadj, sizeof(int) * adj.size()));
+    HIPFEM_CHECK(hipMemcpy(md->d_fnptr, cnt.data(), sizeof(int) * cnt.size(), hipMemcpyHostToDevice));
+    HIPFEM_CHECK(hipMemcpy(md->d_fnadj, adj.data(), sizeof(int) * adj.size(), hipMemcpyHostToDevice));
+    return 0;
+}
+
 static int internal_dev(hfem_model *md, const double *uf, const double *up, double *ff, double *fp, double *ebe3,
                         int update, int op = 0)
 {
@@ -1935,21 +2093,10 @@ static int internal_dev(hfem_model *md, const double *uf, const double *up, doub
     const int npart = (md->nelem + epb - 1) / epb;
     int rc = ensure_partials(md->ctx, 3 * (size_t)std::max(1, npart));
     if (rc) return rc;
-    if (!md->d_fnptr && md->nelem) {
-        // node -> element adjacency of the force gather (built once per mesh)
-        const int nen = md->nen;
-        std::vector<int> cnt((size_t)md->nnode + 1, 0), adj((size_t)md->nelem * nen);
-        for (size_t i = 0; i < (size_t)md->nelem * nen; i++) cnt[md->h_conn[i] + 1]++;
-        for (int n = 0; n < md->nnode; n++) cnt[n + 1] += cnt[n];
-        std::vector<int> fill(cnt.begin(), cnt.end() - 1);
-        for (int e = 0; e < md->nelem; e++)
-            for (int a = 0; a < nen; a++) adj[fill[md->h_conn[(size_t)e * nen + a]]++] = e * nen + a;
-        HIPFEM_CHECK(hipMalloc(&md->d_fnptr, sizeof(int) * cnt.size()));
-        HIPFEM_CHECK(hipMalloc(&md->d_fnadj, sizeof(int) * adj.size()));
-        HIPFEM_CHECK(hipMalloc(&md->d_fbuf, sizeof(double) * 3 * nen * (size_t)md->nelem));
-        HIPFEM_CHECK(hipMemcpy(md->d_fnptr, cnt.data(), sizeof(int) * cnt.size(), hipMemcpyHostToDevice));
-        HIPFEM_CHECK(hipMemcpy(md->d_fnadj, adj.data(), sizeof(int) * adj.size(), hipMemcpyHostToDevice));
-    }
+    rc = node_adjacency(md);
+    if (rc) return rc;
+    if (!md->d_fbuf && md->nelem)
+        HIPFEM_CHECK(hipMalloc(&md->d_fbuf, sizeof(double) * 3 * md->nen * (size_t)md->nelem));
     rc = reset_bad(md);
     if (rc) return rc;
     if (ff && md->neq) k_fill<<<std::min(8192, (md->neq + 255) / 256), 256, 0, st>>>(md->neq, 0.0, ff);
